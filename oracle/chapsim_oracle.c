@@ -1,0 +1,1101 @@
+/*
+ * CPU oracle for the CHAPSim per-RK3-substep solver core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C, FP64 restatement of the
+ * reference Fortran routines named below, loop for loop, so that the HIP
+ * library can be checked against it.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it; the product path never
+ * calls it and has no CPU fallback.
+ *
+ * Index conventions: arrays are Fortran column-major, x fastest.  Macros below
+ * take the Fortran (1-based, ghost-aware) indices and map them to C offsets:
+ *   Q_tg(NCL1, 0:M+1, NCL3, 3), PR/DPH(NCL1, 0:M+1, NCL3)
+ *   RHS/RHSLLPHI/Qtmp(NCL1, M, NCL3), CONVH0(NCL1, M, NCL3, 3)
+ *
+ * Parity status of each part (how it is pinned) is listed in the test suite
+ * headers; every function here has at least one independent check.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    int L, M, N;            /* NCL1, NCL2, NCL3 */
+    int icase;              /* 1 = channel (walls in y), 4 = 3-periodic box */
+    double dx, dz, dxi, dzi, dxqi, dzqi, cvisc, volm;
+    double *ynd;            /* [M+1]  YND(1..NND2)                         */
+    double *ycc;            /* [M]    YCC(1..NCL2)                         */
+    double *dyfi;           /* [M]    DYFI(1..NCL2)                        */
+    double *dyci;           /* [M+1]  DYCI(1..NND2)                        */
+    double *wff, *wfb;      /* [M+1]  YCL2ND_WFF/WFB(1..NND2)              */
+    double *amph, *acph, *apph;   /* [M]                                   */
+    double *amvr, *acvr, *apvr;   /* [3*(M+1)]  (JC, IDR) -> (IDR-1)*(M+1)+JC-1 */
+    double *ak1;            /* [L]    AK1(1..L)                            */
+    double *ak3;            /* [N]    AK3(1..N)                            */
+} orc_grid;
+
+#define ICHANL 1
+#define IBOX3P 4
+
+/* ---------------------------------------------------------------------------
+ * Grid, metrics and coefficient set-up.
+ * COORDJR (src/COORDJR.f90:1-235, channel branch with ISTR2=1),
+ * AREA_INOUTLET (src/COORDJR.f90:355-399), MESH_YWEIGHT_FACTORS
+ * (src/COORDJR.f90:427-470), CONSPARA (src/CONSPARA.f90:15-110),
+ * TDMA_COEF (src/TDMA_COEF.f90:30-116, channel branch),
+ * FFT99_ROOT (src/possion3d_FFT99.f90:124-182).
+ * ------------------------------------------------------------------------- */
+void orc_grid_setup(orc_grid *g, double hx, double hz, double str2, double ren)
+{
+    const int L = g->L, M = g->M, N = g->N, NND2 = M + 1;
+    const double PI = 2.0 * asin(1.0);
+    /* y nodes and cell centres: tanh stretching (src/COORDJR.f90:28-38) */
+    for (int J = 1; J <= NND2; ++J) {
+        double X2 = (double)(J - 1) / (double)(NND2 - 1);
+        g->ynd[J - 1] = tanh(str2 * (X2 - 0.5)) / tanh(str2 * 0.5);
+    }
+    for (int J = 1; J <= M; ++J) {
+        double X2 = ((double)(J - 1) + 0.5) / (double)(NND2 - 1);
+        g->ycc[J - 1] = tanh(str2 * (X2 - 0.5)) / tanh(str2 * 0.5);
+    }
+    for (int J = 1; J <= M; ++J) g->dyfi[J - 1] = 1.0 / (g->ynd[J] - g->ynd[J - 1]);
+    for (int J = 2; J <= NND2 - 1; ++J)
+        g->dyci[J - 1] = 1.0 / ((g->ynd[J] - g->ynd[J - 2]) * 0.5);
+    g->dyci[0] = 2.0 / (g->ynd[1] - g->ynd[0]);
+    g->dyci[NND2 - 1] = 2.0 / (g->ynd[NND2 - 1] - g->ynd[M - 1]);
+    /* y interpolation weights (src/COORDJR.f90:434-446) */
+    for (int JC = 2; JC <= M; ++JC) {
+        int JM = JC - 1;
+        g->wff[JC - 1] = (g->ynd[JC - 1] - g->ycc[JM - 1]) / (g->ycc[JC - 1] - g->ycc[JM - 1]);
+        g->wfb[JC - 1] = (g->ycc[JC - 1] - g->ynd[JC - 1]) / (g->ycc[JC - 1] - g->ycc[JM - 1]);
+    }
+    g->wfb[0] = 1.0; g->wff[0] = 0.0;
+    g->wfb[NND2 - 1] = 0.0; g->wff[NND2 - 1] = 1.0;
+    /* CONSPARA */
+    g->dx = hx / (double)L; g->dxi = 1.0 / g->dx; g->dxqi = g->dxi * g->dxi;
+    g->dz = hz / (double)N; g->dzi = 1.0 / g->dz; g->dzqi = g->dzi * g->dzi;
+    g->cvisc = 1.0 / ren;
+    /* VOLM_tg: sum over cells of 1/RCCI1/DYFI, loop order K,J,I */
+    double v = 0.0;
+    for (int KC = 1; KC <= N; ++KC)
+        for (int JC = 1; JC <= M; ++JC)
+            for (int IC = 1; IC <= L; ++IC) v = v + 1.0 / 1.0 / g->dyfi[JC - 1];
+    g->volm = v;
+    /* TDMA_COEF, channel: Poisson operator with Neumann folded in */
+    for (int JC = 1; JC <= M; ++JC) {
+        int JP = JC + 1;
+        g->amph[JC - 1] = g->dyfi[JC - 1] * g->dyci[JC - 1];
+        g->apph[JC - 1] = g->dyfi[JC - 1] * g->dyci[JP - 1];
+        if (JC == 1) g->amph[JC - 1] = g->dyfi[JC - 1] * g->dyci[JC - 1] * 0.5;
+        if (JP == NND2) g->apph[JC - 1] = g->dyfi[JC - 1] * g->dyci[JP - 1] * 0.5;
+    }
+    for (int JC = 1; JC <= M; ++JC) g->acph[JC - 1] = -(g->amph[JC - 1] + g->apph[JC - 1]);
+    g->acph[0] = g->amph[0] + g->acph[0];
+    g->amph[0] = 0.0;
+    g->acph[M - 1] = g->apph[M - 1] + g->acph[M - 1];
+    g->apph[M - 1] = 0.0;
+#define AV(arr, JC, IDR) (g->arr[((IDR)-1) * (M + 1) + (JC)-1])
+    for (int i = 0; i < 3 * (M + 1); ++i) g->amvr[i] = g->acvr[i] = g->apvr[i] = 0.0;
+    for (int IDR = 1; IDR <= 3; ++IDR) {
+        if (IDR == 1 || IDR == 3) {
+            for (int JC = 2; JC <= M - 1; ++JC) {
+                int JP = JC + 1;
+                AV(amvr, JC, IDR) = g->dyfi[JC - 1] * g->dyci[JC - 1];
+                AV(apvr, JC, IDR) = g->dyfi[JC - 1] * g->dyci[JP - 1];
+                AV(acvr, JC, IDR) = -(AV(amvr, JC, IDR) + AV(apvr, JC, IDR));
+            }
+            double UCAJ = 4.0 / (2.0 / g->dyci[1] + 1.0 / g->dyfi[0]);
+            AV(amvr, 1, IDR) = 0.0;
+            AV(apvr, 1, IDR) = UCAJ * g->dyci[1];
+            AV(acvr, 1, IDR) = -UCAJ * (g->dyci[1] + 2.0 * g->dyfi[0]);
+            UCAJ = 4.0 / (2.0 / g->dyci[M - 1] + 1.0 / g->dyfi[M - 1]);
+            AV(amvr, M, IDR) = UCAJ * g->dyci[M - 1];
+            AV(apvr, M, IDR) = 0.0;
+            AV(acvr, M, IDR) = -UCAJ * (2.0 * g->dyfi[M - 1] + g->dyci[M - 1]);
+        } else {
+            for (int JC = 2; JC <= M; ++JC) {
+                int JM = JC - 1;
+                AV(amvr, JC, IDR) = g->dyfi[JM - 1] * g->dyci[JC - 1];
+                AV(apvr, JC, IDR) = g->dyfi[JC - 1] * g->dyci[JC - 1];
+                AV(acvr, JC, IDR) = -(AV(amvr, JC, IDR) + AV(apvr, JC, IDR));
+            }
+            AV(amvr, 1, IDR) = 0.0; AV(apvr, 1, IDR) = 0.0; AV(acvr, 1, IDR) = 1.0;
+            AV(amvr, NND2, IDR) = 0.0; AV(apvr, NND2, IDR) = 0.0; AV(acvr, NND2, IDR) = 1.0;
+        }
+    }
+#undef AV
+    /* modified wavenumbers (src/possion3d_FFT99.f90:152-179) */
+    int N3MH = N / 2, N3MP = N3MH + 1, N1MH = L / 2, N1MP = N1MH + 1;
+    for (int K = 1; K <= N3MH; ++K) {
+        double AN = (double)(K - 1) * 2.0 * PI;
+        g->ak3[K - 1] = 2.0 * (1.0 - cos(AN / (double)N)) * g->dzqi;
+    }
+    for (int K = N3MP; K <= N; ++K) {
+        double AN = -(double)(N - K + 1) * 2.0 * PI;
+        g->ak3[K - 1] = 2.0 * (1.0 - cos(AN / (double)N)) * g->dzqi;
+    }
+    for (int I = 1; I <= N1MH; ++I) {
+        double AP = (double)(I - 1) * 2.0 * PI;
+        g->ak1[I - 1] = 2.0 * (1.0 - cos(AP / (double)L)) * g->dxqi;
+    }
+    for (int I = N1MP; I <= L; ++I) {
+        double AP = -(double)(L - I + 1) * 2.0 * PI;
+        g->ak1[I - 1] = 2.0 * (1.0 - cos(AP / (double)L)) * g->dxqi;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Tridiagonal line solvers.  Element (line l, row r) lives at l*ls + r*es, so
+ * one routine covers the "IJJ" (march along index 2, es = ISZ, ls = 1) and
+ * "IJI" (march along index 1, es = 1, ls = ISZ) forms.
+ * orc_tdma_noncyc: TDMAIJJ_nonCYC (src/TDMAIJJ_nonCYC.f90:67-123) and
+ *                  TDMAIJI_nonCYC (src/TDMAIJI_nonCYC.f90:67-123).
+ *   bc: BCJ/BCI(line, 1:2), column-major -> bc[l], bc[nlines + l]; may be NULL.
+ * orc_tdma_cyc:    TDMAIJJ_CYC (src/TDMAIJJ_CYC.f90:67-157) and
+ *                  TDMAIJI_CYC (src/TDMAIJI_CYC.f90:67-157).
+ * n >= 3 is required (the reference reads outside its arrays for n < 3).
+ * ------------------------------------------------------------------------- */
+void orc_tdma_noncyc(const double *A, const double *B, const double *C, double *R,
+                     const double *bc, int n, int nlines, long es, long ls)
+{
+#pragma omp parallel
+    {
+        double *H = (double *)malloc(sizeof(double) * (size_t)n);
+        double *G = (double *)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(static)
+        for (int l = 0; l < nlines; ++l) {
+            const double *a = A + (long)l * ls, *b = B + (long)l * ls, *c = C + (long)l * ls;
+            double *r = R + (long)l * ls;
+            double blo = bc ? bc[l] : 0.0, bhi = bc ? bc[nlines + l] : 0.0;
+            H[0] = c[0] / b[0];
+            G[0] = (r[0] - a[0] * blo) / b[0];
+            for (int j = 1; j < n - 1; ++j) {
+                double PP = 1.0 / (b[j * es] - H[j - 1] * a[j * es]);
+                H[j] = c[j * es] * PP;
+                G[j] = (r[j * es] - a[j * es] * G[j - 1]) * PP;
+            }
+            {
+                int j = n - 1;
+                double PP = 1.0 / (b[j * es] - H[j - 1] * a[j * es]);
+                G[j] = (r[j * es] - c[j * es] * bhi - a[j * es] * G[j - 1]) * PP;
+                r[j * es] = G[j];
+            }
+            for (int j = n - 2; j >= 0; --j) r[j * es] = G[j] - H[j] * r[(j + 1) * es];
+        }
+        free(H);
+        free(G);
+    }
+}
+
+void orc_tdma_cyc(const double *A, const double *B, const double *C, double *R,
+                  int n, int nlines, long es, long ls)
+{
+#pragma omp parallel
+    {
+        double *H0 = (double *)calloc((size_t)n, sizeof(double));
+        double *G1 = (double *)calloc((size_t)n, sizeof(double));
+        double *G2 = (double *)calloc((size_t)n, sizeof(double));
+#pragma omp for schedule(static)
+        for (int l = 0; l < nlines; ++l) {
+            const double *a = A + (long)l * ls, *b = B + (long)l * ls, *c = C + (long)l * ls;
+            double *r = R + (long)l * ls;
+            H0[n - 2] = 0.0; /* never set by the reference: stays at its zero fill */
+            H0[0] = c[0] / b[0];
+            G1[0] = r[0] / b[0];
+            G2[0] = -a[0] / b[0];
+            for (int i = 1; i <= n - 3; ++i) {
+                double PP = 1.0 / (b[i * es] - H0[i - 1] * a[i * es]);
+                double QQ1 = G1[i - 1] * a[i * es];
+                double QQ2 = G2[i - 1] * a[i * es];
+                H0[i] = c[i * es] * PP;
+                G1[i] = (r[i * es] - QQ1) * PP;
+                G2[i] = -QQ2 * PP;
+            }
+            {
+                int i = n - 2;
+                double PP = 1.0 / (b[i * es] - H0[i - 1] * a[i * es]);
+                double QQ1 = G1[i - 1] * a[i * es];
+                double QQ2 = G2[i - 1] * a[i * es];
+                G1[i] = (r[i * es] - QQ1) * PP;
+                G2[i] = (-c[i * es] - QQ2) * PP;
+            }
+            for (int i = n - 3; i >= 0; --i) {
+                G1[i] = -H0[i] * G1[i + 1] + G1[i];
+                G2[i] = -H0[i] * G2[i + 1] + G2[i];
+            }
+            {
+                int i = n - 1;
+                double PP = r[i * es] - c[i * es] * G1[0] - a[i * es] * G1[i - 1];
+                double QQ1 = b[i * es] + c[i * es] * G2[0] + a[i * es] * G2[i - 1];
+                r[i * es] = PP / QQ1;
+            }
+            double xn = r[(n - 1) * es];
+            for (int i = n - 2; i >= 0; --i) r[i * es] = G1[i] + G2[i] * xn;
+        }
+        free(H0);
+        free(G1);
+        free(G2);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Discrete Fourier transforms with the conventions of the vendored FFT99
+ * package (libs/fft99/FFT99.f90:98-105, libs/fft99/CFFT99.f90:182-197):
+ *   FFT99 ISIGN=-1: a_k + i b_k = (1/n) sum_j x_j exp(-2 pi i jk/n), k=0..n/2
+ *   FFT99 ISIGN=+1: x_j = sum_k C_k exp(+2 pi i jk/n), C_k = a_k+ib_k,
+ *                   C_{n-k} = a_k - i b_k, with b_0 = b_{n/2} taken as 0
+ *   CFFT99: unnormalised, exp(-...) for ISIGN=-1, exp(+...) for ISIGN=+1.
+ * Computed here by a plain mixed-radix (2,3,5, any other factor by direct
+ * DFT) recursive Cooley-Tukey transform in FP64.
+ * ------------------------------------------------------------------------- */
+static void cfft_rec(const double *xin, double *out, int n, int stride, int sign,
+                     const double *wr, const double *wi, int nfull, double *scratch)
+{
+    /* xin: complex interleaved with element stride 'stride' (in complex units)
+     * out: contiguous n complex.  wr/wi: twiddles of the full length nfull. */
+    if (n == 1) {
+        out[0] = xin[0];
+        out[1] = xin[1];
+        return;
+    }
+    int p = 0;
+    if (n % 4 == 0) p = 4;
+    else if (n % 2 == 0) p = 2;
+    else if (n % 3 == 0) p = 3;
+    else if (n % 5 == 0) p = 5;
+    else p = n;
+    int m = n / p;
+    /* sub-transforms of the p decimated sequences */
+    for (int q = 0; q < p; ++q)
+        cfft_rec(xin + 2 * (long)q * stride, scratch + 2 * (long)q * m, m, stride * p, sign,
+                 wr, wi, nfull, scratch + 2 * (long)n);
+    int tw = nfull / n; /* twiddle step for this length */
+    for (int k = 0; k < m; ++k) {
+        for (int s = 0; s < p; ++s) {
+            /* out[k + s m] = sum_q W_n^{q (k + s m)} sub_q[k] */
+            double accr = 0.0, acci = 0.0;
+            for (int q = 0; q < p; ++q) {
+                long e = ((long)q * (k + s * m)) % n;
+                double c = wr[e * tw], sn = sign * wi[e * tw];
+                double xr = scratch[2 * ((long)q * m + k)], xi = scratch[2 * ((long)q * m + k) + 1];
+                accr += c * xr - sn * xi;
+                acci += c * xi + sn * xr;
+            }
+            out[2 * (k + s * m)] = accr;
+            out[2 * (k + s * m) + 1] = acci;
+        }
+    }
+}
+
+typedef struct {
+    int n;
+    double *wr, *wi; /* cos/sin(2 pi e / n), e = 0..n-1, sign applied at use */
+    double *scratch;
+} cplan;
+
+static void cplan_init(cplan *p, int n)
+{
+    const double PI = 2.0 * asin(1.0);
+    p->n = n;
+    p->wr = (double *)malloc(sizeof(double) * (size_t)n);
+    p->wi = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int e = 0; e < n; ++e) {
+        p->wr[e] = cos(2.0 * PI * (double)e / (double)n);
+        p->wi[e] = -sin(2.0 * PI * (double)e / (double)n); /* exp(-i theta) */
+    }
+    /* scratch: recursion depth <= log2 n, each level needs 2n doubles */
+    p->scratch = (double *)malloc(sizeof(double) * (size_t)(4 * n * 24 + 16));
+}
+
+static void cplan_free(cplan *p)
+{
+    free(p->wr);
+    free(p->wi);
+    free(p->scratch);
+}
+
+/* in-place complex transform of one contiguous interleaved vector;
+ * isign = -1 gives exp(-i..), +1 gives exp(+i..); unnormalised. */
+static void cfft_run(cplan *p, double *x, int isign, double *tmp)
+{
+    cfft_rec(x, tmp, p->n, 1, isign < 0 ? 1 : -1, p->wr, p->wi, p->n, p->scratch);
+    memcpy(x, tmp, sizeof(double) * 2 * (size_t)p->n);
+}
+
+/* single-vector API used by the tests */
+void orc_cfft99(double *x, int n, int isign)
+{
+    cplan p;
+    cplan_init(&p, n);
+    double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    cfft_run(&p, x, isign, tmp);
+    free(tmp);
+    cplan_free(&p);
+}
+
+/* x[n] -> ab[n+2] = a0,b0,a1,b1,...,a_{n/2},b_{n/2} (FFT99 ISIGN=-1) */
+void orc_fft99_fwd(const double *x, double *ab, int n)
+{
+    cplan p;
+    cplan_init(&p, n);
+    double *z = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+    for (int j = 0; j < n; ++j) { z[2 * j] = x[j]; z[2 * j + 1] = 0.0; }
+    cfft_run(&p, z, -1, z + 2 * n);
+    for (int k = 0; k <= n / 2; ++k) {
+        ab[2 * k] = z[2 * k] / (double)n;
+        ab[2 * k + 1] = z[2 * k + 1] / (double)n;
+    }
+    ab[1] = 0.0;
+    ab[n + 1] = 0.0;
+    free(z);
+    cplan_free(&p);
+}
+
+/* ab[n+2] -> x[n] (FFT99 ISIGN=+1) */
+void orc_fft99_inv(const double *ab, double *x, int n)
+{
+    cplan p;
+    cplan_init(&p, n);
+    double *z = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+    z[0] = ab[0]; z[1] = 0.0;
+    for (int k = 1; k < n / 2; ++k) {
+        z[2 * k] = ab[2 * k]; z[2 * k + 1] = ab[2 * k + 1];
+        z[2 * (n - k)] = ab[2 * k]; z[2 * (n - k) + 1] = -ab[2 * k + 1];
+    }
+    z[n] = ab[n]; z[n + 1] = 0.0;
+    cfft_run(&p, z, +1, z + 2 * n);
+    for (int j = 0; j < n; ++j) x[j] = z[2 * j];
+    free(z);
+    cplan_free(&p);
+}
+
+/* ---------------------------------------------------------------------------
+ * Pressure Poisson solve, FFT99_POIS3D_periodicxz (src/possion3d_FFT99.f90:
+ * 254-488) with the coefficients of TDMA_PHI_COEF (:185-220).
+ *   f:   RHSLLPHI(L, M, N)
+ *   dph: DPH(L, 0:M+1, N); interior written, ghost planes zeroed as in :447.
+ *   cyclic_y: use TDMAIJJ_CYC (ICASE == IBOX3P) instead of TDMAIJJ_nonCYC.
+ * The y solves use exactly the reference's Thomas recurrences (orc_tdma_*).
+ * ------------------------------------------------------------------------- */
+void orc_poisson_fft99(const orc_grid *g, const double *f, double *dph, int cyclic_y)
+{
+    const int L = g->L, M = g->M, N = g->N, LH = L / 2 + 1;
+    const size_t spec = (size_t)LH * M * N;
+    double *Re = (double *)calloc(spec, sizeof(double)); /* (LH, M, N) */
+    double *Im = (double *)calloc(spec, sizeof(double));
+#define SP(kx, j, kz) ((size_t)(kx) + (size_t)LH * ((size_t)(j) + (size_t)M * (size_t)(kz)))
+    /* forward: per y-plane, real FFT in x (1/L), complex FFT in z, x 1/N */
+#pragma omp parallel
+    {
+        cplan px, pz;
+        cplan_init(&px, L);
+        cplan_init(&pz, N);
+        double *zx = (double *)malloc(sizeof(double) * 4 * (size_t)L);
+        double *zz = (double *)malloc(sizeof(double) * 4 * (size_t)N);
+        double *XA = (double *)malloc(sizeof(double) * 2 * (size_t)N * LH);
+#pragma omp for schedule(static)
+        for (int j = 0; j < M; ++j) {
+            for (int k = 0; k < N; ++k) {
+                const double *row = f + (size_t)L * ((size_t)j + (size_t)M * k);
+                for (int i = 0; i < L; ++i) { zx[2 * i] = row[i]; zx[2 * i + 1] = 0.0; }
+                cfft_run(&px, zx, -1, zx + 2 * L);
+                for (int kx = 0; kx < LH; ++kx) {
+                    XA[2 * ((size_t)kx * N + k)] = zx[2 * kx] / (double)L;
+                    XA[2 * ((size_t)kx * N + k) + 1] = (kx == 0 || kx == L / 2) ? 0.0 : zx[2 * kx + 1] / (double)L;
+                }
+            }
+            for (int kx = 0; kx < LH; ++kx) {
+                memcpy(zz, XA + 2 * (size_t)kx * N, sizeof(double) * 2 * (size_t)N);
+                cfft_run(&pz, zz, -1, zz + 2 * N);
+                double UN3M = 1.0 / (double)N;
+                for (int kz = 0; kz < N; ++kz) {
+                    Re[SP(kx, j, kz)] = zz[2 * kz] * UN3M;
+                    Im[SP(kx, j, kz)] = zz[2 * kz + 1] * UN3M;
+                }
+            }
+        }
+        free(zx); free(zz); free(XA);
+        cplan_free(&px);
+        cplan_free(&pz);
+    }
+    Re[SP(0, 0, 0)] = 0.0;
+    Im[SP(0, 0, 0)] = 0.0;
+    /* y-solves: lines (kx, kz), march along j with stride LH */
+#pragma omp parallel
+    {
+        double *A = (double *)malloc(sizeof(double) * (size_t)LH * M);
+        double *B = (double *)malloc(sizeof(double) * (size_t)LH * M);
+        double *C = (double *)malloc(sizeof(double) * (size_t)LH * M);
+#pragma omp for schedule(static)
+        for (int kz = 0; kz < N; ++kz) {
+            for (int kx = 0; kx < LH; ++kx)
+                for (int j = 0; j < M; ++j) {
+                    A[kx + (size_t)LH * j] = g->amph[j];
+                    C[kx + (size_t)LH * j] = g->apph[j];
+                    /* RCCI2 = 1 in the channel */
+                    B[kx + (size_t)LH * j] = g->acph[j] + (-g->ak1[kx] / 1.0 - g->ak3[kz]);
+                }
+            if (kz == 0) {
+                B[0] = g->acph[0] + (-g->ak1[0] / 1.0 - g->ak3[0]);
+                C[0] = 0.0;
+                A[0] = 0.0;
+            }
+            for (int pass = 0; pass < 2; ++pass) {
+                double *F = (pass == 0 ? Re : Im) + (size_t)LH * M * kz;
+                if (kz == 0) F[0] = 0.0;
+                /* lines over kx (ls = 1), rows over j (es = LH) -- the real
+                 * matrices are identical for the two passes */
+                /* orc_tdma_* open their own parallel regions; call serial cores */
+                if (cyclic_y) {
+                    double *H0 = (double *)calloc((size_t)M, sizeof(double));
+                    double *G1 = (double *)calloc((size_t)M, sizeof(double));
+                    double *G2 = (double *)calloc((size_t)M, sizeof(double));
+                    for (int l = 0; l < LH; ++l) {
+                        const double *a = A + l, *b = B + l, *c = C + l;
+                        double *r = F + l;
+                        long es = LH;
+                        int n = M;
+                        H0[n - 2] = 0.0;
+                        H0[0] = c[0] / b[0]; G1[0] = r[0] / b[0]; G2[0] = -a[0] / b[0];
+                        for (int i = 1; i <= n - 3; ++i) {
+                            double PP = 1.0 / (b[i * es] - H0[i - 1] * a[i * es]);
+                            double QQ1 = G1[i - 1] * a[i * es], QQ2 = G2[i - 1] * a[i * es];
+                            H0[i] = c[i * es] * PP; G1[i] = (r[i * es] - QQ1) * PP; G2[i] = -QQ2 * PP;
+                        }
+                        {
+                            int i = n - 2;
+                            double PP = 1.0 / (b[i * es] - H0[i - 1] * a[i * es]);
+                            double QQ1 = G1[i - 1] * a[i * es], QQ2 = G2[i - 1] * a[i * es];
+                            G1[i] = (r[i * es] - QQ1) * PP; G2[i] = (-c[i * es] - QQ2) * PP;
+                        }
+                        for (int i = n - 3; i >= 0; --i) {
+                            G1[i] = -H0[i] * G1[i + 1] + G1[i];
+                            G2[i] = -H0[i] * G2[i + 1] + G2[i];
+                        }
+                        {
+                            int i = n - 1;
+                            double PP = r[i * es] - c[i * es] * G1[0] - a[i * es] * G1[i - 1];
+                            double QQ1 = b[i * es] + c[i * es] * G2[0] + a[i * es] * G2[i - 1];
+                            r[i * es] = PP / QQ1;
+                        }
+                        double xn = r[(n - 1) * es];
+                        for (int i = n - 2; i >= 0; --i) r[i * es] = G1[i] + G2[i] * xn;
+                    }
+                    free(H0); free(G1); free(G2);
+                } else {
+                    double *H = (double *)malloc(sizeof(double) * (size_t)M);
+                    double *G = (double *)malloc(sizeof(double) * (size_t)M);
+                    for (int l = 0; l < LH; ++l) {
+                        const double *a = A + l, *b = B + l, *c = C + l;
+                        double *r = F + l;
+                        long es = LH;
+                        int n = M;
+                        H[0] = c[0] / b[0];
+                        G[0] = (r[0] - a[0] * 0.0) / b[0];
+                        for (int j = 1; j < n - 1; ++j) {
+                            double PP = 1.0 / (b[j * es] - H[j - 1] * a[j * es]);
+                            H[j] = c[j * es] * PP;
+                            G[j] = (r[j * es] - a[j * es] * G[j - 1]) * PP;
+                        }
+                        {
+                            int j = n - 1;
+                            double PP = 1.0 / (b[j * es] - H[j - 1] * a[j * es]);
+                            G[j] = (r[j * es] - c[j * es] * 0.0 - a[j * es] * G[j - 1]) * PP;
+                            r[j * es] = G[j];
+                        }
+                        for (int j = n - 2; j >= 0; --j) r[j * es] = G[j] - H[j] * r[(j + 1) * es];
+                    }
+                    free(H); free(G);
+                }
+            }
+        }
+        free(A); free(B); free(C);
+    }
+    /* inverse: complex FFT in z (+1), real FFT in x (+1), both unnormalised */
+    memset(dph, 0, sizeof(double) * (size_t)L * (M + 2) * N);
+#pragma omp parallel
+    {
+        cplan px, pz;
+        cplan_init(&px, L);
+        cplan_init(&pz, N);
+        double *zx = (double *)malloc(sizeof(double) * 4 * (size_t)L);
+        double *zz = (double *)malloc(sizeof(double) * 4 * (size_t)N);
+        double *XA = (double *)malloc(sizeof(double) * 2 * (size_t)N * LH);
+#pragma omp for schedule(static)
+        for (int j = 0; j < M; ++j) {
+            for (int kx = 0; kx < LH; ++kx) {
+                for (int kz = 0; kz < N; ++kz) {
+                    zz[2 * kz] = Re[SP(kx, j, kz)];
+                    zz[2 * kz + 1] = Im[SP(kx, j, kz)];
+                }
+                cfft_run(&pz, zz, +1, zz + 2 * N);
+                memcpy(XA + 2 * (size_t)kx * N, zz, sizeof(double) * 2 * (size_t)N);
+            }
+            for (int k = 0; k < N; ++k) {
+                /* half-complex -> full Hermitian, b_0 = b_{L/2} = 0 */
+                zx[0] = XA[2 * ((size_t)0 * N + k)]; zx[1] = 0.0;
+                for (int kx = 1; kx < L / 2; ++kx) {
+                    double ar = XA[2 * ((size_t)kx * N + k)], ai = XA[2 * ((size_t)kx * N + k) + 1];
+                    zx[2 * kx] = ar; zx[2 * kx + 1] = ai;
+                    zx[2 * (L - kx)] = ar; zx[2 * (L - kx) + 1] = -ai;
+                }
+                zx[L] = XA[2 * ((size_t)(L / 2) * N + k)]; zx[L + 1] = 0.0;
+                cfft_run(&px, zx, +1, zx + 2 * L);
+                double *row = dph + (size_t)L * ((size_t)(j + 1) + (size_t)(M + 2) * k);
+                for (int i = 0; i < L; ++i) row[i] = zx[2 * i];
+            }
+        }
+        free(zx); free(zz); free(XA);
+        cplan_free(&px);
+        cplan_free(&pz);
+    }
+#undef SP
+    free(Re);
+    free(Im);
+}
+
+/* Residual of the discrete Poisson operator, CHECK_FFT_SOLVER
+ * (src/PRESSURE_CORRECTION.f90:159-202): res = L_h(dph) - f, with DPH ghost
+ * planes taken as given (the reference applies BC_WALL_DPH first).  Returns
+ * max |res| and writes the field if res != NULL. */
+double orc_poisson_residual(const orc_grid *g, const double *dph, const double *f, double *res)
+{
+    const int L = g->L, M = g->M, N = g->N;
+    double mx = 0.0;
+#define PH(i, j, k) dph[(size_t)((i)-1) + (size_t)L * ((size_t)(j) + (size_t)(M + 2) * ((k)-1))]
+#pragma omp parallel for reduction(max : mx) schedule(static)
+    for (int JC = 1; JC <= M; ++JC) {
+        int JM = JC - 1, JP = JC + 1;
+        for (int KC = 1; KC <= N; ++KC) {
+            int KP = KC == N ? 1 : KC + 1, KM = KC == 1 ? N : KC - 1;
+            for (int IC = 1; IC <= L; ++IC) {
+                int IP = IC == L ? 1 : IC + 1, IM = IC == 1 ? L : IC - 1;
+                double lx = (PH(IP, JC, KC) - 2.0 * PH(IC, JC, KC) + PH(IM, JC, KC)) * g->dxqi;
+                double ly = PH(IC, JP, KC) * g->apph[JC - 1] + PH(IC, JC, KC) * g->acph[JC - 1] +
+                            PH(IC, JM, KC) * g->amph[JC - 1];
+                double lz = (PH(IC, JC, KP) - 2.0 * PH(IC, JC, KC) + PH(IC, JC, KM)) * g->dzqi;
+                double r = lx + ly + lz - f[(size_t)(IC - 1) + (size_t)L * ((size_t)(JC - 1) + (size_t)M * (KC - 1))];
+                if (res) res[(size_t)(IC - 1) + (size_t)L * ((size_t)(JC - 1) + (size_t)M * (KC - 1))] = r;
+                if (fabs(r) > mx) mx = fabs(r);
+            }
+        }
+    }
+#undef PH
+    return mx;
+}
+
+/* ---------------------------------------------------------------------------
+ * One RK3 substep of the turbulence-generator channel (tg domain), one rank.
+ * SOLVERRK3_MOM_tg (src/SOLVERRK3_MOM_tg.f90:18-61) and everything it calls.
+ * State: q = Q_tg, pr = PR_tg, convh0 = CONVH0_tg.  Work buffers: dph = DPH_tg,
+ * rhs = RHS_tg, rhsllphi = RHSLLPHI_tg, qtmp = Qtmp_tg; their contents on exit
+ * are those the reference leaves (Qtmp holds Hx, RHS holds the last solve).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    double *q, *pr, *convh0;          /* state */
+    double *dph, *rhs, *rhsllphi, *qtmp; /* work, persistent across calls */
+    double dpgrns;                    /* last mass-flux correction (out) */
+} orc_state;
+
+#define GL (g->L)
+#define GM (g->M)
+#define GN (g->N)
+#define QQ(i, j, k, c) s->q[(size_t)((i)-1) + (size_t)GL * ((size_t)(j) + (size_t)(GM + 2) * ((size_t)((k)-1) + (size_t)GN * ((c)-1)))]
+#define G3(arr, i, j, k) arr[(size_t)((i)-1) + (size_t)GL * ((size_t)(j) + (size_t)(GM + 2) * ((k)-1))]
+#define F3(arr, i, j, k) arr[(size_t)((i)-1) + (size_t)GL * ((size_t)((j)-1) + (size_t)GM * ((k)-1))]
+#define CH0(i, j, k, c) s->convh0[(size_t)((i)-1) + (size_t)GL * ((size_t)((j)-1) + (size_t)GM * ((size_t)((k)-1) + (size_t)GN * ((c)-1)))]
+#define IPV(i) ((i) == GL ? 1 : (i) + 1)
+#define IMV(i) ((i) == 1 ? GL : (i)-1)
+#define KPV(k) ((k) == GN ? 1 : (k) + 1)
+#define KMV(k) ((k) == 1 ? GN : (k)-1)
+#define AMV(JC, IDR) g->amvr[((IDR)-1) * (GM + 1) + (JC)-1]
+#define ACV(JC, IDR) g->acvr[((IDR)-1) * (GM + 1) + (JC)-1]
+#define APV(JC, IDR) g->apvr[((IDR)-1) * (GM + 1) + (JC)-1]
+
+static const double TGAM[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double TROH[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
+static double talp(int ns) { return TGAM[ns] + TROH[ns]; }
+
+/* CONVECTION_X_tg (src/CONVECTION_X.f90:23-74) -> Qtmp */
+void orc_convection_x(const orc_grid *g, orc_state *s)
+{
+    memset(s->qtmp, 0, sizeof(double) * (size_t)GL * GM * GN);
+    double COE1 = 0.5 * 0.5 * g->dxi;
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= GM; ++JC) {
+        int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JJ + 1;
+        double COE2 = g->dyfi[JJ - 1] * 0.5 * 1.0;
+        double COE3 = 0.5 * 0.5 * g->dzi * 1.0;
+        for (int KC = 1; KC <= GN; ++KC) {
+            int KM = KMV(KC), KP = KPV(KC);
+            for (int IC = 1; IC <= GL; ++IC) {
+                int IP = IPV(IC), IM = IMV(IC);
+                double H11 = ((QQ(IP, JC, KC, 1) + QQ(IC, JC, KC, 1)) * (QQ(IP, JC, KC, 1) + QQ(IC, JC, KC, 1)) -
+                              (QQ(IM, JC, KC, 1) + QQ(IC, JC, KC, 1)) * (QQ(IM, JC, KC, 1) + QQ(IC, JC, KC, 1))) * COE1;
+                double H12 = ((QQ(IC, JP, KC, 2) + QQ(IM, JP, KC, 2)) *
+                                  (g->wff[JJP - 1] * QQ(IC, JP, KC, 1) + g->wfb[JJP - 1] * QQ(IC, JC, KC, 1)) -
+                              (QQ(IC, JC, KC, 2) + QQ(IM, JC, KC, 2)) *
+                                  (g->wff[JJ - 1] * QQ(IC, JC, KC, 1) + g->wfb[JJ - 1] * QQ(IC, JM, KC, 1))) * COE2;
+                double H13 = ((QQ(IC, JC, KP, 3) + QQ(IM, JC, KP, 3)) * (QQ(IC, JC, KP, 1) + QQ(IC, JC, KC, 1)) -
+                              (QQ(IC, JC, KC, 3) + QQ(IM, JC, KC, 3)) * (QQ(IC, JC, KC, 1) + QQ(IC, JC, KM, 1))) * COE3;
+                F3(s->qtmp, IC, JC, KC) = -(H11 + H12 + H13);
+            }
+        }
+    }
+}
+
+/* CONVECTION_Y_tg (src/CONVECTION_Y.f90:22-93), channel branch -> DPH interior */
+void orc_convection_y(const orc_grid *g, orc_state *s)
+{
+    memset(s->dph, 0, sizeof(double) * (size_t)GL * (GM + 2) * GN);
+    int NYI = 2;
+    double COE1 = g->dxi * 0.5, COE30 = g->dzi * 0.5;
+#pragma omp parallel for schedule(static)
+    for (int JC = NYI; JC <= GM; ++JC) {
+        int JM = JC - 1, JP = JC + 1, JJ = JC;
+        double COE2 = g->dyci[JJ - 1] * 0.5 * 0.5;
+        double COE31 = COE30 * 1.0;
+        for (int KC = 1; KC <= GN; ++KC) {
+            int KM = KMV(KC), KP = KPV(KC);
+            for (int IC = 1; IC <= GL; ++IC) {
+                int IP = IPV(IC), IM = IMV(IC);
+                double H21 = ((g->wff[JJ - 1] * QQ(IP, JC, KC, 1) + g->wfb[JJ - 1] * QQ(IP, JM, KC, 1)) *
+                                  (QQ(IP, JC, KC, 2) + QQ(IC, JC, KC, 2)) -
+                              (g->wff[JJ - 1] * QQ(IC, JC, KC, 1) + g->wfb[JJ - 1] * QQ(IC, JM, KC, 1)) *
+                                  (QQ(IC, JC, KC, 2) + QQ(IM, JC, KC, 2))) * COE1;
+                double H22 = ((QQ(IC, JP, KC, 2) * 1.0 + QQ(IC, JC, KC, 2) * 1.0) * (QQ(IC, JP, KC, 2) + QQ(IC, JC, KC, 2)) -
+                              (QQ(IC, JC, KC, 2) * 1.0 + QQ(IC, JM, KC, 2) * 1.0) * (QQ(IC, JC, KC, 2) + QQ(IC, JM, KC, 2))) * COE2;
+                double H23 = ((g->wff[JJ - 1] * QQ(IC, JC, KP, 3) * 1.0 + g->wfb[JJ - 1] * QQ(IC, JM, KP, 3) * 1.0) *
+                                  (QQ(IC, JC, KP, 2) + QQ(IC, JC, KC, 2)) -
+                              (g->wff[JJ - 1] * QQ(IC, JC, KC, 3) * 1.0 + g->wfb[JJ - 1] * QQ(IC, JM, KC, 3) * 1.0) *
+                                  (QQ(IC, JC, KC, 2) + QQ(IC, JC, KM, 2))) * COE31;
+                G3(s->dph, IC, JC, KC) = -(H21 + H22 + H23);
+            }
+        }
+    }
+}
+
+/* CONVECTION_Z_tg (src/CONVECTION_Z.f90:22-76), channel branch -> RHSLLPHI */
+void orc_convection_z(const orc_grid *g, orc_state *s)
+{
+    memset(s->rhsllphi, 0, sizeof(double) * (size_t)GL * GM * GN);
+    double COE1 = 0.5 * 0.5 * g->dxi;
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= GM; ++JC) {
+        int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JJ + 1;
+        double COE2 = g->dyfi[JJ - 1] * 0.5;
+        double COE3 = 0.5 * 0.5 * g->dzi * 1.0;
+        for (int KC = 1; KC <= GN; ++KC) {
+            int KM = KMV(KC), KP = KPV(KC);
+            for (int IC = 1; IC <= GL; ++IC) {
+                int IP = IPV(IC), IM = IMV(IC);
+                double H31 = ((QQ(IP, JC, KC, 1) + QQ(IP, JC, KM, 1)) * (QQ(IP, JC, KC, 3) + QQ(IC, JC, KC, 3)) -
+                              (QQ(IC, JC, KC, 1) + QQ(IC, JC, KM, 1)) * (QQ(IC, JC, KC, 3) + QQ(IM, JC, KC, 3))) * COE1;
+                double H32 = ((QQ(IC, JP, KC, 2) + QQ(IC, JP, KM, 2)) *
+                                  (g->wff[JJP - 1] * QQ(IC, JP, KC, 3) * 1.0 + g->wfb[JJP - 1] * QQ(IC, JC, KC, 3) * 1.0) -
+                              (QQ(IC, JC, KC, 2) + QQ(IC, JC, KM, 2)) *
+                                  (g->wff[JJ - 1] * QQ(IC, JC, KC, 3) * 1.0 + g->wfb[JJ - 1] * QQ(IC, JM, KC, 3) * 1.0)) * COE2;
+                double H33 = ((QQ(IC, JC, KP, 3) + QQ(IC, JC, KC, 3)) * (QQ(IC, JC, KP, 3) + QQ(IC, JC, KC, 3)) -
+                              (QQ(IC, JC, KC, 3) + QQ(IC, JC, KM, 3)) * (QQ(IC, JC, KC, 3) + QQ(IC, JC, KM, 3))) * COE3;
+                F3(s->rhsllphi, IC, JC, KC) = -(H31 + H32 + H33);
+            }
+        }
+    }
+}
+
+/* RHS_CvLpGpS_tg (src/RHS_CvLpGpS_tg.f90:29-220), constant mass flux */
+void orc_rhs_tg(const orc_grid *g, orc_state *s, int NS, int IDR, double DT, int flowdrvtp)
+{
+    int NII = (IDR == 2) ? 2 : 1;
+    const double *H = IDR == 1 ? s->qtmp : IDR == 3 ? s->rhsllphi : NULL;
+    double COE1 = talp(NS) * g->cvisc;
+#pragma omp parallel for schedule(static)
+    for (int KC = 1; KC <= GN; ++KC) {
+        int KM = KMV(KC), KP = KPV(KC);
+        for (int JC = NII; JC <= GM; ++JC) {
+            int JM = JC - 1, JP = JC + 1, JJ = JC;
+            double RMC2 = 1.0; /* RNDI2/RCCI2 = 1 in the channel */
+            for (int IC = 1; IC <= GL; ++IC) {
+                int IP = IPV(IC), IM = IMV(IC);
+                double conv = IDR == 2 ? G3(s->dph, IC, JC, KC) : F3(H, IC, JC, KC);
+                double RHSC = TGAM[NS] * conv + TROH[NS] * CH0(IC, JC, KC, IDR);
+                double RHSL = COE1 * ((QQ(IP, JC, KC, IDR) - 2.0 * QQ(IC, JC, KC, IDR) + QQ(IM, JC, KC, IDR)) * g->dxqi +
+                                      (QQ(IC, JC, KP, IDR) - 2.0 * QQ(IC, JC, KC, IDR) + QQ(IC, JC, KM, IDR)) * g->dzqi * RMC2 +
+                                      (QQ(IC, JP, KC, IDR) * APV(JJ, IDR) + QQ(IC, JC, KC, IDR) * ACV(JJ, IDR) +
+                                       QQ(IC, JM, KC, IDR) * AMV(JJ, IDR)));
+                CH0(IC, JC, KC, IDR) = conv;
+                F3(s->rhs, IC, JC, KC) = (RHSC + RHSL) * DT;
+            }
+        }
+    }
+    double COE2 = talp(NS) * DT;
+#pragma omp parallel for schedule(static)
+    for (int KC = 1; KC <= GN; ++KC) {
+        for (int JC = NII; JC <= GM; ++JC) {
+            for (int IC = 1; IC <= GL; ++IC) {
+                double PGM;
+                if (IDR == 1) PGM = (G3(s->pr, IC, JC, KC) - G3(s->pr, IMV(IC), JC, KC)) * g->dxi * COE2;
+                else if (IDR == 2) PGM = (G3(s->pr, IC, JC, KC) - G3(s->pr, IC, JC - 1, KC)) * (g->dyci[JC - 1] / 1.0) * COE2;
+                else PGM = (G3(s->pr, IC, JC, KC) - G3(s->pr, IC, JC, KMV(KC))) * g->dzi * COE2;
+                F3(s->rhs, IC, JC, KC) = F3(s->rhs, IC, JC, KC) - PGM;
+            }
+        }
+    }
+    if (IDR == 1) { /* NFLOW = 1 */
+        double DPGRNS = 0.0;
+        if (flowdrvtp == 1) {
+            double INTGRHSY = 0.0;
+            /* summed in a fixed order (IC, JC, KC) as in the reference */
+            for (int IC = 1; IC <= GL; ++IC)
+                for (int JC = 1; JC <= GM; ++JC)
+                    for (int KC = 1; KC <= GN; ++KC)
+                        INTGRHSY = INTGRHSY + F3(s->rhs, IC, JC, KC) / g->dyfi[JC - 1] / 1.0;
+            DPGRNS = INTGRHSY / g->volm;
+        }
+        s->dpgrns = DPGRNS;
+#pragma omp parallel for schedule(static)
+        for (int K = 1; K <= GN; ++K)
+            for (int I = 1; I <= GL; ++I)
+                for (int J = NII; J <= GM; ++J) F3(s->rhs, I, J, K) = F3(s->rhs, I, J, K) - DPGRNS;
+    }
+}
+
+/* MOMFA_tg (src/MOMFA_tg.f90:17-240): x-, z-, y- CN line solves, Q += RHS */
+void orc_momfa_tg(const orc_grid *g, orc_state *s, int NS, int IDR, double DT)
+{
+    double FACOE = 0.5 * talp(NS) * DT * g->cvisc;
+    int N2I = (IDR == 2) ? 2 : 1;
+    int JSZ = GM - N2I + 1;
+    /* MOMFA1_X_tg: per k-plane, lines along i (contiguous), rows j=N2I..M */
+    {
+        double COE = -FACOE * g->dxqi;
+        double *A = (double *)malloc(sizeof(double) * (size_t)GL * GM);
+        double *B = (double *)malloc(sizeof(double) * (size_t)GL * GM);
+        double *C = (double *)malloc(sizeof(double) * (size_t)GL * GM);
+        for (size_t t = 0; t < (size_t)GL * GM; ++t) {
+            C[t] = COE;
+            A[t] = COE;
+            B[t] = 1.0 - C[t] - A[t];
+        }
+        for (int K = 1; K <= GN; ++K) {
+            double *F = &F3(s->rhs, 1, N2I, K);
+            orc_tdma_cyc(A, B, C, F, GL, JSZ, 1, GL);
+        }
+        free(A); free(B); free(C);
+    }
+    /* MOMFA2_Z_tg: per i, lines along k (stride L*M), rows j=N2I..M */
+    {
+        double *A = (double *)malloc(sizeof(double) * (size_t)GN * GM);
+        double *B = (double *)malloc(sizeof(double) * (size_t)GN * GM);
+        double *C = (double *)malloc(sizeof(double) * (size_t)GN * GM);
+        double *FK = (double *)malloc(sizeof(double) * (size_t)GN * GM);
+        for (int I = 1; I <= GL; ++I) {
+            for (int K = 1; K <= GN; ++K)
+                for (int J = N2I; J <= GM; ++J) {
+                    double RMC2 = 1.0 * (-FACOE * g->dzqi);
+                    size_t t = (size_t)(K - 1) + (size_t)GN * (J - N2I);
+                    C[t] = RMC2;
+                    A[t] = RMC2;
+                    B[t] = 1.0 - C[t] - A[t];
+                    FK[t] = F3(s->rhs, I, J, K);
+                }
+            orc_tdma_cyc(A, B, C, FK, GN, JSZ, 1, GN);
+            for (int K = 1; K <= GN; ++K)
+                for (int J = N2I; J <= GM; ++J) F3(s->rhs, I, J, K) = FK[(size_t)(K - 1) + (size_t)GN * (J - N2I)];
+        }
+        free(A); free(B); free(C); free(FK);
+    }
+    /* MOMFA3_Y_tg: per k-plane, lines along j (stride L) over all j=1..M */
+    {
+        double *A = (double *)malloc(sizeof(double) * (size_t)GL * GM);
+        double *B = (double *)malloc(sizeof(double) * (size_t)GL * GM);
+        double *C = (double *)malloc(sizeof(double) * (size_t)GL * GM);
+        for (int I = 1; I <= GL; ++I)
+            for (int J = 1; J <= GM; ++J) {
+                size_t t = (size_t)(I - 1) + (size_t)GL * (J - 1);
+                B[t] = 1.0 - FACOE * ACV(J, IDR);
+                C[t] = -FACOE * APV(J, IDR);
+                A[t] = -FACOE * AMV(J, IDR);
+            }
+        for (int K = 1; K <= GN; ++K) {
+            double *F = &F3(s->rhs, 1, 1, K);
+            orc_tdma_noncyc(A, B, C, F, NULL, GM, GL, GL, 1);
+        }
+        free(A); free(B); free(C);
+    }
+#pragma omp parallel for schedule(static)
+    for (int K = 1; K <= GN; ++K)
+        for (int J = N2I; J <= GM; ++J)
+            for (int I = 1; I <= GL; ++I) QQ(I, J, K, IDR) = F3(s->rhs, I, J, K) + QQ(I, J, K, IDR);
+}
+
+/* BC_WALL_Q_tg (src/BC_WALL.f90:185-227) after INTFC_VARS3 on one rank */
+void orc_bc_wall_q(const orc_grid *g, orc_state *s)
+{
+    for (int K = 1; K <= GN; ++K)
+        for (int I = 1; I <= GL; ++I) {
+            QQ(I, GM + 1, K, 1) = 0.0;
+            QQ(I, GM + 1, K, 3) = 0.0;
+            QQ(I, GM + 1, K, 2) = 0.0;
+            QQ(I, 0, K, 1) = 0.0;
+            QQ(I, 0, K, 3) = 0.0;
+            QQ(I, 1, K, 2) = 0.0;
+            QQ(I, 0, K, 2) = 0.0;
+        }
+}
+
+/* BC_WALL_PR / BC_WALL_DPH (src/BC_WALL.f90:268-408): Neumann ghost copy */
+void orc_bc_wall_neumann(const orc_grid *g, double *arr)
+{
+    for (int K = 1; K <= GN; ++K)
+        for (int I = 1; I <= GL; ++I) {
+            G3(arr, I, GM + 1, K) = G3(arr, I, GM, K);
+            G3(arr, I, 0, K) = G3(arr, I, 1, K);
+        }
+}
+
+/* DIVG_tg (src/DIVG.f90:21-57) -> RHSLLPHI */
+void orc_divg_tg(const orc_grid *g, orc_state *s, int NS, double DT)
+{
+    double COE0 = 1.0 / (DT * talp(NS));
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= GM; ++JC) {
+        int JP = JC + 1;
+        double COE1 = g->dxi / 1.0, COE2 = g->dyfi[JC - 1] / 1.0;
+        for (int KC = 1; KC <= GN; ++KC) {
+            int KP = KPV(KC);
+            for (int IC = 1; IC <= GL; ++IC) {
+                int IP = IPV(IC);
+                double D = (QQ(IP, JC, KC, 1) - QQ(IC, JC, KC, 1)) * COE1 + (QQ(IC, JP, KC, 2) - QQ(IC, JC, KC, 2)) * COE2 +
+                           (QQ(IC, JC, KP, 3) - QQ(IC, JC, KC, 3)) * g->dzi;
+                F3(s->rhsllphi, IC, JC, KC) = D * COE0;
+            }
+        }
+    }
+}
+
+/* VELOUPDT_tg (src/VELO_CORRECTION.f90:19-75) */
+void orc_veloupdt_tg(const orc_grid *g, orc_state *s, int NS, double DT)
+{
+    double COE1 = DT * talp(NS) * g->dxi;
+#pragma omp parallel for schedule(static)
+    for (int KC = 1; KC <= GN; ++KC)
+        for (int JC = 1; JC <= GM; ++JC)
+            for (int IC = 1; IC <= GL; ++IC)
+                QQ(IC, JC, KC, 1) = QQ(IC, JC, KC, 1) - (G3(s->dph, IC, JC, KC) - G3(s->dph, IMV(IC), JC, KC)) * COE1;
+    COE1 = DT * talp(NS) * g->dzi;
+#pragma omp parallel for schedule(static)
+    for (int KC = 1; KC <= GN; ++KC)
+        for (int JC = 1; JC <= GM; ++JC)
+            for (int IC = 1; IC <= GL; ++IC)
+                QQ(IC, JC, KC, 3) = QQ(IC, JC, KC, 3) - (G3(s->dph, IC, JC, KC) - G3(s->dph, IC, JC, KMV(KC))) * COE1;
+#pragma omp parallel for schedule(static)
+    for (int JC = 2; JC <= GM; ++JC) {
+        double C1 = DT * talp(NS) * g->dyci[JC - 1] / 1.0;
+        for (int KC = 1; KC <= GN; ++KC)
+            for (int IC = 1; IC <= GL; ++IC)
+                QQ(IC, JC, KC, 2) = QQ(IC, JC, KC, 2) - (G3(s->dph, IC, JC, KC) - G3(s->dph, IC, JC - 1, KC)) * C1;
+    }
+}
+
+/* PRCALC_tg (src/PRESSURE_CORRECTION.f90:1-20) */
+void orc_prcalc_tg(const orc_grid *g, orc_state *s, int NS, double DT)
+{
+    double BE = 0.5 * talp(NS) * DT * g->cvisc;
+#pragma omp parallel for schedule(static)
+    for (int KC = 1; KC <= GN; ++KC)
+        for (int JC = 1; JC <= GM; ++JC)
+            for (int IC = 1; IC <= GL; ++IC)
+                G3(s->pr, IC, JC, KC) = G3(s->pr, IC, JC, KC) + G3(s->dph, IC, JC, KC) - BE * F3(s->rhsllphi, IC, JC, KC);
+}
+
+/* DIVGCK_tg (src/DIVGCK.f90:16-58): max |div u| */
+double orc_divgck_tg(const orc_grid *g, const orc_state *s)
+{
+    double DIVX = 0.0;
+#pragma omp parallel for reduction(max : DIVX) schedule(static)
+    for (int K = 1; K <= GN; ++K)
+        for (int J = 1; J <= GM; ++J)
+            for (int I = 1; I <= GL; ++I) {
+                double D = (QQ(IPV(I), J, K, 1) - QQ(I, J, K, 1)) * g->dxi +
+                           (QQ(I, J + 1, K, 2) - QQ(I, J, K, 2)) * g->dyfi[J - 1] * 1.0 +
+                           (QQ(I, J, KPV(K), 3) - QQ(I, J, K, 3)) * g->dzi * 1.0;
+                if (fabs(D) > DIVX) DIVX = fabs(D);
+            }
+    return DIVX;
+}
+
+/* CFL_tg (src/CFL.f90:21-59): max over cells of the convective CFL rate */
+double orc_cfl_tg(const orc_grid *g, const orc_state *s)
+{
+    double CFLMA = 0.0;
+#pragma omp parallel for reduction(max : CFLMA) schedule(static)
+    for (int K = 1; K <= GN; ++K)
+        for (int J = 1; J <= GM; ++J)
+            for (int I = 1; I <= GL; ++I) {
+                double QCF = (fabs((QQ(I, J, K, 1) + QQ(IPV(I), J, K, 1)) * g->dxi) +
+                              fabs((QQ(I, J, K, 2) + QQ(I, J + 1, K, 2)) * g->dyfi[J - 1] * 1.0) +
+                              fabs((QQ(I, J, K, 3) + QQ(I, J, KPV(K), 3)) * g->dzi * 1.0)) * 0.5;
+                if (QCF > CFLMA) CFLMA = QCF;
+            }
+    return CFLMA;
+}
+
+/* SOLVERRK3_MOM_tg(NS) */
+void orc_substep_tg(const orc_grid *g, orc_state *s, int NS, double DT, int flowdrvtp)
+{
+    orc_convection_x(g, s);
+    orc_convection_y(g, s);
+    orc_convection_z(g, s);
+    for (int IDR = 1; IDR <= 3; ++IDR) {
+        orc_rhs_tg(g, s, NS, IDR, DT, flowdrvtp);
+        orc_momfa_tg(g, s, NS, IDR, DT);
+    }
+    orc_bc_wall_q(g, s);
+    orc_divg_tg(g, s, NS, DT);
+    orc_poisson_fft99(g, s->rhsllphi, s->dph, g->icase == IBOX3P);
+    orc_bc_wall_neumann(g, s->dph);
+    orc_veloupdt_tg(g, s, NS, DT);
+    orc_bc_wall_q(g, s);
+    orc_prcalc_tg(g, s, NS, DT);
+    orc_bc_wall_neumann(g, s->pr);
+}
+
+#undef GL
+#undef GM
+#undef GN
+
+/* ---------------------------------------------------------------------------
+ * Thermophysical property tables.
+ * CUBIC_SPLINE (src/Flow_State.f90:1693-1788): Forsythe-Malcolm-Moler spline
+ *   with end conditions from third divided differences.
+ * orc_sleval: NIST_SLEVAL_* (src/Flow_State.f90:1015-1070, 1420-1530):
+ *   binary search for x(i) <= u < x(i+1), then the cubic.  Out of range
+ *   returns status 1 (the reference STOPs).  'hack' selects the post-fix:
+ *   0 none, 1: |u| < 1e-10 -> 1 (HT), 2: |u-1| < 1e-10 -> 1 (TD/TM/TK),
+ *   3: |u-1| < 1e-10 -> 0 (TH).
+ * ------------------------------------------------------------------------- */
+void orc_cubic_spline(int N, const double *X, const double *Y, double *B, double *C, double *D)
+{
+    int NM1 = N - 1;
+    if (N < 2) return;
+    if (N < 3) {
+        B[0] = (Y[1] - Y[0]) / (X[1] - X[0]);
+        C[0] = 0.0; D[0] = 0.0;
+        B[1] = B[0]; C[1] = 0.0; D[1] = 0.0;
+        return;
+    }
+    /* Fortran 1-based -> macros */
+#define X_(i) X[(i)-1]
+#define Y_(i) Y[(i)-1]
+#define B_(i) B[(i)-1]
+#define C_(i) C[(i)-1]
+#define D_(i) D[(i)-1]
+    D_(1) = X_(2) - X_(1);
+    C_(2) = (Y_(2) - Y_(1)) / D_(1);
+    for (int I = 2; I <= NM1; ++I) {
+        D_(I) = X_(I + 1) - X_(I);
+        B_(I) = 2.0 * (D_(I - 1) + D_(I));
+        C_(I + 1) = (Y_(I + 1) - Y_(I)) / D_(I);
+        C_(I) = C_(I + 1) - C_(I);
+    }
+    B_(1) = -D_(1);
+    B_(N) = -D_(N - 1);
+    C_(1) = 0.0;
+    C_(N) = 0.0;
+    if (N != 3) {
+        C_(1) = C_(3) / (X_(4) - X_(2)) - C_(2) / (X_(3) - X_(1));
+        C_(N) = C_(N - 1) / (X_(N) - X_(N - 2)) - C_(N - 2) / (X_(N - 1) - X_(N - 3));
+        C_(1) = C_(1) * D_(1) * D_(1) / (X_(4) - X_(1));
+        C_(N) = -C_(N) * D_(N - 1) * D_(N - 1) / (X_(N) - X_(N - 3));
+    }
+    for (int I = 2; I <= N; ++I) {
+        double T = D_(I - 1) / B_(I - 1);
+        B_(I) = B_(I) - T * D_(I - 1);
+        C_(I) = C_(I) - T * C_(I - 1);
+    }
+    C_(N) = C_(N) / B_(N);
+    for (int L = 1; L <= NM1; ++L) {
+        int I = N - L;
+        C_(I) = (C_(I) - D_(I) * C_(I + 1)) / B_(I);
+    }
+    B_(N) = (Y_(N) - Y_(NM1)) / D_(NM1) + D_(NM1) * (C_(NM1) + 2.0 * C_(N));
+    for (int I = 1; I <= NM1; ++I) {
+        B_(I) = (Y_(I + 1) - Y_(I)) / D_(I) - D_(I) * (C_(I + 1) + 2.0 * C_(I));
+        D_(I) = (C_(I + 1) - C_(I)) / D_(I);
+        C_(I) = 3.0 * C_(I);
+    }
+    C_(N) = 3.0 * C_(N);
+    D_(N) = D_(NM1);
+#undef X_
+#undef Y_
+#undef B_
+#undef C_
+#undef D_
+}
+
+int orc_sleval(int n, const double *x, const double *y, const double *b, const double *c, const double *d,
+               double u, int hack, double *eval)
+{
+    if (u <= x[0] || u >= x[n - 1]) return 1;
+    int i = 1, j = n + 1;
+    while (j > i + 1) {
+        int k = (i + j) / 2;
+        if (u < x[k - 1]) j = k;
+        else i = k;
+    }
+    double dx = u - x[i - 1];
+    double e = y[i - 1] + dx * (b[i - 1] + dx * (c[i - 1] + dx * d[i - 1]));
+    if (hack == 1 && fabs(u) < 1.0e-10) e = 1.0;
+    if (hack == 2 && fabs(u - 1.0) < 1.0e-10) e = 1.0;
+    if (hack == 3 && fabs(u - 1.0) < 1.0e-10) e = 0.0;
+    *eval = e;
+    return 0;
+}
+
+/* THERM_PROP_UPDATE (src/THERM_PROP_UPDATE.f90:129-205) over a flat list of
+ * cells, plus c_p = S_TCp(T) (NIST_SLEVAL_TCp) for the surrogate's fifth
+ * output.  Table arrays (each length n): H, T, D, M, K, CP and the spline
+ * coefficient triples of H->T and T->{D, M, K, CP}:
+ *   tab = [H T D M K CP | HTb HTc HTd | TDb TDc TDd | TMb TMc TMd |
+ *          TKb TKc TKd | TCPb TCPc TCPd]  (21 n doubles)
+ * Cells with |h| < 1e-10 leave their outputs untouched, as the reference
+ * does.  Returns the number of out-of-range cells (the reference STOPs);
+ * those cells are left untouched too. */
+long orc_therm_prop_update(int n, const double *tab, long ncell, const double *h,
+                           double *T, double *rho, double *mu, double *kap, double *cp)
+{
+    const double *H = tab, *TT = tab + n, *DD = tab + 2 * n, *MM = tab + 3 * n, *KK = tab + 4 * n,
+                 *CP = tab + 5 * n;
+    const double *ht = tab + 6 * n, *td = tab + 9 * n, *tm = tab + 12 * n, *tk = tab + 15 * n,
+                 *tc = tab + 18 * n;
+    long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (long q = 0; q < ncell; ++q) {
+        double hv = h[q];
+        if (fabs(hv) < 1.0e-10) continue;
+        double t, dv, mv, kv, cv;
+        if (orc_sleval(n, H, TT, ht, ht + n, ht + 2 * n, hv, 1, &t)) { bad++; continue; }
+        int e1 = orc_sleval(n, TT, MM, tm, tm + n, tm + 2 * n, t, 2, &mv);
+        int e2 = orc_sleval(n, TT, KK, tk, tk + n, tk + 2 * n, t, 2, &kv);
+        int e3 = orc_sleval(n, TT, DD, td, td + n, td + 2 * n, t, 2, &dv);
+        int e4 = orc_sleval(n, TT, CP, tc, tc + n, tc + 2 * n, t, 0, &cv);
+        if (e1 | e2 | e3 | e4) { bad++; continue; }
+        T[q] = t; mu[q] = mv; kap[q] = kv; rho[q] = dv;
+        if (cp) cp[q] = cv;
+    }
+    return bad;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,249 @@
+"""ctypes front-end of the CPU oracle (oracle/chapsim_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline, never as part of the product path.
+
+Also loads oracle/_ref/libfft99_ref.so when it exists: the reference's own
+FFT99 package (libs/fft99/*.f90) compiled with amdflang by oracle/Makefile,
+used to pin the oracle's Fourier conventions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libfft99_ref.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+def build(ref: bool = False) -> None:
+    """Compile the oracle (and optionally the reference FFT99 checker)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if ref and os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _ptr(a: np.ndarray):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous), "need contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+class _Grid(C.Structure):
+    _fields_ = [("L", C.c_int), ("M", C.c_int), ("N", C.c_int), ("icase", C.c_int),
+                ("dx", C.c_double), ("dz", C.c_double), ("dxi", C.c_double), ("dzi", C.c_double),
+                ("dxqi", C.c_double), ("dzqi", C.c_double), ("cvisc", C.c_double), ("volm", C.c_double)] + \
+               [(n, _dp) for n in ("ynd", "ycc", "dyfi", "dyci", "wff", "wfb", "amph", "acph", "apph",
+                                   "amvr", "acvr", "apvr", "ak1", "ak3")]
+
+
+class _State(C.Structure):
+    _fields_ = [(n, _dp) for n in ("q", "pr", "convh0", "dph", "rhs", "rhsllphi", "qtmp")] + \
+               [("dpgrns", C.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_grid_setup.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_tdma_noncyc.argtypes = [_dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_long, C.c_long]
+        L.orc_tdma_cyc.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_long, C.c_long]
+        L.orc_cfft99.argtypes = [_dp, C.c_int, C.c_int]
+        L.orc_fft99_fwd.argtypes = [_dp, _dp, C.c_int]
+        L.orc_fft99_inv.argtypes = [_dp, _dp, C.c_int]
+        L.orc_poisson_fft99.argtypes = [C.POINTER(_Grid), _dp, _dp, C.c_int]
+        L.orc_poisson_residual.argtypes = [C.POINTER(_Grid), _dp, _dp, _dp]
+        L.orc_poisson_residual.restype = C.c_double
+        for name in ("orc_convection_x", "orc_convection_y", "orc_convection_z", "orc_bc_wall_q"):
+            getattr(L, name).argtypes = [C.POINTER(_Grid), C.POINTER(_State)]
+        L.orc_rhs_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_int, C.c_double, C.c_int]
+        L.orc_momfa_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_int, C.c_double]
+        for name in ("orc_divg_tg", "orc_veloupdt_tg", "orc_prcalc_tg"):
+            getattr(L, name).argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_double]
+        L.orc_bc_wall_neumann.argtypes = [C.POINTER(_Grid), _dp]
+        L.orc_divgck_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State)]
+        L.orc_divgck_tg.restype = C.c_double
+        L.orc_cfl_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State)]
+        L.orc_cfl_tg.restype = C.c_double
+        L.orc_substep_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_double, C.c_int]
+        L.orc_cubic_spline.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_sleval.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_sleval.restype = C.c_int
+        L.orc_therm_prop_update.argtypes = [C.c_int, _dp, C.c_long, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_therm_prop_update.restype = C.c_long
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_fft99():
+    """The reference FFT99 package, or None when oracle/_ref was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        _ref = C.CDLL(REF_PATH)
+    return _ref
+
+
+# --------------------------------------------------------------------------
+class OracleGrid:
+    """The oracle's own restatement of COORDJR/TDMA_COEF/FFT99_ROOT."""
+
+    def __init__(self, L, M, N, hx=2 * np.pi, hz=np.pi, str2=2.0, ren=2800.0):
+        self.L, self.M, self.N = L, M, N
+        self.arr = {
+            "ynd": np.zeros(M + 1), "ycc": np.zeros(M), "dyfi": np.zeros(M), "dyci": np.zeros(M + 1),
+            "wff": np.zeros(M + 1), "wfb": np.zeros(M + 1), "amph": np.zeros(M), "acph": np.zeros(M),
+            "apph": np.zeros(M), "amvr": np.zeros(3 * (M + 1)), "acvr": np.zeros(3 * (M + 1)),
+            "apvr": np.zeros(3 * (M + 1)), "ak1": np.zeros(L), "ak3": np.zeros(N),
+        }
+        g = _Grid()
+        g.L, g.M, g.N, g.icase = L, M, N, 1
+        for k, v in self.arr.items():
+            setattr(g, k, _ptr(v))
+        lib().orc_grid_setup(C.byref(g), hx, hz, str2, ren)
+        self.g = g
+        for k in ("dx", "dz", "dxi", "dzi", "dxqi", "dzqi", "cvisc", "volm"):
+            setattr(self, k, getattr(g, k))
+        for k, v in self.arr.items():
+            setattr(self, k, v)
+
+
+class OracleState:
+    """Reference-layout state and work arrays for the tg substep."""
+
+    def __init__(self, og: OracleGrid, q, pr, convh0):
+        L, M, N = og.L, og.M, og.N
+        f = lambda a: np.array(a, dtype=np.float64, order="F", copy=True)
+        self.q, self.pr, self.convh0 = f(q), f(pr), f(convh0)
+        self.dph = np.zeros((L, M + 2, N), order="F")
+        self.rhs = np.zeros((L, M, N), order="F")
+        self.rhsllphi = np.zeros((L, M, N), order="F")
+        self.qtmp = np.zeros((L, M, N), order="F")
+        s = _State()
+        for k in ("q", "pr", "convh0", "dph", "rhs", "rhsllphi", "qtmp"):
+            setattr(s, k, _ptr(getattr(self, k)))
+        s.dpgrns = 0.0
+        self.s = s
+        self.og = og
+
+    @property
+    def dpgrns(self):
+        return self.s.dpgrns
+
+    def substep(self, ns: int, dt: float, flowdrvtp: int = 1):
+        lib().orc_substep_tg(C.byref(self.og.g), C.byref(self.s), ns, dt, flowdrvtp)
+
+    def divgck(self) -> float:
+        return lib().orc_divgck_tg(C.byref(self.og.g), C.byref(self.s))
+
+    def cfl(self) -> float:
+        return lib().orc_cfl_tg(C.byref(self.og.g), C.byref(self.s))
+
+    def stage(self, name: str, *args):
+        fn = getattr(lib(), "orc_" + name)
+        return fn(C.byref(self.og.g), C.byref(self.s), *args)
+
+
+# --------------------------------------------------------------------------
+def tdma_noncyc(a, b, c, r, bc, n, nlines, es, ls):
+    lib().orc_tdma_noncyc(_ptr(a), _ptr(b), _ptr(c), _ptr(r), _ptr(bc), n, nlines, es, ls)
+
+
+def tdma_cyc(a, b, c, r, n, nlines, es, ls):
+    lib().orc_tdma_cyc(_ptr(a), _ptr(b), _ptr(c), _ptr(r), n, nlines, es, ls)
+
+
+def fft99_fwd(x):
+    n = x.shape[0]
+    ab = np.zeros(n + 2)
+    lib().orc_fft99_fwd(_ptr(np.ascontiguousarray(x, np.float64)), _ptr(ab), n)
+    return ab
+
+
+def fft99_inv(ab, n):
+    x = np.zeros(n)
+    lib().orc_fft99_inv(_ptr(np.ascontiguousarray(ab, np.float64)), _ptr(x), n)
+    return x
+
+
+def cfft99(z, isign):
+    a = np.ascontiguousarray(np.asarray(z, np.complex128)).view(np.float64).copy()
+    lib().orc_cfft99(_ptr(a), a.shape[0] // 2, isign)
+    return a.view(np.complex128)
+
+
+def poisson(og: OracleGrid, f, cyclic_y=False):
+    dph = np.zeros((og.L, og.M + 2, og.N), order="F")
+    lib().orc_poisson_fft99(C.byref(og.g), _ptr(np.asfortranarray(f)), _ptr(dph), int(cyclic_y))
+    return dph
+
+
+def poisson_residual(og: OracleGrid, dph, f):
+    """max |L_h(dph) - f| with DPH ghosts set by the Neumann wall rule."""
+    d = np.array(dph, order="F", copy=True)
+    lib().orc_bc_wall_neumann(C.byref(og.g), _ptr(d))
+    res = np.zeros((og.L, og.M, og.N), order="F")
+    mx = lib().orc_poisson_residual(C.byref(og.g), _ptr(d), _ptr(np.asfortranarray(f)), _ptr(res))
+    return mx, res
+
+
+def cubic_spline(x, y):
+    n = len(x)
+    b, c, d = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_cubic_spline(n, _ptr(np.ascontiguousarray(x, np.float64)), _ptr(np.ascontiguousarray(y, np.float64)),
+                           _ptr(b), _ptr(c), _ptr(d))
+    return b, c, d
+
+
+def therm_prop_update(tab, n, h, T, rho, mu, kap, cp=None):
+    return lib().orc_therm_prop_update(n, _ptr(tab), h.size, _ptr(h), _ptr(T), _ptr(rho), _ptr(mu), _ptr(kap), _ptr(cp))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# ---- reference FFT99 package (Fortran, by-reference arguments) -----------
+def ref_fft99_transform(data, n, isign):
+    """FFT99 on LOT vectors stored as columns of data (n+2, LOT), INC=1, JUMP=n+2."""
+    R = ref_fft99()
+    a = np.array(data, dtype=np.float64, order="F", copy=True)
+    lot = a.shape[1]
+    ifax = np.zeros(13, np.int32)
+    trigs = np.zeros(3 * n // 2 + 1)
+    ip = lambda v: C.byref(C.c_int(v))
+    R.fftfax_(ip(n), ifax.ctypes.data_as(C.POINTER(C.c_int)), _ptr(trigs))
+    work = np.zeros((n + 1) * lot + 16)
+    R.fft99_(_ptr(a), _ptr(work), _ptr(trigs), ifax.ctypes.data_as(C.POINTER(C.c_int)),
+             ip(1), ip(n + 2), ip(n), ip(lot), ip(isign))
+    return a
+
+
+def ref_cfft99_transform(z, isign):
+    """CFFT99 on LOT complex vectors stored as columns of z (n, LOT), INC=1, JUMP=n."""
+    R = ref_fft99()
+    a = np.array(z, dtype=np.complex128, order="F", copy=True)
+    n, lot = a.shape
+    ifax = np.zeros(13, np.int32)
+    trigs = np.zeros(2 * n)
+    ip = lambda v: C.byref(C.c_int(v))
+    R.cftfax_(ip(n), ifax.ctypes.data_as(C.POINTER(C.c_int)), _ptr(trigs))
+    work = np.zeros(4 * n * lot + 16)
+    R.cfft99_(a.ctypes.data_as(_dp), _ptr(work), _ptr(trigs), ifax.ctypes.data_as(C.POINTER(C.c_int)),
+              ip(1), ip(n), ip(n), ip(lot), ip(isign))
+    return a
