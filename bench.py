@@ -1,0 +1,257 @@
+"""Throughput of the per-RK3-substep solver core at 256x192x256 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one RK3 substep (SOLVERRK3_MOM_tg) of the tg channel at
+256x192x256 (Re_b = 6880, seeded laminar + noise start) on the
+device-resident FP64 state; the timed region holds exactly K substeps
+(NS cycling 1, 2, 3), bracketed by device synchronisation.  The JSON line
+also carries Poisson solves/s and batched-TDMA rates, the mixed-precision
+form, the per-kernel HBM roofline of the dominant kernel (HIP events on the
+library's stream) and the CPU oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+CFG = dict(L=256, M=192, N=256, ren=6880.0)
+
+# algorithmic bytes per cell and launch (FP64 fields x 8 B; half-complex rows
+# count L reals), the figures behind roofline.achieved
+def alg_bytes_per_cell(L, precision="fp64"):
+    sp = 16.0 * (L // 2 + 1) / L        # one spectral sweep (complex, LH per L)
+    if precision == "mixed":
+        sp32 = sp / 2.0
+        return {"fft_x_fwd": 8 + sp32, "fft_z": 2 * sp32, "poisson_y": 2 * sp32, "fft_x_inv": sp32 + 8,
+                "poisson_residual": 24.0,
+                "rhs_x": 104.0, "z_solve": 48.0, "y_solve": 72.0, "divg": 32.0, "update": 80.0}
+    return {"rhs_x": 104.0,          # read Q x3, PR, CONVH0 x3; write RHS x3, CONVH0 x3
+            "z_solve": 48.0,         # read/write RHS x3
+            "y_solve": 72.0,         # read RHS x3, Q x3; write Q x3
+            "divg": 32.0,            # read Q x3; write f
+            "fft_x_fwd": 8 + sp,     # read f; write S
+            "fft_z": 2 * sp,         # read/write S
+            "poisson_y": 2 * sp,     # read/write S
+            "fft_x_inv": sp + 8,     # read S; write phi
+            "update": 80.0}          # read phi, f, Q x3, PR; write Q x3, PR
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def run_substeps(solver, dt, nsteps, ns0=0):
+    seq = [1, 2, 3]
+    for i in range(nsteps):
+        solver.solverrk3_mom_tg(seq[(ns0 + i) % 3], dt)
+
+
+def gpu_bench(steps, warmup):
+    import torch
+    from chapsim1ww_amd import fields
+    from chapsim1ww_amd.grid import ChannelGrid
+    from chapsim1ww_amd.solver import ChannelSolver, tdma_batched
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    g = ChannelGrid(CFG["L"], CFG["M"], CFG["N"], ren=CFG["ren"])
+    q, pr, c0 = fields.init_tg_fields(g, seed=1973)
+    res = {}
+    ncell = g.ncell
+
+    for prec in ("fp64", "mixed"):
+        s = ChannelSolver(g, prec, stream=stream)
+        s.upload_state(q, pr, c0)
+        rate = s.cfl()
+        dt = 0.5 / rate
+        s.solverrk3_mom_tg(0, dt)      # the reference's initial NS=0 step
+        run_substeps(s, dt, warmup)
+        _sync()
+        t0 = time.perf_counter()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run_substeps(s, dt, steps, warmup)
+        e1.record(stream)
+        _sync()
+        wall = time.perf_counter() - t0
+        ms = e0.elapsed_time(e1) / steps
+        div = s.divgck()
+        # per-kernel durations with HIP events over a second region of K substeps
+        s.profile(True)
+        s.profile_reset()
+        run_substeps(s, dt, steps, warmup + steps)
+        _sync()
+        prof = s.profile_read()
+        s.profile(False)
+        iters = s.poisson_stats()
+        # Poisson solves/s on the handle's workspace
+        f = torch.from_numpy(np.asfortranarray(fields.poisson_rhs(g, seed=9)).ravel(order="F").copy()).to(dev)
+        dph = torch.zeros(g.L * (g.M + 2) * g.N, dtype=torch.float64, device=dev)
+        if prec == "mixed":
+            s.set_refinement(1e-10, 10)
+        for _ in range(3):
+            s.fft99_pois3d(f, dph)
+        _sync()
+        npois = max(20, steps)
+        p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        p0.record(stream)
+        for _ in range(npois):
+            s.fft99_pois3d(f, dph)
+        p1.record(stream)
+        _sync()
+        pois_ms = p0.elapsed_time(p1) / npois
+        res[prec] = dict(ms_per_substep=ms, wall_s=wall, maxdiv=div, dt=dt, prof=prof,
+                         poisson_ms=pois_ms, poisson_refine=s.poisson_stats(), substep_refine=iters)
+        s.close()
+        del f, dph
+
+    # batched TDMA, 65536 lines x 192 (lines interleaved: IJJ form)
+    n, nl = 192, 65536
+    rng = np.random.default_rng(2)
+    a = torch.from_numpy(rng.uniform(-0.9, -0.1, n * nl)).to(dev)
+    c = torch.from_numpy(rng.uniform(-0.9, -0.1, n * nl)).to(dev)
+    b = torch.from_numpy(2.0 + np.abs(rng.standard_normal(n * nl))).to(dev)
+    d0 = torch.from_numpy(rng.standard_normal(n * nl)).to(dev)
+    r = d0.clone()
+    tdma = {}
+    for cyc in (False, True):
+        for _ in range(3):
+            r.copy_(d0)
+            tdma_batched(a, b, c, r, None, n, nl, nl, 1, cyc, stream)
+        _sync()
+        reps = 50
+        t_tot = 0.0
+        for _ in range(reps):
+            r.copy_(d0)
+            q0, q1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            q0.record(stream)
+            tdma_batched(a, b, c, r, None, n, nl, nl, 1, cyc, stream)
+            q1.record(stream)
+            q1.synchronize()
+            t_tot += q0.elapsed_time(q1)
+        ms_t = t_tot / reps
+        tdma["cyclic" if cyc else "noncyclic"] = dict(ms=ms_t, batches_per_s=1e3 / ms_t,
+                                                      gbs=40.0 * n * nl / (ms_t * 1e-3) / 1e9)
+    res["tdma"] = tdma
+    res["ncell"] = ncell
+    return res
+
+
+def cpu_baseline(max_seconds=30.0):
+    """The CPU oracle (C, OpenMP) on this host at the same workload: a bounded
+    sample of substeps and Poisson solves."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from chapsim1ww_amd import fields
+    from chapsim1ww_amd.grid import ChannelGrid
+
+    g = ChannelGrid(CFG["L"], CFG["M"], CFG["N"], ren=CFG["ren"])
+    og = O.OracleGrid(CFG["L"], CFG["M"], CFG["N"], ren=CFG["ren"])
+    q, pr, c0 = fields.init_tg_fields(g, seed=1973)
+    st = O.OracleState(og, q, pr, c0)
+    dt = 0.5 / st.cfl()
+    st.substep(0, dt)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        st.substep(1 + n % 3, dt)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > max_seconds / 2 or n >= 3:
+            break
+    sub_rate = n / el
+    f = fields.poisson_rhs(g, seed=9)
+    m, t1 = 0, time.perf_counter()
+    while True:
+        O.poisson(og, f)
+        m += 1
+        el2 = time.perf_counter() - t1
+        if el2 > max_seconds / 4 or m >= 5:
+            break
+    return dict(substeps_per_s=sub_rate, nsub=n, poisson_per_s=m / el2, npois=m, cores=O.num_threads())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is design-only, see DESIGN.md)"}))
+        sys.exit(2)
+    res = gpu_bench(args.steps, args.warmup)
+    ncell = res["ncell"]
+    f64 = res["fp64"]
+    mixed = res["mixed"]
+    # roofline of the dominant kernel of the FP64 substep
+    ab = alg_bytes_per_cell(CFG["L"])
+    kern = {}
+    for name, (ms_tot, calls) in f64["prof"].items():
+        if name in ab and calls:
+            avg = ms_tot / calls
+            kern[name] = dict(avg_ms=avg, calls=calls, total_ms=ms_tot,
+                              gbs=ab[name] * ncell / (avg * 1e-3) / 1e9)
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(dom)
+    roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(kern[dom]["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
+                alg_bytes_per_launch=ab[dom] * ncell, avg_launch_ms=round(kern[dom]["avg_ms"], 5))
+    cpu = None
+    if not args.no_cpu:
+        cb = cpu_baseline()
+        cpu = dict(value=round(cb["substeps_per_s"], 4), unit="substeps/s", cores=cb["cores"], kind="port",
+                   sample=f"{cb['nsub']} FP64 RK3 substeps of the C oracle at 256x192x256 "
+                          f"(+{cb['npois']} Poisson solves: {cb['poisson_per_s']:.3f}/s), OpenMP",
+                   poisson_per_s=round(cb["poisson_per_s"], 4))
+    sub_s = 1e3 / f64["ms_per_substep"]
+    out = {
+        "metric": "RK3 substeps/s & Poisson solves/s at 256x192x256, div≤1e-10, mixed vs FP64",
+        "value": round(sub_s, 3),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f64["ms_per_substep"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded laminar Poiseuille + U(-0.1,0.1) noise)",
+        "config": {"workload": "tg channel RK3 substep, 256x192x256, Re_b=6880, FP64, device-resident state",
+                   "grid": "256x192x256", "parallelism": "single GPU"},
+        "grid_point_substeps_per_s": round(sub_s * ncell, 1),
+        "poisson_solves_per_s": round(1e3 / f64["poisson_ms"], 2),
+        "max_div_after_timed_steps": f64["maxdiv"],
+        "mixed": {"substeps_per_s": round(1e3 / mixed["ms_per_substep"], 3),
+                  "ms_per_substep": round(mixed["ms_per_substep"], 4),
+                  "speedup_vs_fp64": round(f64["ms_per_substep"] / mixed["ms_per_substep"], 3),
+                  "poisson_solves_per_s": round(1e3 / mixed["poisson_ms"], 2),
+                  "poisson_refine_iters_rel": mixed["poisson_refine"],
+                  "max_div_after_timed_steps": mixed["maxdiv"]},
+        "tdma_65536x192": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res["tdma"].items()},
+        "kernels_fp64": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kern.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+// Shared definitions of the CHAPSim MI355X library: error handling, the
+// handle, complex helpers and launch utilities.  gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/chapsim_gpu.h"
+
+namespace chs {
+
+// ---------------------------------------------------------------------------
+// errors: every export returns 0 or a CHS_E_* code; the last message is kept
+// per thread and returned by chs_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+#define CHS_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::chs::set_error(std::string(#call) + ": " + hipGetErrorString(e_));        \
+            return CHS_E_HIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+#define CHS_CHECK(cond, code, msg)                                                      \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            ::chs::set_error(msg);                                                      \
+            return (code);                                                              \
+        }                                                                               \
+    } while (0)
+
+#define CHS_TRY(call)                                                                   \
+    do {                                                                                \
+        int s_ = (call);                                                                \
+        if (s_ != CHS_OK) return s_;                                                    \
+    } while (0)
+
+// launch + immediate launch-error check
+#define CHS_LAUNCH(kernel, grid, block, shmem, stream, ...)                             \
+    do {                                                                                \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);            \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess) {                                                         \
+            ::chs::set_error(std::string("launch " #kernel ": ") + hipGetErrorString(e_)); \
+            return CHS_E_HIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// complex arithmetic on HIP vector types
+// ---------------------------------------------------------------------------
+template <typename R> struct cpx;
+template <> struct cpx<double> { using t = double2; };
+template <> struct cpx<float> { using t = float2; };
+template <typename R> using cpx_t = typename cpx<R>::t;
+
+__host__ __device__ inline double2 mk(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline float2 mk(float r, float i) { return make_float2(r, i); }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (dir = -1) or +i (dir = +1)
+template <typename C> __device__ inline C cmul_i(C a, int dir) {
+    return dir < 0 ? C{a.y, -a.x} : C{-a.y, a.x};
+}
+
+// scalar-or-complex helpers used by the tridiagonal solver (real matrices,
+// real or complex right-hand sides)
+__device__ inline double vscale(double v, double s) { return v * s; }
+__device__ inline double2 vscale(double2 v, double s) { return make_double2(v.x * s, v.y * s); }
+__device__ inline double vaxpy(double v, double a, double x) { return v - a * x; }        // v - a x
+__device__ inline double2 vaxpy(double2 v, double a, double2 x) { return make_double2(v.x - a * x.x, v.y - a * x.y); }
+__device__ inline double vfma(double v, double a, double x) { return v + a * x; }         // v + a x
+__device__ inline double2 vfma(double2 v, double a, double2 x) { return make_double2(v.x + a * x.x, v.y + a * x.y); }
+template <typename V> __device__ inline V vzero();
+template <> __device__ inline double vzero<double>() { return 0.0; }
+template <> __device__ inline double2 vzero<double2>() { return make_double2(0.0, 0.0); }
+
+// ---------------------------------------------------------------------------
+// MI355X has 8 XCDs with private L2s; consecutive workgroup ids are dealt
+// round-robin over the XCDs.  Remap so that each XCD walks a contiguous range
+// of tiles: neighbouring tiles (which share halo rows) then share an L2.
+// ---------------------------------------------------------------------------
+__device__ inline int xcd_tile(int bid, int nblk) {
+    const int nx = 8;
+    int per = (nblk + nx - 1) / nx;
+    int t = (bid % nx) * per + bid / nx;
+    return t;  // may be >= nblk for the padding blocks when nblk % 8 != 0
+}
+
+// ---------------------------------------------------------------------------
+// per-kernel timing with HIP events (enabled by chs_profile_enable)
+// ---------------------------------------------------------------------------
+struct KernelTimer {
+    bool on = false;
+    struct Rec { std::string name; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    std::vector<std::pair<std::string, std::pair<double, long>>> totals;  // name -> (ms, count)
+    void begin(const char* name, hipStream_t s, hipEvent_t* out_a);
+    void end(hipEvent_t a, const char* name, hipStream_t s);
+    void collect();
+};
+
+}  // namespace chs
+
+// The handle behind the opaque chs_handle of the C ABI.
+struct chs_handle {
+    int L, M, N, LH, LHP;
+    int icase, flowdrvtp, precision;
+    double dxi, dzi, dxqi, dzqi, cvisc, volm;
+    hipStream_t stream;
+    bool own_stream;
+    // 1-D coefficients on the device (copied at create)
+    double *dyfi, *dyci, *wff, *wfb, *amph, *acph, *apph, *amvr, *acvr, *apvr, *ak1, *ak3;
+    // FFT twiddle tables W_n^e = exp(-2 pi i e / n), FP64 and FP32 copies
+    double2 *twx, *twz;
+    float2 *twx32, *twz32;
+    // state in the reference layouts
+    double *q, *pr, *convh0;
+    // work: DPH(L,0:M+1,N), RHS x3 (L,M,N), RHSLLPHI(L,M,N), spectral S
+    double *dph, *rhs, *rhsllphi;
+    double2 *spec;
+    float2 *spec32;
+    double *rhs_j1;      // the u-increment on j = 1, which becomes v's row-1 RHS
+    double *partials;    // per-block partial sums / maxima
+    double *scalars;     // [0] DPGRNS, [1] max|div|, [2] CFL, [3..] scratch
+    double *resid;       // mixed-precision residual / correction fields
+    float *f32a, *f32b;
+    int npart;
+    chs::KernelTimer timer;
+    // mixed-precision Poisson refinement: tolerance, cap, statistics of the last call
+    double refine_tol;
+    int refine_max;
+    int last_refine_iters;
+    double last_refine_resid;
+};

@@ -1,0 +1,360 @@
+// Pressure Poisson solve, the MI355X form of FFT99_POIS3D_periodicxz
+// (src/possion3d_FFT99.f90:254-488):
+//
+//   x: real FFT of every (j,k) row          k_fft_x_fwd   (LDS Stockham, 2 rows per complex FFT)
+//   z: complex FFT along k for every (j,kx) k_fft_z       (LDS Stockham, tile of TX kx columns)
+//   y: tridiagonal solve per (kx,kz) line   k_poisson_y   (segmented Thomas, FP64, complex RHS)
+//   z, x inverse transforms                 k_fft_z, k_fft_x_inv
+//
+// The spectral array S is complex, layout S[(kz*M + j)*LHP + kx], so rows of
+// kx are contiguous: the x transform writes whole rows, the z transform reads
+// 128-byte column tiles and the y solve reads 64 consecutive kx per wave.
+// The reference's two separate Re/Im line solves with the same real matrix
+// (:331-443) become one solve with a complex right-hand side.  The transposes
+// TRASP23_Y2Z/Z2Y (src/TRASP23.f90) disappear: on one GPU every direction is
+// addressed in place.
+//
+// Precision: the FFT kernels are templated on the real type (FP64 for the
+// plain form, FP32 for the mixed form); the y elimination is always FP64.
+#include "chs_common.hpp"
+#include "fft_dev.hpp"
+#include "tdma_dev.hpp"
+
+namespace chs {
+
+constexpr int XROWS = 8;   // rows per block in the x transforms (4 complex FFTs)
+constexpr int ZTX = 8;     // kx columns per block in the z transform
+
+// ---- x forward: f rows (real, FP64 input) -> S rows (half spectrum) -------
+// mode 0: input f; mode 1: input = f - L_h(phi) (FP64 residual, for refinement)
+template <typename Rl>
+__global__ void __launch_bounds__(256) k_fft_x_fwd(const double* __restrict__ f, cpx_t<Rl>* __restrict__ S, int L,
+                                                   int nrows, int LHP, FftPlan plan, const cpx_t<Rl>* __restrict__ tw) {
+    using C = cpx_t<Rl>;
+    extern __shared__ unsigned char smem_raw[];
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + (XROWS / 2) * L;
+    const int row0 = blockIdx.x * XROWS;
+    const int npair = XROWS / 2;
+    for (int e = threadIdx.x; e < npair * L; e += blockDim.x) {
+        const int p = e / L, i = e - p * L;
+        const int ra = row0 + 2 * p, rb = ra + 1;
+        const Rl va = ra < nrows ? (Rl)f[(size_t)ra * L + i] : Rl(0);
+        const Rl vb = rb < nrows ? (Rl)f[(size_t)rb * L + i] : Rl(0);
+        b0[p * L + i] = C{va, vb};
+    }
+    __syncthreads();
+    C* Z = fft_lds<Rl>(b0, b1, npair, L, plan, tw, -1);
+    const int LH = L / 2 + 1;
+    for (int e = threadIdx.x; e < npair * LH; e += blockDim.x) {
+        const int p = e / LH, kx = e - p * LH;
+        const C z = Z[p * L + kx];
+        const C zc = Z[p * L + (kx == 0 ? 0 : L - kx)];
+        // A = (z + conj(zc))/2, B = (z - conj(zc))/(2i)
+        const C A = C{Rl(0.5) * (z.x + zc.x), Rl(0.5) * (z.y - zc.y)};
+        const C B = C{Rl(0.5) * (z.y + zc.y), Rl(-0.5) * (z.x - zc.x)};
+        const int ra = row0 + 2 * p, rb = ra + 1;
+        if (ra < nrows) S[(size_t)ra * LHP + kx] = A;
+        if (rb < nrows) S[(size_t)rb * LHP + kx] = B;
+    }
+}
+
+// ---- z transform, in place on S: block (j, kx tile) --------------------------
+template <typename Rl>
+__global__ void __launch_bounds__(256) k_fft_z(cpx_t<Rl>* __restrict__ S, int N, int M, int LH, int LHP, int dir,
+                                               FftPlan plan, const cpx_t<Rl>* __restrict__ tw) {
+    using C = cpx_t<Rl>;
+    extern __shared__ unsigned char smem_raw[];
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + ZTX * N;
+    const int j = blockIdx.y;
+    const int kx0 = blockIdx.x * ZTX;
+    const int tx = min(ZTX, LH - kx0);
+    const size_t kst = (size_t)M * LHP;
+    for (int e = threadIdx.x; e < N * ZTX; e += blockDim.x) {
+        const int k = e / ZTX, c = e - k * ZTX;
+        if (c < tx) b0[c * N + k] = S[(size_t)j * LHP + k * kst + kx0 + c];
+    }
+    __syncthreads();
+    C* Z = fft_lds<Rl>(b0, b1, tx, N, plan, tw, dir);
+    for (int e = threadIdx.x; e < N * ZTX; e += blockDim.x) {
+        const int k = e / ZTX, c = e - k * ZTX;
+        if (c < tx) S[(size_t)j * LHP + k * kst + kx0 + c] = Z[c * N + k];
+    }
+}
+
+// ---- y: tridiagonal solve per (kx, kz) line, FP64 --------------------------
+// Matrix of TDMA_PHI_COEF (src/possion3d_FFT99.f90:185-220):
+//   a_j = AMPH(j), c_j = APPH(j), b_j = ACPH(j) + (-AK1(kx)/RCCI2(j) - AK3(kz)),
+// and on the (0,0) line row 1 pinned: a = c = 0, f = 0 (:206-212, :324-327, :370-372).
+template <int MS, typename Rl>
+__global__ void __launch_bounds__(768) k_poisson_y(cpx_t<Rl>* __restrict__ S, int M, int LH, int LHP, int nlines,
+                                                    int LPB, int P, double scale, const double* __restrict__ amph,
+                                                    const double* __restrict__ acph, const double* __restrict__ apph,
+                                                    const double* __restrict__ ak1, const double* __restrict__ ak3) {
+    extern __shared__ unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int ll = tid % LPB, seg = tid / LPB;
+    const int line = blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const int kz = valid ? line / LH : 0, kx = valid ? line - kz * LH : 0;
+    const size_t base = (size_t)kz * M * LHP + kx;
+    const double lam = -ak1[kx] - ak3[kz];
+    double a[MS], b[MS], c[MS];
+    double2 d[MS];
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        const int j = seg * MS + l;
+        if (valid && j < M) {
+            const cpx_t<Rl> v = S[base + (size_t)j * LHP];
+            d[l] = make_double2((double)v.x * scale, (double)v.y * scale);
+            a[l] = amph[j];
+            c[l] = apph[j];
+            b[l] = acph[j] + lam;
+        } else {
+            a[l] = 0.0;
+            b[l] = 1.0;
+            c[l] = 0.0;
+            d[l] = make_double2(0.0, 0.0);
+        }
+    }
+    if (seg == 0) {
+        a[0] = 0.0;
+        if (kx == 0 && kz == 0) {
+            c[0] = 0.0;
+            d[0] = make_double2(0.0, 0.0);
+        }
+    }
+    seg_solve<MS, double2>(a, b, c, d, LPB, P, ll, seg, valid, false, smem);
+    if (valid) {
+#pragma unroll
+        for (int l = 0; l < MS; ++l) {
+            const int j = seg * MS + l;
+            if (j < M) S[base + (size_t)j * LHP] = cpx_t<Rl>{(Rl)d[l].x, (Rl)d[l].y};
+        }
+    }
+}
+
+// ---- x inverse: S rows -> DPH rows ------------------------------------------
+// ghost: 0 = zero the ghost planes (reference FFT99_POIS3D), 1 = Neumann copy
+// (BC_WALL_DPH), 2 = accumulate into dph (refinement: phi += delta), no ghosts.
+template <typename Rl>
+__global__ void __launch_bounds__(256) k_fft_x_inv(const cpx_t<Rl>* __restrict__ S, double* __restrict__ dph, int L,
+                                                   int M, int nrows, int LHP, int ghost, FftPlan plan,
+                                                   const cpx_t<Rl>* __restrict__ tw) {
+    using C = cpx_t<Rl>;
+    extern __shared__ unsigned char smem_raw[];
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + (XROWS / 2) * L;
+    const int row0 = blockIdx.x * XROWS;
+    const int npair = XROWS / 2;
+    const int LH = L / 2 + 1;
+    for (int e = threadIdx.x; e < npair * LH; e += blockDim.x) {
+        const int p = e / LH, kx = e - p * LH;
+        const int ra = row0 + 2 * p, rb = ra + 1;
+        C A = ra < nrows ? S[(size_t)ra * LHP + kx] : C{Rl(0), Rl(0)};
+        C B = rb < nrows ? S[(size_t)rb * LHP + kx] : C{Rl(0), Rl(0)};
+        if (kx == 0 || 2 * kx == L) {  // b_0 = b_{L/2} = 0 (FFT99 ordering)
+            A.y = Rl(0);
+            B.y = Rl(0);
+        }
+        // Z[kx] = A + iB ; Z[L-kx] = conj(A) + i conj(B)
+        b0[p * L + kx] = C{A.x - B.y, A.y + B.x};
+        if (kx != 0 && 2 * kx != L) b0[p * L + (L - kx)] = C{A.x + B.y, B.x - A.y};
+    }
+    __syncthreads();
+    C* Z = fft_lds<Rl>(b0, b1, npair, L, plan, tw, +1);
+    const size_t plane = (size_t)L * (M + 2);
+    for (int e = threadIdx.x; e < XROWS * L; e += blockDim.x) {
+        const int r = e / L, i = e - r * L;
+        const int row = row0 + r;
+        if (row >= nrows) continue;
+        const int p = r >> 1;
+        const double v = (r & 1) ? (double)Z[p * L + i].y : (double)Z[p * L + i].x;
+        const int j = row % M, k = row / M;
+        double* col = dph + (size_t)k * plane + i;
+        if (ghost == 2) {
+            col[(size_t)(j + 1) * L] += v;
+            continue;
+        }
+        col[(size_t)(j + 1) * L] = v;
+        if (j == 0) col[0] = ghost == 1 ? v : 0.0;
+        if (j == M - 1) col[(size_t)(M + 1) * L] = ghost == 1 ? v : 0.0;
+    }
+}
+
+// ---- FP64 residual of the discrete operator (CHECK_FFT_SOLVER,
+// src/PRESSURE_CORRECTION.f90:169-199): r = f - L_h(phi), with the Neumann
+// wall rule for the y ghosts; also the block max of |r| and |f|.
+__global__ void __launch_bounds__(256) k_poisson_residual(const double* __restrict__ phi, const double* __restrict__ f,
+                                                          double* __restrict__ r, int L, int M, int N, double dxqi,
+                                                          double dzqi, const double* __restrict__ amph,
+                                                          const double* __restrict__ acph,
+                                                          const double* __restrict__ apph, double* __restrict__ part) {
+    __shared__ double red[2][256];
+    const size_t ncell = (size_t)L * M * N;
+    const size_t plane = (size_t)L * (M + 2);
+    double mr = 0.0, mf = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = t % L;
+        const int j = (t / L) % M;
+        const int k = t / ((size_t)L * M);
+        const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
+        const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
+        const double* c0 = phi + (size_t)k * plane + (size_t)(j + 1) * L;
+        const double pc = c0[i];
+        const double pjp = j == M - 1 ? pc : c0[L + i];
+        const double pjm = j == 0 ? pc : c0[i - L];
+        const double lx = (c0[ip] - 2.0 * pc + c0[im]) * dxqi;
+        const double ly = pjp * apph[j] + pc * acph[j] + pjm * amph[j];
+        const double lz = (phi[(size_t)kp * plane + (size_t)(j + 1) * L + i] - 2.0 * pc +
+                           phi[(size_t)km * plane + (size_t)(j + 1) * L + i]) * dzqi;
+        const double fv = f[t];
+        const double rv = fv - (lx + ly + lz);
+        r[t] = rv;
+        mr = fmax(mr, fabs(rv));
+        mf = fmax(mf, fabs(fv));
+    }
+    red[0][threadIdx.x] = mr;
+    red[1][threadIdx.x] = mf;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = red[0][0];
+        part[2 * blockIdx.x + 1] = red[1][0];
+    }
+}
+
+// Max over the block partials written by k_poisson_residual -> out[0..1]
+__global__ void k_reduce_max2(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    __shared__ double red[2][256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        a = fmax(a, part[2 * i]);
+        b = fmax(b, part[2 * i + 1]);
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+    }
+}
+
+__global__ void k_zero(double* __restrict__ p, size_t n) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        p[t] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr);
+
+template <typename Rl>
+static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, cpx_t<Rl>* S, const cpx_t<Rl>* twx,
+                        const cpx_t<Rl>* twz) {
+    const int L = h->L, M = h->M, N = h->N, LH = h->LH, LHP = h->LHP;
+    const hipStream_t s = h->stream;
+    FftPlan px, pz;
+    fft_plan(L, &px);
+    fft_plan(N, &pz);
+    const int nrows = M * N;
+    const dim3 gx((nrows + XROWS - 1) / XROWS);
+    const size_t smx = sizeof(cpx_t<Rl>) * (size_t)XROWS * L;  // two buffers of XROWS/2 vectors
+    hipEvent_t ev;
+    h->timer.begin("fft_x_fwd", s, &ev);
+    CHS_LAUNCH(k_fft_x_fwd<Rl>, gx, dim3(256), smx, s, f, S, L, nrows, LHP, px, twx);
+    h->timer.end(ev, "fft_x_fwd", s);
+    const dim3 gz((LH + ZTX - 1) / ZTX, M);
+    const size_t smz = sizeof(cpx_t<Rl>) * (size_t)2 * ZTX * N;
+    h->timer.begin("fft_z", s, &ev);
+    CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, -1, pz, twz);
+    h->timer.end(ev, "fft_z", s);
+    int MS, P, LPB;
+    if (!tdma_seg_config(M, false, &MS, &P, &LPB, 768)) {
+        set_error("poisson: unsupported NCL2");
+        return CHS_E_ARG;
+    }
+    const int nlines = LH * N;
+    const dim3 gy((nlines + LPB - 1) / LPB), by(LPB * P);
+    const size_t smy = seg_smem_bytes<double2>(LPB, P);
+    const double scale = 1.0 / ((double)L * (double)N);
+    h->timer.begin("poisson_y", s, &ev);
+    switch (MS) {
+        case 16: CHS_LAUNCH((k_poisson_y<16, Rl>), gy, by, smy, s, S, M, LH, LHP, nlines, LPB, P, scale, h->amph, h->acph, h->apph, h->ak1, h->ak3); break;
+        case 12: CHS_LAUNCH((k_poisson_y<12, Rl>), gy, by, smy, s, S, M, LH, LHP, nlines, LPB, P, scale, h->amph, h->acph, h->apph, h->ak1, h->ak3); break;
+        default: CHS_LAUNCH((k_poisson_y<8, Rl>), gy, by, smy, s, S, M, LH, LHP, nlines, LPB, P, scale, h->amph, h->acph, h->apph, h->ak1, h->ak3); break;
+    }
+    h->timer.end(ev, "poisson_y", s);
+    h->timer.begin("fft_z", s, &ev);
+    CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, +1, pz, twz);
+    h->timer.end(ev, "fft_z", s);
+    h->timer.begin("fft_x_inv", s, &ev);
+    CHS_LAUNCH(k_fft_x_inv<Rl>, gx, dim3(256), smx, s, S, dph, L, M, nrows, LHP, ghost, px, twx);
+    h->timer.end(ev, "fft_x_inv", s);
+    return CHS_OK;
+}
+
+// Ghost planes of DPH after a refinement (mode 0: zero, 1: Neumann copy).
+__global__ void k_dph_ghost(double* __restrict__ dph, int L, int M, int N, int mode) {
+    const size_t plane = (size_t)L * (M + 2);
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)L * N; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = t % L, k = t / L;
+        double* col = dph + (size_t)k * plane + i;
+        col[0] = mode == 1 ? col[L] : 0.0;
+        col[(size_t)(M + 1) * L] = mode == 1 ? col[(size_t)M * L] : 0.0;
+    }
+}
+
+// FP64 form: one pass.  Mixed form: FP32 pass on f, then FP32 passes on the
+// FP64 residual until max|f - L phi| <= max(abs_tol, rel_tol * max|f|).
+int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol) {
+    if (h->precision == CHS_FP64) return poisson_pass<double>(h, f, dph, ghost, h->spec, h->twx, h->twz);
+    const hipStream_t s = h->stream;
+    CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
+    const size_t ncell = (size_t)h->L * h->M * h->N;
+    const int nb = 1024;
+    hipEvent_t ev;
+    h->last_refine_iters = 0;
+    h->last_refine_resid = -1.0;
+    double host[2] = {0.0, 0.0};
+    for (int it = 0; it <= h->refine_max; ++it) {
+        h->timer.begin("poisson_residual", s, &ev);
+        CHS_LAUNCH(k_poisson_residual, dim3(nb), dim3(256), 0, s, dph, f, h->resid, h->L, h->M, h->N, h->dxqi,
+                   h->dzqi, h->amph, h->acph, h->apph, h->partials);
+        h->timer.end(ev, "poisson_residual", s);
+        CHS_LAUNCH(k_reduce_max2, dim3(1), dim3(256), 0, s, h->partials, nb, h->scalars + 4);
+        CHS_HIP(hipMemcpyAsync(host, h->scalars + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        CHS_HIP(hipStreamSynchronize(s));
+        const double rel = host[1] > 0.0 ? host[0] / host[1] : 0.0;
+        h->last_refine_resid = rel;
+        if (host[0] <= abs_tol || rel <= h->refine_tol) break;
+        if (it == h->refine_max) {
+            set_error("poisson: mixed-precision refinement did not reach its tolerance");
+            return CHS_E_NOTCONV;
+        }
+        CHS_TRY(poisson_pass<float>(h, h->resid, dph, 2, h->spec32, h->twx32, h->twz32));
+        h->last_refine_iters = it + 1;
+    }
+    if (h->last_refine_iters > 0 && ghost != 2)
+        CHS_LAUNCH(k_dph_ghost, dim3(256), dim3(256), 0, s, dph, h->L, h->M, h->N, ghost);
+    (void)ncell;
+    return CHS_OK;
+}
+
+}  // namespace chs

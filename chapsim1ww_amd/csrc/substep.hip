@@ -1,0 +1,550 @@
+// One RK3 substep of the turbulence-generator channel on the device-resident
+// state: SOLVERRK3_MOM_tg(NS) (src/SOLVERRK3_MOM_tg.f90:18-61).
+//
+//   k_rhs_x     CONVECTION_X/Y/Z_tg + RHS_CvLpGpS_tg for u, v, w, fused with
+//               the x line solve of MOMFA1_X_tg (rows staged in LDS) and the
+//               block sums of the constant-mass-flux integral
+//   k_massflux  DPGRNS = sum / VOLM_tg (deterministic order)
+//   k_z_solve   MOMFA2_Z_tg for u, v, w (u: minus DPGRNS pushed through the x solve)
+//   k_y_solve   MOMFA3_Y_tg for u, v, w + Q += RHS (MOMFA_tg:39-45) + BC_WALL_Q_tg
+//   k_divg      DIVG_tg
+//   Poisson     FFT99_POIS3D_periodicxz + BC_WALL_DPH_tg (poisson.hip)
+//   k_update    VELOUPDT_tg + BC_WALL_Q_tg + PRCALC_tg + BC_WALL_PR_tg
+//
+// Layouts are the reference's: Q(L, 0:M+1, N, 3), PR/DPH(L, 0:M+1, N),
+// RHS(L, M, N) x 3 components, CONVH0(L, M, N, 3), RHSLLPHI(L, M, N).
+// Indices below are 0-based: i in [0,L), j in [0,M) (Fortran J = j+1, stored
+// at ghost-aware row j+1), k in [0,N).
+#include <algorithm>
+
+#include "chs_common.hpp"
+#include "tdma_dev.hpp"
+
+namespace chs {
+
+struct Geo {
+    int L, M, N;
+    double dxi, dzi, dxqi, dzqi;
+    const double *dyfi, *dyci, *wff, *wfb, *amvr, *acvr, *apvr;
+};
+
+__device__ __forceinline__ size_t qidx(const Geo& g, int c, int i, int jg, int k) {
+    return (size_t)i + (size_t)g.L * ((size_t)jg + (size_t)(g.M + 2) * ((size_t)k + (size_t)g.N * c));
+}
+__device__ __forceinline__ size_t gidx(const Geo& g, int i, int jg, int k) {
+    return (size_t)i + (size_t)g.L * ((size_t)jg + (size_t)(g.M + 2) * k);
+}
+__device__ __forceinline__ size_t cidx(const Geo& g, int i, int j, int k) {
+    return (size_t)i + (size_t)g.L * ((size_t)j + (size_t)g.M * k);
+}
+
+constexpr int KR = 4;        // k rows per k_rhs_x block
+constexpr int XLPB = 16;     // line slots of the x solve (3*KR = 12 used)
+constexpr int XMS = 8;       // rows per segment of the x solve
+
+// Convection H_c at (i, j, k) (CONVECTION_X/Y/Z_tg, channel: RCCI1 = RNDI1 = RCCI2 = 1;
+// XND2CL = YND2CL = ZND2CL = 1/2).  Q is read through ghost-aware rows.
+__device__ __forceinline__ double conv_x(const Geo& g, const double* __restrict__ Q, int i, int j, int k) {
+    const int ip = i == g.L - 1 ? 0 : i + 1, im = i == 0 ? g.L - 1 : i - 1;
+    const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
+    const int J = j + 1;
+    const double COE1 = 0.5 * 0.5 * g.dxi;
+    const double COE2 = g.dyfi[j] * 0.5 * 1.0;
+    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
+    const double u_c = Q[qidx(g, 0, i, J, k)], u_p = Q[qidx(g, 0, ip, J, k)], u_m = Q[qidx(g, 0, im, J, k)];
+    const double H11 = ((u_p + u_c) * (u_p + u_c) - (u_m + u_c) * (u_m + u_c)) * COE1;
+    const double H12 = ((Q[qidx(g, 1, i, J + 1, k)] + Q[qidx(g, 1, im, J + 1, k)]) *
+                            (g.wff[j + 1] * Q[qidx(g, 0, i, J + 1, k)] + g.wfb[j + 1] * u_c) -
+                        (Q[qidx(g, 1, i, J, k)] + Q[qidx(g, 1, im, J, k)]) *
+                            (g.wff[j] * u_c + g.wfb[j] * Q[qidx(g, 0, i, J - 1, k)])) * COE2;
+    const double H13 = ((Q[qidx(g, 2, i, J, kp)] + Q[qidx(g, 2, im, J, kp)]) * (Q[qidx(g, 0, i, J, kp)] + u_c) -
+                        (Q[qidx(g, 2, i, J, k)] + Q[qidx(g, 2, im, J, k)]) * (u_c + Q[qidx(g, 0, i, J, km)])) * COE3;
+    return -(H11 + H12 + H13);
+}
+
+__device__ __forceinline__ double conv_y(const Geo& g, const double* __restrict__ Q, int i, int j, int k) {
+    const int ip = i == g.L - 1 ? 0 : i + 1, im = i == 0 ? g.L - 1 : i - 1;
+    const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
+    const int J = j + 1;
+    const double COE1 = g.dxi * 0.5;
+    const double COE2 = g.dyci[j] * 0.5 * 0.5;
+    const double COE31 = g.dzi * 0.5 * 1.0;
+    const double wf = g.wff[j], wb = g.wfb[j];
+    const double v_c = Q[qidx(g, 1, i, J, k)];
+    const double H21 = ((wf * Q[qidx(g, 0, ip, J, k)] + wb * Q[qidx(g, 0, ip, J - 1, k)]) * (Q[qidx(g, 1, ip, J, k)] + v_c) -
+                        (wf * Q[qidx(g, 0, i, J, k)] + wb * Q[qidx(g, 0, i, J - 1, k)]) * (v_c + Q[qidx(g, 1, im, J, k)])) * COE1;
+    const double v_p = Q[qidx(g, 1, i, J + 1, k)], v_m = Q[qidx(g, 1, i, J - 1, k)];
+    const double H22 = ((v_p * 1.0 + v_c * 1.0) * (v_p + v_c) - (v_c * 1.0 + v_m * 1.0) * (v_c + v_m)) * COE2;
+    const double H23 = ((wf * Q[qidx(g, 2, i, J, kp)] * 1.0 + wb * Q[qidx(g, 2, i, J - 1, kp)] * 1.0) * (Q[qidx(g, 1, i, J, kp)] + v_c) -
+                        (wf * Q[qidx(g, 2, i, J, k)] * 1.0 + wb * Q[qidx(g, 2, i, J - 1, k)] * 1.0) * (v_c + Q[qidx(g, 1, i, J, km)])) * COE31;
+    return -(H21 + H22 + H23);
+}
+
+__device__ __forceinline__ double conv_z(const Geo& g, const double* __restrict__ Q, int i, int j, int k) {
+    const int ip = i == g.L - 1 ? 0 : i + 1, im = i == 0 ? g.L - 1 : i - 1;
+    const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
+    const int J = j + 1;
+    const double COE1 = 0.5 * 0.5 * g.dxi;
+    const double COE2 = g.dyfi[j] * 0.5;
+    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
+    const double w_c = Q[qidx(g, 2, i, J, k)];
+    const double H31 = ((Q[qidx(g, 0, ip, J, k)] + Q[qidx(g, 0, ip, J, km)]) * (Q[qidx(g, 2, ip, J, k)] + w_c) -
+                        (Q[qidx(g, 0, i, J, k)] + Q[qidx(g, 0, i, J, km)]) * (w_c + Q[qidx(g, 2, im, J, k)])) * COE1;
+    const double H32 = ((Q[qidx(g, 1, i, J + 1, k)] + Q[qidx(g, 1, i, J + 1, km)]) *
+                            (g.wff[j + 1] * Q[qidx(g, 2, i, J + 1, k)] * 1.0 + g.wfb[j + 1] * w_c * 1.0) -
+                        (Q[qidx(g, 1, i, J, k)] + Q[qidx(g, 1, i, J, km)]) *
+                            (g.wff[j] * w_c * 1.0 + g.wfb[j] * Q[qidx(g, 2, i, J - 1, k)] * 1.0)) * COE2;
+    const double w_p = Q[qidx(g, 2, i, J, kp)], w_m = Q[qidx(g, 2, i, J, km)];
+    const double H33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * COE3;
+    return -(H31 + H32 + H33);
+}
+
+struct RkCoef {
+    double tgam, troh, talp, dt;
+};
+
+// ---------------------------------------------------------------------------
+// k_rhs_x: block = (j, KR consecutive k).  Phase 1 computes RHS_c for the 3
+// components of its rows into LDS and writes CONVH0; phase 2 solves the x
+// lines (cyclic, constant coefficients) from LDS and writes RHS.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
+                                               const double* __restrict__ PR, double* __restrict__ H0,
+                                               double* __restrict__ RHS, double* __restrict__ part, double xa,
+                                               double xb, double xc) {
+    extern __shared__ unsigned char smem[];
+    const int L = g.L;
+    const int LS = L + 1;  // padded LDS row stride (bank spread for the segment reads)
+    // LDS: sR (the 3*KR RHS rows) | red (block sum); the reduced systems of
+    // the x solves reuse the front once sR has been read into registers
+    double* sR = reinterpret_cast<double*>(smem);
+    unsigned char* sred = smem + sizeof(double) * 3 * KR * LS;
+    const int nkb = g.N / KR;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const bool active = t < g.M * nkb;
+    const int j = active ? t / nkb : 0;
+    const int k0 = active ? (t - j * nkb) * KR : 0;
+    const size_t LMN = (size_t)L * g.M * g.N;
+    const int J = j + 1;
+    const double COE1 = rk.talp * cvisc;
+    const double COE2 = rk.talp * rk.dt;
+    double msum = 0.0;
+    if (active) {
+        for (int e = threadIdx.x; e < KR * L; e += blockDim.x) {
+            const int r = e / L, i = e - r * L;
+            const int k = k0 + r;
+            const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
+            const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (c == 1 && j == 0) continue;  // v on the wall face is not advanced (NII = 2)
+                const double conv = c == 0 ? conv_x(g, Q, i, j, k) : c == 1 ? conv_y(g, Q, i, j, k) : conv_z(g, Q, i, j, k);
+                const size_t ci = cidx(g, i, j, k) + c * LMN;
+                const double RHSC = rk.tgam * conv + rk.troh * H0[ci];
+                const double qc = Q[qidx(g, c, i, J, k)];
+                const double RHSL = COE1 * ((Q[qidx(g, c, ip, J, k)] - 2.0 * qc + Q[qidx(g, c, im, J, k)]) * g.dxqi +
+                                            (Q[qidx(g, c, i, J, kp)] - 2.0 * qc + Q[qidx(g, c, i, J, km)]) * g.dzqi * 1.0 +
+                                            (Q[qidx(g, c, i, J + 1, k)] * g.apvr[c * (g.M + 1) + j] +
+                                             qc * g.acvr[c * (g.M + 1) + j] +
+                                             Q[qidx(g, c, i, J - 1, k)] * g.amvr[c * (g.M + 1) + j]));
+                H0[ci] = conv;
+                double rhs = (RHSC + RHSL) * rk.dt;
+                double pgm;
+                if (c == 0)
+                    pgm = (PR[gidx(g, i, J, k)] - PR[gidx(g, im, J, k)]) * g.dxi * COE2;
+                else if (c == 1)
+                    pgm = (PR[gidx(g, i, J, k)] - PR[gidx(g, i, J - 1, k)]) * (g.dyci[j] / 1.0) * COE2;
+                else
+                    pgm = (PR[gidx(g, i, J, k)] - PR[gidx(g, i, J, km)]) * g.dzi * COE2;
+                rhs = rhs - pgm;
+                if (c == 0) msum += rhs / g.dyfi[j] / 1.0;
+                sR[(c * KR + r) * LS + i] = rhs;
+            }
+        }
+    }
+    // block sum of the mass-flux integrand (deterministic tree)
+    {
+        double* red = reinterpret_cast<double*>(sred);
+        red[threadIdx.x] = msum;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && active) part[t] = red[0];
+        __syncthreads();
+    }
+    // phase 2: x solves of the 3*KR rows, XMS rows per segment, L/XMS segments
+    const int P = L / XMS;
+    const int ll = threadIdx.x % XLPB, seg = threadIdx.x / XLPB;
+    const int c = ll / KR, r = ll - c * KR;
+    const bool valid = active && ll < 3 * KR && !(c == 1 && j == 0);
+    double a[XMS], b[XMS], cc[XMS], d[XMS];
+#pragma unroll
+    for (int l = 0; l < XMS; ++l) {
+        a[l] = xa;
+        b[l] = xb;
+        cc[l] = xc;
+        d[l] = valid ? sR[(c * KR + r) * LS + seg * XMS + l] : 0.0;
+    }
+    __syncthreads();  // sR fully read: its space now holds the reduced systems
+    seg_solve<XMS, double>(a, b, cc, d, XLPB, P, ll, seg, valid, true, smem);
+    if (valid) {
+        const size_t base = cidx(g, 0, j, k0 + r) + c * LMN;
+#pragma unroll
+        for (int l = 0; l < XMS; ++l) RHS[base + seg * XMS + l] = d[l];
+    }
+}
+
+// DPGRNS = (sum of block partials) / VOLM_tg, summed in a fixed order.
+__global__ void k_massflux(const double* __restrict__ part, int nb, double volm, double* __restrict__ out) {
+    __shared__ double red[1024];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] / volm;
+}
+
+// ---------------------------------------------------------------------------
+// k_z_solve: cyclic z lines (MOMFA2_Z_tg), lines (c, i, j), rows k.
+// The mass-flux correction of u is applied here: x-solve(R - D) =
+// x-solve(R) - D * xone, xone = 1/(a+b+c) of the constant x system.
+// ---------------------------------------------------------------------------
+template <int MS>
+__global__ void __launch_bounds__(1024) k_z_solve(Geo g, double* __restrict__ RHS, const double* __restrict__ dpg,
+                                                  double xone, double za, double zb, double zc, int LPB, int P,
+                                                  int use_dpg) {
+    extern __shared__ unsigned char smem[];
+    const int LM = g.L * g.M;
+    const int nlines = 3 * LM;
+    const int ll = threadIdx.x % LPB, seg = threadIdx.x / LPB;
+    const int bt = blockIdx.x;
+    const int line = bt * LPB + ll;
+    const int c = line / LM;
+    const int rem = line - c * LM;
+    const int j = rem / g.L;
+    const bool valid = line < nlines && !(c == 1 && j == 0);
+    const size_t LMN = (size_t)LM * g.N;
+    const size_t base = valid ? (size_t)c * LMN + rem : 0;
+    const double D = (use_dpg && c == 0) ? dpg[0] * xone : 0.0;
+    double a[MS], b[MS], cc[MS], d[MS];
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        const int k = seg * MS + l;
+        a[l] = za;
+        b[l] = zb;
+        cc[l] = zc;
+        d[l] = valid ? RHS[base + (size_t)k * LM] - D : 0.0;
+    }
+    seg_solve<MS, double>(a, b, cc, d, LPB, P, ll, seg, valid, true, smem);
+    if (valid) {
+#pragma unroll
+        for (int l = 0; l < MS; ++l) RHS[base + (size_t)(seg * MS + l) * LM] = d[l];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_y_solve: y lines (MOMFA3_Y_tg) for u, then v, then w of the same (i,k)
+// lines, all rows j = 1..M; then Q += RHS from N2I (MOMFA_tg:39-45) and the
+// wall ghost values (BC_WALL_Q_tg).  v's row-1 right-hand side is the RHS_tg
+// value the reference leaves there: the u increment on j = 1.
+// ---------------------------------------------------------------------------
+template <int MS>
+__global__ void __launch_bounds__(768) k_y_solve(Geo g, double facoe, const double* __restrict__ RHS,
+                                                  double* __restrict__ Q, int LPB, int P) {
+    extern __shared__ unsigned char smem[];
+    const int L = g.L, M = g.M;
+    const int nlines = L * g.N;
+    const int ll = threadIdx.x % LPB, seg = threadIdx.x / LPB;
+    const int line = blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const int k = valid ? line / L : 0, i = valid ? line - k * L : 0;
+    const size_t LMN = (size_t)L * M * g.N;
+    const size_t base = (size_t)i + (size_t)L * M * k;
+    double du_row1 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        double a[MS], b[MS], cc[MS], d[MS];
+#pragma unroll
+        for (int l = 0; l < MS; ++l) {
+            const int j = seg * MS + l;
+            if (j < M) {
+                const int o = c * (M + 1) + j;
+                b[l] = 1.0 - facoe * g.acvr[o];
+                cc[l] = -facoe * g.apvr[o];
+                a[l] = -facoe * g.amvr[o];
+                d[l] = valid ? RHS[c * LMN + base + (size_t)j * L] : 0.0;
+            } else {
+                a[l] = 0.0;
+                b[l] = 1.0;
+                cc[l] = 0.0;
+                d[l] = 0.0;
+            }
+        }
+        if (seg == 0) {
+            a[0] = 0.0;  // BCJ = 0
+            if (c == 1) d[0] = du_row1;
+        }
+        {
+            const int jl = M - 1;
+            if (jl / MS == seg) {
+#pragma unroll
+                for (int l = 0; l < MS; ++l)
+                    if (l == jl % MS) cc[l] = 0.0;
+            }
+        }
+        seg_solve<MS, double>(a, b, cc, d, LPB, P, ll, seg, valid, false, smem);
+        if (c == 0 && seg == 0) du_row1 = d[0];
+        if (valid) {
+            const int n2i = c == 1 ? 1 : 0;
+#pragma unroll
+            for (int l = 0; l < MS; ++l) {
+                const int j = seg * MS + l;
+                if (j < M && j >= n2i) {
+                    const size_t qo = qidx(g, c, i, j + 1, k);
+                    Q[qo] = d[l] + Q[qo];
+                }
+            }
+            if (seg == 0) {
+                Q[qidx(g, c, i, 0, k)] = 0.0;
+                if (c == 1) Q[qidx(g, 1, i, 1, k)] = 0.0;
+                Q[qidx(g, c, i, M + 1, k)] = 0.0;
+            }
+        }
+        __syncthreads();  // the LDS of seg_solve is reused by the next component
+    }
+}
+
+// DIVG_tg (src/DIVG.f90:21-57): f = div(Q) / (DT * TALP)
+__global__ void __launch_bounds__(256) k_divg(Geo g, double coe0, const double* __restrict__ Q, double* __restrict__ F) {
+    const size_t ncell = (size_t)g.L * g.M * g.N;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = t % g.L;
+        const int j = (t / g.L) % g.M;
+        const int k = t / ((size_t)g.L * g.M);
+        const int ip = i == g.L - 1 ? 0 : i + 1, kp = k == g.N - 1 ? 0 : k + 1;
+        const int J = j + 1;
+        const double COE1 = g.dxi / 1.0, COE2 = g.dyfi[j] / 1.0;
+        const double D = (Q[qidx(g, 0, ip, J, k)] - Q[qidx(g, 0, i, J, k)]) * COE1 +
+                         (Q[qidx(g, 1, i, J + 1, k)] - Q[qidx(g, 1, i, J, k)]) * COE2 +
+                         (Q[qidx(g, 2, i, J, kp)] - Q[qidx(g, 2, i, J, k)]) * g.dzi;
+        F[t] = D * coe0;
+    }
+}
+
+// VELOUPDT_tg (src/VELO_CORRECTION.f90:37-75) + PRCALC_tg
+// (src/PRESSURE_CORRECTION.f90:11-20) + BC_WALL_Q_tg / BC_WALL_PR_tg.
+__global__ void __launch_bounds__(256) k_update(Geo g, double cu, double cw, double cv, double be,
+                                                const double* __restrict__ DPH, const double* __restrict__ F,
+                                                double* __restrict__ Q, double* __restrict__ PR) {
+    const size_t ncell = (size_t)g.L * g.M * g.N;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = t % g.L;
+        const int j = (t / g.L) % g.M;
+        const int k = t / ((size_t)g.L * g.M);
+        const int im = i == 0 ? g.L - 1 : i - 1, km = k == 0 ? g.N - 1 : k - 1;
+        const int J = j + 1;
+        const double ph = DPH[gidx(g, i, J, k)];
+        const size_t qu = qidx(g, 0, i, J, k), qv = qidx(g, 1, i, J, k), qw = qidx(g, 2, i, J, k);
+        Q[qu] = Q[qu] - (ph - DPH[gidx(g, im, J, k)]) * cu;
+        Q[qw] = Q[qw] - (ph - DPH[gidx(g, i, J, km)]) * cw;
+        if (j >= 1) Q[qv] = Q[qv] - (ph - DPH[gidx(g, i, J - 1, k)]) * (cv * g.dyci[j] / 1.0);
+        const size_t po = gidx(g, i, J, k);
+        const double p = PR[po] + ph - be * F[t];
+        PR[po] = p;
+        if (j == 0) PR[gidx(g, i, 0, k)] = p;
+        if (j == g.M - 1) PR[gidx(g, i, g.M + 1, k)] = p;
+    }
+}
+
+__device__ __forceinline__ double nanmax(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
+
+// max |div u| (DIVGCK_tg, src/DIVGCK.f90:33-46) and max CFL rate (CFL_tg,
+// src/CFL.f90:39-53) in one sweep: part[2b] = div, part[2b+1] = cfl.
+__global__ void __launch_bounds__(256) k_divgck_cfl(Geo g, const double* __restrict__ Q, double* __restrict__ part) {
+    __shared__ double red[2][256];
+    const size_t ncell = (size_t)g.L * g.M * g.N;
+    double md = 0.0, mc = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = t % g.L;
+        const int j = (t / g.L) % g.M;
+        const int k = t / ((size_t)g.L * g.M);
+        const int ip = i == g.L - 1 ? 0 : i + 1, kp = k == g.N - 1 ? 0 : k + 1;
+        const int J = j + 1;
+        const double u0 = Q[qidx(g, 0, i, J, k)], u1 = Q[qidx(g, 0, ip, J, k)];
+        const double v0 = Q[qidx(g, 1, i, J, k)], v1 = Q[qidx(g, 1, i, J + 1, k)];
+        const double w0 = Q[qidx(g, 2, i, J, k)], w1 = Q[qidx(g, 2, i, J, kp)];
+        const double D = (u1 - u0) * g.dxi + (v1 - v0) * g.dyfi[j] * 1.0 + (w1 - w0) * g.dzi * 1.0;
+        const double C = (fabs((u0 + u1) * g.dxi) + fabs((v0 + v1) * g.dyfi[j] * 1.0) + fabs((w0 + w1) * g.dzi * 1.0)) * 0.5;
+        // NaN-propagating maxima: a NaN anywhere makes the result NaN
+        md = nanmax(md, fabs(D));
+        mc = nanmax(mc, C);
+    }
+    red[0][threadIdx.x] = md;
+    red[1][threadIdx.x] = mc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] = nanmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = nanmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = red[0][0];
+        part[2 * blockIdx.x + 1] = red[1][0];
+    }
+}
+
+__global__ void k_reduce_max2_nan(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    __shared__ double red[2][256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        a = nanmax(a, part[2 * i]);
+        b = nanmax(b, part[2 * i + 1]);
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] = nanmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = nanmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr);
+int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
+
+static Geo geo_of(const chs_handle* h) {
+    Geo g;
+    g.L = h->L; g.M = h->M; g.N = h->N;
+    g.dxi = h->dxi; g.dzi = h->dzi; g.dxqi = h->dxqi; g.dzqi = h->dzqi;
+    g.dyfi = h->dyfi; g.dyci = h->dyci; g.wff = h->wff; g.wfb = h->wfb;
+    g.amvr = h->amvr; g.acvr = h->acvr; g.apvr = h->apvr;
+    return g;
+}
+
+static const double TGAM[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double TROH[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+// The x solve of k_rhs_x uses L/XMS segments of one block (XLPB*L/XMS threads,
+// a power of two for the block reductions); the z rows come in groups of KR.
+bool substep_supported(const chs_handle* h) {
+    const int P = h->L / XMS;
+    return h->L % XMS == 0 && P >= 2 && XLPB * P <= 1024 && (P & (P - 1)) == 0 && h->N % KR == 0;
+}
+
+int rk3_substep(chs_handle* h, int ns, double dt) {
+    const Geo g = geo_of(h);
+    const hipStream_t s = h->stream;
+    RkCoef rk;
+    rk.tgam = TGAM[ns];
+    rk.troh = TROH[ns];
+    rk.talp = TGAM[ns] + TROH[ns];
+    rk.dt = dt;
+    const size_t ncell = (size_t)h->L * h->M * h->N;
+    const double facoe = 0.5 * rk.talp * dt * h->cvisc;  // FACOE_TG (src/MOMFA_tg.f90:29)
+    hipEvent_t ev;
+    // 1. RHS + x solve
+    {
+        const double COE = -facoe * h->dxqi;
+        const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
+        const int nblk = h->M * (h->N / KR);
+        const int P = h->L / XMS;
+        const int nthr = XLPB * P;
+        const size_t sm = std::max(sizeof(double) * (3 * KR * (h->L + 1) + (size_t)nthr),
+                                   seg_smem_bytes<double>(XLPB, P));
+        h->timer.begin("rhs_x", s, &ev);
+        CHS_LAUNCH(k_rhs_x, dim3(((nblk + 7) / 8) * 8), dim3(nthr), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0,
+                   h->rhs, h->partials, xa, xb, xc);
+        h->timer.end(ev, "rhs_x", s);
+        h->timer.begin("massflux", s, &ev);
+        CHS_LAUNCH(k_massflux, dim3(1), dim3(1024), 0, s, h->partials, nblk, h->volm, h->scalars);
+        h->timer.end(ev, "massflux", s);
+        // 2. z solves
+        const double RMC2 = 1.0 * (-facoe * h->dzqi);
+        const double za = RMC2, zc = RMC2, zb = 1.0 - zc - za;
+        const double xone = 1.0 / (xa + xb + xc);
+        int MS, Pz, LPB;
+        if (!tdma_seg_config(h->N, true, &MS, &Pz, &LPB, 1024)) {
+            set_error("substep: unsupported NCL3");
+            return CHS_E_ARG;
+        }
+        const int nlines = 3 * h->L * h->M;
+        const dim3 gz((nlines + LPB - 1) / LPB), bz(LPB * Pz);
+        const size_t smz = seg_smem_bytes<double>(LPB, Pz);
+        const int usedpg = h->flowdrvtp == 1 ? 1 : 0;
+        h->timer.begin("z_solve", s, &ev);
+        switch (MS) {
+            case 16: CHS_LAUNCH(k_z_solve<16>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
+            case 12: CHS_LAUNCH(k_z_solve<12>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
+            default: CHS_LAUNCH(k_z_solve<8>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
+        }
+        h->timer.end(ev, "z_solve", s);
+    }
+    // 3. y solves + Q update + BC_WALL_Q
+    {
+        int MS, P, LPB;
+        if (!tdma_seg_config(h->M, false, &MS, &P, &LPB, 768)) {
+            set_error("substep: unsupported NCL2");
+            return CHS_E_ARG;
+        }
+        const int nlines = h->L * h->N;
+        const dim3 gy((nlines + LPB - 1) / LPB), by(LPB * P);
+        const size_t smy = seg_smem_bytes<double>(LPB, P);
+        h->timer.begin("y_solve", s, &ev);
+        switch (MS) {
+            case 16: CHS_LAUNCH(k_y_solve<16>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
+            case 12: CHS_LAUNCH(k_y_solve<12>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
+            default: CHS_LAUNCH(k_y_solve<8>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
+        }
+        h->timer.end(ev, "y_solve", s);
+    }
+    // 4. DIVG
+    const int nbs = (int)std::min<size_t>((ncell + 255) / 256, 8192);
+    h->timer.begin("divg", s, &ev);
+    CHS_LAUNCH(k_divg, dim3(nbs), dim3(256), 0, s, g, 1.0 / (dt * rk.talp), h->q, h->rhsllphi);
+    h->timer.end(ev, "divg", s);
+    // 5. Poisson + BC_WALL_DPH; in the mixed form refine until the projected
+    //    divergence alpha*dt*|f - L phi| is below half the 1e-10 target
+    CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (rk.talp * dt)));
+    // 6. projection + pressure update
+    h->timer.begin("update", s, &ev);
+    CHS_LAUNCH(k_update, dim3(nbs), dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
+               0.5 * rk.talp * dt * h->cvisc, h->dph, h->rhsllphi, h->q, h->pr);
+    h->timer.end(ev, "update", s);
+    return CHS_OK;
+}
+
+int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out) {
+    const Geo g = geo_of(h);
+    const hipStream_t s = h->stream;
+    const int nb = 1024;
+    hipEvent_t ev;
+    h->timer.begin("divgck_cfl", s, &ev);
+    CHS_LAUNCH(k_divgck_cfl, dim3(nb), dim3(256), 0, s, g, h->q, h->partials);
+    CHS_LAUNCH(k_reduce_max2_nan, dim3(1), dim3(256), 0, s, h->partials, nb, h->scalars + 1);
+    h->timer.end(ev, "divgck_cfl", s);
+    double host[2];
+    CHS_HIP(hipMemcpyAsync(host, h->scalars + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHS_HIP(hipStreamSynchronize(s));
+    if (div_out) *div_out = host[0];
+    if (cfl_out) *cfl_out = host[1];
+    return CHS_OK;
+}
+
+}  // namespace chs

@@ -1,0 +1,184 @@
+// Standalone batched tridiagonal solves: the C-ABI chs_tdma_batched, which
+// stands in for the reference's four line solvers
+//   TDMAIJJ_nonCYC (src/TDMAIJJ_nonCYC.f90:67-123)   TDMAIJI_nonCYC (src/TDMAIJI_nonCYC.f90:67-123)
+//   TDMAIJJ_CYC    (src/TDMAIJJ_CYC.f90:67-157)      TDMAIJI_CYC    (src/TDMAIJI_CYC.f90:67-157)
+// on arbitrary line/element strides.
+#include "chs_common.hpp"
+#include "tdma_dev.hpp"
+
+namespace chs {
+
+// Segmented solver: LPB lines per block, P = ceil(n / MS) segments per line.
+template <int MS>
+__global__ void __launch_bounds__(1024) k_tdma_seg(const double* __restrict__ A, const double* __restrict__ B,
+                                                   const double* __restrict__ Cc, double* __restrict__ Rr,
+                                                   const double* __restrict__ bc, int n, int nlines,
+                                                   long long es, long long ls, int LPB, int P, int cyclic) {
+    extern __shared__ unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int ll = tid % LPB, seg = tid / LPB;
+    const long long line = (long long)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    double a[MS], b[MS], c[MS], d[MS];
+    const long long base = valid ? line * ls : 0;
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        const int j = seg * MS + l;
+        if (valid && j < n) {
+            const long long o = base + (long long)j * es;
+            a[l] = A[o];
+            b[l] = B[o];
+            c[l] = Cc[o];
+            d[l] = Rr[o];
+        } else {  // padding rows: identity, decoupled
+            a[l] = 0.0;
+            b[l] = 1.0;
+            c[l] = 0.0;
+            d[l] = 0.0;
+        }
+    }
+    if (!cyclic && valid) {
+        // the end rows take the boundary values BC(line,1:2) (TDMAIJJ_nonCYC.f90:93-97,108-113)
+        if (seg == 0) {
+            if (bc) d[0] = d[0] - a[0] * bc[line];
+            a[0] = 0.0;
+        }
+        const int jl = n - 1;
+        if (jl / MS == seg) {
+            const int l = jl % MS;
+#pragma unroll
+            for (int q = 0; q < MS; ++q)
+                if (q == l) {
+                    if (bc) d[q] = d[q] - c[q] * bc[nlines + line];
+                    c[q] = 0.0;
+                }
+        }
+    }
+    seg_solve<MS, double>(a, b, c, d, LPB, P, ll, seg, valid, cyclic != 0, smem);
+    if (valid) {
+#pragma unroll
+        for (int l = 0; l < MS; ++l) {
+            const int j = seg * MS + l;
+            if (j < n) Rr[base + (long long)j * es] = d[l];
+        }
+    }
+}
+
+// Fallback for shapes the segmented kernel does not cover: one thread per
+// line, the reference recurrences, H/G in a device workspace.
+__global__ void k_tdma_seq(const double* __restrict__ A, const double* __restrict__ B,
+                           const double* __restrict__ Cc, double* __restrict__ Rr, const double* __restrict__ bc,
+                           int n, int nlines, long long es, long long ls, int cyclic, double* __restrict__ ws) {
+    const long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    const double* a = A + line * ls;
+    const double* b = B + line * ls;
+    const double* c = Cc + line * ls;
+    double* r = Rr + line * ls;
+    // workspace rows are interleaved over lines: coalesced
+    double* H = ws + line;
+    double* G = ws + (long long)n * nlines + line;
+    double* G2 = ws + 2LL * n * nlines + line;
+    const long long wst = nlines;
+    if (!cyclic) {
+        const double blo = bc ? bc[line] : 0.0, bhi = bc ? bc[nlines + line] : 0.0;
+        H[0] = c[0] / b[0];
+        G[0] = (r[0] - a[0] * blo) / b[0];
+        for (int j = 1; j < n - 1; ++j) {
+            const double pp = 1.0 / (b[j * es] - H[(j - 1) * wst] * a[j * es]);
+            H[j * wst] = c[j * es] * pp;
+            G[j * wst] = (r[j * es] - a[j * es] * G[(j - 1) * wst]) * pp;
+        }
+        const int j = n - 1;
+        const double pp = 1.0 / (b[j * es] - H[(j - 1) * wst] * a[j * es]);
+        double x = (r[j * es] - c[j * es] * bhi - a[j * es] * G[(j - 1) * wst]) * pp;
+        r[j * es] = x;
+        for (int jj = n - 2; jj >= 0; --jj) {
+            x = G[jj * wst] - H[jj * wst] * x;
+            r[jj * es] = x;
+        }
+        return;
+    }
+    H[(n - 2) * wst] = 0.0;
+    H[0] = c[0] / b[0];
+    G[0] = r[0] / b[0];
+    G2[0] = -a[0] / b[0];
+    for (int i = 1; i <= n - 3; ++i) {
+        const double pp = 1.0 / (b[i * es] - H[(i - 1) * wst] * a[i * es]);
+        const double q1 = G[(i - 1) * wst] * a[i * es], q2 = G2[(i - 1) * wst] * a[i * es];
+        H[i * wst] = c[i * es] * pp;
+        G[i * wst] = (r[i * es] - q1) * pp;
+        G2[i * wst] = -q2 * pp;
+    }
+    {
+        const int i = n - 2;
+        const double pp = 1.0 / (b[i * es] - H[(i - 1) * wst] * a[i * es]);
+        const double q1 = G[(i - 1) * wst] * a[i * es], q2 = G2[(i - 1) * wst] * a[i * es];
+        G[i * wst] = (r[i * es] - q1) * pp;
+        G2[i * wst] = (-c[i * es] - q2) * pp;
+    }
+    for (int i = n - 3; i >= 0; --i) {
+        G[i * wst] = -H[i * wst] * G[(i + 1) * wst] + G[i * wst];
+        G2[i * wst] = -H[i * wst] * G2[(i + 1) * wst] + G2[i * wst];
+    }
+    const int i = n - 1;
+    const double pp = r[i * es] - c[i * es] * G[0] - a[i * es] * G[(i - 1) * wst];
+    const double qq = b[i * es] + c[i * es] * G2[0] + a[i * es] * G2[(i - 1) * wst];
+    const double xn = pp / qq;
+    r[i * es] = xn;
+    for (int k = n - 2; k >= 0; --k) r[k * es] = G[k * wst] + G2[k * wst] * xn;
+}
+
+// Choose the segmented configuration for (n, cyclic) within maxthr threads per
+// block, or return false.  8-row segments keep the solver within 128 VGPRs.
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr) {
+    const int cand[3] = {8, 12, 16};
+    for (int ms : cand) {
+        for (int lpb : {64, 32}) {
+            const int p = (n + ms - 1) / ms;
+            if (p * lpb > maxthr) continue;
+            if (cyclic && (n % ms != 0 || p < 2)) continue;
+            if (!cyclic && n % ms != 0 && n < ms) continue;
+            *MS = ms;
+            *P = p;
+            *LPB = lpb;
+            return true;
+        }
+    }
+    return false;
+}
+
+int tdma_launch(const double* a, const double* b, const double* c, double* r, const double* bc, int n, int nlines,
+                long long es, long long ls, int cyclic, hipStream_t s) {
+    int MS = 0, P = 0, LPB = 0;
+    if (tdma_seg_config(n, cyclic != 0, &MS, &P, &LPB, 1024)) {
+        const dim3 grid((nlines + LPB - 1) / LPB), block(LPB * P);
+        const size_t sm = seg_smem_bytes<double>(LPB, P);
+        switch (MS) {
+            case 16: CHS_LAUNCH(k_tdma_seg<16>, grid, block, sm, s, a, b, c, r, bc, n, nlines, es, ls, LPB, P, cyclic); break;
+            case 12: CHS_LAUNCH(k_tdma_seg<12>, grid, block, sm, s, a, b, c, r, bc, n, nlines, es, ls, LPB, P, cyclic); break;
+            default: CHS_LAUNCH(k_tdma_seg<8>, grid, block, sm, s, a, b, c, r, bc, n, nlines, es, ls, LPB, P, cyclic); break;
+        }
+        return CHS_OK;
+    }
+    double* ws = nullptr;
+    CHS_HIP(hipMallocAsync((void**)&ws, sizeof(double) * 3 * (size_t)n * nlines, s));
+    const int bs = 256;
+    CHS_LAUNCH(k_tdma_seq, dim3((nlines + bs - 1) / bs), dim3(bs), 0, s, a, b, c, r, bc, n, nlines, es, ls, cyclic, ws);
+    CHS_HIP(hipFreeAsync(ws, s));
+    return CHS_OK;
+}
+
+}  // namespace chs
+
+extern "C" int chs_tdma_batched(const double* a_dev, const double* b_dev, const double* c_dev, double* r_dev,
+                                const double* bc_dev, int n, int nlines, long long elem_stride,
+                                long long line_stride, int cyclic, void* stream) {
+    CHS_CHECK(a_dev && b_dev && c_dev && r_dev, CHS_E_ARG, "chs_tdma_batched: null array");
+    CHS_CHECK(n >= 3, CHS_E_ARG, "chs_tdma_batched: n must be >= 3 (the reference reads outside its arrays below that)");
+    CHS_CHECK(nlines >= 0, CHS_E_ARG, "chs_tdma_batched: nlines < 0");
+    CHS_CHECK(elem_stride != 0 || n == 1, CHS_E_ARG, "chs_tdma_batched: zero element stride");
+    if (nlines == 0) return CHS_OK;
+    return chs::tdma_launch(a_dev, b_dev, c_dev, r_dev, cyclic ? nullptr : bc_dev, n, nlines, elem_stride,
+                            line_stride, cyclic, (hipStream_t)stream);
+}

@@ -1,0 +1,105 @@
+"""ctypes binding of libchapsim_gpu.so (the C ABI of include/chapsim_gpu.h).
+
+The library is the only compute path: nothing here falls back to the CPU.
+Loading fails loudly when the shared library is missing; device work needs a
+visible HIP device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchapsim_gpu.so")
+
+CHS_OK, CHS_E_ARG, CHS_E_HIP, CHS_E_NOMEM, CHS_E_RANGE, CHS_E_NOTCONV = range(6)
+CHS_FP64, CHS_MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+
+
+class ChsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"chapsim_gpu error {code}: {msg}")
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("ncl1", C.c_int), ("ncl2", C.c_int), ("ncl3", C.c_int), ("icase", C.c_int),
+                ("flowdrvtp", C.c_int), ("precision", C.c_int),
+                ("dxi", C.c_double), ("dzi", C.c_double), ("dxqi", C.c_double), ("dzqi", C.c_double),
+                ("cvisc", C.c_double), ("volm", C.c_double)] + \
+               [(n, _dp) for n in ("dyfi", "dyci", "wff", "wfb", "amph", "acph", "apph",
+                                   "amvr", "acvr", "apvr", "ak1", "ak3")]
+
+
+# every export of include/chapsim_gpu.h with its ctypes signature
+_SIGS = {
+    "chs_create": ([C.POINTER(GridDesc), C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "chs_destroy": ([C.c_void_p], C.c_int),
+    "chs_upload_state": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
+    "chs_download_state": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
+    "chs_device_arrays": ([C.c_void_p] + [C.POINTER(C.c_void_p)] * 5, C.c_int),
+    "chs_rk3_substep": ([C.c_void_p, C.c_int, C.c_double], C.c_int),
+    "chs_poisson_fft99": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "chs_set_refinement": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
+    "chs_poisson_stats": ([C.c_void_p, C.POINTER(C.c_int), _dp], C.c_int),
+    "chs_divgck": ([C.c_void_p, _dp], C.c_int),
+    "chs_cfl": ([C.c_void_p, _dp], C.c_int),
+    "chs_last_dpgrns": ([C.c_void_p, _dp], C.c_int),
+    "chs_tdma_batched": ([C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.c_void_p],
+                         C.c_int),
+    "chs_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
+    "chs_profile_read": ([C.c_void_p, C.c_int, C.c_char_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_int)],
+                         C.c_int),
+    "chs_profile_reset": ([C.c_void_p], C.c_int),
+    "chs_last_error": ([], C.c_char_p),
+    "chs_version": ([], C.c_char_p),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m chapsim1ww_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def check(code: int) -> None:
+    if code != CHS_OK:
+        raise ChsError(code, load().chs_last_error().decode())
+
+
+def dptr(x) -> int:
+    """Device address of a torch CUDA tensor (or an int address)."""
+    if isinstance(x, int) or x is None:
+        return x
+    if not x.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not x.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    return x.data_ptr()
+
+
+def hptr(a: np.ndarray):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not (a.flags.f_contiguous or a.flags.c_contiguous):
+        raise ValueError("expected a contiguous float64 array")
+    return a.ctypes.data_as(_dp)

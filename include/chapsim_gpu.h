@@ -1,0 +1,127 @@
+/*
+ * chapsim_gpu.h -- C ABI of the MI355X (gfx950) implementation of CHAPSim's
+ * per-RK3-substep solver core.
+ *
+ * Conventions
+ *  - Every entry point returns int: CHS_OK (0) or a CHS_E_* code; the message
+ *    of the last failure on the calling thread is returned by chs_last_error().
+ *    The reference has no status codes: its Fortran routines STOP
+ *    (src/WRTHDL.f90:18-29).  The Fortran shim (fortran/chapsim_gpu.f90) turns
+ *    a non-zero status into a message plus ERROR STOP.
+ *  - Arrays keep the reference's Fortran layouts (column-major, x fastest):
+ *        Q_tg(NCL1, 0:NCL2+1, NCL3, 3)   PR_tg, DPH_tg (NCL1, 0:NCL2+1, NCL3)
+ *        RHSLLPHI_tg, RHS_tg (NCL1, NCL2, NCL3)   CONVH0_tg (NCL1, NCL2, NCL3, 3)
+ *  - "dev" pointers are device (HBM) addresses; "host" pointers are host memory.
+ *  - All work of a handle is ordered on the HIP stream given at creation.
+ *    Calls on one handle must not overlap (not re-entrant per handle).
+ *  - Only the wall-bounded channel (ICASE = 1) on one GPU is supported.
+ */
+#ifndef CHAPSIM_GPU_H
+#define CHAPSIM_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CHS_OK = 0,
+    CHS_E_ARG = 1,      /* invalid argument or unsupported shape              */
+    CHS_E_HIP = 2,      /* HIP runtime error                                  */
+    CHS_E_NOMEM = 3,    /* device allocation failed                           */
+    CHS_E_RANGE = 4,    /* value outside a table range (reference: STOP)      */
+    CHS_E_NOTCONV = 5   /* iterative refinement did not reach its tolerance   */
+};
+
+enum { CHS_FP64 = 0, CHS_MIXED = 1 };
+
+typedef struct chs_handle chs_handle;
+
+/* Mesh and coefficients, as the reference's set-up routines leave them in
+ * its modules (all arrays are host memory, copied at creation):
+ *   dyfi[M]  DYFI(1:NCL2), dyci[M+1] DYCI(1:NND2)   (src/COORDJR.f90:112-131)
+ *   wff/wfb[M+1]  YCL2ND_WFF/WFB(1:NND2)            (src/COORDJR.f90:427-470)
+ *   amph/acph/apph[M]                               (src/TDMA_COEF.f90:45-64)
+ *   amvr/acvr/apvr[3*(M+1)], element (J, IDR) at (IDR-1)*(M+1) + J-1
+ *                                                   (src/TDMA_COEF.f90:67-116)
+ *   ak1[L], ak3[N]  modified wavenumbers            (src/possion3d_FFT99.f90:152-179)
+ * Scalars DXI, DZI, DXQI, DZQI, CVISC=1/REN, VOLM_tg (src/CONSPARA.f90, src/COORDJR.f90:385-396). */
+typedef struct chs_grid_desc {
+    int ncl1, ncl2, ncl3;       /* L, M, N; L and N even with factors 2, 3, 5 */
+    int icase;                  /* 1 = channel                                */
+    int flowdrvtp;              /* 1 = constant mass flux (FLOWDRVTP)          */
+    int precision;              /* CHS_FP64 or CHS_MIXED                       */
+    double dxi, dzi, dxqi, dzqi, cvisc, volm;
+    const double *dyfi, *dyci, *wff, *wfb, *amph, *acph, *apph;
+    const double *amvr, *acvr, *apvr, *ak1, *ak3;
+} chs_grid_desc;
+
+/* Create a solver handle: allocates the device-resident mirrors of the
+ * reference's tg state (Q_tg, PR_tg, CONVH0_tg) and work arrays.
+ * Replaces the allocation in MEM_ALLOCAT (src/MEMALLOCT.f90:99-106) and the
+ * set-up FFT99_POIS3D_INIT (src/possion3d_FFT99.f90:54-117).
+ * stream: a hipStream_t, or NULL to create one. */
+int chs_create(const chs_grid_desc *desc, void *stream, chs_handle **out);
+int chs_destroy(chs_handle *h);
+
+/* Copy the tg state between host arrays (reference layouts) and the device. */
+int chs_upload_state(chs_handle *h, const double *q_host, const double *pr_host, const double *convh0_host);
+int chs_download_state(chs_handle *h, double *q_host, double *pr_host, double *convh0_host);
+/* Device addresses of the handle's arrays (any output pointer may be NULL). */
+int chs_device_arrays(chs_handle *h, double **q, double **pr, double **convh0, double **dph, double **rhsllphi);
+
+/* One RK3 substep NS (0..3) of the tg channel with time step dt, on the
+ * device-resident state: SOLVERRK3_MOM_tg(NS) (src/SOLVERRK3_MOM_tg.f90:18-61):
+ * convection, RHS + constant-mass-flux forcing, CN factorised x/z/y solves,
+ * wall BCs, divergence, Poisson solve, projection and pressure update. */
+int chs_rk3_substep(chs_handle *h, int ns, double dt);
+
+/* Pressure Poisson solve FFT99_POIS3D_periodicxz (src/possion3d_FFT99.f90:254-488):
+ *   f_dev:   RHSLLPHI(NCL1, NCL2, NCL3)          (device)
+ *   dph_dev: DPH(NCL1, 0:NCL2+1, NCL3)           (device); interior written,
+ *            ghost planes set to zero as the reference does (:447).
+ * With CHS_MIXED precision the FFTs run in FP32 and the solve is iteratively
+ * refined with an FP64 residual until max|L phi - f| <= tol * max|f|. */
+int chs_poisson_fft99(chs_handle *h, const double *f_dev, double *dph_dev);
+/* Tolerance and iteration cap of the mixed-precision refinement; the last
+ * call's iteration count and relative residual are reported by chs_poisson_stats. */
+int chs_set_refinement(chs_handle *h, double rel_tol, int max_iter);
+int chs_poisson_stats(chs_handle *h, int *iters, double *rel_resid);
+
+/* max|div u| of the device state: DIVGCK_tg (src/DIVGCK.f90:16-58). */
+int chs_divgck(chs_handle *h, double *maxdiv_host);
+/* max convective CFL rate: CFL_tg (src/CFL.f90:21-59); dt_cfl = CFL/rate. */
+int chs_cfl(chs_handle *h, double *rate_host);
+/* DPGRNS of the last substep (constant-mass-flux correction). */
+int chs_last_dpgrns(chs_handle *h, double *dpgrns_host);
+
+/* Batched tridiagonal line solves on device arrays, the four reference solvers:
+ *   cyclic = 0: TDMAIJJ_nonCYC / TDMAIJI_nonCYC (src/TDMAIJJ_nonCYC.f90:67-123,
+ *               src/TDMAIJI_nonCYC.f90:67-123); bc_dev may be NULL (zero) or
+ *               hold BC(line, 1:2) column-major: bc[l], bc[nlines + l].
+ *   cyclic = 1: TDMAIJJ_CYC / TDMAIJI_CYC (src/TDMAIJJ_CYC.f90:67-157,
+ *               src/TDMAIJI_CYC.f90:67-157); bc_dev ignored.
+ * Element (line l, row j) of a, b, c, r lives at l*line_stride + j*elem_stride:
+ *   IJJ form (march along index 2):  elem_stride = ISZ, line_stride = 1
+ *   IJI form (march along index 1):  elem_stride = 1,   line_stride = ISZ
+ * r is overwritten by the solution.  n >= 3.  stream may be NULL (default). */
+int chs_tdma_batched(const double *a_dev, const double *b_dev, const double *c_dev, double *r_dev,
+                     const double *bc_dev, int n, int nlines, long long elem_stride,
+                     long long line_stride, int cyclic, void *stream);
+
+/* Per-kernel timing with HIP events on the handle's stream. */
+int chs_profile_enable(chs_handle *h, int on);
+/* Fills up to cap entries; returns the number of kernels recorded in *count.
+ * names: cap * 48 chars (NUL-terminated each); ms: total ms; calls: launches. */
+int chs_profile_read(chs_handle *h, int cap, char *names, double *ms, long *calls, int *count);
+int chs_profile_reset(chs_handle *h);
+
+const char *chs_last_error(void);
+const char *chs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHAPSIM_GPU_H */
