@@ -298,6 +298,7 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
     switch (MS) {
         case 16: CHS_LAUNCH((k_poisson_y<16, Rl>), gy, by, smy, s, S, M, LH, LHP, nlines, LPB, P, scale, h->amph, h->acph, h->apph, h->ak1, h->ak3); break;
         case 12: CHS_LAUNCH((k_poisson_y<12, Rl>), gy, by, smy, s, S, M, LH, LHP, nlines, LPB, P, scale, h->amph, h->acph, h->apph, h->ak1, h->ak3); break;
+        case 4: CHS_LAUNCH((k_poisson_y<4, Rl>), gy, by, smy, s, S, M, LH, LHP, nlines, LPB, P, scale, h->amph, h->acph, h->apph, h->ak1, h->ak3); break;
         default: CHS_LAUNCH((k_poisson_y<8, Rl>), gy, by, smy, s, S, M, LH, LHP, nlines, LPB, P, scale, h->amph, h->acph, h->apph, h->ak1, h->ak3); break;
     }
     h->timer.end(ev, "poisson_y", s);

@@ -492,6 +492,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         switch (MS) {
             case 16: CHS_LAUNCH(k_z_solve<16>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
             case 12: CHS_LAUNCH(k_z_solve<12>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
+            case 4: CHS_LAUNCH(k_z_solve<4>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
             default: CHS_LAUNCH(k_z_solve<8>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
         }
         h->timer.end(ev, "z_solve", s);
@@ -510,6 +511,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         switch (MS) {
             case 16: CHS_LAUNCH(k_y_solve<16>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
             case 12: CHS_LAUNCH(k_y_solve<12>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
+            case 4: CHS_LAUNCH(k_y_solve<4>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
             default: CHS_LAUNCH(k_y_solve<8>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
         }
         h->timer.end(ev, "y_solve", s);

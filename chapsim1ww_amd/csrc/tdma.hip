@@ -132,13 +132,13 @@ __global__ void k_tdma_seq(const double* __restrict__ A, const double* __restric
 // Choose the segmented configuration for (n, cyclic) within maxthr threads per
 // block, or return false.  8-row segments keep the solver within 128 VGPRs.
 bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr) {
-    const int cand[3] = {8, 12, 16};
+    const int cand[4] = {8, 12, 16, 4};
     for (int ms : cand) {
         for (int lpb : {64, 32}) {
             const int p = (n + ms - 1) / ms;
             if (p * lpb > maxthr) continue;
+            // cyclic lines cannot be padded (the wrap couples the last real row)
             if (cyclic && (n % ms != 0 || p < 2)) continue;
-            if (!cyclic && n % ms != 0 && n < ms) continue;
             *MS = ms;
             *P = p;
             *LPB = lpb;
@@ -157,6 +157,7 @@ int tdma_launch(const double* a, const double* b, const double* c, double* r, co
         switch (MS) {
             case 16: CHS_LAUNCH(k_tdma_seg<16>, grid, block, sm, s, a, b, c, r, bc, n, nlines, es, ls, LPB, P, cyclic); break;
             case 12: CHS_LAUNCH(k_tdma_seg<12>, grid, block, sm, s, a, b, c, r, bc, n, nlines, es, ls, LPB, P, cyclic); break;
+            case 4: CHS_LAUNCH(k_tdma_seg<4>, grid, block, sm, s, a, b, c, r, bc, n, nlines, es, ls, LPB, P, cyclic); break;
             default: CHS_LAUNCH(k_tdma_seg<8>, grid, block, sm, s, a, b, c, r, bc, n, nlines, es, ls, LPB, P, cyclic); break;
         }
         return CHS_OK;
