@@ -17,6 +17,8 @@ int rk3_substep(chs_handle* h, int ns, double dt);
 int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out);
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
 bool substep_supported(const chs_handle* h);
+void release_stencils(const chs_handle* h);
+int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky);
 bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr);
 
 void KernelTimer::begin(const char* name, hipStream_t s, hipEvent_t* out_a) {
@@ -98,7 +100,8 @@ static bool fft_len_ok(int n) {
 static void free_all(chs_handle* h) {
     void* ptrs[] = {h->dyfi, h->dyci, h->wff, h->wfb, h->amph, h->acph, h->apph, h->amvr, h->acvr, h->apvr,
                     h->ak1, h->ak3, h->twx, h->twz, h->twx32, h->twz32, h->q, h->pr, h->convh0, h->dph, h->rhs,
-                    h->rhsllphi, h->spec, h->spec32, h->rhs_j1, h->partials, h->scalars, h->resid, h->f32a, h->f32b};
+                    h->rhsllphi, h->spec, h->spec32, h->rhs_j1, h->partials, h->scalars, h->resid, h->f32a, h->f32b,
+                    h->rhs2};
     for (void* p : ptrs)
         if (p) hipFree(p);
 }
@@ -181,6 +184,7 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     chk(dalloc(&h->convh0, ncell * 3));
     chk(dalloc(&h->dph, ngh));
     chk(dalloc(&h->rhs, ncell * 3));
+    chk(dalloc(&h->rhs2, ncell * 3));
     chk(dalloc(&h->rhsllphi, ncell));
     const size_t nspec = (size_t)h->LHP * M * N;
     if (h->precision == CHS_FP64)
@@ -218,6 +222,7 @@ int chs_destroy(chs_handle* h) {
     if (!h) return CHS_OK;
     hipStreamSynchronize(h->stream);
     h->timer.collect();
+    release_stencils(h);
     free_all(h);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
@@ -261,6 +266,25 @@ int chs_rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_CHECK(substep_supported(h), CHS_E_ARG,
               "chs_rk3_substep: NCL1 must be a power of two >= 32 and NCL3 a multiple of 4");
     return rk3_substep(h, ns, dt);
+}
+
+int chs_set_cn_solver(chs_handle* h, int mode) {
+    CHS_CHECK(h, CHS_E_ARG, "chs_set_cn_solver: null handle");
+    CHS_CHECK(mode == 0 || mode == 1, CHS_E_ARG, "chs_set_cn_solver: mode must be 0 (auto) or 1 (segmented Thomas)");
+    h->cn_mode = mode;
+    return CHS_OK;
+}
+
+int chs_cn_info(chs_handle* h, int ns, double dt, int* path, int* kx, int* kz, int* ky) {
+    CHS_CHECK(h && path && kx && kz && ky, CHS_E_ARG, "chs_cn_info: null argument");
+    CHS_CHECK(ns >= 0 && ns <= 3 && dt > 0.0, CHS_E_ARG, "chs_cn_info: bad NS or dt");
+    static const double TG[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+    static const double TR[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
+    const double facoe = 0.5 * (TG[ns] + TR[ns]) * dt * h->cvisc;
+    const int p = cn_last_path(h, facoe, kx, kz, ky);
+    CHS_CHECK(p >= 0, p, "chs_cn_info failed");
+    *path = p;
+    return CHS_OK;
 }
 
 int chs_poisson_fft99(chs_handle* h, const double* f, double* dph) {

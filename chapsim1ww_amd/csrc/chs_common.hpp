@@ -133,6 +133,8 @@ struct chs_handle {
     double *q, *pr, *convh0;
     // work: DPH(L,0:M+1,N), RHS x3 (L,M,N), RHSLLPHI(L,M,N), spectral S
     double *dph, *rhs, *rhsllphi;
+    double *rhs2;        // second RHS buffer of the stencil CN path (z output)
+    int cn_mode;         // 0 = auto (stencils when their truncation bound holds), 1 = segmented Thomas
     double2 *spec;
     float2 *spec32;
     double *rhs_j1;      // the u-increment on j = 1, which becomes v's row-1 RHS

@@ -16,6 +16,8 @@
 // Indices below are 0-based: i in [0,L), j in [0,M) (Fortran J = j+1, stored
 // at ghost-aware row j+1), k in [0,N).
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "chs_common.hpp"
 #include "tdma_dev.hpp"
@@ -111,7 +113,7 @@ struct RkCoef {
 __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
                                                const double* __restrict__ PR, double* __restrict__ H0,
                                                double* __restrict__ RHS, double* __restrict__ part, double xa,
-                                               double xb, double xc) {
+                                               double xb, double xc, int Kx, const double* __restrict__ wx) {
     extern __shared__ unsigned char smem[];
     const int L = g.L;
     const int LS = L + 1;  // padded LDS row stride (bank spread for the segment reads)
@@ -174,7 +176,25 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
         if (threadIdx.x == 0 && active) part[t] = red[0];
         __syncthreads();
     }
-    // phase 2: x solves of the 3*KR rows, XMS rows per segment, L/XMS segments
+    // phase 2a: x solve as the truncated inverse of the constant cyclic system,
+    // a (2Kx+1)-point periodic stencil over the LDS rows (Kx >= 0)
+    if (Kx >= 0) {
+        for (int e = threadIdx.x; e < 3 * KR * L; e += blockDim.x) {
+            const int cr = e / L, i = e - cr * L;
+            const int c = cr / KR, r = cr - c * KR;
+            if (!active || (c == 1 && j == 0)) continue;
+            const double* row = sR + cr * LS;
+            double acc = 0.0;
+            for (int m = -Kx; m <= Kx; ++m) {
+                int ii = i + m;
+                ii = ii < 0 ? ii + L : (ii >= L ? ii - L : ii);
+                acc += wx[m + Kx] * row[ii];
+            }
+            RHS[cidx(g, i, j, k0 + r) + c * LMN] = acc;
+        }
+        return;
+    }
+    // phase 2b: x solves of the 3*KR rows, XMS rows per segment, L/XMS segments
     const int P = L / XMS;
     const int ll = threadIdx.x % XLPB, seg = threadIdx.x / XLPB;
     const int c = ll / KR, r = ll - c * KR;
@@ -319,6 +339,107 @@ __global__ void __launch_bounds__(768) k_y_solve(Geo g, double facoe, const doub
     }
 }
 
+// ---------------------------------------------------------------------------
+// CN solves as truncated inverses.  The factorised Crank-Nicolson systems of
+// MOMFA_tg are strongly diagonally dominant at DNS time steps (off-diagonal /
+// diagonal ~ 1e-4..1e-2), so every row of their inverses decays below 1e-17 of
+// its sum within a few rows.  The host computes the exact inverse rows in
+// extended precision and truncates them where the dropped weights sum to
+// < 1e-17 of the row's absolute sum; the solves then become short stencils.
+// Shapes or time steps where that needs more than KMAX rows use the
+// segmented Thomas kernels above instead.
+// ---------------------------------------------------------------------------
+constexpr int KMAX = 16;
+constexpr int ZTK = 32;     // k rows per block of the z stencil
+constexpr int ZTI = 64;     // i columns per block of the z stencil
+constexpr int YTI = 32;     // i columns per block of the y stencil
+
+// z: out(i,j,k) = sum_m wz[m] (R(i,j,k+m) - D) for u (D = DPGRNS pushed through
+// the x solve), plain sum for v (j >= 1) and w.  Reads RHS, writes RHS2.
+__global__ void __launch_bounds__(256) k_z_stencil(Geo g, const double* __restrict__ R, double* __restrict__ R2,
+                                                   const double* __restrict__ dpg, double xone, int use_dpg, int Kz,
+                                                   const double* __restrict__ wz) {
+    __shared__ double tile[(ZTK + 2 * KMAX) * ZTI];
+    const int L = g.L, M = g.M, N = g.N;
+    const int ntx = (L + ZTI - 1) / ZTI, ntk = N / ZTK;
+    int b = blockIdx.x;
+    const int it = b % ntx; b /= ntx;
+    const int kt = b % ntk; b /= ntk;
+    const int j = b % M;
+    const int c = b / M;
+    if (c == 1 && j == 0) return;  // whole block uniform: v on the wall face is not solved
+    const int i0 = it * ZTI, k0 = kt * ZTK;
+    const size_t LM = (size_t)L * M;
+    const size_t base = (size_t)c * LM * N + (size_t)j * L;
+    const double D = (use_dpg && c == 0) ? dpg[0] * xone : 0.0;
+    const int rows = ZTK + 2 * Kz;
+    for (int e = threadIdx.x; e < rows * ZTI; e += blockDim.x) {
+        const int r = e / ZTI, ii = e - r * ZTI;
+        int k = k0 - Kz + r;
+        k = k < 0 ? k + N : (k >= N ? k - N : k);
+        const int i = i0 + ii;
+        tile[e] = i < L ? R[base + (size_t)k * LM + i] - D : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ZTK * ZTI; e += blockDim.x) {
+        const int r = e / ZTI, ii = e - r * ZTI;
+        const int i = i0 + ii;
+        if (i >= L) continue;
+        double acc = 0.0;
+        for (int m = 0; m <= 2 * Kz; ++m) acc += wz[m] * tile[(r + m) * ZTI + ii];
+        R2[base + (size_t)(k0 + r) * LM + i] = acc;
+    }
+}
+
+// y: Delta_c(j) = sum_m W_c[j][m] R_c(j+m), rows clipped to [0, M); for v the
+// row-1 right-hand side is the u increment on j = 1 (what RHS_tg holds there in
+// the reference).  Then Q += Delta from N2I (MOMFA_tg:39-45) and BC_WALL_Q_tg.
+__global__ void __launch_bounds__(256) k_y_stencil(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
+                                                   int Ky, const double* __restrict__ wy) {
+    extern __shared__ unsigned char smem[];
+    const int L = g.L, M = g.M, N = g.N;
+    double* tile = reinterpret_cast<double*>(smem);   // [M][YTI]
+    double* du0 = tile + (size_t)M * YTI;              // [YTI]
+    const int ntx = (L + YTI - 1) / YTI;
+    const int it = blockIdx.x % ntx, k = blockIdx.x / ntx;
+    const int i0 = it * YTI;
+    const size_t LM = (size_t)L * M;
+    const int W = 2 * Ky + 1;
+    for (int c = 0; c < 3; ++c) {
+        const size_t base = (size_t)c * LM * N + (size_t)k * LM;
+        for (int e = threadIdx.x; e < M * YTI; e += blockDim.x) {
+            const int jj = e / YTI, ii = e - jj * YTI;
+            const int i = i0 + ii;
+            double v = 0.0;
+            if (i < L) v = (c == 1 && jj == 0) ? du0[ii] : R2[base + (size_t)jj * L + i];
+            tile[e] = v;
+        }
+        __syncthreads();
+        const double* wc = wy + (size_t)c * M * W;
+        for (int e = threadIdx.x; e < M * YTI; e += blockDim.x) {
+            const int jj = e / YTI, ii = e - jj * YTI;
+            const int i = i0 + ii;
+            if (i >= L) continue;
+            const int m0 = jj - Ky < 0 ? -jj : -Ky;
+            const int m1 = jj + Ky >= M ? M - 1 - jj : Ky;
+            const double* w = wc + (size_t)jj * W + Ky;
+            double acc = 0.0;
+            for (int m = m0; m <= m1; ++m) acc += w[m] * tile[(jj + m) * YTI + ii];
+            if (c == 0 && jj == 0) du0[ii] = acc;
+            if (!(c == 1 && jj == 0)) {
+                const size_t qo = qidx(g, c, i, jj + 1, k);
+                Q[qo] = acc + Q[qo];
+            }
+            if (jj == 0) {
+                Q[qidx(g, c, i, 0, k)] = 0.0;
+                if (c == 1) Q[qidx(g, 1, i, 1, k)] = 0.0;
+            }
+            if (jj == M - 1) Q[qidx(g, c, i, M + 1, k)] = 0.0;
+        }
+        __syncthreads();
+    }
+}
+
 // DIVG_tg (src/DIVG.f90:21-57): f = div(Q) / (DT * TALP)
 __global__ void __launch_bounds__(256) k_divg(Geo g, double coe0, const double* __restrict__ Q, double* __restrict__ F) {
     const size_t ncell = (size_t)g.L * g.M * g.N;
@@ -441,11 +562,177 @@ static Geo geo_of(const chs_handle* h) {
 static const double TGAM[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
 static const double TROH[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
 
+using ld = long double;
+
+// Thomas solve in extended precision (a[0] and c[n-1] ignored).
+static void thomas_ld(int n, const ld* a, const ld* b, const ld* c, ld* x) {
+    std::vector<ld> h(n), gg(n);
+    h[0] = c[0] / b[0];
+    gg[0] = x[0] / b[0];
+    for (int i = 1; i < n; ++i) {
+        const ld den = b[i] - a[i] * h[i - 1];
+        h[i] = c[i] / den;
+        gg[i] = (x[i] - a[i] * gg[i - 1]) / den;
+    }
+    x[n - 1] = gg[n - 1];
+    for (int i = n - 2; i >= 0; --i) x[i] = gg[i] - h[i] * x[i + 1];
+}
+
+// Smallest K whose dropped tail sum_{|m|>K} |row_m| is <= tol * sum |row|
+static int trunc_width(const std::vector<ld>& row, int centre, bool cyclic) {
+    const int n = (int)row.size();
+    ld tot = 0;
+    for (ld v : row) tot += fabsl(v);
+    for (int K = 0; K <= KMAX; ++K) {
+        ld tail = 0;
+        for (int m = 0; m < n; ++m) {
+            int d = m - centre;
+            if (cyclic) {
+                d = ((d % n) + n) % n;
+                if (d > n / 2) d -= n;
+            }
+            if (d > K || d < -K) tail += fabsl(row[m]);
+        }
+        if (tail <= 1e-17L * tot) return K;
+    }
+    return KMAX + 1;
+}
+
+// Inverse of the constant cyclic system (a, b, c) of size n, row 0: the
+// periodic stencil.  Returns K (> KMAX when too wide) and fills w[0..2K].
+static int cyclic_stencil(int n, double a, double b, double c, std::vector<double>& w) {
+    // A v = e_0 by the Sherman-Morrison form of the cyclic system
+    std::vector<ld> A(n, (ld)a), B(n, (ld)b), Cc(n, (ld)c), x(n, 0), u(n, 0);
+    // cyclic -> tridiagonal with corrections: gamma = -b0
+    const ld gam = -B[0];
+    std::vector<ld> bb(B);
+    bb[0] = B[0] - gam;
+    bb[n - 1] = B[n - 1] - (ld)c * (ld)a / gam;
+    x[0] = 1;
+    thomas_ld(n, A.data(), bb.data(), Cc.data(), x.data());
+    u[0] = gam;
+    u[n - 1] = (ld)c;
+    thomas_ld(n, A.data(), bb.data(), Cc.data(), u.data());
+    const ld fact = (x[0] + (ld)a * x[n - 1] / gam) / (1 + u[0] + (ld)a * u[n - 1] / gam);
+    std::vector<ld> v(n);
+    for (int i = 0; i < n; ++i) v[i] = x[i] - fact * u[i];
+    // symmetric circulant (a == c): row 0 of the inverse = column 0
+    const int K = trunc_width(v, 0, true);
+    if (K > KMAX) return K;
+    w.assign(2 * K + 1, 0.0);
+    for (int m = -K; m <= K; ++m) w[m + K] = (double)v[((m % n) + n) % n];
+    return K;
+}
+
+struct CnStencils {
+    double facoe = -1.0;
+    int Kx = -1, Kz = -1, Ky = -1;
+    bool ok = false;
+    double* dev = nullptr;   // wx[2Kx+1] | wz[2Kz+1] | wy[3][M][2Ky+1]
+    size_t ox = 0, oz = 0, oy = 0;
+};
+
+static std::vector<CnStencils>& stencil_cache(const chs_handle* h) {
+    static std::vector<std::pair<const chs_handle*, std::vector<CnStencils>>> all;
+    for (auto& e : all)
+        if (e.first == h) return e.second;
+    all.push_back({h, std::vector<CnStencils>()});
+    return all.back().second;
+}
+
+void release_stencils(const chs_handle* h) {
+    auto& v = stencil_cache(h);
+    for (auto& c : v)
+        if (c.dev) hipFree(c.dev);
+    v.clear();
+}
+
+static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
+    auto& cache = stencil_cache(h);
+    for (auto& c : cache)
+        if (c.facoe == facoe) {
+            *out = &c;
+            return CHS_OK;
+        }
+    CnStencils st;
+    st.facoe = facoe;
+    const int M = h->M;
+    const double COE = -facoe * h->dxqi;
+    const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
+    const double RMC2 = 1.0 * (-facoe * h->dzqi);
+    const double za = RMC2, zc = RMC2, zb = 1.0 - zc - za;
+    std::vector<double> wx, wz;
+    st.Kx = cyclic_stencil(h->L, xa, xb, xc, wx);
+    st.Kz = cyclic_stencil(h->N, za, zb, zc, wz);
+    // y: full inverse of each component's non-cyclic system (rows j = 0..M-1)
+    std::vector<std::vector<ld>> inv(3 * (size_t)M);
+    std::vector<double> amvr(3 * (M + 1)), acvr(3 * (M + 1)), apvr(3 * (M + 1));
+    CHS_HIP(hipMemcpy(amvr.data(), h->amvr, amvr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    CHS_HIP(hipMemcpy(acvr.data(), h->acvr, acvr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    CHS_HIP(hipMemcpy(apvr.data(), h->apvr, apvr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    int Ky = 0;
+    for (int c = 0; c < 3; ++c) {
+        std::vector<ld> a(M), b(M), cc(M), col(M);
+        for (int j = 0; j < M; ++j) {
+            const int o = c * (M + 1) + j;
+            b[j] = (ld)(1.0 - facoe * acvr[o]);
+            cc[j] = (ld)(-facoe * apvr[o]);
+            a[j] = (ld)(-facoe * amvr[o]);
+        }
+        a[0] = 0;
+        cc[M - 1] = 0;
+        for (int j = 0; j < M; ++j) inv[c * M + j].assign(M, 0);
+        for (int m = 0; m < M; ++m) {  // column m of the inverse
+            std::fill(col.begin(), col.end(), (ld)0);
+            col[m] = 1;
+            thomas_ld(M, a.data(), b.data(), cc.data(), col.data());
+            for (int j = 0; j < M; ++j) inv[c * M + j][m] = col[j];
+        }
+        for (int j = 0; j < M; ++j) Ky = std::max(Ky, trunc_width(inv[c * M + j], j, false));
+    }
+    st.Ky = Ky;
+    st.ok = st.Kx <= KMAX && st.Kz <= KMAX && st.Ky <= KMAX;
+    if (st.ok) {
+        const int W = 2 * Ky + 1;
+        std::vector<double> buf(wx.size() + wz.size() + 3 * (size_t)M * W, 0.0);
+        st.ox = 0;
+        st.oz = wx.size();
+        st.oy = wx.size() + wz.size();
+        std::copy(wx.begin(), wx.end(), buf.begin());
+        std::copy(wz.begin(), wz.end(), buf.begin() + st.oz);
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < M; ++j)
+                for (int m = -Ky; m <= Ky; ++m) {
+                    const int jm = j + m;
+                    if (jm >= 0 && jm < M) buf[st.oy + ((size_t)c * M + j) * W + (m + Ky)] = (double)inv[c * M + j][jm];
+                }
+        CHS_HIP(hipMalloc((void**)&st.dev, buf.size() * sizeof(double)));
+        CHS_HIP(hipMemcpy(st.dev, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (cache.size() >= 8) {
+        if (cache.front().dev) hipFree(cache.front().dev);
+        cache.erase(cache.begin());
+    }
+    cache.push_back(st);
+    *out = &cache.back();
+    return CHS_OK;
+}
+
 // The x solve of k_rhs_x uses L/XMS segments of one block (XLPB*L/XMS threads,
 // a power of two for the block reductions); the z rows come in groups of KR.
 bool substep_supported(const chs_handle* h) {
     const int P = h->L / XMS;
     return h->L % XMS == 0 && P >= 2 && XLPB * P <= 1024 && (P & (P - 1)) == 0 && h->N % KR == 0;
+}
+
+// last substep's CN solve path: 1 = truncated-inverse stencils, 0 = segmented Thomas
+int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky) {
+    const CnStencils* cn = nullptr;
+    CHS_TRY(cn_stencils(h, facoe, &cn));
+    *kx = cn->Kx;
+    *kz = cn->Kz;
+    *ky = cn->Ky;
+    return cn->ok && h->cn_mode != 1 && h->N % ZTK == 0 && (size_t)h->M * YTI * 8 + YTI * 8 <= 64 * 1024 ? 1 : 0;
 }
 
 int rk3_substep(chs_handle* h, int ns, double dt) {
@@ -459,6 +746,10 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     const size_t ncell = (size_t)h->L * h->M * h->N;
     const double facoe = 0.5 * rk.talp * dt * h->cvisc;  // FACOE_TG (src/MOMFA_tg.f90:29)
     hipEvent_t ev;
+    const CnStencils* cn = nullptr;
+    CHS_TRY(cn_stencils(h, facoe, &cn));
+    const bool use_st = cn->ok && h->rhs2 != nullptr && h->cn_mode != 1 && h->N % ZTK == 0 &&
+                        (size_t)h->M * YTI * 8 + YTI * 8 <= 64 * 1024;
     // 1. RHS + x solve
     {
         const double COE = -facoe * h->dxqi;
@@ -470,7 +761,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
                                    seg_smem_bytes<double>(XLPB, P));
         h->timer.begin("rhs_x", s, &ev);
         CHS_LAUNCH(k_rhs_x, dim3(((nblk + 7) / 8) * 8), dim3(nthr), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0,
-                   h->rhs, h->partials, xa, xb, xc);
+                   h->rhs, h->partials, xa, xb, xc, use_st ? cn->Kx : -1, use_st ? cn->dev + cn->ox : nullptr);
         h->timer.end(ev, "rhs_x", s);
         h->timer.begin("massflux", s, &ev);
         CHS_LAUNCH(k_massflux, dim3(1), dim3(1024), 0, s, h->partials, nblk, h->volm, h->scalars);
@@ -479,6 +770,14 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const double RMC2 = 1.0 * (-facoe * h->dzqi);
         const double za = RMC2, zc = RMC2, zb = 1.0 - zc - za;
         const double xone = 1.0 / (xa + xb + xc);
+        const int usedpg0 = h->flowdrvtp == 1 ? 1 : 0;
+        if (use_st) {
+            const int nblkz = 3 * h->M * (h->N / ZTK) * ((h->L + ZTI - 1) / ZTI);
+            h->timer.begin("z_solve", s, &ev);
+            CHS_LAUNCH(k_z_stencil, dim3(nblkz), dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0,
+                       cn->Kz, cn->dev + cn->oz);
+            h->timer.end(ev, "z_solve", s);
+        }
         int MS, Pz, LPB;
         if (!tdma_seg_config(h->N, true, &MS, &Pz, &LPB, 1024)) {
             set_error("substep: unsupported NCL3");
@@ -488,6 +787,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const dim3 gz((nlines + LPB - 1) / LPB), bz(LPB * Pz);
         const size_t smz = seg_smem_bytes<double>(LPB, Pz);
         const int usedpg = h->flowdrvtp == 1 ? 1 : 0;
+        if (!use_st) {
         h->timer.begin("z_solve", s, &ev);
         switch (MS) {
             case 16: CHS_LAUNCH(k_z_solve<16>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
@@ -496,9 +796,16 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
             default: CHS_LAUNCH(k_z_solve<8>, gz, bz, smz, s, g, h->rhs, h->scalars, xone, za, zb, zc, LPB, Pz, usedpg); break;
         }
         h->timer.end(ev, "z_solve", s);
+        }
     }
     // 3. y solves + Q update + BC_WALL_Q
-    {
+    if (use_st) {
+        const int nblky = h->N * ((h->L + YTI - 1) / YTI);
+        const size_t smy = sizeof(double) * ((size_t)h->M * YTI + YTI);
+        h->timer.begin("y_solve", s, &ev);
+        CHS_LAUNCH(k_y_stencil, dim3(nblky), dim3(256), smy, s, g, h->rhs2, h->q, cn->Ky, cn->dev + cn->oy);
+        h->timer.end(ev, "y_solve", s);
+    } else {
         int MS, P, LPB;
         if (!tdma_seg_config(h->M, false, &MS, &P, &LPB, 768)) {
             set_error("substep: unsupported NCL2");

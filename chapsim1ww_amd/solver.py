@@ -98,6 +98,15 @@ class ChannelSolver:
         nat.check(self.lib.chs_poisson_stats(self.h, C.byref(it), C.byref(rel)))
         return it.value, rel.value
 
+    def set_cn_solver(self, mode: str):
+        """'auto' (truncated-inverse stencils where their bound holds) or 'thomas'."""
+        nat.check(self.lib.chs_set_cn_solver(self.h, {"auto": 0, "thomas": 1}[mode]))
+
+    def cn_info(self, ns: int, dt: float):
+        p, kx, kz, ky = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        nat.check(self.lib.chs_cn_info(self.h, ns, dt, C.byref(p), C.byref(kx), C.byref(kz), C.byref(ky)))
+        return {"path": "stencil" if p.value else "thomas", "kx": kx.value, "kz": kz.value, "ky": ky.value}
+
     def divgck(self) -> float:
         v = C.c_double()
         nat.check(self.lib.chs_divgck(self.h, C.byref(v)))

@@ -89,6 +89,17 @@ int chs_poisson_fft99(chs_handle *h, const double *f_dev, double *dph_dev);
 int chs_set_refinement(chs_handle *h, double rel_tol, int max_iter);
 int chs_poisson_stats(chs_handle *h, int *iters, double *rel_resid);
 
+/* Crank-Nicolson line solves of MOMFA_tg (src/MOMFA_tg.f90:17-240) inside
+ * chs_rk3_substep.  mode 0 (default): where every row of the inverse of the
+ * x, z and y CN systems drops below 1e-17 of its absolute sum within 16 rows,
+ * the solves run as the truncated inverses (short stencils, computed exactly
+ * on the host in extended precision); otherwise, and always with mode 1, as
+ * segmented Thomas solves. */
+int chs_set_cn_solver(chs_handle *h, int mode);
+/* Which path substep NS with time step dt takes (1 = stencils, 0 = Thomas)
+ * and the stencil half-widths in x, z, y (> 16 when too wide). */
+int chs_cn_info(chs_handle *h, int ns, double dt, int *path, int *kx, int *kz, int *ky);
+
 /* max|div u| of the device state: DIVGCK_tg (src/DIVGCK.f90:16-58). */
 int chs_divgck(chs_handle *h, double *maxdiv_host);
 /* max convective CFL rate: CFL_tg (src/CFL.f90:21-59); dt_cfl = CFL/rate. */

@@ -146,13 +146,18 @@ def test_poisson_mixed_refined(solver_mod, dims):
 # --------------------------------------------------------------------------
 # full RK3 substeps of the tg channel
 # --------------------------------------------------------------------------
-def _run_pair(dims, nsteps, seed=1973, precision="fp64", solver_mod=None):
+def _run_pair(dims, nsteps, seed=1973, precision="fp64", solver_mod=None, cn="auto", expect_path=None):
     g = ChannelGrid(*dims)
     og = O.OracleGrid(*dims)
     q, pr, c0 = fields.init_tg_fields(g, seed=seed)
     st = O.OracleState(og, q, pr, c0)
     dt = 0.5 / st.cfl()
     s = solver_mod.ChannelSolver(g, precision)
+    s.set_cn_solver(cn)
+    if expect_path is not None:
+        for ns in (0, 1, 2, 3):
+            info = s.cn_info(ns, dt)
+            assert info["path"] == expect_path, info
     s.upload_state(q, pr, c0)
     out = []
     ns_seq = [0] + [1, 2, 3] * nsteps
@@ -166,14 +171,18 @@ def _run_pair(dims, nsteps, seed=1973, precision="fp64", solver_mod=None):
     return g, out
 
 
-@pytest.mark.parametrize("dims", [(32, 16, 16), (64, 64, 64)])
-def test_substep_fp64(solver_mod, dims):
-    print(f"substep fp64 {dims}", flush=True)
-    g, out = _run_pair(dims, 2, solver_mod=solver_mod)
+@pytest.mark.parametrize("dims,cn,path", [((32, 16, 16), "auto", "thomas"), ((64, 64, 64), "auto", "stencil"),
+                                          ((64, 64, 64), "thomas", "thomas"), ((128, 96, 64), "auto", "stencil")])
+def test_substep_fp64(solver_mod, dims, cn, path):
+    """7 substeps (NS = 0, then 1,2,3 twice) against the oracle; every state
+    field within 1e-12 of the oracle after every substep."""
+    print(f"substep fp64 {dims} cn={cn}", flush=True)
+    g, out = _run_pair(dims, 2, solver_mod=solver_mod, cn=cn, expect_path=path)
     for i, (ns, gpu, ref, dg, dr, pg, pr_) in enumerate(out):
-        for name, a, b in zip(("Q", "PR", "CONVH0"), gpu, ref):
-            e = _rel(a, b)
-            assert e < FP64_TOL * (10 ** min(i, 2)), (i, ns, name, e)
+        errs = {name: _rel(a, b) for name, a, b in zip(("Q", "PR", "CONVH0"), gpu, ref)}
+        print(i, ns, errs, dg, flush=True)
+        for name, e in errs.items():
+            assert e < FP64_TOL, (i, ns, name, e)
         assert dg < 1e-10 and dr < 1e-10
         assert abs(pg - pr_) <= 1e-12 * max(abs(pr_), 1e-300) + 1e-18
 
