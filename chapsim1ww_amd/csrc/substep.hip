@@ -40,7 +40,8 @@ __device__ __forceinline__ size_t cidx(const Geo& g, int i, int j, int k) {
     return (size_t)i + (size_t)g.L * ((size_t)j + (size_t)g.M * k);
 }
 
-constexpr int KR = 4;        // k rows per k_rhs_x block
+constexpr int KR = 4;        // k rows per k_rhs_x block (segmented x solve)
+constexpr int KRS = 8;       // k rows per k_rhs_x block (stencil x solve)
 constexpr int XLPB = 16;     // line slots of the x solve (3*KR = 12 used)
 constexpr int XMS = 8;       // rows per segment of the x solve
 
@@ -113,26 +114,26 @@ struct RkCoef {
 __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
                                                const double* __restrict__ PR, double* __restrict__ H0,
                                                double* __restrict__ RHS, double* __restrict__ part, double xa,
-                                               double xb, double xc, int Kx, const double* __restrict__ wx) {
+                                               double xb, double xc, int Kx, const double* __restrict__ wx, int kr) {
     extern __shared__ unsigned char smem[];
     const int L = g.L;
     const int LS = L + 1;  // padded LDS row stride (bank spread for the segment reads)
-    // LDS: sR (the 3*KR RHS rows) | red (block sum); the reduced systems of
+    // LDS: sR (the 3*kr RHS rows) | red (block sum); the reduced systems of
     // the x solves reuse the front once sR has been read into registers
     double* sR = reinterpret_cast<double*>(smem);
-    unsigned char* sred = smem + sizeof(double) * 3 * KR * LS;
-    const int nkb = g.N / KR;
+    unsigned char* sred = smem + sizeof(double) * 3 * kr * LS;
+    const int nkb = g.N / kr;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const bool active = t < g.M * nkb;
     const int j = active ? t / nkb : 0;
-    const int k0 = active ? (t - j * nkb) * KR : 0;
+    const int k0 = active ? (t - j * nkb) * kr : 0;
     const size_t LMN = (size_t)L * g.M * g.N;
     const int J = j + 1;
     const double COE1 = rk.talp * cvisc;
     const double COE2 = rk.talp * rk.dt;
     double msum = 0.0;
     if (active) {
-        for (int e = threadIdx.x; e < KR * L; e += blockDim.x) {
+        for (int e = threadIdx.x; e < kr * L; e += blockDim.x) {
             const int r = e / L, i = e - r * L;
             const int k = k0 + r;
             const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
                     pgm = (PR[gidx(g, i, J, k)] - PR[gidx(g, i, J, km)]) * g.dzi * COE2;
                 rhs = rhs - pgm;
                 if (c == 0) msum += rhs / g.dyfi[j] / 1.0;
-                sR[(c * KR + r) * LS + i] = rhs;
+                sR[(c * kr + r) * LS + i] = rhs;
             }
         }
     }
@@ -179,9 +180,9 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
     // phase 2a: x solve as the truncated inverse of the constant cyclic system,
     // a (2Kx+1)-point periodic stencil over the LDS rows (Kx >= 0)
     if (Kx >= 0) {
-        for (int e = threadIdx.x; e < 3 * KR * L; e += blockDim.x) {
+        for (int e = threadIdx.x; e < 3 * kr * L; e += blockDim.x) {
             const int cr = e / L, i = e - cr * L;
-            const int c = cr / KR, r = cr - c * KR;
+            const int c = cr / kr, r = cr - c * kr;
             if (!active || (c == 1 && j == 0)) continue;
             const double* row = sR + cr * LS;
             double acc = 0.0;
@@ -194,18 +195,18 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
         }
         return;
     }
-    // phase 2b: x solves of the 3*KR rows, XMS rows per segment, L/XMS segments
+    // phase 2b: x solves of the 3*kr rows, XMS rows per segment, L/XMS segments
     const int P = L / XMS;
     const int ll = threadIdx.x % XLPB, seg = threadIdx.x / XLPB;
-    const int c = ll / KR, r = ll - c * KR;
-    const bool valid = active && ll < 3 * KR && !(c == 1 && j == 0);
+    const int c = ll / kr, r = ll - c * kr;
+    const bool valid = active && ll < 3 * kr && !(c == 1 && j == 0);
     double a[XMS], b[XMS], cc[XMS], d[XMS];
 #pragma unroll
     for (int l = 0; l < XMS; ++l) {
         a[l] = xa;
         b[l] = xb;
         cc[l] = xc;
-        d[l] = valid ? sR[(c * KR + r) * LS + seg * XMS + l] : 0.0;
+        d[l] = valid ? sR[(c * kr + r) * LS + seg * XMS + l] : 0.0;
     }
     __syncthreads();  // sR fully read: its space now holds the reduced systems
     seg_solve<XMS, double>(a, b, cc, d, XLPB, P, ll, seg, valid, true, smem);
@@ -352,7 +353,6 @@ __global__ void __launch_bounds__(768) k_y_solve(Geo g, double facoe, const doub
 constexpr int KMAX = 16;
 constexpr int ZTK = 32;     // k rows per block of the z stencil
 constexpr int ZTI = 64;     // i columns per block of the z stencil
-constexpr int YTI = 32;     // i columns per block of the y stencil
 
 // z: out(i,j,k) = sum_m wz[m] (R(i,j,k+m) - D) for u (D = DPGRNS pushed through
 // the x solve), plain sum for v (j >= 1) and w.  Reads RHS, writes RHS2.
@@ -394,49 +394,58 @@ __global__ void __launch_bounds__(256) k_z_stencil(Geo g, const double* __restri
 // y: Delta_c(j) = sum_m W_c[j][m] R_c(j+m), rows clipped to [0, M); for v the
 // row-1 right-hand side is the u increment on j = 1 (what RHS_tg holds there in
 // the reference).  Then Q += Delta from N2I (MOMFA_tg:39-45) and BC_WALL_Q_tg.
-__global__ void __launch_bounds__(256) k_y_stencil(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
-                                                   int Ky, const double* __restrict__ wy) {
-    extern __shared__ unsigned char smem[];
+// y, register form: one thread per (i, k) column and chunk of YJC rows; the
+// window R(j-KW..j+KW) slides along j in registers and every lane of a wave
+// (64 consecutive i) applies the same row weights W_c[j][.] (scalar loads).
+// wy rows are zero-padded to 2KW+1 entries (zero outside [0, M)).
+constexpr int YJC = 48;
+template <int KW>
+__global__ void __launch_bounds__(256) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
+                                                  const double* __restrict__ wy) {
     const int L = g.L, M = g.M, N = g.N;
-    double* tile = reinterpret_cast<double*>(smem);   // [M][YTI]
-    double* du0 = tile + (size_t)M * YTI;              // [YTI]
-    const int ntx = (L + YTI - 1) / YTI;
+    const int ntx = (L + 63) / 64;
+    const int nch = (M + YJC - 1) / YJC;
+    const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;   // blockDim = 64 * nch
     const int it = blockIdx.x % ntx, k = blockIdx.x / ntx;
-    const int i0 = it * YTI;
+    const int i = it * 64 + lane;
+    const bool valid = i < L && ch < nch;
+    const int j0 = ch * YJC, j1 = min(M, j0 + YJC);
     const size_t LM = (size_t)L * M;
-    const int W = 2 * Ky + 1;
+    constexpr int W = 2 * KW + 1;
+    double du0 = 0.0;
+    if (!valid) return;
     for (int c = 0; c < 3; ++c) {
-        const size_t base = (size_t)c * LM * N + (size_t)k * LM;
-        for (int e = threadIdx.x; e < M * YTI; e += blockDim.x) {
-            const int jj = e / YTI, ii = e - jj * YTI;
-            const int i = i0 + ii;
-            double v = 0.0;
-            if (i < L) v = (c == 1 && jj == 0) ? du0[ii] : R2[base + (size_t)jj * L + i];
-            tile[e] = v;
-        }
-        __syncthreads();
+        const double* Rc = R2 + (size_t)c * LM * N + (size_t)k * LM + i;
         const double* wc = wy + (size_t)c * M * W;
-        for (int e = threadIdx.x; e < M * YTI; e += blockDim.x) {
-            const int jj = e / YTI, ii = e - jj * YTI;
-            const int i = i0 + ii;
-            if (i >= L) continue;
-            const int m0 = jj - Ky < 0 ? -jj : -Ky;
-            const int m1 = jj + Ky >= M ? M - 1 - jj : Ky;
-            const double* w = wc + (size_t)jj * W + Ky;
+        double win[W];
+#pragma unroll
+        for (int m = 0; m < W; ++m) {
+            const int jj = j0 - KW + m;
+            double v = 0.0;
+            if (jj >= 0 && jj < M) v = (c == 1 && jj == 0) ? du0 : Rc[(size_t)jj * L];
+            win[m] = v;
+        }
+        for (int j = j0; j < j1; ++j) {
+            const double* w = wc + (size_t)j * W;
             double acc = 0.0;
-            for (int m = m0; m <= m1; ++m) acc += w[m] * tile[(jj + m) * YTI + ii];
-            if (c == 0 && jj == 0) du0[ii] = acc;
-            if (!(c == 1 && jj == 0)) {
-                const size_t qo = qidx(g, c, i, jj + 1, k);
+#pragma unroll
+            for (int m = 0; m < W; ++m) acc += w[m] * win[m];
+            if (c == 0 && j == 0) du0 = acc;
+            if (!(c == 1 && j == 0)) {
+                const size_t qo = qidx(g, c, i, j + 1, k);
                 Q[qo] = acc + Q[qo];
             }
-            if (jj == 0) {
-                Q[qidx(g, c, i, 0, k)] = 0.0;
-                if (c == 1) Q[qidx(g, 1, i, 1, k)] = 0.0;
-            }
-            if (jj == M - 1) Q[qidx(g, c, i, M + 1, k)] = 0.0;
+            // slide the window by one row
+#pragma unroll
+            for (int m = 0; m < W - 1; ++m) win[m] = win[m + 1];
+            const int jn = j + KW + 1;
+            win[W - 1] = (jn < M) ? Rc[(size_t)jn * L] : 0.0;
         }
-        __syncthreads();
+        if (ch == 0) {
+            Q[qidx(g, c, i, 0, k)] = 0.0;
+            if (c == 1) Q[qidx(g, 1, i, 1, k)] = 0.0;
+        }
+        if (ch == nch - 1) Q[qidx(g, c, i, M + 1, k)] = 0.0;
     }
 }
 
@@ -626,7 +635,7 @@ static int cyclic_stencil(int n, double a, double b, double c, std::vector<doubl
 
 struct CnStencils {
     double facoe = -1.0;
-    int Kx = -1, Kz = -1, Ky = -1;
+    int Kx = -1, Kz = -1, Ky = -1, KW = 4;
     bool ok = false;
     double* dev = nullptr;   // wx[2Kx+1] | wz[2Kz+1] | wy[3][M][2Ky+1]
     size_t ox = 0, oz = 0, oy = 0;
@@ -692,8 +701,10 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
     }
     st.Ky = Ky;
     st.ok = st.Kx <= KMAX && st.Kz <= KMAX && st.Ky <= KMAX;
+    st.KW = Ky <= 4 ? 4 : Ky <= 8 ? 8 : Ky <= 12 ? 12 : 16;
     if (st.ok) {
-        const int W = 2 * Ky + 1;
+        const int W = 2 * st.KW + 1;
+        Ky = st.KW;
         std::vector<double> buf(wx.size() + wz.size() + 3 * (size_t)M * W, 0.0);
         st.ox = 0;
         st.oz = wx.size();
@@ -704,7 +715,8 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
             for (int j = 0; j < M; ++j)
                 for (int m = -Ky; m <= Ky; ++m) {
                     const int jm = j + m;
-                    if (jm >= 0 && jm < M) buf[st.oy + ((size_t)c * M + j) * W + (m + Ky)] = (double)inv[c * M + j][jm];
+                    if (jm >= 0 && jm < M && m >= -st.Ky && m <= st.Ky)
+                        buf[st.oy + ((size_t)c * M + j) * W + (m + Ky)] = (double)inv[c * M + j][jm];
                 }
         CHS_HIP(hipMalloc((void**)&st.dev, buf.size() * sizeof(double)));
         CHS_HIP(hipMemcpy(st.dev, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -732,7 +744,7 @@ int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky) {
     *kx = cn->Kx;
     *kz = cn->Kz;
     *ky = cn->Ky;
-    return cn->ok && h->cn_mode != 1 && h->N % ZTK == 0 && (size_t)h->M * YTI * 8 + YTI * 8 <= 64 * 1024 ? 1 : 0;
+    return cn->ok && h->cn_mode != 1 && h->N % ZTK == 0 && (h->M + YJC - 1) / YJC <= 4 && h->M > 16 ? 1 : 0;
 }
 
 int rk3_substep(chs_handle* h, int ns, double dt) {
@@ -748,20 +760,24 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     hipEvent_t ev;
     const CnStencils* cn = nullptr;
     CHS_TRY(cn_stencils(h, facoe, &cn));
+    // stencil path needs: N in whole z tiles, at most 4 y chunks of YJC rows
+    // (blocks of 64*chunks threads), and chunks longer than the window
     const bool use_st = cn->ok && h->rhs2 != nullptr && h->cn_mode != 1 && h->N % ZTK == 0 &&
-                        (size_t)h->M * YTI * 8 + YTI * 8 <= 64 * 1024;
+                        (h->M + YJC - 1) / YJC <= 4 && h->M > 16;
     // 1. RHS + x solve
     {
         const double COE = -facoe * h->dxqi;
         const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
-        const int nblk = h->M * (h->N / KR);
+        const int kr = use_st && h->N % KRS == 0 ? KRS : KR;
+        const int nblk = h->M * (h->N / kr);
         const int P = h->L / XMS;
-        const int nthr = XLPB * P;
-        const size_t sm = std::max(sizeof(double) * (3 * KR * (h->L + 1) + (size_t)nthr),
-                                   seg_smem_bytes<double>(XLPB, P));
+        const int nthr = use_st ? 256 : XLPB * P;
+        const size_t sm = use_st ? sizeof(double) * (3 * kr * (h->L + 1) + (size_t)nthr)
+                                 : std::max(sizeof(double) * (3 * kr * (h->L + 1) + (size_t)nthr),
+                                            seg_smem_bytes<double>(XLPB, P));
         h->timer.begin("rhs_x", s, &ev);
         CHS_LAUNCH(k_rhs_x, dim3(((nblk + 7) / 8) * 8), dim3(nthr), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0,
-                   h->rhs, h->partials, xa, xb, xc, use_st ? cn->Kx : -1, use_st ? cn->dev + cn->ox : nullptr);
+                   h->rhs, h->partials, xa, xb, xc, use_st ? cn->Kx : -1, use_st ? cn->dev + cn->ox : nullptr, kr);
         h->timer.end(ev, "rhs_x", s);
         h->timer.begin("massflux", s, &ev);
         CHS_LAUNCH(k_massflux, dim3(1), dim3(1024), 0, s, h->partials, nblk, h->volm, h->scalars);
@@ -800,10 +816,16 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     }
     // 3. y solves + Q update + BC_WALL_Q
     if (use_st) {
-        const int nblky = h->N * ((h->L + YTI - 1) / YTI);
-        const size_t smy = sizeof(double) * ((size_t)h->M * YTI + YTI);
+        const int nch = (h->M + YJC - 1) / YJC;
+        const dim3 gy(h->N * ((h->L + 63) / 64)), by(64 * nch);
+        const double* wy = cn->dev + cn->oy;
         h->timer.begin("y_solve", s, &ev);
-        CHS_LAUNCH(k_y_stencil, dim3(nblky), dim3(256), smy, s, g, h->rhs2, h->q, cn->Ky, cn->dev + cn->oy);
+        switch (cn->KW) {
+            case 4: CHS_LAUNCH(k_y_window<4>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
+            case 8: CHS_LAUNCH(k_y_window<8>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
+            case 12: CHS_LAUNCH(k_y_window<12>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
+            default: CHS_LAUNCH(k_y_window<16>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
+        }
         h->timer.end(ev, "y_solve", s);
     } else {
         int MS, P, LPB;
