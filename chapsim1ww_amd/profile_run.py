@@ -1,0 +1,64 @@
+"""Short, fixed workload for hardware-counter profiles (rocprofv3 --pmc / --kernel-trace).
+
+    rocprofv3 --pmc FETCH_SIZE -- python -m chapsim1ww_amd.profile_run
+
+Runs, at 256x192x256: NSUB FP64 RK3 substeps, NPOIS FP64 Poisson solves,
+NTDMA batched tridiagonal solves (65536 lines x 192, IJJ form, non-cyclic
+and cyclic), and a plain device copy of 1 GiB (read 1 GiB + write 1 GiB)
+whose counter readings calibrate the byte counters of the others.
+"""
+from __future__ import annotations
+
+import argparse
+
+import numpy as np
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nsub", type=int, default=4)
+    ap.add_argument("--npois", type=int, default=4)
+    ap.add_argument("--ntdma", type=int, default=4)
+    ap.add_argument("--precision", default="fp64")
+    a = ap.parse_args()
+    import torch
+
+    from chapsim1ww_amd import fields
+    from chapsim1ww_amd.grid import ChannelGrid
+    from chapsim1ww_amd.solver import ChannelSolver, tdma_batched
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev)
+    g = ChannelGrid(256, 192, 256, ren=6880.0)
+    q, pr, c0 = fields.init_tg_fields(g, seed=1973)
+    s = ChannelSolver(g, a.precision, stream=st)
+    s.upload_state(q, pr, c0)
+    dt = 0.5 / s.cfl()
+    s.solverrk3_mom_tg(0, dt)
+    for i in range(a.nsub):
+        s.solverrk3_mom_tg(1 + i % 3, dt)
+    f = torch.from_numpy(np.asfortranarray(fields.poisson_rhs(g, seed=9)).ravel(order="F").copy()).to(dev)
+    dph = torch.zeros(g.L * (g.M + 2) * g.N, dtype=torch.float64, device=dev)
+    for _ in range(a.npois):
+        s.fft99_pois3d(f, dph)
+    n, nl = 192, 65536
+    rng = np.random.default_rng(2)
+    A = torch.from_numpy(rng.uniform(-0.9, -0.1, n * nl)).to(dev)
+    C = torch.from_numpy(rng.uniform(-0.9, -0.1, n * nl)).to(dev)
+    B = torch.from_numpy(2.0 + np.abs(rng.standard_normal(n * nl))).to(dev)
+    R = torch.from_numpy(rng.standard_normal(n * nl)).to(dev)
+    for i in range(a.ntdma):
+        tdma_batched(A, B, C, R, None, n, nl, nl, 1, i % 2 == 1, st)
+    # calibration: 1 GiB device-to-device copy
+    x = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    s.close()
+    print("profile_run done", flush=True)
+
+
+if __name__ == "__main__":
+    main()
