@@ -41,7 +41,7 @@ __device__ __forceinline__ size_t cidx(const Geo& g, int i, int j, int k) {
 }
 
 constexpr int KR = 4;        // k rows per k_rhs_x block (segmented x solve)
-constexpr int KRS = 8;       // k rows per k_rhs_x block (stencil x solve)
+constexpr int KRS = 4;       // k rows per k_rhs_x block (stencil x solve)
 constexpr int XLPB = 16;     // line slots of the x solve (3*KR = 12 used)
 constexpr int XMS = 8;       // rows per segment of the x solve
 
@@ -351,45 +351,6 @@ __global__ void __launch_bounds__(768) k_y_solve(Geo g, double facoe, const doub
 // segmented Thomas kernels above instead.
 // ---------------------------------------------------------------------------
 constexpr int KMAX = 16;
-constexpr int ZTK = 32;     // k rows per block of the z stencil
-constexpr int ZTI = 64;     // i columns per block of the z stencil
-
-// z: out(i,j,k) = sum_m wz[m] (R(i,j,k+m) - D) for u (D = DPGRNS pushed through
-// the x solve), plain sum for v (j >= 1) and w.  Reads RHS, writes RHS2.
-__global__ void __launch_bounds__(256) k_z_stencil(Geo g, const double* __restrict__ R, double* __restrict__ R2,
-                                                   const double* __restrict__ dpg, double xone, int use_dpg, int Kz,
-                                                   const double* __restrict__ wz) {
-    __shared__ double tile[(ZTK + 2 * KMAX) * ZTI];
-    const int L = g.L, M = g.M, N = g.N;
-    const int ntx = (L + ZTI - 1) / ZTI, ntk = N / ZTK;
-    int b = blockIdx.x;
-    const int it = b % ntx; b /= ntx;
-    const int kt = b % ntk; b /= ntk;
-    const int j = b % M;
-    const int c = b / M;
-    if (c == 1 && j == 0) return;  // whole block uniform: v on the wall face is not solved
-    const int i0 = it * ZTI, k0 = kt * ZTK;
-    const size_t LM = (size_t)L * M;
-    const size_t base = (size_t)c * LM * N + (size_t)j * L;
-    const double D = (use_dpg && c == 0) ? dpg[0] * xone : 0.0;
-    const int rows = ZTK + 2 * Kz;
-    for (int e = threadIdx.x; e < rows * ZTI; e += blockDim.x) {
-        const int r = e / ZTI, ii = e - r * ZTI;
-        int k = k0 - Kz + r;
-        k = k < 0 ? k + N : (k >= N ? k - N : k);
-        const int i = i0 + ii;
-        tile[e] = i < L ? R[base + (size_t)k * LM + i] - D : 0.0;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < ZTK * ZTI; e += blockDim.x) {
-        const int r = e / ZTI, ii = e - r * ZTI;
-        const int i = i0 + ii;
-        if (i >= L) continue;
-        double acc = 0.0;
-        for (int m = 0; m <= 2 * Kz; ++m) acc += wz[m] * tile[(r + m) * ZTI + ii];
-        R2[base + (size_t)(k0 + r) * LM + i] = acc;
-    }
-}
 
 // y: Delta_c(j) = sum_m W_c[j][m] R_c(j+m), rows clipped to [0, M); for v the
 // row-1 right-hand side is the u increment on j = 1 (what RHS_tg holds there in
@@ -398,25 +359,29 @@ __global__ void __launch_bounds__(256) k_z_stencil(Geo g, const double* __restri
 // window R(j-KW..j+KW) slides along j in registers and every lane of a wave
 // (64 consecutive i) applies the same row weights W_c[j][.] (scalar loads).
 // wy rows are zero-padded to 2KW+1 entries (zero outside [0, M)).
-constexpr int YJC = 48;
+constexpr int YJC = 32;     // rows per thread chunk of the y window
+constexpr int WU = 4;       // rows fetched ahead per step of the windows
 template <int KW>
-__global__ void __launch_bounds__(256) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
+__global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
                                                   const double* __restrict__ wy) {
     const int L = g.L, M = g.M, N = g.N;
     const int ntx = (L + 63) / 64;
     const int nch = (M + YJC - 1) / YJC;
     const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;   // blockDim = 64 * nch
     const int it = blockIdx.x % ntx, k = blockIdx.x / ntx;
-    const int i = it * 64 + lane;
-    const bool valid = i < L && ch < nch;
-    const int j0 = ch * YJC, j1 = min(M, j0 + YJC);
+    extern __shared__ double sw[];   // the component's row weights [M][W]
+    const bool valid = it * 64 + lane < L;
+    const int i = valid ? it * 64 + lane : 0;
+    const int j0 = ch * YJC, j1 = valid ? min(M, j0 + YJC) : j0;   // invalid lanes do no rows
     const size_t LM = (size_t)L * M;
     constexpr int W = 2 * KW + 1;
     double du0 = 0.0;
-    if (!valid) return;
     for (int c = 0; c < 3; ++c) {
         const double* Rc = R2 + (size_t)c * LM * N + (size_t)k * LM + i;
-        const double* wc = wy + (size_t)c * M * W;
+        __syncthreads();
+        for (int e = threadIdx.x; e < M * W; e += blockDim.x) sw[e] = wy[(size_t)c * M * W + e];
+        __syncthreads();
+        if (!valid) continue;
         double win[W];
 #pragma unroll
         for (int m = 0; m < W; ++m) {
@@ -425,27 +390,95 @@ __global__ void __launch_bounds__(256) k_y_window(Geo g, const double* __restric
             if (jj >= 0 && jj < M) v = (c == 1 && jj == 0) ? du0 : Rc[(size_t)jj * L];
             win[m] = v;
         }
+        // prefetch queues: the window rows and the Q rows WU steps ahead
+        double rq[WU], qq[WU];
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+            const int jn = j0 + u + KW + 1;
+            rq[u] = jn < M ? Rc[(size_t)jn * L] : 0.0;
+            qq[u] = j0 + u < j1 ? Q[qidx(g, c, i, j0 + u + 1, k)] : 0.0;
+        }
         for (int j = j0; j < j1; ++j) {
-            const double* w = wc + (size_t)j * W;
+            const double* w = sw + (size_t)j * W;
             double acc = 0.0;
 #pragma unroll
             for (int m = 0; m < W; ++m) acc += w[m] * win[m];
             if (c == 0 && j == 0) du0 = acc;
-            if (!(c == 1 && j == 0)) {
-                const size_t qo = qidx(g, c, i, j + 1, k);
-                Q[qo] = acc + Q[qo];
-            }
-            // slide the window by one row
+            if (!(c == 1 && j == 0)) Q[qidx(g, c, i, j + 1, k)] = acc + qq[0];
 #pragma unroll
             for (int m = 0; m < W - 1; ++m) win[m] = win[m + 1];
-            const int jn = j + KW + 1;
-            win[W - 1] = (jn < M) ? Rc[(size_t)jn * L] : 0.0;
+            win[W - 1] = rq[0];
+#pragma unroll
+            for (int u = 0; u < WU - 1; ++u) {
+                rq[u] = rq[u + 1];
+                qq[u] = qq[u + 1];
+            }
+            const int jn = j + WU + KW + 1, jq = j + WU;
+            rq[WU - 1] = jn < M ? Rc[(size_t)jn * L] : 0.0;
+            qq[WU - 1] = jq < j1 ? Q[qidx(g, c, i, jq + 1, k)] : 0.0;
         }
         if (ch == 0) {
             Q[qidx(g, c, i, 0, k)] = 0.0;
             if (c == 1) Q[qidx(g, 1, i, 1, k)] = 0.0;
         }
         if (ch == nch - 1) Q[qidx(g, c, i, M + 1, k)] = 0.0;
+    }
+}
+
+// z, register form: one thread per (i, j) column and chunk of ZKC rows of k
+// (cyclic), the window slides along k; weights are the same for every lane.
+// u subtracts D = DPGRNS * xone on load (the mass-flux correction pushed
+// through the x solve).  Reads RHS, writes RHS2.
+constexpr int ZKC = 64;
+template <int KW>
+__global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restrict__ R, double* __restrict__ R2,
+                                                  const double* __restrict__ dpg, double xone, int use_dpg,
+                                                  const double* __restrict__ wz) {
+    const int L = g.L, M = g.M, N = g.N;
+    const int ntx = (L + 63) / 64;
+    const int nch = N / ZKC;
+    int b = blockIdx.x;
+    const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;     // blockDim = 64 * 4
+    const int it = b % ntx; b /= ntx;
+    const int j = b % M;
+    const int cc = b / M;                                          // component
+    const int chunk = blockIdx.y * 4 + ch;
+    const int i = it * 64 + lane;
+    if (i >= L || chunk >= nch || (cc == 1 && j == 0)) return;
+    const size_t LM = (size_t)L * M;
+    const double* Rc = R + (size_t)cc * LM * N + (size_t)j * L + i;
+    double* Oc = R2 + (size_t)cc * LM * N + (size_t)j * L + i;
+    const double D = (use_dpg && cc == 0) ? dpg[0] * xone : 0.0;
+    constexpr int W = 2 * KW + 1;
+    double w[W];
+#pragma unroll
+    for (int m = 0; m < W; ++m) w[m] = wz[m];
+    const int k0 = chunk * ZKC;
+    double win[W];
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+        int kk = k0 - KW + m;
+        kk = kk < 0 ? kk + N : (kk >= N ? kk - N : kk);
+        win[m] = Rc[(size_t)kk * LM] - D;
+    }
+    for (int kb = k0; kb < k0 + ZKC; kb += WU) {
+        double rn[WU];
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+            int kn = kb + u + KW + 1;
+            kn = kn >= N ? kn - N : kn;
+            rn[u] = Rc[(size_t)kn * LM] - D;
+        }
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < W; ++m) acc += w[m] * win[m];
+            Oc[(size_t)(kb + u) * LM] = acc;
+#pragma unroll
+            for (int m = 0; m < W - 1; ++m) win[m] = win[m + 1];
+            win[W - 1] = rn[u];
+        }
     }
 }
 
@@ -635,7 +668,7 @@ static int cyclic_stencil(int n, double a, double b, double c, std::vector<doubl
 
 struct CnStencils {
     double facoe = -1.0;
-    int Kx = -1, Kz = -1, Ky = -1, KW = 4;
+    int Kx = -1, Kz = -1, Ky = -1, KW = 4, KWz = 4;
     bool ok = false;
     double* dev = nullptr;   // wx[2Kx+1] | wz[2Kz+1] | wy[3][M][2Ky+1]
     size_t ox = 0, oz = 0, oy = 0;
@@ -702,15 +735,17 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
     st.Ky = Ky;
     st.ok = st.Kx <= KMAX && st.Kz <= KMAX && st.Ky <= KMAX;
     st.KW = Ky <= 4 ? 4 : Ky <= 8 ? 8 : Ky <= 12 ? 12 : 16;
+    st.KWz = st.Kz <= 4 ? 4 : st.Kz <= 8 ? 8 : st.Kz <= 12 ? 12 : 16;
     if (st.ok) {
         const int W = 2 * st.KW + 1;
         Ky = st.KW;
-        std::vector<double> buf(wx.size() + wz.size() + 3 * (size_t)M * W, 0.0);
+        const size_t nwz = 2 * (size_t)st.KWz + 1;   // z weights zero-padded to the window
+        std::vector<double> buf(wx.size() + nwz + 3 * (size_t)M * W, 0.0);
         st.ox = 0;
         st.oz = wx.size();
-        st.oy = wx.size() + wz.size();
+        st.oy = wx.size() + nwz;
         std::copy(wx.begin(), wx.end(), buf.begin());
-        std::copy(wz.begin(), wz.end(), buf.begin() + st.oz);
+        std::copy(wz.begin(), wz.end(), buf.begin() + st.oz + (st.KWz - st.Kz));
         for (int c = 0; c < 3; ++c)
             for (int j = 0; j < M; ++j)
                 for (int m = -Ky; m <= Ky; ++m) {
@@ -744,7 +779,7 @@ int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky) {
     *kx = cn->Kx;
     *kz = cn->Kz;
     *ky = cn->Ky;
-    return cn->ok && h->cn_mode != 1 && h->N % ZTK == 0 && (h->M + YJC - 1) / YJC <= 4 && h->M > 16 ? 1 : 0;
+    return cn->ok && h->cn_mode != 1 && h->N % ZKC == 0 && (h->M + YJC - 1) / YJC <= 8 && h->M > 16 ? 1 : 0;
 }
 
 int rk3_substep(chs_handle* h, int ns, double dt) {
@@ -762,13 +797,13 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_TRY(cn_stencils(h, facoe, &cn));
     // stencil path needs: N in whole z tiles, at most 4 y chunks of YJC rows
     // (blocks of 64*chunks threads), and chunks longer than the window
-    const bool use_st = cn->ok && h->rhs2 != nullptr && h->cn_mode != 1 && h->N % ZTK == 0 &&
-                        (h->M + YJC - 1) / YJC <= 4 && h->M > 16;
+    const bool use_st = cn->ok && h->rhs2 != nullptr && h->cn_mode != 1 && h->N % ZKC == 0 &&
+                        (h->M + YJC - 1) / YJC <= 8 && h->M > 16;
     // 1. RHS + x solve
     {
         const double COE = -facoe * h->dxqi;
         const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
-        const int kr = use_st && h->N % KRS == 0 ? KRS : KR;
+        const int kr = use_st && h->N % KRS == 0 ? KRS : KR;  // KRS = 4 rows with 256 threads measured best
         const int nblk = h->M * (h->N / kr);
         const int P = h->L / XMS;
         const int nthr = use_st ? 256 : XLPB * P;
@@ -788,10 +823,15 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const double xone = 1.0 / (xa + xb + xc);
         const int usedpg0 = h->flowdrvtp == 1 ? 1 : 0;
         if (use_st) {
-            const int nblkz = 3 * h->M * (h->N / ZTK) * ((h->L + ZTI - 1) / ZTI);
+            const dim3 gzw(3 * h->M * ((h->L + 63) / 64), (h->N / ZKC + 3) / 4);
+            const double* wz = cn->dev + cn->oz;
             h->timer.begin("z_solve", s, &ev);
-            CHS_LAUNCH(k_z_stencil, dim3(nblkz), dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0,
-                       cn->Kz, cn->dev + cn->oz);
+            switch (cn->KWz) {
+                case 4: CHS_LAUNCH(k_z_window<4>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
+                case 8: CHS_LAUNCH(k_z_window<8>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
+                case 12: CHS_LAUNCH(k_z_window<12>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
+                default: CHS_LAUNCH(k_z_window<16>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
+            }
             h->timer.end(ev, "z_solve", s);
         }
         int MS, Pz, LPB;
@@ -819,12 +859,13 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const int nch = (h->M + YJC - 1) / YJC;
         const dim3 gy(h->N * ((h->L + 63) / 64)), by(64 * nch);
         const double* wy = cn->dev + cn->oy;
+        const size_t smw = sizeof(double) * (size_t)h->M * (2 * cn->KW + 1);
         h->timer.begin("y_solve", s, &ev);
         switch (cn->KW) {
-            case 4: CHS_LAUNCH(k_y_window<4>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
-            case 8: CHS_LAUNCH(k_y_window<8>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
-            case 12: CHS_LAUNCH(k_y_window<12>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
-            default: CHS_LAUNCH(k_y_window<16>, gy, by, 0, s, g, h->rhs2, h->q, wy); break;
+            case 4: CHS_LAUNCH(k_y_window<4>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+            case 8: CHS_LAUNCH(k_y_window<8>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+            case 12: CHS_LAUNCH(k_y_window<12>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+            default: CHS_LAUNCH(k_y_window<16>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
         }
         h->timer.end(ev, "y_solve", s);
     } else {
