@@ -125,8 +125,11 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
     const int nkb = g.N / kr;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const bool active = t < g.M * nkb;
-    const int j = active ? t / nkb : 0;
-    const int k0 = active ? (t - j * nkb) * kr : 0;
+    // k-group-major: consecutive tiles (same XCD) walk j, so the j-1/j+1 rows
+    // a tile reads were loaded by its neighbours moments before (L2 hits)
+    const int kg = active ? t / g.M : 0;
+    const int j = active ? t - kg * g.M : 0;
+    const int k0 = kg * kr;
     const size_t LMN = (size_t)L * g.M * g.N;
     const int J = j + 1;
     const double COE1 = rk.talp * cvisc;
@@ -376,12 +379,18 @@ __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restric
     const size_t LM = (size_t)L * M;
     constexpr int W = 2 * KW + 1;
     double du0 = 0.0;
-    for (int c = 0; c < 3; ++c) {
+    const int c = blockIdx.y;
+    {
         const double* Rc = R2 + (size_t)c * LM * N + (size_t)k * LM + i;
-        __syncthreads();
         for (int e = threadIdx.x; e < M * W; e += blockDim.x) sw[e] = wy[(size_t)c * M * W + e];
         __syncthreads();
-        if (!valid) continue;
+        if (!valid) return;
+        if (c == 1 && ch == 0) {
+            // v's row-1 right-hand side: the u increment on j = 1 (row 0 of u's solve)
+            const double* Ru = R2 + (size_t)k * LM + i;
+#pragma unroll
+            for (int m = KW; m < W; ++m) du0 += wy[m] * Ru[(size_t)(m - KW) * L];
+        }
         double win[W];
 #pragma unroll
         for (int m = 0; m < W; ++m) {
@@ -403,7 +412,6 @@ __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restric
             double acc = 0.0;
 #pragma unroll
             for (int m = 0; m < W; ++m) acc += w[m] * win[m];
-            if (c == 0 && j == 0) du0 = acc;
             if (!(c == 1 && j == 0)) Q[qidx(g, c, i, j + 1, k)] = acc + qq[0];
 #pragma unroll
             for (int m = 0; m < W - 1; ++m) win[m] = win[m + 1];
@@ -857,7 +865,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     // 3. y solves + Q update + BC_WALL_Q
     if (use_st) {
         const int nch = (h->M + YJC - 1) / YJC;
-        const dim3 gy(h->N * ((h->L + 63) / 64)), by(64 * nch);
+        const dim3 gy(h->N * ((h->L + 63) / 64), 3), by(64 * nch);
         const double* wy = cn->dev + cn->oy;
         const size_t smw = sizeof(double) * (size_t)h->M * (2 * cn->KW + 1);
         h->timer.begin("y_solve", s, &ev);
