@@ -124,6 +124,8 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     CHS_CHECK(fft_len_ok(d->ncl1) && fft_len_ok(d->ncl3), CHS_E_ARG,
               "chs_create: NCL1 and NCL3 must be even, >= 4, with prime factors 2, 3, 5 only (FFTFAX)");
     CHS_CHECK(d->ncl2 >= 4, CHS_E_ARG, "chs_create: NCL2 must be >= 4");
+    CHS_CHECK(3.0 * d->ncl1 * (d->ncl2 + 2.0) * d->ncl3 < 2147483647.0, CHS_E_ARG,
+              "chs_create: grid too large for 32-bit element offsets (3*NCL1*(NCL2+2)*NCL3 >= 2^31)");
     int ms, p, lpb;
     CHS_CHECK(tdma_seg_config(d->ncl2, false, &ms, &p, &lpb, 768), CHS_E_ARG, "chs_create: unsupported NCL2");
     const double* arrs[] = {d->dyfi, d->dyci, d->wff, d->wfb, d->amph, d->acph, d->apph,

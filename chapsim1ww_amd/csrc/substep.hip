@@ -30,14 +30,16 @@ struct Geo {
     const double *dyfi, *dyci, *wff, *wfb, *amvr, *acvr, *apvr;
 };
 
-__device__ __forceinline__ size_t qidx(const Geo& g, int c, int i, int jg, int k) {
-    return (size_t)i + (size_t)g.L * ((size_t)jg + (size_t)(g.M + 2) * ((size_t)k + (size_t)g.N * c));
+// 32-bit offsets: chs_create rejects grids whose 3-component arrays reach 2^31
+// elements, so every index below fits an int (cheaper address arithmetic).
+__device__ __forceinline__ int qidx(const Geo& g, int c, int i, int jg, int k) {
+    return i + g.L * (jg + (g.M + 2) * (k + g.N * c));
 }
-__device__ __forceinline__ size_t gidx(const Geo& g, int i, int jg, int k) {
-    return (size_t)i + (size_t)g.L * ((size_t)jg + (size_t)(g.M + 2) * k);
+__device__ __forceinline__ int gidx(const Geo& g, int i, int jg, int k) {
+    return i + g.L * (jg + (g.M + 2) * k);
 }
-__device__ __forceinline__ size_t cidx(const Geo& g, int i, int j, int k) {
-    return (size_t)i + (size_t)g.L * ((size_t)j + (size_t)g.M * k);
+__device__ __forceinline__ int cidx(const Geo& g, int i, int j, int k) {
+    return i + g.L * (j + g.M * k);
 }
 
 constexpr int KR = 4;        // k rows per k_rhs_x block (segmented x solve)
