@@ -183,3 +183,34 @@ extern "C" int chs_tdma_batched(const double* a_dev, const double* b_dev, const 
     return chs::tdma_launch(a_dev, b_dev, c_dev, r_dev, cyclic ? nullptr : bc_dev, n, nlines, elem_stride,
                             line_stride, cyclic, (hipStream_t)stream);
 }
+
+// Host-array form for a drop-in of the reference subroutines called on host
+// (Fortran) arrays: copies the lines to the device, solves, copies r back.
+// Element (l, j) at l*line_stride + j*elem_stride within each host array of
+// span = (nlines-1)*line_stride + (n-1)*elem_stride + 1 doubles.
+extern "C" int chs_tdma_batched_host(const double* a, const double* b, const double* c, double* r,
+                                     const double* bc, int n, int nlines, long long elem_stride,
+                                     long long line_stride, int cyclic) {
+    CHS_CHECK(a && b && c && r, CHS_E_ARG, "chs_tdma_batched_host: null array");
+    CHS_CHECK(n >= 3 && nlines >= 0 && elem_stride > 0 && line_stride > 0, CHS_E_ARG,
+              "chs_tdma_batched_host: need n >= 3 and positive strides");
+    if (nlines == 0) return CHS_OK;
+    const size_t span = (size_t)(nlines - 1) * line_stride + (size_t)(n - 1) * elem_stride + 1;
+    double* d = nullptr;
+    const size_t nb = bc && !cyclic ? 2 * (size_t)nlines : 0;
+    CHS_HIP(hipMalloc((void**)&d, sizeof(double) * (4 * span + nb)));
+    int st = CHS_OK;
+    const double* src[4] = {a, b, c, r};
+    for (int q = 0; q < 4 && st == CHS_OK; ++q)
+        if (hipMemcpy(d + q * span, src[q], span * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) st = CHS_E_HIP;
+    if (st == CHS_OK && nb && hipMemcpy(d + 4 * span, bc, nb * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        st = CHS_E_HIP;
+    if (st == CHS_OK)
+        st = chs::tdma_launch(d, d + span, d + 2 * span, d + 3 * span, nb ? d + 4 * span : nullptr, n, nlines,
+                              elem_stride, line_stride, cyclic, nullptr);
+    if (st == CHS_OK && hipMemcpy(r, d + 3 * span, span * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        st = CHS_E_HIP;
+    hipFree(d);
+    if (st == CHS_E_HIP) chs::set_error("chs_tdma_batched_host: HIP copy or launch failed");
+    return st;
+}

@@ -54,6 +54,7 @@ _SIGS = {
     "chs_last_dpgrns": ([C.c_void_p, _dp], C.c_int),
     "chs_tdma_batched": ([C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.c_void_p],
                          C.c_int),
+    "chs_tdma_batched_host": ([_dp] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int], C.c_int),
     "chs_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "chs_profile_read": ([C.c_void_p, C.c_int, C.c_char_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_int)],
                          C.c_int),

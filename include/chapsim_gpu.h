@@ -121,6 +121,13 @@ int chs_tdma_batched(const double *a_dev, const double *b_dev, const double *c_d
                      const double *bc_dev, int n, int nlines, long long elem_stride,
                      long long line_stride, int cyclic, void *stream);
 
+/* The same on host arrays (what the Fortran subroutines receive): copies the
+ * arrays to the device, solves, copies r back; synchronous.  For a per-call
+ * drop-in of TDMAIJ?_(non)CYC on host data; PCIe-bound. */
+int chs_tdma_batched_host(const double *a, const double *b, const double *c, double *r,
+                          const double *bc, int n, int nlines, long long elem_stride,
+                          long long line_stride, int cyclic);
+
 /* Per-kernel timing with HIP events on the handle's stream. */
 int chs_profile_enable(chs_handle *h, int on);
 /* Fills up to cap entries; returns the number of kernels recorded in *count.
