@@ -112,7 +112,8 @@ def gpu_bench(steps, warmup):
         _sync()
         pois_ms = p0.elapsed_time(p1) / npois
         res[prec] = dict(ms_per_substep=ms, wall_s=wall, maxdiv=div, dt=dt, prof=prof,
-                         poisson_ms=pois_ms, poisson_refine=s.poisson_stats(), substep_refine=iters)
+                         poisson_ms=pois_ms, poisson_refine=s.poisson_stats(), substep_refine=iters,
+                         cn={ns: s.cn_info(ns, dt) for ns in (1, 2, 3)})
         s.close()
         del f, dph
 
@@ -239,6 +240,8 @@ def main():
         "grid_point_substeps_per_s": round(sub_s * ncell, 1),
         "poisson_solves_per_s": round(1e3 / f64["poisson_ms"], 2),
         "max_div_after_timed_steps": f64["maxdiv"],
+        "dt": f64["dt"],
+        "cn_solver": f64["cn"],
         "mixed": {"substeps_per_s": round(1e3 / mixed["ms_per_substep"], 3),
                   "ms_per_substep": round(mixed["ms_per_substep"], 4),
                   "speedup_vs_fp64": round(f64["ms_per_substep"] / mixed["ms_per_substep"], 3),
