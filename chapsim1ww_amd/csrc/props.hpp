@@ -1,0 +1,124 @@
+// Thermophysical-property state shared by the table lookup (props.hip) and
+// the learned surrogate (mlp.hip).
+#pragma once
+
+#include "chs_common.hpp"
+
+namespace chs {
+struct Mlp;
+}
+
+// The property handle of the C ABI (declared opaque in chapsim_gpu.h).
+struct chs_props {
+    int n = 0;                 // knots of the scaled table
+    int nbh = 0, nbt = 0;      // buckets of the H and T interval searches
+    double h_lo = 0, h_hi = 0, t_lo = 0, t_hi = 0;
+    double hbi = 0, tbi = 0;   // 1 / bucket width
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // device table: knots, bucket starts, per-interval coefficient records
+    double* xh = nullptr;      // H knots [n]
+    double* xt = nullptr;      // T knots [n]
+    int* sh = nullptr;         // H bucket -> first candidate interval [nbh]
+    int* st = nullptr;         // T bucket -> first candidate interval [nbt]
+    double* rh = nullptr;      // [n][4]: T, b, c, d of the H -> T spline
+    double* rt = nullptr;      // [n][16]: (y, b, c, d) of T -> D, M, K, CP
+    unsigned long long* counters = nullptr;   // [4] device counters
+    // host copies used by the surrogate's certification
+    std::vector<double> hH, hT, hrh, hrt;
+    // learned surrogate (mlp.hip)
+    chs::Mlp* mlp = nullptr;
+};
+
+namespace chs {
+
+// Rows of cells: row r holds rowlen consecutive values starting at r*stride
+// (the interior j-block of one k plane of a ghost-aware array, or one flat run).
+struct Rows {
+    long long nrow, rowlen, stride;
+};
+
+// LDS bytes of the staged search tables of one lookup block
+inline size_t props_lds_bytes(const chs_props* p) {
+    return sizeof(double) * 2 * (size_t)p->n + sizeof(int) * ((size_t)p->nbh + p->nbt);
+}
+
+// Interval i (0-based) with x[i] <= u < x[i+1], for x[0] < u < x[n-1]: the
+// result of the reference's bisection (NIST_SLEVAL_*, src/Flow_State.f90:
+// 1050-1061).  A bucket gives the first candidate; short walks make it exact
+// whatever the rounding of the bucket index.
+__device__ __forceinline__ int find_interval(const double* x, const int* start, int n, int nb, double lo,
+                                             double bi, double u) {
+    int b = (int)((u - lo) * bi);
+    b = b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+    int i = start[b];
+    while (i < n - 2 && x[i + 1] <= u) ++i;
+    while (i > 0 && x[i] > u) --i;
+    return i;
+}
+
+// y + dx*(b + dx*(c + dx*d)) evaluated as written, without contraction into
+// fused multiply-adds (the same roundings as the reference's FP64 code).
+__device__ __forceinline__ double cubic_nc(double y, double b, double c, double d, double dx) {
+#pragma clang fp contract(off)
+    return y + dx * (b + dx * (c + dx * d));
+}
+
+// device view of the table
+struct PropTab {
+    int n, nbh, nbt;
+    double h_lo, h_hi, t_lo, t_hi, hbi, tbi;
+    const double *xh, *xt, *rh, *rt;
+    const int *sh, *st;
+};
+
+inline PropTab tab_of(const chs_props* p) {
+    PropTab t;
+    t.n = p->n;
+    t.nbh = p->nbh;
+    t.nbt = p->nbt;
+    t.h_lo = p->h_lo;
+    t.h_hi = p->h_hi;
+    t.t_lo = p->t_lo;
+    t.t_hi = p->t_hi;
+    t.hbi = p->hbi;
+    t.tbi = p->tbi;
+    t.xh = p->xh;
+    t.xt = p->xt;
+    t.rh = p->rh;
+    t.rt = p->rt;
+    t.sh = p->sh;
+    t.st = p->st;
+    return t;
+}
+
+// Table lookup of one cell; returns false (outputs untouched) when h or T
+// leaves the table (the reference STOPs there).
+__device__ __forceinline__ bool lookup_cell(const PropTab& t, const double* xh, const double* xt, const int* sh,
+                                            const int* st, double hv, double& T, double& D, double& Mu, double& K,
+                                            double& CP) {
+    // the reference's range tests; a NaN passes them and its lookups give NaN
+    if (hv != hv) {
+        T = D = Mu = K = CP = hv;
+        return true;
+    }
+    if (hv <= t.h_lo || hv >= t.h_hi) return false;
+    const int i = find_interval(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hv);
+    const double4 r = reinterpret_cast<const double4*>(t.rh)[i];
+    const double tv = cubic_nc(r.x, r.y, r.z, r.w, hv - xh[i]);
+    if (tv <= t.t_lo || tv >= t.t_hi) return false;
+    const int k = find_interval(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tv);
+    const double4* q = reinterpret_cast<const double4*>(t.rt) + 4 * (size_t)k;
+    const double4 cd = q[0], cm = q[1], ck = q[2], cc = q[3];
+    const double dt = tv - xt[k];
+    T = tv;
+    D = cubic_nc(cd.x, cd.y, cd.z, cd.w, dt);
+    Mu = cubic_nc(cm.x, cm.y, cm.z, cm.w, dt);
+    K = cubic_nc(ck.x, ck.y, ck.z, ck.w, dt);
+    CP = cubic_nc(cc.x, cc.y, cc.z, cc.w, dt);
+    // NIST_SLEVAL_TD/TM/TK (src/Flow_State.f90:1415,1470): T == 1 -> 1
+    if (fabs(tv - 1.0) < 1.0e-10) D = Mu = K = 1.0;
+    return true;
+}
+
+}  // namespace chs

@@ -47,132 +47,204 @@ constexpr int KRS = 4;       // k rows per k_rhs_x block (stencil x solve)
 constexpr int XLPB = 16;     // line slots of the x solve (3*KR = 12 used)
 constexpr int XMS = 8;       // rows per segment of the x solve
 
-// Convection H_c at (i, j, k) (CONVECTION_X/Y/Z_tg, channel: RCCI1 = RNDI1 = RCCI2 = 1;
-// XND2CL = YND2CL = ZND2CL = 1/2).  Q is read through ghost-aware rows.
-__device__ __forceinline__ double conv_x(const Geo& g, const double* __restrict__ Q, int i, int j, int k) {
-    const int ip = i == g.L - 1 ? 0 : i + 1, im = i == 0 ? g.L - 1 : i - 1;
-    const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
-    const int J = j + 1;
-    const double COE1 = 0.5 * 0.5 * g.dxi;
-    const double COE2 = g.dyfi[j] * 0.5 * 1.0;
-    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
-    const double u_c = Q[qidx(g, 0, i, J, k)], u_p = Q[qidx(g, 0, ip, J, k)], u_m = Q[qidx(g, 0, im, J, k)];
-    const double H11 = ((u_p + u_c) * (u_p + u_c) - (u_m + u_c) * (u_m + u_c)) * COE1;
-    const double H12 = ((Q[qidx(g, 1, i, J + 1, k)] + Q[qidx(g, 1, im, J + 1, k)]) *
-                            (g.wff[j + 1] * Q[qidx(g, 0, i, J + 1, k)] + g.wfb[j + 1] * u_c) -
-                        (Q[qidx(g, 1, i, J, k)] + Q[qidx(g, 1, im, J, k)]) *
-                            (g.wff[j] * u_c + g.wfb[j] * Q[qidx(g, 0, i, J - 1, k)])) * COE2;
-    const double H13 = ((Q[qidx(g, 2, i, J, kp)] + Q[qidx(g, 2, im, J, kp)]) * (Q[qidx(g, 0, i, J, kp)] + u_c) -
-                        (Q[qidx(g, 2, i, J, k)] + Q[qidx(g, 2, im, J, k)]) * (u_c + Q[qidx(g, 0, i, J, km)])) * COE3;
-    return -(H11 + H12 + H13);
-}
-
-__device__ __forceinline__ double conv_y(const Geo& g, const double* __restrict__ Q, int i, int j, int k) {
-    const int ip = i == g.L - 1 ? 0 : i + 1, im = i == 0 ? g.L - 1 : i - 1;
-    const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
-    const int J = j + 1;
-    const double COE1 = g.dxi * 0.5;
-    const double COE2 = g.dyci[j] * 0.5 * 0.5;
-    const double COE31 = g.dzi * 0.5 * 1.0;
-    const double wf = g.wff[j], wb = g.wfb[j];
-    const double v_c = Q[qidx(g, 1, i, J, k)];
-    const double H21 = ((wf * Q[qidx(g, 0, ip, J, k)] + wb * Q[qidx(g, 0, ip, J - 1, k)]) * (Q[qidx(g, 1, ip, J, k)] + v_c) -
-                        (wf * Q[qidx(g, 0, i, J, k)] + wb * Q[qidx(g, 0, i, J - 1, k)]) * (v_c + Q[qidx(g, 1, im, J, k)])) * COE1;
-    const double v_p = Q[qidx(g, 1, i, J + 1, k)], v_m = Q[qidx(g, 1, i, J - 1, k)];
-    const double H22 = ((v_p * 1.0 + v_c * 1.0) * (v_p + v_c) - (v_c * 1.0 + v_m * 1.0) * (v_c + v_m)) * COE2;
-    const double H23 = ((wf * Q[qidx(g, 2, i, J, kp)] * 1.0 + wb * Q[qidx(g, 2, i, J - 1, kp)] * 1.0) * (Q[qidx(g, 1, i, J, kp)] + v_c) -
-                        (wf * Q[qidx(g, 2, i, J, k)] * 1.0 + wb * Q[qidx(g, 2, i, J - 1, k)] * 1.0) * (v_c + Q[qidx(g, 1, i, J, km)])) * COE31;
-    return -(H21 + H22 + H23);
-}
-
-__device__ __forceinline__ double conv_z(const Geo& g, const double* __restrict__ Q, int i, int j, int k) {
-    const int ip = i == g.L - 1 ? 0 : i + 1, im = i == 0 ? g.L - 1 : i - 1;
-    const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
-    const int J = j + 1;
-    const double COE1 = 0.5 * 0.5 * g.dxi;
-    const double COE2 = g.dyfi[j] * 0.5;
-    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
-    const double w_c = Q[qidx(g, 2, i, J, k)];
-    const double H31 = ((Q[qidx(g, 0, ip, J, k)] + Q[qidx(g, 0, ip, J, km)]) * (Q[qidx(g, 2, ip, J, k)] + w_c) -
-                        (Q[qidx(g, 0, i, J, k)] + Q[qidx(g, 0, i, J, km)]) * (w_c + Q[qidx(g, 2, im, J, k)])) * COE1;
-    const double H32 = ((Q[qidx(g, 1, i, J + 1, k)] + Q[qidx(g, 1, i, J + 1, km)]) *
-                            (g.wff[j + 1] * Q[qidx(g, 2, i, J + 1, k)] * 1.0 + g.wfb[j + 1] * w_c * 1.0) -
-                        (Q[qidx(g, 1, i, J, k)] + Q[qidx(g, 1, i, J, km)]) *
-                            (g.wff[j] * w_c * 1.0 + g.wfb[j] * Q[qidx(g, 2, i, J - 1, k)] * 1.0)) * COE2;
-    const double w_p = Q[qidx(g, 2, i, J, kp)], w_m = Q[qidx(g, 2, i, J, km)];
-    const double H33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * COE3;
-    return -(H31 + H32 + H33);
-}
-
 struct RkCoef {
     double tgam, troh, talp, dt;
 };
 
+// Neighbour offsets of a cell (i, J, k) of a (L, 0:M+1, N) array, shared by
+// every component of Q and by PR: i+1, i-1, k+1, k-1 (both periodic) and J+1.
+struct Nb {
+    int ip, im, kp, km, L;
+};
+
+// y coefficients of the block's row j (uniform over the block)
+struct YCoef {
+    double dyfi, dyci, wf0, wb0, wf1, wb1;
+};
+
+// Convection H_c (CONVECTION_X/Y/Z_tg: src/CONVECTION_X.f90:23-74,
+// src/CONVECTION_Y.f90:22-127, src/CONVECTION_Z.f90:22-112; channel: RCCI1 =
+// RNDI1 = RCCI2 = 1, XND2CL = YND2CL = ZND2CL = 1/2).  U, V, W point at
+// (i, J, k) of the three components of Q.
+__device__ __forceinline__ double conv_x(const Geo& g, const YCoef& y, const double* __restrict__ U,
+                                         const double* __restrict__ V, const double* __restrict__ W, const Nb& n) {
+    const double COE1 = 0.5 * 0.5 * g.dxi;
+    const double COE2 = y.dyfi * 0.5 * 1.0;
+    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
+    const double u_c = U[0], u_p = U[n.ip], u_m = U[n.im];
+    const double H11 = ((u_p + u_c) * (u_p + u_c) - (u_m + u_c) * (u_m + u_c)) * COE1;
+    const double H12 = ((V[n.L] + V[n.im + n.L]) * (y.wf1 * U[n.L] + y.wb1 * u_c) -
+                        (V[0] + V[n.im]) * (y.wf0 * u_c + y.wb0 * U[-n.L])) * COE2;
+    const double H13 = ((W[n.kp] + W[n.im + n.kp]) * (U[n.kp] + u_c) -
+                        (W[0] + W[n.im]) * (u_c + U[n.km])) * COE3;
+    return -(H11 + H12 + H13);
+}
+
+__device__ __forceinline__ double conv_y(const Geo& g, const YCoef& y, const double* __restrict__ U,
+                                         const double* __restrict__ V, const double* __restrict__ W, const Nb& n) {
+    const double COE1 = g.dxi * 0.5;
+    const double COE2 = y.dyci * 0.5 * 0.5;
+    const double COE31 = g.dzi * 0.5 * 1.0;
+    const double wf = y.wf0, wb = y.wb0;
+    const double v_c = V[0];
+    const double H21 = ((wf * U[n.ip] + wb * U[n.ip - n.L]) * (V[n.ip] + v_c) -
+                        (wf * U[0] + wb * U[-n.L]) * (v_c + V[n.im])) * COE1;
+    const double v_p = V[n.L], v_m = V[-n.L];
+    const double H22 = ((v_p * 1.0 + v_c * 1.0) * (v_p + v_c) - (v_c * 1.0 + v_m * 1.0) * (v_c + v_m)) * COE2;
+    const double H23 = ((wf * W[n.kp] * 1.0 + wb * W[n.kp - n.L] * 1.0) * (V[n.kp] + v_c) -
+                        (wf * W[0] * 1.0 + wb * W[-n.L] * 1.0) * (v_c + V[n.km])) * COE31;
+    return -(H21 + H22 + H23);
+}
+
+__device__ __forceinline__ double conv_z(const Geo& g, const YCoef& y, const double* __restrict__ U,
+                                         const double* __restrict__ V, const double* __restrict__ W, const Nb& n) {
+    const double COE1 = 0.5 * 0.5 * g.dxi;
+    const double COE2 = y.dyfi * 0.5;
+    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
+    const double w_c = W[0];
+    const double H31 = ((U[n.ip] + U[n.ip + n.km]) * (W[n.ip] + w_c) -
+                        (U[0] + U[n.km]) * (w_c + W[n.im])) * COE1;
+    const double H32 = ((V[n.L] + V[n.L + n.km]) * (y.wf1 * W[n.L] * 1.0 + y.wb1 * w_c * 1.0) -
+                        (V[0] + V[n.km]) * (y.wf0 * w_c * 1.0 + y.wb0 * W[-n.L] * 1.0)) * COE2;
+    const double w_p = W[n.kp], w_m = W[n.km];
+    const double H33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * COE3;
+    return -(H31 + H32 + H33);
+}
+
+// Position of padded element p of an LDS row.  KH == 0: plain.  KH > 0 (rows
+// of L + 2*KH entries, KH even): element p sits in sub-row p % 4 at p / 4, so
+// the lanes of a wave reading elements 4*lane + t (t fixed) hit consecutive
+// words.
+template <int KH>
+__device__ __forceinline__ int row_pos(int p, int Q4) {
+    return KH > 0 ? (p & 3) * Q4 + (p >> 2) : p;
+}
+
+// Phase 1 of the fused RHS (RHS_CvLpGpS_tg, src/RHS_CvLpGpS_tg.f90:117-176)
+// for the kr rows (j, k0 .. k0+kr-1) of u, v, w: CONVH0 <- H, and the x-solve
+// right-hand side into LDS row c*kr + r (stride LS) at padded position KH + i;
+// with KH > 0 the KH entries either side are the periodic halo.  Returns this
+// thread's part of the mass-flux integrand sum RHS_u / DYFI.
+template <int KH>
+__device__ __forceinline__ double rhs_rows(const Geo& g, const RkCoef& rk, double cvisc,
+                                           const double* __restrict__ Q, const double* __restrict__ PR,
+                                           double* __restrict__ H0, double* sR, int LS, int kr, int lgL, int j,
+                                           int k0) {
+    const int L = g.L, M = g.M, N = g.N;
+    const int LP = L * (M + 2);  // one k plane of a ghost-aware array
+    const int CS = LP * N;       // component stride of Q
+    const int LMN = L * M * N;
+    const int J = j + 1;
+    const int Q4 = LS >> 2;
+    YCoef y;
+    y.dyfi = g.dyfi[j];
+    y.dyci = g.dyci[j];
+    y.wf0 = g.wff[j];
+    y.wb0 = g.wfb[j];
+    y.wf1 = g.wff[j + 1];
+    y.wb1 = g.wfb[j + 1];
+    double ap[3], ac[3], am[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ap[c] = g.apvr[c * (M + 1) + j];
+        ac[c] = g.acvr[c * (M + 1) + j];
+        am[c] = g.amvr[c * (M + 1) + j];
+    }
+    const double COE1 = rk.talp * cvisc;
+    const double COE2 = rk.talp * rk.dt;
+    double msum = 0.0;
+    for (int e = threadIdx.x; e < (kr << lgL); e += blockDim.x) {
+        const int r = e >> lgL, i = e & (L - 1);
+        const int k = k0 + r;
+        Nb n;
+        n.ip = i == L - 1 ? 1 - L : 1;
+        n.im = i == 0 ? L - 1 : -1;
+        n.kp = k == N - 1 ? (1 - N) * LP : LP;
+        n.km = k == 0 ? (N - 1) * LP : -LP;
+        n.L = L;
+        const int o = i + L * J + LP * k;
+        const double* U = Q + o;
+        const double* V = U + CS;
+        const double* W = V + CS;
+        const double* P = PR + o;
+        const int ci = i + L * j + L * M * k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c == 1 && j == 0) continue;  // v on the wall face is not advanced (NII = 2)
+            const double* X = c == 0 ? U : c == 1 ? V : W;
+            const double conv = c == 0 ? conv_x(g, y, U, V, W, n) : c == 1 ? conv_y(g, y, U, V, W, n)
+                                                                            : conv_z(g, y, U, V, W, n);
+            const int hc = ci + c * LMN;
+            const double RHSC = rk.tgam * conv + rk.troh * H0[hc];
+            const double qc = X[0];
+            const double RHSL = COE1 * ((X[n.ip] - 2.0 * qc + X[n.im]) * g.dxqi +
+                                        (X[n.kp] - 2.0 * qc + X[n.km]) * g.dzqi * 1.0 +
+                                        (X[n.L] * ap[c] + qc * ac[c] + X[-n.L] * am[c]));
+            H0[hc] = conv;
+            double rhs = (RHSC + RHSL) * rk.dt;
+            double pgm;
+            if (c == 0)
+                pgm = (P[0] - P[n.im]) * g.dxi * COE2;
+            else if (c == 1)
+                pgm = (P[0] - P[-n.L]) * (y.dyci / 1.0) * COE2;
+            else
+                pgm = (P[0] - P[n.km]) * g.dzi * COE2;
+            rhs = rhs - pgm;
+            if (c == 0) msum += rhs / y.dyfi / 1.0;
+            double* row = sR + (c * kr + r) * LS;
+            row[row_pos<KH>(KH + i, Q4)] = rhs;
+            if (KH > 0) {
+                if (i < KH) row[row_pos<KH>(KH + i + L, Q4)] = rhs;
+                if (i >= L - KH) row[row_pos<KH>(KH + i - L, Q4)] = rhs;
+            }
+        }
+    }
+    return msum;
+}
+
+// Block sum in a fixed order: a shuffle tree per wave, then the wave sums in
+// wave order by thread 0 (the result is valid there).  blockDim % 64 == 0.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+    return s;
+}
+
 // ---------------------------------------------------------------------------
 // k_rhs_x: block = (j, KR consecutive k).  Phase 1 computes RHS_c for the 3
 // components of its rows into LDS and writes CONVH0; phase 2 solves the x
-// lines (cyclic, constant coefficients) from LDS and writes RHS.
+// lines (cyclic, constant coefficients) from LDS by the segmented Thomas
+// method and writes RHS.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
                                                const double* __restrict__ PR, double* __restrict__ H0,
                                                double* __restrict__ RHS, double* __restrict__ part, double xa,
-                                               double xb, double xc, int Kx, const double* __restrict__ wx, int kr) {
+                                               double xb, double xc, int lgL) {
     extern __shared__ unsigned char smem[];
     const int L = g.L;
     const int LS = L + 1;  // padded LDS row stride (bank spread for the segment reads)
-    // LDS: sR (the 3*kr RHS rows) | red (block sum); the reduced systems of
+    // LDS: sR (the 3*KR RHS rows) | red (block sum); the reduced systems of
     // the x solves reuse the front once sR has been read into registers
     double* sR = reinterpret_cast<double*>(smem);
-    unsigned char* sred = smem + sizeof(double) * 3 * kr * LS;
-    const int nkb = g.N / kr;
+    const int nkb = g.N / KR;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const bool active = t < g.M * nkb;
     // k-group-major: consecutive tiles (same XCD) walk j, so the j-1/j+1 rows
     // a tile reads were loaded by its neighbours moments before (L2 hits)
     const int kg = active ? t / g.M : 0;
     const int j = active ? t - kg * g.M : 0;
-    const int k0 = kg * kr;
-    const size_t LMN = (size_t)L * g.M * g.N;
-    const int J = j + 1;
-    const double COE1 = rk.talp * cvisc;
-    const double COE2 = rk.talp * rk.dt;
+    const int k0 = kg * KR;
+    const int LMN = L * g.M * g.N;
     double msum = 0.0;
-    if (active) {
-        for (int e = threadIdx.x; e < kr * L; e += blockDim.x) {
-            const int r = e / L, i = e - r * L;
-            const int k = k0 + r;
-            const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
-            const int kp = k == g.N - 1 ? 0 : k + 1, km = k == 0 ? g.N - 1 : k - 1;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if (c == 1 && j == 0) continue;  // v on the wall face is not advanced (NII = 2)
-                const double conv = c == 0 ? conv_x(g, Q, i, j, k) : c == 1 ? conv_y(g, Q, i, j, k) : conv_z(g, Q, i, j, k);
-                const size_t ci = cidx(g, i, j, k) + c * LMN;
-                const double RHSC = rk.tgam * conv + rk.troh * H0[ci];
-                const double qc = Q[qidx(g, c, i, J, k)];
-                const double RHSL = COE1 * ((Q[qidx(g, c, ip, J, k)] - 2.0 * qc + Q[qidx(g, c, im, J, k)]) * g.dxqi +
-                                            (Q[qidx(g, c, i, J, kp)] - 2.0 * qc + Q[qidx(g, c, i, J, km)]) * g.dzqi * 1.0 +
-                                            (Q[qidx(g, c, i, J + 1, k)] * g.apvr[c * (g.M + 1) + j] +
-                                             qc * g.acvr[c * (g.M + 1) + j] +
-                                             Q[qidx(g, c, i, J - 1, k)] * g.amvr[c * (g.M + 1) + j]));
-                H0[ci] = conv;
-                double rhs = (RHSC + RHSL) * rk.dt;
-                double pgm;
-                if (c == 0)
-                    pgm = (PR[gidx(g, i, J, k)] - PR[gidx(g, im, J, k)]) * g.dxi * COE2;
-                else if (c == 1)
-                    pgm = (PR[gidx(g, i, J, k)] - PR[gidx(g, i, J - 1, k)]) * (g.dyci[j] / 1.0) * COE2;
-                else
-                    pgm = (PR[gidx(g, i, J, k)] - PR[gidx(g, i, J, km)]) * g.dzi * COE2;
-                rhs = rhs - pgm;
-                if (c == 0) msum += rhs / g.dyfi[j] / 1.0;
-                sR[(c * kr + r) * LS + i] = rhs;
-            }
-        }
-    }
+    if (active) msum = rhs_rows<0>(g, rk, cvisc, Q, PR, H0, sR, LS, KR, lgL, j, k0);
     // block sum of the mass-flux integrand (deterministic tree)
     {
-        double* red = reinterpret_cast<double*>(sred);
+        double* red = reinterpret_cast<double*>(smem + sizeof(double) * 3 * KR * LS);
         red[threadIdx.x] = msum;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -182,43 +254,82 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
         if (threadIdx.x == 0 && active) part[t] = red[0];
         __syncthreads();
     }
-    // phase 2a: x solve as the truncated inverse of the constant cyclic system,
-    // a (2Kx+1)-point periodic stencil over the LDS rows (Kx >= 0)
-    if (Kx >= 0) {
-        for (int e = threadIdx.x; e < 3 * kr * L; e += blockDim.x) {
-            const int cr = e / L, i = e - cr * L;
-            const int c = cr / kr, r = cr - c * kr;
-            if (!active || (c == 1 && j == 0)) continue;
-            const double* row = sR + cr * LS;
-            double acc = 0.0;
-            for (int m = -Kx; m <= Kx; ++m) {
-                int ii = i + m;
-                ii = ii < 0 ? ii + L : (ii >= L ? ii - L : ii);
-                acc += wx[m + Kx] * row[ii];
-            }
-            RHS[cidx(g, i, j, k0 + r) + c * LMN] = acc;
-        }
-        return;
-    }
-    // phase 2b: x solves of the 3*kr rows, XMS rows per segment, L/XMS segments
+    // x solves of the 3*KR rows, XMS rows per segment, L/XMS segments
     const int P = L / XMS;
     const int ll = threadIdx.x % XLPB, seg = threadIdx.x / XLPB;
-    const int c = ll / kr, r = ll - c * kr;
-    const bool valid = active && ll < 3 * kr && !(c == 1 && j == 0);
+    const int c = ll / KR, r = ll - c * KR;
+    const bool valid = active && ll < 3 * KR && !(c == 1 && j == 0);
     double a[XMS], b[XMS], cc[XMS], d[XMS];
 #pragma unroll
     for (int l = 0; l < XMS; ++l) {
         a[l] = xa;
         b[l] = xb;
         cc[l] = xc;
-        d[l] = valid ? sR[(c * kr + r) * LS + seg * XMS + l] : 0.0;
+        d[l] = valid ? sR[(c * KR + r) * LS + seg * XMS + l] : 0.0;
     }
     __syncthreads();  // sR fully read: its space now holds the reduced systems
     seg_solve<XMS, double>(a, b, cc, d, XLPB, P, ll, seg, valid, true, smem);
     if (valid) {
-        const size_t base = cidx(g, 0, j, k0 + r) + c * LMN;
+        const int base = c * LMN + L * (j + g.M * (k0 + r));
 #pragma unroll
         for (int l = 0; l < XMS; ++l) RHS[base + seg * XMS + l] = d[l];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs_xw: k_rhs_x with the x solve of MOMFA1_X_tg as the truncated inverse
+// of its constant cyclic system: a periodic (2KX+1)-point stencil (weights
+// zero-padded to the window).  Rows sit in LDS with KX halo entries each side
+// (row_pos layout); each thread forms 4 consecutive outputs from one register
+// window of 2KX+4 entries.  Block = 256 threads, rows (j, KRS consecutive k).
+// ---------------------------------------------------------------------------
+template <int KX>
+__global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
+                                               const double* __restrict__ PR, double* __restrict__ H0,
+                                               double* __restrict__ RHS, double* __restrict__ part,
+                                               const double* __restrict__ wx, int lgL) {
+    extern __shared__ double sw[];
+    constexpr int W = 2 * KX + 1;
+    const int L = g.L;
+    const int LS = L + 2 * KX;
+    const int Q4 = LS >> 2;
+    double* red = sw + 3 * KRS * LS;
+    const int nkb = g.N / KRS;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const bool active = t < g.M * nkb;
+    const int kg = active ? t / g.M : 0;
+    const int j = active ? t - kg * g.M : 0;
+    const int k0 = kg * KRS;
+    double msum = 0.0;
+    if (active) msum = rhs_rows<KX>(g, rk, cvisc, Q, PR, H0, sw, LS, KRS, lgL, j, k0);
+    const double bs = block_sum(msum, red);  // its barrier also publishes the LDS rows
+    if (threadIdx.x == 0 && active) part[t] = bs;
+    if (!active) return;
+    double w[W];
+#pragma unroll
+    for (int m = 0; m < W; ++m) w[m] = wx[m];
+    const int lg4 = lgL - 2;
+    const int LMN = L * g.M * g.N;
+    for (int e = threadIdx.x; e < ((3 * KRS) << lg4); e += blockDim.x) {
+        const int cr = e >> lg4, g4 = e & ((1 << lg4) - 1);
+        const int c = cr / KRS, r = cr - c * KRS;
+        if (c == 1 && j == 0) continue;
+        // padded elements 4*g4 + t, t = 0 .. W+2: sub-row t % 4, entry g4 + t / 4
+        const double* row = sw + cr * LS + g4;
+        double x[W + 3];
+#pragma unroll
+        for (int tt = 0; tt < W + 3; ++tt) x[tt] = row[(tt & 3) * Q4 + (tt >> 2)];
+        double o[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < W; ++m) acc += w[m] * x[m + u];
+            o[u] = acc;
+        }
+        double* dst = RHS + (c * LMN + 4 * g4 + L * (j + g.M * (k0 + r)));
+        reinterpret_cast<double2*>(dst)[0] = make_double2(o[0], o[1]);
+        reinterpret_cast<double2*>(dst)[1] = make_double2(o[2], o[3]);
     }
 }
 
@@ -678,9 +789,9 @@ static int cyclic_stencil(int n, double a, double b, double c, std::vector<doubl
 
 struct CnStencils {
     double facoe = -1.0;
-    int Kx = -1, Kz = -1, Ky = -1, KW = 4, KWz = 4;
+    int Kx = -1, Kz = -1, Ky = -1, KW = 4, KWz = 4, KWx = 4;
     bool ok = false;
-    double* dev = nullptr;   // wx[2Kx+1] | wz[2Kz+1] | wy[3][M][2Ky+1]
+    double* dev = nullptr;   // wx[2KWx+1] | wz[2KWz+1] | wy[3][M][2KW+1], zero-padded windows
     size_t ox = 0, oz = 0, oy = 0;
 };
 
@@ -743,18 +854,22 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
         for (int j = 0; j < M; ++j) Ky = std::max(Ky, trunc_width(inv[c * M + j], j, false));
     }
     st.Ky = Ky;
-    st.ok = st.Kx <= KMAX && st.Kz <= KMAX && st.Ky <= KMAX;
-    st.KW = Ky <= 4 ? 4 : Ky <= 8 ? 8 : Ky <= 12 ? 12 : 16;
-    st.KWz = st.Kz <= 4 ? 4 : st.Kz <= 8 ? 8 : st.Kz <= 12 ? 12 : 16;
+    auto window = [](int K) { return K <= 4 ? 4 : K <= 8 ? 8 : K <= 12 ? 12 : 16; };
+    st.KW = window(Ky);
+    st.KWz = window(st.Kz);
+    st.KWx = window(st.Kx);
+    // a periodic window must not wrap onto itself: 2*KW + 1 <= the line length
+    st.ok = st.Kx <= KMAX && st.Kz <= KMAX && st.Ky <= KMAX && 2 * st.KWx + 1 <= h->L &&
+            2 * st.KWz + 1 <= h->N;
     if (st.ok) {
         const int W = 2 * st.KW + 1;
         Ky = st.KW;
-        const size_t nwz = 2 * (size_t)st.KWz + 1;   // z weights zero-padded to the window
-        std::vector<double> buf(wx.size() + nwz + 3 * (size_t)M * W, 0.0);
+        const size_t nwx = 2 * (size_t)st.KWx + 1, nwz = 2 * (size_t)st.KWz + 1;
+        std::vector<double> buf(nwx + nwz + 3 * (size_t)M * W, 0.0);
         st.ox = 0;
-        st.oz = wx.size();
-        st.oy = wx.size() + nwz;
-        std::copy(wx.begin(), wx.end(), buf.begin());
+        st.oz = nwx;
+        st.oy = nwx + nwz;
+        std::copy(wx.begin(), wx.end(), buf.begin() + (st.KWx - st.Kx));
         std::copy(wz.begin(), wz.end(), buf.begin() + st.oz + (st.KWz - st.Kz));
         for (int c = 0; c < 3; ++c)
             for (int j = 0; j < M; ++j)
@@ -813,16 +928,28 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     {
         const double COE = -facoe * h->dxqi;
         const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
-        const int kr = use_st && h->N % KRS == 0 ? KRS : KR;  // KRS = 4 rows with 256 threads measured best
+        const int lgL = __builtin_ctz((unsigned)h->L);  // L is a power of two (substep_supported)
+        const int kr = use_st ? KRS : KR;
         const int nblk = h->M * (h->N / kr);
-        const int P = h->L / XMS;
-        const int nthr = use_st ? 256 : XLPB * P;
-        const size_t sm = use_st ? sizeof(double) * (3 * kr * (h->L + 1) + (size_t)nthr)
-                                 : std::max(sizeof(double) * (3 * kr * (h->L + 1) + (size_t)nthr),
-                                            seg_smem_bytes<double>(XLPB, P));
+        const dim3 grx(((nblk + 7) / 8) * 8);
         h->timer.begin("rhs_x", s, &ev);
-        CHS_LAUNCH(k_rhs_x, dim3(((nblk + 7) / 8) * 8), dim3(nthr), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0,
-                   h->rhs, h->partials, xa, xb, xc, use_st ? cn->Kx : -1, use_st ? cn->dev + cn->ox : nullptr, kr);
+        if (use_st) {
+            const size_t sm = sizeof(double) * (3 * KRS * (size_t)(h->L + 2 * cn->KWx) + 4);
+            const double* wx = cn->dev + cn->ox;
+            switch (cn->KWx) {
+                case 4: CHS_LAUNCH(k_rhs_xw<4>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
+                case 8: CHS_LAUNCH(k_rhs_xw<8>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
+                case 12: CHS_LAUNCH(k_rhs_xw<12>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
+                default: CHS_LAUNCH(k_rhs_xw<16>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
+            }
+        } else {
+            const int P = h->L / XMS;
+            const int nthr = XLPB * P;
+            const size_t sm = std::max(sizeof(double) * (3 * KR * (size_t)(h->L + 1) + (size_t)nthr),
+                                       seg_smem_bytes<double>(XLPB, P));
+            CHS_LAUNCH(k_rhs_x, grx, dim3(nthr), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials,
+                       xa, xb, xc, lgL);
+        }
         h->timer.end(ev, "rhs_x", s);
         h->timer.begin("massflux", s, &ev);
         CHS_LAUNCH(k_massflux, dim3(1), dim3(1024), 0, s, h->partials, nblk, h->volm, h->scalars);

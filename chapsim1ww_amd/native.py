@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libchapsim_gpu.so")
 
 CHS_OK, CHS_E_ARG, CHS_E_HIP, CHS_E_NOMEM, CHS_E_RANGE, CHS_E_NOTCONV = range(6)
 CHS_FP64, CHS_MIXED = 0, 1
+CHS_PROPS_TABLE, CHS_PROPS_SURROGATE = 0, 1
 
 _dp = C.POINTER(C.c_double)
 
@@ -55,6 +56,16 @@ _SIGS = {
     "chs_tdma_batched": ([C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.c_void_p],
                          C.c_int),
     "chs_tdma_batched_host": ([_dp] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int], C.c_int),
+    "chs_props_create": ([C.c_int] + [_dp] * 6 + [C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "chs_props_destroy": ([C.c_void_p], C.c_int),
+    "chs_therm_prop_update": ([C.c_void_p, C.c_int, C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong]
+                              + [C.c_void_p] * 5 + [C.POINTER(C.c_longlong)], C.c_int),
+    "chs_surrogate_train": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_double, C.c_ulonglong,
+                             _dp], C.c_int),
+    "chs_surrogate_certify": ([C.c_void_p, C.c_double, C.c_int, _dp], C.c_int),
+    "chs_surrogate_grad": ([C.c_void_p, C.c_void_p, C.c_longlong, C.POINTER(C.c_float), _dp], C.c_int),
+    "chs_surrogate_eval": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p], C.c_int),
+    "chs_surrogate_params": ([C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int, _dp], C.c_int),
     "chs_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "chs_profile_read": ([C.c_void_p, C.c_int, C.c_char_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_int)],
                          C.c_int),

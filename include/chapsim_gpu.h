@@ -128,6 +128,64 @@ int chs_tdma_batched_host(const double *a, const double *b, const double *c, dou
                           const double *bc, int n, int nlines, long long elem_stride,
                           long long line_stride, int cyclic);
 
+/* ---- thermophysical properties (THERM_PROP_UPDATE) ---------------------- */
+typedef struct chs_props chs_props;
+enum { CHS_PROPS_TABLE = 0, CHS_PROPS_SURROGATE = 1 };
+
+/* Property table after NIST_TABLE_SCALING (src/Flow_State.f90:343-440): n
+ * knots (4 <= n <= 4096) sorted by H, columns H, T, D, M, K, CP (host arrays;
+ * H and T strictly increasing).  Builds the cubic splines H -> T and
+ * T -> D, M, K, CP (NIST_SPLINE_H / NIST_SPLINE_T with CUBIC_SPLINE,
+ * src/Flow_State.f90:916-1014, 1693-1788) and uploads them.
+ * stream: a hipStream_t, or NULL to create one. */
+int chs_props_create(int n, const double *H, const double *T, const double *D, const double *M,
+                     const double *K, const double *CP, void *stream, chs_props **out);
+int chs_props_destroy(chs_props *p);
+
+/* THERM_PROP_UPDATE (src/THERM_PROP_UPDATE.f90:129-205) on device arrays:
+ * per cell T = S_HT(h), rho = S_TD(T), mu = S_TM(T), kappa = S_TK(T) and
+ * c_p = S_TCp(T) (cp_dev may be NULL).  Cells with |h| < 1e-10 are left
+ * unchanged, as the reference does.  The cells are nrow rows of rowlen
+ * consecutive values, row r starting at r * row_stride (for ENTHALPY(NCL1,
+ * 0:N2DO+1, NCL3): pass the address of (1,1,1), nrow = NCL3, rowlen =
+ * NCL1*N2DO, row_stride = NCL1*(N2DO+2); for a flat array nrow = 1).
+ * mode CHS_PROPS_TABLE: spline lookup.  mode CHS_PROPS_SURROGATE: the
+ * trained MLP where it is certified, the table elsewhere; *n_fallback (may
+ * be NULL) receives the number of cells the surrogate handed to the table.
+ * Cells whose h or T leaves the table are left unchanged and the call
+ * returns CHS_E_RANGE (the reference STOPs, src/Flow_State.f90:1030-1046).
+ * Synchronous: returns after the update has finished. */
+int chs_therm_prop_update(chs_props *p, int mode, const double *h_dev, long long nrow, long long rowlen,
+                          long long row_stride, double *T_dev, double *rho_dev, double *mu_dev,
+                          double *kap_dev, double *cp_dev, long long *n_fallback);
+
+/* ---- learned property surrogate ------------------------------------------
+ * The reference has no surrogate: these entries train and certify the network
+ * that chs_therm_prop_update(..., CHS_PROPS_SURROGATE, ...) runs in place of
+ * the table where certified.  Network: h -> (T, rho, mu, kappa, c_p),
+ * 1 -> 64 -> 64 -> 5, tanh, BF16 weights, FP32 accumulation; parameters are a
+ * flat float vector of 4613: w1[64] b1[64] W2[64][64] b2[64] W3[5][64] b3[5].
+ *
+ * Train with Adam (cosine learning-rate decay from lr to lr/100) on nsamp
+ * device enthalpy samples labelled by the table; batch is a multiple of 64.
+ * The trained range is [min h, max h] of the samples.  A new network must be
+ * certified before use.  *final_loss: mean squared relative error over all
+ * samples. */
+int chs_surrogate_train(chs_props *p, const double *h_dev, long long nsamp, int steps, int batch, double lr,
+                        unsigned long long seed, double *final_loss);
+/* Split the trained range into nbins bins; a bin is certified when the
+ * network is within tol/2 (relative, every output) of the table on 33 points
+ * spanning it.  Uncertified bins and cells outside the range use the table. */
+int chs_surrogate_certify(chs_props *p, double tol, int nbins, double *certified_fraction);
+/* Loss and gradient (host, 4613 floats) over exactly the n device samples
+ * (n <= 8192), with no update: the training kernels' gradient, for checking. */
+int chs_surrogate_grad(chs_props *p, const double *h_dev, long long n, float *grad_host, double *loss);
+/* Raw network outputs, no fallback: out_dev[k*n + q], k = T, rho, mu, kappa, c_p. */
+int chs_surrogate_eval(chs_props *p, const double *h_dev, long long n, double *out_dev);
+/* op 0: read the parameters (and norm[14] = hc, hs, h_lo, h_hi, m[5], s[5]);
+ * op 1: set them (count = 4613); x = (h - hc) hs, y_k = m_k + s_k o_k. */
+int chs_surrogate_params(chs_props *p, int op, float *host, int count, double *norm);
+
 /* Per-kernel timing with HIP events on the handle's stream. */
 int chs_profile_enable(chs_handle *h, int on);
 /* Fills up to cap entries; returns the number of kernels recorded in *count.

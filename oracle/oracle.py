@@ -210,6 +210,18 @@ def cubic_spline(x, y):
     return b, c, d
 
 
+def build_prop_tab(scaled):
+    """The 21n-value table of orc_therm_prop_update from scaled columns
+    (NIST_SPLINE_H/T): [H T D M K CP | HT(b c d) | TD | TM | TK | TCP]."""
+    H, T = np.asarray(scaled["H"], np.float64), np.asarray(scaled["T"], np.float64)
+    cols = [H, T] + [np.asarray(scaled[k], np.float64) for k in ("D", "M", "K", "CP")]
+    parts = list(cols)
+    parts += list(cubic_spline(H, T))
+    for k in ("D", "M", "K", "CP"):
+        parts += list(cubic_spline(T, scaled[k]))
+    return np.ascontiguousarray(np.concatenate(parts)), len(H)
+
+
 def therm_prop_update(tab, n, h, T, rho, mu, kap, cp=None):
     return lib().orc_therm_prop_update(n, _ptr(tab), h.size, _ptr(h), _ptr(T), _ptr(rho), _ptr(mu), _ptr(kap), _ptr(cp))
 
