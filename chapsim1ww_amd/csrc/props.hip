@@ -178,6 +178,22 @@ int props_create(int n, const double* H, const double* T, const double* D, const
             p->hrt[16 * i + 4 * f + 3] = d[i];
         }
     }
+    // RHOHmax as NIST_SPLINE_H finds it (src/Flow_State.f90:960-990): the
+    // largest D(H) * H of the H -> D spline over 3n - 1 equidistant points
+    {
+        fmm_spline(n, H, D, b.data(), c.data(), d.data());
+        double best = -1.0e-14;
+        for (int I = 2; I <= 3 * n; ++I) {
+#pragma clang fp contract(off)
+            const double hv = H[0] + (double)(I - 1) * (H[n - 1] - H[0]) / (double)(3 * n);
+            const int i = (int)(std::upper_bound(H, H + n, hv) - H) - 1;   // H[i] <= hv < H[i+1]
+            const double dx = hv - H[i];
+            double dv = D[i] + dx * (b[i] + dx * (c[i] + dx * d[i]));
+            if (std::fabs(hv) < 1.0e-10) dv = 1.0;
+            if (dv * hv > best) best = dv * hv;
+        }
+        p->rhoh_max = best;
+    }
     std::vector<int> sh, st;
     bucket_starts(n, H, p->nbh, p->h_lo, p->hbi, sh);
     bucket_starts(n, T, p->nbt, p->t_lo, p->tbi, st);
@@ -272,6 +288,12 @@ int chs_props_create(int n, const double* H, const double* T, const double* D, c
 }
 
 int chs_props_destroy(chs_props* p) { return chs::props_destroy(p); }
+
+int chs_props_rhoh_max(chs_props* p, double* rhoh_max) {
+    CHS_CHECK(p != nullptr && rhoh_max != nullptr, CHS_E_ARG, "chs_props_rhoh_max: NULL argument");
+    *rhoh_max = p->rhoh_max;
+    return CHS_OK;
+}
 
 int chs_therm_prop_update(chs_props* p, int mode, const double* h_dev, long long nrow, long long rowlen,
                           long long row_stride, double* T_dev, double* rho_dev, double* mu_dev, double* kap_dev,

@@ -14,6 +14,7 @@ struct chs_props {
     int nbh = 0, nbt = 0;      // buckets of the H and T interval searches
     double h_lo = 0, h_hi = 0, t_lo = 0, t_hi = 0;
     double hbi = 0, tbi = 0;   // 1 / bucket width
+    double rhoh_max = 0;       // RHOHmax of NIST_SPLINE_H (the limiter of the energy update)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     // device table: knots, bucket starts, per-interval coefficient records
@@ -92,33 +93,75 @@ inline PropTab tab_of(const chs_props* p) {
     return t;
 }
 
-// Table lookup of one cell; returns false (outputs untouched) when h or T
-// leaves the table (the reference STOPs there).
-__device__ __forceinline__ bool lookup_cell(const PropTab& t, const double* xh, const double* xt, const int* sh,
-                                            const int* st, double hv, double& T, double& D, double& Mu, double& K,
-                                            double& CP) {
-    // the reference's range tests; a NaN passes them and its lookups give NaN
+// S_HT (NIST_SLEVAL_HT, src/Flow_State.f90:1015-1070): false when h leaves
+// the table (the reference STOPs); hack: |h| < 1e-10 gives T = 1.  A NaN
+// passes the range tests and gives NaN, as there.
+__device__ __forceinline__ bool spline_ht(const PropTab& t, const double* xh, const int* sh, double hv, double& T,
+                                          bool hack) {
     if (hv != hv) {
-        T = D = Mu = K = CP = hv;
+        T = hv;
         return true;
     }
     if (hv <= t.h_lo || hv >= t.h_hi) return false;
     const int i = find_interval(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hv);
     const double4 r = reinterpret_cast<const double4*>(t.rh)[i];
-    const double tv = cubic_nc(r.x, r.y, r.z, r.w, hv - xh[i]);
+    T = cubic_nc(r.x, r.y, r.z, r.w, hv - xh[i]);
+    if (hack && fabs(hv) < 1.0e-10) T = 1.0;
+    return true;
+}
+
+// S_TD, S_TM, S_TK, S_TCp at T (NIST_SLEVAL_TD/TM/TK/TCp, src/Flow_State.f90:
+// 1415-1530): one interval search for all four; T == 1 gives D = M = K = 1.
+__device__ __forceinline__ bool spline_t(const PropTab& t, const double* xt, const int* st, double tv, double& D,
+                                         double& Mu, double& K, double& CP) {
+    if (tv != tv) {
+        D = Mu = K = CP = tv;
+        return true;
+    }
     if (tv <= t.t_lo || tv >= t.t_hi) return false;
     const int k = find_interval(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tv);
     const double4* q = reinterpret_cast<const double4*>(t.rt) + 4 * (size_t)k;
     const double4 cd = q[0], cm = q[1], ck = q[2], cc = q[3];
     const double dt = tv - xt[k];
-    T = tv;
     D = cubic_nc(cd.x, cd.y, cd.z, cd.w, dt);
     Mu = cubic_nc(cm.x, cm.y, cm.z, cm.w, dt);
     K = cubic_nc(ck.x, ck.y, ck.z, ck.w, dt);
     CP = cubic_nc(cc.x, cc.y, cc.z, cc.w, dt);
-    // NIST_SLEVAL_TD/TM/TK (src/Flow_State.f90:1415,1470): T == 1 -> 1
     if (fabs(tv - 1.0) < 1.0e-10) D = Mu = K = 1.0;
     return true;
+}
+
+// THERM_PROP_UPDATE of one cell (the caller skips |h| < 1e-10); false
+// (outputs untouched) when h or T leaves the table.
+__device__ __forceinline__ bool lookup_cell(const PropTab& t, const double* xh, const double* xt, const int* sh,
+                                            const int* st, double hv, double& T, double& D, double& Mu, double& K,
+                                            double& CP) {
+    double tv;
+    if (!spline_ht(t, xh, sh, hv, tv, false)) return false;
+    if (!spline_t(t, xt, st, tv, D, Mu, K, CP)) return false;
+    T = tv;
+    return true;
+}
+
+// Stage the search tables of t in LDS (all threads of the block; ends with a
+// barrier).  Layout: xh[n] xt[n] sh[nbh] st[nbt], props_lds_bytes() bytes.
+__device__ __forceinline__ void stage_tables(const PropTab& t, unsigned char* smem, const double*& xh,
+                                             const double*& xt, const int*& sh, const int*& st) {
+    double* a = reinterpret_cast<double*>(smem);
+    double* b = a + t.n;
+    int* c = reinterpret_cast<int*>(b + t.n);
+    int* d = c + t.nbh;
+    for (int e = threadIdx.x; e < t.n; e += blockDim.x) {
+        a[e] = t.xh[e];
+        b[e] = t.xt[e];
+    }
+    for (int e = threadIdx.x; e < t.nbh; e += blockDim.x) c[e] = t.sh[e];
+    for (int e = threadIdx.x; e < t.nbt; e += blockDim.x) d[e] = t.st[e];
+    __syncthreads();
+    xh = a;
+    xt = b;
+    sh = c;
+    st = d;
 }
 
 }  // namespace chs

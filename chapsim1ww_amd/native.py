@@ -36,6 +36,12 @@ class GridDesc(C.Structure):
                                    "amvr", "acvr", "apvr", "ak1", "ak3")]
 
 
+class ThermalFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("rhoh", "enthalpy", "temperature", "density", "density0", "density1",
+                                          "viscosity", "viscosity0", "thermcondt", "rhs_energy0", "g", "g0",
+                                          "wallflux")]
+
+
 # every export of include/chapsim_gpu.h with its ctypes signature
 _SIGS = {
     "chs_create": ([C.POINTER(GridDesc), C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
@@ -60,6 +66,9 @@ _SIGS = {
     "chs_props_destroy": ([C.c_void_p], C.c_int),
     "chs_therm_prop_update": ([C.c_void_p, C.c_int, C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong]
                               + [C.c_void_p] * 5 + [C.POINTER(C.c_longlong)], C.c_int),
+    "chs_props_rhoh_max": ([C.c_void_p, _dp], C.c_int),
+    "chs_energy_substep": ([C.c_void_p, C.c_void_p, C.POINTER(ThermalFields), C.c_int, C.c_double, C.c_double],
+                           C.c_int),
     "chs_surrogate_train": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_double, C.c_ulonglong,
                              _dp], C.c_int),
     "chs_surrogate_certify": ([C.c_void_p, C.c_double, C.c_int, _dp], C.c_int),

@@ -122,6 +122,12 @@ class PropertyTable:
         nat.check(self.lib.chs_props_create(self.n, *[nat.hptr(c) for c in cols], s, C.byref(p)))
         self.p = p
 
+    def rhoh_max(self) -> float:
+        """RHOHmax of the table (the limiter of the energy update)."""
+        v = C.c_double()
+        nat.check(self.lib.chs_props_rhoh_max(self.p, C.byref(v)))
+        return v.value
+
     def close(self):
         if getattr(self, "p", None):
             self.lib.chs_props_destroy(self.p)

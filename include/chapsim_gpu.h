@@ -159,6 +159,34 @@ int chs_therm_prop_update(chs_props *p, int mode, const double *h_dev, long long
                           long long row_stride, double *T_dev, double *rho_dev, double *mu_dev,
                           double *kap_dev, double *cp_dev, long long *n_fallback);
 
+/* RHOHmax of the table (NIST_SPLINE_H, src/Flow_State.f90:960-990): the
+ * limiter of the energy update. */
+int chs_props_rhoh_max(chs_props *p, double *rhoh_max);
+
+/* ---- energy equation (variable-property heated channel) ------------------
+ * Device arrays of the thermal state, reference layouts:
+ *   (NCL1, 0:NCL2+1, NCL3): RHOH, ENTHALPY, TEMPERATURE, DENSITY, DENSITY0,
+ *       DENSITY1, VISCOUSITY, VISCOUSITY0, THERMCONDT
+ *   RHS_ENERGY0 (NCL1, NCL2, NCL3)
+ *   G_io, G0_io (NCL1, 0:NCL2+1, NCL3, 3)   mass fluxes of this and the last substep
+ *   WALLFLUX (NCL1, 2)                       wall heat flux, bottom then top */
+typedef struct chs_thermal_fields {
+    double *rhoh, *enthalpy, *temperature, *density, *density0, *density1;
+    double *viscosity, *viscosity0, *thermcondt, *rhs_energy0;
+    const double *g, *g0, *wallflux;
+} chs_thermal_fields;
+
+/* One energy substep SOLVERRK3_ENG_IO(NS) (src/SOLVERRK3_ENG_IO.f90:1-77) on
+ * the grid of h (periodic x and z, iso-flux walls, table properties of p):
+ * RHS_ENERGY_EXPLICIT (src/RHS_ENERGY_EXPLICIT.f90:1-150), the RK3 update of
+ * RHOH with the RHOHmax limiter, ENTHALPY_UPDATE / DENSITY_UPDATE /
+ * THERM_PROP_UPDATE (src/THERM_PROP_UPDATE.f90:1-205) and BC_WALL_THERMAL
+ * (src/BC_WALL.f90:522-650).  cthecd = 1/(REN PRT0).  Work is ordered on h's
+ * stream; returns CHS_E_RANGE when a lookup leaves the table (the reference
+ * STOPs).  Synchronous. */
+int chs_energy_substep(chs_handle *h, chs_props *p, const chs_thermal_fields *f, int ns, double dt,
+                       double cthecd);
+
 /* ---- learned property surrogate ------------------------------------------
  * The reference has no surrogate: these entries train and certify the network
  * that chs_therm_prop_update(..., CHS_PROPS_SURROGATE, ...) runs in place of

@@ -1099,3 +1099,175 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ---------------------------------------------------------------------------
+ * Energy substep of the heated channel, one rank, periodic x and z, iso-flux
+ * walls (BCWALLHEAT = isoFluxWall at both walls), property table lookup:
+ *   SOLVERRK3_ENG_IO (src/SOLVERRK3_ENG_IO.f90:1-77):
+ *     VISCOUSITY0 = VISCOUSITY, DENSITY1 = DENSITY0, DENSITY0 = DENSITY
+ *     RHS_ENERGY_EXPLICIT (src/RHS_ENERGY_EXPLICIT.f90:1-150)
+ *     RHOH += TGAM*DT*RHS + TROH*DT*RHS0, RHOH = min(RHOHmax + REALMIN, RHOH);
+ *     RHS0 = RHS
+ *     ENTHALPY_UPDATE, DENSITY_UPDATE, ENTHALPY_UPDATE, THERM_PROP_UPDATE
+ *     (src/THERM_PROP_UPDATE.f90:1-33, 70-128, 129-205)
+ *     INTFC_MFD_THERMAL_io (src/INTFC_THERMAL_io.f90): on one rank every
+ *     field it exchanges is then overwritten at the walls, so it is omitted
+ *     BC_WALL_THERMAL -> BC_WALL_ISOFLUX top, then bottom
+ *     (src/BC_WALL.f90:522-650)
+ * Channel metrics: RCCI1 = RNDI1 = RCCI2 = 1, XND2CL = ZND2CL = 1/2.
+ * Arrays (L, 0:M+1, N): rhoh h T rho rho0 rho1 mu mu0 kap; rhs0 (L, M, N);
+ * g, g0 (L, 0:M+1, N, 3); wallflux (L, 2).  tab: as orc_therm_prop_update.
+ * Returns the number of table lookups out of range (the reference STOPs).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    double *rhoh, *h, *T, *rho, *rho0, *rho1, *mu, *mu0, *kap, *rhs0;
+    const double *g, *g0, *wallflux;
+} orc_thermal;
+
+#define TL (g->L)
+#define TM (g->M)
+#define TN (g->N)
+#define T3(arr, i, j, k) (arr)[(size_t)((i)-1) + (size_t)TL * ((size_t)(j) + (size_t)(TM + 2) * ((k)-1))]
+#define TF3(arr, i, j, k) (arr)[(size_t)((i)-1) + (size_t)TL * ((size_t)((j)-1) + (size_t)TM * ((k)-1))]
+#define TG(arr, i, j, k, c) (arr)[(size_t)((i)-1) + (size_t)TL * ((size_t)(j) + (size_t)(TM + 2) * ((size_t)((k)-1) + (size_t)TN * ((c)-1)))]
+#define TIPV(i) ((i) == TL ? 1 : (i) + 1)
+#define TIMV(i) ((i) == 1 ? TL : (i)-1)
+#define TKPV(k) ((k) == TN ? 1 : (k) + 1)
+#define TKMV(k) ((k) == 1 ? TN : (k)-1)
+
+static int sl_ht(int n, const double *tab, double h, double *T)
+{
+    return orc_sleval(n, tab, tab + n, tab + 6 * n, tab + 7 * n, tab + 8 * n, h, 1, T);
+}
+/* T -> D, M, K with the NIST_SLEVAL_TD/TM/TK post-fix */
+static int sl_t(int n, const double *tab, double T, double *D, double *Mu, double *K)
+{
+    int e = 0;
+    if (Mu) e |= orc_sleval(n, tab + n, tab + 3 * n, tab + 12 * n, tab + 13 * n, tab + 14 * n, T, 2, Mu);
+    if (K) e |= orc_sleval(n, tab + n, tab + 4 * n, tab + 15 * n, tab + 16 * n, tab + 17 * n, T, 2, K);
+    if (D) e |= orc_sleval(n, tab + n, tab + 2 * n, tab + 9 * n, tab + 10 * n, tab + 11 * n, T, 2, D);
+    return e;
+}
+
+long orc_energy_substep(const orc_grid *g, orc_thermal *t, int n, const double *tab, int NS, double DT,
+                        double CTHECD, double RHOHMAX)
+{
+    const size_t nall = (size_t)TL * (TM + 2) * TN;
+    long bad = 0;
+    memcpy(t->mu0, t->mu, nall * sizeof(double));
+    memcpy(t->rho1, t->rho0, nall * sizeof(double));
+    memcpy(t->rho0, t->rho, nall * sizeof(double));
+    /* RHS_ENERGY_EXPLICIT + the RK3 update of RHOH */
+    const double COE1 = g->dxi * 0.5 * 0.5;
+    const double COE11 = g->dxqi * 0.5 * CTHECD;
+    const double RK1 = TGAM[NS] * DT, RK0 = TROH[NS] * DT;
+#pragma omp parallel for schedule(static)
+    for (int KC = 1; KC <= TN; ++KC) {
+        const int KP = TKPV(KC), KM = TKMV(KC);
+        for (int JC = 1; JC <= TM; ++JC) {
+            const int JP = JC + 1, JM = JC - 1, JJ = JC, JJP = JJ + 1;
+            const double COE2 = g->dyfi[JJ - 1] * 1.0 * 0.5;
+            const double COE3 = g->dzi * 0.5 * 1.0 * 0.5;
+            const double COE33 = g->dzqi * 0.5 * 1.0 * CTHECD;
+            const double COE22 = g->dyfi[JJ - 1] * 1.0 * CTHECD;
+            double COE221, COE222;
+            if (JJ == 1) {
+                COE221 = g->dyci[JJP - 1] / 1.0;
+                COE222 = 1.0 / 1.0;                       /* iso-flux bottom wall */
+            } else if (JJ == TM) {
+                COE222 = g->dyci[JJ - 1] / 1.0;
+                COE221 = 1.0 / 1.0;                       /* iso-flux top wall */
+            } else {
+                COE221 = g->dyci[JJP - 1] / 1.0;
+                COE222 = g->dyci[JJ - 1] / 1.0;
+            }
+            const double wffp = g->wff[JJP - 1], wfbp = g->wfb[JJP - 1], wff = g->wff[JJ - 1], wfb = g->wfb[JJ - 1];
+            for (int IC = 1; IC <= TL; ++IC) {
+                const int IP = TIPV(IC), IM = TIMV(IC);
+                const double *H = t->h, *TT = t->T, *KK = t->kap;
+                double DX_RHOHU = (T3(H, IP, JC, KC) + T3(H, IC, JC, KC)) * (TG(t->g0, IP, JC, KC, 1) + TG(t->g, IP, JC, KC, 1)) -
+                                  (T3(H, IC, JC, KC) + T3(H, IM, JC, KC)) * (TG(t->g0, IC, JC, KC, 1) + TG(t->g, IC, JC, KC, 1));
+                DX_RHOHU = DX_RHOHU * COE1;
+                double DY_RHOHV = (wffp * T3(H, IC, JP, KC) + wfbp * T3(H, IC, JC, KC)) *
+                                      (TG(t->g0, IC, JP, KC, 2) + TG(t->g, IC, JP, KC, 2)) -
+                                  (wff * T3(H, IC, JC, KC) + wfb * T3(H, IC, JM, KC)) *
+                                      (TG(t->g0, IC, JC, KC, 2) + TG(t->g, IC, JC, KC, 2));
+                DY_RHOHV = DY_RHOHV * COE2;
+                double DZ_RHOHW = (T3(H, IC, JC, KP) + T3(H, IC, JC, KC)) * (TG(t->g0, IC, JC, KP, 3) + TG(t->g, IC, JC, KP, 3)) -
+                                  (T3(H, IC, JC, KC) + T3(H, IC, JC, KM)) * (TG(t->g0, IC, JC, KC, 3) + TG(t->g, IC, JC, KC, 3));
+                DZ_RHOHW = DZ_RHOHW * COE3;
+                double DX_KDT = (T3(KK, IP, JC, KC) + T3(KK, IC, JC, KC)) * (T3(TT, IP, JC, KC) - T3(TT, IC, JC, KC)) -
+                                (T3(KK, IC, JC, KC) + T3(KK, IM, JC, KC)) * (T3(TT, IC, JC, KC) - T3(TT, IM, JC, KC));
+                DX_KDT = DX_KDT * COE11;
+                double KDTy1 = (wffp * T3(KK, IC, JP, KC) + wfbp * T3(KK, IC, JC, KC)) *
+                               (T3(TT, IC, JP, KC) - T3(TT, IC, JC, KC)) * COE221;
+                double KDTy2 = (wff * T3(KK, IC, JC, KC) + wfb * T3(KK, IC, JM, KC)) *
+                               (T3(TT, IC, JC, KC) - T3(TT, IC, JM, KC)) * COE222;
+                if (JJ == 1)
+                    KDTy2 = -t->wallflux[(IC - 1)] * COE222;
+                else if (JJ == TM)
+                    KDTy1 = -t->wallflux[(IC - 1) + TL] * COE221;
+                const double DY_KDT = (KDTy1 - KDTy2) * COE22;
+                double DZ_KDT = (T3(KK, IC, JC, KP) + T3(KK, IC, JC, KC)) * (T3(TT, IC, JC, KP) - T3(TT, IC, JC, KC)) -
+                                (T3(KK, IC, JC, KM) + T3(KK, IC, JC, KC)) * (T3(TT, IC, JC, KC) - T3(TT, IC, JC, KM));
+                DZ_KDT = DZ_KDT * COE33;
+                const double RHS = -DX_RHOHU - DY_RHOHV - DZ_RHOHW + DX_KDT + DY_KDT + DZ_KDT;
+                double r = T3(t->rhoh, IC, JC, KC) + RK1 * RHS + RK0 * TF3(t->rhs0, IC, JC, KC);
+                r = fmin(RHOHMAX + 1.0e-14, r);
+                T3(t->rhoh, IC, JC, KC) = r;
+                TF3(t->rhs0, IC, JC, KC) = RHS;
+            }
+        }
+    }
+    /* ENTHALPY_UPDATE, DENSITY_UPDATE, ENTHALPY_UPDATE, THERM_PROP_UPDATE (interior) */
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int KC = 1; KC <= TN; ++KC)
+        for (int JC = 1; JC <= TM; ++JC)
+            for (int IC = 1; IC <= TL; ++IC) {
+                double hv = T3(t->rhoh, IC, JC, KC) / T3(t->rho, IC, JC, KC);
+                double Tt, D;
+                if (sl_ht(n, tab, hv, &Tt) || sl_t(n, tab, Tt, &D, NULL, NULL)) { bad++; continue; }
+                T3(t->rho, IC, JC, KC) = D;
+                hv = T3(t->rhoh, IC, JC, KC) / D;
+                T3(t->h, IC, JC, KC) = hv;
+                if (fabs(hv) < 1.0e-10) continue;
+                double Mu, K;
+                if (sl_ht(n, tab, hv, &Tt) || sl_t(n, tab, Tt, &D, &Mu, &K)) { bad++; continue; }
+                T3(t->T, IC, JC, KC) = Tt;
+                T3(t->mu, IC, JC, KC) = Mu;
+                T3(t->kap, IC, JC, KC) = K;
+                T3(t->rho, IC, JC, KC) = D;
+            }
+    /* BC_WALL_ISOFLUX: top wall (J = M+1), then bottom wall (J = 0) */
+    for (int wall = 0; wall < 2; ++wall) {
+        const int J = wall == 0 ? TM + 1 : 0;
+        for (int KC = 1; KC <= TN; ++KC)
+            for (int IC = 1; IC <= TL; ++IC) {
+                double hv;
+                if (J == 0)
+                    hv = 2.0 * T3(t->h, IC, 1, KC) - g->wff[2 - 1] * T3(t->h, IC, 2, KC) - g->wfb[2 - 1] * T3(t->h, IC, 1, KC);
+                else
+                    hv = 2.0 * T3(t->h, IC, J - 1, KC) - g->wff[J - 1 - 1] * T3(t->h, IC, J - 1, KC) -
+                         g->wfb[J - 1 - 1] * T3(t->h, IC, J - 2, KC);
+                T3(t->h, IC, J, KC) = hv;
+                double Tt, Mu, K, D;
+                if (sl_ht(n, tab, hv, &Tt) || sl_t(n, tab, Tt, &D, &Mu, &K)) { bad++; continue; }
+                T3(t->rho, IC, J, KC) = D;
+                T3(t->T, IC, J, KC) = Tt;
+                T3(t->mu, IC, J, KC) = Mu;
+                T3(t->kap, IC, J, KC) = K;
+                T3(t->rhoh, IC, J, KC) = hv * D;
+            }
+    }
+    return bad;
+}
+#undef TL
+#undef TM
+#undef TN
+#undef T3
+#undef TF3
+#undef TG
+#undef TIPV
+#undef TIMV
+#undef TKPV
+#undef TKMV

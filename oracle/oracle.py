@@ -86,6 +86,9 @@ def lib():
         L.orc_sleval.restype = C.c_int
         L.orc_therm_prop_update.argtypes = [C.c_int, _dp, C.c_long, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_therm_prop_update.restype = C.c_long
+        L.orc_energy_substep.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_int, C.c_double, C.c_double,
+                                         C.c_double]
+        L.orc_energy_substep.restype = C.c_long
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -224,6 +227,75 @@ def build_prop_tab(scaled):
 
 def therm_prop_update(tab, n, h, T, rho, mu, kap, cp=None):
     return lib().orc_therm_prop_update(n, _ptr(tab), h.size, _ptr(h), _ptr(T), _ptr(rho), _ptr(mu), _ptr(kap), _ptr(cp))
+
+
+class _Thermal(C.Structure):
+    _fields_ = [(n, _dp) for n in ("rhoh", "h", "T", "rho", "rho0", "rho1", "mu", "mu0", "kap", "rhs0", "g", "g0",
+                                   "wallflux")]
+
+
+THERMAL_FIELDS = ("rhoh", "h", "T", "rho", "rho0", "rho1", "mu", "mu0", "kap", "rhs0")
+
+
+def energy_substep(og: "OracleGrid", st: dict, tab, n, ns, dt, cthecd, rhohmax):
+    """SOLVERRK3_ENG_IO on the numpy arrays of st (Fortran order, updated in
+    place): rhoh h T rho rho0 rho1 mu mu0 kap (L, M+2, N), rhs0 (L, M, N),
+    g g0 (L, M+2, N, 3), wallflux (L, 2).  Returns the out-of-range count."""
+    for k in THERMAL_FIELDS + ("g", "g0", "wallflux"):
+        a = st[k]
+        assert a.dtype == np.float64 and a.flags.f_contiguous, k
+    t = _Thermal(*[_ptr(st[k]) for k in THERMAL_FIELDS + ("g", "g0", "wallflux")])
+    return lib().orc_energy_substep(C.byref(og.g), C.byref(t), n, _ptr(tab), ns, dt, cthecd, rhohmax)
+
+
+def rhoh_max(scaled):
+    """RHOHmax of NIST_SPLINE_H (src/Flow_State.f90:960-990): max of D(H) * H
+    over H_I = H_1 + (I-1)(H_n - H_1)/(3n), I = 2..3n, D from the H -> D spline."""
+    H = np.ascontiguousarray(scaled["H"], np.float64)
+    D = np.ascontiguousarray(scaled["D"], np.float64)
+    n = len(H)
+    b, c, d = cubic_spline(H, D)
+    best = -1e-14
+    e = np.zeros(1)
+    for I in range(2, 3 * n + 1):
+        hv = H[0] + float(I - 1) * (H[-1] - H[0]) / float(3 * n)
+        assert lib().orc_sleval(n, _ptr(H), _ptr(D), _ptr(b), _ptr(c), _ptr(d), hv, 1, _ptr(e)) == 0
+        if e[0] * hv > best:
+            best = e[0] * hv
+    return best
+
+
+def thermal_case(og: "OracleGrid", scaled, seed=0, qw=0.1):
+    """A consistent random thermal state on og's grid: h smooth + noise inside
+    the table, T, rho, mu, kappa from the oracle's table lookup, rho*h,
+    histories, random mass fluxes with zero wall-normal flux at the walls."""
+    L, M, N = og.L, og.M, og.N
+    rng = np.random.default_rng(seed)
+    tab, n = build_prop_tab(scaled)
+    y = np.concatenate([[og.ynd[0]], og.ycc, [og.ynd[-1]]])
+    x = np.arange(L) / L * 2 * np.pi
+    z = np.arange(N) / N * 2 * np.pi
+    T0 = 1.0 + 0.25 * (1 - y[None, :, None] ** 2) + 0.03 * np.sin(x)[:, None, None] * np.cos(z)[None, None, :]
+    T0 = T0 + 0.005 * rng.standard_normal((L, M + 2, N))
+    h = np.interp(T0.ravel(), scaled["T"], scaled["H"]).reshape(T0.shape)
+    f = lambda a: np.asfortranarray(a, dtype=np.float64)
+    h = f(h)
+    out = [np.zeros(h.size) for _ in range(5)]
+    assert therm_prop_update(tab, n, np.ascontiguousarray(h.ravel(order="F")), *out) == 0
+    T, rho, mu, kap, _ = (f(o.reshape(h.shape, order="F")) for o in out)
+    st = {"h": h, "T": T, "rho": rho, "mu": mu, "kap": kap, "rhoh": f(h * rho),
+          "rho0": f(rho * (1 + 1e-3 * rng.standard_normal(rho.shape))), "rho1": f(rho.copy()),
+          "mu0": f(mu.copy()), "rhs0": f(1e-3 * rng.standard_normal((L, M, N)))}
+    g = 1.0 + 0.2 * rng.standard_normal((L, M + 2, N, 3))
+    g[:, :, :, 1] *= 0.1
+    g[:, 1, :, 1] = 0.0          # wall faces: no wall-normal mass flux
+    g[:, M + 1, :, 1] = 0.0
+    g0 = g * (1 + 0.01 * rng.standard_normal(g.shape))
+    g0[:, 1, :, 1] = 0.0
+    g0[:, M + 1, :, 1] = 0.0
+    st["g"], st["g0"] = f(g), f(g0)
+    st["wallflux"] = f(np.stack([np.full(L, qw), np.full(L, qw)], 1))
+    return st, tab, n
 
 
 def num_threads() -> int:
