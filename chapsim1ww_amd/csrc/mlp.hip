@@ -7,12 +7,15 @@
 //
 // Network (float32 master parameters, flat, NPAR = 4613):
 //   w1[64] b1[64] W2[64][64] b2[64] W3[5][64] b3[5]        W[out][in]
-//   x = (h - hc) * hs;  a1 = tanh(bf(w1) x + b1)
-//   a2 = tanh(bf(W2) (hi(a1) + lo(a1)) + b2);  o = bf(W3) (hi(a2) + lo(a2)) + b3
-//   y = m + s o
-// bf() rounds to bfloat16 (nearest even); activations enter the matrix cores
-// as a bfloat16 pair hi + lo (hi = bf(a), lo = bf(a - hi)), so a product keeps
-// ~16 significant bits of the activation while the weights are BF16.
+// Every weight is stored as a pair of bfloat16 values, q(w) = hi(w) + lo(w)
+// with hi(v) = bf(v) and lo(v) = bf(v - hi(v)) (bf = round to bfloat16,
+// nearest even); activations enter the matrix cores the same way:
+//   x = (h - hc) * hs;  a1 = tanh(q(w1) x + b1)
+//   a2 = tanh(hi(W2) (hi(a1) + lo(a1)) + lo(W2) hi(a1) + b2)
+//   o  = hi(W3) (hi(a2) + lo(a2)) + lo(W3) hi(a2) + b3;   y = m + s o
+// -- three BF16 products with FP32 accumulation per weight block, keeping
+// ~16 significant bits of weights and activations (a single BF16 weight
+// limits the fit to ~1e-3 relative, above the 1e-4 the fallback demands).
 //
 // Matrix-core tiling (v_mfma_f32_16x16x32_bf16, wave64): a tile is 16 samples.
 // The result of a 16-unit x 16-sample product sits in lane l = 16 g + i as
@@ -37,8 +40,6 @@ constexpr int NPT = 33;             // certification points per bin (edges inclu
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 -> float, round to nearest even (the hardware conversion)
-__device__ inline float bfround(float f) { return (float)(__bf16)f; }
 
 // tanh(z) = sign(z) (1 - 2 / (exp(2|z|) + 1)); absolute error ~1e-7
 __device__ inline float tanh_f(float z) {
@@ -56,52 +57,44 @@ struct MlpNorm {
     float m[NOUT], s[NOUT];
 };
 
-// per-lane register copy of the network
-struct WReg {
-    bf8v w2[4][2];        // A operands of W2: [unit tile][32-unit block]
-    bf8v w3[2];           // A operand of W3 (rows = outputs, 5 used)
-    float w1[16], b1[16]; // layer 1 for the lane's units 16t + 4g + r (index 4t + r)
-    float b2[16];         // layer 2 biases, same units
-    float b3[4];          // outputs 4g + r
+// The network's matrix operands in LDS, each lane's 8 values in its slot
+// order: op 0..7 = W2 hi [unit tile ut][block kb] at 2 ut + kb, 8..15 = W2 lo,
+// 16..17 = W3 hi [kb], 18..19 = W3 lo; plus layer 1 and the biases.
+constexpr int NOPS = 20;
+struct WLds {
+    bf8v op[NOPS][64];
+    float w1[H1], b1[H1], b2[H2], b3[8];
 };
 
-__device__ inline bf8v pack8(const float* v) {
-    bf8v b;
+// wq holds every weight as hi + lo (two bfloat16 values, exact in float):
+// split it back into its matrix operands
+__device__ inline void stage_weights(WLds& W, const float* __restrict__ wq) {
+    for (int e = threadIdx.x; e < NOPS * 64; e += blockDim.x) {
+        const int op = e >> 6, lane = e & 63, i = lane & 15, g = lane >> 4;
+        const bool lo = (op >= 8 && op < 16) || op >= 18;
+        bf8v v;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) b[q] = (__bf16)v[q];
-    return b;
-}
-
-// weights: wq = bf16-rounded parameters stored as floats, biases as FP32
-__device__ inline void load_wreg(WReg& W, const float* __restrict__ wq, int lane) {
-    const int i = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int ut = 0; ut < 4; ++ut)
-#pragma unroll
-        for (int kb = 0; kb < 2; ++kb) {
-            float v[8];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) v[s] = wq[O_W2 + (16 * ut + i) * H1 + 32 * kb + slot_unit(g, s)];
-            W.w2[ut][kb] = pack8(v);
+        for (int s = 0; s < 8; ++s) {
+            float w;
+            if (op < 16) {
+                const int q = op & 7, ut = q >> 1, kb = q & 1;
+                w = wq[O_W2 + (16 * ut + i) * H1 + 32 * kb + slot_unit(g, s)];
+            } else {
+                const int kb = (op - 16) & 1;
+                w = i < NOUT ? wq[O_W3 + i * H2 + 32 * kb + slot_unit(g, s)] : 0.0f;
+            }
+            const __bf16 h = (__bf16)w;
+            v[s] = lo ? (__bf16)(w - (float)h) : h;
         }
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
-        float v[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) v[s] = i < NOUT ? wq[O_W3 + i * H2 + 32 * kb + slot_unit(g, s)] : 0.0f;
-        W.w3[kb] = pack8(v);
+        W.op[op][lane] = v;
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int u = 16 * t + 4 * g + r;
-            W.w1[4 * t + r] = wq[O_W1 + u];
-            W.b1[4 * t + r] = wq[O_B1 + u];
-            W.b2[4 * t + r] = wq[O_B2 + u];
-        }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) W.b3[r] = 4 * g + r < NOUT ? wq[O_B3 + 4 * g + r] : 0.0f;
+    for (int e = threadIdx.x; e < H1; e += blockDim.x) {
+        W.w1[e] = wq[O_W1 + e];
+        W.b1[e] = wq[O_B1 + e];
+        W.b2[e] = wq[O_B2 + e];
+    }
+    if (threadIdx.x < 8) W.b3[threadIdx.x] = threadIdx.x < NOUT ? wq[O_B3 + threadIdx.x] : 0.0f;
+    __syncthreads();
 }
 
 // hi/lo bfloat16 operands of the 32-unit block kb of a (16 values per lane)
@@ -115,37 +108,57 @@ __device__ inline void split_block(const float* a, int kb, bf8v& hi, bf8v& lo) {
     }
 }
 
+// W a with W = Wh + Wl and a = ah + al: Wh ah + Wh al + Wl ah on the matrix cores
+__device__ inline f4v mm3(const bf8v& wh, const bf8v& wl, const bf8v& ah, const bf8v& al, f4v acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, ah, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, al, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, ah, acc, 0, 0, 0);
+    return acc;
+}
+
 // Forward of one 16-sample tile: lane (g, i) holds sample i's x; returns
 // o for outputs 4g + r, and a1 / a2 for units 16t + 4g + r (index 4t + r).
-__device__ inline f4v forward_tile(const WReg& W, float x, float* a1, float* a2) {
+__device__ inline f4v forward_tile(const WLds& W, int lane, float x, float* a1, float* a2) {
+    const int g = lane >> 4;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) a1[q] = tanh_f(fmaf(W.w1[q], x, W.b1[q]));
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int u = 16 * t + 4 * g + r;
+            a1[4 * t + r] = tanh_f(fmaf(W.w1[u], x, W.b1[u]));
+        }
     bf8v h0, l0, h1, l1;
     split_block(a1, 0, h0, l0);
     split_block(a1, 1, h1, l1);
 #pragma unroll
     for (int ut = 0; ut < 4; ++ut) {
-        f4v acc = {W.b2[4 * ut], W.b2[4 * ut + 1], W.b2[4 * ut + 2], W.b2[4 * ut + 3]};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w2[ut][0], h0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w2[ut][0], l0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w2[ut][1], h1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w2[ut][1], l1, acc, 0, 0, 0);
+        f4v acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = W.b2[16 * ut + 4 * g + r];
+        acc = mm3(W.op[2 * ut][lane], W.op[8 + 2 * ut][lane], h0, l0, acc);
+        acc = mm3(W.op[2 * ut + 1][lane], W.op[8 + 2 * ut + 1][lane], h1, l1, acc);
 #pragma unroll
         for (int r = 0; r < 4; ++r) a2[4 * ut + r] = tanh_f(acc[r]);
     }
     split_block(a2, 0, h0, l0);
     split_block(a2, 1, h1, l1);
-    f4v o = {W.b3[0], W.b3[1], W.b3[2], W.b3[3]};
-    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w3[0], h0, o, 0, 0, 0);
-    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w3[0], l0, o, 0, 0, 0);
-    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w3[1], h1, o, 0, 0, 0);
-    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.w3[1], l1, o, 0, 0, 0);
+    f4v o;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = W.b3[(4 * g + r) & 7];
+    o = mm3(W.op[16][lane], W.op[18][lane], h0, l0, o);
+    o = mm3(W.op[17][lane], W.op[19][lane], h1, l1, o);
     return o;
+}
+
+// weight as hi + lo bfloat16 pair (exact sum in float)
+__device__ inline float wsplit(float w) {
+    const __bf16 h = (__bf16)w;
+    return (float)h + (float)(__bf16)(w - (float)h);
 }
 
 struct Mlp {
     float* par = nullptr;     // FP32 master parameters [NPAR]
-    float* wq = nullptr;      // bf16-rounded weights and FP32 biases [NPAR] (what the forward uses)
+    float* wq = nullptr;      // weights as q(w) = hi + lo, FP32 biases [NPAR] (what the forward uses)
     float* am = nullptr;      // Adam moments [NPAR] x 2
     float* av = nullptr;
     float* partial = nullptr; // per-block gradient partials [maxblk][NPAR + 1] (last: loss)
@@ -168,8 +181,6 @@ struct TrainLds {
     float o[NOUT][TB];     // outputs, then their gradients
     float t[NOUT][TB];
     float x[TB];
-    float W2[H2][H1];      // bf16-rounded weights as floats
-    float W3[NOUT][H2];
     float red[256];
 };
 
@@ -189,6 +200,7 @@ __global__ void __launch_bounds__(256) k_mlp_train(const float* __restrict__ wq,
                                                    float* __restrict__ partial) {
     extern __shared__ unsigned char smem[];
     TrainLds& S = *reinterpret_cast<TrainLds*>(smem);
+    WLds& WL = *reinterpret_cast<WLds*>(smem + sizeof(TrainLds));
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // stage samples and weights
     for (int s = tid; s < TB; s += 256) {
@@ -198,18 +210,14 @@ __global__ void __launch_bounds__(256) k_mlp_train(const float* __restrict__ wq,
         S.x[s] = q >= 0 ? px[q] : 0.0f;
         for (int k = 0; k < NOUT; ++k) S.t[k][s] = q >= 0 ? pt[q * NOUT + k] : 0.0f;   // 0 marks padding
     }
-    for (int e = tid; e < H2 * H1; e += 256) S.W2[e / H1][e % H1] = wq[O_W2 + e];
-    for (int e = tid; e < NOUT * H2; e += 256) S.W3[e / H2][e % H2] = wq[O_W3 + e];
-    __syncthreads();
-    // forward: each wave two tiles of 16 samples
+    stage_weights(WL, wq);   // ends with a barrier
+    // forward: each wave one tile of 16 samples
     {
-        WReg W;
-        load_wreg(W, wq, lane);
         const int i = lane & 15, g = lane >> 4;
         for (int tt = 0; tt < TB / 64; ++tt) {
             const int s = 16 * (wv * (TB / 64) + tt) + i;
             float a1[16], a2[16];
-            const f4v o = forward_tile(W, S.x[s], a1, a2);
+            const f4v o = forward_tile(WL, lane, S.x[s], a1, a2);
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -266,7 +274,7 @@ __global__ void __launch_bounds__(256) k_mlp_train(const float* __restrict__ wq,
         const int u = e / TB, s = e % TB;
         float acc = 0.0f;
 #pragma unroll
-        for (int k = 0; k < NOUT; ++k) acc = fmaf(S.W3[k][u], S.o[k][s], acc);
+        for (int k = 0; k < NOUT; ++k) acc = fmaf(wq[O_W3 + k * H2 + u], S.o[k][s], acc);
         const float a = S.a2[u][s];
         S.a2[u][s] = acc * (1.0f - a * a);
     }
@@ -302,7 +310,7 @@ __global__ void __launch_bounds__(256) k_mlp_train(const float* __restrict__ wq,
     for (int e = tid; e < H1 * TB; e += 256) {
         const int v = e % H1, s = e / H1;
         float acc = 0.0f;
-        for (int u = 0; u < H2; ++u) acc = fmaf(S.W2[u][v], S.a2[u][s], acc);
+        for (int u = 0; u < H2; ++u) acc = fmaf(wq[O_W2 + u * H1 + v], S.a2[u][s], acc);
         const float a = S.a1[v][s];
         S.a1[v][s] = acc * (1.0f - a * a);
     }
@@ -319,7 +327,7 @@ __global__ void __launch_bounds__(256) k_mlp_train(const float* __restrict__ wq,
 }
 
 // sum of the block partials (fixed order) -> gradient (and loss); Adam step
-// when lr > 0; wq <- bf16-rounded weights, FP32 biases
+// when lr > 0; wq <- q(w) for weights, FP32 biases
 __global__ void k_mlp_adam(const float* __restrict__ partial, int nblk, float* __restrict__ par,
                            float* __restrict__ am, float* __restrict__ av, float* __restrict__ wq,
                            float* __restrict__ grad_out, float lr, float b1t, float b2t) {
@@ -338,14 +346,14 @@ __global__ void k_mlp_adam(const float* __restrict__ partial, int nblk, float* _
     const float w = par[p] - lr * mh / (sqrtf(vh) + 1e-8f);
     par[p] = w;
     const bool bias = (p >= O_B1 && p < O_W2) || (p >= O_B2 && p < O_W3) || p >= O_B3;
-    wq[p] = bias ? w : bfround(w);
+    wq[p] = bias ? w : wsplit(w);
 }
 
 __global__ void k_mlp_round(const float* __restrict__ par, float* __restrict__ wq) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= NPAR) return;
     const bool bias = (p >= O_B1 && p < O_W2) || (p >= O_B2 && p < O_W3) || p >= O_B3;
-    wq[p] = bias ? par[p] : bfround(par[p]);
+    wq[p] = bias ? par[p] : wsplit(par[p]);
 }
 
 // training pool: x = (h - hc) hs (float), targets from the table (float)
@@ -368,9 +376,10 @@ __global__ void k_mlp_pool(const double* __restrict__ h, long long n, MlpNorm nm
 __global__ void __launch_bounds__(256) k_mlp_eval(const float* __restrict__ wq, MlpNorm nm,
                                                   const double* __restrict__ h, long long n,
                                                   double* __restrict__ out) {
+    extern __shared__ unsigned char smem[];
+    WLds& W = *reinterpret_cast<WLds*>(smem);
+    stage_weights(W, wq);
     const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
-    WReg W;
-    load_wreg(W, wq, lane);
     const long long ntile = (n + 15) / 16;
     const long long wid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
@@ -379,7 +388,7 @@ __global__ void __launch_bounds__(256) k_mlp_eval(const float* __restrict__ wq, 
         const double hv = q < n ? h[q] : nm.hc;
         const float x = (float)((hv - nm.hc) * nm.hs);
         float a1[16], a2[16];
-        const f4v o = forward_tile(W, x, a1, a2);
+        const f4v o = forward_tile(W, lane, x, a1, a2);
         if (q < n) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -404,22 +413,14 @@ __global__ void __launch_bounds__(ST) k_mlp_props(PropTab t, const float* __rest
                                                   double* __restrict__ kap, double* __restrict__ cp,
                                                   unsigned long long* __restrict__ cnt) {
     extern __shared__ unsigned char smem[];
-    double* xh = reinterpret_cast<double*>(smem);
-    double* xt = xh + t.n;
-    int* sh = reinterpret_cast<int*>(xt + t.n);
-    int* st = sh + t.nbh;
-    unsigned char* sc = reinterpret_cast<unsigned char*>(st + t.nbt);
-    for (int e = threadIdx.x; e < t.n; e += blockDim.x) {
-        xh[e] = t.xh[e];
-        xt[e] = t.xt[e];
-    }
-    for (int e = threadIdx.x; e < t.nbh; e += blockDim.x) sh[e] = t.sh[e];
-    for (int e = threadIdx.x; e < t.nbt; e += blockDim.x) st[e] = t.st[e];
+    WLds& W = *reinterpret_cast<WLds*>(smem);
+    unsigned char* sc = smem + sizeof(WLds);
     for (int e = threadIdx.x; e < nbins; e += blockDim.x) sc[e] = cert[e];
-    __syncthreads();
+    stage_weights(W, wq);   // ends with a barrier
+    // the table fallback searches the knots in global memory (L2-resident)
+    const double *xh = t.xh, *xt = t.xt;
+    const int *sh = t.sh, *st = t.st;
     const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
-    WReg W;
-    load_wreg(W, wq, lane);
     const double binv = (double)nbins / (nm.h_hi - nm.h_lo);
     const long long chunk = (long long)ST / 4 * SPASS;        // cells per chunk
     const long long cpr = (rw.rowlen + chunk - 1) / chunk;
@@ -442,8 +443,11 @@ __global__ void __launch_bounds__(ST) k_mlp_props(PropTab t, const float* __rest
                 use_net = sc[b] != 0;
             }
             const float x = (float)((use_net ? hv - nm.hc : 0.0) * nm.hs);
-            float a1[16], a2[16];
-            const f4v o = forward_tile(W, x, a1, a2);   // every lane takes part (wave-wide matrix ops)
+            f4v o = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (__any(use_net)) {   // wave-uniform: the matrix ops run with every lane active
+                float a1[16], a2[16];
+                o = forward_tile(W, lane, x, a1, a2);
+            }
             if (!in || fabs(hv) < 1.0e-10) continue;
             if (use_net) {
 #pragma unroll
@@ -561,7 +565,7 @@ static int set_params(chs_props* p, const float* host) {
     return CHS_OK;
 }
 
-static size_t mlp_props_lds(const chs_props* p) { return props_lds_bytes(p) + (size_t)(p->mlp ? p->mlp->nbins : 0); }
+static size_t mlp_props_lds(const chs_props* p) { return sizeof(WLds) + (size_t)(p->mlp ? p->mlp->nbins : 0); }
 
 int mlp_update(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap, double* cp,
                unsigned long long* cnt) {
@@ -675,7 +679,7 @@ static int train(chs_props* p, const double* h, long long nsamp, int steps, int 
         CHS_TRY(set_params(p, par.data()));
     }
     m = p->mlp;
-    const size_t lds = sizeof(TrainLds);
+    const size_t lds = sizeof(TrainLds) + sizeof(WLds);
     double b1t = 1.0, b2t = 1.0;
     for (long long s0 = 0; s0 < (long long)steps; ++s0) {
         // cosine decay from lr0 to lr0/100
@@ -728,7 +732,7 @@ static int grad(chs_props* p, const double* h, long long n, float* grad_host, do
     CHS_TRY(make_pool(p, h, n, pool));
     float* g = nullptr;
     CHS_HIP(hipMalloc((void**)&g, sizeof(float) * (NPAR + 1)));
-    hipLaunchKernelGGL(k_mlp_train, dim3(nblk), dim3(256), sizeof(TrainLds), p->stream, m->wq, m->norm, pool.px,
+    hipLaunchKernelGGL(k_mlp_train, dim3(nblk), dim3(256), sizeof(TrainLds) + sizeof(WLds), p->stream, m->wq, m->norm, pool.px,
                        pool.pt, n, n, 0ull, 0ll, 0, 1.0 / (5.0 * n), m->partial);
     hipLaunchKernelGGL(k_mlp_adam, dim3((NPAR + 1 + 255) / 256), dim3(256), 0, p->stream, m->partial, nblk, m->par,
                        m->am, m->av, m->wq, g, 0.0f, 0.0f, 0.0f);
@@ -772,7 +776,7 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
     Rows rw{1, np, np};
     st = props_table_launch(p, hp, rw, tab, tab + np, tab + 2 * np, tab + 3 * np, tab + 4 * np, p->counters);
     if (st == CHS_OK) {
-        hipLaunchKernelGGL(k_mlp_eval, dim3(1024), dim3(256), 0, p->stream, m->wq, m->norm, hp, np, net);
+        hipLaunchKernelGGL(k_mlp_eval, dim3(1024), dim3(256), sizeof(WLds), p->stream, m->wq, m->norm, hp, np, net);
         if (hipGetLastError() != hipSuccess) st = CHS_E_HIP;
     }
     std::vector<double> ht(5 * (size_t)np), hn(5 * (size_t)np);
@@ -818,7 +822,7 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
 static int eval(chs_props* p, const double* h, long long n, double* out) {
     CHS_CHECK(p->mlp && p->mlp->trained, CHS_E_ARG, "chs_surrogate_eval: no network");
     CHS_CHECK(h && out && n > 0, CHS_E_ARG, "chs_surrogate_eval: bad arguments");
-    CHS_LAUNCH(k_mlp_eval, dim3((unsigned)std::min<long long>((n + 63) / 64, 2048)), dim3(256), 0, p->stream,
+    CHS_LAUNCH(k_mlp_eval, dim3((unsigned)std::min<long long>((n + 63) / 64, 2048)), dim3(256), sizeof(WLds), p->stream,
                p->mlp->wq, p->mlp->norm, h, n, out);
     CHS_HIP(hipStreamSynchronize(p->stream));
     return CHS_OK;

@@ -4,7 +4,8 @@
 
 Runs, at 256x192x256: NSUB FP64 RK3 substeps, NPOIS FP64 Poisson solves,
 NTDMA batched tridiagonal solves (65536 lines x 192, IJJ form, non-cyclic
-and cyclic), and a plain device copy of 1 GiB (read 1 GiB + write 1 GiB)
+and cyclic), NTDMA table-lookup THERM_PROP_UPDATEs over the grid, and a plain
+device copy of 1 GiB (read 1 GiB + write 1 GiB)
 whose counter readings calibrate the byte counters of the others.
 """
 from __future__ import annotations
@@ -49,6 +50,16 @@ def main() -> None:
     R = torch.from_numpy(rng.standard_normal(n * nl)).to(dev)
     for i in range(a.ntdma):
         tdma_batched(A, B, C, R, None, n, nl, nl, 1, i % 2 == 1, st)
+    # THERM_PROP_UPDATE by table lookup over the whole grid
+    from chapsim1ww_amd import props
+    sc = props.scale_table(props.synthetic_table())
+    pt = props.PropertyTable(sc, stream=st)
+    rng2 = np.random.default_rng(4)
+    hh = torch.from_numpy(rng2.uniform(sc["H"][1], sc["H"][-2], g.ncell)).to(dev)
+    outs = [torch.empty_like(hh) for _ in range(5)]
+    for _ in range(a.ntdma):
+        pt.update(hh, *outs)
+    pt.close()
     # calibration: 1 GiB device-to-device copy
     x = torch.empty(1 << 27, dtype=torch.float64, device=dev)
     y = torch.empty_like(x)

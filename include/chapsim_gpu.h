@@ -191,8 +191,10 @@ int chs_energy_substep(chs_handle *h, chs_props *p, const chs_thermal_fields *f,
  * The reference has no surrogate: these entries train and certify the network
  * that chs_therm_prop_update(..., CHS_PROPS_SURROGATE, ...) runs in place of
  * the table where certified.  Network: h -> (T, rho, mu, kappa, c_p),
- * 1 -> 64 -> 64 -> 5, tanh, BF16 weights, FP32 accumulation; parameters are a
- * flat float vector of 4613: w1[64] b1[64] W2[64][64] b2[64] W3[5][64] b3[5].
+ * 1 -> 64 -> 64 -> 5, tanh; weights held as pairs of BF16 values (hi + lo),
+ * activations split the same way, BF16 matrix-core products with FP32
+ * accumulation; parameters are a flat float vector of 4613:
+ * w1[64] b1[64] W2[64][64] b2[64] W3[5][64] b3[5].
  *
  * Train with Adam (cosine learning-rate decay from lr to lr/100) on nsamp
  * device enthalpy samples labelled by the table; batch is a multiple of 64.

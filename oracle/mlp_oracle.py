@@ -1,8 +1,8 @@
 """CPU oracle of the property surrogate (TEST INFRASTRUCTURE ONLY).
 
 A numpy restatement of the network that the device library trains and runs
-for THERM_PROP_UPDATE: 1 -> 64 -> 64 -> 5, tanh, BF16 weights, FP32
-accumulation.  Only tests/ load it.  The reference has no surrogate; what it
+for THERM_PROP_UPDATE: 1 -> 64 -> 64 -> 5, tanh, weights and activations as
+pairs of BF16 values, BF16 products with FP32 accumulation.  Only tests/ load it.  The reference has no surrogate; what it
 computes is fixed by the property table it replaces (the labels), so the
 network's definition below is this build's own and is pinned by
 
@@ -14,17 +14,19 @@ and the device kernels are checked against this oracle.
 
 Parameters (flat float32 vector, NPAR = 4613):
     w1[64] b1[64] W2[64][64] b2[64] W3[5][64] b3[5]      (W[out][in])
-Forward of one sample h (normalisation hc, hs, m[5], s[5]):
+Forward of one sample h (normalisation hc, hs, m[5], s[5]), with bf()
+rounding to bfloat16, hi(v) = bf(v), lo(v) = bf(v - hi(v)), q(w) = hi(w) + lo(w):
     x  = (h - hc) * hs
-    a1 = tanh(bf(w1) * x + b1)
-    a2 = tanh(bf(W2) @ (hi(a1) + lo(a1)) + b2)
-    o  = bf(W3) @ (hi(a2) + lo(a2)) + b3
+    a1 = tanh(q(w1) * x + b1)
+    a2 = tanh(hi(W2) @ (hi(a1) + lo(a1)) + lo(W2) @ hi(a1) + b2)
+    o  = hi(W3) @ (hi(a2) + lo(a2)) + lo(W3) @ hi(a2) + b3
     y  = m + s * o            (T, rho, mu, kappa, c_p)
-with bf() rounding to bfloat16, hi(a) = bf(a), lo(a) = bf(a - hi(a)).
+where W stands for q(W) (the stored weight).
 Loss over a batch of B samples with table values t:
     L = 1/(5B) sum_s sum_k ((y_k - t_k) / t_k)^2
-Gradients are taken with respect to the rounded weights and applied to the
-FP32 master weights (straight-through).
+Gradients are taken with respect to the stored weights q(W) (the backward
+uses q(W) as the weight) and applied to the FP32 master weights
+(straight-through).
 """
 from __future__ import annotations
 
@@ -55,28 +57,42 @@ def split(p):
             p[O_W3:O_B3].reshape(NOUT, H2), p[O_B3:NPAR])
 
 
+def hilo(v):
+    """(hi, lo) bfloat16 pair of float32 values, as float64."""
+    v = np.asarray(v, np.float32)
+    h = bf16(v)
+    return h.astype(np.float64), bf16(v - h).astype(np.float64)
+
+
+def qw(w):
+    """The stored weight q(w) = hi(w) + lo(w) (exact in float32)."""
+    h, l = hilo(w)
+    return h + l
+
+
 def forward(p, norm, h, rounding=True):
     """Network outputs y[B, 5] and the activations (float64 arithmetic on the
-    float32-rounded operands)."""
+    float32 operands)."""
     hc, hs, m, s = norm
-    w1, b1, W2, b2, W3, b3 = (np.asarray(v, np.float64) for v in split(np.asarray(p, np.float32)))
-    rq = (lambda v: bf16(v).astype(np.float64)) if rounding else (lambda v: np.asarray(v, np.float64))
+    w1, b1, W2, b2, W3, b3 = split(np.asarray(p, np.float32))
+    b1, b2, b3 = (np.asarray(v, np.float64) for v in (b1, b2, b3))
     x = ((np.asarray(h, np.float64) - hc) * hs).astype(np.float32).astype(np.float64)
-    a1 = np.tanh(np.outer(x, rq(w1)) + b1)                              # [B, 64]
+
+    def mm(a, W):
+        if not rounding:
+            return a @ np.asarray(W, np.float64).T
+        Wh, Wl = hilo(qw(W))
+        ah, al = hilo(a)
+        return (ah + al) @ Wh.T + ah @ Wl.T
+
+    w1q = qw(w1) if rounding else np.asarray(w1, np.float64)
+    a1 = np.tanh(np.outer(x, w1q) + b1)                                  # [B, 64]
     if rounding:
         a1 = a1.astype(np.float32).astype(np.float64)
-        a1h = bf16(a1).astype(np.float64)
-        a1q = a1h + bf16(a1 - a1h).astype(np.float64)
-    else:
-        a1q = a1
-    a2 = np.tanh(a1q @ rq(W2).T + b2)
+    a2 = np.tanh(mm(a1, W2) + b2)
     if rounding:
         a2 = a2.astype(np.float32).astype(np.float64)
-        a2h = bf16(a2).astype(np.float64)
-        a2q = a2h + bf16(a2 - a2h).astype(np.float64)
-    else:
-        a2q = a2
-    o = a2q @ rq(W3).T + b3
+    o = mm(a2, W3) + b3
     y = np.asarray(m, np.float64) + np.asarray(s, np.float64) * o
     return y, (x, a1, a2, o)
 
@@ -86,7 +102,7 @@ def loss_grad(p, norm, h, t, rounding=True):
     hc, hs, m, s = norm
     p = np.asarray(p, np.float32)
     w1, b1, W2, b2, W3, b3 = split(p)
-    rq = (lambda v: bf16(v).astype(np.float64)) if rounding else (lambda v: np.asarray(v, np.float64))
+    rq = qw if rounding else (lambda v: np.asarray(v, np.float64))
     y, (x, a1, a2, o) = forward(p, norm, h, rounding)
     t = np.asarray(t, np.float64)
     B = len(x)

@@ -62,9 +62,14 @@ def test_table_lookup_at_knots_and_edges(scaled, ptab):
     h = np.concatenate([H[1:-1], np.nextafter(H[1:-1], np.inf), np.nextafter(H[1:-1], -np.inf),
                         [np.nextafter(H[0], np.inf), np.nextafter(H[-1], -np.inf)]])
     bad, ref = _oracle(scaled, h, 0.0)
-    assert bad == 0
+    # one ulp above the first knot, S_HT gives T = T_1 exactly: out of range
+    # for the T splines, as in the reference (and the oracle)
+    assert bad == 1
     outs = [torch.zeros(h.size, dtype=torch.float64, device="cuda") for _ in range(5)]
-    ptab.update(_dev(h), *outs)
+    from chapsim1ww_amd.native import CHS_E_RANGE, ChsError
+    with pytest.raises(ChsError) as ei:
+        ptab.update(_dev(h), *outs)
+    assert ei.value.code == CHS_E_RANGE and "1 cells" in str(ei.value)
     for a, b in zip(outs, ref):
         assert np.array_equal(a.cpu().numpy(), b)
 
