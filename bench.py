@@ -95,6 +95,7 @@ def gpu_bench(steps, warmup):
         prof = s.profile_read()
         s.profile(False)
         iters = s.poisson_stats()
+        state = s.download_state()     # both forms ran the same substep sequence from the same start
         # Poisson solves/s on the handle's workspace
         f = torch.from_numpy(np.asfortranarray(fields.poisson_rhs(g, seed=9)).ravel(order="F").copy()).to(dev)
         dph = torch.zeros(g.L * (g.M + 2) * g.N, dtype=torch.float64, device=dev)
@@ -113,7 +114,7 @@ def gpu_bench(steps, warmup):
         pois_ms = p0.elapsed_time(p1) / npois
         res[prec] = dict(ms_per_substep=ms, wall_s=wall, maxdiv=div, dt=dt, prof=prof,
                          poisson_ms=pois_ms, poisson_refine=s.poisson_stats(), substep_refine=iters,
-                         cn={ns: s.cn_info(ns, dt) for ns in (1, 2, 3)})
+                         cn={ns: s.cn_info(ns, dt) for ns in (1, 2, 3)}, state=state)
         s.close()
         del f, dph
 
@@ -146,7 +147,95 @@ def gpu_bench(steps, warmup):
                                                       gbs=40.0 * n * nl / (ms_t * 1e-3) / 1e9)
     res["tdma"] = tdma
     res["ncell"] = ncell
+    res["props"] = props_bench(g, res["fp64"]["state"][0], stream, steps)
     return res
+
+
+def props_bench(g, q_fp64, stream, steps):
+    """THERM_PROP_UPDATE (table and surrogate) over the 256x192x256 grid and
+    the energy substep, on the synthetic heated-channel table."""
+    import torch
+    from chapsim1ww_amd import props
+    from chapsim1ww_amd.energy import ThermalState
+    from chapsim1ww_amd.solver import ChannelSolver
+    dev = torch.device("cuda:0")
+    sc = props.scale_table(props.synthetic_table())
+    pt = props.PropertyTable(sc, stream=stream)
+    n = g.ncell
+    # enthalpy field from the FP64 state's streamwise velocity (heated-wall map)
+    h_host = props.snapshot_enthalpy(q_fp64[:, 1:-1, :, 0], sc, n, seed=7, uniform_share=0.0)
+    h = torch.from_numpy(h_host).to(dev)
+    outs = [torch.empty_like(h) for _ in range(5)]
+    out = {}
+
+    def timed(fn, reps):
+        fn()
+        _sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        _sync()
+        return e0.elapsed_time(e1) / reps
+
+    reps = max(10, steps)
+    ms_t = timed(lambda: pt.update(h, *outs), reps)
+    out["table"] = dict(ms=round(ms_t, 4), cells_per_s=round(n / (ms_t * 1e-3), 1),
+                        gbs=round(48.0 * n / (ms_t * 1e-3) / 1e9, 1))
+    ref = [o.clone() for o in outs]
+    # surrogate: trained on samples of this snapshot (plus a uniform share over its range)
+    samples = torch.from_numpy(props.snapshot_enthalpy(q_fp64[:, 1:-1, :, 0], sc, 1 << 20, seed=3)).to(dev)
+    t0 = time.perf_counter()
+    pt.train(samples, 0, seed=11)
+    loss = pt.train(samples, 40000, 16384, 2e-3, seed=11)
+    _sync()
+    t_train = time.perf_counter() - t0
+    cert = {tol: pt.certify(tol, 2048) for tol in (1e-3, 1e-4)}     # the 1e-4 certificate stays in force
+    nf = pt.update(h, *outs, mode="surrogate")
+    err = max(float(((o - r).abs() / r.abs()).max()) for o, r in zip(outs, ref))
+    ms_s = timed(lambda: pt.update(h, *outs, mode="surrogate"), reps)
+    # the network alone on every cell (what a fully certified field would cost)
+    net_out = torch.empty(5 * n, dtype=torch.float64, device=dev)
+    ms_n = timed(lambda: pt.eval(h, net_out), max(3, reps // 3))
+    del net_out
+    out["surrogate"] = {"ms": round(ms_s, 4), "gain_vs_table": round(ms_t / ms_s, 3), "train_s": round(t_train, 2),
+                        "train_steps": 40000, "final_loss": loss, "certified_fraction_tol1e-4": cert[1e-4],
+                        "certified_fraction_tol1e-3": cert[1e-3], "fallback_fraction": round(nf / n, 4),
+                        "max_rel_err_vs_table": err, "network_only_ms": round(ms_n, 4),
+                        "network_tflops": round(8960.0 * 3 / 2 * n / (ms_n * 1e-3) / 1e12, 2)}
+    # energy substep (iso-flux walls, q_w = 0.1, Pr = 1, Re = 2800) on this grid
+    s = ChannelSolver(g, stream=stream)
+    ts = ThermalState(s, pt)
+    L, M, N = g.L, g.M, g.N
+    yc = np.concatenate([[-1.0], np.asarray(g.ycc), [1.0]])
+    xx = np.arange(L) / L * 2 * np.pi
+    zz = np.arange(N) / N * 2 * np.pi
+    T3 = 1.2 - 0.3 * yc[None, :, None] ** 2 + 0.02 * np.sin(xx)[:, None, None] * np.cos(zz)[None, None, :]
+    hg = torch.from_numpy(np.interp(T3.ravel(order="F"), sc["T"], sc["H"])).to(dev)
+    f = ts.dev
+    f["enthalpy"].copy_(hg)
+    pt.update(hg, f["temperature"], f["density"], f["viscosity"], f["thermcondt"])
+    f["rhoh"].copy_(hg * f["density"])
+    for k in ("density0", "density1"):
+        f[k].copy_(f["density"])
+    f["viscosity0"].copy_(f["viscosity"])
+    qd = torch.from_numpy(np.asfortranarray(q_fp64).ravel(order="F")).to(dev)
+    f["g"].copy_(qd)
+    f["g0"].copy_(qd)
+    f["wallflux"].fill_(0.1)
+    seq = [1, 2, 3]
+    cnt = [0]
+
+    def eng():
+        ts.solverrk3_eng_io(seq[cnt[0] % 3], 1e-3, 1.0 / 2800.0)
+        cnt[0] += 1
+    ms_e = timed(eng, reps)
+    out["energy_substep"] = dict(ms=round(ms_e, 4), substeps_per_s=round(1e3 / ms_e, 2),
+                                 note="SOLVERRK3_ENG_IO: RHS + RK3 + property cascade + iso-flux walls, synchronous")
+    s.close()
+    pt.close()
+    return out
 
 
 def cpu_baseline(max_seconds=30.0):
@@ -180,6 +269,18 @@ def cpu_baseline(max_seconds=30.0):
         if el2 > max_seconds / 4 or m >= 5:
             break
     return dict(substeps_per_s=sub_rate, nsub=n, poisson_per_s=m / el2, npois=m, cores=O.num_threads())
+
+
+def mixed_errors(a, b):
+    """Max-norm relative difference of the mixed form's state from the FP64
+    form's after the same substeps, per field, and of the xz-mean u profile."""
+    (qa, pa, _), (qb, pb, _) = a, b
+    rel = lambda x, y: float(np.abs(x - y).max() / max(np.abs(x).max(), 1e-300))
+    out = {n: rel(qa[:, 1:-1, :, c], qb[:, 1:-1, :, c]) for c, n in enumerate("uvw")}
+    out["p"] = rel(pa[:, 1:-1, :], pb[:, 1:-1, :])
+    ua, ub = qa[:, 1:-1, :, 0].mean(axis=(0, 2)), qb[:, 1:-1, :, 0].mean(axis=(0, 2))
+    out["u_mean_profile"] = float(np.abs(ua - ub).max())
+    return {k: float(f"{v:.3e}") for k, v in out.items()}
 
 
 def main():
@@ -242,12 +343,14 @@ def main():
         "max_div_after_timed_steps": f64["maxdiv"],
         "dt": f64["dt"],
         "cn_solver": f64["cn"],
-        "mixed": {"substeps_per_s": round(1e3 / mixed["ms_per_substep"], 3),
+        "mixed": {"err_vs_fp64": mixed_errors(f64["state"], mixed["state"]),
+                  "substeps_per_s": round(1e3 / mixed["ms_per_substep"], 3),
                   "ms_per_substep": round(mixed["ms_per_substep"], 4),
                   "speedup_vs_fp64": round(f64["ms_per_substep"] / mixed["ms_per_substep"], 3),
                   "poisson_solves_per_s": round(1e3 / mixed["poisson_ms"], 2),
                   "poisson_refine_iters_rel": mixed["poisson_refine"],
                   "max_div_after_timed_steps": mixed["maxdiv"]},
+        "properties_256x192x256": res["props"],
         "tdma_65536x192": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res["tdma"].items()},
         "kernels_fp64": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kern.items()},
         "roofline": roof,

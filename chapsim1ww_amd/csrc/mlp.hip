@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(ST) k_mlp_props(PropTab t, const float* __rest
 // host
 // ---------------------------------------------------------------------------
 int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap,
-                       double* cp, unsigned long long* cnt);
+                       double* cp, unsigned long long* cnt, int count_lookups = 0);
 
 void mlp_release(chs_props* p) {
     if (!p || !p->mlp) return;
@@ -572,6 +572,8 @@ int mlp_update(chs_props* p, const double* h, Rows rw, double* T, double* rho, d
     CHS_CHECK(p->mlp && p->mlp->trained && p->mlp->cert, CHS_E_ARG,
               "chs_therm_prop_update: the surrogate is not trained and certified");
     const Mlp* m = p->mlp;
+    // no certified bin: every cell takes the table, so run the table kernel
+    if (m->cert_frac == 0.0) return props_table_launch(p, h, rw, T, rho, mu, kap, cp, cnt, 1);
     const long long chunk = (long long)ST / 4 * SPASS;
     const long long nch = ((rw.rowlen + chunk - 1) / chunk) * rw.nrow;
     if (nch == 0) return CHS_OK;

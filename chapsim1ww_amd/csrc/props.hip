@@ -76,7 +76,8 @@ constexpr int PCELL = 4;     // cells per thread and chunk
 __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const double* __restrict__ h,
                                                     double* __restrict__ To, double* __restrict__ rho,
                                                     double* __restrict__ mu, double* __restrict__ kap,
-                                                    double* __restrict__ cp, unsigned long long* __restrict__ cnt) {
+                                                    double* __restrict__ cp, unsigned long long* __restrict__ cnt,
+                                                    int count_lookups) {
     extern __shared__ unsigned char smem[];
     double* xh = reinterpret_cast<double*>(smem);
     double* xt = xh + t.n;
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
     const long long chunk = (long long)PT * PCELL;
     const long long cpr = (rw.rowlen + chunk - 1) / chunk;   // chunks per row
     const long long nch = cpr * rw.nrow;
-    unsigned long long bad = 0;
+    unsigned long long bad = 0, looked = 0;
     for (long long ch = blockIdx.x; ch < nch; ch += gridDim.x) {
         const long long r = ch / cpr;                          // block-uniform
         const long long c0 = (ch - r * cpr) * chunk;
@@ -104,6 +105,7 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
             const long long q = base + c;
             const double hv = h[q];
             if (fabs(hv) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells as they are
+            ++looked;
             double T, D, Mu, K, CP;
             if (!lookup_cell(t, xh, xt, sh, st, hv, T, D, Mu, K, CP)) {
                 ++bad;
@@ -116,19 +118,25 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
             if (cp) cp[q] = CP;
         }
     }
-    // out-of-range count: one atomic per wave that saw any
-    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
-    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(cnt, bad);
+    // out-of-range count (and, on request, the lookups): one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) {
+        bad += __shfl_xor(bad, o);
+        looked += __shfl_xor(looked, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (bad) atomicAdd(cnt, bad);
+        if (count_lookups && looked) atomicAdd(cnt + 1, looked);
+    }
 }
 
 int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap,
-                       double* cp, unsigned long long* cnt) {
+                       double* cp, unsigned long long* cnt, int count_lookups) {
     const long long chunk = (long long)PT * PCELL;
     const long long nch = ((rw.rowlen + chunk - 1) / chunk) * rw.nrow;
     if (nch == 0) return CHS_OK;
     const int grid = (int)std::min<long long>(nch, 512);   // two resident blocks per CU
     CHS_LAUNCH(k_props_table, dim3(grid), dim3(PT), props_lds_bytes(p), p->stream, tab_of(p), rw, h, T, rho, mu,
-               kap, cp, cnt);
+               kap, cp, cnt, count_lookups);
     return CHS_OK;
 }
 
@@ -263,7 +271,7 @@ int props_update(chs_props* p, int mode, const double* h, long long nrow, long l
     Rows rw{nrow, rowlen, nrow > 1 ? stride : rowlen};
     CHS_LAUNCH(k_zero_u64, dim3(1), dim3(64), 0, p->stream, p->counters, 4);
     if (mode == CHS_PROPS_TABLE)
-        CHS_TRY(props_table_launch(p, h, rw, T, rho, mu, kap, cp, p->counters));
+        CHS_TRY(props_table_launch(p, h, rw, T, rho, mu, kap, cp, p->counters, 0));
     else
         CHS_TRY(mlp_update(p, h, rw, T, rho, mu, kap, cp, p->counters));
     unsigned long long host[2] = {0, 0};

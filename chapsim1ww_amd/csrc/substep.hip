@@ -43,7 +43,8 @@ __device__ __forceinline__ int cidx(const Geo& g, int i, int j, int k) {
 }
 
 constexpr int KR = 4;        // k rows per k_rhs_x block (segmented x solve)
-constexpr int KRS = 4;       // k rows per k_rhs_x block (stencil x solve)
+constexpr int KRS = 4;       // k rows per k_rhs_xw block
+constexpr int KRS8 = 8;      // the alternative tile height (instantiated, not selected)
 constexpr int XLPB = 16;     // line slots of the x solve (3*KR = 12 used)
 constexpr int XMS = 8;       // rows per segment of the x solve
 
@@ -281,9 +282,9 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
 // of its constant cyclic system: a periodic (2KX+1)-point stencil (weights
 // zero-padded to the window).  Rows sit in LDS with KX halo entries each side
 // (row_pos layout); each thread forms 4 consecutive outputs from one register
-// window of 2KX+4 entries.  Block = 256 threads, rows (j, KRS consecutive k).
+// window of 2KX+4 entries.  Block = 256 threads, rows (j, KR consecutive k).
 // ---------------------------------------------------------------------------
-template <int KX>
+template <int KX, int KR>
 __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
                                                const double* __restrict__ PR, double* __restrict__ H0,
                                                double* __restrict__ RHS, double* __restrict__ part,
@@ -293,15 +294,15 @@ __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, 
     const int L = g.L;
     const int LS = L + 2 * KX;
     const int Q4 = LS >> 2;
-    double* red = sw + 3 * KRS * LS;
-    const int nkb = g.N / KRS;
+    double* red = sw + 3 * KR * LS;
+    const int nkb = g.N / KR;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const bool active = t < g.M * nkb;
     const int kg = active ? t / g.M : 0;
     const int j = active ? t - kg * g.M : 0;
-    const int k0 = kg * KRS;
+    const int k0 = kg * KR;
     double msum = 0.0;
-    if (active) msum = rhs_rows<KX>(g, rk, cvisc, Q, PR, H0, sw, LS, KRS, lgL, j, k0);
+    if (active) msum = rhs_rows<KX>(g, rk, cvisc, Q, PR, H0, sw, LS, KR, lgL, j, k0);
     const double bs = block_sum(msum, red);  // its barrier also publishes the LDS rows
     if (threadIdx.x == 0 && active) part[t] = bs;
     if (!active) return;
@@ -310,9 +311,9 @@ __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, 
     for (int m = 0; m < W; ++m) w[m] = wx[m];
     const int lg4 = lgL - 2;
     const int LMN = L * g.M * g.N;
-    for (int e = threadIdx.x; e < ((3 * KRS) << lg4); e += blockDim.x) {
+    for (int e = threadIdx.x; e < ((3 * KR) << lg4); e += blockDim.x) {
         const int cr = e >> lg4, g4 = e & ((1 << lg4) - 1);
-        const int c = cr / KRS, r = cr - c * KRS;
+        const int c = cr / KR, r = cr - c * KR;
         if (c == 1 && j == 0) continue;
         // padded elements 4*g4 + t, t = 0 .. W+2: sub-row t % 4, entry g4 + t / 4
         const double* row = sw + cr * LS + g4;
@@ -929,19 +930,33 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const double COE = -facoe * h->dxqi;
         const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
         const int lgL = __builtin_ctz((unsigned)h->L);  // L is a power of two (substep_supported)
-        const int kr = use_st ? KRS : KR;
+        // 4 k rows per block: measured at 256x192x256, 8 rows (fewer k halo
+        // rows per output) ran 1.3% slower (0.452 vs 0.446 ms)
+        const int krs = KRS;
+        const int kr = use_st ? krs : KR;
         const int nblk = h->M * (h->N / kr);
         const dim3 grx(((nblk + 7) / 8) * 8);
         h->timer.begin("rhs_x", s, &ev);
         if (use_st) {
-            const size_t sm = sizeof(double) * (3 * KRS * (size_t)(h->L + 2 * cn->KWx) + 4);
+            const size_t sm = sizeof(double) * (3 * (size_t)krs * (size_t)(h->L + 2 * cn->KWx) + 4);
             const double* wx = cn->dev + cn->ox;
-            switch (cn->KWx) {
-                case 4: CHS_LAUNCH(k_rhs_xw<4>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
-                case 8: CHS_LAUNCH(k_rhs_xw<8>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
-                case 12: CHS_LAUNCH(k_rhs_xw<12>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
-                default: CHS_LAUNCH(k_rhs_xw<16>, grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL); break;
+#define CHS_RHSXW(KX, KR) CHS_LAUNCH((k_rhs_xw<KX, KR>), grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL)
+            if (krs == KRS8) {
+                switch (cn->KWx) {
+                    case 4: CHS_RHSXW(4, KRS8); break;
+                    case 8: CHS_RHSXW(8, KRS8); break;
+                    case 12: CHS_RHSXW(12, KRS8); break;
+                    default: CHS_RHSXW(16, KRS8); break;
+                }
+            } else {
+                switch (cn->KWx) {
+                    case 4: CHS_RHSXW(4, KRS); break;
+                    case 8: CHS_RHSXW(8, KRS); break;
+                    case 12: CHS_RHSXW(12, KRS); break;
+                    default: CHS_RHSXW(16, KRS); break;
+                }
             }
+#undef CHS_RHSXW
         } else {
             const int P = h->L / XMS;
             const int nthr = XLPB * P;

@@ -95,17 +95,23 @@ def scale_table(cols: dict, t0: float = 1.0) -> dict:
 
 
 def snapshot_enthalpy(u: np.ndarray, scaled: dict, n: int, seed: int = 0, t_lo: float = 0.82,
-                      t_hi: float = 1.58) -> np.ndarray:
+                      t_hi: float = 1.58, uniform_share: float = 0.5) -> np.ndarray:
     """Enthalpy samples drawn from a channel-solver snapshot: the streamwise
     velocity u of the snapshot becomes a heated-wall temperature field
     T = t_hi - (t_hi - t_lo) u / max u (slow fluid near the walls is hot),
-    mapped to h through the scaled table; n cells are drawn at random."""
+    mapped to h through the scaled table; cells are drawn at random.  The
+    snapshot leaves the hot near-wall tail thinly sampled, so a share of the
+    n samples (uniform_share) is drawn uniformly over the snapshot's h range."""
     uu = np.asarray(u, np.float64).ravel(order="F")
     umax = np.abs(uu).max()
     T = np.clip(t_hi - (t_hi - t_lo) * np.abs(uu) / umax, t_lo, t_hi)
     rng = np.random.default_rng(seed)
-    Ts = T[rng.integers(0, T.size, n)]
-    return np.interp(Ts, scaled["T"], scaled["H"])
+    nu = int(round(n * uniform_share))
+    Ts = T[rng.integers(0, T.size, n - nu)]
+    hs = np.interp(Ts, scaled["T"], scaled["H"])
+    lo, hi = np.interp([T.min(), T.max()], scaled["T"], scaled["H"])
+    out = np.concatenate([hs, rng.uniform(lo, hi, nu)])
+    return out[rng.permutation(n)]
 
 
 class PropertyTable:

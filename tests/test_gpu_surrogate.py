@@ -95,11 +95,13 @@ def test_train_certify_and_update(scaled, snapshot_h):
     pt = props.PropertyTable(scaled)
     dh = _dev(snapshot_h)
     l0 = pt.train(dh, steps=0, seed=11)
-    l1 = pt.train(dh, steps=6000, batch=4096, lr=3e-3, seed=11)
+    l1 = pt.train(dh, steps=20000, batch=8192, lr=2e-3, seed=11)
     print(f"loss init {l0:.3e} trained {l1:.3e}", flush=True)
-    assert l1 < 1e-3 * l0
+    assert l1 < 1e-4 * l0
+    loose = pt.certify(1e-3, 2048)
     frac = pt.certify(1e-4, 2048)
-    print(f"certified fraction of the trained range: {frac:.4f}", flush=True)
+    print(f"certified fraction of the trained range: {frac:.4f} at 1e-4 ({loose:.4f} at 1e-3)", flush=True)
+    assert loose > 0.0
     par, norm = pt.params()
     # a field: the snapshot distribution plus cells outside the trained range
     rng = np.random.default_rng(9)
