@@ -203,7 +203,11 @@ def props_bench(g, q_fp64, stream, steps):
                         "train_steps": 40000, "final_loss": loss, "certified_fraction_tol1e-4": cert[1e-4],
                         "certified_fraction_tol1e-3": cert[1e-3], "fallback_fraction": round(nf / n, 4),
                         "max_rel_err_vs_table": err, "network_only_ms": round(ms_n, 4),
-                        "network_tflops": round(8960.0 * 3 / 2 * n / (ms_n * 1e-3) / 1e12, 2)}
+                        # model FLOPs (8960 per cell) and the matrix-core FLOPs executed (three BF16
+                        # products per weight block, layer 3 padded to 16 rows: 30720 per cell)
+                        "network_model_tflops": round(8960.0 * n / (ms_n * 1e-3) / 1e12, 2),
+                        "network_mfma_tflops": round(30720.0 * n / (ms_n * 1e-3) / 1e12, 2),
+                        "mfma_peak_bf16_tflops": 2500.0}
     # energy substep (iso-flux walls, q_w = 0.1, Pr = 1, Re = 2800) on this grid
     s = ChannelSolver(g, stream=stream)
     ts = ThermalState(s, pt)

@@ -83,6 +83,177 @@ __global__ void __launch_bounds__(256) k_fft_z(cpx_t<Rl>* __restrict__ S, int N,
     }
 }
 
+// ---------------------------------------------------------------------------
+// 256-point transforms as 16 x 16 (x and z lengths of 256).  A transform is
+// held by 16 threads: pass A gives each thread 16 values x[16 n1 + n2]
+// (n2 = its index) in registers for a 16-point DFT over n1 (4 x 4 radix-4),
+// multiplies by W256^(n2 k1) and writes to LDS transposed (row stride 17
+// against bank conflicts); pass B gives thread k1 the values of its k1 for a
+// 16-point DFT over n2, producing X[k1 + 16 k2].  One LDS exchange per
+// transform instead of one per radix-4 stage, shifts instead of index
+// divisions, and global loads/stores straight from/to registers.
+// ---------------------------------------------------------------------------
+constexpr int F16S = 17;            // LDS row stride of the 16 x 16 exchange
+constexpr int F16T = 16 * F16S;     // LDS complex values per transform
+
+// 16-point DFT in registers, natural order in and out; dir -1 forward, +1 inverse
+template <typename C, typename Rl>
+__device__ __forceinline__ void dft16(C (&v)[16], int dir) {
+    // W16^m = cos(pi m/8) + dir i sin(pi m/8) for the m = n2 k1 that occur
+    const Rl c8 = Rl(0.92387953251128675613), s8 = Rl(0.38268343236508977173), h2 = Rl(0.70710678118654752440);
+    const Rl wc[10] = {Rl(1), c8, h2, s8, Rl(0), Rl(0), -h2, Rl(0), Rl(0), -c8};
+    const Rl ws[10] = {Rl(0), s8, h2, c8, Rl(1), Rl(0), h2, Rl(0), Rl(0), -s8};
+    C y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {   // n = 4 n1 + n2: DFT-4 over n1
+        C t[5] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2], v[0]};
+        dft_small<4, C, Rl>(t, dir);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            const int m = n2 * k1;
+            C z = t[k1];
+            if (m != 0) {
+                const C w = C{wc[m], dir > 0 ? ws[m] : -ws[m]};
+                z = cmul(z, w);
+            }
+            y[4 * n2 + k1] = z;
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {   // DFT-4 over n2: X[k1 + 4 k2]
+        C t[5] = {y[k1], y[4 + k1], y[8 + k1], y[12 + k1], y[0]};
+        dft_small<4, C, Rl>(t, dir);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
+// pass A tail + exchange + pass B for one transform held by 16 threads:
+// v = x[16 n1 + n2] on entry (n2 = q), X[q + 16 k2] on exit.  buf: this
+// transform's F16T values of LDS; all threads of the block must call it.
+template <typename C, typename Rl>
+__device__ __forceinline__ void fft256_regs(C (&v)[16], int q, C* buf, const C* __restrict__ tw, int dir) {
+    dft16<C, Rl>(v, dir);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+        C w = tw[q * k1];   // W256^(n2 k1), n2 k1 <= 225
+        if (dir > 0) w.y = -w.y;
+        buf[k1 * F16S + q] = k1 == 0 ? v[0] : cmul(v[k1], w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = buf[q * F16S + n2];
+    dft16<C, Rl>(v, dir);
+}
+
+// x forward, L = 256: 16 rows (8 complex transforms) per block of 128 threads
+template <typename Rl>
+__global__ void __launch_bounds__(128) k_fft256_x_fwd(const double* __restrict__ f, cpx_t<Rl>* __restrict__ S,
+                                                      int nrows, int LHP, const cpx_t<Rl>* __restrict__ tw) {
+    using C = cpx_t<Rl>;
+    extern __shared__ unsigned char smem_raw[];
+    C* sh = reinterpret_cast<C*>(smem_raw);
+    const int g = threadIdx.x >> 4, q = threadIdx.x & 15;
+    const int ra = blockIdx.x * 16 + 2 * g, rb = ra + 1;
+    C v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int i = 16 * n1 + q;
+        const Rl a = ra < nrows ? (Rl)f[(size_t)ra * 256 + i] : Rl(0);
+        const Rl b = rb < nrows ? (Rl)f[(size_t)rb * 256 + i] : Rl(0);
+        v[n1] = C{a, b};
+    }
+    C* buf = sh + g * F16T;
+    fft256_regs<C, Rl>(v, q, buf, tw, -1);
+    __syncthreads();   // every thread has read its exchange column
+    C* Z = sh + g * 256;   // the transform in natural order (fits inside the exchange space)
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) Z[q + 16 * k2] = v[k2];
+    __syncthreads();
+    // split the two real rows: A = (Z_k + conj Z_{L-k}) / 2, B = (Z_k - conj Z_{L-k}) / (2i)
+    for (int e = threadIdx.x; e < 8 * 129; e += blockDim.x) {
+        const int p = e / 129, kx = e - p * 129;
+        const C z = sh[p * 256 + kx];
+        const C zc = sh[p * 256 + ((256 - kx) & 255)];
+        const C A = C{Rl(0.5) * (z.x + zc.x), Rl(0.5) * (z.y - zc.y)};
+        const C B = C{Rl(0.5) * (z.y + zc.y), Rl(-0.5) * (z.x - zc.x)};
+        const int r0 = blockIdx.x * 16 + 2 * p;
+        if (r0 < nrows) S[(size_t)r0 * LHP + kx] = A;
+        if (r0 + 1 < nrows) S[(size_t)(r0 + 1) * LHP + kx] = B;
+    }
+}
+
+// z transform, N = 256, in place on S: 8 kx columns of one j per block of 128 threads
+template <typename Rl>
+__global__ void __launch_bounds__(128) k_fft256_z(cpx_t<Rl>* __restrict__ S, int M, int LH, int LHP, int dir,
+                                                  const cpx_t<Rl>* __restrict__ tw) {
+    using C = cpx_t<Rl>;
+    extern __shared__ unsigned char smem_raw[];
+    C* sh = reinterpret_cast<C*>(smem_raw);
+    const int c = threadIdx.x & 7, q = threadIdx.x >> 3;
+    const int j = blockIdx.y;
+    const int kx = blockIdx.x * 8 + c;
+    const bool ok = kx < LH;
+    const size_t kst = (size_t)M * LHP;
+    C* col = S + (size_t)j * LHP + (ok ? kx : 0);
+    C v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = ok ? col[(size_t)(16 * n1 + q) * kst] : C{Rl(0), Rl(0)};
+    fft256_regs<C, Rl>(v, q, sh + c * (F16T + 1), tw, dir);
+    if (ok) {
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) col[(size_t)(q + 16 * k2) * kst] = v[k2];
+    }
+}
+
+// x inverse, L = 256: 16 rows per block of 128 threads; ghost as k_fft_x_inv
+template <typename Rl>
+__global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restrict__ S, double* __restrict__ dph,
+                                                      int M, int nrows, int LHP, int ghost,
+                                                      const cpx_t<Rl>* __restrict__ tw) {
+    using C = cpx_t<Rl>;
+    extern __shared__ unsigned char smem_raw[];
+    C* sh = reinterpret_cast<C*>(smem_raw);
+    const int g = threadIdx.x >> 4, q = threadIdx.x & 15;
+    const int ra = blockIdx.x * 16 + 2 * g, rb = ra + 1;
+    C v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int n = 16 * n1 + q;
+        const int m = n <= 128 ? n : 256 - n;
+        C A = ra < nrows ? S[(size_t)ra * LHP + m] : C{Rl(0), Rl(0)};
+        C B = rb < nrows ? S[(size_t)rb * LHP + m] : C{Rl(0), Rl(0)};
+        if (m == 0 || m == 128) {   // b_0 = b_{L/2} = 0 (FFT99 ordering)
+            A.y = Rl(0);
+            B.y = Rl(0);
+        }
+        // Z[m] = A + iB; Z[L-m] = conj(A) + i conj(B)
+        v[n1] = n <= 128 ? C{A.x - B.y, A.y + B.x} : C{A.x + B.y, B.x - A.y};
+    }
+    fft256_regs<C, Rl>(v, q, sh + g * F16T, tw, +1);
+    const size_t plane = (size_t)256 * (M + 2);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int row = h ? rb : ra;
+        if (row >= nrows) continue;
+        const int jj = row % M, k = row / M;
+        double* base = dph + (size_t)k * plane + q;     // column q of plane k
+        double* rowp = base + (size_t)(jj + 1) * 256;
+        const bool lo = jj == 0, hi = jj == M - 1;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            const double val = h ? (double)v[k2].y : (double)v[k2].x;
+            if (ghost == 2) {
+                rowp[16 * k2] += val;
+                continue;
+            }
+            rowp[16 * k2] = val;
+            if (lo) base[16 * k2] = ghost == 1 ? val : 0.0;
+            if (hi) base[(size_t)(M + 1) * 256 + 16 * k2] = ghost == 1 ? val : 0.0;
+        }
+    }
+}
+
 // ---- y: tridiagonal solve per (kx, kz) line, FP64 --------------------------
 // Matrix of TDMA_PHI_COEF (src/possion3d_FFT99.f90:185-220):
 //   a_j = AMPH(j), c_j = APPH(j), b_j = ACPH(j) + (-AK1(kx)/RCCI2(j) - AK3(kz)),
@@ -277,13 +448,25 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
     const dim3 gx((nrows + XROWS - 1) / XROWS);
     const size_t smx = sizeof(cpx_t<Rl>) * (size_t)XROWS * L;  // two buffers of XROWS/2 vectors
     hipEvent_t ev;
+    // 256-point lengths use the 16 x 16 register transforms
+    const bool x256 = L == 256, z256 = N == 256;
+    const dim3 gx16((nrows + 15) / 16);
+    const size_t sm16x = sizeof(cpx_t<Rl>) * (size_t)8 * F16T;
+    const size_t sm16z = sizeof(cpx_t<Rl>) * (size_t)8 * (F16T + 1);
+    const dim3 gz16((LH + 7) / 8, M);
     h->timer.begin("fft_x_fwd", s, &ev);
-    CHS_LAUNCH(k_fft_x_fwd<Rl>, gx, dim3(256), smx, s, f, S, L, nrows, LHP, px, twx);
+    if (x256)
+        CHS_LAUNCH(k_fft256_x_fwd<Rl>, gx16, dim3(128), sm16x, s, f, S, nrows, LHP, twx);
+    else
+        CHS_LAUNCH(k_fft_x_fwd<Rl>, gx, dim3(256), smx, s, f, S, L, nrows, LHP, px, twx);
     h->timer.end(ev, "fft_x_fwd", s);
     const dim3 gz((LH + ZTX - 1) / ZTX, M);
     const size_t smz = sizeof(cpx_t<Rl>) * (size_t)2 * ZTX * N;
     h->timer.begin("fft_z", s, &ev);
-    CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, -1, pz, twz);
+    if (z256)
+        CHS_LAUNCH(k_fft256_z<Rl>, gz16, dim3(128), sm16z, s, S, M, LH, LHP, -1, twz);
+    else
+        CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, -1, pz, twz);
     h->timer.end(ev, "fft_z", s);
     int MS, P, LPB;
     if (!tdma_seg_config(M, false, &MS, &P, &LPB, 768)) {
@@ -303,10 +486,16 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
     }
     h->timer.end(ev, "poisson_y", s);
     h->timer.begin("fft_z", s, &ev);
-    CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, +1, pz, twz);
+    if (z256)
+        CHS_LAUNCH(k_fft256_z<Rl>, gz16, dim3(128), sm16z, s, S, M, LH, LHP, +1, twz);
+    else
+        CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, +1, pz, twz);
     h->timer.end(ev, "fft_z", s);
     h->timer.begin("fft_x_inv", s, &ev);
-    CHS_LAUNCH(k_fft_x_inv<Rl>, gx, dim3(256), smx, s, S, dph, L, M, nrows, LHP, ghost, px, twx);
+    if (x256)
+        CHS_LAUNCH(k_fft256_x_inv<Rl>, gx16, dim3(128), sm16x, s, S, dph, M, nrows, LHP, ghost, twx);
+    else
+        CHS_LAUNCH(k_fft_x_inv<Rl>, gx, dim3(256), smx, s, S, dph, L, M, nrows, LHP, ghost, px, twx);
     h->timer.end(ev, "fft_x_inv", s);
     return CHS_OK;
 }
