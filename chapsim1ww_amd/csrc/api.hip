@@ -19,7 +19,7 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
 bool substep_supported(const chs_handle* h);
 void release_stencils(const chs_handle* h);
 int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky);
-bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr);
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small = false);
 
 void KernelTimer::begin(const char* name, hipStream_t s, hipEvent_t* out_a) {
     (void)name;

@@ -434,7 +434,7 @@ __global__ void k_zero(double* __restrict__ p, size_t n) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr);
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small = false);
 
 template <typename Rl>
 static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, cpx_t<Rl>* S, const cpx_t<Rl>* twx,
@@ -469,7 +469,7 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
         CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, -1, pz, twz);
     h->timer.end(ev, "fft_z", s);
     int MS, P, LPB;
-    if (!tdma_seg_config(M, false, &MS, &P, &LPB, 768)) {
+    if (!tdma_seg_config(M, false, &MS, &P, &LPB, 768, true)) {
         set_error("poisson: unsupported NCL2");
         return CHS_E_ARG;
     }

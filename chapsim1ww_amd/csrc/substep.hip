@@ -476,7 +476,7 @@ constexpr int KMAX = 16;
 // window R(j-KW..j+KW) slides along j in registers and every lane of a wave
 // (64 consecutive i) applies the same row weights W_c[j][.] (scalar loads).
 // wy rows are zero-padded to 2KW+1 entries (zero outside [0, M)).
-constexpr int YJC = 32;     // rows per thread chunk of the y window
+constexpr int YJC = 64;     // rows per thread chunk of the y window (measured: 32 -> 64 rows, 257 -> 240 us)
 constexpr int WU = 4;       // rows fetched ahead per step of the windows
 template <int KW>
 __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
@@ -711,7 +711,7 @@ __global__ void k_reduce_max2_nan(const double* __restrict__ part, int nb, doubl
 // ---------------------------------------------------------------------------
 // host
 // ---------------------------------------------------------------------------
-bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr);
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small = false);
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
 
 static Geo geo_of(const chs_handle* h) {

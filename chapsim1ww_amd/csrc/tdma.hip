@@ -131,10 +131,16 @@ __global__ void k_tdma_seq(const double* __restrict__ A, const double* __restric
 
 // Choose the segmented configuration for (n, cyclic) within maxthr threads per
 // block, or return false.  8-row segments keep the solver within 128 VGPRs.
-bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr) {
+// lpb_small: blocks of 8 lines (several blocks per CU overlap their load,
+// reduce and store phases; measured faster for the complex-valued Poisson
+// lines, slower for the real batched solver)
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small) {
     const int cand[4] = {8, 12, 16, 4};
+    const int lpbs_big[2] = {64, 32}, lpbs_small[2] = {8, 8};
+    const int* lpbs = lpb_small ? lpbs_small : lpbs_big;
     for (int ms : cand) {
-        for (int lpb : {64, 32}) {
+        for (int q = 0; q < 2; ++q) {
+            const int lpb = lpbs[q];
             const int p = (n + ms - 1) / ms;
             if (p * lpb > maxthr) continue;
             // cyclic lines cannot be padded (the wrap couples the last real row)
@@ -151,7 +157,7 @@ bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr) 
 int tdma_launch(const double* a, const double* b, const double* c, double* r, const double* bc, int n, int nlines,
                 long long es, long long ls, int cyclic, hipStream_t s) {
     int MS = 0, P = 0, LPB = 0;
-    if (tdma_seg_config(n, cyclic != 0, &MS, &P, &LPB, 1024)) {
+    if (tdma_seg_config(n, cyclic != 0, &MS, &P, &LPB, 1024, false)) {
         const dim3 grid((nlines + LPB - 1) / LPB), block(LPB * P);
         const size_t sm = seg_smem_bytes<double>(LPB, P);
         switch (MS) {

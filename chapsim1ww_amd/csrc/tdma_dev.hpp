@@ -146,6 +146,69 @@ __device__ inline void reduced_cyclic(double* A, const double* B, double* C, V* 
     D[(n - 1) * st] = xn;
 }
 
+// Step 2c: non-cyclic reduced system by parallel cyclic reduction: every
+// segment thread keeps its two rows in registers; each of the ceil(log2 R)
+// steps eliminates the couplings at distance s from both neighbours, after
+// which every row stands alone.  Replaces the serial sweep of one thread per
+// line (2R dependent steps) by ~log2 R steps over all threads of the block.
+// All threads must call it (barriers).  Returns this thread's two unknowns.
+template <typename V>
+__device__ __forceinline__ void reduced_pcr(double* sA, double* sB, double* sC, V* sD, int R, int LPB, int ll,
+                                            int seg, V& x0, V& x1) {
+    double a[2], b[2], c[2];
+    V d[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int m = (2 * seg + q) * LPB + ll;
+        a[q] = sA[m];
+        b[q] = sB[m];
+        c[q] = sC[m];
+        d[q] = sD[m];
+    }
+    for (int st = 1; st < R; st <<= 1) {
+        double na[2], nb[2], nc[2];
+        V nd[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int r = 2 * seg + q;
+            na[q] = 0.0;
+            nc[q] = 0.0;
+            nb[q] = b[q];
+            nd[q] = d[q];
+            if (r - st >= 0) {
+                const int m = (r - st) * LPB + ll;
+                const double al = -a[q] / sB[m];
+                na[q] = al * sA[m];
+                nb[q] = nb[q] + al * sC[m];
+                nd[q] = vfma(nd[q], al, sD[m]);
+            }
+            if (r + st < R) {
+                const int m = (r + st) * LPB + ll;
+                const double ga = -c[q] / sB[m];
+                nc[q] = ga * sC[m];
+                nb[q] = nb[q] + ga * sA[m];
+                nd[q] = vfma(nd[q], ga, sD[m]);
+            }
+        }
+        __syncthreads();   // every thread has read the rows of this step
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int m = (2 * seg + q) * LPB + ll;
+            a[q] = na[q];
+            b[q] = nb[q];
+            c[q] = nc[q];
+            d[q] = nd[q];
+            sA[m] = na[q];
+            sB[m] = nb[q];
+            sC[m] = nc[q];
+            sD[m] = nd[q];
+        }
+        __syncthreads();
+    }
+    x0 = vscale(d[0], 1.0 / b[0]);
+    x1 = vscale(d[1], 1.0 / b[1]);
+}
+
 // LDS bytes needed by seg_solve for LPB lines and P segments.
 template <typename V>
 __host__ __device__ constexpr size_t seg_smem_bytes(int LPB, int P) {
@@ -171,15 +234,15 @@ __device__ __forceinline__ void seg_solve(double (&a)[MS], double (&b)[MS], doub
     sA[r0 * LPB + ll] = A0; sB[r0 * LPB + ll] = B0; sC[r0 * LPB + ll] = C0; sD[r0 * LPB + ll] = D0;
     sA[r1 * LPB + ll] = A1; sB[r1 * LPB + ll] = B1; sC[r1 * LPB + ll] = C1; sD[r1 * LPB + ll] = D1;
     __syncthreads();
-    if (seg == 0 && valid) {
-        if (cyclic)
-            reduced_cyclic<V>(sA + ll, sB + ll, sC + ll, sD + ll, R, LPB);
-        else
-            reduced_thomas<V>(sA + ll, sB + ll, sC + ll, sD + ll, R, LPB);
+    V F, E;
+    if (!cyclic) {
+        reduced_pcr<V>(sA, sB, sC, sD, R, LPB, ll, seg, F, E);
+    } else {
+        if (seg == 0 && valid) reduced_cyclic<V>(sA + ll, sB + ll, sC + ll, sD + ll, R, LPB);
+        __syncthreads();
+        F = sD[r0 * LPB + ll];
+        E = sD[r1 * LPB + ll];
     }
-    __syncthreads();
-    const V F = sD[r0 * LPB + ll];
-    const V E = sD[r1 * LPB + ll];
     seg_reconstruct<MS, V>(d, be, ga, F, E);
 }
 
