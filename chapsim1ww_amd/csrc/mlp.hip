@@ -326,15 +326,28 @@ __global__ void __launch_bounds__(256) k_mlp_train(const float* __restrict__ wq,
     }
 }
 
-// sum of the block partials (fixed order) -> gradient (and loss); Adam step
-// when lr > 0; wq <- q(w) for weights, FP32 biases
-__global__ void k_mlp_adam(const float* __restrict__ partial, int nblk, float* __restrict__ par,
-                           float* __restrict__ am, float* __restrict__ av, float* __restrict__ wq,
-                           float* __restrict__ grad_out, float lr, float b1t, float b2t) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > NPAR) return;
+// sum of the block partials -> gradient (and loss); Adam step when lr > 0;
+// wq <- q(w) for weights, FP32 biases.  Block = 32 parameters x 8 threads:
+// thread s of a parameter sums partials s, s+8, ..., then the 8 sums are
+// added in a fixed order (deterministic).
+constexpr int AD_P = 32, AD_S = 8;
+__global__ void __launch_bounds__(AD_P * AD_S) k_mlp_adam(const float* __restrict__ partial, int nblk,
+                                                         float* __restrict__ par, float* __restrict__ am,
+                                                         float* __restrict__ av, float* __restrict__ wq,
+                                                         float* __restrict__ grad_out, float lr, float b1t,
+                                                         float b2t) {
+    __shared__ double red[AD_S][AD_P];
+    const int pl = threadIdx.x % AD_P, sl = threadIdx.x / AD_P;
+    const int p = blockIdx.x * AD_P + pl;
+    double gs = 0.0;
+    if (p <= NPAR)
+        for (int b = sl; b < nblk; b += AD_S) gs += partial[(size_t)b * (NPAR + 1) + p];
+    red[sl][pl] = gs;
+    __syncthreads();
+    if (sl != 0 || p > NPAR) return;
     double g = 0.0;
-    for (int b = 0; b < nblk; ++b) g += partial[(size_t)b * (NPAR + 1) + p];
+#pragma unroll
+    for (int q = 0; q < AD_S; ++q) g += red[q][pl];
     if (grad_out) grad_out[p] = (float)g;
     if (p == NPAR || lr <= 0.0f) return;
     const float gf = (float)g;
@@ -692,7 +705,7 @@ static int train(chs_props* p, const double* h, long long nsamp, int steps, int 
         b2t = std::pow(0.999, (double)m->step);
         CHS_LAUNCH(k_mlp_train, dim3(nblk), dim3(256), lds, p->stream, m->wq, m->norm, pool.px, pool.pt, nsamp,
                    (long long)batch, seed, m->step, 1, 1.0 / (5.0 * batch), m->partial);
-        CHS_LAUNCH(k_mlp_adam, dim3((NPAR + 1 + 255) / 256), dim3(256), 0, p->stream, m->partial, nblk, m->par, m->am,
+        CHS_LAUNCH(k_mlp_adam, dim3((NPAR + 1 + AD_P - 1) / AD_P), dim3(AD_P * AD_S), 0, p->stream, m->partial, nblk, m->par, m->am,
                    m->av, m->wq, (float*)nullptr, (float)lr, (float)b1t, (float)b2t);
     }
     // loss over the whole pool with the final weights
@@ -736,7 +749,7 @@ static int grad(chs_props* p, const double* h, long long n, float* grad_host, do
     CHS_HIP(hipMalloc((void**)&g, sizeof(float) * (NPAR + 1)));
     hipLaunchKernelGGL(k_mlp_train, dim3(nblk), dim3(256), sizeof(TrainLds) + sizeof(WLds), p->stream, m->wq, m->norm, pool.px,
                        pool.pt, n, n, 0ull, 0ll, 0, 1.0 / (5.0 * n), m->partial);
-    hipLaunchKernelGGL(k_mlp_adam, dim3((NPAR + 1 + 255) / 256), dim3(256), 0, p->stream, m->partial, nblk, m->par,
+    hipLaunchKernelGGL(k_mlp_adam, dim3((NPAR + 1 + AD_P - 1) / AD_P), dim3(AD_P * AD_S), 0, p->stream, m->partial, nblk, m->par,
                        m->am, m->av, m->wq, g, 0.0f, 0.0f, 0.0f);
     std::vector<float> gh(NPAR + 1);
     hipError_t e = hipGetLastError();

@@ -95,19 +95,25 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
     const long long nch = cpr * rw.nrow;
     unsigned long long bad = 0, looked = 0;
     for (long long ch = blockIdx.x; ch < nch; ch += gridDim.x) {
-        const long long r = ch / cpr;                          // block-uniform
+        const long long r = rw.nrow == 1 ? 0 : ch / cpr;       // block-uniform
         const long long c0 = (ch - r * cpr) * chunk;
         const long long base = r * rw.stride;
+        // all loads of h first (independent), then the lookups
+        double hv[PCELL];
+        bool in[PCELL];
 #pragma unroll
         for (int u = 0; u < PCELL; ++u) {
             const long long c = c0 + u * PT + threadIdx.x;
-            if (c >= rw.rowlen) continue;
-            const long long q = base + c;
-            const double hv = h[q];
-            if (fabs(hv) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells as they are
+            in[u] = c < rw.rowlen;
+            hv[u] = in[u] ? h[base + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PCELL; ++u) {
+            if (!in[u] || fabs(hv[u]) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells as they are
             ++looked;
+            const long long q = base + c0 + u * PT + threadIdx.x;
             double T, D, Mu, K, CP;
-            if (!lookup_cell(t, xh, xt, sh, st, hv, T, D, Mu, K, CP)) {
+            if (!lookup_cell(t, xh, xt, sh, st, hv[u], T, D, Mu, K, CP)) {
                 ++bad;
                 continue;
             }
