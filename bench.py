@@ -7,8 +7,11 @@ A step is one RK3 substep (SOLVERRK3_MOM_tg) of the tg channel at
 device-resident FP64 state; the timed region holds exactly K substeps
 (NS cycling 1, 2, 3), bracketed by device synchronisation.  The JSON line
 also carries Poisson solves/s and batched-TDMA rates, the mixed-precision
-form, the per-kernel HBM roofline of the dominant kernel (HIP events on the
-library's stream) and the CPU oracle timed on this host.
+form (both forms continued to NS = 0 + 500 RK3 steps, then their fields and
+xz-mean profiles compared), the per-kernel HBM roofline of the dominant
+kernel (HIP events on the library's stream), and the CPU oracle timed on this
+host: a bounded sample at 256x192x256 and the whole 64^3, Re_b = 2800,
+100-step case beside the device path over the same trajectory.
 """
 from __future__ import annotations
 
@@ -46,6 +49,11 @@ def alg_bytes_per_cell(L, precision="fp64"):
             "update": 80.0}          # read phi, f, Q x3, PR; write Q x3, PR
 
 
+def _note(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def _sync():
     import torch
     torch.cuda.synchronize()
@@ -57,7 +65,7 @@ def run_substeps(solver, dt, nsteps, ns0=0):
         solver.solverrk3_mom_tg(seq[(ns0 + i) % 3], dt)
 
 
-def gpu_bench(steps, warmup):
+def gpu_bench(steps, warmup, cmp_steps):
     import torch
     from chapsim1ww_amd import fields
     from chapsim1ww_amd.grid import ChannelGrid
@@ -95,7 +103,16 @@ def gpu_bench(steps, warmup):
         prof = s.profile_read()
         s.profile(False)
         iters = s.poisson_stats()
-        state = s.download_state()     # both forms ran the same substep sequence from the same start
+        # continue both forms to the same substep count (NS = 0, then cmp_steps
+        # RK3 steps) for the mixed-vs-FP64 comparison
+        done = warmup + 2 * steps
+        if 3 * cmp_steps > done:
+            run_substeps(s, dt, 3 * cmp_steps - done, done)
+        nsub = 1 + max(3 * cmp_steps, done)
+        state = s.download_state()
+        means = s.mean_xz()
+        _note(f"{prec}: {ms:.4f} ms/substep, compared after {nsub} substeps")
+        end = dict(substeps=nsub, maxdiv=s.divgck(), cfl=s.cfl() * dt)
         # Poisson solves/s on the handle's workspace
         f = torch.from_numpy(np.asfortranarray(fields.poisson_rhs(g, seed=9)).ravel(order="F").copy()).to(dev)
         dph = torch.zeros(g.L * (g.M + 2) * g.N, dtype=torch.float64, device=dev)
@@ -114,7 +131,8 @@ def gpu_bench(steps, warmup):
         pois_ms = p0.elapsed_time(p1) / npois
         res[prec] = dict(ms_per_substep=ms, wall_s=wall, maxdiv=div, dt=dt, prof=prof,
                          poisson_ms=pois_ms, poisson_refine=s.poisson_stats(), substep_refine=iters,
-                         cn={ns: s.cn_info(ns, dt) for ns in (1, 2, 3)}, state=state)
+                         cn={ns: s.cn_info(ns, dt) for ns in (1, 2, 3)}, state=state,
+                         means=means, end=end)
         s.close()
         del f, dph
 
@@ -147,7 +165,9 @@ def gpu_bench(steps, warmup):
                                                       gbs=40.0 * n * nl / (ms_t * 1e-3) / 1e9)
     res["tdma"] = tdma
     res["ncell"] = ncell
+    _note("tdma done")
     res["props"] = props_bench(g, res["fp64"]["state"][0], stream, steps)
+    _note("properties done")
     return res
 
 
@@ -275,16 +295,80 @@ def cpu_baseline(max_seconds=30.0):
     return dict(substeps_per_s=sub_rate, nsub=n, poisson_per_s=m / el2, npois=m, cores=O.num_threads())
 
 
-def mixed_errors(a, b):
-    """Max-norm relative difference of the mixed form's state from the FP64
-    form's after the same substeps, per field, and of the xz-mean u profile."""
+MEAN_COLS = {"u": 0, "v": 1, "w": 2, "p": 3, "uu": 4, "uv": 5, "vv": 7, "ww": 9}
+
+
+def mean_err(a, b):
+    """Largest difference of two (NCL2, 13) xz-moment arrays, each column
+    scaled by the largest value of its group (<u_i>, <p>, <u_i u_j>,
+    <u_i p>; <v> vanishes analytically and holds only round-off)."""
+    err = 0.0
+    for grp in ([0, 1, 2], [3], list(range(4, 10)), [10, 11, 12]):
+        sc = np.abs(b[:, grp]).max()
+        err = max(err, float(np.abs(a[:, grp] - b[:, grp]).max() / (sc if sc > 0 else 1.0)))
+    return err
+
+
+def small_case(steps=100):
+    """The CPU-runnable channel case (64^3, Re_b = 2800, CFL 0.5, NS = 0 then
+    `steps` RK3 steps): the oracle and the FP64 device path over the same
+    trajectory, each timed, and their difference at the end."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import torch
+    from chapsim1ww_amd import fields
+    from chapsim1ww_amd.grid import ChannelGrid
+    from chapsim1ww_amd.solver import ChannelSolver
+
+    g = ChannelGrid(64, 64, 64, ren=2800.0)
+    og = O.OracleGrid(64, 64, 64, ren=2800.0)
+    q, pr, c0 = fields.init_tg_fields(g, seed=1973)
+    st = O.OracleState(og, q, pr, c0)
+    dt = 0.5 / st.cfl()
+    seq = [0] + [1, 2, 3] * steps
+    t0 = time.perf_counter()
+    for ns in seq:
+        st.substep(ns, dt)
+    cpu_s = time.perf_counter() - t0
+    s = ChannelSolver(g, "fp64", stream=torch.cuda.current_stream())
+    s.upload_state(q, pr, c0)
+    s.solverrk3_mom_tg(0, dt)
+    _sync()
+    t1 = time.perf_counter()
+    for ns in seq[1:]:
+        s.solverrk3_mom_tg(ns, dt)
+    _sync()
+    gpu_s = time.perf_counter() - t1
+    qg, pg, _ = s.download_state()
+    a, b = s.mean_xz(), st.mean_xz()
+    rel = lambda x, y: float(np.abs(x - y).max() / np.abs(y).max())
+    _note(f"64^3 case: oracle {cpu_s:.1f} s, device {gpu_s:.2f} s")
+    out = dict(grid="64x64x64", ren=2800.0, substeps=len(seq),
+               cpu_substeps_per_s=round(len(seq) / cpu_s, 3),
+               gpu_substeps_per_s=round((len(seq) - 1) / gpu_s, 2),
+               q_rel_err=float(f"{rel(qg, st.q):.3e}"), pr_rel_err=float(f"{rel(pg, st.pr):.3e}"),
+               mean_rel_err=float(f"{mean_err(a, b):.3e}"),
+               gpu_max_div=s.divgck(), cpu_max_div=st.divgck())
+    s.close()
+    return out
+
+
+def mixed_errors(a, b, ma, mb):
+    """Max-norm relative difference of the mixed form's state (b) from the FP64
+    form's (a) after the same substeps, per field; and the largest difference
+    of the xz-mean profiles (device PP_MEAN_ZX moments, velocities in units of
+    the bulk velocity), absolute and relative to the profile's maximum."""
     (qa, pa, _), (qb, pb, _) = a, b
     rel = lambda x, y: float(np.abs(x - y).max() / max(np.abs(x).max(), 1e-300))
     out = {n: rel(qa[:, 1:-1, :, c], qb[:, 1:-1, :, c]) for c, n in enumerate("uvw")}
     out["p"] = rel(pa[:, 1:-1, :], pb[:, 1:-1, :])
-    ua, ub = qa[:, 1:-1, :, 0].mean(axis=(0, 2)), qb[:, 1:-1, :, 0].mean(axis=(0, 2))
-    out["u_mean_profile"] = float(np.abs(ua - ub).max())
-    return {k: float(f"{v:.3e}") for k, v in out.items()}
+    out = {k: float(f"{v:.3e}") for k, v in out.items()}
+    d = np.abs(ma - mb)
+    out["mean_profile_abs"] = {k: float(f"{d[:, c].max():.3e}") for k, c in MEAN_COLS.items()}
+    out["mean_profile_rel"] = {k: float(f"{d[:, c].max() / max(np.abs(ma[:, c]).max(), 1e-300):.3e}")
+                               for k, c in MEAN_COLS.items() if k in ("u", "p", "uu", "vv", "ww")}
+    out["u_mean_profile"] = out["mean_profile_abs"]["u"]
+    return out
 
 
 def main():
@@ -293,12 +377,14 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compare-steps", type=int, default=500,
+                    help="RK3 steps of both forms before the mixed-vs-FP64 comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is design-only, see DESIGN.md)"}))
         sys.exit(2)
-    res = gpu_bench(args.steps, args.warmup)
+    res = gpu_bench(args.steps, args.warmup, args.compare_steps)
     ncell = res["ncell"]
     f64 = res["fp64"]
     mixed = res["mixed"]
@@ -322,10 +408,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cb = cpu_baseline()
+        _note("cpu baseline done")
         cpu = dict(value=round(cb["substeps_per_s"], 4), unit="substeps/s", cores=cb["cores"], kind="port",
                    sample=f"{cb['nsub']} FP64 RK3 substeps of the C oracle at 256x192x256 "
                           f"(+{cb['npois']} Poisson solves: {cb['poisson_per_s']:.3f}/s), OpenMP",
-                   poisson_per_s=round(cb["poisson_per_s"], 4))
+                   poisson_per_s=round(cb["poisson_per_s"], 4), case_64cube=small_case())
     sub_s = 1e3 / f64["ms_per_substep"]
     out = {
         "metric": "RK3 substeps/s & Poisson solves/s at 256x192x256, div≤1e-10, mixed vs FP64",
@@ -347,7 +434,9 @@ def main():
         "max_div_after_timed_steps": f64["maxdiv"],
         "dt": f64["dt"],
         "cn_solver": f64["cn"],
-        "mixed": {"err_vs_fp64": mixed_errors(f64["state"], mixed["state"]),
+        "mixed": {"err_vs_fp64": mixed_errors(f64["state"], mixed["state"], f64["means"], mixed["means"]),
+                  "compared_after": {"substeps": mixed["end"]["substeps"], "fp64_max_div": f64["end"]["maxdiv"],
+                                     "mixed_max_div": mixed["end"]["maxdiv"], "cfl": round(f64["end"]["cfl"], 4)},
                   "substeps_per_s": round(1e3 / mixed["ms_per_substep"], 3),
                   "ms_per_substep": round(mixed["ms_per_substep"], 4),
                   "speedup_vs_fp64": round(f64["ms_per_substep"] / mixed["ms_per_substep"], 3),

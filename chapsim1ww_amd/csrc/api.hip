@@ -15,6 +15,7 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 int rk3_substep(chs_handle* h, int ns, double dt);
 int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out);
+int mean_xz(chs_handle* h, double* out_host);
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
 bool substep_supported(const chs_handle* h);
 void release_stencils(const chs_handle* h);
@@ -317,6 +318,11 @@ int chs_divgck(chs_handle* h, double* maxdiv) {
 int chs_cfl(chs_handle* h, double* rate) {
     CHS_CHECK(h && rate, CHS_E_ARG, "chs_cfl: null argument");
     return divgck_cfl(h, nullptr, rate);
+}
+
+int chs_mean_xz(chs_handle* h, double* out) {
+    CHS_CHECK(h && out, CHS_E_ARG, "chs_mean_xz: null argument");
+    return mean_xz(h, out);
 }
 
 int chs_last_dpgrns(chs_handle* h, double* v) {

@@ -685,6 +685,55 @@ __global__ void __launch_bounds__(256) k_divgck_cfl(Geo g, const double* __restr
     }
 }
 
+// xz-mean statistics of PP_MEAN_ZX_FLOW_TG (src/processing/PP_MEAN_ZX.f90:
+// 104-237), first and second moments: one block per row j, out[13 j + m] as the
+// C ABI documents (cell-centred u, v, w sums, p, their products, u p).
+__global__ void __launch_bounds__(256) k_mean_xz(Geo g, const double* __restrict__ Q, const double* __restrict__ PR,
+                                                 double* __restrict__ out) {
+    __shared__ double red[13][256];
+    const int L = g.L, N = g.N, j = blockIdx.x, J = j + 1;
+    const int LP = L * (g.M + 2), CS = LP * N;
+    double acc[13];
+#pragma unroll
+    for (int m = 0; m < 13; ++m) acc[m] = 0.0;
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        const int k = e / L, i = e - k * L;
+        const int ip = i == L - 1 ? 0 : i + 1, kp = k == N - 1 ? 0 : k + 1;
+        const int o = i + L * J + LP * k;
+        double U[3];
+        U[0] = Q[o] + Q[o - i + ip];
+        U[1] = Q[CS + o] + Q[CS + o + L];
+        U[2] = Q[2 * CS + o] + Q[2 * CS + o + LP * (kp - k)];
+        const double P = PR[o];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            acc[m] += U[m];
+            acc[10 + m] += U[m] * P;
+        }
+        acc[3] += P;
+        int l = 4;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = m; n < 3; ++n) acc[l++] += U[m] * U[n];
+    }
+#pragma unroll
+    for (int m = 0; m < 13; ++m) red[m][threadIdx.x] = acc[m];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+#pragma unroll
+            for (int m = 0; m < 13; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 13) {
+        const int m = threadIdx.x;
+        const double VL = 1.0 / (double)(L * N);
+        const double c = m == 3 ? VL : (m >= 4 && m < 10) ? VL * (0.5 * 0.5) : VL * 0.5;
+        out[13 * j + m] = red[m][0] * c;
+    }
+}
+
 __global__ void k_reduce_max2_nan(const double* __restrict__ part, int nb, double* __restrict__ out) {
     __shared__ double red[2][256];
     double a = 0.0, b = 0.0;
@@ -1068,6 +1117,16 @@ int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out) {
     CHS_HIP(hipStreamSynchronize(s));
     if (div_out) *div_out = host[0];
     if (cfl_out) *cfl_out = host[1];
+    return CHS_OK;
+}
+
+int mean_xz(chs_handle* h, double* out_host) {
+    const Geo g = geo_of(h);
+    const hipStream_t s = h->stream;
+    CHS_CHECK(13 * (size_t)h->M <= 2 * (size_t)h->npart, CHS_E_ARG, "chs_mean_xz: NCL2 too large for the work buffer");
+    CHS_LAUNCH(k_mean_xz, dim3(h->M), dim3(256), 0, s, g, h->q, h->pr, h->partials);
+    CHS_HIP(hipMemcpyAsync(out_host, h->partials, 13 * (size_t)h->M * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHS_HIP(hipStreamSynchronize(s));
     return CHS_OK;
 }
 

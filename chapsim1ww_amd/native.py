@@ -59,6 +59,7 @@ _SIGS = {
     "chs_divgck": ([C.c_void_p, _dp], C.c_int),
     "chs_cfl": ([C.c_void_p, _dp], C.c_int),
     "chs_last_dpgrns": ([C.c_void_p, _dp], C.c_int),
+    "chs_mean_xz": ([C.c_void_p, _dp], C.c_int),
     "chs_tdma_batched": ([C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.c_void_p],
                          C.c_int),
     "chs_tdma_batched_host": ([_dp] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int], C.c_int),

@@ -117,6 +117,15 @@ class ChannelSolver:
         nat.check(self.lib.chs_cfl(self.h, C.byref(v)))
         return v.value
 
+    def mean_xz(self):
+        """xz-plane means of the device state, PP_MEAN_ZX_FLOW_TG
+        (src/processing/PP_MEAN_ZX.f90:104-237): array (NCL2, 13) with columns
+        U1xzL(1:4) = <u>, <v>, <w>, <p>; U2xzL(1:6) = <uu> <uv> <uw> <vv> <vw> <ww>;
+        UPxzL(1:3) = <up> <vp> <wp> (cell-centred, the reference's scaling)."""
+        out = np.zeros(13 * self.grid.M)
+        nat.check(self.lib.chs_mean_xz(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(self.grid.M, 13)
+
     def last_dpgrns(self) -> float:
         v = C.c_double()
         nat.check(self.lib.chs_last_dpgrns(self.h, C.byref(v)))

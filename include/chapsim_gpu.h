@@ -104,6 +104,17 @@ int chs_cn_info(chs_handle *h, int ns, double dt, int *path, int *kx, int *kz, i
 int chs_divgck(chs_handle *h, double *maxdiv_host);
 /* max convective CFL rate: CFL_tg (src/CFL.f90:21-59); dt_cfl = CFL/rate. */
 int chs_cfl(chs_handle *h, double *rate_host);
+/* xz-plane means of the device state, the first and second moments that
+ * PP_MEAN_ZX_FLOW_TG accumulates (src/processing/PP_MEAN_ZX.f90:1-237): with
+ * cell-centred sums U1 = u(i)+u(i+1), U2 = v(j)+v(j+1), U3 = w(k)+w(k+1),
+ * out[13 j + m] for row j = 0..NCL2-1 holds
+ *   m = 0..2   mean(U_m) / 2            (U1xzL_tg(J,1:3))
+ *   m = 3      mean(p)                  (U1xzL_tg(J,4))
+ *   m = 4..9   mean(U_a U_b) / 4, (a,b) = 11 12 13 22 23 33   (U2xzL_tg)
+ *   m = 10..12 mean(U_m p) / 2          (UPxzL_tg)
+ * out is host memory of 13*NCL2 doubles.  One block per row sums in a fixed
+ * order: the result is deterministic. */
+int chs_mean_xz(chs_handle *h, double *out_host);
 /* DPGRNS of the last substep (constant-mass-flux correction). */
 int chs_last_dpgrns(chs_handle *h, double *dpgrns_host);
 

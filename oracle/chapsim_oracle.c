@@ -945,6 +945,48 @@ double orc_cfl_tg(const orc_grid *g, const orc_state *s)
     return CFLMA;
 }
 
+/* xz-mean statistics of PP_MEAN_ZX_FLOW_TG (src/processing/PP_MEAN_ZX.f90:
+ * 104-237), channel (RCCI1 = 1), first and second moments only: per row J
+ * (out[13*(J-1) + m]):
+ *   m = 0..2  U1xz(J,1:3) = VL1313/2 sum (cell-centred u, v, w sums)
+ *   m = 3     U1xz(J,4)   = VL1313 sum p
+ *   m = 4..9  U2xz(J,1:6) = VL1313/4 sum Ucct(M) Ucct(N), M <= N
+ *   m = 10..12 UPxz(J,1:3) = VL1313/2 sum Ucct(M) p
+ * with Ucct(1) = u(I)+u(I+1), Ucct(2) = v(J)+v(J+1), Ucct(3) = w(K)+w(K+1). */
+void orc_mean_xz(const orc_grid *g, const orc_state *s, double *out)
+{
+    const double VL = 1.0 / (double)(GL * GN);
+    const double COE0 = VL, COE1 = VL * 0.5, COE2 = VL * (0.5 * 0.5);
+#pragma omp parallel for schedule(static)
+    for (int J = 1; J <= GM; ++J) {
+        double u1[4] = {0, 0, 0, 0}, u2[6] = {0, 0, 0, 0, 0, 0}, up[3] = {0, 0, 0};
+        for (int I = 1; I <= GL; ++I) {
+            const int IP = IPV(I);
+            for (int K = 1; K <= GN; ++K) {
+                const int KP = KPV(K);
+                double U[3];
+                U[0] = QQ(I, J, K, 1) + QQ(IP, J, K, 1);
+                U[2] = (QQ(I, J, K, 3) + QQ(I, J, KP, 3)) * 1.0;
+                U[1] = (QQ(I, J, K, 2) + QQ(I, J + 1, K, 2)) * 1.0;
+                const double P = G3(s->pr, I, J, K);
+                for (int M = 0; M < 3; ++M) {
+                    u1[M] += U[M];
+                    up[M] += U[M] * P;
+                }
+                u1[3] += P;
+                int L = 0;
+                for (int M = 0; M < 3; ++M)
+                    for (int N = M; N < 3; ++N) u2[L++] += U[M] * U[N];
+            }
+        }
+        double *o = out + 13 * (J - 1);
+        o[3] = u1[3] * COE0;
+        for (int M = 0; M < 3; ++M) o[M] = u1[M] * COE1;
+        for (int L = 0; L < 6; ++L) o[4 + L] = u2[L] * COE2;
+        for (int M = 0; M < 3; ++M) o[10 + M] = up[M] * COE1;
+    }
+}
+
 /* SOLVERRK3_MOM_tg(NS) */
 void orc_substep_tg(const orc_grid *g, orc_state *s, int NS, double DT, int flowdrvtp)
 {

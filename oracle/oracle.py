@@ -80,6 +80,8 @@ def lib():
         L.orc_divgck_tg.restype = C.c_double
         L.orc_cfl_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State)]
         L.orc_cfl_tg.restype = C.c_double
+        L.orc_mean_xz.argtypes = [C.POINTER(_Grid), C.POINTER(_State), _dp]
+        L.orc_mean_xz.restype = None
         L.orc_substep_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_double, C.c_int]
         L.orc_cubic_spline.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_sleval.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
@@ -156,6 +158,12 @@ class OracleState:
 
     def cfl(self) -> float:
         return lib().orc_cfl_tg(C.byref(self.og.g), C.byref(self.s))
+
+    def mean_xz(self):
+        """PP_MEAN_ZX_FLOW_TG first and second moments, (M, 13)."""
+        out = np.zeros(13 * self.og.M)
+        lib().orc_mean_xz(C.byref(self.og.g), C.byref(self.s), _ptr(out))
+        return out.reshape(self.og.M, 13)
 
     def stage(self, name: str, *args):
         fn = getattr(lib(), "orc_" + name)

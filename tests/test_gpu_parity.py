@@ -210,3 +210,77 @@ def test_divgck_cfl_and_nan(solver_mod):
     s.upload_state(q2, None, None)
     assert np.isnan(s.divgck()) and np.isnan(s.cfl())
     s.close()
+
+
+MEAN_GROUPS = ([0, 1, 2], [3], list(range(4, 10)), [10, 11, 12])
+
+
+def _mean_err(a, b):
+    """Largest difference of two (NCL2, 13) moment arrays, each column scaled
+    by the largest value of its group (<u_i>, <p>, <u_i u_j>, <u_i p>): the
+    xz-mean of v vanishes analytically, so that column holds only round-off
+    and is measured against the velocity scale.  An all-zero group (p = 0 on
+    the start state) is compared absolutely."""
+    err = 0.0
+    for grp in MEAN_GROUPS:
+        sc = np.abs(b[:, grp]).max()
+        err = max(err, float(np.abs(a[:, grp] - b[:, grp]).max() / (sc if sc > 0 else 1.0)))
+    return err
+
+
+@pytest.mark.parametrize("dims", [(32, 24, 16), (64, 48, 32)])
+def test_mean_xz(solver_mod, dims):
+    """xz-plane moments (PP_MEAN_ZX_FLOW_TG) of the device state against the
+    oracle: on the uploaded state (same fields, only the summation order
+    differs) and after NS = 0, 1, 2, 3 (fields within 1e-12 of each other)."""
+    print(f"mean_xz {dims}", flush=True)
+    g = ChannelGrid(*dims)
+    og = O.OracleGrid(*dims)
+    q, pr, c0 = fields.init_tg_fields(g, seed=11)
+    st = O.OracleState(og, q, pr, c0)
+    s = solver_mod.ChannelSolver(g)
+    s.upload_state(q, pr, c0)
+    dt = 0.5 / st.cfl()
+
+    def close(tol):
+        a, b = s.mean_xz(), st.mean_xz()
+        assert a.shape == (dims[1], 13)
+        err = _mean_err(a, b)
+        print("rel", err, flush=True)
+        assert err < tol, err
+
+    close(1e-14)
+    for ns in (0, 1, 2, 3):
+        st.substep(ns, dt)
+        s.solverrk3_mom_tg(ns, dt)
+    close(1e-12)
+    s.close()
+
+
+def test_trajectory_64cube_re2800(solver_mod):
+    """The CPU-runnable channel case: 64^3, Re_b = 2800, seeded laminar +
+    noise start, CFL 0.5, 100 RK3 steps (NS = 0 then 1,2,3 x 100), FP64 device
+    path against the oracle.  Round-off differences grow along the
+    trajectory; after 301 substeps the state and the xz-mean moments stay
+    within 1e-10 of the oracle's."""
+    print("trajectory 64^3 Re_b=2800, 100 steps", flush=True)
+    dims = (64, 64, 64)
+    g = ChannelGrid(*dims, ren=2800.0)
+    og = O.OracleGrid(*dims, ren=2800.0)
+    q, pr, c0 = fields.init_tg_fields(g, seed=1973)
+    st = O.OracleState(og, q, pr, c0)
+    dt = 0.5 / st.cfl()
+    s = solver_mod.ChannelSolver(g)
+    s.upload_state(q, pr, c0)
+    for i, ns in enumerate([0] + [1, 2, 3] * 100):
+        st.substep(ns, dt)
+        s.solverrk3_mom_tg(ns, dt)
+        if i % 60 == 0 or i == 300:
+            (qg, pg, _), dg = s.download_state(), s.divgck()
+            eq, ep = _rel(qg, st.q), _rel(pg, st.pr)
+            a, b = s.mean_xz(), st.mean_xz()
+            em = _mean_err(a, b)
+            print(i, ns, f"Q {eq:.2e} PR {ep:.2e} means {em:.2e} div {dg:.2e}", flush=True)
+            assert eq < 1e-10 and ep < 1e-10 and em < 1e-10, (i, eq, ep, em)
+            assert dg < 1e-10
+    s.close()

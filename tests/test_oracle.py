@@ -326,3 +326,27 @@ def test_planted_mistake_in_tdma_is_caught():
     O.tdma_noncyc(a_bad, b, c, r, None, n, 1, 1, n)
     x = np.linalg.solve(_dense(a, b, c, False), d)
     assert np.abs(r - x).max() > 1e-3
+
+
+# --------------------------------------------------------------------------
+# xz-mean statistics (PP_MEAN_ZX_FLOW_TG first and second moments)
+# --------------------------------------------------------------------------
+def test_mean_xz_against_numpy():
+    """The oracle's moments against a vectorised numpy evaluation of the
+    cell-centred definitions, and the laminar profile's known mean."""
+    g = ChannelGrid(16, 12, 8)
+    og = O.OracleGrid(16, 12, 8)
+    q, pr, c0 = fields.init_tg_fields(g, seed=4)
+    pr = np.asfortranarray(RNG.standard_normal(pr.shape))
+    st = O.OracleState(og, q, pr, c0)
+    out = st.mean_xz()
+    u = q[:, 1:-1, :, 0] + np.roll(q[:, 1:-1, :, 0], -1, axis=0)
+    v = q[:, 1:-1, :, 1] + q[:, 2:, :, 1]
+    w = q[:, 1:-1, :, 2] + np.roll(q[:, 1:-1, :, 2], -1, axis=2)
+    p = pr[:, 1:-1, :]
+    m = lambda a: a.mean(axis=(0, 2))
+    ref = np.stack([m(u) / 2, m(v) / 2, m(w) / 2, m(p), m(u * u) / 4, m(u * v) / 4, m(u * w) / 4, m(v * v) / 4,
+                    m(v * w) / 4, m(w * w) / 4, m(u * p) / 2, m(v * p) / 2, m(w * p) / 2], 1)
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-14)
+    # bulk velocity of the rescaled initial field: sum over rows of mean u / DYFI / 2 = 1
+    assert np.sum(out[:, 0] / og.dyfi) / 2.0 == pytest.approx(1.0, rel=1e-12)
