@@ -1313,3 +1313,526 @@ long orc_energy_substep(const orc_grid *g, orc_thermal *t, int n, const double *
 #undef TIMV
 #undef TKPV
 #undef TKMV
+
+/* ---------------------------------------------------------------------------
+ * The io (main-domain) momentum substep of the heated periodic channel:
+ * variable properties, mass fluxes G = rho u, explicit viscous terms
+ * (visthemflg = 0, the reference's default), no weighted pressure.
+ *
+ * orc_io_prepare(NS): what SOLVE runs between SOLVERRK3_ENG_IO(NS) and
+ *   SOLVERRK3_MOM_IO(NS) (src/SOLVE.f90:121-133):
+ *     DENSITY_TIME_DERIVATION(NS)   src/THERM_PROP_UPDATE.f90:225-291
+ *     DENSITY_Staggered             src/MASSFLUX_CALC_IO.f90:242-311
+ *     MU_Staggered                  src/MASSFLUX_CALC_IO.f90:147-237
+ *     VELOCITY_CALC_io              src/MASSFLUX_CALC_IO.f90:82-142
+ *   (the first three only with thermal = 1; otherwise RHO_STG = MU_STG = 1
+ *   and DrhoDtP = 0 as allocated, src/MEMALLOCT.f90:197-201).
+ * orc_io_substep(NS): SOLVERRK3_MOM_IO (src/SOLVERRK3_MOM_IO.f90:1-168):
+ *     G0 = G
+ *     CONVECTION_X_io / _Y_io / _Z_io  src/CONVECTION_X.f90:77-176,
+ *                                      src/CONVECTION_Y.f90:128-287,
+ *                                      src/CONVECTION_Z.f90:115-314
+ *     DIVG_U_io                        src/DIVG.f90:153-259
+ *     VISCOUS_ALL_EXPLT_X/Y/Z_io       src/VISCOUS_X_EXPLICIT_io.f90:1-113,
+ *                                      src/VISCOUS_Y_EXPLICIT_io.f90:1-99,
+ *                                      src/VISCOUS_Z_EXPLICIT_io.f90:1-100
+ *     RHS_MOM_EXPLICIT_io(NS, IDR)     src/RHS_MOM_EXPLICIT_io.f90:1-297
+ *     MASSFLUX_CALC_IO                 src/MASSFLUX_CALC_IO.f90:1-79
+ *     BC_WALL_G_io                     src/BC_WALL.f90:412-458, 682-694
+ *     DIVG_io(NS)                      src/DIVG.f90:60-150
+ *     the FFT99 Poisson solve, the reference's "Method One" for the io
+ *     domain (src/SOLVERRK3_MOM_IO.f90:123), then BC_WALL_DPH_io
+ *     PRCALC_io (explicit branch)      src/PRESSURE_CORRECTION.f90:88-156
+ *     MASSFLUX_UPDATE_IO               src/VELO_CORRECTION.f90:109-185
+ * INTFC_VARS1/3 on one GPU is the periodic y wrap that the reference's
+ * exchange produces across its ranks (src/INTFC_VARS.f90:41-80): row 0 <-
+ * row M, row M+1 <- row 1; wall routines then overwrite what they set.
+ * Channel metrics: RCCI1 = RNDI1 = RCCI2 = 1, XND2CL = YND2CL = ZND2CL = 1/2.
+ * Arrays: q g g0 rho_stg mu_stg (L, 0:M+1, N, 3); pr rho rho0 mu mu0
+ * drhodtp divu dph (L, 0:M+1, N); explt0 (L, M, N, 3); hx hy hz rhsllphi
+ * (L, M, N).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    double *q, *g, *g0, *pr;
+    const double *rho, *rho0, *mu, *mu0;
+    double *drhodtp, *divu, *rho_stg, *mu_stg, *explt0;
+    double *dph, *hx, *hy, *hz, *rhsllphi;
+    double dpgrns;
+} orc_io;
+
+typedef struct {
+    int thermal;      /* thermlflg */
+    int nflow;        /* NFLOW: streamwise direction 1..3 */
+    int flowdrvtp;    /* FLOWDRVTP: 1 constant mass flux, 2 constant pressure gradient */
+    int gravdir;      /* GRAVDIR: 0 none, 1..3 */
+    double cfgv;      /* CFGV (FLOWDRVTP = 2) */
+    double f_a;       /* F_A (src/Flow_State.f90:97-110) */
+} orc_io_params;
+
+#define WL (g->L)
+#define WM (g->M)
+#define WN (g->N)
+#define W3(arr, i, j, k) (arr)[(size_t)((i)-1) + (size_t)WL * ((size_t)(j) + (size_t)(WM + 2) * ((k)-1))]
+#define WF3(arr, i, j, k) (arr)[(size_t)((i)-1) + (size_t)WL * ((size_t)((j)-1) + (size_t)WM * ((k)-1))]
+#define WG(arr, i, j, k, c) (arr)[(size_t)((i)-1) + (size_t)WL * ((size_t)(j) + (size_t)(WM + 2) * ((size_t)((k)-1) + (size_t)WN * ((c)-1)))]
+#define WE(arr, i, j, k, c) (arr)[(size_t)((i)-1) + (size_t)WL * ((size_t)((j)-1) + (size_t)WM * ((size_t)((k)-1) + (size_t)WN * ((c)-1)))]
+#define WIPV(i) ((i) == WL ? 1 : (i) + 1)
+#define WIMV(i) ((i) == 1 ? WL : (i)-1)
+#define WKPV(k) ((k) == WN ? 1 : (k) + 1)
+#define WKMV(k) ((k) == 1 ? WN : (k)-1)
+
+/* INTFC_VARS on one GPU: the periodic y wrap of a (L, 0:M+1, N) array */
+static void io_wrap(const orc_grid *g, double *a)
+{
+    for (int K = 1; K <= WN; ++K)
+        for (int I = 1; I <= WL; ++I) {
+            W3(a, I, 0, K) = W3(a, I, WM, K);
+            W3(a, I, WM + 1, K) = W3(a, I, 1, K);
+        }
+}
+
+static void io_wrap3(const orc_grid *g, double *a)
+{
+    const size_t ngh = (size_t)WL * (WM + 2) * WN;
+    for (int c = 0; c < 3; ++c) io_wrap(g, a + c * ngh);
+}
+
+/* BC_WALL_Q / BC_WALL_G for the io domain (src/BC_WALL.f90:185-263, 412-458) */
+static void io_bc_wall_vec(const orc_grid *g, double *a)
+{
+    for (int K = 1; K <= WN; ++K)
+        for (int I = 1; I <= WL; ++I) {
+            WG(a, I, WM + 1, K, 1) = 0.0;
+            WG(a, I, WM + 1, K, 3) = 0.0;
+            WG(a, I, WM + 1, K, 2) = 0.0;
+            WG(a, I, 0, K, 1) = 0.0;
+            WG(a, I, 0, K, 3) = 0.0;
+            WG(a, I, 1, K, 2) = 0.0;
+            WG(a, I, 0, K, 2) = 0.0;
+        }
+}
+
+void orc_io_prepare(const orc_grid *g, orc_io *s, const orc_io_params *p, int NS)
+{
+    if (p->thermal) {
+        /* DENSITY_TIME_DERIVATION(NS) */
+#pragma omp parallel for schedule(static)
+        for (int JC = 1; JC <= WM; ++JC) {
+            const int JP = JC + 1;
+            const double COE2 = g->dyfi[JC - 1] * 1.0, COE3 = g->dzi * 1.0;
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IP = WIPV(IC);
+                for (int KC = 1; KC <= WN; ++KC) {
+                    const int KP = WKPV(KC);
+                    double DIVX = (WG(s->g, IP, JC, KC, 1) - WG(s->g, IC, JC, KC, 1)) * g->dxi;
+                    double DIVX0 = (WG(s->g0, IP, JC, KC, 1) - WG(s->g0, IC, JC, KC, 1)) * g->dxi;
+                    double DIVY = (WG(s->g, IC, JP, KC, 2) - WG(s->g, IC, JC, KC, 2)) * COE2;
+                    double DIVY0 = (WG(s->g0, IC, JP, KC, 2) - WG(s->g0, IC, JC, KC, 2)) * COE2;
+                    double DIVZ = (WG(s->g, IC, JC, KP, 3) - WG(s->g, IC, JC, KC, 3)) * COE3;
+                    double DIVZ0 = (WG(s->g0, IC, JC, KP, 3) - WG(s->g0, IC, JC, KC, 3)) * COE3;
+                    W3(s->drhodtp, IC, JC, KC) =
+                        ((TGAM[NS] + TROH[NS]) * (DIVX + DIVY + DIVZ) + TROH[NS] * (DIVX0 + DIVY0 + DIVZ0)) /
+                        (TGAM[NS] - 2.0);
+                }
+            }
+        }
+        /* DENSITY_Staggered */
+        const size_t ngh = (size_t)WL * (WM + 2) * WN;
+        const double *D = s->rho, *D0 = s->rho0;
+        double *RS = s->rho_stg;
+#pragma omp parallel for schedule(static)
+        for (int JC = 1; JC <= WM; ++JC)
+            for (int KC = 1; KC <= WN; ++KC)
+                for (int IC = 1; IC <= WL; ++IC) {
+                    const int IM = WIMV(IC), KM = WKMV(KC);
+                    WG(RS, IC, JC, KC, 1) =
+                        (W3(D, IC, JC, KC) + W3(D, IM, JC, KC) + W3(D0, IC, JC, KC) + W3(D0, IM, JC, KC)) * 0.5 * 0.5;
+                    WG(RS, IC, JC, KC, 3) =
+                        (W3(D, IC, JC, KC) + W3(D, IC, JC, KM) + W3(D0, IC, JC, KC) + W3(D0, IC, JC, KM)) * 0.5 * 0.5;
+                    if (JC >= 2) {
+                        const int JM = JC - 1;
+                        WG(RS, IC, JC, KC, 2) = (g->wff[JC - 1] * (W3(D, IC, JC, KC) + W3(D0, IC, JC, KC)) +
+                                                 g->wfb[JC - 1] * (W3(D, IC, JM, KC) + W3(D0, IC, JM, KC))) *
+                                                0.5;
+                    }
+                }
+        io_wrap3(g, RS);
+        for (int KC = 1; KC <= WN; ++KC)
+            for (int IC = 1; IC <= WL; ++IC) {
+                WG(RS, IC, 1, KC, 2) = 0.5 * (W3(D0, IC, 0, KC) + W3(D, IC, 0, KC));
+                WG(RS, IC, WM + 1, KC, 2) = 0.5 * (W3(D0, IC, WM + 1, KC) + W3(D, IC, WM + 1, KC));
+            }
+        /* MU_Staggered */
+        const double *V = s->mu, *V0 = s->mu0;
+        double *MS = s->mu_stg;
+#pragma omp parallel for schedule(static)
+        for (int JC = 1; JC <= WM; ++JC)
+            for (int KC = 1; KC <= WN; ++KC)
+                for (int IC = 1; IC <= WL; ++IC) {
+                    const int IM = WIMV(IC), KM = WKMV(KC), JM = JC - 1;
+                    if (JC >= 2) {
+                        WG(MS, IC, JC, KC, 1) =
+                            ((W3(V, IC, JC, KC) + W3(V, IM, JC, KC) + W3(V0, IC, JC, KC) + W3(V0, IM, JC, KC)) *
+                                 g->wff[JC - 1] +
+                             (W3(V, IC, JM, KC) + W3(V, IM, JM, KC) + W3(V0, IC, JM, KC) + W3(V0, IM, JM, KC)) *
+                                 g->wfb[JC - 1]) *
+                            0.5 * 0.5;
+                        WG(MS, IC, JC, KC, 3) =
+                            ((W3(V, IC, JC, KM) + W3(V, IC, JC, KC) + W3(V0, IC, JC, KM) + W3(V0, IC, JC, KC)) *
+                                 g->wff[JC - 1] +
+                             (W3(V, IC, JM, KM) + W3(V, IC, JM, KC) + W3(V0, IC, JM, KM) + W3(V0, IC, JM, KC)) *
+                                 g->wfb[JC - 1]) *
+                            0.5 * 0.5;
+                    }
+                    WG(MS, IC, JC, KC, 2) = (W3(V, IC, JC, KC) + W3(V, IM, JC, KC) + W3(V0, IC, JC, KC) +
+                                             W3(V0, IM, JC, KC) + W3(V, IC, JC, KM) + W3(V, IM, JC, KM) +
+                                             W3(V0, IC, JC, KM) + W3(V0, IM, JC, KM)) *
+                                            0.5 * 0.5 * 0.5;
+                }
+        io_wrap3(g, MS);
+        for (int KC = 1; KC <= WN; ++KC)
+            for (int IC = 1; IC <= WL; ++IC) {
+                const double wb = 0.5 * (W3(V0, IC, 0, KC) + W3(V, IC, 0, KC));
+                const double wt = 0.5 * (W3(V0, IC, WM + 1, KC) + W3(V, IC, WM + 1, KC));
+                WG(MS, IC, 1, KC, 1) = wb;
+                WG(MS, IC, 1, KC, 3) = wb;
+                WG(MS, IC, WM + 1, KC, 1) = wt;
+                WG(MS, IC, WM + 1, KC, 3) = wt;
+            }
+        (void)ngh;
+    }
+    /* VELOCITY_CALC_io */
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= WM; ++JC)
+        for (int KC = 1; KC <= WN; ++KC)
+            for (int IC = 1; IC <= WL; ++IC) {
+                WG(s->q, IC, JC, KC, 1) = WG(s->g, IC, JC, KC, 1) / WG(s->rho_stg, IC, JC, KC, 1);
+                WG(s->q, IC, JC, KC, 3) = WG(s->g, IC, JC, KC, 3) / WG(s->rho_stg, IC, JC, KC, 3);
+                if (JC >= 2) WG(s->q, IC, JC, KC, 2) = WG(s->g, IC, JC, KC, 2) / WG(s->rho_stg, IC, JC, KC, 2);
+            }
+    io_wrap3(g, s->q);
+    io_bc_wall_vec(g, s->q);
+}
+
+/* RHS_MOM_EXPLICIT_io(NS, IDR) on the stage array h (Qtmp, DPH or RHSLLPHI
+ * of the reference), explicit branch: h <- RHS. */
+static void io_rhs_mom(const orc_grid *g, orc_io *s, const orc_io_params *p, int NS, int IDR, double DT, double *h)
+{
+    const int NYI = IDR == 2 ? 2 : 1;
+    const double TA = talp(NS);
+    double *RHS = (double *)calloc((size_t)WL * WM * WN, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int JC = NYI; JC <= WM; ++JC) {
+        const int JM = JC - 1;
+        for (int KC = 1; KC <= WN; ++KC) {
+            const int KM = WKMV(KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IM = WIMV(IC);
+                const double RHSC = WF3(h, IC, JC, KC), RHSL = WE(s->explt0, IC, JC, KC, IDR);
+                double r = (TGAM[NS] * RHSC + TROH[NS] * RHSL) * DT;
+                WE(s->explt0, IC, JC, KC, IDR) = RHSC;
+                double PGM;
+                if (IDR == 1)
+                    PGM = (W3(s->pr, IC, JC, KC) - W3(s->pr, IM, JC, KC)) * (TA * DT * g->dxi);
+                else if (IDR == 2)
+                    PGM = (W3(s->pr, IC, JC, KC) - W3(s->pr, IC, JM, KC)) * (TA * DT * g->dyci[JC - 1] / 1.0);
+                else
+                    PGM = (W3(s->pr, IC, JC, KC) - W3(s->pr, IC, JC, KM)) * (TA * DT * g->dzi);
+                r = r - PGM;
+                if (p->gravdir == IDR) r = r + WG(s->rho_stg, IC, JC, KC, IDR) * (TA * DT * p->f_a / 1.0);
+                WF3(RHS, IC, JC, KC) = r;
+            }
+        }
+    }
+    if (IDR == p->nflow) {
+        double DPGRNS = 0.0;
+        if (p->flowdrvtp == 1) {
+            double INTGRHSY = 0.0;
+            for (int JC = 1; JC <= WM; ++JC)
+                for (int IC = 1; IC <= WL; ++IC)
+                    for (int KC = 1; KC <= WN; ++KC) INTGRHSY = INTGRHSY + WF3(RHS, IC, JC, KC) / g->dyfi[JC - 1] / 1.0;
+            DPGRNS = INTGRHSY / g->volm;
+        } else if (p->flowdrvtp == 2) {
+            DPGRNS = -0.5 * p->cfgv * (TA * DT);
+        }
+        s->dpgrns = DPGRNS;
+        for (int JC = NYI; JC <= WM; ++JC)
+            for (int KC = 1; KC <= WN; ++KC)
+                for (int IC = 1; IC <= WL; ++IC) WF3(RHS, IC, JC, KC) = WF3(RHS, IC, JC, KC) - DPGRNS;
+    }
+    for (int JC = NYI; JC <= WM; ++JC)
+        for (int KC = 1; KC <= WN; ++KC)
+            for (int IC = 1; IC <= WL; ++IC) WF3(h, IC, JC, KC) = WF3(RHS, IC, JC, KC);
+    free(RHS);
+}
+
+/* The explicit terms: CONVECTION_X/Y/Z_io, DIVG_U_io and
+ * VISCOUS_ALL_EXPLT_X/Y/Z_io -> hx, hy (row 1 zero), hz. */
+void orc_io_terms(const orc_grid *g, orc_io *s)
+{
+    const size_t ncell = (size_t)WL * WM * WN;
+    const double CV = g->cvisc;
+    memset(s->hy, 0, ncell * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= WM; ++JC) {
+        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1;
+        const double wff = g->wff[JJ - 1], wfb = g->wfb[JJ - 1], wffp = g->wff[JJP - 1], wfbp = g->wfb[JJP - 1];
+        for (int KC = 1; KC <= WN; ++KC) {
+            const int KM = WKMV(KC), KP = WKPV(KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IP = WIPV(IC), IM = WIMV(IC);
+                const double *G = s->g, *Q = s->q;
+                /* CONVECTION_X_io */
+                {
+                    const double COE1 = g->dxi * 0.5 * 0.5, COE2 = g->dyfi[JJ - 1] * 0.5 * 1.0;
+                    const double COE32 = g->dzi * 0.5 * 0.5 * 1.0;
+                    double H11F = (WG(G, IP, JC, KC, 1) + WG(G, IC, JC, KC, 1)) * (WG(Q, IP, JC, KC, 1) + WG(Q, IC, JC, KC, 1));
+                    double H11B = (WG(G, IC, JC, KC, 1) + WG(G, IM, JC, KC, 1)) * (WG(Q, IC, JC, KC, 1) + WG(Q, IM, JC, KC, 1));
+                    double H11 = (H11F - H11B) * COE1;
+                    double H12F = (WG(G, IC, JP, KC, 2) + WG(G, IM, JP, KC, 2)) *
+                                  (wffp * WG(Q, IC, JP, KC, 1) + wfbp * WG(Q, IC, JC, KC, 1));
+                    double H12B = (WG(G, IC, JC, KC, 2) + WG(G, IM, JC, KC, 2)) *
+                                  (wff * WG(Q, IC, JC, KC, 1) + wfb * WG(Q, IC, JM, KC, 1));
+                    double H12 = (H12F - H12B) * COE2;
+                    double H13F = (WG(G, IC, JC, KP, 3) + WG(G, IM, JC, KP, 3)) * (WG(Q, IC, JC, KP, 1) + WG(Q, IC, JC, KC, 1));
+                    double H13B = (WG(G, IC, JC, KC, 3) + WG(G, IM, JC, KC, 3)) * (WG(Q, IC, JC, KC, 1) + WG(Q, IC, JC, KM, 1));
+                    double H13 = (H13F - H13B) * COE32;
+                    WF3(s->hx, IC, JC, KC) = -(H11 + H12 + H13);
+                }
+                /* CONVECTION_Y_io (rows 2..M; row 1 stays zero) */
+                if (JC >= 2) {
+                    const double COE1 = g->dxi * 0.5, COE2 = g->dyci[JJ - 1] * 0.5 * 0.5, COE31 = g->dzi * 0.5 * 1.0;
+                    double H21F = (wff * WG(G, IP, JC, KC, 1) + wfb * WG(G, IP, JM, KC, 1)) *
+                                  (WG(Q, IP, JC, KC, 2) + WG(Q, IC, JC, KC, 2));
+                    double H21B = (wff * WG(G, IC, JC, KC, 1) + wfb * WG(G, IC, JM, KC, 1)) *
+                                  (WG(Q, IC, JC, KC, 2) + WG(Q, IM, JC, KC, 2));
+                    double H21 = (H21F - H21B) * COE1;
+                    double H22F = (WG(G, IC, JP, KC, 2) + WG(G, IC, JC, KC, 2)) * (WG(Q, IC, JP, KC, 2) + WG(Q, IC, JC, KC, 2));
+                    double H22B = (WG(G, IC, JC, KC, 2) + WG(G, IC, JM, KC, 2)) * (WG(Q, IC, JC, KC, 2) + WG(Q, IC, JM, KC, 2));
+                    double H22 = (H22F - H22B) * COE2;
+                    double H23F = (wff * WG(G, IC, JC, KP, 3) + wfb * WG(G, IC, JM, KP, 3)) *
+                                  (WG(Q, IC, JC, KP, 2) + WG(Q, IC, JC, KC, 2));
+                    double H23B = (wff * WG(G, IC, JC, KC, 3) + wfb * WG(G, IC, JM, KC, 3)) *
+                                  (WG(Q, IC, JC, KC, 2) + WG(Q, IC, JC, KM, 2));
+                    double H23 = (H23F - H23B) * COE31;
+                    WF3(s->hy, IC, JC, KC) = -(H21 + H22 + H23);
+                }
+                /* CONVECTION_Z_io */
+                {
+                    const double COE1 = g->dxi * 0.5 * 0.5, COE2 = g->dyfi[JJ - 1] * 0.5;
+                    const double COE31 = g->dzi * 0.5 * 0.5 * 1.0;
+                    double H31F = (WG(G, IP, JC, KC, 1) + WG(G, IP, JC, KM, 1)) * (WG(Q, IP, JC, KC, 3) + WG(Q, IC, JC, KC, 3));
+                    double H31B = (WG(G, IC, JC, KC, 1) + WG(G, IC, JC, KM, 1)) * (WG(Q, IC, JC, KC, 3) + WG(Q, IM, JC, KC, 3));
+                    double H31 = (H31F - H31B) * COE1;
+                    double H32F = (WG(G, IC, JP, KC, 2) + WG(G, IC, JP, KM, 2)) *
+                                  (wffp * WG(Q, IC, JP, KC, 3) + wfbp * WG(Q, IC, JC, KC, 3));
+                    double H32B = (WG(G, IC, JC, KC, 2) + WG(G, IC, JC, KM, 2)) *
+                                  (wff * WG(Q, IC, JC, KC, 3) + wfb * WG(Q, IC, JM, KC, 3));
+                    double H32 = (H32F - H32B) * COE2;
+                    double H33F = (WG(G, IC, JC, KP, 3) + WG(G, IC, JC, KC, 3)) * (WG(Q, IC, JC, KP, 3) + WG(Q, IC, JC, KC, 3));
+                    double H33B = (WG(G, IC, JC, KC, 3) + WG(G, IC, JC, KM, 3)) * (WG(Q, IC, JC, KC, 3) + WG(Q, IC, JC, KM, 3));
+                    double H33 = (H33F - H33B) * COE31;
+                    WF3(s->hz, IC, JC, KC) = -(H31 + H32 + H33);
+                }
+                /* DIVG_U_io */
+                {
+                    double DIVX = (WG(Q, IP, JC, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dxi;
+                    double DIVY = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JC, KC, 2)) * (g->dyfi[JJ - 1] * 1.0);
+                    double DIVZ = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JC, KC, 3)) * (g->dzi * 1.0);
+                    W3(s->divu, IC, JC, KC) = (DIVX + DIVY + DIVZ) / 3.0;
+                }
+            }
+        }
+    }
+    io_wrap(g, s->divu);
+    /* VISCOUS_ALL_EXPLT_X_io, _Y_io, _Z_io */
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= WM; ++JC) {
+        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC - 1;
+        for (int KC = 1; KC <= WN; ++KC) {
+            const int KM = WKMV(KC), KP = WKPV(KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IP = WIPV(IC), IM = WIMV(IC);
+                const double *Q = s->q, *MU = s->mu, *MU0 = s->mu0, *DU = s->divu, *MS = s->mu_stg;
+                {
+                    const double COE1 = 2.0 * g->dxi * CV, COE2 = g->dyfi[JJ - 1] * CV * 1.0, COE32 = g->dzi * CV * 1.0;
+                    double DUDX = (WG(Q, IP, JC, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dxi;
+                    double TAU11F = 0.5 * (W3(MU0, IC, JC, KC) + W3(MU, IC, JC, KC)) * (DUDX - W3(DU, IC, JC, KC));
+                    DUDX = (WG(Q, IC, JC, KC, 1) - WG(Q, IM, JC, KC, 1)) * g->dxi;
+                    double TAU11B = 0.5 * (W3(MU0, IM, JC, KC) + W3(MU, IM, JC, KC)) * (DUDX - W3(DU, IM, JC, KC));
+                    double DTAU11DX = (TAU11F - TAU11B) * COE1;
+                    double DUDY = (WG(Q, IC, JP, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dyci[JJP - 1] / 1.0;
+                    double DVDX = (WG(Q, IC, JP, KC, 2) - WG(Q, IM, JP, KC, 2)) * g->dxi;
+                    double TAU12F = WG(MS, IC, JP, KC, 1) * (DUDY + DVDX);
+                    DUDY = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JM, KC, 1)) * g->dyci[JJ - 1] / 1.0;
+                    DVDX = (WG(Q, IC, JC, KC, 2) - WG(Q, IM, JC, KC, 2)) * g->dxi;
+                    double TAU12B = WG(MS, IC, JC, KC, 1) * (DUDY + DVDX);
+                    double DTAU12DY = (TAU12F - TAU12B) * COE2;
+                    double DUDZ = (WG(Q, IC, JC, KP, 1) - WG(Q, IC, JC, KC, 1)) * g->dzi;
+                    double DWDX = (WG(Q, IC, JC, KP, 3) - WG(Q, IM, JC, KP, 3)) * g->dxi;
+                    double TAU13F = WG(MS, IC, JC, KP, 2) * (DUDZ + DWDX);
+                    DUDZ = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JC, KM, 1)) * g->dzi;
+                    DWDX = (WG(Q, IC, JC, KC, 3) - WG(Q, IM, JC, KC, 3)) * g->dxi;
+                    double TAU13B = WG(MS, IC, JC, KC, 2) * (DUDZ + DWDX);
+                    double DTAU13DZ = (TAU13F - TAU13B) * COE32;
+                    WF3(s->hx, IC, JC, KC) = WF3(s->hx, IC, JC, KC) + DTAU11DX + DTAU12DY + DTAU13DZ;
+                }
+                if (JC >= 2) {
+                    const double COE1 = g->dxi * CV, COE3 = g->dzi * CV, COE2 = 2.0 * g->dyci[JJ - 1] * CV;
+                    double DVDY = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JC, KC, 2)) * g->dyfi[JJ - 1];
+                    double TAU22F = 0.5 * (W3(MU0, IC, JC, KC) + W3(MU, IC, JC, KC)) * (DVDY - W3(DU, IC, JC, KC));
+                    DVDY = (WG(Q, IC, JC, KC, 2) - WG(Q, IC, JM, KC, 2)) * g->dyfi[JJM - 1];
+                    double TAU22B = 0.5 * (W3(MU0, IC, JM, KC) + W3(MU, IC, JM, KC)) * (DVDY - W3(DU, IC, JM, KC));
+                    double DTAU22DY = (TAU22F - TAU22B) * COE2;
+                    double DVDX = (WG(Q, IP, JC, KC, 2) - WG(Q, IC, JC, KC, 2)) * g->dxi;
+                    double DUDY = (WG(Q, IP, JC, KC, 1) - WG(Q, IP, JM, KC, 1)) * g->dyci[JJ - 1];
+                    double TAU21F = WG(MS, IP, JC, KC, 1) * (DVDX + DUDY);
+                    DVDX = (WG(Q, IC, JC, KC, 2) - WG(Q, IM, JC, KC, 2)) * g->dxi;
+                    DUDY = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JM, KC, 1)) * g->dyci[JJ - 1];
+                    double TAU21B = WG(MS, IC, JC, KC, 1) * (DVDX + DUDY);
+                    double DTAU21DX = (TAU21F - TAU21B) * COE1;
+                    double DVDZ = (WG(Q, IC, JC, KP, 2) - WG(Q, IC, JC, KC, 2)) * g->dzi;
+                    double DWDY = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JM, KP, 3)) * g->dyci[JJ - 1];
+                    double TAU23F = WG(MS, IC, JC, KP, 3) * (DVDZ + DWDY);
+                    DVDZ = (WG(Q, IC, JC, KC, 2) - WG(Q, IC, JC, KM, 2)) * g->dzi;
+                    DWDY = (WG(Q, IC, JC, KC, 3) - WG(Q, IC, JM, KC, 3)) * g->dyci[JJ - 1];
+                    double TAU23B = WG(MS, IC, JC, KC, 3) * (DVDZ + DWDY);
+                    double DTAU23DZ = (TAU23F - TAU23B) * COE3;
+                    WF3(s->hy, IC, JC, KC) = WF3(s->hy, IC, JC, KC) + DTAU21DX + DTAU22DY + DTAU23DZ;
+                }
+                {
+                    const double COE1 = g->dxi * CV, COE3 = 2.0 * g->dzi * CV, COE2 = g->dyfi[JJ - 1] * CV;
+                    double DWDZ = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JC, KC, 3)) * g->dzi;
+                    double TAU33F = 0.5 * (W3(MU0, IC, JC, KC) + W3(MU, IC, JC, KC)) * (DWDZ - W3(DU, IC, JC, KC));
+                    DWDZ = (WG(Q, IC, JC, KC, 3) - WG(Q, IC, JC, KM, 3)) * g->dzi;
+                    double TAU33B = 0.5 * (W3(MU0, IC, JC, KM) + W3(MU, IC, JC, KM)) * (DWDZ - W3(DU, IC, JC, KM));
+                    double DTAU33DZ = (TAU33F - TAU33B) * COE3;
+                    double DWDX = (WG(Q, IP, JC, KC, 3) - WG(Q, IC, JC, KC, 3)) * g->dxi;
+                    double DUDZ = (WG(Q, IP, JC, KC, 1) - WG(Q, IP, JC, KM, 1)) * g->dzi;
+                    double TAU31F = WG(MS, IP, JC, KC, 2) * (DWDX + DUDZ);
+                    DWDX = (WG(Q, IC, JC, KC, 3) - WG(Q, IM, JC, KC, 3)) * g->dxi;
+                    DUDZ = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JC, KM, 1)) * g->dzi;
+                    double TAU31B = WG(MS, IC, JC, KC, 2) * (DWDX + DUDZ);
+                    double DTAU31DX = (TAU31F - TAU31B) * COE1;
+                    double DWDY = (WG(Q, IC, JP, KC, 3) - WG(Q, IC, JC, KC, 3)) * g->dyci[JJP - 1];
+                    double DVDZ = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JP, KM, 2)) * g->dzi;
+                    double TAU32F = WG(MS, IC, JP, KC, 3) * (DWDY + DVDZ);
+                    DWDY = (WG(Q, IC, JC, KC, 3) - WG(Q, IC, JM, KC, 3)) * g->dyci[JJ - 1];
+                    DVDZ = (WG(Q, IC, JC, KC, 2) - WG(Q, IC, JC, KM, 2)) * g->dzi;
+                    double TAU32B = WG(MS, IC, JC, KC, 3) * (DWDY + DVDZ);
+                    double DTAU32DY = (TAU32F - TAU32B) * COE2;
+                    WF3(s->hz, IC, JC, KC) = WF3(s->hz, IC, JC, KC) + DTAU31DX + DTAU32DY + DTAU33DZ;
+                }
+            }
+        }
+    }
+}
+
+void orc_io_substep(const orc_grid *g, orc_io *s, const orc_io_params *p, int NS, double DT)
+{
+    const size_t ngh = (size_t)WL * (WM + 2) * WN;
+    memcpy(s->g0, s->g, 3 * ngh * sizeof(double));
+    orc_io_terms(g, s);
+    io_rhs_mom(g, s, p, NS, 1, DT, s->hx);
+    io_rhs_mom(g, s, p, NS, 2, DT, s->hy);
+    io_rhs_mom(g, s, p, NS, 3, DT, s->hz);
+    /* MASSFLUX_CALC_IO, then INTFC_VARS3 + BC_WALL_G_io */
+    for (int KC = 1; KC <= WN; ++KC)
+        for (int JC = 1; JC <= WM; ++JC)
+            for (int IC = 1; IC <= WL; ++IC) {
+                WG(s->g, IC, JC, KC, 1) = WG(s->g, IC, JC, KC, 1) + WF3(s->hx, IC, JC, KC);
+                WG(s->g, IC, JC, KC, 3) = WG(s->g, IC, JC, KC, 3) + WF3(s->hz, IC, JC, KC);
+                if (JC >= 2) WG(s->g, IC, JC, KC, 2) = WG(s->g, IC, JC, KC, 2) + WF3(s->hy, IC, JC, KC);
+            }
+    io_wrap3(g, s->g);
+    io_bc_wall_vec(g, s->g);
+    /* DIVG_io(NS) */
+    const double COE0 = 1.0 / DT / talp(NS);
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= WM; ++JC) {
+        const int JP = JC + 1;
+        const double COE1 = g->dxi / 1.0, COE2 = g->dyfi[JC - 1] / 1.0;
+        for (int KC = 1; KC <= WN; ++KC) {
+            const int KP = WKPV(KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IP = WIPV(IC);
+                double DIVX = (WG(s->g, IP, JC, KC, 1) - WG(s->g, IC, JC, KC, 1)) * COE1;
+                double DIVY = (WG(s->g, IC, JP, KC, 2) - WG(s->g, IC, JC, KC, 2)) * COE2;
+                double DIVZ = (WG(s->g, IC, JC, KP, 3) - WG(s->g, IC, JC, KC, 3)) * g->dzi;
+                double DTRHO = W3(s->drhodtp, IC, JC, KC) / 1.0;
+                double DQCAP = DIVX + DIVY + DIVZ + DTRHO;
+                WF3(s->rhsllphi, IC, JC, KC) = DQCAP * COE0;
+            }
+        }
+    }
+    orc_poisson_fft99(g, s->rhsllphi, s->dph, 0);
+    orc_bc_wall_neumann(g, s->dph);
+    /* PRCALC_io, explicit branch; INTFC_VARS1 + BC_WALL_PR_io */
+    for (int KC = 1; KC <= WN; ++KC)
+        for (int JC = 1; JC <= WM; ++JC)
+            for (int IC = 1; IC <= WL; ++IC) W3(s->pr, IC, JC, KC) = W3(s->pr, IC, JC, KC) + W3(s->dph, IC, JC, KC);
+    io_wrap(g, s->pr);
+    orc_bc_wall_neumann(g, s->pr);
+    /* MASSFLUX_UPDATE_IO */
+    {
+        const double CX = DT * talp(NS) * g->dxi, CZ = DT * talp(NS) * g->dzi;
+#pragma omp parallel for schedule(static)
+        for (int JC = 1; JC <= WM; ++JC)
+            for (int KC = 1; KC <= WN; ++KC) {
+                const int KM = WKMV(KC);
+                const double CY = DT * talp(NS) * g->dyci[JC - 1] / 1.0;
+                for (int IC = 1; IC <= WL; ++IC) {
+                    const int IM = WIMV(IC);
+                    double DFX11 = W3(s->dph, IC, JC, KC) - W3(s->dph, IM, JC, KC);
+                    WG(s->g, IC, JC, KC, 1) = WG(s->g, IC, JC, KC, 1) - DFX11 * CX;
+                    double DFX33 = W3(s->dph, IC, JC, KC) - W3(s->dph, IC, JC, KM);
+                    WG(s->g, IC, JC, KC, 3) = WG(s->g, IC, JC, KC, 3) - DFX33 * CZ;
+                    if (JC >= 2) {
+                        double DFX22 = W3(s->dph, IC, JC, KC) - W3(s->dph, IC, JC - 1, KC);
+                        WG(s->g, IC, JC, KC, 2) = WG(s->g, IC, JC, KC, 2) - DFX22 * CY;
+                    }
+                }
+            }
+    }
+    io_wrap3(g, s->g);
+    io_bc_wall_vec(g, s->g);
+}
+
+/* DIVGCK_io (src/DIVGCK.f90:61-124, periodic x): max |div G + DrhoDtP| and
+ * max |div Q| over the interior. */
+void orc_divgck_io(const orc_grid *g, const orc_io *s, double *out)
+{
+    double DG = 0.0, DQ = 0.0;
+#pragma omp parallel for reduction(max : DG, DQ) schedule(static)
+    for (int JC = 1; JC <= WM; ++JC) {
+        const int JP = JC + 1;
+        const double COE2 = g->dyfi[JC - 1] * 1.0, COE3 = g->dzi * 1.0;
+        for (int KC = 1; KC <= WN; ++KC) {
+            const int KP = WKPV(KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IP = WIPV(IC);
+                double d = (WG(s->g, IP, JC, KC, 1) - WG(s->g, IC, JC, KC, 1)) * g->dxi +
+                           (WG(s->g, IC, JP, KC, 2) - WG(s->g, IC, JC, KC, 2)) * COE2 +
+                           (WG(s->g, IC, JC, KP, 3) - WG(s->g, IC, JC, KC, 3)) * COE3 + W3(s->drhodtp, IC, JC, KC);
+                if (fabs(d) > DG) DG = fabs(d);
+                d = (WG(s->q, IP, JC, KC, 1) - WG(s->q, IC, JC, KC, 1)) * g->dxi +
+                    (WG(s->q, IC, JP, KC, 2) - WG(s->q, IC, JC, KC, 2)) * COE2 +
+                    (WG(s->q, IC, JC, KP, 3) - WG(s->q, IC, JC, KC, 3)) * COE3;
+                if (fabs(d) > DQ) DQ = fabs(d);
+            }
+        }
+    }
+    out[0] = DG;
+    out[1] = DQ;
+}
+#undef WL
+#undef WM
+#undef WN
+#undef W3
+#undef WF3
+#undef WG
+#undef WE
+#undef WIPV
+#undef WIMV
+#undef WKPV
+#undef WKMV

@@ -92,6 +92,14 @@ def lib():
                                          C.c_double]
         L.orc_energy_substep.restype = C.c_long
         L.orc_num_threads.restype = C.c_int
+        for name in ("orc_io_prepare", "orc_io_substep"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int] + \
+                ([C.c_double] if name == "orc_io_substep" else [])
+            getattr(L, name).restype = None
+        L.orc_io_terms.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_io_terms.restype = None
+        L.orc_divgck_io.argtypes = [C.c_void_p, C.c_void_p, _dp]
+        L.orc_divgck_io.restype = None
         _lib = L
     return _lib
 
@@ -303,6 +311,83 @@ def thermal_case(og: "OracleGrid", scaled, seed=0, qw=0.1):
     g0[:, M + 1, :, 1] = 0.0
     st["g"], st["g0"] = f(g), f(g0)
     st["wallflux"] = f(np.stack([np.full(L, qw), np.full(L, qw)], 1))
+    return st, tab, n
+
+
+class _IO(C.Structure):
+    _fields_ = [(n, _dp) for n in ("q", "g", "g0", "pr", "rho", "rho0", "mu", "mu0", "drhodtp", "divu", "rho_stg",
+                                   "mu_stg", "explt0", "dph", "hx", "hy", "hz", "rhsllphi")] + [("dpgrns", C.c_double)]
+
+
+class _IOParams(C.Structure):
+    _fields_ = [("thermal", C.c_int), ("nflow", C.c_int), ("flowdrvtp", C.c_int), ("gravdir", C.c_int),
+                ("cfgv", C.c_double), ("f_a", C.c_double)]
+
+
+IO_FIELDS = ("q", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0")
+
+
+class OracleIO:
+    """The io momentum substep (SOLVERRK3_MOM_IO, explicit viscous terms) and
+    the property staging before it, on the numpy arrays of st (Fortran order,
+    updated in place).  st holds g g0 rho rho0 mu mu0 (shared with the energy
+    substep's dict) and q pr drhodtp divu rho_stg mu_stg explt0."""
+
+    def __init__(self, og: "OracleGrid", st: dict, thermal=1, nflow=1, flowdrvtp=1, gravdir=0, cfgv=0.0, f_a=0.0):
+        L, M, N = og.L, og.M, og.N
+        for k in ("g", "g0", "rho", "rho0", "mu", "mu0") + IO_FIELDS:
+            a = st[k]
+            assert a.dtype == np.float64 and a.flags.f_contiguous, k
+        self.og, self.st = og, st
+        self.work = {"dph": np.zeros((L, M + 2, N), order="F")}
+        for k in ("hx", "hy", "hz", "rhsllphi"):
+            self.work[k] = np.zeros((L, M, N), order="F")
+        self.p = _IOParams(thermal, nflow, flowdrvtp, gravdir, cfgv, f_a)
+        self.s = _IO(*[_ptr(st[k]) for k in ("q", "g", "g0", "pr", "rho", "rho0", "mu", "mu0", "drhodtp", "divu",
+                                               "rho_stg", "mu_stg", "explt0")],
+                     *[_ptr(self.work[k]) for k in ("dph", "hx", "hy", "hz", "rhsllphi")], 0.0)
+
+    @property
+    def dpgrns(self):
+        return self.s.dpgrns
+
+    def prepare(self, ns: int):
+        """DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered, VELOCITY_CALC_io."""
+        lib().orc_io_prepare(C.byref(self.og.g), C.byref(self.s), C.byref(self.p), ns)
+
+    def terms(self):
+        """Convection, DIVG_U and explicit viscous terms -> work hx, hy, hz."""
+        lib().orc_io_terms(C.byref(self.og.g), C.byref(self.s))
+        return self.work["hx"], self.work["hy"], self.work["hz"]
+
+    def substep(self, ns: int, dt: float):
+        lib().orc_io_substep(C.byref(self.og.g), C.byref(self.s), C.byref(self.p), ns, dt)
+
+    def divgck(self):
+        """DIVGCK_io: (max|div G + DrhoDtP|, max|div Q|)."""
+        out = np.zeros(2)
+        lib().orc_divgck_io(C.byref(self.og.g), C.byref(self.s), _ptr(out))
+        return float(out[0]), float(out[1])
+
+
+def io_case(og: "OracleGrid", scaled, seed=0, qw=0.1):
+    """thermal_case plus the io momentum state: a smooth pressure, small
+    explicit-term histories, and the staged arrays initialised as the
+    reference allocates them (RHO_STG = MU_STG = 1, DrhoDtP = 0, Q = 0)."""
+    st, tab, n = thermal_case(og, scaled, seed=seed, qw=qw)
+    L, M, N = og.L, og.M, og.N
+    rng = np.random.default_rng(seed + 1000)
+    f = lambda a: np.asfortranarray(a, dtype=np.float64)
+    x = np.arange(L) / L * 2 * np.pi
+    z = np.arange(N) / N * 2 * np.pi
+    pr = 0.01 * np.sin(x)[:, None, None] * np.cos(2 * z)[None, None, :] * np.ones((1, M + 2, 1))
+    st["pr"] = f(pr + 1e-3 * rng.standard_normal((L, M + 2, N)))
+    st["q"] = f(np.zeros((L, M + 2, N, 3)))
+    st["drhodtp"] = f(np.zeros((L, M + 2, N)))
+    st["divu"] = f(np.zeros((L, M + 2, N)))
+    st["rho_stg"] = f(np.ones((L, M + 2, N, 3)))
+    st["mu_stg"] = f(np.ones((L, M + 2, N, 3)))
+    st["explt0"] = f(1e-2 * rng.standard_normal((L, M, N, 3)))
     return st, tab, n
 
 
