@@ -1,0 +1,619 @@
+// The io (main-domain) momentum substep of the heated periodic channel on one
+// GPU: variable properties, mass fluxes G = rho u, explicit viscous terms
+// (visthemflg = 0), no weighted pressure.
+//
+//   k_io_prepare   DENSITY_TIME_DERIVATION(NS) (src/THERM_PROP_UPDATE.f90:
+//                  225-291), DENSITY_Staggered, MU_Staggered and
+//                  VELOCITY_CALC_io (src/MASSFLUX_CALC_IO.f90:82-311)
+//   k_io_divu      DIVG_U_io (src/DIVG.f90:153-259)
+//   k_io_terms     CONVECTION_X/Y/Z_io (src/CONVECTION_X.f90:77-176,
+//                  src/CONVECTION_Y.f90:128-287, src/CONVECTION_Z.f90:115-314),
+//                  VISCOUS_ALL_EXPLT_X/Y/Z_io (src/VISCOUS_X_EXPLICIT_io.f90:
+//                  1-113, src/VISCOUS_Y_EXPLICIT_io.f90:1-99,
+//                  src/VISCOUS_Z_EXPLICIT_io.f90:1-100) and the explicit
+//                  branch of RHS_MOM_EXPLICIT_io (src/RHS_MOM_EXPLICIT_io.f90:
+//                  1-297) up to DPGRNS, whose volume integral it sums per block
+//   k_io_dpgrns    the constant-mass-flux DPGRNS from the block sums
+//   k_io_massflux  RHS - DPGRNS, MASSFLUX_CALC_IO (src/MASSFLUX_CALC_IO.f90:
+//                  1-79) and BC_WALL_G_io (src/BC_WALL.f90:412-458)
+//   k_io_divg      DIVG_io(NS) (src/DIVG.f90:60-150)
+//   FFT99 Poisson  the reference's "Method One" for the io domain
+//                  (src/SOLVERRK3_MOM_IO.f90:123), Neumann ghosts
+//   k_io_update    PRCALC_io (explicit branch, src/PRESSURE_CORRECTION.f90:
+//                  88-156) and MASSFLUX_UPDATE_IO (src/VELO_CORRECTION.f90:
+//                  109-185) with the wall conditions
+//   k_io_divgck    DIVGCK_io (src/DIVGCK.f90:61-124)
+//
+// INTFC_VARS on one GPU is the periodic y wrap the reference's exchange
+// produces across its ranks (row 0 <- row M, row M+1 <- row 1), after which
+// the wall routines overwrite what they set.  Channel metrics: RCCI1 = RNDI1 =
+// RCCI2 = 1, XND2CL = YND2CL = ZND2CL = 1/2.  Every expression is evaluated
+// in the reference's order without contraction into fused multiply-adds, so
+// the element-wise results are an FP64 evaluation of the reference's
+// expressions; only the DPGRNS volume sum is summed in a different order.
+//
+// Arrays (x fastest): Q, G, G0, RHO_STG, MU_STG (NCL1, 0:NCL2+1, NCL3, 3);
+// PR, DrhoDtP, DivU, DENSITY*, VISCOUSITY* (NCL1, 0:NCL2+1, NCL3);
+// EXPLT0 (NCL1, NCL2, NCL3, 3).
+#include <algorithm>
+#include <cmath>
+
+#include "chs_common.hpp"
+
+namespace chs {
+
+int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
+
+struct IoGeo {
+    int L, M, N;
+    double dxi, dzi, cvisc, volm;
+    const double *dyfi, *dyci, *wff, *wfb;
+};
+
+struct IoArrays {
+    double *q, *g, *g0, *pr, *drhodtp, *divu, *rs, *ms, *explt0;
+    const double *rho, *rho0, *mu, *mu0;
+};
+
+struct IoParams {
+    int thermal, nflow, flowdrvtp, gravdir;
+    double cfgv, f_a;
+};
+
+static const double TGAM_IO[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double TROH_IO[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+#define IO_CELL                                                                  \
+    const int L = g.L, M = g.M, N = g.N;                                         \
+    const int IC = blockIdx.x * 64 + threadIdx.x;                                \
+    const int KC = blockIdx.y * 4 + threadIdx.y;                                 \
+    if (IC >= L || KC >= N) return;                                              \
+    const int IP = IC == L - 1 ? 0 : IC + 1, IM = IC == 0 ? L - 1 : IC - 1;      \
+    const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;      \
+    const int LP = L * (M + 2), CS = LP * N;                                     \
+    auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };           \
+    (void)IP; (void)IM; (void)KP; (void)KM; (void)CS
+
+// DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered (thermal = 1) and
+// VELOCITY_CALC_io with BC_WALL_Q_io; one thread per cell of rows 0..M+1.
+__global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, IoArrays a, int thermal, double ca, double cb,
+                                                    double cd) {
+#pragma clang fp contract(off)
+    IO_CELL;
+    const int J = blockIdx.z;   // ghost-aware row 0..M+1
+    const int c = at(IC, J, KC);
+    const double *D = a.rho, *D0 = a.rho0, *V = a.mu, *V0 = a.mu0;
+    if (thermal) {
+        if (J >= 1 && J <= M) {
+            const double* G = a.g;
+            const double* G0 = a.g0;
+            const double COE2 = g.dyfi[J - 1] * 1.0, COE3 = g.dzi * 1.0;
+            const double DIVX = (G[at(IP, J, KC)] - G[c]) * g.dxi;
+            const double DIVX0 = (G0[at(IP, J, KC)] - G0[c]) * g.dxi;
+            const double DIVY = (G[CS + at(IC, J + 1, KC)] - G[CS + c]) * COE2;
+            const double DIVY0 = (G0[CS + at(IC, J + 1, KC)] - G0[CS + c]) * COE2;
+            const double DIVZ = (G[2 * CS + at(IC, J, KP)] - G[2 * CS + c]) * COE3;
+            const double DIVZ0 = (G0[2 * CS + at(IC, J, KP)] - G0[2 * CS + c]) * COE3;
+            a.drhodtp[c] = (ca * (DIVX + DIVY + DIVZ) + cb * (DIVX0 + DIVY0 + DIVZ0)) / cd;
+        }
+        // rows whose values the y wrap copies: 0 <- M, M+1 <- 1
+        const int Jw = J == 0 ? M : (J == M + 1 ? 1 : J);
+        // RHO_STG x and z faces: rows 1..M, wrapped ghosts
+        a.rs[c] = (D[at(IC, Jw, KC)] + D[at(IM, Jw, KC)] + D0[at(IC, Jw, KC)] + D0[at(IM, Jw, KC)]) * 0.5 * 0.5;
+        a.rs[2 * CS + c] =
+            (D[at(IC, Jw, KC)] + D[at(IC, Jw, KM)] + D0[at(IC, Jw, KC)] + D0[at(IC, Jw, KM)]) * 0.5 * 0.5;
+        // RHO_STG y face: rows 2..M; row 0 <- M; rows 1 and M+1 are the wall values
+        if (J == 1 || J == M + 1) {
+            a.rs[CS + c] = 0.5 * (D0[at(IC, J == 1 ? 0 : M + 1, KC)] + D[at(IC, J == 1 ? 0 : M + 1, KC)]);
+        } else {
+            const int Jr = J == 0 ? M : J, Jm = Jr - 1;
+            a.rs[CS + c] = (g.wff[Jr - 1] * (D[at(IC, Jr, KC)] + D0[at(IC, Jr, KC)]) +
+                            g.wfb[Jr - 1] * (D[at(IC, Jm, KC)] + D0[at(IC, Jm, KC)])) *
+                           0.5;
+        }
+        // MU_STG x-z edge: rows 1..M, wrapped ghosts
+        a.ms[CS + c] = (V[at(IC, Jw, KC)] + V[at(IM, Jw, KC)] + V0[at(IC, Jw, KC)] + V0[at(IM, Jw, KC)] +
+                        V[at(IC, Jw, KM)] + V[at(IM, Jw, KM)] + V0[at(IC, Jw, KM)] + V0[at(IM, Jw, KM)]) *
+                       0.5 * 0.5 * 0.5;
+        // MU_STG x-y and y-z edges: rows 2..M; row 0 <- M; rows 1, M+1 wall values
+        if (J == 1 || J == M + 1) {
+            const int Jg = J == 1 ? 0 : M + 1;
+            const double w = 0.5 * (V0[at(IC, Jg, KC)] + V[at(IC, Jg, KC)]);
+            a.ms[c] = w;
+            a.ms[2 * CS + c] = w;
+        } else {
+            const int Jr = J == 0 ? M : J, Jm = Jr - 1;
+            const double wff = g.wff[Jr - 1], wfb = g.wfb[Jr - 1];
+            a.ms[c] = ((V[at(IC, Jr, KC)] + V[at(IM, Jr, KC)] + V0[at(IC, Jr, KC)] + V0[at(IM, Jr, KC)]) * wff +
+                       (V[at(IC, Jm, KC)] + V[at(IM, Jm, KC)] + V0[at(IC, Jm, KC)] + V0[at(IM, Jm, KC)]) * wfb) *
+                      0.5 * 0.5;
+            a.ms[2 * CS + c] =
+                ((V[at(IC, Jr, KM)] + V[at(IC, Jr, KC)] + V0[at(IC, Jr, KM)] + V0[at(IC, Jr, KC)]) * wff +
+                 (V[at(IC, Jm, KM)] + V[at(IC, Jm, KC)] + V0[at(IC, Jm, KM)] + V0[at(IC, Jm, KC)]) * wfb) *
+                0.5 * 0.5;
+        }
+    }
+    // VELOCITY_CALC_io + BC_WALL_Q_io (the wrapped ghost rows are then zeroed)
+    if (J == 0 || J == M + 1) {
+        a.q[c] = 0.0;
+        a.q[CS + c] = 0.0;
+        a.q[2 * CS + c] = 0.0;
+    } else {
+        a.q[c] = a.g[c] / a.rs[c];
+        a.q[2 * CS + c] = a.g[2 * CS + c] / a.rs[2 * CS + c];
+        a.q[CS + c] = J >= 2 ? a.g[CS + c] / a.rs[CS + c] : 0.0;
+    }
+}
+
+// DIVG_U_io on rows 1..M and the y wrap of its ghost rows
+__global__ void __launch_bounds__(256) k_io_divu(IoGeo g, const double* __restrict__ Q, double* __restrict__ divu) {
+#pragma clang fp contract(off)
+    IO_CELL;
+    const int J = blockIdx.z;
+    const int Jr = J == 0 ? M : (J == M + 1 ? 1 : J);
+    const int c = at(IC, Jr, KC);
+    const double DIVX = (Q[at(IP, Jr, KC)] - Q[c]) * g.dxi;
+    const double DIVY = (Q[CS + at(IC, Jr + 1, KC)] - Q[CS + c]) * (g.dyfi[Jr - 1] * 1.0);
+    const double DIVZ = (Q[2 * CS + at(IC, Jr, KP)] - Q[2 * CS + c]) * (g.dzi * 1.0);
+    divu[at(IC, J, KC)] = (DIVX + DIVY + DIVZ) / 3.0;
+}
+
+template <int NT>
+__device__ __forceinline__ double io_block_sum(double v, double* red) {
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    red[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// Explicit terms and RHS_MOM_EXPLICIT_io up to DPGRNS, rows 1..M.
+// inc[c] <- RHS of each component (row 1 of y untouched), explt0 <- the
+// explicit terms, part[block] <- sum of RHS/DYFI of the NFLOW component.
+__global__ void __launch_bounds__(256) k_io_terms(IoGeo g, IoArrays a, IoParams p, double tg, double tr, double dt,
+                                                  double adt, double* __restrict__ inc, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    const int L = g.L, M = g.M, N = g.N;
+    const int IC = blockIdx.x * 64 + threadIdx.x;
+    const int KC = blockIdx.y * 4 + threadIdx.y;
+    const int JC = blockIdx.z + 1;
+    const bool live = IC < L && KC < N;
+    double flow = 0.0;
+    if (live) {
+        const int IP = IC == L - 1 ? 0 : IC + 1, IM = IC == 0 ? L - 1 : IC - 1;
+        const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;
+        const int LP = L * (M + 2), CS = LP * N, NC = L * M * N;
+        auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
+        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC - 1;
+        const double wff = g.wff[JJ - 1], wfb = g.wfb[JJ - 1], wffp = g.wff[JJP - 1], wfbp = g.wfb[JJP - 1];
+        const double CV = g.cvisc;
+        const double* G = a.g;
+        const double* Q = a.q;
+        const double* MU = a.mu;
+        const double* MU0 = a.mu0;
+        const double* DU = a.divu;
+        const double* MS = a.ms;
+        const double* PR = a.pr;
+        const double *G1 = G, *G2 = G + CS, *G3 = G + 2 * CS;
+        const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
+        const double *MS1 = MS, *MS2 = MS + CS, *MS3 = MS + 2 * CS;
+        const int c = at(IC, JC, KC);
+        const int e = IC + L * (JC - 1) + L * M * KC;   // (L, M, N) index
+        // ---- x: CONVECTION_X_io + VISCOUS_ALL_EXPLT_X_io ----
+        double hx;
+        {
+            const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5 * 1.0;
+            const double COE32 = g.dzi * 0.5 * 0.5 * 1.0;
+            const double H11F = (G1[at(IP, JC, KC)] + G1[c]) * (Q1[at(IP, JC, KC)] + Q1[c]);
+            const double H11B = (G1[c] + G1[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IM, JC, KC)]);
+            const double H11 = (H11F - H11B) * COE1;
+            const double H12F = (G2[at(IC, JP, KC)] + G2[at(IM, JP, KC)]) * (wffp * Q1[at(IC, JP, KC)] + wfbp * Q1[c]);
+            const double H12B = (G2[c] + G2[at(IM, JC, KC)]) * (wff * Q1[c] + wfb * Q1[at(IC, JM, KC)]);
+            const double H12 = (H12F - H12B) * COE2;
+            const double H13F = (G3[at(IC, JC, KP)] + G3[at(IM, JC, KP)]) * (Q1[at(IC, JC, KP)] + Q1[c]);
+            const double H13B = (G3[c] + G3[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IC, JC, KM)]);
+            const double H13 = (H13F - H13B) * COE32;
+            hx = -(H11 + H12 + H13);
+        }
+        {
+            const double COE1 = 2.0 * g.dxi * CV, COE2 = g.dyfi[JJ - 1] * CV * 1.0, COE32 = g.dzi * CV * 1.0;
+            double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
+            const double TAU11F = 0.5 * (MU0[c] + MU[c]) * (DUDX - DU[c]);
+            DUDX = (Q1[c] - Q1[at(IM, JC, KC)]) * g.dxi;
+            const int cim = at(IM, JC, KC);
+            const double TAU11B = 0.5 * (MU0[cim] + MU[cim]) * (DUDX - DU[cim]);
+            const double DTAU11DX = (TAU11F - TAU11B) * COE1;
+            double DUDY = (Q1[at(IC, JP, KC)] - Q1[c]) * g.dyci[JJP - 1] / 1.0;
+            double DVDX = (Q2[at(IC, JP, KC)] - Q2[at(IM, JP, KC)]) * g.dxi;
+            const double TAU12F = MS1[at(IC, JP, KC)] * (DUDY + DVDX);
+            DUDY = (Q1[c] - Q1[at(IC, JM, KC)]) * g.dyci[JJ - 1] / 1.0;
+            DVDX = (Q2[c] - Q2[cim]) * g.dxi;
+            const double TAU12B = MS1[c] * (DUDY + DVDX);
+            const double DTAU12DY = (TAU12F - TAU12B) * COE2;
+            double DUDZ = (Q1[at(IC, JC, KP)] - Q1[c]) * g.dzi;
+            double DWDX = (Q3[at(IC, JC, KP)] - Q3[at(IM, JC, KP)]) * g.dxi;
+            const double TAU13F = MS2[at(IC, JC, KP)] * (DUDZ + DWDX);
+            DUDZ = (Q1[c] - Q1[at(IC, JC, KM)]) * g.dzi;
+            DWDX = (Q3[c] - Q3[cim]) * g.dxi;
+            const double TAU13B = MS2[c] * (DUDZ + DWDX);
+            const double DTAU13DZ = (TAU13F - TAU13B) * COE32;
+            hx = hx + DTAU11DX + DTAU12DY + DTAU13DZ;
+        }
+        // ---- y (rows 2..M): CONVECTION_Y_io + VISCOUS_ALL_EXPLT_Y_io ----
+        double hy = 0.0;
+        if (JC >= 2) {
+            {
+                const double COE1 = g.dxi * 0.5, COE2 = g.dyci[JJ - 1] * 0.5 * 0.5, COE31 = g.dzi * 0.5 * 1.0;
+                const double H21F = (wff * G1[at(IP, JC, KC)] + wfb * G1[at(IP, JM, KC)]) * (Q2[at(IP, JC, KC)] + Q2[c]);
+                const double H21B = (wff * G1[c] + wfb * G1[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IM, JC, KC)]);
+                const double H21 = (H21F - H21B) * COE1;
+                const double H22F = (G2[at(IC, JP, KC)] + G2[c]) * (Q2[at(IC, JP, KC)] + Q2[c]);
+                const double H22B = (G2[c] + G2[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JM, KC)]);
+                const double H22 = (H22F - H22B) * COE2;
+                const double H23F = (wff * G3[at(IC, JC, KP)] + wfb * G3[at(IC, JM, KP)]) * (Q2[at(IC, JC, KP)] + Q2[c]);
+                const double H23B = (wff * G3[c] + wfb * G3[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JC, KM)]);
+                const double H23 = (H23F - H23B) * COE31;
+                hy = -(H21 + H22 + H23);
+            }
+            {
+                const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = 2.0 * g.dyci[JJ - 1] * CV;
+                const int cjm = at(IC, JM, KC);
+                double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1];
+                const double TAU22F = 0.5 * (MU0[c] + MU[c]) * (DVDY - DU[c]);
+                DVDY = (Q2[c] - Q2[cjm]) * g.dyfi[JJM - 1];
+                const double TAU22B = 0.5 * (MU0[cjm] + MU[cjm]) * (DVDY - DU[cjm]);
+                const double DTAU22DY = (TAU22F - TAU22B) * COE2;
+                double DVDX = (Q2[at(IP, JC, KC)] - Q2[c]) * g.dxi;
+                double DUDY = (Q1[at(IP, JC, KC)] - Q1[at(IP, JM, KC)]) * g.dyci[JJ - 1];
+                const double TAU21F = MS1[at(IP, JC, KC)] * (DVDX + DUDY);
+                DVDX = (Q2[c] - Q2[at(IM, JC, KC)]) * g.dxi;
+                DUDY = (Q1[c] - Q1[cjm]) * g.dyci[JJ - 1];
+                const double TAU21B = MS1[c] * (DVDX + DUDY);
+                const double DTAU21DX = (TAU21F - TAU21B) * COE1;
+                double DVDZ = (Q2[at(IC, JC, KP)] - Q2[c]) * g.dzi;
+                double DWDY = (Q3[at(IC, JC, KP)] - Q3[at(IC, JM, KP)]) * g.dyci[JJ - 1];
+                const double TAU23F = MS3[at(IC, JC, KP)] * (DVDZ + DWDY);
+                DVDZ = (Q2[c] - Q2[at(IC, JC, KM)]) * g.dzi;
+                DWDY = (Q3[c] - Q3[cjm]) * g.dyci[JJ - 1];
+                const double TAU23B = MS3[c] * (DVDZ + DWDY);
+                const double DTAU23DZ = (TAU23F - TAU23B) * COE3;
+                hy = hy + DTAU21DX + DTAU22DY + DTAU23DZ;
+            }
+        }
+        // ---- z: CONVECTION_Z_io + VISCOUS_ALL_EXPLT_Z_io ----
+        double hz;
+        {
+            const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5;
+            const double COE31 = g.dzi * 0.5 * 0.5 * 1.0;
+            const double H31F = (G1[at(IP, JC, KC)] + G1[at(IP, JC, KM)]) * (Q3[at(IP, JC, KC)] + Q3[c]);
+            const double H31B = (G1[c] + G1[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IM, JC, KC)]);
+            const double H31 = (H31F - H31B) * COE1;
+            const double H32F = (G2[at(IC, JP, KC)] + G2[at(IC, JP, KM)]) * (wffp * Q3[at(IC, JP, KC)] + wfbp * Q3[c]);
+            const double H32B = (G2[c] + G2[at(IC, JC, KM)]) * (wff * Q3[c] + wfb * Q3[at(IC, JM, KC)]);
+            const double H32 = (H32F - H32B) * COE2;
+            const double H33F = (G3[at(IC, JC, KP)] + G3[c]) * (Q3[at(IC, JC, KP)] + Q3[c]);
+            const double H33B = (G3[c] + G3[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IC, JC, KM)]);
+            const double H33 = (H33F - H33B) * COE31;
+            hz = -(H31 + H32 + H33);
+        }
+        {
+            const double COE1 = g.dxi * CV, COE3 = 2.0 * g.dzi * CV, COE2 = g.dyfi[JJ - 1] * CV;
+            const int ckm = at(IC, JC, KM);
+            double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi;
+            const double TAU33F = 0.5 * (MU0[c] + MU[c]) * (DWDZ - DU[c]);
+            DWDZ = (Q3[c] - Q3[ckm]) * g.dzi;
+            const double TAU33B = 0.5 * (MU0[ckm] + MU[ckm]) * (DWDZ - DU[ckm]);
+            const double DTAU33DZ = (TAU33F - TAU33B) * COE3;
+            double DWDX = (Q3[at(IP, JC, KC)] - Q3[c]) * g.dxi;
+            double DUDZ = (Q1[at(IP, JC, KC)] - Q1[at(IP, JC, KM)]) * g.dzi;
+            const double TAU31F = MS2[at(IP, JC, KC)] * (DWDX + DUDZ);
+            DWDX = (Q3[c] - Q3[at(IM, JC, KC)]) * g.dxi;
+            DUDZ = (Q1[c] - Q1[ckm]) * g.dzi;
+            const double TAU31B = MS2[c] * (DWDX + DUDZ);
+            const double DTAU31DX = (TAU31F - TAU31B) * COE1;
+            double DWDY = (Q3[at(IC, JP, KC)] - Q3[c]) * g.dyci[JJP - 1];
+            double DVDZ = (Q2[at(IC, JP, KC)] - Q2[at(IC, JP, KM)]) * g.dzi;
+            const double TAU32F = MS3[at(IC, JP, KC)] * (DWDY + DVDZ);
+            DWDY = (Q3[c] - Q3[at(IC, JM, KC)]) * g.dyci[JJ - 1];
+            DVDZ = (Q2[c] - Q2[ckm]) * g.dzi;
+            const double TAU32B = MS3[c] * (DWDY + DVDZ);
+            const double DTAU32DY = (TAU32F - TAU32B) * COE2;
+            hz = hz + DTAU31DX + DTAU32DY + DTAU33DZ;
+        }
+        // ---- RHS_MOM_EXPLICIT_io: RK3 combine, pressure gradient, buoyancy ----
+        double* E = a.explt0;
+        {
+            const double RHSL = E[e];
+            double r = (tg * hx + tr * RHSL) * dt;
+            E[e] = hx;
+            const double PGM = (PR[c] - PR[at(IM, JC, KC)]) * (adt * g.dxi);
+            r = r - PGM;
+            if (p.gravdir == 1) r = r + a.rs[c] * (adt * p.f_a / 1.0);
+            inc[e] = r;
+            if (p.nflow == 1) flow = r / g.dyfi[JC - 1] / 1.0;
+        }
+        if (JC >= 2) {
+            const double RHSL = E[NC + e];
+            double r = (tg * hy + tr * RHSL) * dt;
+            E[NC + e] = hy;
+            const double PGM = (PR[c] - PR[at(IC, JM, KC)]) * (adt * g.dyci[JC - 1] / 1.0);
+            r = r - PGM;
+            if (p.gravdir == 2) r = r + a.rs[CS + c] * (adt * p.f_a / 1.0);
+            inc[NC + e] = r;
+            if (p.nflow == 2) flow = r / g.dyfi[JC - 1] / 1.0;
+        }
+        {
+            const double RHSL = E[2 * NC + e];
+            double r = (tg * hz + tr * RHSL) * dt;
+            E[2 * NC + e] = hz;
+            const double PGM = (PR[c] - PR[at(IC, JC, KM)]) * (adt * g.dzi);
+            r = r - PGM;
+            if (p.gravdir == 3) r = r + a.rs[2 * CS + c] * (adt * p.f_a / 1.0);
+            inc[2 * NC + e] = r;
+            if (p.nflow == 3) flow = r / g.dyfi[JC - 1] / 1.0;
+        }
+    }
+    const double bs = io_block_sum<256>(flow, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = bs;
+}
+
+// DPGRNS of the constant-mass-flux drive: the block sums of RHS/DYFI in a
+// fixed order, divided by VOLM; for the pressure-gradient drive the
+// constant the host passes.
+__global__ void __launch_bounds__(256) k_io_dpgrns(const double* __restrict__ part, int nb, double volm,
+                                                   int flowdrvtp, double cst, double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = flowdrvtp == 1 ? red[0] / volm : cst;
+}
+
+// RHS - DPGRNS along NFLOW, MASSFLUX_CALC_IO, then INTFC_VARS3 + BC_WALL_G_io
+// (every ghost row zero, v on the bottom wall face zero); rows 0..M+1.
+__global__ void __launch_bounds__(256) k_io_massflux(IoGeo g, double* __restrict__ G, const double* __restrict__ inc,
+                                                     const double* __restrict__ dpg, int nflow) {
+#pragma clang fp contract(off)
+    IO_CELL;
+    const int J = blockIdx.z;
+    const int c = at(IC, J, KC);
+    if (J == 0 || J == M + 1) {
+        G[c] = 0.0;
+        G[CS + c] = 0.0;
+        G[2 * CS + c] = 0.0;
+        return;
+    }
+    const int NC = L * M * N;
+    const int e = IC + L * (J - 1) + L * M * KC;
+    const double dp = *dpg;
+    G[c] = G[c] + (nflow == 1 ? inc[e] - dp : inc[e]);
+    G[2 * CS + c] = G[2 * CS + c] + (nflow == 3 ? inc[2 * NC + e] - dp : inc[2 * NC + e]);
+    if (J >= 2)
+        G[CS + c] = G[CS + c] + (nflow == 2 ? inc[NC + e] - dp : inc[NC + e]);
+    else
+        G[CS + c] = 0.0;
+}
+
+// DIVG_io(NS): RHSLLPHI = (div G + DrhoDtP) / (DT TALP), rows 1..M
+__global__ void __launch_bounds__(256) k_io_divg(IoGeo g, const double* __restrict__ G,
+                                                 const double* __restrict__ drhodtp, double coe0,
+                                                 double* __restrict__ f) {
+#pragma clang fp contract(off)
+    IO_CELL;
+    const int JC = blockIdx.z + 1;
+    const int c = at(IC, JC, KC);
+    const double DIVX = (G[at(IP, JC, KC)] - G[c]) * (g.dxi / 1.0);
+    const double DIVY = (G[CS + at(IC, JC + 1, KC)] - G[CS + c]) * (g.dyfi[JC - 1] / 1.0);
+    const double DIVZ = (G[2 * CS + at(IC, JC, KP)] - G[2 * CS + c]) * g.dzi;
+    const double DTRHO = drhodtp[c] / 1.0;
+    const double DQCAP = DIVX + DIVY + DIVZ + DTRHO;
+    f[IC + L * (JC - 1) + L * M * KC] = DQCAP * coe0;
+}
+
+// PRCALC_io (explicit) with BC_WALL_PR_io and MASSFLUX_UPDATE_IO, rows 1..M
+__global__ void __launch_bounds__(256) k_io_update(IoGeo g, double* __restrict__ G, double* __restrict__ PR,
+                                                   const double* __restrict__ DPH, double dta) {
+#pragma clang fp contract(off)
+    IO_CELL;
+    const int JC = blockIdx.z + 1;
+    const int c = at(IC, JC, KC);
+    const double pn = PR[c] + DPH[c];
+    PR[c] = pn;
+    if (JC == 1) PR[at(IC, 0, KC)] = pn;
+    if (JC == M) PR[at(IC, M + 1, KC)] = pn;
+    const double DFX11 = DPH[c] - DPH[at(IM, JC, KC)];
+    G[c] = G[c] - DFX11 * (dta * g.dxi);
+    const double DFX33 = DPH[c] - DPH[at(IC, JC, KM)];
+    G[2 * CS + c] = G[2 * CS + c] - DFX33 * (dta * g.dzi);
+    if (JC >= 2) {
+        const double DFX22 = DPH[c] - DPH[at(IC, JC - 1, KC)];
+        G[CS + c] = G[CS + c] - DFX22 * (dta * g.dyci[JC - 1] / 1.0);
+    }
+}
+
+// NaN-propagating maximum: a NaN operand wins and stays
+__device__ __forceinline__ double io_nanmax(double a, double b) { return (b != b || b > a) ? b : a; }
+
+// DIVGCK_io: block maxima of |div G + DrhoDtP| and |div Q| over rows 1..M
+__global__ void __launch_bounds__(256) k_io_divgck(IoGeo g, const double* __restrict__ G, const double* __restrict__ Q,
+                                                   const double* __restrict__ drhodtp, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double r0[256], r1[256];
+    const int L = g.L, M = g.M, N = g.N;
+    const int LP = L * (M + 2), CS = LP * N;
+    const long ncell = (long)L * M * N;
+    double m0 = 0.0, m1 = 0.0;
+    for (long e = blockIdx.x * 256 + threadIdx.x; e < ncell; e += (long)gridDim.x * 256) {
+        const int IC = (int)(e % L), JC = (int)((e / L) % M) + 1, KC = (int)(e / ((long)L * M));
+        const int IP = IC == L - 1 ? 0 : IC + 1, KP = KC == N - 1 ? 0 : KC + 1;
+        const int c = IC + L * JC + LP * KC, ckp = IC + L * JC + LP * KP;
+        const double COE2 = g.dyfi[JC - 1] * 1.0, COE3 = g.dzi * 1.0;
+        double d = (G[c - IC + IP] - G[c]) * g.dxi + (G[CS + c + L] - G[CS + c]) * COE2 +
+                   (G[2 * CS + ckp] - G[2 * CS + c]) * COE3 + drhodtp[c];
+        m0 = io_nanmax(m0, fabs(d));
+        d = (Q[c - IC + IP] - Q[c]) * g.dxi + (Q[CS + c + L] - Q[CS + c]) * COE2 + (Q[2 * CS + ckp] - Q[2 * CS + c]) * COE3;
+        m1 = io_nanmax(m1, fabs(d));
+    }
+    r0[threadIdx.x] = m0;
+    r1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            r0[threadIdx.x] = io_nanmax(r0[threadIdx.x], r0[threadIdx.x + st]);
+            r1[threadIdx.x] = io_nanmax(r1[threadIdx.x], r1[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r0[0];
+        part[2 * blockIdx.x + 1] = r1[0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_io_max2(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    __shared__ double r0[256], r1[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        a = io_nanmax(a, part[2 * i]);
+        b = io_nanmax(b, part[2 * i + 1]);
+    }
+    r0[threadIdx.x] = a;
+    r1[threadIdx.x] = b;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            r0[threadIdx.x] = io_nanmax(r0[threadIdx.x], r0[threadIdx.x + st]);
+            r1[threadIdx.x] = io_nanmax(r1[threadIdx.x], r1[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = r0[0];
+        out[1] = r1[0];
+    }
+}
+
+static IoGeo io_geo(const chs_handle* h) {
+    return IoGeo{h->L, h->M, h->N, h->dxi, h->dzi, h->cvisc, h->volm, h->dyfi, h->dyci, h->wff, h->wfb};
+}
+
+static int io_arrays(const chs_thermal_fields* tf, const chs_io_fields* f, IoArrays* a, const char* who) {
+    CHS_CHECK(tf && f, CHS_E_ARG, std::string(who) + ": NULL argument");
+    const double* req[] = {f->q, f->g, f->g0, f->pr, f->drhodtp, f->divu, f->rho_stg, f->mu_stg, f->explt0,
+                           tf->density, tf->density0, tf->viscosity, tf->viscosity0};
+    for (const double* r : req) CHS_CHECK(r != nullptr, CHS_E_ARG, std::string(who) + ": NULL field");
+    *a = IoArrays{f->q, f->g, f->g0, f->pr, f->drhodtp, f->divu, f->rho_stg, f->mu_stg, f->explt0,
+                  tf->density, tf->density0, tf->viscosity, tf->viscosity0};
+    return CHS_OK;
+}
+
+static int io_check(const chs_handle* h, const chs_io_params* prm, int ns, const char* who) {
+    CHS_CHECK(h && prm, CHS_E_ARG, std::string(who) + ": NULL argument");
+    CHS_CHECK(ns >= 1 && ns <= 3, CHS_E_ARG, std::string(who) + ": NS must be 1..3");
+    CHS_CHECK(h->M >= 2, CHS_E_ARG, std::string(who) + ": NCL2 >= 2 needed");
+    CHS_CHECK(h->icase == 1, CHS_E_ARG, std::string(who) + ": the channel (ICASE = 1) only");
+    CHS_CHECK(prm->thermal == 0 || prm->thermal == 1, CHS_E_ARG, std::string(who) + ": thermal must be 0 or 1");
+    CHS_CHECK(prm->nflow >= 1 && prm->nflow <= 3, CHS_E_ARG, std::string(who) + ": NFLOW must be 1..3");
+    CHS_CHECK(prm->flowdrvtp == 1 || prm->flowdrvtp == 2, CHS_E_ARG, std::string(who) + ": FLOWDRVTP must be 1 or 2");
+    CHS_CHECK(prm->gravdir >= 0 && prm->gravdir <= 3, CHS_E_ARG, std::string(who) + ": GRAVDIR must be 0..3");
+    return CHS_OK;
+}
+
+static dim3 io_grid(const chs_handle* h, int rows) { return dim3((h->L + 63) / 64, (h->N + 3) / 4, rows); }
+
+int io_prepare(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields* f, const chs_io_params* prm,
+               int ns) {
+    CHS_TRY(io_check(h, prm, ns, "chs_io_prepare"));
+    IoArrays a;
+    CHS_TRY(io_arrays(tf, f, &a, "chs_io_prepare"));
+    const hipStream_t s = h->stream;
+    hipEvent_t ev;
+    h->timer.begin("io_prepare", s, &ev);
+    CHS_LAUNCH(k_io_prepare, io_grid(h, h->M + 2), dim3(64, 4), 0, s, io_geo(h), a, prm->thermal,
+               TGAM_IO[ns] + TROH_IO[ns], TROH_IO[ns], TGAM_IO[ns] - 2.0);
+    h->timer.end(ev, "io_prepare", s);
+    return CHS_OK;
+}
+
+int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields* f, const chs_io_params* prm,
+               int ns, double dt) {
+    CHS_TRY(io_check(h, prm, ns, "chs_solverrk3_mom_io"));
+    IoArrays a;
+    CHS_TRY(io_arrays(tf, f, &a, "chs_solverrk3_mom_io"));
+    const hipStream_t s = h->stream;
+    const IoGeo g = io_geo(h);
+    const dim3 gi = io_grid(h, h->M);
+    const long nb = (long)gi.x * gi.y * gi.z;
+    CHS_CHECK(nb <= 2L * h->npart, CHS_E_ARG, "chs_solverrk3_mom_io: grid too large for the block-sum buffer");
+    const size_t ngh = (size_t)h->L * (h->M + 2) * h->N;
+    const double talp = TGAM_IO[ns] + TROH_IO[ns];
+    hipEvent_t ev;
+    // G0 = G (src/SOLVERRK3_MOM_IO.f90:15)
+    CHS_HIP(hipMemcpyAsync(f->g0, f->g, 3 * ngh * sizeof(double), hipMemcpyDeviceToDevice, s));
+    h->timer.begin("io_divu", s, &ev);
+    CHS_LAUNCH(k_io_divu, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, f->q, f->divu);
+    h->timer.end(ev, "io_divu", s);
+    IoParams p{prm->thermal, prm->nflow, prm->flowdrvtp, prm->gravdir, prm->cfgv, prm->f_a};
+    h->timer.begin("io_terms", s, &ev);
+    CHS_LAUNCH(k_io_terms, gi, dim3(64, 4), 0, s, g, a, p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs,
+               h->partials);
+    h->timer.end(ev, "io_terms", s);
+    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(256), 0, s, h->partials, (int)nb, h->volm, prm->flowdrvtp,
+               -0.5 * prm->cfgv * (talp * dt), h->scalars);
+    h->timer.begin("io_massflux", s, &ev);
+    CHS_LAUNCH(k_io_massflux, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, f->g, h->rhs, h->scalars, prm->nflow);
+    h->timer.end(ev, "io_massflux", s);
+    h->timer.begin("io_divg", s, &ev);
+    CHS_LAUNCH(k_io_divg, gi, dim3(64, 4), 0, s, g, f->g, f->drhodtp, 1.0 / dt / talp, h->rhsllphi);
+    h->timer.end(ev, "io_divg", s);
+    CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (talp * dt)));
+    h->timer.begin("io_update", s, &ev);
+    CHS_LAUNCH(k_io_update, gi, dim3(64, 4), 0, s, g, f->g, f->pr, h->dph, dt * talp);
+    h->timer.end(ev, "io_update", s);
+    return CHS_OK;
+}
+
+int io_divgck(chs_handle* h, const chs_io_fields* f, double* div_g, double* div_q) {
+    CHS_CHECK(h && f && f->g && f->q && f->drhodtp, CHS_E_ARG, "chs_divgck_io: NULL argument");
+    const hipStream_t s = h->stream;
+    const int nb = 1024;
+    CHS_LAUNCH(k_io_divgck, dim3(nb), dim3(256), 0, s, io_geo(h), f->g, f->q, f->drhodtp, h->partials);
+    CHS_LAUNCH(k_io_max2, dim3(1), dim3(256), 0, s, h->partials, nb, h->scalars + 1);
+    double host[2];
+    CHS_HIP(hipMemcpyAsync(host, h->scalars + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHS_HIP(hipStreamSynchronize(s));
+    if (div_g) *div_g = host[0];
+    if (div_q) *div_q = host[1];
+    return CHS_OK;
+}
+
+}  // namespace chs
+
+extern "C" {
+
+int chs_io_prepare(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields* f, const chs_io_params* p,
+                   int ns) {
+    return chs::io_prepare(h, tf, f, p, ns);
+}
+
+int chs_solverrk3_mom_io(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields* f, const chs_io_params* p,
+                         int ns, double dt) {
+    return chs::io_substep(h, tf, f, p, ns, dt);
+}
+
+int chs_divgck_io(chs_handle* h, const chs_io_fields* f, double* maxdiv_g, double* maxdiv_q) {
+    return chs::io_divgck(h, f, maxdiv_g, maxdiv_q);
+}
+
+}  // extern "C"

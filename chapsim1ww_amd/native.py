@@ -42,6 +42,15 @@ class ThermalFields(C.Structure):
                                           "wallflux")]
 
 
+class IoFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("q", "g", "g0", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0")]
+
+
+class IoParams(C.Structure):
+    _fields_ = [("thermal", C.c_int), ("nflow", C.c_int), ("flowdrvtp", C.c_int), ("gravdir", C.c_int),
+                ("cfgv", C.c_double), ("f_a", C.c_double)]
+
+
 # every export of include/chapsim_gpu.h with its ctypes signature
 _SIGS = {
     "chs_create": ([C.POINTER(GridDesc), C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
@@ -70,6 +79,11 @@ _SIGS = {
     "chs_props_rhoh_max": ([C.c_void_p, _dp], C.c_int),
     "chs_energy_substep": ([C.c_void_p, C.c_void_p, C.POINTER(ThermalFields), C.c_int, C.c_double, C.c_double],
                            C.c_int),
+    "chs_io_prepare": ([C.c_void_p, C.POINTER(ThermalFields), C.POINTER(IoFields), C.POINTER(IoParams), C.c_int],
+                       C.c_int),
+    "chs_solverrk3_mom_io": ([C.c_void_p, C.POINTER(ThermalFields), C.POINTER(IoFields), C.POINTER(IoParams),
+                              C.c_int, C.c_double], C.c_int),
+    "chs_divgck_io": ([C.c_void_p, C.POINTER(IoFields), _dp, _dp], C.c_int),
     "chs_surrogate_train": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_double, C.c_ulonglong,
                              _dp], C.c_int),
     "chs_surrogate_certify": ([C.c_void_p, C.c_double, C.c_int, _dp], C.c_int),

@@ -198,6 +198,52 @@ typedef struct chs_thermal_fields {
 int chs_energy_substep(chs_handle *h, chs_props *p, const chs_thermal_fields *f, int ns, double dt,
                        double cthecd);
 
+/* ---- io momentum substep of the heated channel ----------------------------
+ * The main-domain (io) momentum equation with variable properties in mass-flux
+ * form G = rho u, periodic x and z, channel walls, explicit viscous terms
+ * (visthemflg = 0) and no weighted pressure.  Arrays in the reference's
+ * layouts (x fastest):
+ *   q, g, g0        Q_io, G_io, G0_io (NCL1, 0:NCL2+1, NCL3, 3)
+ *   pr              PR_io (NCL1, 0:NCL2+1, NCL3)
+ *   drhodtp, divu   DrhoDtP, DivU_io (NCL1, 0:NCL2+1, NCL3)
+ *   rho_stg, mu_stg RHO_STG, MU_STG (NCL1, 0:NCL2+1, NCL3, 3)
+ *   explt0          EXPLT0_io (NCL1, NCL2, NCL3, 3)
+ * g and g0 are the arrays the energy substep reads (chs_thermal_fields.g/g0).
+ * The properties come from chs_thermal_fields: density, density0,
+ * viscosity, viscosity0 (used as given, ghost rows included). */
+typedef struct chs_io_fields {
+    double *q, *g, *g0, *pr, *drhodtp, *divu, *rho_stg, *mu_stg, *explt0;
+} chs_io_fields;
+
+/* thermal: thermlflg (1 = variable properties: DrhoDtP, RHO_STG and MU_STG
+ *          are computed; 0 = they are left as the caller set them, 0 and 1);
+ * nflow: NFLOW 1..3; flowdrvtp: 1 constant mass flux, 2 constant pressure
+ * gradient with cfgv = CFGV; gravdir: GRAVDIR 0..3 with f_a = F_A
+ * (src/Flow_State.f90:97-110). */
+typedef struct chs_io_params {
+    int thermal, nflow, flowdrvtp, gravdir;
+    double cfgv, f_a;
+} chs_io_params;
+
+/* What SOLVE runs between SOLVERRK3_ENG_IO(NS) and SOLVERRK3_MOM_IO(NS)
+ * (src/SOLVE.f90:121-133): DENSITY_TIME_DERIVATION(NS)
+ * (src/THERM_PROP_UPDATE.f90:225-291), DENSITY_Staggered, MU_Staggered and
+ * VELOCITY_CALC_io (src/MASSFLUX_CALC_IO.f90:82-311).  ns = 1..3. */
+int chs_io_prepare(chs_handle *h, const chs_thermal_fields *tf, const chs_io_fields *f, const chs_io_params *p,
+                   int ns);
+/* SOLVERRK3_MOM_IO(NS) (src/SOLVERRK3_MOM_IO.f90:1-168), explicit viscous
+ * branch: CONVECTION_*_io, DIVG_U_io, VISCOUS_ALL_EXPLT_*_io,
+ * RHS_MOM_EXPLICIT_io (src/RHS_MOM_EXPLICIT_io.f90:1-297), MASSFLUX_CALC_IO,
+ * DIVG_io (src/DIVG.f90:60-150), the FFT99 Poisson solve (the reference's
+ * alternative for the io domain, src/SOLVERRK3_MOM_IO.f90:123), PRCALC_io and
+ * MASSFLUX_UPDATE_IO (src/VELO_CORRECTION.f90:109-185).  Updates g, g0, pr,
+ * divu, explt0.  Asynchronous on h's stream. */
+int chs_solverrk3_mom_io(chs_handle *h, const chs_thermal_fields *tf, const chs_io_fields *f,
+                         const chs_io_params *p, int ns, double dt);
+/* DIVGCK_io (src/DIVGCK.f90:61-124): max |div G + DrhoDtP| and max |div Q|
+ * over the interior (NaN propagates).  Either output may be NULL. */
+int chs_divgck_io(chs_handle *h, const chs_io_fields *f, double *maxdiv_g, double *maxdiv_q);
+
 /* ---- learned property surrogate ------------------------------------------
  * The reference has no surrogate: these entries train and certify the network
  * that chs_therm_prop_update(..., CHS_PROPS_SURROGATE, ...) runs in place of

@@ -1,0 +1,123 @@
+"""The heated channel's io momentum substep on the device against the oracle:
+the property staging bit for bit, and full substeps (energy, staging, io
+momentum in SOLVE's order) within 1e-12 of the oracle, field by field."""
+import numpy as np
+import pytest
+
+import oracle as O
+from chapsim1ww_amd import props
+from chapsim1ww_amd.grid import ChannelGrid
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+CTHECD = 1.0 / 2800.0
+MAP = {"rhoh": "rhoh", "h": "enthalpy", "T": "temperature", "rho": "density", "rho0": "density0",
+       "rho1": "density1", "mu": "viscosity", "mu0": "viscosity0", "kap": "thermcondt", "rhs0": "rhs_energy0",
+       "g": "g", "g0": "g0", "wallflux": "wallflux", "q": "q", "pr": "pr", "drhodtp": "drhodtp", "divu": "divu",
+       "rho_stg": "rho_stg", "mu_stg": "mu_stg", "explt0": "explt0"}
+FP64_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def scaled():
+    return props.scale_table(props.synthetic_table(), t0=1.0)
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _pair(dims, scaled, seed, **kw):
+    from chapsim1ww_amd.heated import HeatedChannel
+    from chapsim1ww_amd.solver import ChannelSolver
+    g = ChannelGrid(*dims)
+    og = O.OracleGrid(*dims)
+    st, tab, n = O.io_case(og, scaled, seed=seed)
+    s = ChannelSolver(g)
+    pt = props.PropertyTable(scaled)
+    hc = HeatedChannel(s, pt, **kw)
+    hc.upload({MAP[k]: v for k, v in st.items()})
+    io = O.OracleIO(og, st, **kw)
+    return og, st, tab, n, s, pt, hc, io
+
+
+@pytest.mark.parametrize("dims", [(16, 12, 8), (64, 48, 32)])
+def test_io_prepare_bitwise(scaled, dims):
+    print(f"io prepare {dims}", flush=True)
+    og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=sum(dims))
+    for ns in (1, 2):
+        io.prepare(ns)
+        hc.io_prepare(ns)
+        got = hc.download()
+        for k in ("drhodtp", "rho_stg", "mu_stg", "q"):
+            assert np.array_equal(got[k], st[k]), (ns, k, np.abs(got[k] - st[k]).max())
+    pt.close()
+    s.close()
+
+
+@pytest.mark.parametrize("dims,kw", [((16, 12, 8), dict(thermal=1, gravdir=1, f_a=-0.05)),
+                                     ((64, 48, 32), dict(thermal=1, gravdir=2, f_a=0.02)),
+                                     ((32, 24, 16), dict(thermal=1, flowdrvtp=2, cfgv=0.01, gravdir=3, f_a=0.03))])
+def test_heated_substeps(scaled, dims, kw):
+    """Two RK3 steps (NS = 1, 2, 3 twice) of energy + staging + io momentum;
+    every field within 1e-12 of the oracle after every substep and the
+    projected divergence below 1e-10 on both sides."""
+    print(f"heated substeps {dims} {kw}", flush=True)
+    og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=3 + sum(dims), **kw)
+    rmax = O.rhoh_max(scaled)
+    dt = 1e-3
+    # The random start is far from divergence-free, so the first projections
+    # put pressures of O(100-1000) into PR that later substeps cancel: PR is
+    # compared relative to the largest |PR| of the trajectory so far (the
+    # operands of that cancellation), the other fields to their own maximum.
+    pr_scale = np.abs(st["pr"]).max()
+    for step, ns in enumerate((1, 2, 3, 1, 2, 3)):
+        assert O.energy_substep(og, st, tab, n, ns, dt, CTHECD, rmax) == 0
+        io.prepare(ns)
+        io.substep(ns, dt)
+        hc.substep(ns, dt, CTHECD)
+        got = hc.download()
+        errs = {k: _rel(got[MAP[k]], st[k]) for k in ("g", "g0", "explt0", "divu", "q", "rho_stg", "mu_stg",
+                                                      "drhodtp", "rho", "T", "h")}
+        pr_scale = max(pr_scale, np.abs(st["pr"]).max())
+        errs["pr"] = float(np.abs(got["pr"] - st["pr"]).max() / pr_scale)
+        dg, _ = hc.divgck_io()
+        do, _ = io.divgck()
+        print(step, ns, {k: f"{v:.1e}" for k, v in errs.items()}, f"div {dg:.1e} {do:.1e}", flush=True)
+        for k, e in errs.items():
+            assert e < FP64_TOL, (step, ns, k, e)
+        assert dg < 1e-10 and do < 1e-10
+    pt.close()
+    s.close()
+
+
+def test_isothermal_io_and_divgck_nan(scaled):
+    """thermal = 0 (RHO_STG = MU_STG = 1, DrhoDtP = 0 as allocated): the
+    momentum substep alone against the oracle, then NaN propagation of
+    DIVGCK_io."""
+    print("isothermal io", flush=True)
+    dims = (32, 24, 16)
+    og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=5, thermal=0)
+    L, M, N = dims
+    for k in ("rho", "rho0", "mu", "mu0"):
+        st[k][...] = 1.0
+    hc.upload({MAP[k]: st[k] for k in ("rho", "rho0", "mu", "mu0")})
+    pr_scale = np.abs(st["pr"]).max()
+    for ns in (1, 2, 3):
+        io.prepare(ns)
+        io.substep(ns, 2e-3)
+        hc.io_prepare(ns)
+        hc.solverrk3_mom_io(ns, 2e-3)
+        pr_scale = max(pr_scale, np.abs(st["pr"]).max())
+    got = hc.download()
+    for k in ("g", "explt0", "q"):
+        assert _rel(got[k], st[k]) < FP64_TOL, k
+    assert np.abs(got["pr"] - st["pr"]).max() < FP64_TOL * pr_scale
+    g = got["g"].copy(order="F")
+    g[3, 5, 7, 0] = np.nan
+    hc.upload({"g": g})
+    dg, _ = hc.divgck_io()
+    assert np.isnan(dg)
+    pt.close()
+    s.close()
