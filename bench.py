@@ -176,7 +176,7 @@ def props_bench(g, q_fp64, stream, steps):
     the energy substep, on the synthetic heated-channel table."""
     import torch
     from chapsim1ww_amd import props
-    from chapsim1ww_amd.energy import ThermalState
+    from chapsim1ww_amd.heated import HeatedChannel
     from chapsim1ww_amd.solver import ChannelSolver
     dev = torch.device("cuda:0")
     sc = props.scale_table(props.synthetic_table())
@@ -230,7 +230,7 @@ def props_bench(g, q_fp64, stream, steps):
                         "mfma_peak_bf16_tflops": 2500.0}
     # energy substep (iso-flux walls, q_w = 0.1, Pr = 1, Re = 2800) on this grid
     s = ChannelSolver(g, stream=stream)
-    ts = ThermalState(s, pt)
+    ts = HeatedChannel(s, pt, thermal=1, nflow=1, flowdrvtp=1, gravdir=0)
     L, M, N = g.L, g.M, g.N
     yc = np.concatenate([[-1.0], np.asarray(g.ycc), [1.0]])
     xx = np.arange(L) / L * 2 * np.pi
@@ -257,6 +257,42 @@ def props_bench(g, q_fp64, stream, steps):
     ms_e = timed(eng, reps)
     out["energy_substep"] = dict(ms=round(ms_e, 4), substeps_per_s=round(1e3 / ms_e, 2),
                                  note="SOLVERRK3_ENG_IO: RHS + RK3 + property cascade + iso-flux walls, synchronous")
+    # the whole heated-channel substep in SOLVE's order: energy, property
+    # staging, io momentum (explicit viscous terms, FFT99 projection)
+
+    def heated():
+        ts.substep(seq[cnt[0] % 3], 1e-3, 1.0 / 2800.0)
+        cnt[0] += 1
+
+    def mom():
+        ns = seq[cnt[0] % 3]
+        ts.io_prepare(ns)
+        ts.solverrk3_mom_io(ns, 1e-3)
+        cnt[0] += 1
+    cnt[0] = 0
+    ms_h = timed(heated, reps)
+    ms_m = timed(mom, reps)
+    s.profile(True)
+    s.profile_reset()
+    for _ in range(reps):
+        mom()
+    _sync()
+    prof = s.profile_read()
+    s.profile(False)
+    dg, _ = ts.divgck_io()
+    ab = {"io_prepare": 160.0, "io_divu": 32.0, "io_terms": 176.0, "io_massflux": 72.0, "io_divg": 40.0,
+          "io_update": 72.0}
+    kern = {}
+    for name, (ms_tot, calls) in prof.items():
+        if calls:
+            kern[name] = dict(avg_ms=round(ms_tot / calls, 5), calls=calls)
+            if name in ab:
+                kern[name]["alg_gbs"] = round(ab[name] * n / (ms_tot / calls * 1e-3) / 1e9, 1)
+    out["heated_substep"] = dict(ms=round(ms_h, 4), substeps_per_s=round(1e3 / ms_h, 2),
+                                 momentum_ms=round(ms_m, 4), max_div_g=dg, kernels=kern,
+                                 note="SOLVE order: SOLVERRK3_ENG_IO, DENSITY_TIME_DERIVATION + DENSITY/MU_Staggered"
+                                      " + VELOCITY_CALC_io, SOLVERRK3_MOM_IO (explicit viscous, FFT99 projection)")
+    _note("heated substep done")
     s.close()
     pt.close()
     return out
