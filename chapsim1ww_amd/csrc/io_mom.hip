@@ -63,93 +63,129 @@ struct IoParams {
 static const double TGAM_IO[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
 static const double TROH_IO[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
 
-#define IO_CELL                                                                  \
+// Tiles of 64 (i) x 4 (k) cells of one row j, run one per block of a 1-D
+// grid of 8 * ceil(ntiles / 8) blocks.  Block b runs on XCD b % 8; tile
+// t = (b % 8) * per + b / 8 gives each XCD a contiguous range of tiles,
+// ordered i-tile fastest, then j, then k-tile, so the tiles an XCD runs
+// back to back are j-neighbours and their j +- 1 rows are in its L2.
+// Returns false for the padding blocks.
+__device__ __forceinline__ bool io_tile(int L, int N, int nj, int* it, int* j, int* kt, int* tile) {
+    const int ni = (L + 63) / 64, nk = (N + 3) / 4;
+    const int nt = ni * nj * nk, per = (nt + 7) / 8;
+    const int b = blockIdx.x;
+    const int t = (b % 8) * per + b / 8;
+    if (t >= nt) return false;
+    *it = t % ni;
+    const int r = t / ni;
+    *j = r % nj;
+    *kt = r / nj;
+    *tile = t;
+    return true;
+}
+
+#define IO_CELL(NJ, J0)                                                          \
     const int L = g.L, M = g.M, N = g.N;                                         \
-    const int IC = blockIdx.x * 64 + threadIdx.x;                                \
-    const int KC = blockIdx.y * 4 + threadIdx.y;                                 \
+    int it_, jt_, kt_, tile_;                                                    \
+    if (!io_tile(L, N, (NJ), &it_, &jt_, &kt_, &tile_)) return;                  \
+    const int IC = it_ * 64 + threadIdx.x;                                       \
+    const int KC = kt_ * 4 + threadIdx.y;                                        \
+    const int JT = jt_ + (J0);                                                   \
     if (IC >= L || KC >= N) return;                                              \
     const int IP = IC == L - 1 ? 0 : IC + 1, IM = IC == 0 ? L - 1 : IC - 1;      \
     const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;      \
     const int LP = L * (M + 2), CS = LP * N;                                     \
     auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };           \
-    (void)IP; (void)IM; (void)KP; (void)KM; (void)CS
+    (void)IP; (void)IM; (void)KP; (void)KM; (void)CS; (void)JT
 
 // DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered (thermal = 1) and
 // VELOCITY_CALC_io with BC_WALL_Q_io; one thread per cell of rows 0..M+1.
-__global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, IoArrays a, int thermal, double ca, double cb,
+// Every array is distinct (restrict): all loads can be issued before the
+// first store.
+__global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __restrict__ D,
+                                                    const double* __restrict__ D0, const double* __restrict__ V,
+                                                    const double* __restrict__ V0, const double* __restrict__ G,
+                                                    const double* __restrict__ G0, double* __restrict__ drhodtp,
+                                                    double* __restrict__ rs, double* __restrict__ ms,
+                                                    double* __restrict__ q, int thermal, double ca, double cb,
                                                     double cd) {
 #pragma clang fp contract(off)
-    IO_CELL;
-    const int J = blockIdx.z;   // ghost-aware row 0..M+1
+    IO_CELL(g.M + 2, 0);
+    const int J = JT;   // ghost-aware row 0..M+1
     const int c = at(IC, J, KC);
-    const double *D = a.rho, *D0 = a.rho0, *V = a.mu, *V0 = a.mu0;
+    double r1, r2, r3;
     if (thermal) {
-        if (J >= 1 && J <= M) {
-            const double* G = a.g;
-            const double* G0 = a.g0;
-            const double COE2 = g.dyfi[J - 1] * 1.0, COE3 = g.dzi * 1.0;
-            const double DIVX = (G[at(IP, J, KC)] - G[c]) * g.dxi;
-            const double DIVX0 = (G0[at(IP, J, KC)] - G0[c]) * g.dxi;
-            const double DIVY = (G[CS + at(IC, J + 1, KC)] - G[CS + c]) * COE2;
-            const double DIVY0 = (G0[CS + at(IC, J + 1, KC)] - G0[CS + c]) * COE2;
-            const double DIVZ = (G[2 * CS + at(IC, J, KP)] - G[2 * CS + c]) * COE3;
-            const double DIVZ0 = (G0[2 * CS + at(IC, J, KP)] - G0[2 * CS + c]) * COE3;
-            a.drhodtp[c] = (ca * (DIVX + DIVY + DIVZ) + cb * (DIVX0 + DIVY0 + DIVZ0)) / cd;
+        // Every branch below only selects values: the ghost rows evaluate
+        // the interior expressions at clamped rows and discard them, so the
+        // loads form one straight line.
+        const bool interior = J >= 1 && J <= M;
+        double dr;
+        {
+            const int Jd = J < 1 ? 1 : (J > M ? M : J), cd0 = at(IC, Jd, KC);
+            const double COE2 = g.dyfi[Jd - 1] * 1.0, COE3 = g.dzi * 1.0;
+            const double DIVX = (G[at(IP, Jd, KC)] - G[cd0]) * g.dxi;
+            const double DIVX0 = (G0[at(IP, Jd, KC)] - G0[cd0]) * g.dxi;
+            const double DIVY = (G[CS + at(IC, Jd + 1, KC)] - G[CS + cd0]) * COE2;
+            const double DIVY0 = (G0[CS + at(IC, Jd + 1, KC)] - G0[CS + cd0]) * COE2;
+            const double DIVZ = (G[2 * CS + at(IC, Jd, KP)] - G[2 * CS + cd0]) * COE3;
+            const double DIVZ0 = (G0[2 * CS + at(IC, Jd, KP)] - G0[2 * CS + cd0]) * COE3;
+            dr = (ca * (DIVX + DIVY + DIVZ) + cb * (DIVX0 + DIVY0 + DIVZ0)) / cd;
         }
         // rows whose values the y wrap copies: 0 <- M, M+1 <- 1
         const int Jw = J == 0 ? M : (J == M + 1 ? 1 : J);
         // RHO_STG x and z faces: rows 1..M, wrapped ghosts
-        a.rs[c] = (D[at(IC, Jw, KC)] + D[at(IM, Jw, KC)] + D0[at(IC, Jw, KC)] + D0[at(IM, Jw, KC)]) * 0.5 * 0.5;
-        a.rs[2 * CS + c] =
-            (D[at(IC, Jw, KC)] + D[at(IC, Jw, KM)] + D0[at(IC, Jw, KC)] + D0[at(IC, Jw, KM)]) * 0.5 * 0.5;
-        // RHO_STG y face: rows 2..M; row 0 <- M; rows 1 and M+1 are the wall values
-        if (J == 1 || J == M + 1) {
-            a.rs[CS + c] = 0.5 * (D0[at(IC, J == 1 ? 0 : M + 1, KC)] + D[at(IC, J == 1 ? 0 : M + 1, KC)]);
-        } else {
-            const int Jr = J == 0 ? M : J, Jm = Jr - 1;
-            a.rs[CS + c] = (g.wff[Jr - 1] * (D[at(IC, Jr, KC)] + D0[at(IC, Jr, KC)]) +
-                            g.wfb[Jr - 1] * (D[at(IC, Jm, KC)] + D0[at(IC, Jm, KC)])) *
-                           0.5;
-        }
+        r1 = (D[at(IC, Jw, KC)] + D[at(IM, Jw, KC)] + D0[at(IC, Jw, KC)] + D0[at(IM, Jw, KC)]) * 0.5 * 0.5;
+        r3 = (D[at(IC, Jw, KC)] + D[at(IC, Jw, KM)] + D0[at(IC, Jw, KC)] + D0[at(IC, Jw, KM)]) * 0.5 * 0.5;
         // MU_STG x-z edge: rows 1..M, wrapped ghosts
-        a.ms[CS + c] = (V[at(IC, Jw, KC)] + V[at(IM, Jw, KC)] + V0[at(IC, Jw, KC)] + V0[at(IM, Jw, KC)] +
-                        V[at(IC, Jw, KM)] + V[at(IM, Jw, KM)] + V0[at(IC, Jw, KM)] + V0[at(IM, Jw, KM)]) *
-                       0.5 * 0.5 * 0.5;
-        // MU_STG x-y and y-z edges: rows 2..M; row 0 <- M; rows 1, M+1 wall values
-        if (J == 1 || J == M + 1) {
-            const int Jg = J == 1 ? 0 : M + 1;
-            const double w = 0.5 * (V0[at(IC, Jg, KC)] + V[at(IC, Jg, KC)]);
-            a.ms[c] = w;
-            a.ms[2 * CS + c] = w;
-        } else {
-            const int Jr = J == 0 ? M : J, Jm = Jr - 1;
-            const double wff = g.wff[Jr - 1], wfb = g.wfb[Jr - 1];
-            a.ms[c] = ((V[at(IC, Jr, KC)] + V[at(IM, Jr, KC)] + V0[at(IC, Jr, KC)] + V0[at(IM, Jr, KC)]) * wff +
-                       (V[at(IC, Jm, KC)] + V[at(IM, Jm, KC)] + V0[at(IC, Jm, KC)] + V0[at(IM, Jm, KC)]) * wfb) *
-                      0.5 * 0.5;
-            a.ms[2 * CS + c] =
-                ((V[at(IC, Jr, KM)] + V[at(IC, Jr, KC)] + V0[at(IC, Jr, KM)] + V0[at(IC, Jr, KC)]) * wff +
-                 (V[at(IC, Jm, KM)] + V[at(IC, Jm, KC)] + V0[at(IC, Jm, KM)] + V0[at(IC, Jm, KC)]) * wfb) *
-                0.5 * 0.5;
-        }
+        const double m2 = (V[at(IC, Jw, KC)] + V[at(IM, Jw, KC)] + V0[at(IC, Jw, KC)] + V0[at(IM, Jw, KC)] +
+                           V[at(IC, Jw, KM)] + V[at(IM, Jw, KM)] + V0[at(IC, Jw, KM)] + V0[at(IM, Jw, KM)]) *
+                          0.5 * 0.5 * 0.5;
+        // RHO_STG y face and MU_STG x-y, y-z edges: rows 2..M; row 0 <- M;
+        // rows 1 and M+1 are the wall values
+        const bool wall = J == 1 || J == M + 1;
+        const int Jg = J == M + 1 ? M + 1 : 0;
+        const double r2w = 0.5 * (D0[at(IC, Jg, KC)] + D[at(IC, Jg, KC)]);
+        const double m1w = 0.5 * (V0[at(IC, Jg, KC)] + V[at(IC, Jg, KC)]);
+        const int Jr = J == 0 ? M : J, Jm = Jr - 1;
+        const double wff = g.wff[Jr - 1], wfb = g.wfb[Jr - 1];
+        const double r2i =
+            (wff * (D[at(IC, Jr, KC)] + D0[at(IC, Jr, KC)]) + wfb * (D[at(IC, Jm, KC)] + D0[at(IC, Jm, KC)])) * 0.5;
+        const double m1i = ((V[at(IC, Jr, KC)] + V[at(IM, Jr, KC)] + V0[at(IC, Jr, KC)] + V0[at(IM, Jr, KC)]) * wff +
+                            (V[at(IC, Jm, KC)] + V[at(IM, Jm, KC)] + V0[at(IC, Jm, KC)] + V0[at(IM, Jm, KC)]) * wfb) *
+                           0.5 * 0.5;
+        const double m3i = ((V[at(IC, Jr, KM)] + V[at(IC, Jr, KC)] + V0[at(IC, Jr, KM)] + V0[at(IC, Jr, KC)]) * wff +
+                            (V[at(IC, Jm, KM)] + V[at(IC, Jm, KC)] + V0[at(IC, Jm, KM)] + V0[at(IC, Jm, KC)]) * wfb) *
+                           0.5 * 0.5;
+        r2 = wall ? r2w : r2i;
+        const double m1 = wall ? m1w : m1i, m3 = wall ? m1w : m3i;
+        if (interior) drhodtp[c] = dr;
+        rs[c] = r1;
+        rs[CS + c] = r2;
+        rs[2 * CS + c] = r3;
+        ms[c] = m1;
+        ms[CS + c] = m2;
+        ms[2 * CS + c] = m3;
+    } else {
+        r1 = rs[c];
+        r2 = rs[CS + c];
+        r3 = rs[2 * CS + c];
     }
     // VELOCITY_CALC_io + BC_WALL_Q_io (the wrapped ghost rows are then zeroed)
     if (J == 0 || J == M + 1) {
-        a.q[c] = 0.0;
-        a.q[CS + c] = 0.0;
-        a.q[2 * CS + c] = 0.0;
+        q[c] = 0.0;
+        q[CS + c] = 0.0;
+        q[2 * CS + c] = 0.0;
     } else {
-        a.q[c] = a.g[c] / a.rs[c];
-        a.q[2 * CS + c] = a.g[2 * CS + c] / a.rs[2 * CS + c];
-        a.q[CS + c] = J >= 2 ? a.g[CS + c] / a.rs[CS + c] : 0.0;
+        q[c] = G[c] / r1;
+        q[2 * CS + c] = G[2 * CS + c] / r3;
+        q[CS + c] = J >= 2 ? G[CS + c] / r2 : 0.0;
     }
 }
 
 // DIVG_U_io on rows 1..M and the y wrap of its ghost rows
 __global__ void __launch_bounds__(256) k_io_divu(IoGeo g, const double* __restrict__ Q, double* __restrict__ divu) {
 #pragma clang fp contract(off)
-    IO_CELL;
-    const int J = blockIdx.z;
+    IO_CELL(g.M + 2, 0);
+    const int J = JT;
     const int Jr = J == 0 ? M : (J == M + 1 ? 1 : J);
     const int c = at(IC, Jr, KC);
     const double DIVX = (Q[at(IP, Jr, KC)] - Q[c]) * g.dxi;
@@ -174,14 +210,21 @@ __device__ __forceinline__ double io_block_sum(double v, double* red) {
 // Explicit terms and RHS_MOM_EXPLICIT_io up to DPGRNS, rows 1..M.
 // inc[c] <- RHS of each component (row 1 of y untouched), explt0 <- the
 // explicit terms, part[block] <- sum of RHS/DYFI of the NFLOW component.
-__global__ void __launch_bounds__(256) k_io_terms(IoGeo g, IoArrays a, IoParams p, double tg, double tr, double dt,
+template <int GRAV>
+__global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restrict__ G, const double* __restrict__ Q,
+                                                  const double* __restrict__ MU, const double* __restrict__ MU0,
+                                                  const double* __restrict__ DU, const double* __restrict__ MS,
+                                                  const double* __restrict__ PR, const double* __restrict__ RS,
+                                                  double* __restrict__ E, IoParams p, double tg, double tr, double dt,
                                                   double adt, double* __restrict__ inc, double* __restrict__ part) {
 #pragma clang fp contract(off)
     __shared__ double red[256];
     const int L = g.L, M = g.M, N = g.N;
-    const int IC = blockIdx.x * 64 + threadIdx.x;
-    const int KC = blockIdx.y * 4 + threadIdx.y;
-    const int JC = blockIdx.z + 1;
+    int it, jt, kt, tile;
+    if (!io_tile(L, N, M, &it, &jt, &kt, &tile)) return;   // padding block: whole block exits
+    const int IC = it * 64 + threadIdx.x;
+    const int KC = kt * 4 + threadIdx.y;
+    const int JC = jt + 1;
     const bool live = IC < L && KC < N;
     double flow = 0.0;
     if (live) {
@@ -189,21 +232,18 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, IoArrays a, IoParams 
         const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;
         const int LP = L * (M + 2), CS = LP * N, NC = L * M * N;
         auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
-        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC - 1;
+        // row 1 of the y component is not part of the stencil (v sits on the
+        // wall face); it is evaluated with a clamped metric index and
+        // discarded, so that the whole kernel body is one straight line.
+        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC >= 2 ? JC - 1 : 1;
         const double wff = g.wff[JJ - 1], wfb = g.wfb[JJ - 1], wffp = g.wff[JJP - 1], wfbp = g.wfb[JJP - 1];
         const double CV = g.cvisc;
-        const double* G = a.g;
-        const double* Q = a.q;
-        const double* MU = a.mu;
-        const double* MU0 = a.mu0;
-        const double* DU = a.divu;
-        const double* MS = a.ms;
-        const double* PR = a.pr;
         const double *G1 = G, *G2 = G + CS, *G3 = G + 2 * CS;
         const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
         const double *MS1 = MS, *MS2 = MS + CS, *MS3 = MS + 2 * CS;
         const int c = at(IC, JC, KC);
         const int e = IC + L * (JC - 1) + L * M * KC;   // (L, M, N) index
+        const double E1 = E[e], E2 = E[NC + e], E3 = E[2 * NC + e];
         // ---- x: CONVECTION_X_io + VISCOUS_ALL_EXPLT_X_io ----
         double hx;
         {
@@ -245,8 +285,8 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, IoArrays a, IoParams 
             hx = hx + DTAU11DX + DTAU12DY + DTAU13DZ;
         }
         // ---- y (rows 2..M): CONVECTION_Y_io + VISCOUS_ALL_EXPLT_Y_io ----
-        double hy = 0.0;
-        if (JC >= 2) {
+        double hy;
+        {
             {
                 const double COE1 = g.dxi * 0.5, COE2 = g.dyci[JJ - 1] * 0.5 * 0.5, COE31 = g.dzi * 0.5 * 1.0;
                 const double H21F = (wff * G1[at(IP, JC, KC)] + wfb * G1[at(IP, JM, KC)]) * (Q2[at(IP, JC, KC)] + Q2[c]);
@@ -326,41 +366,26 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, IoArrays a, IoParams 
             hz = hz + DTAU31DX + DTAU32DY + DTAU33DZ;
         }
         // ---- RHS_MOM_EXPLICIT_io: RK3 combine, pressure gradient, buoyancy ----
-        double* E = a.explt0;
-        {
-            const double RHSL = E[e];
-            double r = (tg * hx + tr * RHSL) * dt;
-            E[e] = hx;
-            const double PGM = (PR[c] - PR[at(IM, JC, KC)]) * (adt * g.dxi);
-            r = r - PGM;
-            if (p.gravdir == 1) r = r + a.rs[c] * (adt * p.f_a / 1.0);
-            inc[e] = r;
-            if (p.nflow == 1) flow = r / g.dyfi[JC - 1] / 1.0;
-        }
+        const double rx = (tg * hx + tr * E1) * dt - (PR[c] - PR[at(IM, JC, KC)]) * (adt * g.dxi);
+        const double ry = (tg * hy + tr * E2) * dt - (PR[c] - PR[at(IC, JM, KC)]) * (adt * g.dyci[JC - 1] / 1.0);
+        const double rz = (tg * hz + tr * E3) * dt - (PR[c] - PR[at(IC, JC, KM)]) * (adt * g.dzi);
+        double r1 = rx, r2 = ry, r3 = rz;
+        if (GRAV == 1) r1 = r1 + RS[c] * (adt * p.f_a / 1.0);
+        if (GRAV == 2) r2 = r2 + RS[CS + c] * (adt * p.f_a / 1.0);
+        if (GRAV == 3) r3 = r3 + RS[2 * CS + c] * (adt * p.f_a / 1.0);
+        E[e] = hx;
+        inc[e] = r1;
+        E[2 * NC + e] = hz;
+        inc[2 * NC + e] = r3;
         if (JC >= 2) {
-            const double RHSL = E[NC + e];
-            double r = (tg * hy + tr * RHSL) * dt;
             E[NC + e] = hy;
-            const double PGM = (PR[c] - PR[at(IC, JM, KC)]) * (adt * g.dyci[JC - 1] / 1.0);
-            r = r - PGM;
-            if (p.gravdir == 2) r = r + a.rs[CS + c] * (adt * p.f_a / 1.0);
-            inc[NC + e] = r;
-            if (p.nflow == 2) flow = r / g.dyfi[JC - 1] / 1.0;
+            inc[NC + e] = r2;
         }
-        {
-            const double RHSL = E[2 * NC + e];
-            double r = (tg * hz + tr * RHSL) * dt;
-            E[2 * NC + e] = hz;
-            const double PGM = (PR[c] - PR[at(IC, JC, KM)]) * (adt * g.dzi);
-            r = r - PGM;
-            if (p.gravdir == 3) r = r + a.rs[2 * CS + c] * (adt * p.f_a / 1.0);
-            inc[2 * NC + e] = r;
-            if (p.nflow == 3) flow = r / g.dyfi[JC - 1] / 1.0;
-        }
+        const double rf = p.nflow == 1 ? r1 : (p.nflow == 2 ? (JC >= 2 ? r2 : 0.0) : r3);
+        flow = rf / g.dyfi[JC - 1] / 1.0;
     }
     const double bs = io_block_sum<256>(flow, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-        part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = bs;
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = bs;
 }
 
 // DPGRNS of the constant-mass-flux drive: the block sums of RHS/DYFI in a
@@ -385,8 +410,8 @@ __global__ void __launch_bounds__(256) k_io_dpgrns(const double* __restrict__ pa
 __global__ void __launch_bounds__(256) k_io_massflux(IoGeo g, double* __restrict__ G, const double* __restrict__ inc,
                                                      const double* __restrict__ dpg, int nflow) {
 #pragma clang fp contract(off)
-    IO_CELL;
-    const int J = blockIdx.z;
+    IO_CELL(g.M + 2, 0);
+    const int J = JT;
     const int c = at(IC, J, KC);
     if (J == 0 || J == M + 1) {
         G[c] = 0.0;
@@ -410,8 +435,8 @@ __global__ void __launch_bounds__(256) k_io_divg(IoGeo g, const double* __restri
                                                  const double* __restrict__ drhodtp, double coe0,
                                                  double* __restrict__ f) {
 #pragma clang fp contract(off)
-    IO_CELL;
-    const int JC = blockIdx.z + 1;
+    IO_CELL(g.M, 1);
+    const int JC = JT;
     const int c = at(IC, JC, KC);
     const double DIVX = (G[at(IP, JC, KC)] - G[c]) * (g.dxi / 1.0);
     const double DIVY = (G[CS + at(IC, JC + 1, KC)] - G[CS + c]) * (g.dyfi[JC - 1] / 1.0);
@@ -425,8 +450,8 @@ __global__ void __launch_bounds__(256) k_io_divg(IoGeo g, const double* __restri
 __global__ void __launch_bounds__(256) k_io_update(IoGeo g, double* __restrict__ G, double* __restrict__ PR,
                                                    const double* __restrict__ DPH, double dta) {
 #pragma clang fp contract(off)
-    IO_CELL;
-    const int JC = blockIdx.z + 1;
+    IO_CELL(g.M, 1);
+    const int JC = JT;
     const int c = at(IC, JC, KC);
     const double pn = PR[c] + DPH[c];
     PR[c] = pn;
@@ -530,7 +555,8 @@ static int io_check(const chs_handle* h, const chs_io_params* prm, int ns, const
     return CHS_OK;
 }
 
-static dim3 io_grid(const chs_handle* h, int rows) { return dim3((h->L + 63) / 64, (h->N + 3) / 4, rows); }
+static long io_tiles(const chs_handle* h, int rows) { return (long)((h->L + 63) / 64) * ((h->N + 3) / 4) * rows; }
+static dim3 io_grid(const chs_handle* h, int rows) { return dim3((unsigned)(8 * ((io_tiles(h, rows) + 7) / 8))); }
 
 int io_prepare(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields* f, const chs_io_params* prm,
                int ns) {
@@ -540,8 +566,9 @@ int io_prepare(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     const hipStream_t s = h->stream;
     hipEvent_t ev;
     h->timer.begin("io_prepare", s, &ev);
-    CHS_LAUNCH(k_io_prepare, io_grid(h, h->M + 2), dim3(64, 4), 0, s, io_geo(h), a, prm->thermal,
-               TGAM_IO[ns] + TROH_IO[ns], TROH_IO[ns], TGAM_IO[ns] - 2.0);
+    CHS_LAUNCH(k_io_prepare, io_grid(h, h->M + 2), dim3(64, 4), 0, s, io_geo(h), a.rho, a.rho0, a.mu, a.mu0, a.g,
+               a.g0, a.drhodtp, a.rs, a.ms, a.q, prm->thermal, TGAM_IO[ns] + TROH_IO[ns], TROH_IO[ns],
+               TGAM_IO[ns] - 2.0);
     h->timer.end(ev, "io_prepare", s);
     return CHS_OK;
 }
@@ -554,7 +581,7 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     const hipStream_t s = h->stream;
     const IoGeo g = io_geo(h);
     const dim3 gi = io_grid(h, h->M);
-    const long nb = (long)gi.x * gi.y * gi.z;
+    const long nb = io_tiles(h, h->M);
     CHS_CHECK(nb <= 2L * h->npart, CHS_E_ARG, "chs_solverrk3_mom_io: grid too large for the block-sum buffer");
     const size_t ngh = (size_t)h->L * (h->M + 2) * h->N;
     const double talp = TGAM_IO[ns] + TROH_IO[ns];
@@ -566,8 +593,16 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     h->timer.end(ev, "io_divu", s);
     IoParams p{prm->thermal, prm->nflow, prm->flowdrvtp, prm->gravdir, prm->cfgv, prm->f_a};
     h->timer.begin("io_terms", s, &ev);
-    CHS_LAUNCH(k_io_terms, gi, dim3(64, 4), 0, s, g, a, p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs,
-               h->partials);
+#define IO_TERMS(GR)                                                                                              \
+    CHS_LAUNCH(k_io_terms<GR>, gi, dim3(64, 4), 0, s, g, a.g, a.q, a.mu, a.mu0, a.divu, a.ms, a.pr, a.rs, a.explt0, \
+               p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs, h->partials)
+    switch (prm->gravdir) {
+        case 1: IO_TERMS(1); break;
+        case 2: IO_TERMS(2); break;
+        case 3: IO_TERMS(3); break;
+        default: IO_TERMS(0); break;
+    }
+#undef IO_TERMS
     h->timer.end(ev, "io_terms", s);
     CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(256), 0, s, h->partials, (int)nb, h->volm, prm->flowdrvtp,
                -0.5 * prm->cfgv * (talp * dt), h->scalars);

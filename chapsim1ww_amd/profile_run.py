@@ -4,7 +4,8 @@
 
 Runs, at 256x192x256: NSUB FP64 RK3 substeps, NPOIS FP64 Poisson solves,
 NTDMA batched tridiagonal solves (65536 lines x 192, IJJ form, non-cyclic
-and cyclic), NTDMA table-lookup THERM_PROP_UPDATEs over the grid, and a plain
+and cyclic), NTDMA table-lookup THERM_PROP_UPDATEs over the grid, NHEAT
+heated-channel substeps (energy, property staging, io momentum), and a plain
 device copy of 1 GiB (read 1 GiB + write 1 GiB)
 whose counter readings calibrate the byte counters of the others.
 """
@@ -20,6 +21,7 @@ def main() -> None:
     ap.add_argument("--nsub", type=int, default=4)
     ap.add_argument("--npois", type=int, default=4)
     ap.add_argument("--ntdma", type=int, default=4)
+    ap.add_argument("--nheat", type=int, default=3)
     ap.add_argument("--precision", default="fp64")
     a = ap.parse_args()
     import torch
@@ -59,6 +61,27 @@ def main() -> None:
     outs = [torch.empty_like(hh) for _ in range(5)]
     for _ in range(a.ntdma):
         pt.update(hh, *outs)
+    # heated channel: energy + staging + io momentum from the tg velocity as mass flux
+    from chapsim1ww_amd.heated import HeatedChannel
+    hc = HeatedChannel(s, pt)
+    L, M, N = g.L, g.M, g.N
+    yc = np.concatenate([[-1.0], np.asarray(g.ycc), [1.0]])
+    T3 = 1.2 - 0.3 * yc[None, :, None] ** 2 * np.ones((L, 1, N))
+    hg = torch.from_numpy(np.interp(T3.ravel(order="F"), sc["T"], sc["H"])).to(dev)
+    fd = hc.dev
+    fd["enthalpy"].copy_(hg)
+    pt.update(hg, fd["temperature"], fd["density"], fd["viscosity"], fd["thermcondt"])
+    fd["rhoh"].copy_(hg * fd["density"])
+    for k in ("density0", "density1"):
+        fd[k].copy_(fd["density"])
+    fd["viscosity0"].copy_(fd["viscosity"])
+    qs, _, _ = s.download_state()
+    qd = torch.from_numpy(np.asfortranarray(qs).ravel(order="F")).to(dev)
+    fd["g"].copy_(qd)
+    fd["g0"].copy_(qd)
+    fd["wallflux"].fill_(0.1)
+    for i in range(a.nheat):
+        hc.substep(1 + i % 3, 1e-3, 1.0 / 2800.0)
     pt.close()
     # calibration: 1 GiB device-to-device copy
     x = torch.empty(1 << 27, dtype=torch.float64, device=dev)
