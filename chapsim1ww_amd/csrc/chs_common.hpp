@@ -115,6 +115,32 @@ struct KernelTimer {
     void collect();
 };
 
+// Tiles of 64 (i) x 4 (k) cells of one row j, run one per block of a 1-D
+// grid of 8 * ceil(ntiles / 8) blocks.  Block b runs on XCD b % 8; tile
+// t = (b % 8) * per + b / 8 gives each XCD a contiguous range of tiles,
+// ordered i-tile fastest, then j, then k-tile, so the tiles an XCD runs
+// back to back are j-neighbours and their j +- 1 rows are in its L2.
+// Returns false for the padding blocks.
+__device__ __forceinline__ bool row_tile(int L, int N, int nj, int* it, int* j, int* kt, int* tile) {
+    const int ni = (L + 63) / 64, nk = (N + 3) / 4;
+    const int nt = ni * nj * nk, per = (nt + 7) / 8;
+    const int b = blockIdx.x;
+    const int t = (b % 8) * per + b / 8;
+    if (t >= nt) return false;
+    *it = t % ni;
+    const int r = t / ni;
+    *j = r % nj;
+    *kt = r / nj;
+    *tile = t;
+    return true;
+}
+
+// blocks of the 1-D grid for row_tile over nj rows
+inline unsigned row_tile_blocks(int L, int N, int nj) {
+    const long nt = (long)((L + 63) / 64) * ((N + 3) / 4) * nj;
+    return (unsigned)(8 * ((nt + 7) / 8));
+}
+
 }  // namespace chs
 
 // The handle behind the opaque chs_handle of the C ABI.

@@ -42,9 +42,11 @@ __global__ void __launch_bounds__(256) k_energy_rhs(EGeo g, EFields f, double rk
                                                     double rhoh_cap) {
 #pragma clang fp contract(off)
     const int L = g.L, M = g.M, N = g.N;
-    const int IC = blockIdx.x * 64 + threadIdx.x;   // 0-based i
-    const int KC = blockIdx.y * 4 + threadIdx.y;
-    const int JC = blockIdx.z + 1;                   // Fortran J (ghost-aware row)
+    int it, jt, kt, tile;
+    if (!row_tile(L, N, M, &it, &jt, &kt, &tile)) return;
+    const int IC = it * 64 + threadIdx.x;            // 0-based i
+    const int KC = kt * 4 + threadIdx.y;
+    const int JC = jt + 1;                           // Fortran J (ghost-aware row)
     if (IC >= L || KC >= N) return;
     const int IP = IC == L - 1 ? 0 : IC + 1, IM = IC == 0 ? L - 1 : IC - 1;
     const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;
@@ -219,7 +221,7 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     hipEvent_t ev;
     CHS_LAUNCH(k_zero_u64_e, dim3(1), dim3(64), 0, s, p->counters, 4);
     h->timer.begin("energy_rhs", s, &ev);
-    CHS_LAUNCH(k_energy_rhs, dim3((h->L + 63) / 64, (h->N + 3) / 4, h->M), dim3(64, 4), 0, s, g, f, TGAM_E[ns] * dt,
+    CHS_LAUNCH(k_energy_rhs, dim3(row_tile_blocks(h->L, h->N, h->M)), dim3(64, 4), 0, s, g, f, TGAM_E[ns] * dt,
                TROH_E[ns] * dt, cthecd, cap);
     h->timer.end(ev, "energy_rhs", s);
     const PropTab t = tab_of(p);

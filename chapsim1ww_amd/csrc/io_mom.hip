@@ -63,30 +63,10 @@ struct IoParams {
 static const double TGAM_IO[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
 static const double TROH_IO[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
 
-// Tiles of 64 (i) x 4 (k) cells of one row j, run one per block of a 1-D
-// grid of 8 * ceil(ntiles / 8) blocks.  Block b runs on XCD b % 8; tile
-// t = (b % 8) * per + b / 8 gives each XCD a contiguous range of tiles,
-// ordered i-tile fastest, then j, then k-tile, so the tiles an XCD runs
-// back to back are j-neighbours and their j +- 1 rows are in its L2.
-// Returns false for the padding blocks.
-__device__ __forceinline__ bool io_tile(int L, int N, int nj, int* it, int* j, int* kt, int* tile) {
-    const int ni = (L + 63) / 64, nk = (N + 3) / 4;
-    const int nt = ni * nj * nk, per = (nt + 7) / 8;
-    const int b = blockIdx.x;
-    const int t = (b % 8) * per + b / 8;
-    if (t >= nt) return false;
-    *it = t % ni;
-    const int r = t / ni;
-    *j = r % nj;
-    *kt = r / nj;
-    *tile = t;
-    return true;
-}
-
 #define IO_CELL(NJ, J0)                                                          \
     const int L = g.L, M = g.M, N = g.N;                                         \
     int it_, jt_, kt_, tile_;                                                    \
-    if (!io_tile(L, N, (NJ), &it_, &jt_, &kt_, &tile_)) return;                  \
+    if (!row_tile(L, N, (NJ), &it_, &jt_, &kt_, &tile_)) return;                  \
     const int IC = it_ * 64 + threadIdx.x;                                       \
     const int KC = kt_ * 4 + threadIdx.y;                                        \
     const int JT = jt_ + (J0);                                                   \
@@ -221,7 +201,7 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
     __shared__ double red[256];
     const int L = g.L, M = g.M, N = g.N;
     int it, jt, kt, tile;
-    if (!io_tile(L, N, M, &it, &jt, &kt, &tile)) return;   // padding block: whole block exits
+    if (!row_tile(L, N, M, &it, &jt, &kt, &tile)) return;   // padding block: whole block exits
     const int IC = it * 64 + threadIdx.x;
     const int KC = kt * 4 + threadIdx.y;
     const int JC = jt + 1;

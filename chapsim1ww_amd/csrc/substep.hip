@@ -169,20 +169,23 @@ __device__ __forceinline__ double rhs_rows(const Geo& g, const RkCoef& rk, doubl
         const double* W = V + CS;
         const double* P = PR + o;
         const int ci = i + L * j + L * M * k;
+        // All three components are evaluated before anything is stored, so
+        // their loads issue as one batch.  v on the bottom wall face (j = 0)
+        // is not advanced (NII = 2): its values are computed from in-bounds
+        // rows and neither CONVH0 nor its LDS row (never solved) keeps them.
+        double conv[3], rhs[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            if (c == 1 && j == 0) continue;  // v on the wall face is not advanced (NII = 2)
             const double* X = c == 0 ? U : c == 1 ? V : W;
-            const double conv = c == 0 ? conv_x(g, y, U, V, W, n) : c == 1 ? conv_y(g, y, U, V, W, n)
-                                                                            : conv_z(g, y, U, V, W, n);
+            conv[c] = c == 0 ? conv_x(g, y, U, V, W, n) : c == 1 ? conv_y(g, y, U, V, W, n)
+                                                                   : conv_z(g, y, U, V, W, n);
             const int hc = ci + c * LMN;
-            const double RHSC = rk.tgam * conv + rk.troh * H0[hc];
+            const double RHSC = rk.tgam * conv[c] + rk.troh * H0[hc];
             const double qc = X[0];
             const double RHSL = COE1 * ((X[n.ip] - 2.0 * qc + X[n.im]) * g.dxqi +
                                         (X[n.kp] - 2.0 * qc + X[n.km]) * g.dzqi * 1.0 +
                                         (X[n.L] * ap[c] + qc * ac[c] + X[-n.L] * am[c]));
-            H0[hc] = conv;
-            double rhs = (RHSC + RHSL) * rk.dt;
+            double r = (RHSC + RHSL) * rk.dt;
             double pgm;
             if (c == 0)
                 pgm = (P[0] - P[n.im]) * g.dxi * COE2;
@@ -190,13 +193,17 @@ __device__ __forceinline__ double rhs_rows(const Geo& g, const RkCoef& rk, doubl
                 pgm = (P[0] - P[-n.L]) * (y.dyci / 1.0) * COE2;
             else
                 pgm = (P[0] - P[n.km]) * g.dzi * COE2;
-            rhs = rhs - pgm;
-            if (c == 0) msum += rhs / y.dyfi / 1.0;
+            rhs[c] = r - pgm;
+        }
+        msum += rhs[0] / y.dyfi / 1.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
             double* row = sR + (c * kr + r) * LS;
-            row[row_pos<KH>(KH + i, Q4)] = rhs;
+            row[row_pos<KH>(KH + i, Q4)] = rhs[c];
             if (KH > 0) {
-                if (i < KH) row[row_pos<KH>(KH + i + L, Q4)] = rhs;
-                if (i >= L - KH) row[row_pos<KH>(KH + i - L, Q4)] = rhs;
+                if (i < KH) row[row_pos<KH>(KH + i + L, Q4)] = rhs[c];
+                if (i >= L - KH) row[row_pos<KH>(KH + i - L, Q4)] = rhs[c];
             }
         }
     }
