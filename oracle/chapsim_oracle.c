@@ -1358,6 +1358,9 @@ typedef struct {
     double *drhodtp, *divu, *rho_stg, *mu_stg, *explt0;
     double *dph, *hx, *hy, *hz, *rhsllphi;
     double dpgrns;
+    double *drhoi_stg;   /* DRHOI_STG (L, 0:M+1, N, 3), visimplicit only */
+    double *im;          /* implicit viscous parts (RHS_io of VISCOUS_PAR_*), (L, M, N, 3) */
+    double *ad;          /* their density corrections (DIVU_io of VISCOUS_PAR_*), (L, M, N, 3) */
 } orc_io;
 
 typedef struct {
@@ -1367,6 +1370,7 @@ typedef struct {
     int gravdir;      /* GRAVDIR: 0 none, 1..3 */
     double cfgv;      /* CFGV (FLOWDRVTP = 2) */
     double f_a;       /* F_A (src/Flow_State.f90:97-110) */
+    int visimplicit;  /* visthemflg: 0 explicit viscous terms, 1 RK-CN (MOMFA_io) */
 } orc_io_params;
 
 #define WL (g->L)
@@ -1500,6 +1504,30 @@ void orc_io_prepare(const orc_grid *g, orc_io *s, const orc_io_params *p, int NS
                 WG(MS, IC, WM + 1, KC, 3) = wt;
             }
         (void)ngh;
+        if (p->visimplicit) {
+            /* DENSITY_implicit_add (src/MASSFLUX_CALC_IO.f90:316-400) */
+            double *DR = s->drhoi_stg;
+#pragma omp parallel for schedule(static)
+            for (int JC = 1; JC <= WM; ++JC)
+                for (int KC = 1; KC <= WN; ++KC)
+                    for (int IC = 1; IC <= WL; ++IC) {
+                        const int IM = WIMV(IC), KM = WKMV(KC);
+                        double DEN_STG = (W3(D0, IC, JC, KC) + W3(D0, IM, JC, KC)) * 0.5;
+                        WG(DR, IC, JC, KC, 1) = 1.0 / WG(RS, IC, JC, KC, 1) - 1.0 / DEN_STG;
+                        DEN_STG = (W3(D0, IC, JC, KC) + W3(D0, IC, JC, KM)) * 0.5;
+                        WG(DR, IC, JC, KC, 3) = 1.0 / WG(RS, IC, JC, KC, 3) - 1.0 / DEN_STG;
+                        if (JC >= 2) {
+                            DEN_STG = g->wff[JC - 1] * W3(D0, IC, JC, KC) + g->wfb[JC - 1] * W3(D0, IC, JC - 1, KC);
+                            WG(DR, IC, JC, KC, 2) = 1.0 / WG(RS, IC, JC, KC, 2) - 1.0 / DEN_STG;
+                        }
+                    }
+            io_wrap3(g, DR);
+            for (int KC = 1; KC <= WN; ++KC)
+                for (int IC = 1; IC <= WL; ++IC) {
+                    WG(DR, IC, 1, KC, 2) = 0.0;
+                    WG(DR, IC, WM + 1, KC, 2) = 0.0;
+                }
+        }
     }
     /* VELOCITY_CALC_io */
 #pragma omp parallel for schedule(static)
@@ -1530,6 +1558,10 @@ static void io_rhs_mom(const orc_grid *g, orc_io *s, const orc_io_params *p, int
                 const int IM = WIMV(IC);
                 const double RHSC = WF3(h, IC, JC, KC), RHSL = WE(s->explt0, IC, JC, KC, IDR);
                 double r = (TGAM[NS] * RHSC + TROH[NS] * RHSL) * DT;
+                if (p->visimplicit) {
+                    const double COE2 = TA * DT;
+                    r = r + COE2 * WE(s->im, IC, JC, KC, IDR) + 0.5 * COE2 * WE(s->ad, IC, JC, KC, IDR);
+                }
                 WE(s->explt0, IC, JC, KC, IDR) = RHSC;
                 double PGM;
                 if (IDR == 1)
@@ -1566,12 +1598,10 @@ static void io_rhs_mom(const orc_grid *g, orc_io *s, const orc_io_params *p, int
     free(RHS);
 }
 
-/* The explicit terms: CONVECTION_X/Y/Z_io, DIVG_U_io and
- * VISCOUS_ALL_EXPLT_X/Y/Z_io -> hx, hy (row 1 zero), hz. */
-void orc_io_terms(const orc_grid *g, orc_io *s)
+/* CONVECTION_X/Y/Z_io -> hx, hy (row 1 zero), hz */
+static void io_convection(const orc_grid *g, orc_io *s)
 {
     const size_t ncell = (size_t)WL * WM * WN;
-    const double CV = g->cvisc;
     memset(s->hy, 0, ncell * sizeof(double));
 #pragma omp parallel for schedule(static)
     for (int JC = 1; JC <= WM; ++JC) {
@@ -1634,13 +1664,30 @@ void orc_io_terms(const orc_grid *g, orc_io *s)
                     double H33 = (H33F - H33B) * COE31;
                     WF3(s->hz, IC, JC, KC) = -(H31 + H32 + H33);
                 }
-                /* DIVG_U_io */
-                {
-                    double DIVX = (WG(Q, IP, JC, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dxi;
-                    double DIVY = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JC, KC, 2)) * (g->dyfi[JJ - 1] * 1.0);
-                    double DIVZ = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JC, KC, 3)) * (g->dzi * 1.0);
-                    W3(s->divu, IC, JC, KC) = (DIVX + DIVY + DIVZ) / 3.0;
-                }
+            }
+        }
+    }
+}
+
+/* The explicit terms: CONVECTION_X/Y/Z_io, DIVG_U_io and
+ * VISCOUS_ALL_EXPLT_X/Y/Z_io -> hx, hy (row 1 zero), hz. */
+void orc_io_terms(const orc_grid *g, orc_io *s)
+{
+    const double CV = g->cvisc;
+    io_convection(g, s);
+    /* DIVG_U_io */
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= WM; ++JC) {
+        const int JP = JC + 1, JJ = JC;
+        for (int KC = 1; KC <= WN; ++KC) {
+            const int KP = WKPV(KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IP = WIPV(IC);
+                const double *Q = s->q;
+                double DIVX = (WG(Q, IP, JC, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dxi;
+                double DIVY = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JC, KC, 2)) * (g->dyfi[JJ - 1] * 1.0);
+                double DIVZ = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JC, KC, 3)) * (g->dzi * 1.0);
+                W3(s->divu, IC, JC, KC) = (DIVX + DIVY + DIVZ) / 3.0;
             }
         }
     }
@@ -1728,22 +1775,350 @@ void orc_io_terms(const orc_grid *g, orc_io *s)
     }
 }
 
+/* VISCOUS_PAR_EXPLT_X/Y/Z_io (src/VISCOUS_X_EXPLICIT_io.f90:116-289,
+ * src/VISCOUS_Y_EXPLICIT_io.f90:264-423, src/VISCOUS_Z_EXPLICIT_io.f90:289-460,
+ * channel branches): the viscous stress split for RK-CN.  The explicit parts
+ * are added to the convection terms (EX + h); the implicit parts (evaluated
+ * explicitly) go to im, the density corrections (with DRHOI_STG) to ad.
+ * Rows 2..M for y (row 1 of im, ad zero).  DIVU_io ends as the z correction
+ * on rows 1..M, as the reference leaves it. */
+static void io_viscous_par(const orc_grid *g, orc_io *s)
+{
+    const double CV = g->cvisc;
+    const size_t ncell = (size_t)WL * WM * WN;
+    memset(s->im, 0, 3 * ncell * sizeof(double));
+    memset(s->ad, 0, 3 * ncell * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int JC = 1; JC <= WM; ++JC) {
+        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC - 1;
+        for (int KC = 1; KC <= WN; ++KC) {
+            const int KM = WKMV(KC), KP = WKPV(KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                const int IP = WIPV(IC), IM = WIMV(IC);
+                const double *Q = s->q, *G = s->g, *MU = s->mu, *MS = s->mu_stg, *DR = s->drhoi_stg;
+                {   /* x */
+                    const double COE1 = -2.0 / 3.0 * g->dxi * CV, COE2 = g->dyfi[JJ - 1] * CV * 1.0;
+                    const double COE32 = g->dzi * CV * 1.0;
+                    double DUDX = (WG(Q, IP, JC, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dxi;
+                    double DVDY = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JC, KC, 2)) * g->dyfi[JJ - 1] * 1.0;
+                    double DWDZ = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JC, KC, 3)) * g->dzi * 1.0;
+                    const double F_EX = W3(MU, IC, JC, KC) * (DVDY + DWDZ), F_IM = W3(MU, IC, JC, KC) * DUDX;
+                    DUDX = (WG(G, IP, JC, KC, 1) * WG(DR, IP, JC, KC, 1) - WG(G, IC, JC, KC, 1) * WG(DR, IC, JC, KC, 1)) * g->dxi;
+                    const double F_AD = W3(MU, IC, JC, KC) * DUDX;
+                    DUDX = (WG(Q, IC, JC, KC, 1) - WG(Q, IM, JC, KC, 1)) * g->dxi;
+                    DVDY = (WG(Q, IM, JP, KC, 2) - WG(Q, IM, JC, KC, 2)) * g->dyfi[JJ - 1] * 1.0;
+                    DWDZ = (WG(Q, IM, JC, KP, 3) - WG(Q, IM, JC, KC, 3)) * g->dzi * 1.0;
+                    const double B_EX = W3(MU, IM, JC, KC) * (DVDY + DWDZ), B_IM = W3(MU, IM, JC, KC) * DUDX;
+                    DUDX = (WG(G, IC, JC, KC, 1) * WG(DR, IC, JC, KC, 1) - WG(G, IM, JC, KC, 1) * WG(DR, IM, JC, KC, 1)) * g->dxi;
+                    const double B_AD = W3(MU, IM, JC, KC) * DUDX;
+                    const double X_EX = (F_EX - B_EX) * COE1, X_IM = (F_IM - B_IM) * COE1 * (-2.0);
+                    const double X_AD = (F_AD - B_AD) * COE1 * (-2.0);
+                    double VP = WG(MS, IC, JP, KC, 1);
+                    double DVDX = (WG(Q, IC, JP, KC, 2) - WG(Q, IM, JP, KC, 2)) * g->dxi;
+                    double DUDY = (WG(Q, IC, JP, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dyci[JJP - 1] / 1.0;
+                    const double F2_EX = VP * DVDX, F2_IM = VP * DUDY;
+                    DUDY = (WG(G, IC, JP, KC, 1) * WG(DR, IC, JP, KC, 1) - WG(G, IC, JC, KC, 1) * WG(DR, IC, JC, KC, 1)) *
+                           g->dyci[JJP - 1] / 1.0;
+                    const double F2_AD = VP * DUDY;
+                    double VC = WG(MS, IC, JC, KC, 1);
+                    DUDY = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JM, KC, 1)) * g->dyci[JJ - 1] / 1.0;
+                    DVDX = (WG(Q, IC, JC, KC, 2) - WG(Q, IM, JC, KC, 2)) * g->dxi;
+                    const double B2_EX = VC * DVDX, B2_IM = VC * DUDY;
+                    DUDY = (WG(G, IC, JC, KC, 1) * WG(DR, IC, JC, KC, 1) - WG(G, IC, JM, KC, 1) * WG(DR, IC, JM, KC, 1)) *
+                           g->dyci[JJ - 1] / 1.0;
+                    const double B2_AD = VC * DUDY;
+                    const double Y_EX = (F2_EX - B2_EX) * COE2, Y_IM = (F2_IM - B2_IM) * COE2, Y_AD = (F2_AD - B2_AD) * COE2;
+                    VP = WG(MS, IC, JC, KP, 2);
+                    double DUDZ = (WG(Q, IC, JC, KP, 1) - WG(Q, IC, JC, KC, 1)) * g->dzi;
+                    double DWDX = (WG(Q, IC, JC, KP, 3) - WG(Q, IM, JC, KP, 3)) * g->dxi;
+                    const double F3_EX = VP * DWDX, F3_IM = VP * DUDZ;
+                    DUDZ = (WG(Q, IC, JC, KP, 1) * WG(DR, IC, JC, KP, 1) - WG(Q, IC, JC, KC, 1) * WG(DR, IC, JC, KC, 1)) * g->dzi;
+                    const double F3_AD = VP * DUDZ;
+                    VC = WG(MS, IC, JC, KC, 2);
+                    DUDZ = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JC, KM, 1)) * g->dzi;
+                    DWDX = (WG(Q, IC, JC, KC, 3) - WG(Q, IM, JC, KC, 3)) * g->dxi;
+                    const double B3_EX = VC * DWDX, B3_IM = VC * DUDZ;
+                    DUDZ = (WG(Q, IC, JC, KC, 1) * WG(DR, IC, JC, KC, 1) - WG(Q, IC, JC, KM, 1) * WG(DR, IC, JC, KM, 1)) * g->dzi;
+                    const double B3_AD = VC * DUDZ;
+                    const double Z_EX = (F3_EX - B3_EX) * COE32, Z_IM = (F3_IM - B3_IM) * COE32, Z_AD = (F3_AD - B3_AD) * COE32;
+                    WF3(s->hx, IC, JC, KC) = X_EX + Y_EX + Z_EX + WF3(s->hx, IC, JC, KC);
+                    WE(s->im, IC, JC, KC, 1) = X_IM + Y_IM + Z_IM;
+                    WE(s->ad, IC, JC, KC, 1) = X_AD + Y_AD + Z_AD;
+                }
+                if (JC >= 2) {   /* y */
+                    const double COE1 = g->dxi * CV, COE3 = g->dzi * CV, COE2 = -2.0 / 3.0 * g->dyci[JJ - 1] * CV;
+                    double DUDX = (WG(Q, IP, JC, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dxi;
+                    double DVDY = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JC, KC, 2)) * g->dyfi[JJ - 1];
+                    double DWDZ = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JC, KC, 3)) * g->dzi;
+                    const double F_EX = W3(MU, IC, JC, KC) * (DUDX + DWDZ), F_IM = W3(MU, IC, JC, KC) * DVDY;
+                    DVDY = (WG(G, IC, JP, KC, 2) * WG(DR, IC, JP, KC, 2) - WG(G, IC, JC, KC, 2) * WG(DR, IC, JC, KC, 2)) *
+                           g->dyfi[JJ - 1];
+                    const double F_AD = W3(MU, IC, JC, KC) * DVDY;
+                    DUDX = (WG(Q, IP, JM, KC, 1) - WG(Q, IC, JM, KC, 1)) * g->dxi;
+                    DVDY = (WG(Q, IC, JC, KC, 2) - WG(Q, IC, JM, KC, 2)) * g->dyfi[JJM - 1];
+                    DWDZ = (WG(Q, IC, JM, KP, 3) - WG(Q, IC, JM, KC, 3)) * g->dzi;
+                    const double B_EX = W3(MU, IC, JM, KC) * (DUDX + DWDZ), B_IM = W3(MU, IC, JM, KC) * DVDY;
+                    DVDY = (WG(G, IC, JC, KC, 2) * WG(DR, IC, JC, KC, 2) - WG(G, IC, JM, KC, 2) * WG(DR, IC, JM, KC, 2)) *
+                           g->dyfi[JJM - 1];
+                    const double B_AD = W3(MU, IC, JM, KC) * DVDY;
+                    const double Y_EX = (F_EX - B_EX) * COE2, Y_IM = (F_IM - B_IM) * COE2 * (-2.0);
+                    const double Y_AD = (F_AD - B_AD) * COE2 * (-2.0);
+                    double VP = WG(MS, IP, JC, KC, 1);
+                    double DVDX = (WG(Q, IP, JC, KC, 2) - WG(Q, IC, JC, KC, 2)) * g->dxi;
+                    double DUDY = (WG(Q, IP, JC, KC, 1) - WG(Q, IP, JM, KC, 1)) * g->dyci[JJ - 1];
+                    const double F1_EX = VP * DUDY, F1_IM = VP * DVDX;
+                    DVDX = (WG(G, IP, JC, KC, 2) * WG(DR, IP, JC, KC, 2) - WG(G, IC, JC, KC, 2) * WG(DR, IC, JC, KC, 2)) * g->dxi;
+                    const double F1_AD = VP * DVDX;
+                    double VC = WG(MS, IC, JC, KC, 1);
+                    DVDX = (WG(Q, IC, JC, KC, 2) - WG(Q, IM, JC, KC, 2)) * g->dxi;
+                    DUDY = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JM, KC, 1)) * g->dyci[JJ - 1];
+                    const double B1_EX = VC * DUDY, B1_IM = VC * DVDX;
+                    DVDX = (WG(G, IC, JC, KC, 2) * WG(DR, IC, JC, KC, 2) - WG(G, IM, JC, KC, 2) * WG(DR, IM, JC, KC, 2)) * g->dxi;
+                    const double B1_AD = VC * DVDX;
+                    const double X_EX = (F1_EX - B1_EX) * COE1, X_IM = (F1_IM - B1_IM) * COE1, X_AD = (F1_AD - B1_AD) * COE1;
+                    VP = WG(MS, IC, JC, KP, 3);
+                    double DVDZ = (WG(Q, IC, JC, KP, 2) - WG(Q, IC, JC, KC, 2)) * g->dzi;
+                    double DWDY = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JM, KP, 3)) * g->dyci[JJ - 1];
+                    const double F3_EX = VP * DWDY, F3_IM = VP * DVDZ;
+                    DVDZ = (WG(G, IC, JC, KP, 2) * WG(DR, IC, JC, KP, 2) - WG(G, IC, JC, KC, 2) * WG(DR, IC, JC, KC, 2)) * g->dzi;
+                    const double F3_AD = VP * DVDZ;
+                    VC = WG(MS, IC, JC, KC, 3);
+                    DVDZ = (WG(Q, IC, JC, KC, 2) - WG(Q, IC, JC, KM, 2)) * g->dzi;
+                    DWDY = (WG(Q, IC, JC, KC, 3) - WG(Q, IC, JM, KC, 3)) * g->dyci[JJ - 1];
+                    const double B3_EX = VC * DWDY, B3_IM = VC * DVDZ;
+                    DVDZ = (WG(G, IC, JC, KC, 2) * WG(DR, IC, JC, KC, 2) - WG(G, IC, JC, KM, 2) * WG(DR, IC, JC, KM, 2)) * g->dzi;
+                    const double B3_AD = VC * DVDZ;
+                    const double Z_EX = (F3_EX - B3_EX) * COE3, Z_IM = (F3_IM - B3_IM) * COE3, Z_AD = (F3_AD - B3_AD) * COE3;
+                    WF3(s->hy, IC, JC, KC) = X_EX + Y_EX + Z_EX + WF3(s->hy, IC, JC, KC);
+                    WE(s->im, IC, JC, KC, 2) = X_IM + Y_IM + Z_IM;
+                    WE(s->ad, IC, JC, KC, 2) = X_AD + Y_AD + Z_AD;
+                }
+                {   /* z */
+                    const double COE1 = g->dxi * CV, COE3 = -2.0 / 3.0 * g->dzi * CV, COE2 = g->dyfi[JJ - 1] * CV;
+                    double DUDX = (WG(Q, IP, JC, KC, 1) - WG(Q, IC, JC, KC, 1)) * g->dxi;
+                    double DVDY = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JC, KC, 2)) * g->dyfi[JJ - 1];
+                    double DWDZ = (WG(Q, IC, JC, KP, 3) - WG(Q, IC, JC, KC, 3)) * g->dzi;
+                    const double F_EX = W3(MU, IC, JC, KC) * (DUDX + DVDY), F_IM = W3(MU, IC, JC, KC) * DWDZ;
+                    DWDZ = (WG(G, IC, JC, KP, 3) * WG(DR, IC, JC, KP, 3) - WG(G, IC, JC, KC, 3) * WG(DR, IC, JC, KC, 3)) * g->dzi;
+                    const double F_AD = W3(MU, IC, JC, KC) * DWDZ;
+                    DUDX = (WG(Q, IP, JC, KM, 1) - WG(Q, IC, JC, KM, 1)) * g->dxi;
+                    DVDY = (WG(Q, IC, JP, KM, 2) - WG(Q, IC, JC, KM, 2)) * g->dyfi[JJ - 1];
+                    DWDZ = (WG(Q, IC, JC, KC, 3) - WG(Q, IC, JC, KM, 3)) * g->dzi;
+                    const double B_EX = W3(MU, IC, JC, KM) * (DUDX + DVDY), B_IM = W3(MU, IC, JC, KM) * DWDZ;
+                    DWDZ = (WG(G, IC, JC, KC, 3) * WG(DR, IC, JC, KC, 3) - WG(G, IC, JC, KM, 3) * WG(DR, IC, JC, KM, 3)) * g->dzi;
+                    const double B_AD = W3(MU, IC, JC, KM) * DWDZ;
+                    const double Z_EX = (F_EX - B_EX) * COE3, Z_IM = (F_IM - B_IM) * COE3 * (-2.0);
+                    const double Z_AD = (F_AD - B_AD) * COE3 * (-2.0);
+                    double VP = WG(MS, IP, JC, KC, 2);
+                    double DWDX = (WG(Q, IP, JC, KC, 3) - WG(Q, IC, JC, KC, 3)) * g->dxi;
+                    double DUDZ = (WG(Q, IP, JC, KC, 1) - WG(Q, IP, JC, KM, 1)) * g->dzi;
+                    const double F1_EX = VP * DUDZ, F1_IM = VP * DWDX;
+                    DWDX = (WG(G, IP, JC, KC, 3) * WG(DR, IP, JC, KC, 3) - WG(G, IC, JC, KC, 3) * WG(DR, IC, JC, KC, 3)) * g->dxi;
+                    const double F1_AD = VP * DWDX;
+                    double VC = WG(MS, IC, JC, KC, 2);
+                    DWDX = (WG(Q, IC, JC, KC, 3) - WG(Q, IM, JC, KC, 3)) * g->dxi;
+                    DUDZ = (WG(Q, IC, JC, KC, 1) - WG(Q, IC, JC, KM, 1)) * g->dzi;
+                    const double B1_EX = VC * DUDZ, B1_IM = VC * DWDX;
+                    DWDX = (WG(G, IC, JC, KC, 3) * WG(DR, IC, JC, KC, 3) - WG(G, IM, JC, KC, 3) * WG(DR, IM, JC, KC, 3)) * g->dxi;
+                    const double B1_AD = VC * DWDX;
+                    const double X_EX = (F1_EX - B1_EX) * COE1, X_IM = (F1_IM - B1_IM) * COE1, X_AD = (F1_AD - B1_AD) * COE1;
+                    VP = WG(MS, IC, JP, KC, 3);
+                    double DWDY = (WG(Q, IC, JP, KC, 3) - WG(Q, IC, JC, KC, 3)) * g->dyci[JJP - 1];
+                    double DVDZ = (WG(Q, IC, JP, KC, 2) - WG(Q, IC, JP, KM, 2)) * g->dzi;
+                    const double F2_EX = VP * DVDZ, F2_IM = VP * DWDY;
+                    DWDY = (WG(G, IC, JP, KC, 3) * WG(DR, IC, JP, KC, 3) - WG(G, IC, JC, KC, 3) * WG(DR, IC, JC, KC, 3)) *
+                           g->dyci[JJP - 1];
+                    const double F2_AD = VP * DWDY;
+                    VC = WG(MS, IC, JC, KC, 3);
+                    DWDY = (WG(Q, IC, JC, KC, 3) - WG(Q, IC, JM, KC, 3)) * g->dyci[JJ - 1];
+                    DVDZ = (WG(Q, IC, JC, KC, 2) - WG(Q, IC, JC, KM, 2)) * g->dzi;
+                    const double B2_EX = VC * DVDZ, B2_IM = VC * DWDY;
+                    DWDY = (WG(G, IC, JC, KC, 3) * WG(DR, IC, JC, KC, 3) - WG(G, IC, JM, KC, 3) * WG(DR, IC, JM, KC, 3)) *
+                           g->dyci[JJ - 1];
+                    const double B2_AD = VC * DWDY;
+                    const double Y_EX = (F2_EX - B2_EX) * COE2, Y_IM = (F2_IM - B2_IM) * COE2, Y_AD = (F2_AD - B2_AD) * COE2;
+                    WF3(s->hz, IC, JC, KC) = X_EX + Y_EX + Z_EX + WF3(s->hz, IC, JC, KC);
+                    WE(s->im, IC, JC, KC, 3) = X_IM + Y_IM + Z_IM;
+                    WE(s->ad, IC, JC, KC, 3) = X_AD + Y_AD + Z_AD;
+                    W3(s->divu, IC, JC, KC) = WE(s->ad, IC, JC, KC, 3);
+                }
+            }
+        }
+    }
+}
+
+#define WAMV(JC, IDR) g->amvr[((IDR)-1) * (WM + 1) + (JC)-1]
+#define WAPV(JC, IDR) g->apvr[((IDR)-1) * (WM + 1) + (JC)-1]
+/* MOMFA_io(NS, IDR) (src/MOMFA_io.f90:17-48): MOMFA1_X_io, MOMFA2_Z_io,
+ * MOMFA3_Y_io (src/MOMFA_io.f90:51-377) on r, then G += r from row N2I. */
+static void io_momfa(const orc_grid *g, orc_io *s, int NS, int IDR, double DT, double *r)
+{
+    const int N2I = IDR == 2 ? 2 : 1, JSZ = WM - N2I + 1;
+    const double FACOE = 0.5 * talp(NS) * DT * g->cvisc;
+    const double *MU = s->mu, *MS = s->mu_stg, *RS = s->rho_stg;
+    /* x: cyclic lines along i, one set per k (TDMAIJI_CYC) */
+    {
+        const double COE1 = FACOE * 4.0 / 3.0 * g->dxqi, COE2 = FACOE * g->dxqi;
+        const size_t pl = (size_t)WL * JSZ;
+        double *A = (double *)malloc(4 * pl * sizeof(double)), *B = A + pl, *Cc = B + pl, *F = Cc + pl;
+        for (int KC = 1; KC <= WN; ++KC) {
+            for (int JC = N2I; JC <= WM; ++JC)
+                for (int IC = 1; IC <= WL; ++IC) {
+                    const int IM = WIMV(IC), IP = WIPV(IC);
+                    double AMU, APU;
+                    if (IDR == 1) {
+                        AMU = -COE1 * W3(MU, IM, JC, KC);
+                        APU = -COE1 * W3(MU, IC, JC, KC);
+                    } else if (IDR == 2) {
+                        AMU = -COE2 * WG(MS, IC, JC, KC, 1);
+                        APU = -COE2 * WG(MS, IP, JC, KC, 1);
+                    } else {
+                        AMU = -COE2 * WG(MS, IC, JC, KC, 2);
+                        APU = -COE2 * WG(MS, IP, JC, KC, 2);
+                    }
+                    const size_t o = (size_t)(IC - 1) + (size_t)WL * (JC - N2I);
+                    A[o] = AMU / WG(RS, IM, JC, KC, IDR);
+                    Cc[o] = APU / WG(RS, IP, JC, KC, IDR);
+                    B[o] = 1.0 - (AMU + APU) / WG(RS, IC, JC, KC, IDR);
+                    F[o] = WF3(r, IC, JC, KC);
+                }
+            orc_tdma_cyc(A, B, Cc, F, WL, JSZ, 1, WL);
+            for (int JC = N2I; JC <= WM; ++JC)
+                for (int IC = 1; IC <= WL; ++IC) WF3(r, IC, JC, KC) = F[(size_t)(IC - 1) + (size_t)WL * (JC - N2I)];
+        }
+        free(A);
+    }
+    /* z: cyclic lines along k, one set per i */
+    {
+        const double COE1 = FACOE * 4.0 / 3.0 * g->dzqi, COE2 = FACOE * g->dzqi;
+        const size_t pl = (size_t)WN * JSZ;
+        double *A = (double *)malloc(4 * pl * sizeof(double)), *B = A + pl, *Cc = B + pl, *F = Cc + pl;
+        for (int IC = 1; IC <= WL; ++IC) {
+            for (int KC = 1; KC <= WN; ++KC) {
+                const int KM = WKMV(KC), KP = WKPV(KC);
+                for (int JC = N2I; JC <= WM; ++JC) {
+                    double AMU, APU;
+                    if (IDR == 1) {
+                        AMU = -COE2 * WG(MS, IC, JC, KC, 2) * 1.0;
+                        APU = -COE2 * WG(MS, IC, JC, KP, 2) * 1.0;
+                    } else if (IDR == 2) {
+                        AMU = -COE2 * WG(MS, IC, JC, KC, 3) * 1.0;
+                        APU = -COE2 * WG(MS, IC, JC, KP, 3) * 1.0;
+                    } else {
+                        AMU = -COE1 * W3(MU, IC, JC, KM) * 1.0;
+                        APU = -COE1 * W3(MU, IC, JC, KC) * 1.0;
+                    }
+                    const size_t o = (size_t)(KC - 1) + (size_t)WN * (JC - N2I);
+                    A[o] = AMU / WG(RS, IC, JC, KM, IDR);
+                    Cc[o] = APU / WG(RS, IC, JC, KP, IDR);
+                    B[o] = 1.0 - (AMU + APU) / WG(RS, IC, JC, KC, IDR);
+                    F[o] = WF3(r, IC, JC, KC);
+                }
+            }
+            orc_tdma_cyc(A, B, Cc, F, WN, JSZ, 1, WN);
+            for (int KC = 1; KC <= WN; ++KC)
+                for (int JC = N2I; JC <= WM; ++JC) WF3(r, IC, JC, KC) = F[(size_t)(KC - 1) + (size_t)WN * (JC - N2I)];
+        }
+        free(A);
+    }
+    /* y: non-cyclic lines along j (rows 1..M), one set per k (TDMAIJJ_nonCYC,
+     * zero boundary values); the wall rows of u, w use AMVR1 / APVRN, the
+     * coefficients TDMA_COEF sets aside (src/TDMA_COEF.f90:86-96) */
+    {
+        const double COE1 = FACOE * 4.0 / 3.0, COE2 = FACOE;
+        const double AMVR1 = 4.0 / (2.0 / g->dyci[1] + 1.0 / g->dyfi[0]) * (2.0 * g->dyfi[0]);
+        const double APVRN = 4.0 / (2.0 / g->dyci[WM - 1] + 1.0 / g->dyfi[WM - 1]) * (2.0 * g->dyfi[WM - 1]);
+        const size_t pl = (size_t)WL * WM;
+        double *A = (double *)malloc(4 * pl * sizeof(double)), *B = A + pl, *Cc = B + pl, *F = Cc + pl;
+        for (int KC = 1; KC <= WN; ++KC) {
+            for (int JC = 1; JC <= WM; ++JC) {
+                const int JM = JC - 1, JP = JC + 1;
+                for (int IC = 1; IC <= WL; ++IC) {
+                    const size_t o = (size_t)(IC - 1) + (size_t)WL * (JC - 1);
+                    double AMU, APU, a, c, b;
+                    if (IDR == 1 || IDR == 3) {
+                        const double *FM = MS + (size_t)(IDR - 1) * WL * (WM + 2) * WN;
+                        const double *FD = RS + (size_t)(IDR - 1) * WL * (WM + 2) * WN;
+                        if (JC == 1) {
+                            const double MUQT = (W3(MU, IC, JC, KC) + W3(MU, IC, JM, KC)) * 0.5;
+                            AMU = -COE2 * AMVR1 * MUQT;
+                            APU = -COE2 * WAPV(JC, IDR) * W3(FM, IC, JP, KC);
+                            a = 0.0;
+                            c = APU / W3(FD, IC, JP, KC);
+                        } else if (JC == WM) {
+                            const double MUQT = (W3(MU, IC, WM, KC) + W3(MU, IC, WM + 1, KC)) * 0.5;
+                            AMU = -COE2 * WAMV(JC, IDR) * W3(FM, IC, JC, KC);
+                            APU = -COE2 * APVRN * MUQT;
+                            a = AMU / W3(FD, IC, JM, KC);
+                            c = 0.0;
+                        } else {
+                            AMU = -COE2 * WAMV(JC, IDR) * W3(FM, IC, JC, KC);
+                            APU = -COE2 * WAPV(JC, IDR) * W3(FM, IC, JP, KC);
+                            a = AMU / W3(FD, IC, JM, KC);
+                            c = APU / W3(FD, IC, JP, KC);
+                        }
+                        b = 1.0 - (AMU + APU) / W3(FD, IC, JC, KC);
+                    } else {
+                        if (JC == 1) {
+                            AMU = -COE1 * WAMV(JC, 2) * W3(MU, IC, JM, KC);
+                            APU = -COE1 * WAPV(JC, 2) * W3(MU, IC, JC, KC);
+                            a = AMU / WG(RS, IC, JM, KC, 2);
+                            c = APU / WG(RS, IC, JP, KC, 2);
+                            b = 1.0 - (AMU + APU) / WG(RS, IC, JC, KC, 2);
+                        } else {
+                            AMU = -COE1 * WAMV(JC, 2) * W3(MU, IC, JM, KC);
+                            APU = -COE1 * WAPV(JC, 2) * W3(MU, IC, JC, KC);
+                            a = AMU / WG(RS, IC, JM, KC, 2);
+                            c = APU / WG(RS, IC, JC, KC, 2);
+                            b = 1.0 - (AMU + APU) / WG(RS, IC, JC, KC, 2);
+                        }
+                    }
+                    A[o] = a;
+                    B[o] = b;
+                    Cc[o] = c;
+                    F[o] = WF3(r, IC, JC, KC);
+                }
+            }
+            orc_tdma_noncyc(A, B, Cc, F, NULL, WM, WL, WL, 1);
+            for (int JC = 1; JC <= WM; ++JC)
+                for (int IC = 1; IC <= WL; ++IC) WF3(r, IC, JC, KC) = F[(size_t)(IC - 1) + (size_t)WL * (JC - 1)];
+        }
+        free(A);
+    }
+    for (int KC = 1; KC <= WN; ++KC)
+        for (int JC = N2I; JC <= WM; ++JC)
+            for (int IC = 1; IC <= WL; ++IC) WG(s->g, IC, JC, KC, IDR) = WF3(r, IC, JC, KC) + WG(s->g, IC, JC, KC, IDR);
+}
+
 void orc_io_substep(const orc_grid *g, orc_io *s, const orc_io_params *p, int NS, double DT)
 {
     const size_t ngh = (size_t)WL * (WM + 2) * WN;
     memcpy(s->g0, s->g, 3 * ngh * sizeof(double));
-    orc_io_terms(g, s);
-    io_rhs_mom(g, s, p, NS, 1, DT, s->hx);
-    io_rhs_mom(g, s, p, NS, 2, DT, s->hy);
-    io_rhs_mom(g, s, p, NS, 3, DT, s->hz);
-    /* MASSFLUX_CALC_IO, then INTFC_VARS3 + BC_WALL_G_io */
-    for (int KC = 1; KC <= WN; ++KC)
-        for (int JC = 1; JC <= WM; ++JC)
-            for (int IC = 1; IC <= WL; ++IC) {
-                WG(s->g, IC, JC, KC, 1) = WG(s->g, IC, JC, KC, 1) + WF3(s->hx, IC, JC, KC);
-                WG(s->g, IC, JC, KC, 3) = WG(s->g, IC, JC, KC, 3) + WF3(s->hz, IC, JC, KC);
-                if (JC >= 2) WG(s->g, IC, JC, KC, 2) = WG(s->g, IC, JC, KC, 2) + WF3(s->hy, IC, JC, KC);
-            }
+    if (p->visimplicit) {
+        /* VISCOUS_PAR_EXPLT_*_io, RHS_MOM_EXPLICIT_io and MOMFA_io per IDR
+         * (src/SOLVERRK3_MOM_IO.f90:76-90); MOMFA_io adds to G itself */
+        io_convection(g, s);
+        io_viscous_par(g, s);
+        double *h[3] = {s->hx, s->hy, s->hz};
+        for (int IDR = 1; IDR <= 3; ++IDR) {
+            io_rhs_mom(g, s, p, NS, IDR, DT, h[IDR - 1]);
+            io_momfa(g, s, NS, IDR, DT, h[IDR - 1]);
+        }
+    } else {
+        orc_io_terms(g, s);
+        io_rhs_mom(g, s, p, NS, 1, DT, s->hx);
+        io_rhs_mom(g, s, p, NS, 2, DT, s->hy);
+        io_rhs_mom(g, s, p, NS, 3, DT, s->hz);
+        /* MASSFLUX_CALC_IO */
+        for (int KC = 1; KC <= WN; ++KC)
+            for (int JC = 1; JC <= WM; ++JC)
+                for (int IC = 1; IC <= WL; ++IC) {
+                    WG(s->g, IC, JC, KC, 1) = WG(s->g, IC, JC, KC, 1) + WF3(s->hx, IC, JC, KC);
+                    WG(s->g, IC, JC, KC, 3) = WG(s->g, IC, JC, KC, 3) + WF3(s->hz, IC, JC, KC);
+                    if (JC >= 2) WG(s->g, IC, JC, KC, 2) = WG(s->g, IC, JC, KC, 2) + WF3(s->hy, IC, JC, KC);
+                }
+    }
+    /* INTFC_VARS3 + BC_WALL_G_io */
     io_wrap3(g, s->g);
     io_bc_wall_vec(g, s->g);
     /* DIVG_io(NS) */
@@ -1767,10 +2142,17 @@ void orc_io_substep(const orc_grid *g, orc_io *s, const orc_io_params *p, int NS
     }
     orc_poisson_fft99(g, s->rhsllphi, s->dph, 0);
     orc_bc_wall_neumann(g, s->dph);
-    /* PRCALC_io, explicit branch; INTFC_VARS1 + BC_WALL_PR_io */
+    /* PRCALC_io; INTFC_VARS1 + BC_WALL_PR_io */
+    const double BE = 0.5 * talp(NS) * DT * g->cvisc;
     for (int KC = 1; KC <= WN; ++KC)
         for (int JC = 1; JC <= WM; ++JC)
-            for (int IC = 1; IC <= WL; ++IC) W3(s->pr, IC, JC, KC) = W3(s->pr, IC, JC, KC) + W3(s->dph, IC, JC, KC);
+            for (int IC = 1; IC <= WL; ++IC) {
+                if (p->visimplicit)
+                    W3(s->pr, IC, JC, KC) =
+                        W3(s->pr, IC, JC, KC) + W3(s->dph, IC, JC, KC) - BE * WF3(s->rhsllphi, IC, JC, KC);
+                else
+                    W3(s->pr, IC, JC, KC) = W3(s->pr, IC, JC, KC) + W3(s->dph, IC, JC, KC);
+            }
     io_wrap(g, s->pr);
     orc_bc_wall_neumann(g, s->pr);
     /* MASSFLUX_UPDATE_IO */
@@ -1825,6 +2207,8 @@ void orc_divgck_io(const orc_grid *g, const orc_io *s, double *out)
     out[0] = DG;
     out[1] = DQ;
 }
+#undef WAMV
+#undef WAPV
 #undef WL
 #undef WM
 #undef WN

@@ -316,24 +316,27 @@ def thermal_case(og: "OracleGrid", scaled, seed=0, qw=0.1):
 
 class _IO(C.Structure):
     _fields_ = [(n, _dp) for n in ("q", "g", "g0", "pr", "rho", "rho0", "mu", "mu0", "drhodtp", "divu", "rho_stg",
-                                   "mu_stg", "explt0", "dph", "hx", "hy", "hz", "rhsllphi")] + [("dpgrns", C.c_double)]
+                                   "mu_stg", "explt0", "dph", "hx", "hy", "hz", "rhsllphi")] + \
+               [("dpgrns", C.c_double), ("drhoi_stg", _dp), ("im", _dp), ("ad", _dp)]
 
 
 class _IOParams(C.Structure):
     _fields_ = [("thermal", C.c_int), ("nflow", C.c_int), ("flowdrvtp", C.c_int), ("gravdir", C.c_int),
-                ("cfgv", C.c_double), ("f_a", C.c_double)]
+                ("cfgv", C.c_double), ("f_a", C.c_double), ("visimplicit", C.c_int)]
 
 
-IO_FIELDS = ("q", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0")
+IO_FIELDS = ("q", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0", "drhoi_stg")
 
 
 class OracleIO:
     """The io momentum substep (SOLVERRK3_MOM_IO, explicit viscous terms) and
     the property staging before it, on the numpy arrays of st (Fortran order,
     updated in place).  st holds g g0 rho rho0 mu mu0 (shared with the energy
-    substep's dict) and q pr drhodtp divu rho_stg mu_stg explt0."""
+    substep's dict) and q pr drhodtp divu rho_stg mu_stg explt0 drhoi_stg.
+    visimplicit = 1 selects the RK-CN viscous branch (MOMFA_io)."""
 
-    def __init__(self, og: "OracleGrid", st: dict, thermal=1, nflow=1, flowdrvtp=1, gravdir=0, cfgv=0.0, f_a=0.0):
+    def __init__(self, og: "OracleGrid", st: dict, thermal=1, nflow=1, flowdrvtp=1, gravdir=0, cfgv=0.0, f_a=0.0,
+                 visimplicit=0):
         L, M, N = og.L, og.M, og.N
         for k in ("g", "g0", "rho", "rho0", "mu", "mu0") + IO_FIELDS:
             a = st[k]
@@ -342,10 +345,13 @@ class OracleIO:
         self.work = {"dph": np.zeros((L, M + 2, N), order="F")}
         for k in ("hx", "hy", "hz", "rhsllphi"):
             self.work[k] = np.zeros((L, M, N), order="F")
-        self.p = _IOParams(thermal, nflow, flowdrvtp, gravdir, cfgv, f_a)
+        for k in ("im", "ad"):
+            self.work[k] = np.zeros((L, M, N, 3), order="F")
+        self.p = _IOParams(thermal, nflow, flowdrvtp, gravdir, cfgv, f_a, visimplicit)
         self.s = _IO(*[_ptr(st[k]) for k in ("q", "g", "g0", "pr", "rho", "rho0", "mu", "mu0", "drhodtp", "divu",
                                                "rho_stg", "mu_stg", "explt0")],
-                     *[_ptr(self.work[k]) for k in ("dph", "hx", "hy", "hz", "rhsllphi")], 0.0)
+                     *[_ptr(self.work[k]) for k in ("dph", "hx", "hy", "hz", "rhsllphi")], 0.0,
+                     _ptr(st["drhoi_stg"]), _ptr(self.work["im"]), _ptr(self.work["ad"]))
 
     @property
     def dpgrns(self):
@@ -388,6 +394,7 @@ def io_case(og: "OracleGrid", scaled, seed=0, qw=0.1):
     st["rho_stg"] = f(np.ones((L, M + 2, N, 3)))
     st["mu_stg"] = f(np.ones((L, M + 2, N, 3)))
     st["explt0"] = f(1e-2 * rng.standard_normal((L, M, N, 3)))
+    st["drhoi_stg"] = f(np.zeros((L, M + 2, N, 3)))
     return st, tab, n
 
 
