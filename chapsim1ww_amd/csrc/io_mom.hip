@@ -37,12 +37,16 @@
 // EXPLT0 (NCL1, NCL2, NCL3, 3).
 #include <algorithm>
 #include <cmath>
+#include <string>
+#include <vector>
 
 #include "chs_common.hpp"
+#include "tdma_dev.hpp"
 
 namespace chs {
 
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
+bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small);
 
 struct IoGeo {
     int L, M, N;
@@ -86,8 +90,8 @@ __global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __res
                                                     const double* __restrict__ V0, const double* __restrict__ G,
                                                     const double* __restrict__ G0, double* __restrict__ drhodtp,
                                                     double* __restrict__ rs, double* __restrict__ ms,
-                                                    double* __restrict__ q, int thermal, double ca, double cb,
-                                                    double cd) {
+                                                    double* __restrict__ q, double* __restrict__ drhoi, int thermal,
+                                                    double ca, double cb, double cd) {
 #pragma clang fp contract(off)
     IO_CELL(g.M + 2, 0);
     const int J = JT;   // ghost-aware row 0..M+1
@@ -137,6 +141,19 @@ __global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __res
                            0.5 * 0.5;
         r2 = wall ? r2w : r2i;
         const double m1 = wall ? m1w : m1i, m3 = wall ? m1w : m3i;
+        if (drhoi) {
+            // DENSITY_implicit_add (src/MASSFLUX_CALC_IO.f90:316-400): x and z
+            // faces on rows 1..M with the wrapped ghosts, y faces on rows
+            // 2..M, row 0 <- M, zero on the wall faces
+            const int Jr2 = J == 0 ? M : J;
+            const double d1 = 1.0 / r1 - 1.0 / ((D0[at(IC, Jw, KC)] + D0[at(IM, Jw, KC)]) * 0.5);
+            const double d3 = 1.0 / r3 - 1.0 / ((D0[at(IC, Jw, KC)] + D0[at(IC, Jw, KM)]) * 0.5);
+            const double d2 = 1.0 / r2i -
+                              1.0 / (g.wff[Jr2 - 1] * D0[at(IC, Jr2, KC)] + g.wfb[Jr2 - 1] * D0[at(IC, Jr2 - 1, KC)]);
+            drhoi[c] = d1;
+            drhoi[CS + c] = wall ? 0.0 : d2;
+            drhoi[2 * CS + c] = d3;
+        }
         if (interior) drhodtp[c] = dr;
         rs[c] = r1;
         rs[CS + c] = r2;
@@ -187,6 +204,63 @@ __device__ __forceinline__ double io_block_sum(double v, double* red) {
     return red[0];
 }
 
+// CONVECTION_X/Y/Z_io at one cell (c = (IC, JC, KC)): hx, hy, hz.  hy is
+// evaluated on row 1 too (from in-bounds rows); the callers discard it there.
+__device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict__ G, const double* __restrict__ Q,
+                                        int IC, int IP, int IM, int JC, int KC, int KP, int KM, double& hx,
+                                        double& hy, double& hz) {
+#pragma clang fp contract(off)
+    const int L = g.L, M = g.M, N = g.N;
+    const int LP = L * (M + 2), CS = LP * N;
+    auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
+    const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1;
+    const double wff = g.wff[JJ - 1], wfb = g.wfb[JJ - 1], wffp = g.wff[JJP - 1], wfbp = g.wfb[JJP - 1];
+    const double *G1 = G, *G2 = G + CS, *G3 = G + 2 * CS;
+    const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
+    const int c = at(IC, JC, KC);
+    {
+        const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5 * 1.0;
+        const double COE32 = g.dzi * 0.5 * 0.5 * 1.0;
+        const double H11F = (G1[at(IP, JC, KC)] + G1[c]) * (Q1[at(IP, JC, KC)] + Q1[c]);
+        const double H11B = (G1[c] + G1[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IM, JC, KC)]);
+        const double H11 = (H11F - H11B) * COE1;
+        const double H12F = (G2[at(IC, JP, KC)] + G2[at(IM, JP, KC)]) * (wffp * Q1[at(IC, JP, KC)] + wfbp * Q1[c]);
+        const double H12B = (G2[c] + G2[at(IM, JC, KC)]) * (wff * Q1[c] + wfb * Q1[at(IC, JM, KC)]);
+        const double H12 = (H12F - H12B) * COE2;
+        const double H13F = (G3[at(IC, JC, KP)] + G3[at(IM, JC, KP)]) * (Q1[at(IC, JC, KP)] + Q1[c]);
+        const double H13B = (G3[c] + G3[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IC, JC, KM)]);
+        const double H13 = (H13F - H13B) * COE32;
+        hx = -(H11 + H12 + H13);
+    }
+    {
+        const double COE1 = g.dxi * 0.5, COE2 = g.dyci[JJ - 1] * 0.5 * 0.5, COE31 = g.dzi * 0.5 * 1.0;
+        const double H21F = (wff * G1[at(IP, JC, KC)] + wfb * G1[at(IP, JM, KC)]) * (Q2[at(IP, JC, KC)] + Q2[c]);
+        const double H21B = (wff * G1[c] + wfb * G1[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IM, JC, KC)]);
+        const double H21 = (H21F - H21B) * COE1;
+        const double H22F = (G2[at(IC, JP, KC)] + G2[c]) * (Q2[at(IC, JP, KC)] + Q2[c]);
+        const double H22B = (G2[c] + G2[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JM, KC)]);
+        const double H22 = (H22F - H22B) * COE2;
+        const double H23F = (wff * G3[at(IC, JC, KP)] + wfb * G3[at(IC, JM, KP)]) * (Q2[at(IC, JC, KP)] + Q2[c]);
+        const double H23B = (wff * G3[c] + wfb * G3[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JC, KM)]);
+        const double H23 = (H23F - H23B) * COE31;
+        hy = -(H21 + H22 + H23);
+    }
+    {
+        const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5;
+        const double COE31 = g.dzi * 0.5 * 0.5 * 1.0;
+        const double H31F = (G1[at(IP, JC, KC)] + G1[at(IP, JC, KM)]) * (Q3[at(IP, JC, KC)] + Q3[c]);
+        const double H31B = (G1[c] + G1[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IM, JC, KC)]);
+        const double H31 = (H31F - H31B) * COE1;
+        const double H32F = (G2[at(IC, JP, KC)] + G2[at(IC, JP, KM)]) * (wffp * Q3[at(IC, JP, KC)] + wfbp * Q3[c]);
+        const double H32B = (G2[c] + G2[at(IC, JC, KM)]) * (wff * Q3[c] + wfb * Q3[at(IC, JM, KC)]);
+        const double H32 = (H32F - H32B) * COE2;
+        const double H33F = (G3[at(IC, JC, KP)] + G3[c]) * (Q3[at(IC, JC, KP)] + Q3[c]);
+        const double H33B = (G3[c] + G3[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IC, JC, KM)]);
+        const double H33 = (H33F - H33B) * COE31;
+        hz = -(H31 + H32 + H33);
+    }
+}
+
 // Explicit terms and RHS_MOM_EXPLICIT_io up to DPGRNS, rows 1..M.
 // inc[c] <- RHS of each component (row 1 of y untouched), explt0 <- the
 // explicit terms, part[block] <- sum of RHS/DYFI of the NFLOW component.
@@ -224,22 +298,10 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         const int c = at(IC, JC, KC);
         const int e = IC + L * (JC - 1) + L * M * KC;   // (L, M, N) index
         const double E1 = E[e], E2 = E[NC + e], E3 = E[2 * NC + e];
-        // ---- x: CONVECTION_X_io + VISCOUS_ALL_EXPLT_X_io ----
+        // ---- CONVECTION_X/Y/Z_io, then VISCOUS_ALL_EXPLT_X_io ----
         double hx;
-        {
-            const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5 * 1.0;
-            const double COE32 = g.dzi * 0.5 * 0.5 * 1.0;
-            const double H11F = (G1[at(IP, JC, KC)] + G1[c]) * (Q1[at(IP, JC, KC)] + Q1[c]);
-            const double H11B = (G1[c] + G1[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IM, JC, KC)]);
-            const double H11 = (H11F - H11B) * COE1;
-            const double H12F = (G2[at(IC, JP, KC)] + G2[at(IM, JP, KC)]) * (wffp * Q1[at(IC, JP, KC)] + wfbp * Q1[c]);
-            const double H12B = (G2[c] + G2[at(IM, JC, KC)]) * (wff * Q1[c] + wfb * Q1[at(IC, JM, KC)]);
-            const double H12 = (H12F - H12B) * COE2;
-            const double H13F = (G3[at(IC, JC, KP)] + G3[at(IM, JC, KP)]) * (Q1[at(IC, JC, KP)] + Q1[c]);
-            const double H13B = (G3[c] + G3[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IC, JC, KM)]);
-            const double H13 = (H13F - H13B) * COE32;
-            hx = -(H11 + H12 + H13);
-        }
+        double hy, hz;
+        io_conv(g, G, Q, IC, IP, IM, JC, KC, KP, KM, hx, hy, hz);
         {
             const double COE1 = 2.0 * g.dxi * CV, COE2 = g.dyfi[JJ - 1] * CV * 1.0, COE32 = g.dzi * CV * 1.0;
             double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
@@ -264,22 +326,8 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
             const double DTAU13DZ = (TAU13F - TAU13B) * COE32;
             hx = hx + DTAU11DX + DTAU12DY + DTAU13DZ;
         }
-        // ---- y (rows 2..M): CONVECTION_Y_io + VISCOUS_ALL_EXPLT_Y_io ----
-        double hy;
+        // ---- y (rows 2..M): VISCOUS_ALL_EXPLT_Y_io ----
         {
-            {
-                const double COE1 = g.dxi * 0.5, COE2 = g.dyci[JJ - 1] * 0.5 * 0.5, COE31 = g.dzi * 0.5 * 1.0;
-                const double H21F = (wff * G1[at(IP, JC, KC)] + wfb * G1[at(IP, JM, KC)]) * (Q2[at(IP, JC, KC)] + Q2[c]);
-                const double H21B = (wff * G1[c] + wfb * G1[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IM, JC, KC)]);
-                const double H21 = (H21F - H21B) * COE1;
-                const double H22F = (G2[at(IC, JP, KC)] + G2[c]) * (Q2[at(IC, JP, KC)] + Q2[c]);
-                const double H22B = (G2[c] + G2[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JM, KC)]);
-                const double H22 = (H22F - H22B) * COE2;
-                const double H23F = (wff * G3[at(IC, JC, KP)] + wfb * G3[at(IC, JM, KP)]) * (Q2[at(IC, JC, KP)] + Q2[c]);
-                const double H23B = (wff * G3[c] + wfb * G3[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JC, KM)]);
-                const double H23 = (H23F - H23B) * COE31;
-                hy = -(H21 + H22 + H23);
-            }
             {
                 const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = 2.0 * g.dyci[JJ - 1] * CV;
                 const int cjm = at(IC, JM, KC);
@@ -305,22 +353,7 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
                 hy = hy + DTAU21DX + DTAU22DY + DTAU23DZ;
             }
         }
-        // ---- z: CONVECTION_Z_io + VISCOUS_ALL_EXPLT_Z_io ----
-        double hz;
-        {
-            const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5;
-            const double COE31 = g.dzi * 0.5 * 0.5 * 1.0;
-            const double H31F = (G1[at(IP, JC, KC)] + G1[at(IP, JC, KM)]) * (Q3[at(IP, JC, KC)] + Q3[c]);
-            const double H31B = (G1[c] + G1[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IM, JC, KC)]);
-            const double H31 = (H31F - H31B) * COE1;
-            const double H32F = (G2[at(IC, JP, KC)] + G2[at(IC, JP, KM)]) * (wffp * Q3[at(IC, JP, KC)] + wfbp * Q3[c]);
-            const double H32B = (G2[c] + G2[at(IC, JC, KM)]) * (wff * Q3[c] + wfb * Q3[at(IC, JM, KC)]);
-            const double H32 = (H32F - H32B) * COE2;
-            const double H33F = (G3[at(IC, JC, KP)] + G3[c]) * (Q3[at(IC, JC, KP)] + Q3[c]);
-            const double H33B = (G3[c] + G3[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IC, JC, KM)]);
-            const double H33 = (H33F - H33B) * COE31;
-            hz = -(H31 + H32 + H33);
-        }
+        // ---- z: VISCOUS_ALL_EXPLT_Z_io ----
         {
             const double COE1 = g.dxi * CV, COE3 = 2.0 * g.dzi * CV, COE2 = g.dyfi[JJ - 1] * CV;
             const int ckm = at(IC, JC, KM);
@@ -399,6 +432,10 @@ __global__ void __launch_bounds__(256) k_io_massflux(IoGeo g, double* __restrict
         G[2 * CS + c] = 0.0;
         return;
     }
+    if (!inc) {   // the RK-CN branch updated G in its y solves: wall values only
+        if (J == 1) G[CS + c] = 0.0;
+        return;
+    }
     const int NC = L * M * N;
     const int e = IC + L * (J - 1) + L * M * KC;
     const double dp = *dpg;
@@ -428,12 +465,14 @@ __global__ void __launch_bounds__(256) k_io_divg(IoGeo g, const double* __restri
 
 // PRCALC_io (explicit) with BC_WALL_PR_io and MASSFLUX_UPDATE_IO, rows 1..M
 __global__ void __launch_bounds__(256) k_io_update(IoGeo g, double* __restrict__ G, double* __restrict__ PR,
-                                                   const double* __restrict__ DPH, double dta) {
+                                                   const double* __restrict__ DPH, double dta,
+                                                   const double* __restrict__ f, double be) {
 #pragma clang fp contract(off)
     IO_CELL(g.M, 1);
     const int JC = JT;
     const int c = at(IC, JC, KC);
-    const double pn = PR[c] + DPH[c];
+    // PRCALC_io: PR + DPH, less BE * RHSLLPHI in the RK-CN branch
+    const double pn = f ? PR[c] + DPH[c] - be * f[IC + L * (JC - 1) + L * M * KC] : PR[c] + DPH[c];
     PR[c] = pn;
     if (JC == 1) PR[at(IC, 0, KC)] = pn;
     if (JC == M) PR[at(IC, M + 1, KC)] = pn;
@@ -509,6 +548,360 @@ __global__ void __launch_bounds__(256) k_io_max2(const double* __restrict__ part
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// RK-CN branch (visthemflg = 1)
+// ---------------------------------------------------------------------------
+
+// VISCOUS_PAR_EXPLT_X/Y/Z_io (src/VISCOUS_X_EXPLICIT_io.f90:116-289,
+// src/VISCOUS_Y_EXPLICIT_io.f90:264-423, src/VISCOUS_Z_EXPLICIT_io.f90:289-460)
+// with the convection, and the RK-CN branch of RHS_MOM_EXPLICIT_io
+// (src/RHS_MOM_EXPLICIT_io.f90:79-93) up to DPGRNS.  inc <- RHS (row 1 of v
+// zero), explt0 <- convection + explicit stress part, divu <- the z density
+// correction (what the reference leaves in DIVU_io), part <- block sums.
+template <int GRAV>
+__global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __restrict__ G,
+                                                     const double* __restrict__ Q, const double* __restrict__ MU,
+                                                     const double* __restrict__ MS, const double* __restrict__ DR,
+                                                     const double* __restrict__ PR, const double* __restrict__ RS,
+                                                     double* __restrict__ E, double* __restrict__ DUo, IoParams p,
+                                                     double tg, double tr, double dt, double adt,
+                                                     double* __restrict__ inc, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    const int L = g.L, M = g.M, N = g.N;
+    int it, jt, kt, tile;
+    if (!row_tile(L, N, M, &it, &jt, &kt, &tile)) return;
+    const int IC = it * 64 + threadIdx.x;
+    const int KC = kt * 4 + threadIdx.y;
+    const int JC = jt + 1;
+    const bool live = IC < L && KC < N;
+    double flow = 0.0;
+    if (live) {
+        const int IP = IC == L - 1 ? 0 : IC + 1, IM = IC == 0 ? L - 1 : IC - 1;
+        const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;
+        const int LP = L * (M + 2), CS = LP * N, NC = L * M * N;
+        auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
+        // row 1 of v is evaluated with a clamped metric index and discarded
+        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC >= 2 ? JC - 1 : 1;
+        const double CV = g.cvisc;
+        const double *G1 = G, *G2 = G + CS, *G3 = G + 2 * CS;
+        const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
+        const double *MS1 = MS, *MS2 = MS + CS, *MS3 = MS + 2 * CS;
+        const double *D1 = DR, *D2 = DR + CS, *D3 = DR + 2 * CS;
+        const int c = at(IC, JC, KC);
+        const int e = IC + L * (JC - 1) + L * M * KC;
+        const double E1 = E[e], E2 = E[NC + e], E3 = E[2 * NC + e];
+        double hx, hy, hz;
+        io_conv(g, G, Q, IC, IP, IM, JC, KC, KP, KM, hx, hy, hz);
+        double IM1, AD1, IM2, AD2, IM3, AD3;
+        {   // x
+            const double COE1 = -2.0 / 3.0 * g.dxi * CV, COE2 = g.dyfi[JJ - 1] * CV * 1.0, COE32 = g.dzi * CV * 1.0;
+            const int cim = at(IM, JC, KC);
+            double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
+            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1] * 1.0;
+            double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi * 1.0;
+            const double F_EX = MU[c] * (DVDY + DWDZ), F_IM = MU[c] * DUDX;
+            DUDX = (G1[at(IP, JC, KC)] * D1[at(IP, JC, KC)] - G1[c] * D1[c]) * g.dxi;
+            const double F_AD = MU[c] * DUDX;
+            DUDX = (Q1[c] - Q1[cim]) * g.dxi;
+            DVDY = (Q2[at(IM, JP, KC)] - Q2[cim]) * g.dyfi[JJ - 1] * 1.0;
+            DWDZ = (Q3[at(IM, JC, KP)] - Q3[cim]) * g.dzi * 1.0;
+            const double B_EX = MU[cim] * (DVDY + DWDZ), B_IM = MU[cim] * DUDX;
+            DUDX = (G1[c] * D1[c] - G1[cim] * D1[cim]) * g.dxi;
+            const double B_AD = MU[cim] * DUDX;
+            const double X_EX = (F_EX - B_EX) * COE1, X_IM = (F_IM - B_IM) * COE1 * (-2.0);
+            const double X_AD = (F_AD - B_AD) * COE1 * (-2.0);
+            double VP = MS1[at(IC, JP, KC)];
+            double DVDX = (Q2[at(IC, JP, KC)] - Q2[at(IM, JP, KC)]) * g.dxi;
+            double DUDY = (Q1[at(IC, JP, KC)] - Q1[c]) * g.dyci[JJP - 1] / 1.0;
+            const double F2_EX = VP * DVDX, F2_IM = VP * DUDY;
+            DUDY = (G1[at(IC, JP, KC)] * D1[at(IC, JP, KC)] - G1[c] * D1[c]) * g.dyci[JJP - 1] / 1.0;
+            const double F2_AD = VP * DUDY;
+            double VC = MS1[c];
+            DUDY = (Q1[c] - Q1[at(IC, JM, KC)]) * g.dyci[JJ - 1] / 1.0;
+            DVDX = (Q2[c] - Q2[cim]) * g.dxi;
+            const double B2_EX = VC * DVDX, B2_IM = VC * DUDY;
+            DUDY = (G1[c] * D1[c] - G1[at(IC, JM, KC)] * D1[at(IC, JM, KC)]) * g.dyci[JJ - 1] / 1.0;
+            const double B2_AD = VC * DUDY;
+            const double Y_EX = (F2_EX - B2_EX) * COE2, Y_IM = (F2_IM - B2_IM) * COE2, Y_AD = (F2_AD - B2_AD) * COE2;
+            VP = MS2[at(IC, JC, KP)];
+            double DUDZ = (Q1[at(IC, JC, KP)] - Q1[c]) * g.dzi;
+            double DWDX = (Q3[at(IC, JC, KP)] - Q3[at(IM, JC, KP)]) * g.dxi;
+            const double F3_EX = VP * DWDX, F3_IM = VP * DUDZ;
+            DUDZ = (Q1[at(IC, JC, KP)] * D1[at(IC, JC, KP)] - Q1[c] * D1[c]) * g.dzi;
+            const double F3_AD = VP * DUDZ;
+            VC = MS2[c];
+            DUDZ = (Q1[c] - Q1[at(IC, JC, KM)]) * g.dzi;
+            DWDX = (Q3[c] - Q3[cim]) * g.dxi;
+            const double B3_EX = VC * DWDX, B3_IM = VC * DUDZ;
+            DUDZ = (Q1[c] * D1[c] - Q1[at(IC, JC, KM)] * D1[at(IC, JC, KM)]) * g.dzi;
+            const double B3_AD = VC * DUDZ;
+            const double Z_EX = (F3_EX - B3_EX) * COE32, Z_IM = (F3_IM - B3_IM) * COE32, Z_AD = (F3_AD - B3_AD) * COE32;
+            hx = X_EX + Y_EX + Z_EX + hx;
+            IM1 = X_IM + Y_IM + Z_IM;
+            AD1 = X_AD + Y_AD + Z_AD;
+        }
+        {   // y (rows 2..M)
+            const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = -2.0 / 3.0 * g.dyci[JJ - 1] * CV;
+            const int cjm = at(IC, JM, KC);
+            double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
+            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1];
+            double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi;
+            const double F_EX = MU[c] * (DUDX + DWDZ), F_IM = MU[c] * DVDY;
+            DVDY = (G2[at(IC, JP, KC)] * D2[at(IC, JP, KC)] - G2[c] * D2[c]) * g.dyfi[JJ - 1];
+            const double F_AD = MU[c] * DVDY;
+            DUDX = (Q1[at(IP, JM, KC)] - Q1[cjm]) * g.dxi;
+            DVDY = (Q2[c] - Q2[cjm]) * g.dyfi[JJM - 1];
+            DWDZ = (Q3[at(IC, JM, KP)] - Q3[cjm]) * g.dzi;
+            const double B_EX = MU[cjm] * (DUDX + DWDZ), B_IM = MU[cjm] * DVDY;
+            DVDY = (G2[c] * D2[c] - G2[cjm] * D2[cjm]) * g.dyfi[JJM - 1];
+            const double B_AD = MU[cjm] * DVDY;
+            const double Y_EX = (F_EX - B_EX) * COE2, Y_IM = (F_IM - B_IM) * COE2 * (-2.0);
+            const double Y_AD = (F_AD - B_AD) * COE2 * (-2.0);
+            double VP = MS1[at(IP, JC, KC)];
+            double DVDX = (Q2[at(IP, JC, KC)] - Q2[c]) * g.dxi;
+            double DUDY = (Q1[at(IP, JC, KC)] - Q1[at(IP, JM, KC)]) * g.dyci[JJ - 1];
+            const double F1_EX = VP * DUDY, F1_IM = VP * DVDX;
+            DVDX = (G2[at(IP, JC, KC)] * D2[at(IP, JC, KC)] - G2[c] * D2[c]) * g.dxi;
+            const double F1_AD = VP * DVDX;
+            double VC = MS1[c];
+            DVDX = (Q2[c] - Q2[at(IM, JC, KC)]) * g.dxi;
+            DUDY = (Q1[c] - Q1[cjm]) * g.dyci[JJ - 1];
+            const double B1_EX = VC * DUDY, B1_IM = VC * DVDX;
+            DVDX = (G2[c] * D2[c] - G2[at(IM, JC, KC)] * D2[at(IM, JC, KC)]) * g.dxi;
+            const double B1_AD = VC * DVDX;
+            const double X_EX = (F1_EX - B1_EX) * COE1, X_IM = (F1_IM - B1_IM) * COE1, X_AD = (F1_AD - B1_AD) * COE1;
+            VP = MS3[at(IC, JC, KP)];
+            double DVDZ = (Q2[at(IC, JC, KP)] - Q2[c]) * g.dzi;
+            double DWDY = (Q3[at(IC, JC, KP)] - Q3[at(IC, JM, KP)]) * g.dyci[JJ - 1];
+            const double F3_EX = VP * DWDY, F3_IM = VP * DVDZ;
+            DVDZ = (G2[at(IC, JC, KP)] * D2[at(IC, JC, KP)] - G2[c] * D2[c]) * g.dzi;
+            const double F3_AD = VP * DVDZ;
+            VC = MS3[c];
+            DVDZ = (Q2[c] - Q2[at(IC, JC, KM)]) * g.dzi;
+            DWDY = (Q3[c] - Q3[cjm]) * g.dyci[JJ - 1];
+            const double B3_EX = VC * DWDY, B3_IM = VC * DVDZ;
+            DVDZ = (G2[c] * D2[c] - G2[at(IC, JC, KM)] * D2[at(IC, JC, KM)]) * g.dzi;
+            const double B3_AD = VC * DVDZ;
+            const double Z_EX = (F3_EX - B3_EX) * COE3, Z_IM = (F3_IM - B3_IM) * COE3, Z_AD = (F3_AD - B3_AD) * COE3;
+            hy = X_EX + Y_EX + Z_EX + hy;
+            IM2 = X_IM + Y_IM + Z_IM;
+            AD2 = X_AD + Y_AD + Z_AD;
+        }
+        {   // z
+            const double COE1 = g.dxi * CV, COE3 = -2.0 / 3.0 * g.dzi * CV, COE2 = g.dyfi[JJ - 1] * CV;
+            const int ckm = at(IC, JC, KM);
+            double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
+            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1];
+            double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi;
+            const double F_EX = MU[c] * (DUDX + DVDY), F_IM = MU[c] * DWDZ;
+            DWDZ = (G3[at(IC, JC, KP)] * D3[at(IC, JC, KP)] - G3[c] * D3[c]) * g.dzi;
+            const double F_AD = MU[c] * DWDZ;
+            DUDX = (Q1[at(IP, JC, KM)] - Q1[ckm]) * g.dxi;
+            DVDY = (Q2[at(IC, JP, KM)] - Q2[ckm]) * g.dyfi[JJ - 1];
+            DWDZ = (Q3[c] - Q3[ckm]) * g.dzi;
+            const double B_EX = MU[ckm] * (DUDX + DVDY), B_IM = MU[ckm] * DWDZ;
+            DWDZ = (G3[c] * D3[c] - G3[ckm] * D3[ckm]) * g.dzi;
+            const double B_AD = MU[ckm] * DWDZ;
+            const double Z_EX = (F_EX - B_EX) * COE3, Z_IM = (F_IM - B_IM) * COE3 * (-2.0);
+            const double Z_AD = (F_AD - B_AD) * COE3 * (-2.0);
+            double VP = MS2[at(IP, JC, KC)];
+            double DWDX = (Q3[at(IP, JC, KC)] - Q3[c]) * g.dxi;
+            double DUDZ = (Q1[at(IP, JC, KC)] - Q1[at(IP, JC, KM)]) * g.dzi;
+            const double F1_EX = VP * DUDZ, F1_IM = VP * DWDX;
+            DWDX = (G3[at(IP, JC, KC)] * D3[at(IP, JC, KC)] - G3[c] * D3[c]) * g.dxi;
+            const double F1_AD = VP * DWDX;
+            double VC = MS2[c];
+            DWDX = (Q3[c] - Q3[at(IM, JC, KC)]) * g.dxi;
+            DUDZ = (Q1[c] - Q1[ckm]) * g.dzi;
+            const double B1_EX = VC * DUDZ, B1_IM = VC * DWDX;
+            DWDX = (G3[c] * D3[c] - G3[at(IM, JC, KC)] * D3[at(IM, JC, KC)]) * g.dxi;
+            const double B1_AD = VC * DWDX;
+            const double X_EX = (F1_EX - B1_EX) * COE1, X_IM = (F1_IM - B1_IM) * COE1, X_AD = (F1_AD - B1_AD) * COE1;
+            VP = MS3[at(IC, JP, KC)];
+            double DWDY = (Q3[at(IC, JP, KC)] - Q3[c]) * g.dyci[JJP - 1];
+            double DVDZ = (Q2[at(IC, JP, KC)] - Q2[at(IC, JP, KM)]) * g.dzi;
+            const double F2_EX = VP * DVDZ, F2_IM = VP * DWDY;
+            DWDY = (G3[at(IC, JP, KC)] * D3[at(IC, JP, KC)] - G3[c] * D3[c]) * g.dyci[JJP - 1];
+            const double F2_AD = VP * DWDY;
+            VC = MS3[c];
+            DWDY = (Q3[c] - Q3[at(IC, JM, KC)]) * g.dyci[JJ - 1];
+            DVDZ = (Q2[c] - Q2[ckm]) * g.dzi;
+            const double B2_EX = VC * DVDZ, B2_IM = VC * DWDY;
+            DWDY = (G3[c] * D3[c] - G3[at(IC, JM, KC)] * D3[at(IC, JM, KC)]) * g.dyci[JJ - 1];
+            const double B2_AD = VC * DWDY;
+            const double Y_EX = (F2_EX - B2_EX) * COE2, Y_IM = (F2_IM - B2_IM) * COE2, Y_AD = (F2_AD - B2_AD) * COE2;
+            hz = X_EX + Y_EX + Z_EX + hz;
+            IM3 = X_IM + Y_IM + Z_IM;
+            AD3 = X_AD + Y_AD + Z_AD;
+        }
+        double r1 = (tg * hx + tr * E1) * dt;
+        r1 = r1 + adt * IM1 + 0.5 * adt * AD1;
+        r1 = r1 - (PR[c] - PR[at(IM, JC, KC)]) * (adt * g.dxi);
+        double r2 = (tg * hy + tr * E2) * dt;
+        r2 = r2 + adt * IM2 + 0.5 * adt * AD2;
+        r2 = r2 - (PR[c] - PR[at(IC, JM, KC)]) * (adt * g.dyci[JC - 1] / 1.0);
+        double r3 = (tg * hz + tr * E3) * dt;
+        r3 = r3 + adt * IM3 + 0.5 * adt * AD3;
+        r3 = r3 - (PR[c] - PR[at(IC, JC, KM)]) * (adt * g.dzi);
+        if (GRAV == 1) r1 = r1 + RS[c] * (adt * p.f_a / 1.0);
+        if (GRAV == 2) r2 = r2 + RS[CS + c] * (adt * p.f_a / 1.0);
+        if (GRAV == 3) r3 = r3 + RS[2 * CS + c] * (adt * p.f_a / 1.0);
+        E[e] = hx;
+        E[2 * NC + e] = hz;
+        inc[e] = r1;
+        inc[2 * NC + e] = r3;
+        inc[NC + e] = JC >= 2 ? r2 : 0.0;
+        if (JC >= 2) E[NC + e] = hy;
+        DUo[c] = AD3;
+        const double rf = p.nflow == 1 ? r1 : (p.nflow == 2 ? (JC >= 2 ? r2 : 0.0) : r3);
+        flow = rf / g.dyfi[JC - 1] / 1.0;
+    }
+    const double bs = io_block_sum<256>(flow, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = bs;
+}
+
+struct IoCnCoef {
+    double c1, c2;             // FACOE * 4/3 * (DXQI | DZQI | 1), FACOE * (DXQI | DZQI | 1)
+    const double *amvr, *apvr; // AMVR, APVR (NND2, 3)
+};
+
+// MOMFA1_X_io / MOMFA2_Z_io / MOMFA3_Y_io (src/MOMFA_io.f90:51-377) for one
+// component IDR: the coefficients of each line are formed on load from the
+// properties, then the segmented solver; DIR 0 = x (cyclic, lines (j, k)),
+// 2 = z (cyclic, lines (i, j)), 1 = y (non-cyclic, lines (i, k), zero
+// boundary values).  The x pass subtracts DPGRNS from the NFLOW component
+// (rows it applies to); the y pass adds the result to G from row N2I.
+template <int MS, int DIR>
+__global__ void __launch_bounds__(1024) k_io_cn(IoGeo g, int idr, int sub_dpg, const double* __restrict__ MU,
+                                               const double* __restrict__ MSTG, const double* __restrict__ RS,
+                                               double* __restrict__ R, double* __restrict__ Gc,
+                                               const double* __restrict__ dpg, IoCnCoef cf, int LPB, int P) {
+#pragma clang fp contract(off)
+    extern __shared__ unsigned char smem[];
+    const int L = g.L, M = g.M, N = g.N;
+    const int LP = L * (M + 2), CS = LP * N;
+    const int n = DIR == 0 ? L : (DIR == 2 ? N : M);
+    const int nlines = DIR == 0 ? M * N : (DIR == 2 ? L * M : L * N);
+    const int ll = threadIdx.x % LPB, seg = threadIdx.x / LPB;
+    const int line = blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    int li = 0, lj = 1, lk = 0;   // the line's fixed indices (j is the ghost-aware row)
+    if (valid) {
+        if (DIR == 0) { lj = line % M + 1; lk = line / M; }
+        else if (DIR == 2) { li = line % L; lj = line / L + 1; }
+        else { li = line % L; lk = line / L; }
+    }
+    const double* RSc = RS + (size_t)(idr - 1) * CS;
+    const double dp = sub_dpg ? *dpg : 0.0;
+    // wall-row coefficients of u, w in y that TDMA_COEF sets aside
+    // (src/TDMA_COEF.f90:86-96)
+    const double amvr1 = 4.0 / (2.0 / g.dyci[1] + 1.0 / g.dyfi[0]) * (2.0 * g.dyfi[0]);
+    const double apvrn = 4.0 / (2.0 / g.dyci[M - 1] + 1.0 / g.dyfi[M - 1]) * (2.0 * g.dyfi[M - 1]);
+    double a[MS], b[MS], c[MS], d[MS];
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        const int m = seg * MS + l;
+        a[l] = 0.0; b[l] = 1.0; c[l] = 0.0; d[l] = 0.0;
+        if (!valid || m >= n) continue;
+        const int IC = DIR == 0 ? m : li, JC = DIR == 1 ? m + 1 : lj, KC = DIR == 2 ? m : lk;
+        const int cc = IC + L * JC + LP * KC;
+        const int e = IC + L * (JC - 1) + L * M * KC;
+        double AMU, APU, am, ap, ac;
+        if (DIR == 0) {
+            const int IM = IC == 0 ? L - 1 : IC - 1, IP = IC == L - 1 ? 0 : IC + 1;
+            const int cm = cc - IC + IM, cp = cc - IC + IP;
+            if (idr == 1) {
+                AMU = -cf.c1 * MU[cm];
+                APU = -cf.c1 * MU[cc];
+            } else {
+                const double* X = MSTG + (size_t)(idr - 2) * CS;   // MU_STG(., 1) for v, (., 2) for w
+                AMU = -cf.c2 * X[cc];
+                APU = -cf.c2 * X[cp];
+            }
+            am = AMU / RSc[cm];
+            ap = APU / RSc[cp];
+            ac = 1.0 - (AMU + APU) / RSc[cc];
+        } else if (DIR == 2) {
+            const int KM = KC == 0 ? N - 1 : KC - 1, KP = KC == N - 1 ? 0 : KC + 1;
+            const int cm = cc + LP * (KM - KC), cp = cc + LP * (KP - KC);
+            if (idr == 3) {
+                AMU = -cf.c1 * MU[cm] * 1.0;
+                APU = -cf.c1 * MU[cc] * 1.0;
+            } else {
+                const double* X = MSTG + (size_t)(idr == 1 ? 1 : 2) * CS;   // MU_STG(., 2) for u, (., 3) for v
+                AMU = -cf.c2 * X[cc] * 1.0;
+                APU = -cf.c2 * X[cp] * 1.0;
+            }
+            am = AMU / RSc[cm];
+            ap = APU / RSc[cp];
+            ac = 1.0 - (AMU + APU) / RSc[cc];
+        } else {
+            const int cm = cc - L, cp = cc + L;
+            const double amv = cf.amvr[(idr - 1) * (M + 1) + JC - 1], apv = cf.apvr[(idr - 1) * (M + 1) + JC - 1];
+            if (idr == 1 || idr == 3) {
+                const double* FM = MSTG + (size_t)(idr - 1) * CS;
+                if (JC == 1) {
+                    const double MUQT = (MU[cc] + MU[cm]) * 0.5;
+                    AMU = -cf.c2 * amvr1 * MUQT;
+                    APU = -cf.c2 * apv * FM[cp];
+                    am = 0.0;
+                    ap = APU / RSc[cp];
+                } else if (JC == M) {
+                    const double MUQT = (MU[cc] + MU[cp]) * 0.5;
+                    AMU = -cf.c2 * amv * FM[cc];
+                    APU = -cf.c2 * apvrn * MUQT;
+                    am = AMU / RSc[cm];
+                    ap = 0.0;
+                } else {
+                    AMU = -cf.c2 * amv * FM[cc];
+                    APU = -cf.c2 * apv * FM[cp];
+                    am = AMU / RSc[cm];
+                    ap = APU / RSc[cp];
+                }
+                ac = 1.0 - (AMU + APU) / RSc[cc];
+            } else {
+                AMU = -cf.c1 * amv * MU[cm];
+                APU = -cf.c1 * apv * MU[cc];
+                am = AMU / RSc[cm];
+                ap = APU / RSc[JC == 1 ? cp : cc];
+                ac = 1.0 - (AMU + APU) / RSc[cc];
+            }
+        }
+        a[l] = am;
+        b[l] = ac;
+        c[l] = ap;
+        d[l] = (DIR == 0 && (idr != 2 || JC >= 2)) ? R[e] - dp : R[e];
+    }
+    if (DIR == 1) {   // zero boundary values: the end rows lose their outer coefficient
+        if (seg == 0) a[0] = 0.0;
+        const int jl = n - 1;
+        if (jl / MS == seg) {
+#pragma unroll
+            for (int q = 0; q < MS; ++q)
+                if (q == jl % MS) c[q] = 0.0;
+        }
+    }
+    seg_solve<MS, double>(a, b, c, d, LPB, P, ll, seg, valid, DIR != 1, smem);
+    if (!valid) return;
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        const int m = seg * MS + l;
+        if (m >= n) continue;
+        const int IC = DIR == 0 ? m : li, JC = DIR == 1 ? m + 1 : lj, KC = DIR == 2 ? m : lk;
+        const int e = IC + L * (JC - 1) + L * M * KC;
+        if (DIR == 1) {
+            if (JC >= (idr == 2 ? 2 : 1)) {
+                const int cc = IC + L * JC + LP * KC;
+                Gc[cc] = d[l] + Gc[cc];
+            }
+        } else {
+            R[e] = d[l];
+        }
+    }
+}
+
 static IoGeo io_geo(const chs_handle* h) {
     return IoGeo{h->L, h->M, h->N, h->dxi, h->dzi, h->cvisc, h->volm, h->dyfi, h->dyci, h->wff, h->wfb};
 }
@@ -532,6 +925,8 @@ static int io_check(const chs_handle* h, const chs_io_params* prm, int ns, const
     CHS_CHECK(prm->nflow >= 1 && prm->nflow <= 3, CHS_E_ARG, std::string(who) + ": NFLOW must be 1..3");
     CHS_CHECK(prm->flowdrvtp == 1 || prm->flowdrvtp == 2, CHS_E_ARG, std::string(who) + ": FLOWDRVTP must be 1 or 2");
     CHS_CHECK(prm->gravdir >= 0 && prm->gravdir <= 3, CHS_E_ARG, std::string(who) + ": GRAVDIR must be 0..3");
+    CHS_CHECK(prm->visimplicit == 0 || prm->visimplicit == 1, CHS_E_ARG,
+              std::string(who) + ": visimplicit must be 0 or 1");
     return CHS_OK;
 }
 
@@ -546,10 +941,95 @@ int io_prepare(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     const hipStream_t s = h->stream;
     hipEvent_t ev;
     h->timer.begin("io_prepare", s, &ev);
+    double* drhoi = nullptr;
+    if (prm->visimplicit && prm->thermal) {
+        CHS_CHECK(f->drhoi_stg != nullptr, CHS_E_ARG, "chs_io_prepare: drhoi_stg is needed with visimplicit = 1");
+        drhoi = f->drhoi_stg;
+    }
     CHS_LAUNCH(k_io_prepare, io_grid(h, h->M + 2), dim3(64, 4), 0, s, io_geo(h), a.rho, a.rho0, a.mu, a.mu0, a.g,
-               a.g0, a.drhodtp, a.rs, a.ms, a.q, prm->thermal, TGAM_IO[ns] + TROH_IO[ns], TROH_IO[ns],
+               a.g0, a.drhodtp, a.rs, a.ms, a.q, drhoi, prm->thermal, TGAM_IO[ns] + TROH_IO[ns], TROH_IO[ns],
                TGAM_IO[ns] - 2.0);
     h->timer.end(ev, "io_prepare", s);
+    return CHS_OK;
+}
+
+// launch one factorised line solve of the RK-CN branch
+template <int DIR>
+static int io_cn_launch(chs_handle* h, const IoGeo& g, int idr, int sub_dpg, const IoArrays& a, double* R,
+                        double* Gc, const IoCnCoef& cf) {
+    const int n = DIR == 0 ? h->L : (DIR == 2 ? h->N : h->M);
+    const int nlines = DIR == 0 ? h->M * h->N : (DIR == 2 ? h->L * h->M : h->L * h->N);
+    int MS = 0, P = 0, LPB = 0;
+    CHS_CHECK(tdma_seg_config(n, DIR != 1, &MS, &P, &LPB, 1024, false), CHS_E_ARG,
+              "chs_solverrk3_mom_io: no segmented solver layout for a line length of " + std::to_string(n));
+    const dim3 grid((nlines + LPB - 1) / LPB), block(LPB * P);
+    const size_t sm = seg_smem_bytes<double>(LPB, P);
+    const hipStream_t s = h->stream;
+#define IO_CN(MSV) \
+    CHS_LAUNCH((k_io_cn<MSV, DIR>), grid, block, sm, s, g, idr, sub_dpg, a.mu, a.ms, a.rs, R, Gc, h->scalars, cf, LPB, P)
+    switch (MS) {
+        case 16: IO_CN(16); break;
+        case 12: IO_CN(12); break;
+        case 4: IO_CN(4); break;
+        default: IO_CN(8); break;
+    }
+#undef IO_CN
+    return CHS_OK;
+}
+
+// SOLVERRK3_MOM_IO with visthemflg = 1 (src/SOLVERRK3_MOM_IO.f90:76-90): the
+// split viscous terms and RHS for all three components (they read only G of
+// their own component and the velocities of the substep's start, so one pass
+// computes what the reference computes component by component), DPGRNS, then
+// MOMFA_io per component, the wall values of G, the projection and PRCALC_io
+// with its BE * RHSLLPHI term.
+static int io_substep_cn(chs_handle* h, const chs_io_fields* f, const IoArrays& a, const IoParams& p, int ns,
+                         double dt, double talp) {
+    CHS_CHECK(f->drhoi_stg != nullptr, CHS_E_ARG, "chs_solverrk3_mom_io: drhoi_stg is needed with visimplicit = 1");
+    const hipStream_t s = h->stream;
+    const IoGeo g = io_geo(h);
+    const dim3 gi = io_grid(h, h->M);
+    const long nb = io_tiles(h, h->M);
+    const size_t ncell = (size_t)h->L * h->M * h->N;
+    hipEvent_t ev;
+    h->timer.begin("io_terms_cn", s, &ev);
+#define IO_TERMS_CN(GR)                                                                                          \
+    CHS_LAUNCH(k_io_terms_cn<GR>, gi, dim3(64, 4), 0, s, g, a.g, a.q, a.mu, a.ms, f->drhoi_stg, a.pr, a.rs,      \
+               a.explt0, a.divu, p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs, h->partials)
+    switch (p.gravdir) {
+        case 1: IO_TERMS_CN(1); break;
+        case 2: IO_TERMS_CN(2); break;
+        case 3: IO_TERMS_CN(3); break;
+        default: IO_TERMS_CN(0); break;
+    }
+#undef IO_TERMS_CN
+    h->timer.end(ev, "io_terms_cn", s);
+    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(256), 0, s, h->partials, (int)nb, h->volm, p.flowdrvtp,
+               -0.5 * p.cfgv * (talp * dt), h->scalars);
+    // MOMFA_io: FACOE_io = alpha dt / (2 Re) (src/MOMFA_io.f90:29)
+    const double FACOE = 0.5 * talp * dt * h->cvisc;
+    const IoCnCoef cx{FACOE * 4.0 / 3.0 * h->dxqi, FACOE * h->dxqi, h->amvr, h->apvr};
+    const IoCnCoef cz{FACOE * 4.0 / 3.0 * h->dzqi, FACOE * h->dzqi, h->amvr, h->apvr};
+    const IoCnCoef cy{FACOE * 4.0 / 3.0, FACOE, h->amvr, h->apvr};
+    const size_t CS = (size_t)h->L * (h->M + 2) * h->N;
+    h->timer.begin("io_momfa", s, &ev);
+    for (int idr = 1; idr <= 3; ++idr) {
+        double* R = h->rhs + (size_t)(idr - 1) * ncell;
+        double* Gc = a.g + (size_t)(idr - 1) * CS;
+        CHS_TRY(io_cn_launch<0>(h, g, idr, p.nflow == idr, a, R, Gc, cx));
+        CHS_TRY(io_cn_launch<2>(h, g, idr, 0, a, R, Gc, cz));
+        CHS_TRY(io_cn_launch<1>(h, g, idr, 0, a, R, Gc, cy));
+    }
+    h->timer.end(ev, "io_momfa", s);
+    CHS_LAUNCH(k_io_massflux, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, a.g, nullptr, h->scalars, p.nflow);
+    h->timer.begin("io_divg", s, &ev);
+    CHS_LAUNCH(k_io_divg, gi, dim3(64, 4), 0, s, g, a.g, a.drhodtp, 1.0 / dt / talp, h->rhsllphi);
+    h->timer.end(ev, "io_divg", s);
+    CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (talp * dt)));
+    h->timer.begin("io_update", s, &ev);
+    CHS_LAUNCH(k_io_update, gi, dim3(64, 4), 0, s, g, a.g, a.pr, h->dph, dt * talp, h->rhsllphi,
+               0.5 * talp * dt * h->cvisc);
+    h->timer.end(ev, "io_update", s);
     return CHS_OK;
 }
 
@@ -568,10 +1048,11 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     hipEvent_t ev;
     // G0 = G (src/SOLVERRK3_MOM_IO.f90:15)
     CHS_HIP(hipMemcpyAsync(f->g0, f->g, 3 * ngh * sizeof(double), hipMemcpyDeviceToDevice, s));
+    IoParams p{prm->thermal, prm->nflow, prm->flowdrvtp, prm->gravdir, prm->cfgv, prm->f_a};
+    if (prm->visimplicit) return io_substep_cn(h, f, a, p, ns, dt, talp);   // no DIVG_U_io in that branch
     h->timer.begin("io_divu", s, &ev);
     CHS_LAUNCH(k_io_divu, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, f->q, f->divu);
     h->timer.end(ev, "io_divu", s);
-    IoParams p{prm->thermal, prm->nflow, prm->flowdrvtp, prm->gravdir, prm->cfgv, prm->f_a};
     h->timer.begin("io_terms", s, &ev);
 #define IO_TERMS(GR)                                                                                              \
     CHS_LAUNCH(k_io_terms<GR>, gi, dim3(64, 4), 0, s, g, a.g, a.q, a.mu, a.mu0, a.divu, a.ms, a.pr, a.rs, a.explt0, \
@@ -594,7 +1075,7 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     h->timer.end(ev, "io_divg", s);
     CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (talp * dt)));
     h->timer.begin("io_update", s, &ev);
-    CHS_LAUNCH(k_io_update, gi, dim3(64, 4), 0, s, g, f->g, f->pr, h->dph, dt * talp);
+    CHS_LAUNCH(k_io_update, gi, dim3(64, 4), 0, s, g, f->g, f->pr, h->dph, dt * talp, nullptr, 0.0);
     h->timer.end(ev, "io_update", s);
     return CHS_OK;
 }

@@ -1,10 +1,11 @@
 """Host mirror of the heated periodic channel's substep on device-resident
 fields: the energy equation (SOLVERRK3_ENG_IO), the property staging and the
-io momentum equation (SOLVERRK3_MOM_IO, explicit viscous terms), in the order
+io momentum equation (SOLVERRK3_MOM_IO, explicit viscous terms or, with
+visimplicit = 1, the RK-CN branch), in the order
 SOLVE runs them for each RK3 substep (src/SOLVE.f90:119-133).
 
     HeatedChannel(solver, table, thermal=1, nflow=1, flowdrvtp=1, gravdir=0,
-                  cfgv=0.0, f_a=0.0)
+                  cfgv=0.0, f_a=0.0, visimplicit=0)
     HeatedChannel.upload(fields) / download()
     HeatedChannel.solverrk3_eng_io(ns, dt, cthecd)    SOLVERRK3_ENG_IO(NS)
     HeatedChannel.io_prepare(ns)                      DENSITY_TIME_DERIVATION,
@@ -16,7 +17,7 @@ SOLVE runs them for each RK3 substep (src/SOLVE.f90:119-133).
 
 Fields use the reference layouts (Fortran order): the thermal fields of
 energy.ThermalState plus Q_io, PR_io, DrhoDtP, DivU_io (NCL1, 0:NCL2+1,
-NCL3[, 3]), RHO_STG, MU_STG (NCL1, 0:NCL2+1, NCL3, 3), EXPLT0_io
+NCL3[, 3]), RHO_STG, MU_STG, DRHOI_STG (NCL1, 0:NCL2+1, NCL3, 3), EXPLT0_io
 (NCL1, NCL2, NCL3, 3).  G_io and G0_io are shared by both equations.
 """
 from __future__ import annotations
@@ -29,19 +30,20 @@ from . import native as nat
 from .energy import NAMES as THERMAL_NAMES
 from .energy import ThermalState
 
-IO_NAMES = ("q", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0")
+IO_NAMES = ("q", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0", "drhoi_stg")
 
 
 class HeatedChannel(ThermalState):
     """Device fields of the heated channel on the grid of a ChannelSolver, with
     the property table (props.PropertyTable) that closes them."""
 
-    def __init__(self, solver, table, thermal=1, nflow=1, flowdrvtp=1, gravdir=0, cfgv=0.0, f_a=0.0):
+    def __init__(self, solver, table, thermal=1, nflow=1, flowdrvtp=1, gravdir=0, cfgv=0.0, f_a=0.0,
+                 visimplicit=0):
         import torch
         super().__init__(solver, table)
         g = solver.grid
         L, M, N = g.L, g.M, g.N
-        for k in ("q", "rho_stg", "mu_stg"):
+        for k in ("q", "rho_stg", "mu_stg", "drhoi_stg"):
             self.shapes[k] = (L, M + 2, N, 3)
         for k in ("pr", "drhodtp", "divu"):
             self.shapes[k] = (L, M + 2, N)
@@ -50,14 +52,15 @@ class HeatedChannel(ThermalState):
             self.dev[k] = torch.zeros(int(np.prod(self.shapes[k])), dtype=torch.float64, device="cuda")
         self.dev["rho_stg"].fill_(1.0)
         self.dev["mu_stg"].fill_(1.0)
-        self.params = nat.IoParams(int(thermal), int(nflow), int(flowdrvtp), int(gravdir), float(cfgv), float(f_a))
+        self.params = nat.IoParams(int(thermal), int(nflow), int(flowdrvtp), int(gravdir), float(cfgv), float(f_a),
+                                   int(visimplicit))
 
     def _tf(self):
         return nat.ThermalFields(*[nat.dptr(self.dev[k]) for k in THERMAL_NAMES])
 
     def _iof(self):
         return nat.IoFields(*[nat.dptr(self.dev[k]) for k in ("q", "g", "g0", "pr", "drhodtp", "divu", "rho_stg",
-                                                               "mu_stg", "explt0")])
+                                                               "mu_stg", "explt0", "drhoi_stg")])
 
     def io_prepare(self, ns: int):
         tf, f = self._tf(), self._iof()

@@ -43,12 +43,13 @@ class ThermalFields(C.Structure):
 
 
 class IoFields(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("q", "g", "g0", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0")]
+    _fields_ = [(n, C.c_void_p) for n in ("q", "g", "g0", "pr", "drhodtp", "divu", "rho_stg", "mu_stg", "explt0",
+                                          "drhoi_stg")]
 
 
 class IoParams(C.Structure):
     _fields_ = [("thermal", C.c_int), ("nflow", C.c_int), ("flowdrvtp", C.c_int), ("gravdir", C.c_int),
-                ("cfgv", C.c_double), ("f_a", C.c_double)]
+                ("cfgv", C.c_double), ("f_a", C.c_double), ("visimplicit", C.c_int)]
 
 
 # every export of include/chapsim_gpu.h with its ctypes signature

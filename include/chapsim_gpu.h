@@ -213,16 +213,18 @@ int chs_energy_substep(chs_handle *h, chs_props *p, const chs_thermal_fields *f,
  * viscosity, viscosity0 (used as given, ghost rows included). */
 typedef struct chs_io_fields {
     double *q, *g, *g0, *pr, *drhodtp, *divu, *rho_stg, *mu_stg, *explt0;
+    double *drhoi_stg;   /* DRHOI_STG (NCL1, 0:NCL2+1, NCL3, 3): visimplicit = 1 only (may be NULL otherwise) */
 } chs_io_fields;
 
 /* thermal: thermlflg (1 = variable properties: DrhoDtP, RHO_STG and MU_STG
  *          are computed; 0 = they are left as the caller set them, 0 and 1);
  * nflow: NFLOW 1..3; flowdrvtp: 1 constant mass flux, 2 constant pressure
  * gradient with cfgv = CFGV; gravdir: GRAVDIR 0..3 with f_a = F_A
- * (src/Flow_State.f90:97-110). */
+ * (src/Flow_State.f90:97-110); visimplicit: visthemflg (src/READINI.f90:225). */
 typedef struct chs_io_params {
     int thermal, nflow, flowdrvtp, gravdir;
     double cfgv, f_a;
+    int visimplicit;     /* visthemflg: 0 explicit viscous terms, 1 RK-CN (VISCOUS_PAR_EXPLT_*_io + MOMFA_io) */
 } chs_io_params;
 
 /* What SOLVE runs between SOLVERRK3_ENG_IO(NS) and SOLVERRK3_MOM_IO(NS)
@@ -231,9 +233,11 @@ typedef struct chs_io_params {
  * VELOCITY_CALC_io (src/MASSFLUX_CALC_IO.f90:82-311).  ns = 1..3. */
 int chs_io_prepare(chs_handle *h, const chs_thermal_fields *tf, const chs_io_fields *f, const chs_io_params *p,
                    int ns);
-/* SOLVERRK3_MOM_IO(NS) (src/SOLVERRK3_MOM_IO.f90:1-168), explicit viscous
+/* SOLVERRK3_MOM_IO(NS) (src/SOLVERRK3_MOM_IO.f90:1-168).  Explicit viscous
  * branch: CONVECTION_*_io, DIVG_U_io, VISCOUS_ALL_EXPLT_*_io,
- * RHS_MOM_EXPLICIT_io (src/RHS_MOM_EXPLICIT_io.f90:1-297), MASSFLUX_CALC_IO,
+ * RHS_MOM_EXPLICIT_io (src/RHS_MOM_EXPLICIT_io.f90:1-297), MASSFLUX_CALC_IO;
+ * RK-CN branch (visimplicit = 1): VISCOUS_PAR_EXPLT_*_io, RHS_MOM_EXPLICIT_io
+ * and MOMFA_io (src/MOMFA_io.f90:17-377) per component; then in both
  * DIVG_io (src/DIVG.f90:60-150), the FFT99 Poisson solve (the reference's
  * alternative for the io domain, src/SOLVERRK3_MOM_IO.f90:123), PRCALC_io and
  * MASSFLUX_UPDATE_IO (src/VELO_CORRECTION.f90:109-185).  Updates g, g0, pr,

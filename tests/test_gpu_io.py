@@ -15,7 +15,7 @@ CTHECD = 1.0 / 2800.0
 MAP = {"rhoh": "rhoh", "h": "enthalpy", "T": "temperature", "rho": "density", "rho0": "density0",
        "rho1": "density1", "mu": "viscosity", "mu0": "viscosity0", "kap": "thermcondt", "rhs0": "rhs_energy0",
        "g": "g", "g0": "g0", "wallflux": "wallflux", "q": "q", "pr": "pr", "drhodtp": "drhodtp", "divu": "divu",
-       "rho_stg": "rho_stg", "mu_stg": "mu_stg", "explt0": "explt0"}
+       "rho_stg": "rho_stg", "mu_stg": "mu_stg", "explt0": "explt0", "drhoi_stg": "drhoi_stg"}
 FP64_TOL = 1e-12
 
 
@@ -45,12 +45,12 @@ def _pair(dims, scaled, seed, **kw):
 @pytest.mark.parametrize("dims", [(16, 12, 8), (64, 48, 32)])
 def test_io_prepare_bitwise(scaled, dims):
     print(f"io prepare {dims}", flush=True)
-    og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=sum(dims))
+    og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=sum(dims), visimplicit=1)
     for ns in (1, 2):
         io.prepare(ns)
         hc.io_prepare(ns)
         got = hc.download()
-        for k in ("drhodtp", "rho_stg", "mu_stg", "q"):
+        for k in ("drhodtp", "rho_stg", "mu_stg", "q", "drhoi_stg"):
             assert np.array_equal(got[k], st[k]), (ns, k, np.abs(got[k] - st[k]).max())
     pt.close()
     s.close()
@@ -58,7 +58,9 @@ def test_io_prepare_bitwise(scaled, dims):
 
 @pytest.mark.parametrize("dims,kw", [((16, 12, 8), dict(thermal=1, gravdir=1, f_a=-0.05)),
                                      ((64, 48, 32), dict(thermal=1, gravdir=2, f_a=0.02)),
-                                     ((32, 24, 16), dict(thermal=1, flowdrvtp=2, cfgv=0.01, gravdir=3, f_a=0.03))])
+                                     ((32, 24, 16), dict(thermal=1, flowdrvtp=2, cfgv=0.01, gravdir=3, f_a=0.03)),
+                                     ((32, 24, 16), dict(thermal=1, gravdir=1, f_a=-0.05, visimplicit=1)),
+                                     ((64, 48, 32), dict(thermal=1, nflow=3, visimplicit=1))])
 def test_heated_substeps(scaled, dims, kw):
     """Two RK3 steps (NS = 1, 2, 3 twice) of energy + staging + io momentum;
     every field within 1e-12 of the oracle after every substep and the
@@ -79,7 +81,7 @@ def test_heated_substeps(scaled, dims, kw):
         hc.substep(ns, dt, CTHECD)
         got = hc.download()
         errs = {k: _rel(got[MAP[k]], st[k]) for k in ("g", "g0", "explt0", "divu", "q", "rho_stg", "mu_stg",
-                                                      "drhodtp", "rho", "T", "h")}
+                                                      "drhodtp", "rho", "T", "h", "drhoi_stg")}
         pr_scale = max(pr_scale, np.abs(st["pr"]).max())
         errs["pr"] = float(np.abs(got["pr"] - st["pr"]).max() / pr_scale)
         dg, _ = hc.divgck_io()
