@@ -280,6 +280,18 @@ def props_bench(g, q_fp64, stream, steps):
     prof = s.profile_read()
     s.profile(False)
     dg, _ = ts.divgck_io()
+    # the RK-CN branch (visthemflg = 1) of the momentum part on the same state
+    ts.params.visimplicit = 1
+    ms_cn = timed(mom, reps)
+    s.profile(True)
+    s.profile_reset()
+    for _ in range(reps):
+        mom()
+    _sync()
+    prof_cn = s.profile_read()
+    s.profile(False)
+    dg_cn, _ = ts.divgck_io()
+    ts.params.visimplicit = 0
     ab = {"io_prepare": 160.0, "io_divu": 32.0, "io_terms": 176.0, "io_massflux": 72.0, "io_divg": 40.0,
           "io_update": 72.0}
     kern = {}
@@ -290,6 +302,8 @@ def props_bench(g, q_fp64, stream, steps):
                 kern[name]["alg_gbs"] = round(ab[name] * n / (ms_tot / calls * 1e-3) / 1e9, 1)
     out["heated_substep"] = dict(ms=round(ms_h, 4), substeps_per_s=round(1e3 / ms_h, 2),
                                  momentum_ms=round(ms_m, 4), max_div_g=dg, kernels=kern,
+                                 momentum_cn_ms=round(ms_cn, 4), max_div_g_cn=dg_cn,
+                                 kernels_cn={k: round(v[0] / v[1], 5) for k, v in prof_cn.items() if v[1]},
                                  note="SOLVE order: SOLVERRK3_ENG_IO, DENSITY_TIME_DERIVATION + DENSITY/MU_Staggered"
                                       " + VELOCITY_CALC_io, SOLVERRK3_MOM_IO (explicit viscous, FFT99 projection)")
     _note("heated substep done")

@@ -960,11 +960,16 @@ static int io_cn_launch(chs_handle* h, const IoGeo& g, int idr, int sub_dpg, con
     const int n = DIR == 0 ? h->L : (DIR == 2 ? h->N : h->M);
     const int nlines = DIR == 0 ? h->M * h->N : (DIR == 2 ? h->L * h->M : h->L * h->N);
     int MS = 0, P = 0, LPB = 0;
-    CHS_CHECK(tdma_seg_config(n, DIR != 1, &MS, &P, &LPB, 1024, false), CHS_E_ARG,
+    // x lines are contiguous in memory: few lines per block (8) put a wave's
+    // lanes on consecutive segments of the same lines, so its loads coalesce
+    CHS_CHECK(tdma_seg_config(n, DIR != 1, &MS, &P, &LPB, 1024, DIR == 0), CHS_E_ARG,
               "chs_solverrk3_mom_io: no segmented solver layout for a line length of " + std::to_string(n));
     const dim3 grid((nlines + LPB - 1) / LPB), block(LPB * P);
     const size_t sm = seg_smem_bytes<double>(LPB, P);
     const hipStream_t s = h->stream;
+    const char* nm = DIR == 0 ? "io_cn_x" : (DIR == 2 ? "io_cn_z" : "io_cn_y");
+    hipEvent_t ev;
+    h->timer.begin(nm, s, &ev);
 #define IO_CN(MSV) \
     CHS_LAUNCH((k_io_cn<MSV, DIR>), grid, block, sm, s, g, idr, sub_dpg, a.mu, a.ms, a.rs, R, Gc, h->scalars, cf, LPB, P)
     switch (MS) {
@@ -974,6 +979,7 @@ static int io_cn_launch(chs_handle* h, const IoGeo& g, int idr, int sub_dpg, con
         default: IO_CN(8); break;
     }
 #undef IO_CN
+    h->timer.end(ev, nm, s);
     return CHS_OK;
 }
 
@@ -1012,7 +1018,6 @@ static int io_substep_cn(chs_handle* h, const chs_io_fields* f, const IoArrays& 
     const IoCnCoef cz{FACOE * 4.0 / 3.0 * h->dzqi, FACOE * h->dzqi, h->amvr, h->apvr};
     const IoCnCoef cy{FACOE * 4.0 / 3.0, FACOE, h->amvr, h->apvr};
     const size_t CS = (size_t)h->L * (h->M + 2) * h->N;
-    h->timer.begin("io_momfa", s, &ev);
     for (int idr = 1; idr <= 3; ++idr) {
         double* R = h->rhs + (size_t)(idr - 1) * ncell;
         double* Gc = a.g + (size_t)(idr - 1) * CS;
@@ -1020,7 +1025,6 @@ static int io_substep_cn(chs_handle* h, const chs_io_fields* f, const IoArrays& 
         CHS_TRY(io_cn_launch<2>(h, g, idr, 0, a, R, Gc, cz));
         CHS_TRY(io_cn_launch<1>(h, g, idr, 0, a, R, Gc, cy));
     }
-    h->timer.end(ev, "io_momfa", s);
     CHS_LAUNCH(k_io_massflux, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, a.g, nullptr, h->scalars, p.nflow);
     h->timer.begin("io_divg", s, &ev);
     CHS_LAUNCH(k_io_divg, gi, dim3(64, 4), 0, s, g, a.g, a.drhodtp, 1.0 / dt / talp, h->rhsllphi);
