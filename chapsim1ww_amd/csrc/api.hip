@@ -28,7 +28,7 @@ void KernelTimer::begin(const char* name, hipStream_t s, hipEvent_t* out_a) {
     if (!on) return;
     hipEvent_t a;
     if (hipEventCreate(&a) != hipSuccess) return;
-    hipEventRecord(a, s);
+    (void)hipEventRecord(a, s);
     *out_a = a;
 }
 
@@ -36,15 +36,15 @@ void KernelTimer::end(hipEvent_t a, const char* name, hipStream_t s) {
     if (!on || !a) return;
     hipEvent_t b;
     if (hipEventCreate(&b) != hipSuccess) return;
-    hipEventRecord(b, s);
+    (void)hipEventRecord(b, s);
     pending.push_back({name, a, b});
 }
 
 void KernelTimer::collect() {
     for (auto& r : pending) {
-        hipEventSynchronize(r.b);
+        (void)hipEventSynchronize(r.b);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, r.a, r.b);
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
         bool found = false;
         for (auto& t : totals)
             if (t.first == r.name) {
@@ -53,8 +53,8 @@ void KernelTimer::collect() {
                 found = true;
             }
         if (!found) totals.push_back({r.name, {(double)ms, 1L}});
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
     }
     pending.clear();
 }
@@ -104,7 +104,7 @@ static void free_all(chs_handle* h) {
                     h->rhsllphi, h->spec, h->spec32, h->rhs_j1, h->partials, h->scalars, h->resid, h->f32a, h->f32b,
                     h->rhs2};
     for (void* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
 }
 
 }  // namespace chs
@@ -201,7 +201,7 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     chk(dalloc(&h->scalars, 16));
     if (st != CHS_OK) {
         free_all(h);
-        if (h->own_stream) hipStreamDestroy(h->stream);
+        if (h->own_stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return st;
     }
@@ -223,11 +223,11 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
 
 int chs_destroy(chs_handle* h) {
     if (!h) return CHS_OK;
-    hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->stream);
     h->timer.collect();
     release_stencils(h);
     free_all(h);
-    if (h->own_stream) hipStreamDestroy(h->stream);
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CHS_OK;
 }

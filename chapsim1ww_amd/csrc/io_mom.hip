@@ -290,9 +290,7 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         // wall face); it is evaluated with a clamped metric index and
         // discarded, so that the whole kernel body is one straight line.
         const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC >= 2 ? JC - 1 : 1;
-        const double wff = g.wff[JJ - 1], wfb = g.wfb[JJ - 1], wffp = g.wff[JJP - 1], wfbp = g.wfb[JJP - 1];
         const double CV = g.cvisc;
-        const double *G1 = G, *G2 = G + CS, *G3 = G + 2 * CS;
         const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
         const double *MS1 = MS, *MS2 = MS + CS, *MS3 = MS + 2 * CS;
         const int c = at(IC, JC, KC);

@@ -507,9 +507,9 @@ int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double
 void mlp_release(chs_props* p) {
     if (!p || !p->mlp) return;
     Mlp* m = p->mlp;
-    if (m->par) hipFree(m->par);
-    if (m->partial) hipFree(m->partial);
-    if (m->cert) hipFree(m->cert);
+    if (m->par) (void)hipFree(m->par);
+    if (m->partial) (void)hipFree(m->partial);
+    if (m->cert) (void)hipFree(m->cert);
     delete m;
     p->mlp = nullptr;
 }
@@ -545,7 +545,7 @@ static int mlp_alloc(chs_props* p, int maxblk) {
     // one allocation: par | wq | am | av
     if (hipMalloc((void**)&m->par, sizeof(float) * 4 * (size_t)(NPAR + 3)) != hipSuccess ||
         hipMalloc((void**)&m->partial, sizeof(float) * (size_t)maxblk * (NPAR + 1)) != hipSuccess) {
-        if (m->par) hipFree(m->par);
+        if (m->par) (void)hipFree(m->par);
         delete m;
         set_error("surrogate: device allocation failed");
         return CHS_E_NOMEM;
@@ -605,7 +605,7 @@ struct Pool {
     float* px = nullptr;
     float* pt = nullptr;
     ~Pool() {
-        if (px) hipFree(px);
+        if (px) (void)hipFree(px);
     }
 };
 
@@ -615,7 +615,7 @@ static int make_pool(chs_props* p, const double* h, long long n, Pool& pool) {
     CHS_HIP(hipMalloc((void**)&lab, sizeof(double) * 5 * (size_t)n));
     float* buf = nullptr;
     if (hipMalloc((void**)&buf, sizeof(float) * 6 * (size_t)n) != hipSuccess) {
-        hipFree(lab);
+        (void)hipFree(lab);
         set_error("surrogate: device allocation failed");
         return CHS_E_NOMEM;
     }
@@ -639,7 +639,7 @@ static int make_pool(chs_props* p, const double* h, long long n, Pool& pool) {
                            lab + 3 * n, lab + 4 * n, pool.px, pool.pt);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) st = CHS_E_HIP;
     }
-    hipFree(lab);
+    (void)hipFree(lab);
     return st;
 }
 
@@ -727,7 +727,7 @@ static int train(chs_props* p, const double* h, long long nsamp, int steps, int 
     }
     m->trained = true;
     if (m->cert) {   // a new network needs a new certificate
-        hipFree(m->cert);
+        (void)hipFree(m->cert);
         m->cert = nullptr;
         m->cert_frac = 0.0;
     }
@@ -755,7 +755,7 @@ static int grad(chs_props* p, const double* h, long long n, float* grad_host, do
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(gh.data(), g, sizeof(float) * (NPAR + 1), hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-    hipFree(g);
+    (void)hipFree(g);
     CHS_HIP(e);
     if (grad_host) std::copy(gh.begin(), gh.begin() + NPAR, grad_host);
     if (loss) *loss = gh[NPAR];
@@ -802,7 +802,7 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
          hipMemcpyAsync(&bad, p->counters, sizeof(bad), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
          hipStreamSynchronize(p->stream) != hipSuccess))
         st = CHS_E_HIP;
-    hipFree(buf);
+    (void)hipFree(buf);
     if (st != CHS_OK) {
         set_error("chs_surrogate_certify: device evaluation failed");
         return st;
@@ -824,7 +824,7 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
         flag[b] = worst <= 0.5 * tol ? 1 : 0;
         ok += flag[b];
     }
-    if (m->cert) hipFree(m->cert);
+    if (m->cert) (void)hipFree(m->cert);
     m->cert = nullptr;
     CHS_HIP(hipMalloc((void**)&m->cert, nbins));
     CHS_HIP(hipMemcpy(m->cert, flag.data(), nbins, hipMemcpyHostToDevice));
@@ -875,7 +875,7 @@ static int params(chs_props* p, int op, float* host, int count, double* norm) {
     }
     m->trained = true;
     if (m->cert) {
-        hipFree(m->cert);
+        (void)hipFree(m->cert);
         m->cert = nullptr;
     }
     CHS_HIP(hipStreamSynchronize(p->stream));

@@ -257,9 +257,9 @@ int mlp_update(chs_props* p, const double* h, Rows rw, double* T, double* rho, d
 int props_destroy(chs_props* p) {
     if (!p) return CHS_OK;
     mlp_release(p);
-    if (p->rh) hipFree(p->rh);
-    if (p->counters) hipFree(p->counters);
-    if (p->own_stream && p->stream) hipStreamDestroy(p->stream);
+    if (p->rh) (void)hipFree(p->rh);
+    if (p->counters) (void)hipFree(p->counters);
+    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return CHS_OK;
 }

@@ -863,7 +863,7 @@ static std::vector<CnStencils>& stencil_cache(const chs_handle* h) {
 void release_stencils(const chs_handle* h) {
     auto& v = stencil_cache(h);
     for (auto& c : v)
-        if (c.dev) hipFree(c.dev);
+        if (c.dev) (void)hipFree(c.dev);
     v.clear();
 }
 
@@ -939,7 +939,7 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
         CHS_HIP(hipMemcpy(st.dev, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     if (cache.size() >= 8) {
-        if (cache.front().dev) hipFree(cache.front().dev);
+        if (cache.front().dev) (void)hipFree(cache.front().dev);
         cache.erase(cache.begin());
     }
     cache.push_back(st);

@@ -216,7 +216,7 @@ extern "C" int chs_tdma_batched_host(const double* a, const double* b, const dou
                               elem_stride, line_stride, cyclic, nullptr);
     if (st == CHS_OK && hipMemcpy(r, d + 3 * span, span * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         st = CHS_E_HIP;
-    hipFree(d);
+    (void)hipFree(d);
     if (st == CHS_E_HIP) chs::set_error("chs_tdma_batched_host: HIP copy or launch failed");
     return st;
 }
