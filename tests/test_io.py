@@ -284,3 +284,46 @@ def test_cn_heated_substeps_project(scaled):
         io.substep(ns, 1e-3)
         assert abs(flux() - f0) <= 1e-9 * abs(f0)
         assert io.divgck()[0] < 1e-10
+
+
+def test_planted_mistakes_in_the_closed_forms_are_caught():
+    """The closed-form checks see plausible slips: the CN wall row without
+    the AMVR1 term, and DPGRNS with the wrong sign."""
+    og = O.OracleGrid(16, 12, 8, ren=500.0)
+    L, M, N = og.L, og.M, og.N
+    y = np.asarray(og.ycc)
+    f = (1 - y[None, :, None] ** 2) * np.ones((L, 1, N))
+    cv, dt, ns = 1.0 / 500.0, 2e-3, 1
+    a = TGAM[ns] + TROH[ns]
+    F = 0.5 * a * dt * cv
+    r = a * dt * cv * _lap_y(og, f)
+    r = r - (r / og.dyfi[None, :, None]).sum() / og.volm
+    st = _uniform_case(og)
+    st["g"][:, 1:-1, :, 0] = f
+    st["g0"][...] = st["g"]
+    io = O.OracleIO(og, st, thermal=1, nflow=1, flowdrvtp=1, visimplicit=1)
+    io.prepare(ns)
+    io.substep(ns, dt)
+    got = st["g"][:, 1:-1, :, 0]
+    solve = lambda A, rr: np.moveaxis(np.linalg.solve(A, np.moveaxis(rr, 1, 0).reshape(M, -1))
+                                      .reshape(np.moveaxis(rr, 1, 0).shape), 0, 1)
+    Ay = _ly_matrix(og, "u", F)
+    assert np.abs(got - (f + solve(Ay, r))).max() < 1e-13       # the right form holds
+    Ay_bad = Ay.copy()
+    am = np.asarray(og.arr["amvr"]).reshape(3, M + 1)[0]
+    ucaj = 4.0 / (2.0 / og.dyci[1] + 1.0 / og.dyfi[0])
+    Ay_bad[0, 0] -= F * ucaj * 2.0 * og.dyfi[0]                  # drop the wall term of row 1
+    assert am[0] == 0.0
+    assert np.abs(got - (f + solve(Ay_bad, r))).max() > 1e-11    # 100x the right form's residual
+    # the explicit branch with the mass-flux correction's sign flipped
+    st = _uniform_case(og)
+    st["g"][:, 1:-1, :, 0] = f
+    st["g0"][...] = st["g"]
+    io = O.OracleIO(og, st, thermal=1, nflow=1, flowdrvtp=1)
+    io.prepare(ns)
+    io.substep(ns, dt)
+    rx = TGAM[ns] * dt * cv * _lap_y(og, f)
+    dpg = (rx / og.dyfi[None, :, None]).sum() / og.volm
+    got = st["g"][:, 1:-1, :, 0]
+    assert np.abs(got - (f + rx - dpg)).max() < 1e-13
+    assert np.abs(got - (f + rx + dpg)).max() > 1e-11
