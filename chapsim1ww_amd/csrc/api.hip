@@ -157,6 +157,13 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
         }
         h->own_stream = true;
     }
+    {
+        int dev = 0, v = 0;
+        h->lds_max = hipGetDevice(&dev) == hipSuccess &&
+                             hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess
+                         ? v
+                         : 65536;
+    }
     const size_t L = h->L, M = h->M, N = h->N;
     const size_t ncell = L * M * N, ngh = L * (M + 2) * N;
     int st = CHS_OK;
@@ -273,7 +280,8 @@ int chs_rk3_substep(chs_handle* h, int ns, double dt) {
 
 int chs_set_cn_solver(chs_handle* h, int mode) {
     CHS_CHECK(h, CHS_E_ARG, "chs_set_cn_solver: null handle");
-    CHS_CHECK(mode == 0 || mode == 1, CHS_E_ARG, "chs_set_cn_solver: mode must be 0 (auto) or 1 (segmented Thomas)");
+    CHS_CHECK(mode >= 0 && mode <= 2, CHS_E_ARG,
+              "chs_set_cn_solver: mode must be 0 (auto), 1 (segmented Thomas) or 2 (auto, per-row RHS kernel)");
     h->cn_mode = mode;
     return CHS_OK;
 }

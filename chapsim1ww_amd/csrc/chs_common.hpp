@@ -160,7 +160,9 @@ struct chs_handle {
     // work: DPH(L,0:M+1,N), RHS x3 (L,M,N), RHSLLPHI(L,M,N), spectral S
     double *dph, *rhs, *rhsllphi;
     double *rhs2;        // second RHS buffer of the stencil CN path (z output)
-    int cn_mode;         // 0 = auto (stencils when their truncation bound holds), 1 = segmented Thomas
+    int cn_mode;         // 0 = auto (stencils when their truncation bound holds), 1 = segmented Thomas,
+                         // 2 = auto with the per-row RHS kernel instead of the y-marching one
+    int lds_max;         // LDS bytes a block may use on the handle's device
     double2 *spec;
     float2 *spec32;
     double *rhs_j1;      // the u-increment on j = 1, which becomes v's row-1 RHS

@@ -52,10 +52,25 @@ struct RkCoef {
     double tgam, troh, talp, dt;
 };
 
-// Neighbour offsets of a cell (i, J, k) of a (L, 0:M+1, N) array, shared by
-// every component of Q and by PR: i+1, i-1, k+1, k-1 (both periodic) and J+1.
+// Neighbour offsets of a cell within its row plane: i+1, i-1 (periodic) and
+// k+1, k-1 (periodic in a (L, 0:M+1, N) array; plain in an LDS plane that
+// carries the z neighbours as halo rows).
 struct Nb {
-    int ip, im, kp, km, L;
+    int ip, im, kp, km;
+};
+
+// One component of Q (or PR) around a cell: v(dj, o) is the value at row
+// J + dj (dj in -1..1, a constant at every call site) and in-row offset o.
+// GCol reads the ghost-aware global array (rows L apart); RCol reads the LDS
+// ring of k_rhs_xm, which holds rows J-1, J, J+1 as separate planes.
+struct GCol {
+    const double* __restrict__ p;
+    int L;
+    __device__ __forceinline__ double operator()(int dj, int o) const { return p[dj * L + o]; }
+};
+struct RCol {
+    const double *m, *c, *p;
+    __device__ __forceinline__ double operator()(int dj, int o) const { return (dj < 0 ? m : dj > 0 ? p : c)[o]; }
 };
 
 // y coefficients of the block's row j (uniform over the block)
@@ -63,53 +78,117 @@ struct YCoef {
     double dyfi, dyci, wf0, wb0, wf1, wb1;
 };
 
+__device__ __forceinline__ YCoef ycoef(const Geo& g, int j) {
+    YCoef y;
+    y.dyfi = g.dyfi[j];
+    y.dyci = g.dyci[j];
+    y.wf0 = g.wff[j];
+    y.wb0 = g.wfb[j];
+    y.wf1 = g.wff[j + 1];
+    y.wb1 = g.wfb[j + 1];
+    return y;
+}
+
 // Convection H_c (CONVECTION_X/Y/Z_tg: src/CONVECTION_X.f90:23-74,
 // src/CONVECTION_Y.f90:22-127, src/CONVECTION_Z.f90:22-112; channel: RCCI1 =
-// RNDI1 = RCCI2 = 1, XND2CL = YND2CL = ZND2CL = 1/2).  U, V, W point at
-// (i, J, k) of the three components of Q.
-__device__ __forceinline__ double conv_x(const Geo& g, const YCoef& y, const double* __restrict__ U,
-                                         const double* __restrict__ V, const double* __restrict__ W, const Nb& n) {
+// RNDI1 = RCCI2 = 1, XND2CL = YND2CL = ZND2CL = 1/2) at the cell (i, J, k)
+// of U, V, W.
+template <class A>
+__device__ __forceinline__ double conv_x(const Geo& g, const YCoef& y, const A& U, const A& V, const A& W,
+                                         const Nb& n) {
     const double COE1 = 0.5 * 0.5 * g.dxi;
     const double COE2 = y.dyfi * 0.5 * 1.0;
     const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
-    const double u_c = U[0], u_p = U[n.ip], u_m = U[n.im];
+    const double u_c = U(0, 0), u_p = U(0, n.ip), u_m = U(0, n.im);
     const double H11 = ((u_p + u_c) * (u_p + u_c) - (u_m + u_c) * (u_m + u_c)) * COE1;
-    const double H12 = ((V[n.L] + V[n.im + n.L]) * (y.wf1 * U[n.L] + y.wb1 * u_c) -
-                        (V[0] + V[n.im]) * (y.wf0 * u_c + y.wb0 * U[-n.L])) * COE2;
-    const double H13 = ((W[n.kp] + W[n.im + n.kp]) * (U[n.kp] + u_c) -
-                        (W[0] + W[n.im]) * (u_c + U[n.km])) * COE3;
+    const double H12 = ((V(1, 0) + V(1, n.im)) * (y.wf1 * U(1, 0) + y.wb1 * u_c) -
+                        (V(0, 0) + V(0, n.im)) * (y.wf0 * u_c + y.wb0 * U(-1, 0))) * COE2;
+    const double H13 = ((W(0, n.kp) + W(0, n.im + n.kp)) * (U(0, n.kp) + u_c) -
+                        (W(0, 0) + W(0, n.im)) * (u_c + U(0, n.km))) * COE3;
     return -(H11 + H12 + H13);
 }
 
-__device__ __forceinline__ double conv_y(const Geo& g, const YCoef& y, const double* __restrict__ U,
-                                         const double* __restrict__ V, const double* __restrict__ W, const Nb& n) {
+template <class A>
+__device__ __forceinline__ double conv_y(const Geo& g, const YCoef& y, const A& U, const A& V, const A& W,
+                                         const Nb& n) {
     const double COE1 = g.dxi * 0.5;
     const double COE2 = y.dyci * 0.5 * 0.5;
     const double COE31 = g.dzi * 0.5 * 1.0;
     const double wf = y.wf0, wb = y.wb0;
-    const double v_c = V[0];
-    const double H21 = ((wf * U[n.ip] + wb * U[n.ip - n.L]) * (V[n.ip] + v_c) -
-                        (wf * U[0] + wb * U[-n.L]) * (v_c + V[n.im])) * COE1;
-    const double v_p = V[n.L], v_m = V[-n.L];
+    const double v_c = V(0, 0);
+    const double H21 = ((wf * U(0, n.ip) + wb * U(-1, n.ip)) * (V(0, n.ip) + v_c) -
+                        (wf * U(0, 0) + wb * U(-1, 0)) * (v_c + V(0, n.im))) * COE1;
+    const double v_p = V(1, 0), v_m = V(-1, 0);
     const double H22 = ((v_p * 1.0 + v_c * 1.0) * (v_p + v_c) - (v_c * 1.0 + v_m * 1.0) * (v_c + v_m)) * COE2;
-    const double H23 = ((wf * W[n.kp] * 1.0 + wb * W[n.kp - n.L] * 1.0) * (V[n.kp] + v_c) -
-                        (wf * W[0] * 1.0 + wb * W[-n.L] * 1.0) * (v_c + V[n.km])) * COE31;
+    const double H23 = ((wf * W(0, n.kp) * 1.0 + wb * W(-1, n.kp) * 1.0) * (V(0, n.kp) + v_c) -
+                        (wf * W(0, 0) * 1.0 + wb * W(-1, 0) * 1.0) * (v_c + V(0, n.km))) * COE31;
     return -(H21 + H22 + H23);
 }
 
-__device__ __forceinline__ double conv_z(const Geo& g, const YCoef& y, const double* __restrict__ U,
-                                         const double* __restrict__ V, const double* __restrict__ W, const Nb& n) {
+template <class A>
+__device__ __forceinline__ double conv_z(const Geo& g, const YCoef& y, const A& U, const A& V, const A& W,
+                                         const Nb& n) {
     const double COE1 = 0.5 * 0.5 * g.dxi;
     const double COE2 = y.dyfi * 0.5;
     const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
-    const double w_c = W[0];
-    const double H31 = ((U[n.ip] + U[n.ip + n.km]) * (W[n.ip] + w_c) -
-                        (U[0] + U[n.km]) * (w_c + W[n.im])) * COE1;
-    const double H32 = ((V[n.L] + V[n.L + n.km]) * (y.wf1 * W[n.L] * 1.0 + y.wb1 * w_c * 1.0) -
-                        (V[0] + V[n.km]) * (y.wf0 * w_c * 1.0 + y.wb0 * W[-n.L] * 1.0)) * COE2;
-    const double w_p = W[n.kp], w_m = W[n.km];
+    const double w_c = W(0, 0);
+    const double H31 = ((U(0, n.ip) + U(0, n.ip + n.km)) * (W(0, n.ip) + w_c) -
+                        (U(0, 0) + U(0, n.km)) * (w_c + W(0, n.im))) * COE1;
+    const double H32 = ((V(1, 0) + V(1, n.km)) * (y.wf1 * W(1, 0) * 1.0 + y.wb1 * w_c * 1.0) -
+                        (V(0, 0) + V(0, n.km)) * (y.wf0 * w_c * 1.0 + y.wb0 * W(-1, 0) * 1.0)) * COE2;
+    const double w_p = W(0, n.kp), w_m = W(0, n.km);
     const double H33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * COE3;
     return -(H31 + H32 + H33);
+}
+
+// Per-row coefficients of the RHS: the y viscous stencil of each component
+// and the RK3 factors.
+struct RowCoef {
+    YCoef y;
+    double ap[3], ac[3], am[3];
+};
+
+__device__ __forceinline__ RowCoef row_coef(const Geo& g, int j) {
+    RowCoef rc;
+    rc.y = ycoef(g, j);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        rc.ap[c] = g.apvr[c * (g.M + 1) + j];
+        rc.ac[c] = g.acvr[c * (g.M + 1) + j];
+        rc.am[c] = g.amvr[c * (g.M + 1) + j];
+    }
+    return rc;
+}
+
+// RHS_CvLpGpS_tg (src/RHS_CvLpGpS_tg.f90:117-176) at one cell for u, v, w:
+// conv[c] = H_c (the new CONVH0), rhs[c] = the x-solve right-hand side.
+// h0[c] is the cell's CONVH0 of the previous substep, P accesses PR.
+template <class A>
+__device__ __forceinline__ void rhs_cell(const Geo& g, const RkCoef& rk, double cvisc, const RowCoef& rc,
+                                         const A& U, const A& V, const A& W, const A& P, const Nb& n,
+                                         const double* h0, double* conv, double* rhs) {
+    const double COE1 = rk.talp * cvisc;
+    const double COE2 = rk.talp * rk.dt;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const A& X = c == 0 ? U : c == 1 ? V : W;
+        conv[c] = c == 0 ? conv_x(g, rc.y, U, V, W, n) : c == 1 ? conv_y(g, rc.y, U, V, W, n)
+                                                                 : conv_z(g, rc.y, U, V, W, n);
+        const double RHSC = rk.tgam * conv[c] + rk.troh * h0[c];
+        const double qc = X(0, 0);
+        const double RHSL = COE1 * ((X(0, n.ip) - 2.0 * qc + X(0, n.im)) * g.dxqi +
+                                    (X(0, n.kp) - 2.0 * qc + X(0, n.km)) * g.dzqi * 1.0 +
+                                    (X(1, 0) * rc.ap[c] + qc * rc.ac[c] + X(-1, 0) * rc.am[c]));
+        const double r = (RHSC + RHSL) * rk.dt;
+        double pgm;
+        if (c == 0)
+            pgm = (P(0, 0) - P(0, n.im)) * g.dxi * COE2;
+        else if (c == 1)
+            pgm = (P(0, 0) - P(-1, 0)) * (rc.y.dyci / 1.0) * COE2;
+        else
+            pgm = (P(0, 0) - P(0, n.km)) * g.dzi * COE2;
+        rhs[c] = r - pgm;
+    }
 }
 
 // Position of padded element p of an LDS row.  KH == 0: plain.  KH > 0 (rows
@@ -137,22 +216,7 @@ __device__ __forceinline__ double rhs_rows(const Geo& g, const RkCoef& rk, doubl
     const int LMN = L * M * N;
     const int J = j + 1;
     const int Q4 = LS >> 2;
-    YCoef y;
-    y.dyfi = g.dyfi[j];
-    y.dyci = g.dyci[j];
-    y.wf0 = g.wff[j];
-    y.wb0 = g.wfb[j];
-    y.wf1 = g.wff[j + 1];
-    y.wb1 = g.wfb[j + 1];
-    double ap[3], ac[3], am[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        ap[c] = g.apvr[c * (M + 1) + j];
-        ac[c] = g.acvr[c * (M + 1) + j];
-        am[c] = g.amvr[c * (M + 1) + j];
-    }
-    const double COE1 = rk.talp * cvisc;
-    const double COE2 = rk.talp * rk.dt;
+    const RowCoef rc = row_coef(g, j);
     double msum = 0.0;
     for (int e = threadIdx.x; e < (kr << lgL); e += blockDim.x) {
         const int r = e >> lgL, i = e & (L - 1);
@@ -162,40 +226,18 @@ __device__ __forceinline__ double rhs_rows(const Geo& g, const RkCoef& rk, doubl
         n.im = i == 0 ? L - 1 : -1;
         n.kp = k == N - 1 ? (1 - N) * LP : LP;
         n.km = k == 0 ? (N - 1) * LP : -LP;
-        n.L = L;
         const int o = i + L * J + LP * k;
-        const double* U = Q + o;
-        const double* V = U + CS;
-        const double* W = V + CS;
-        const double* P = PR + o;
+        const GCol U{Q + o, L}, V{Q + o + CS, L}, W{Q + o + 2 * CS, L}, P{PR + o, L};
         const int ci = i + L * j + L * M * k;
         // All three components are evaluated before anything is stored, so
         // their loads issue as one batch.  v on the bottom wall face (j = 0)
         // is not advanced (NII = 2): its values are computed from in-bounds
         // rows and neither CONVH0 nor its LDS row (never solved) keeps them.
-        double conv[3], rhs[3];
+        double h0[3], conv[3], rhs[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double* X = c == 0 ? U : c == 1 ? V : W;
-            conv[c] = c == 0 ? conv_x(g, y, U, V, W, n) : c == 1 ? conv_y(g, y, U, V, W, n)
-                                                                   : conv_z(g, y, U, V, W, n);
-            const int hc = ci + c * LMN;
-            const double RHSC = rk.tgam * conv[c] + rk.troh * H0[hc];
-            const double qc = X[0];
-            const double RHSL = COE1 * ((X[n.ip] - 2.0 * qc + X[n.im]) * g.dxqi +
-                                        (X[n.kp] - 2.0 * qc + X[n.km]) * g.dzqi * 1.0 +
-                                        (X[n.L] * ap[c] + qc * ac[c] + X[-n.L] * am[c]));
-            double r = (RHSC + RHSL) * rk.dt;
-            double pgm;
-            if (c == 0)
-                pgm = (P[0] - P[n.im]) * g.dxi * COE2;
-            else if (c == 1)
-                pgm = (P[0] - P[-n.L]) * (y.dyci / 1.0) * COE2;
-            else
-                pgm = (P[0] - P[n.km]) * g.dzi * COE2;
-            rhs[c] = r - pgm;
-        }
-        msum += rhs[0] / y.dyfi / 1.0;
+        for (int c = 0; c < 3; ++c) h0[c] = H0[ci + c * LMN];
+        rhs_cell(g, rk, cvisc, rc, U, V, W, P, n, h0, conv, rhs);
+        msum += rhs[0] / rc.y.dyfi / 1.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
@@ -338,6 +380,165 @@ __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, 
         double* dst = RHS + (c * LMN + 4 * g4 + L * (j + g.M * (k0 + r)));
         reinterpret_cast<double2*>(dst)[0] = make_double2(o[0], o[1]);
         reinterpret_cast<double2*>(dst)[1] = make_double2(o[2], o[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs_xm: k_rhs_xw marching along y.  A block owns KRS consecutive k rows
+// of every x line and a chunk of consecutive y rows j0 .. j1-1, one cell of
+// each row per thread (KRS*L threads).  LDS holds a ring of the rows J-1, J,
+// J+1 of u, v, w (KRS + 2 k rows each: the periodic z neighbours as halo)
+// and row J of PR (KRS + 1 k rows), so each value of Q and PR comes from
+// global memory once per block instead of once per stencil use, and the
+// rows of the next step load while the current one is computed.  Per row:
+// RHS of the three components from the ring into the x-solve rows, barrier,
+// the x stencil and the RHS stores, the prefetched rows into the ring,
+// barrier.  The mass-flux partial of row j goes to part[kg*M + j], the slot
+// k_rhs_xw's tile (j, kg) writes.
+// ---------------------------------------------------------------------------
+constexpr int XM_KP = KRS + 2;  // k rows of a ring plane
+
+inline size_t rhs_xm_smem(int L, int KX) {
+    return sizeof(double) * ((size_t)9 * XM_KP * L + (size_t)(KRS + 1) * L + (size_t)3 * KRS * (L + 2 * KX) + 16);
+}
+
+template <int KX>
+__global__ void __launch_bounds__(1024) k_rhs_xm(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
+                                                const double* __restrict__ PR, double* __restrict__ H0,
+                                                double* __restrict__ RHS, double* __restrict__ part,
+                                                const double* __restrict__ wx, int lgL, int nyc, int jc) {
+    extern __shared__ double sm[];
+    constexpr int KR = KRS, KP = XM_KP, W = 2 * KX + 1;
+    const int L = g.L, M = g.M, N = g.N;
+    const int PL = KP * L;            // one component of one ring row
+    const int SL = 3 * PL;            // one ring row (u, v, w)
+    double* ring = sm;                // 3 ring rows
+    double* pln = sm + 3 * SL;        // PR row J, k0-1 .. k0+KR-1
+    const int LS = L + 2 * KX, Q4 = LS >> 2;
+    double* xr = pln + (KR + 1) * L;  // 3*KR x-solve rows (row_pos layout)
+    double* red = xr + 3 * KR * LS;
+    const int nt = (N / KR) * nyc;
+    const int per = (nt + 7) / 8;
+    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD x: tiles x*per .. (x+1)*per-1
+    if (t >= nt) return;
+    const int kg = t / nyc;
+    const int j0 = (t - kg * nyc) * jc, j1 = min(M, j0 + jc);
+    const int k0 = kg * KR;
+    const int tid = threadIdx.x;
+    const int r = tid >> lgL, i = tid & (L - 1);
+    const int k = k0 + r;
+    const int LP = L * (M + 2), CS = LP * N, LMN = L * M * N;
+    // the halo k row this thread also loads: k0-1 (r == 0) or k0+KR (r == KR-1)
+    const bool lo = r == 0, hi = r == KR - 1;
+    const int kh = lo ? (k0 == 0 ? N - 1 : k0 - 1) : (k0 + KR == N ? 0 : k0 + KR);
+    const int go = i + LP * k, gh = i + LP * kh;                // global, row 0
+    const int lo_own = (r + 1) * L + i, lo_h = (lo ? 0 : KP - 1) * L + i;  // in a ring plane
+    Nb n;
+    n.ip = i == L - 1 ? 1 - L : 1;
+    n.im = i == 0 ? L - 1 : -1;
+    n.kp = L;
+    n.km = -L;
+    // fill the ring with rows j0 .. j0+2 and PR row j0+1
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int row = L * (j0 + s);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ring[s * SL + c * PL + lo_own] = Q[go + row + c * CS];
+            if (lo || hi) ring[s * SL + c * PL + lo_h] = Q[gh + row + c * CS];
+        }
+    }
+    pln[lo_own] = PR[go + L * (j0 + 1)];
+    if (lo) pln[i] = PR[gh + L * (j0 + 1)];
+    double pm = PR[go + L * j0];  // own PR at row J-1
+    double h0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) h0[c] = H0[i + L * j0 + L * M * k + c * LMN];
+    double w[W];
+#pragma unroll
+    for (int m = 0; m < W; ++m) w[m] = wx[m];
+    const int lg4 = lgL - 2;
+    const int nw = blockDim.x >> 6;
+    __syncthreads();
+    int sM = 0, s0 = 1, sP = 2;
+    for (int j = j0; j < j1; ++j) {
+        const int J = j + 1;
+        // prefetch: Q row J+2 (clamped on the chunk's last row: loaded, never
+        // used), PR row J+1, CONVH0 row j+1 (clamped likewise)
+        const int rq = L * min(J + 2, M + 1), rp = L * (J + 1), rh = L * min(j + 1, M - 1);
+        double qn[3], qh[3], hn[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            qn[c] = Q[go + rq + c * CS];
+            qh[c] = (lo || hi) ? Q[gh + rq + c * CS] : 0.0;
+            hn[c] = H0[i + rh + L * M * k + c * LMN];
+        }
+        const double pn = PR[go + rp];
+        const double ph = lo ? PR[gh + rp] : 0.0;
+        // RHS of row j from the ring
+        const RowCoef rc = row_coef(g, j);
+        const RCol U{ring + sM * SL + lo_own, ring + s0 * SL + lo_own, ring + sP * SL + lo_own};
+        const RCol V{U.m + PL, U.c + PL, U.p + PL};
+        const RCol Wc{V.m + PL, V.c + PL, V.p + PL};
+        const double pc = pln[lo_own];
+        const RCol P{&pm, pln + lo_own, nullptr};
+        double conv[3], rhs[3];
+        rhs_cell(g, rk, cvisc, rc, U, V, Wc, P, n, h0, conv, rhs);
+        const int ci = i + L * j + L * M * k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
+            double* row = xr + (c * KR + r) * LS;
+            row[row_pos<KX>(KX + i, Q4)] = rhs[c];
+            if (i < KX) row[row_pos<KX>(KX + i + L, Q4)] = rhs[c];
+            if (i >= L - KX) row[row_pos<KX>(KX + i - L, Q4)] = rhs[c];
+        }
+        double v = rhs[0] / rc.y.dyfi / 1.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((tid & 63) == 0) red[tid >> 6] = v;
+        __syncthreads();
+        if (tid == 0) {
+            double bs = 0.0;
+            for (int q = 0; q < nw; ++q) bs += red[q];
+            part[kg * M + j] = bs;
+        }
+        // x stencil of the 3*KR rows, 4 outputs per work item
+        for (int e = tid; e < ((3 * KR) << lg4); e += blockDim.x) {
+            const int cr = e >> lg4, g4 = e & ((1 << lg4) - 1);
+            const int c = cr / KR, rr = cr - c * KR;
+            if (c == 1 && j == 0) continue;
+            const double* xrow = xr + cr * LS + g4;
+            double x[W + 3];
+#pragma unroll
+            for (int tt = 0; tt < W + 3; ++tt) x[tt] = xrow[(tt & 3) * Q4 + (tt >> 2)];
+            double o4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < W; ++m) acc += w[m] * x[m + u];
+                o4[u] = acc;
+            }
+            double* dst = RHS + (c * LMN + 4 * g4 + L * (j + M * (k0 + rr)));
+            reinterpret_cast<double2*>(dst)[0] = make_double2(o4[0], o4[1]);
+            reinterpret_cast<double2*>(dst)[1] = make_double2(o4[2], o4[3]);
+        }
+        // row J+2 replaces row J-1 in the ring, PR row J+1 replaces row J
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ring[sM * SL + c * PL + lo_own] = qn[c];
+            if (lo || hi) ring[sM * SL + c * PL + lo_h] = qh[c];
+            h0[c] = hn[c];
+        }
+        pln[lo_own] = pn;
+        if (lo) pln[i] = ph;
+        pm = pc;
+        __syncthreads();
+        const int s_ = sM;
+        sM = s0;
+        s0 = sP;
+        sP = s_;
     }
 }
 
@@ -992,8 +1193,36 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const int kr = use_st ? krs : KR;
         const int nblk = h->M * (h->N / kr);
         const dim3 grx(((nblk + 7) / 8) * 8);
+        // y-marching form: one cell of each row per thread (KRS*L threads in
+        // whole waves) and its LDS ring on this device
+        const bool use_xm = use_st && h->cn_mode == 0 && h->L >= 16 && KRS * h->L <= 1024 &&
+                            rhs_xm_smem(h->L, cn->KWx) <= (size_t)h->lds_max;
         h->timer.begin("rhs_x", s, &ev);
-        if (use_st) {
+        if (use_xm) {
+            // about one block per CU (the ring takes most of a CU's LDS at
+            // L = 256), chunks of >= 4 rows, none empty
+            const int nkb = h->N / KRS;
+            int nyc = std::max(1, std::min(h->M / 4, (256 + nkb - 1) / nkb));
+            const int jc = (h->M + nyc - 1) / nyc;
+            nyc = (h->M + jc - 1) / jc;
+            const dim3 gm(8 * ((nkb * nyc + 7) / 8));
+            const size_t sm = rhs_xm_smem(h->L, cn->KWx);
+            const double* wx = cn->dev + cn->ox;
+#define CHS_RHSXM(KX)                                                                                  \
+    do {                                                                                               \
+        (void)hipFuncSetAttribute((const void*)k_rhs_xm<KX>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                  (int)sm);                                                            \
+        CHS_LAUNCH(k_rhs_xm<KX>, gm, dim3(KRS * h->L), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0,  \
+                   h->rhs, h->partials, wx, lgL, nyc, jc);                                             \
+    } while (0)
+            switch (cn->KWx) {
+                case 4: CHS_RHSXM(4); break;
+                case 8: CHS_RHSXM(8); break;
+                case 12: CHS_RHSXM(12); break;
+                default: CHS_RHSXM(16); break;
+            }
+#undef CHS_RHSXM
+        } else if (use_st) {
             const size_t sm = sizeof(double) * (3 * (size_t)krs * (size_t)(h->L + 2 * cn->KWx) + 4);
             const double* wx = cn->dev + cn->ox;
 #define CHS_RHSXW(KX, KR) CHS_LAUNCH((k_rhs_xw<KX, KR>), grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL)

@@ -99,8 +99,10 @@ class ChannelSolver:
         return it.value, rel.value
 
     def set_cn_solver(self, mode: str):
-        """'auto' (truncated-inverse stencils where their bound holds) or 'thomas'."""
-        nat.check(self.lib.chs_set_cn_solver(self.h, {"auto": 0, "thomas": 1}[mode]))
+        """'auto' (truncated-inverse stencils where their bound holds), 'thomas',
+        or 'rowtile' ('auto' with the per-row RHS kernel in place of the
+        y-marching one)."""
+        nat.check(self.lib.chs_set_cn_solver(self.h, {"auto": 0, "thomas": 1, "rowtile": 2}[mode]))
 
     def cn_info(self, ns: int, dt: float):
         p, kx, kz, ky = C.c_int(), C.c_int(), C.c_int(), C.c_int()
