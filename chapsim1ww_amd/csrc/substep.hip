@@ -78,14 +78,22 @@ struct YCoef {
     double dyfi, dyci, wf0, wb0, wf1, wb1;
 };
 
+// A read of a 1-D coefficient array, which no kernel writes, through the
+// constant address space: with a uniform index it becomes a scalar load
+// (SGPRs, counted apart from the wave's vector loads, so that using the
+// value does not wait for the loads issued before it).
+__device__ __forceinline__ double ldc(const double* p, int idx) {
+    return ((const __attribute__((address_space(4))) double*)p)[idx];
+}
+
 __device__ __forceinline__ YCoef ycoef(const Geo& g, int j) {
     YCoef y;
-    y.dyfi = g.dyfi[j];
-    y.dyci = g.dyci[j];
-    y.wf0 = g.wff[j];
-    y.wb0 = g.wfb[j];
-    y.wf1 = g.wff[j + 1];
-    y.wb1 = g.wfb[j + 1];
+    y.dyfi = ldc(g.dyfi, j);
+    y.dyci = ldc(g.dyci, j);
+    y.wf0 = ldc(g.wff, j);
+    y.wb0 = ldc(g.wfb, j);
+    y.wf1 = ldc(g.wff, j + 1);
+    y.wb1 = ldc(g.wfb, j + 1);
     return y;
 }
 
@@ -153,9 +161,9 @@ __device__ __forceinline__ RowCoef row_coef(const Geo& g, int j) {
     rc.y = ycoef(g, j);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        rc.ap[c] = g.apvr[c * (g.M + 1) + j];
-        rc.ac[c] = g.acvr[c * (g.M + 1) + j];
-        rc.am[c] = g.amvr[c * (g.M + 1) + j];
+        rc.ap[c] = ldc(g.apvr, c * (g.M + 1) + j);
+        rc.ac[c] = ldc(g.acvr, c * (g.M + 1) + j);
+        rc.am[c] = ldc(g.amvr, c * (g.M + 1) + j);
     }
     return rc;
 }
@@ -384,29 +392,33 @@ __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, 
 }
 
 // ---------------------------------------------------------------------------
-// k_rhs_xm: k_rhs_xw marching along y.  A block owns KRS consecutive k rows
-// of every x line and a chunk of consecutive y rows j0 .. j1-1, one cell of
-// each row per thread (KRS*L threads).  LDS holds a ring of the rows J-1, J,
-// J+1 of u, v, w (KRS + 2 k rows each: the periodic z neighbours as halo)
-// and row J of PR (KRS + 1 k rows), so each value of Q and PR comes from
-// global memory once per block instead of once per stencil use, and the
-// rows of the next step load while the current one is computed.  Per row:
-// RHS of the three components from the ring into the x-solve rows, barrier,
-// the x stencil and the RHS stores, the prefetched rows into the ring,
-// barrier.  The mass-flux partial of row j goes to part[kg*M + j], the slot
-// k_rhs_xw's tile (j, kg) writes.
+// k_rhs_xm: k_rhs_xw marching along y.  A block owns KRS = 4 consecutive k
+// rows of every x line and a chunk of consecutive y rows j0 .. j1-1; thread
+// (rb, i) of its (KRS/CPT)*L threads handles the CPT cells (i, k0 + rb +
+// u*KRS/CPT), u < CPT, of each row.  LDS holds a ring of the rows J-1, J, J+1 of u, v, w (KRS + 2 k
+// rows each: the periodic z neighbours as halo) and row J of PR (KRS + 1 k
+// rows), so each value of Q and PR comes from global memory once per block
+// instead of once per stencil use; the rows a step adds to the ring are
+// loaded PD steps ahead, in registers.  Per row: RHS of the three components
+// from the ring into the x-solve rows, barrier, the x stencil and the RHS
+// stores, the prefetched rows into the ring, barrier.  The mass-flux partial
+// of row j goes to part[kg*M + j], the slot k_rhs_xw's tile (j, kg) writes.
 // ---------------------------------------------------------------------------
 constexpr int XM_KP = KRS + 2;  // k rows of a ring plane
+constexpr int XM_PD = 1;        // steps the ring's next rows are loaded ahead
+constexpr int XM_CPT = 1;       // cells of a row per thread (1 or 2)
 
 inline size_t rhs_xm_smem(int L, int KX) {
     return sizeof(double) * ((size_t)9 * XM_KP * L + (size_t)(KRS + 1) * L + (size_t)3 * KRS * (L + 2 * KX) + 16);
 }
 
-template <int KX>
-__global__ void __launch_bounds__(1024) k_rhs_xm(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
-                                                const double* __restrict__ PR, double* __restrict__ H0,
-                                                double* __restrict__ RHS, double* __restrict__ part,
-                                                const double* __restrict__ wx, int lgL, int nyc, int jc) {
+template <int KX, int PD, int CPT>
+__global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
+                                               const double* __restrict__ PR, double* __restrict__ H0,
+                                               double* __restrict__ RHS, double* __restrict__ part,
+                                               const double* __restrict__ wx, int lgL, int nyc, int jc) {
+    static_assert(KRS % CPT == 0 && KRS / CPT >= 2, "CPT cells per thread, a low and a high halo row");
+    constexpr int RB = KRS / CPT;  // thread rows
     extern __shared__ double sm[];
     constexpr int KR = KRS, KP = XM_KP, W = 2 * KX + 1;
     const int L = g.L, M = g.M, N = g.N;
@@ -425,14 +437,24 @@ __global__ void __launch_bounds__(1024) k_rhs_xm(Geo g, RkCoef rk, double cvisc,
     const int j0 = (t - kg * nyc) * jc, j1 = min(M, j0 + jc);
     const int k0 = kg * KR;
     const int tid = threadIdx.x;
-    const int r = tid >> lgL, i = tid & (L - 1);
-    const int k = k0 + r;
+    const int rb = tid >> lgL, i = tid & (L - 1);
     const int LP = L * (M + 2), CS = LP * N, LMN = L * M * N;
-    // the halo k row this thread also loads: k0-1 (r == 0) or k0+KR (r == KR-1)
-    const bool lo = r == 0, hi = r == KR - 1;
-    const int kh = lo ? (k0 == 0 ? N - 1 : k0 - 1) : (k0 + KR == N ? 0 : k0 + KR);
-    const int go = i + LP * k, gh = i + LP * kh;                // global, row 0
-    const int lo_own = (r + 1) * L + i, lo_h = (lo ? 0 : KP - 1) * L + i;  // in a ring plane
+    // The thread's cells (k rows rb + u*RB) and one halo row: k0-1 (rb == 0),
+    // k0+KR (rb == RB-1), or, for the rows between, its first cell once more
+    // (same value, same place); PR needs only the k0-1 halo and the other
+    // threads move their first cell's PR twice likewise.  So every load and
+    // LDS store is unconditional and a step's loads issue as one batch.
+    const bool lo = rb == 0, hi = rb == RB - 1;
+    int gc[CPT], lc[CPT], kc[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        kc[u] = k0 + rb + RB * u;
+        gc[u] = i + LP * kc[u];             // global, row 0
+        lc[u] = (rb + RB * u + 1) * L + i;  // in a ring plane
+    }
+    const int kh = lo ? (k0 == 0 ? N - 1 : k0 - 1) : hi ? (k0 + KR == N ? 0 : k0 + KR) : kc[0];
+    const int gh = i + LP * kh, lh = lo ? i : hi ? (KP - 1) * L + i : lc[0];
+    const int gph = lo ? gh : gc[0], lph = lo ? i : lc[0];
     Nb n;
     n.ip = i == L - 1 ? 1 - L : 1;
     n.im = i == 0 ? L - 1 : -1;
@@ -444,65 +466,125 @@ __global__ void __launch_bounds__(1024) k_rhs_xm(Geo g, RkCoef rk, double cvisc,
         const int row = L * (j0 + s);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            ring[s * SL + c * PL + lo_own] = Q[go + row + c * CS];
-            if (lo || hi) ring[s * SL + c * PL + lo_h] = Q[gh + row + c * CS];
+            double a[CPT];
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) a[u] = Q[gc[u] + row + c * CS];
+            const double ah = Q[gh + row + c * CS];
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) ring[s * SL + c * PL + lc[u]] = a[u];
+            ring[s * SL + c * PL + lh] = ah;
         }
     }
-    pln[lo_own] = PR[go + L * (j0 + 1)];
-    if (lo) pln[i] = PR[gh + L * (j0 + 1)];
-    double pm = PR[go + L * j0];  // own PR at row J-1
-    double h0[3];
+    {
+        const int row = L * (j0 + 1);
+        double a[CPT];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) h0[c] = H0[i + L * j0 + L * M * k + c * LMN];
-    double w[W];
+        for (int u = 0; u < CPT; ++u) a[u] = PR[gc[u] + row];
+        const double ah = PR[gph + row];
 #pragma unroll
-    for (int m = 0; m < W; ++m) w[m] = wx[m];
+        for (int u = 0; u < CPT; ++u) pln[lc[u]] = a[u];
+        pln[lph] = ah;
+    }
+    double pm[CPT], h0[CPT][3];  // own PR at row J-1, own CONVH0 of row j
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        pm[u] = PR[gc[u] + L * j0];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h0[u][c] = H0[i + L * j0 + L * M * kc[u] + c * LMN];
+    }
     const int lg4 = lgL - 2;
     const int nw = blockDim.x >> 6;
+    // Nothing loaded above may still be in flight when the loop starts:
+    // otherwise every step would open with a wait for all outstanding memory
+    // operations (the previous step's stores included).  s_waitcnt vmcnt(0).
+    __builtin_amdgcn_s_waitcnt(0xF70);
     __syncthreads();
+    // What step j puts into the ring at its end (Q row J+2, PR row J+1) and
+    // the CONVH0 row j+1 of the next step.  Rows past the chunk's end are
+    // clamped into the array: loaded, never used.
+    struct Pre {
+        double q[CPT][3], qh[3], h[CPT][3], p[CPT], ph;
+    };
+    // CONVH0 is loaded first: the wait for the Q rows at the ring store then
+    // covers it, so that using it in the next step waits for nothing (the
+    // vector memory counter is in order: a later wait would also wait for
+    // the stores issued since).
+    auto load_pre = [&](int jj, Pre& x) {
+        const int rq = L * min(jj + 3, M + 1), rp = L * min(jj + 2, M + 1), rh = L * min(jj + 1, M - 1);
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x.h[u][c] = H0[i + rh + L * M * kc[u] + c * LMN];
+            x.p[u] = PR[gc[u] + rp];
+        }
+        x.ph = PR[gph + rp];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) x.q[u][c] = Q[gc[u] + rq + c * CS];
+            x.qh[c] = Q[gh + rq + c * CS];
+        }
+    };
+    Pre pa;
+    if (PD == 2) load_pre(j0, pa);
     int sM = 0, s0 = 1, sP = 2;
     for (int j = j0; j < j1; ++j) {
-        const int J = j + 1;
-        // prefetch: Q row J+2 (clamped on the chunk's last row: loaded, never
-        // used), PR row J+1, CONVH0 row j+1 (clamped likewise)
-        const int rq = L * min(J + 2, M + 1), rp = L * (J + 1), rh = L * min(j + 1, M - 1);
-        double qn[3], qh[3], hn[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            qn[c] = Q[go + rq + c * CS];
-            qh[c] = (lo || hi) ? Q[gh + rq + c * CS] : 0.0;
-            hn[c] = H0[i + rh + L * M * k + c * LMN];
-        }
-        const double pn = PR[go + rp];
-        const double ph = lo ? PR[gh + rp] : 0.0;
+        Pre pb;
+        if (PD == 2)
+            load_pre(j + 1, pb);
+        else
+            load_pre(j, pa);
         // RHS of row j from the ring
         const RowCoef rc = row_coef(g, j);
-        const RCol U{ring + sM * SL + lo_own, ring + s0 * SL + lo_own, ring + sP * SL + lo_own};
-        const RCol V{U.m + PL, U.c + PL, U.p + PL};
-        const RCol Wc{V.m + PL, V.c + PL, V.p + PL};
-        const double pc = pln[lo_own];
-        const RCol P{&pm, pln + lo_own, nullptr};
-        double conv[3], rhs[3];
-        rhs_cell(g, rk, cvisc, rc, U, V, Wc, P, n, h0, conv, rhs);
-        const int ci = i + L * j + L * M * k;
+        double v = 0.0, pc[CPT];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
-            double* row = xr + (c * KR + r) * LS;
-            row[row_pos<KX>(KX + i, Q4)] = rhs[c];
-            if (i < KX) row[row_pos<KX>(KX + i + L, Q4)] = rhs[c];
-            if (i >= L - KX) row[row_pos<KX>(KX + i - L, Q4)] = rhs[c];
+        for (int u = 0; u < CPT; ++u) {
+            const int o = lc[u];
+            const RCol U{ring + sM * SL + o, ring + s0 * SL + o, ring + sP * SL + o};
+            const RCol V{U.m + PL, U.c + PL, U.p + PL};
+            const RCol Wc{V.m + PL, V.c + PL, V.p + PL};
+            pc[u] = pln[o];
+            const RCol P{&pm[u], pln + o, nullptr};
+            double conv[3], rhs[3];
+            rhs_cell(g, rk, cvisc, rc, U, V, Wc, P, n, h0[u], conv, rhs);
+            const int ci = i + L * j + L * M * kc[u];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
+                double* row = xr + (c * KR + rb + RB * u) * LS;
+                row[row_pos<KX>(KX + i, Q4)] = rhs[c];
+                if (i < KX) row[row_pos<KX>(KX + i + L, Q4)] = rhs[c];
+                if (i >= L - KX) row[row_pos<KX>(KX + i - L, Q4)] = rhs[c];
+            }
+            v += rhs[0] / rc.y.dyfi / 1.0;
         }
-        double v = rhs[0] / rc.y.dyfi / 1.0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
         if ((tid & 63) == 0) red[tid >> 6] = v;
         __syncthreads();
-        if (tid == 0) {
-            double bs = 0.0;
-            for (int q = 0; q < nw; ++q) bs += red[q];
-            part[kg * M + j] = bs;
+        if (tid < 64) {  // wave 0: the wave sums by a shuffle tree
+            double bs = tid < nw ? red[tid] : 0.0;
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) bs += __shfl_xor(bs, o);
+            if (tid == 0) part[kg * M + j] = bs;
         }
+        // row J+2 replaces row J-1 in the ring, PR row J+1 replaces row J
+        // (both free since the barrier), before the RHS stores are issued
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                ring[sM * SL + c * PL + lc[u]] = pa.q[u][c];
+                h0[u][c] = pa.h[u][c];
+            }
+            ring[sM * SL + c * PL + lh] = pa.qh[c];
+        }
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            pln[lc[u]] = pa.p[u];
+            pm[u] = pc[u];
+        }
+        pln[lph] = pa.ph;
         // x stencil of the 3*KR rows, 4 outputs per work item
         for (int e = tid; e < ((3 * KR) << lg4); e += blockDim.x) {
             const int cr = e >> lg4, g4 = e & ((1 << lg4) - 1);
@@ -517,23 +599,14 @@ __global__ void __launch_bounds__(1024) k_rhs_xm(Geo g, RkCoef rk, double cvisc,
             for (int u = 0; u < 4; ++u) {
                 double acc = 0.0;
 #pragma unroll
-                for (int m = 0; m < W; ++m) acc += w[m] * x[m + u];
+                for (int m = 0; m < W; ++m) acc += ldc(wx, m) * x[m + u];
                 o4[u] = acc;
             }
             double* dst = RHS + (c * LMN + 4 * g4 + L * (j + M * (k0 + rr)));
             reinterpret_cast<double2*>(dst)[0] = make_double2(o4[0], o4[1]);
             reinterpret_cast<double2*>(dst)[1] = make_double2(o4[2], o4[3]);
         }
-        // row J+2 replaces row J-1 in the ring, PR row J+1 replaces row J
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            ring[sM * SL + c * PL + lo_own] = qn[c];
-            if (lo || hi) ring[sM * SL + c * PL + lo_h] = qh[c];
-            h0[c] = hn[c];
-        }
-        pln[lo_own] = pn;
-        if (lo) pln[i] = ph;
-        pm = pc;
+        if (PD == 2) pa = pb;
         __syncthreads();
         const int s_ = sM;
         sM = s0;
@@ -1193,9 +1266,10 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const int kr = use_st ? krs : KR;
         const int nblk = h->M * (h->N / kr);
         const dim3 grx(((nblk + 7) / 8) * 8);
-        // y-marching form: one cell of each row per thread (KRS*L threads in
-        // whole waves) and its LDS ring on this device
-        const bool use_xm = use_st && h->cn_mode == 0 && h->L >= 16 && KRS * h->L <= 1024 &&
+        // y-marching form: XM_CPT cells of each row per thread (whole waves)
+        // and its LDS ring on this device
+        const bool use_xm = use_st && h->cn_mode == 0 && (KRS / XM_CPT) * h->L >= 64 &&
+                            (KRS / XM_CPT) * h->L <= (XM_CPT == 1 ? 1024 : 512) &&
                             rhs_xm_smem(h->L, cn->KWx) <= (size_t)h->lds_max;
         h->timer.begin("rhs_x", s, &ev);
         if (use_xm) {
@@ -1210,9 +1284,9 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
             const double* wx = cn->dev + cn->ox;
 #define CHS_RHSXM(KX)                                                                                  \
     do {                                                                                               \
-        (void)hipFuncSetAttribute((const void*)k_rhs_xm<KX>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+        (void)hipFuncSetAttribute((const void*)k_rhs_xm<KX, XM_PD, XM_CPT>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                   (int)sm);                                                            \
-        CHS_LAUNCH(k_rhs_xm<KX>, gm, dim3(KRS * h->L), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0,  \
+        CHS_LAUNCH((k_rhs_xm<KX, XM_PD, XM_CPT>), gm, dim3((KRS / XM_CPT) * h->L), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0,  \
                    h->rhs, h->partials, wx, lgL, nyc, jc);                                             \
     } while (0)
             switch (cn->KWx) {
