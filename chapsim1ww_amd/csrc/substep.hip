@@ -758,7 +758,8 @@ constexpr int KMAX = 16;
 // (64 consecutive i) applies the same row weights W_c[j][.] (scalar loads).
 // wy rows are zero-padded to 2KW+1 entries (zero outside [0, M)).
 constexpr int YJC = 64;     // rows per thread chunk of the y window (measured: 32 -> 64 rows, 257 -> 240 us)
-constexpr int WU = 4;       // rows fetched ahead per step of the windows
+constexpr int WUY = 2;      // rows fetched ahead by the y window (measured at 256x192x256: 4 -> 2, 240 -> 214 us)
+constexpr int WU = 4;       // rows fetched ahead per step of the z window (2: 140 -> 153 us)
 template <int KW>
 __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
                                                   const double* __restrict__ wy) {
@@ -794,10 +795,10 @@ __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restric
             if (jj >= 0 && jj < M) v = (c == 1 && jj == 0) ? du0 : Rc[(size_t)jj * L];
             win[m] = v;
         }
-        // prefetch queues: the window rows and the Q rows WU steps ahead
-        double rq[WU], qq[WU];
+        // prefetch queues: the window rows and the Q rows WUY steps ahead
+        double rq[WUY], qq[WUY];
 #pragma unroll
-        for (int u = 0; u < WU; ++u) {
+        for (int u = 0; u < WUY; ++u) {
             const int jn = j0 + u + KW + 1;
             rq[u] = jn < M ? Rc[(size_t)jn * L] : 0.0;
             qq[u] = j0 + u < j1 ? Q[qidx(g, c, i, j0 + u + 1, k)] : 0.0;
@@ -812,13 +813,13 @@ __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restric
             for (int m = 0; m < W - 1; ++m) win[m] = win[m + 1];
             win[W - 1] = rq[0];
 #pragma unroll
-            for (int u = 0; u < WU - 1; ++u) {
+            for (int u = 0; u < WUY - 1; ++u) {
                 rq[u] = rq[u + 1];
                 qq[u] = qq[u + 1];
             }
-            const int jn = j + WU + KW + 1, jq = j + WU;
-            rq[WU - 1] = jn < M ? Rc[(size_t)jn * L] : 0.0;
-            qq[WU - 1] = jq < j1 ? Q[qidx(g, c, i, jq + 1, k)] : 0.0;
+            const int jn = j + WUY + KW + 1, jq = j + WUY;
+            rq[WUY - 1] = jn < M ? Rc[(size_t)jn * L] : 0.0;
+            qq[WUY - 1] = jq < j1 ? Q[qidx(g, c, i, jq + 1, k)] : 0.0;
         }
         if (ch == 0) {
             Q[qidx(g, c, i, 0, k)] = 0.0;
@@ -833,6 +834,7 @@ __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restric
 // u subtracts D = DPGRNS * xone on load (the mass-flux correction pushed
 // through the x solve).  Reads RHS, writes RHS2.
 constexpr int ZKC = 64;
+static_assert(ZKC % WU == 0, "k_z_window steps a chunk WU rows at a time: a partial step would run past the chunk");
 template <int KW>
 __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restrict__ R, double* __restrict__ R2,
                                                   const double* __restrict__ dpg, double xone, int use_dpg,
@@ -885,46 +887,51 @@ __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restric
     }
 }
 
-// DIVG_tg (src/DIVG.f90:21-57): f = div(Q) / (DT * TALP)
+// DIVG_tg (src/DIVG.f90:21-57): f = div(Q) / (DT * TALP).  One thread per
+// cell in row tiles of 64 i x 4 k (row_tile: each XCD walks its tiles in j
+// order, so the J+1 row of v is in its L2), 32-bit offsets.
 __global__ void __launch_bounds__(256) k_divg(Geo g, double coe0, const double* __restrict__ Q, double* __restrict__ F) {
-    const size_t ncell = (size_t)g.L * g.M * g.N;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (size_t)gridDim.x * blockDim.x) {
-        const int i = t % g.L;
-        const int j = (t / g.L) % g.M;
-        const int k = t / ((size_t)g.L * g.M);
-        const int ip = i == g.L - 1 ? 0 : i + 1, kp = k == g.N - 1 ? 0 : k + 1;
-        const int J = j + 1;
-        const double COE1 = g.dxi / 1.0, COE2 = g.dyfi[j] / 1.0;
-        const double D = (Q[qidx(g, 0, ip, J, k)] - Q[qidx(g, 0, i, J, k)]) * COE1 +
-                         (Q[qidx(g, 1, i, J + 1, k)] - Q[qidx(g, 1, i, J, k)]) * COE2 +
-                         (Q[qidx(g, 2, i, J, kp)] - Q[qidx(g, 2, i, J, k)]) * g.dzi;
-        F[t] = D * coe0;
-    }
+    int it, j, kt, tile;
+    if (!row_tile(g.L, g.N, g.M, &it, &j, &kt, &tile)) return;
+    const int L = g.L, N = g.N;
+    const int i = it * 64 + (threadIdx.x & 63), k = kt * 4 + (threadIdx.x >> 6);
+    if (i >= L || k >= N) return;
+    const int LP = L * (g.M + 2), CS = LP * N;
+    const int ip = i == L - 1 ? 1 - L : 1, kp = k == N - 1 ? (1 - N) * LP : LP;
+    const double* U = Q + (i + L * (j + 1) + LP * k);
+    const double* V = U + CS;
+    const double* W = V + CS;
+    const double COE1 = g.dxi / 1.0, COE2 = ldc(g.dyfi, j) / 1.0;
+    const double D = (U[ip] - U[0]) * COE1 + (V[L] - V[0]) * COE2 + (W[kp] - W[0]) * g.dzi;
+    F[i + L * (j + g.M * k)] = D * coe0;
 }
 
 // VELOUPDT_tg (src/VELO_CORRECTION.f90:37-75) + PRCALC_tg
-// (src/PRESSURE_CORRECTION.f90:11-20) + BC_WALL_Q_tg / BC_WALL_PR_tg.
+// (src/PRESSURE_CORRECTION.f90:11-20) + BC_WALL_Q_tg / BC_WALL_PR_tg, in the
+// row tiles of k_divg.  Every load is issued before any store (v's and the
+// ghost rows' conditions are uniform over a tile).
 __global__ void __launch_bounds__(256) k_update(Geo g, double cu, double cw, double cv, double be,
                                                 const double* __restrict__ DPH, const double* __restrict__ F,
                                                 double* __restrict__ Q, double* __restrict__ PR) {
-    const size_t ncell = (size_t)g.L * g.M * g.N;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (size_t)gridDim.x * blockDim.x) {
-        const int i = t % g.L;
-        const int j = (t / g.L) % g.M;
-        const int k = t / ((size_t)g.L * g.M);
-        const int im = i == 0 ? g.L - 1 : i - 1, km = k == 0 ? g.N - 1 : k - 1;
-        const int J = j + 1;
-        const double ph = DPH[gidx(g, i, J, k)];
-        const size_t qu = qidx(g, 0, i, J, k), qv = qidx(g, 1, i, J, k), qw = qidx(g, 2, i, J, k);
-        Q[qu] = Q[qu] - (ph - DPH[gidx(g, im, J, k)]) * cu;
-        Q[qw] = Q[qw] - (ph - DPH[gidx(g, i, J, km)]) * cw;
-        if (j >= 1) Q[qv] = Q[qv] - (ph - DPH[gidx(g, i, J - 1, k)]) * (cv * g.dyci[j] / 1.0);
-        const size_t po = gidx(g, i, J, k);
-        const double p = PR[po] + ph - be * F[t];
-        PR[po] = p;
-        if (j == 0) PR[gidx(g, i, 0, k)] = p;
-        if (j == g.M - 1) PR[gidx(g, i, g.M + 1, k)] = p;
-    }
+    int it, j, kt, tile;
+    if (!row_tile(g.L, g.N, g.M, &it, &j, &kt, &tile)) return;
+    const int L = g.L, M = g.M, N = g.N;
+    const int i = it * 64 + (threadIdx.x & 63), k = kt * 4 + (threadIdx.x >> 6);
+    if (i >= L || k >= N) return;
+    const int LP = L * (M + 2), CS = LP * N;
+    const int im = i == 0 ? L - 1 : -1, km = k == 0 ? (N - 1) * LP : -LP;
+    const int go = i + L * (j + 1) + LP * k;  // (i, J, k); J-1 >= 0 is always in the array
+    const double ph = DPH[go], dxm = DPH[go + im], dkm = DPH[go + km], djm = DPH[go - L];
+    const double qu = Q[go], qv = Q[go + CS], qw = Q[go + 2 * CS];
+    const double f = F[i + L * (j + M * k)], pr = PR[go];
+    const double cvj = cv * ldc(g.dyci, j) / 1.0;
+    Q[go] = qu - (ph - dxm) * cu;
+    Q[go + 2 * CS] = qw - (ph - dkm) * cw;
+    if (j >= 1) Q[go + CS] = qv - (ph - djm) * cvj;
+    const double p = pr + ph - be * f;
+    PR[go] = p;
+    if (j == 0) PR[go - L] = p;
+    if (j == M - 1) PR[go + L] = p;
 }
 
 __device__ __forceinline__ double nanmax(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
@@ -1398,16 +1405,16 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         h->timer.end(ev, "y_solve", s);
     }
     // 4. DIVG
-    const int nbs = (int)std::min<size_t>((ncell + 255) / 256, 8192);
+    const dim3 gt(row_tile_blocks(h->L, h->N, h->M));
     h->timer.begin("divg", s, &ev);
-    CHS_LAUNCH(k_divg, dim3(nbs), dim3(256), 0, s, g, 1.0 / (dt * rk.talp), h->q, h->rhsllphi);
+    CHS_LAUNCH(k_divg, gt, dim3(256), 0, s, g, 1.0 / (dt * rk.talp), h->q, h->rhsllphi);
     h->timer.end(ev, "divg", s);
     // 5. Poisson + BC_WALL_DPH; in the mixed form refine until the projected
     //    divergence alpha*dt*|f - L phi| is below half the 1e-10 target
     CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (rk.talp * dt)));
     // 6. projection + pressure update
     h->timer.begin("update", s, &ev);
-    CHS_LAUNCH(k_update, dim3(nbs), dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
+    CHS_LAUNCH(k_update, gt, dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
                0.5 * rk.talp * dt * h->cvisc, h->dph, h->rhsllphi, h->q, h->pr);
     h->timer.end(ev, "update", s);
     return CHS_OK;
