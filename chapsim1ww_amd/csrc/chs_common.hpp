@@ -95,6 +95,14 @@ template <> __device__ inline double2 vzero<double2>() { return make_double2(0.0
 // round-robin over the XCDs.  Remap so that each XCD walks a contiguous range
 // of tiles: neighbouring tiles (which share halo rows) then share an L2.
 // ---------------------------------------------------------------------------
+// A read of a 1-D coefficient array, which no kernel writes, through the
+// constant address space: with a uniform index it becomes a scalar load
+// (SGPRs, counted apart from the wave's vector loads, so that using the
+// value does not wait for the loads issued before it).
+__device__ __forceinline__ double ldc(const double* p, int idx) {
+    return ((const __attribute__((address_space(4))) double*)p)[idx];
+}
+
 __device__ inline int xcd_tile(int bid, int nblk) {
     const int nx = 8;
     int per = (nblk + nx - 1) / nx;

@@ -55,22 +55,22 @@ __global__ void __launch_bounds__(256) k_energy_rhs(EGeo g, EFields f, double rk
     auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
     const double COE1 = g.dxi * 0.5 * 0.5;
     const double COE11 = g.dxqi * 0.5 * cthecd;
-    const double COE2 = g.dyfi[JJ - 1] * 1.0 * 0.5;
+    const double COE2 = ldc(g.dyfi, JJ - 1) * 1.0 * 0.5;
     const double COE3 = g.dzi * 0.5 * 1.0 * 0.5;
     const double COE33 = g.dzqi * 0.5 * 1.0 * cthecd;
-    const double COE22 = g.dyfi[JJ - 1] * 1.0 * cthecd;
+    const double COE22 = ldc(g.dyfi, JJ - 1) * 1.0 * cthecd;
     double COE221, COE222;
     if (JJ == 1) {
-        COE221 = g.dyci[JJP - 1] / 1.0;
+        COE221 = ldc(g.dyci, JJP - 1) / 1.0;
         COE222 = 1.0 / 1.0;
     } else if (JJ == M) {
-        COE222 = g.dyci[JJ - 1] / 1.0;
+        COE222 = ldc(g.dyci, JJ - 1) / 1.0;
         COE221 = 1.0 / 1.0;
     } else {
-        COE221 = g.dyci[JJP - 1] / 1.0;
-        COE222 = g.dyci[JJ - 1] / 1.0;
+        COE221 = ldc(g.dyci, JJP - 1) / 1.0;
+        COE222 = ldc(g.dyci, JJ - 1) / 1.0;
     }
-    const double wffp = g.wff[JJP - 1], wfbp = g.wfb[JJP - 1], wff = g.wff[JJ - 1], wfb = g.wfb[JJ - 1];
+    const double wffp = ldc(g.wff, JJP - 1), wfbp = ldc(g.wfb, JJP - 1), wff = ldc(g.wff, JJ - 1), wfb = ldc(g.wfb, JJ - 1);
     const double* H = f.h;
     const double* TT = f.T;
     const double* KK = f.kap;
@@ -177,10 +177,10 @@ __global__ void __launch_bounds__(256) k_energy_wall(PropTab t, EGeo g, EFields 
         if (top) {
             const int J = M + 1;
             q = base + (long long)L * J;
-            hv = 2.0 * f.h[q - L] - g.wff[J - 2] * f.h[q - L] - g.wfb[J - 2] * f.h[q - 2 * L];
+            hv = 2.0 * f.h[q - L] - ldc(g.wff, J - 2) * f.h[q - L] - ldc(g.wfb, J - 2) * f.h[q - 2 * L];
         } else {
             q = base;
-            hv = 2.0 * f.h[q + L] - g.wff[1] * f.h[q + 2 * L] - g.wfb[1] * f.h[q + L];
+            hv = 2.0 * f.h[q + L] - ldc(g.wff, 1) * f.h[q + 2 * L] - ldc(g.wfb, 1) * f.h[q + L];
         }
         f.h[q] = hv;
         double Tt, D, Mu, K, CP;

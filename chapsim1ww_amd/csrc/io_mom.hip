@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __res
         double dr;
         {
             const int Jd = J < 1 ? 1 : (J > M ? M : J), cd0 = at(IC, Jd, KC);
-            const double COE2 = g.dyfi[Jd - 1] * 1.0, COE3 = g.dzi * 1.0;
+            const double COE2 = ldc(g.dyfi, Jd - 1) * 1.0, COE3 = g.dzi * 1.0;
             const double DIVX = (G[at(IP, Jd, KC)] - G[cd0]) * g.dxi;
             const double DIVX0 = (G0[at(IP, Jd, KC)] - G0[cd0]) * g.dxi;
             const double DIVY = (G[CS + at(IC, Jd + 1, KC)] - G[CS + cd0]) * COE2;
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __res
         const double r2w = 0.5 * (D0[at(IC, Jg, KC)] + D[at(IC, Jg, KC)]);
         const double m1w = 0.5 * (V0[at(IC, Jg, KC)] + V[at(IC, Jg, KC)]);
         const int Jr = J == 0 ? M : J, Jm = Jr - 1;
-        const double wff = g.wff[Jr - 1], wfb = g.wfb[Jr - 1];
+        const double wff = ldc(g.wff, Jr - 1), wfb = ldc(g.wfb, Jr - 1);
         const double r2i =
             (wff * (D[at(IC, Jr, KC)] + D0[at(IC, Jr, KC)]) + wfb * (D[at(IC, Jm, KC)] + D0[at(IC, Jm, KC)])) * 0.5;
         const double m1i = ((V[at(IC, Jr, KC)] + V[at(IM, Jr, KC)] + V0[at(IC, Jr, KC)] + V0[at(IM, Jr, KC)]) * wff +
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __res
             const double d1 = 1.0 / r1 - 1.0 / ((D0[at(IC, Jw, KC)] + D0[at(IM, Jw, KC)]) * 0.5);
             const double d3 = 1.0 / r3 - 1.0 / ((D0[at(IC, Jw, KC)] + D0[at(IC, Jw, KM)]) * 0.5);
             const double d2 = 1.0 / r2i -
-                              1.0 / (g.wff[Jr2 - 1] * D0[at(IC, Jr2, KC)] + g.wfb[Jr2 - 1] * D0[at(IC, Jr2 - 1, KC)]);
+                              1.0 / (ldc(g.wff, Jr2 - 1) * D0[at(IC, Jr2, KC)] + ldc(g.wfb, Jr2 - 1) * D0[at(IC, Jr2 - 1, KC)]);
             drhoi[c] = d1;
             drhoi[CS + c] = wall ? 0.0 : d2;
             drhoi[2 * CS + c] = d3;
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_io_divu(IoGeo g, const double* __restri
     const int Jr = J == 0 ? M : (J == M + 1 ? 1 : J);
     const int c = at(IC, Jr, KC);
     const double DIVX = (Q[at(IP, Jr, KC)] - Q[c]) * g.dxi;
-    const double DIVY = (Q[CS + at(IC, Jr + 1, KC)] - Q[CS + c]) * (g.dyfi[Jr - 1] * 1.0);
+    const double DIVY = (Q[CS + at(IC, Jr + 1, KC)] - Q[CS + c]) * (ldc(g.dyfi, Jr - 1) * 1.0);
     const double DIVZ = (Q[2 * CS + at(IC, Jr, KP)] - Q[2 * CS + c]) * (g.dzi * 1.0);
     divu[at(IC, J, KC)] = (DIVX + DIVY + DIVZ) / 3.0;
 }
@@ -214,12 +214,12 @@ __device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict
     const int LP = L * (M + 2), CS = LP * N;
     auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
     const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1;
-    const double wff = g.wff[JJ - 1], wfb = g.wfb[JJ - 1], wffp = g.wff[JJP - 1], wfbp = g.wfb[JJP - 1];
+    const double wff = ldc(g.wff, JJ - 1), wfb = ldc(g.wfb, JJ - 1), wffp = ldc(g.wff, JJP - 1), wfbp = ldc(g.wfb, JJP - 1);
     const double *G1 = G, *G2 = G + CS, *G3 = G + 2 * CS;
     const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
     const int c = at(IC, JC, KC);
     {
-        const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5 * 1.0;
+        const double COE1 = g.dxi * 0.5 * 0.5, COE2 = ldc(g.dyfi, JJ - 1) * 0.5 * 1.0;
         const double COE32 = g.dzi * 0.5 * 0.5 * 1.0;
         const double H11F = (G1[at(IP, JC, KC)] + G1[c]) * (Q1[at(IP, JC, KC)] + Q1[c]);
         const double H11B = (G1[c] + G1[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IM, JC, KC)]);
@@ -233,7 +233,7 @@ __device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict
         hx = -(H11 + H12 + H13);
     }
     {
-        const double COE1 = g.dxi * 0.5, COE2 = g.dyci[JJ - 1] * 0.5 * 0.5, COE31 = g.dzi * 0.5 * 1.0;
+        const double COE1 = g.dxi * 0.5, COE2 = ldc(g.dyci, JJ - 1) * 0.5 * 0.5, COE31 = g.dzi * 0.5 * 1.0;
         const double H21F = (wff * G1[at(IP, JC, KC)] + wfb * G1[at(IP, JM, KC)]) * (Q2[at(IP, JC, KC)] + Q2[c]);
         const double H21B = (wff * G1[c] + wfb * G1[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IM, JC, KC)]);
         const double H21 = (H21F - H21B) * COE1;
@@ -246,7 +246,7 @@ __device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict
         hy = -(H21 + H22 + H23);
     }
     {
-        const double COE1 = g.dxi * 0.5 * 0.5, COE2 = g.dyfi[JJ - 1] * 0.5;
+        const double COE1 = g.dxi * 0.5 * 0.5, COE2 = ldc(g.dyfi, JJ - 1) * 0.5;
         const double COE31 = g.dzi * 0.5 * 0.5 * 1.0;
         const double H31F = (G1[at(IP, JC, KC)] + G1[at(IP, JC, KM)]) * (Q3[at(IP, JC, KC)] + Q3[c]);
         const double H31B = (G1[c] + G1[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IM, JC, KC)]);
@@ -301,17 +301,17 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         double hy, hz;
         io_conv(g, G, Q, IC, IP, IM, JC, KC, KP, KM, hx, hy, hz);
         {
-            const double COE1 = 2.0 * g.dxi * CV, COE2 = g.dyfi[JJ - 1] * CV * 1.0, COE32 = g.dzi * CV * 1.0;
+            const double COE1 = 2.0 * g.dxi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV * 1.0, COE32 = g.dzi * CV * 1.0;
             double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
             const double TAU11F = 0.5 * (MU0[c] + MU[c]) * (DUDX - DU[c]);
             DUDX = (Q1[c] - Q1[at(IM, JC, KC)]) * g.dxi;
             const int cim = at(IM, JC, KC);
             const double TAU11B = 0.5 * (MU0[cim] + MU[cim]) * (DUDX - DU[cim]);
             const double DTAU11DX = (TAU11F - TAU11B) * COE1;
-            double DUDY = (Q1[at(IC, JP, KC)] - Q1[c]) * g.dyci[JJP - 1] / 1.0;
+            double DUDY = (Q1[at(IC, JP, KC)] - Q1[c]) * ldc(g.dyci, JJP - 1) / 1.0;
             double DVDX = (Q2[at(IC, JP, KC)] - Q2[at(IM, JP, KC)]) * g.dxi;
             const double TAU12F = MS1[at(IC, JP, KC)] * (DUDY + DVDX);
-            DUDY = (Q1[c] - Q1[at(IC, JM, KC)]) * g.dyci[JJ - 1] / 1.0;
+            DUDY = (Q1[c] - Q1[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1) / 1.0;
             DVDX = (Q2[c] - Q2[cim]) * g.dxi;
             const double TAU12B = MS1[c] * (DUDY + DVDX);
             const double DTAU12DY = (TAU12F - TAU12B) * COE2;
@@ -327,25 +327,25 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         // ---- y (rows 2..M): VISCOUS_ALL_EXPLT_Y_io ----
         {
             {
-                const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = 2.0 * g.dyci[JJ - 1] * CV;
+                const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = 2.0 * ldc(g.dyci, JJ - 1) * CV;
                 const int cjm = at(IC, JM, KC);
-                double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1];
+                double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * ldc(g.dyfi, JJ - 1);
                 const double TAU22F = 0.5 * (MU0[c] + MU[c]) * (DVDY - DU[c]);
-                DVDY = (Q2[c] - Q2[cjm]) * g.dyfi[JJM - 1];
+                DVDY = (Q2[c] - Q2[cjm]) * ldc(g.dyfi, JJM - 1);
                 const double TAU22B = 0.5 * (MU0[cjm] + MU[cjm]) * (DVDY - DU[cjm]);
                 const double DTAU22DY = (TAU22F - TAU22B) * COE2;
                 double DVDX = (Q2[at(IP, JC, KC)] - Q2[c]) * g.dxi;
-                double DUDY = (Q1[at(IP, JC, KC)] - Q1[at(IP, JM, KC)]) * g.dyci[JJ - 1];
+                double DUDY = (Q1[at(IP, JC, KC)] - Q1[at(IP, JM, KC)]) * ldc(g.dyci, JJ - 1);
                 const double TAU21F = MS1[at(IP, JC, KC)] * (DVDX + DUDY);
                 DVDX = (Q2[c] - Q2[at(IM, JC, KC)]) * g.dxi;
-                DUDY = (Q1[c] - Q1[cjm]) * g.dyci[JJ - 1];
+                DUDY = (Q1[c] - Q1[cjm]) * ldc(g.dyci, JJ - 1);
                 const double TAU21B = MS1[c] * (DVDX + DUDY);
                 const double DTAU21DX = (TAU21F - TAU21B) * COE1;
                 double DVDZ = (Q2[at(IC, JC, KP)] - Q2[c]) * g.dzi;
-                double DWDY = (Q3[at(IC, JC, KP)] - Q3[at(IC, JM, KP)]) * g.dyci[JJ - 1];
+                double DWDY = (Q3[at(IC, JC, KP)] - Q3[at(IC, JM, KP)]) * ldc(g.dyci, JJ - 1);
                 const double TAU23F = MS3[at(IC, JC, KP)] * (DVDZ + DWDY);
                 DVDZ = (Q2[c] - Q2[at(IC, JC, KM)]) * g.dzi;
-                DWDY = (Q3[c] - Q3[cjm]) * g.dyci[JJ - 1];
+                DWDY = (Q3[c] - Q3[cjm]) * ldc(g.dyci, JJ - 1);
                 const double TAU23B = MS3[c] * (DVDZ + DWDY);
                 const double DTAU23DZ = (TAU23F - TAU23B) * COE3;
                 hy = hy + DTAU21DX + DTAU22DY + DTAU23DZ;
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         }
         // ---- z: VISCOUS_ALL_EXPLT_Z_io ----
         {
-            const double COE1 = g.dxi * CV, COE3 = 2.0 * g.dzi * CV, COE2 = g.dyfi[JJ - 1] * CV;
+            const double COE1 = g.dxi * CV, COE3 = 2.0 * g.dzi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV;
             const int ckm = at(IC, JC, KM);
             double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi;
             const double TAU33F = 0.5 * (MU0[c] + MU[c]) * (DWDZ - DU[c]);
@@ -367,10 +367,10 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
             DUDZ = (Q1[c] - Q1[ckm]) * g.dzi;
             const double TAU31B = MS2[c] * (DWDX + DUDZ);
             const double DTAU31DX = (TAU31F - TAU31B) * COE1;
-            double DWDY = (Q3[at(IC, JP, KC)] - Q3[c]) * g.dyci[JJP - 1];
+            double DWDY = (Q3[at(IC, JP, KC)] - Q3[c]) * ldc(g.dyci, JJP - 1);
             double DVDZ = (Q2[at(IC, JP, KC)] - Q2[at(IC, JP, KM)]) * g.dzi;
             const double TAU32F = MS3[at(IC, JP, KC)] * (DWDY + DVDZ);
-            DWDY = (Q3[c] - Q3[at(IC, JM, KC)]) * g.dyci[JJ - 1];
+            DWDY = (Q3[c] - Q3[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1);
             DVDZ = (Q2[c] - Q2[ckm]) * g.dzi;
             const double TAU32B = MS3[c] * (DWDY + DVDZ);
             const double DTAU32DY = (TAU32F - TAU32B) * COE2;
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         }
         // ---- RHS_MOM_EXPLICIT_io: RK3 combine, pressure gradient, buoyancy ----
         const double rx = (tg * hx + tr * E1) * dt - (PR[c] - PR[at(IM, JC, KC)]) * (adt * g.dxi);
-        const double ry = (tg * hy + tr * E2) * dt - (PR[c] - PR[at(IC, JM, KC)]) * (adt * g.dyci[JC - 1] / 1.0);
+        const double ry = (tg * hy + tr * E2) * dt - (PR[c] - PR[at(IC, JM, KC)]) * (adt * ldc(g.dyci, JC - 1) / 1.0);
         const double rz = (tg * hz + tr * E3) * dt - (PR[c] - PR[at(IC, JC, KM)]) * (adt * g.dzi);
         double r1 = rx, r2 = ry, r3 = rz;
         if (GRAV == 1) r1 = r1 + RS[c] * (adt * p.f_a / 1.0);
@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
             inc[NC + e] = r2;
         }
         const double rf = p.nflow == 1 ? r1 : (p.nflow == 2 ? (JC >= 2 ? r2 : 0.0) : r3);
-        flow = rf / g.dyfi[JC - 1] / 1.0;
+        flow = rf / ldc(g.dyfi, JC - 1) / 1.0;
     }
     const double bs = io_block_sum<256>(flow, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = bs;
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(256) k_io_divg(IoGeo g, const double* __restri
     const int JC = JT;
     const int c = at(IC, JC, KC);
     const double DIVX = (G[at(IP, JC, KC)] - G[c]) * (g.dxi / 1.0);
-    const double DIVY = (G[CS + at(IC, JC + 1, KC)] - G[CS + c]) * (g.dyfi[JC - 1] / 1.0);
+    const double DIVY = (G[CS + at(IC, JC + 1, KC)] - G[CS + c]) * (ldc(g.dyfi, JC - 1) / 1.0);
     const double DIVZ = (G[2 * CS + at(IC, JC, KP)] - G[2 * CS + c]) * g.dzi;
     const double DTRHO = drhodtp[c] / 1.0;
     const double DQCAP = DIVX + DIVY + DIVZ + DTRHO;
@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(256) k_io_update(IoGeo g, double* __restrict__
     G[2 * CS + c] = G[2 * CS + c] - DFX33 * (dta * g.dzi);
     if (JC >= 2) {
         const double DFX22 = DPH[c] - DPH[at(IC, JC - 1, KC)];
-        G[CS + c] = G[CS + c] - DFX22 * (dta * g.dyci[JC - 1] / 1.0);
+        G[CS + c] = G[CS + c] - DFX22 * (dta * ldc(g.dyci, JC - 1) / 1.0);
     }
 }
 
@@ -500,7 +500,7 @@ __global__ void __launch_bounds__(256) k_io_divgck(IoGeo g, const double* __rest
         const int IC = (int)(e % L), JC = (int)((e / L) % M) + 1, KC = (int)(e / ((long)L * M));
         const int IP = IC == L - 1 ? 0 : IC + 1, KP = KC == N - 1 ? 0 : KC + 1;
         const int c = IC + L * JC + LP * KC, ckp = IC + L * JC + LP * KP;
-        const double COE2 = g.dyfi[JC - 1] * 1.0, COE3 = g.dzi * 1.0;
+        const double COE2 = ldc(g.dyfi, JC - 1) * 1.0, COE3 = g.dzi * 1.0;
         double d = (G[c - IC + IP] - G[c]) * g.dxi + (G[CS + c + L] - G[CS + c]) * COE2 +
                    (G[2 * CS + ckp] - G[2 * CS + c]) * COE3 + drhodtp[c];
         m0 = io_nanmax(m0, fabs(d));
@@ -594,16 +594,16 @@ __global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __re
         io_conv(g, G, Q, IC, IP, IM, JC, KC, KP, KM, hx, hy, hz);
         double IM1, AD1, IM2, AD2, IM3, AD3;
         {   // x
-            const double COE1 = -2.0 / 3.0 * g.dxi * CV, COE2 = g.dyfi[JJ - 1] * CV * 1.0, COE32 = g.dzi * CV * 1.0;
+            const double COE1 = -2.0 / 3.0 * g.dxi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV * 1.0, COE32 = g.dzi * CV * 1.0;
             const int cim = at(IM, JC, KC);
             double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
-            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1] * 1.0;
+            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * ldc(g.dyfi, JJ - 1) * 1.0;
             double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi * 1.0;
             const double F_EX = MU[c] * (DVDY + DWDZ), F_IM = MU[c] * DUDX;
             DUDX = (G1[at(IP, JC, KC)] * D1[at(IP, JC, KC)] - G1[c] * D1[c]) * g.dxi;
             const double F_AD = MU[c] * DUDX;
             DUDX = (Q1[c] - Q1[cim]) * g.dxi;
-            DVDY = (Q2[at(IM, JP, KC)] - Q2[cim]) * g.dyfi[JJ - 1] * 1.0;
+            DVDY = (Q2[at(IM, JP, KC)] - Q2[cim]) * ldc(g.dyfi, JJ - 1) * 1.0;
             DWDZ = (Q3[at(IM, JC, KP)] - Q3[cim]) * g.dzi * 1.0;
             const double B_EX = MU[cim] * (DVDY + DWDZ), B_IM = MU[cim] * DUDX;
             DUDX = (G1[c] * D1[c] - G1[cim] * D1[cim]) * g.dxi;
@@ -612,15 +612,15 @@ __global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __re
             const double X_AD = (F_AD - B_AD) * COE1 * (-2.0);
             double VP = MS1[at(IC, JP, KC)];
             double DVDX = (Q2[at(IC, JP, KC)] - Q2[at(IM, JP, KC)]) * g.dxi;
-            double DUDY = (Q1[at(IC, JP, KC)] - Q1[c]) * g.dyci[JJP - 1] / 1.0;
+            double DUDY = (Q1[at(IC, JP, KC)] - Q1[c]) * ldc(g.dyci, JJP - 1) / 1.0;
             const double F2_EX = VP * DVDX, F2_IM = VP * DUDY;
-            DUDY = (G1[at(IC, JP, KC)] * D1[at(IC, JP, KC)] - G1[c] * D1[c]) * g.dyci[JJP - 1] / 1.0;
+            DUDY = (G1[at(IC, JP, KC)] * D1[at(IC, JP, KC)] - G1[c] * D1[c]) * ldc(g.dyci, JJP - 1) / 1.0;
             const double F2_AD = VP * DUDY;
             double VC = MS1[c];
-            DUDY = (Q1[c] - Q1[at(IC, JM, KC)]) * g.dyci[JJ - 1] / 1.0;
+            DUDY = (Q1[c] - Q1[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1) / 1.0;
             DVDX = (Q2[c] - Q2[cim]) * g.dxi;
             const double B2_EX = VC * DVDX, B2_IM = VC * DUDY;
-            DUDY = (G1[c] * D1[c] - G1[at(IC, JM, KC)] * D1[at(IC, JM, KC)]) * g.dyci[JJ - 1] / 1.0;
+            DUDY = (G1[c] * D1[c] - G1[at(IC, JM, KC)] * D1[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1) / 1.0;
             const double B2_AD = VC * DUDY;
             const double Y_EX = (F2_EX - B2_EX) * COE2, Y_IM = (F2_IM - B2_IM) * COE2, Y_AD = (F2_AD - B2_AD) * COE2;
             VP = MS2[at(IC, JC, KP)];
@@ -641,44 +641,44 @@ __global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __re
             AD1 = X_AD + Y_AD + Z_AD;
         }
         {   // y (rows 2..M)
-            const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = -2.0 / 3.0 * g.dyci[JJ - 1] * CV;
+            const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = -2.0 / 3.0 * ldc(g.dyci, JJ - 1) * CV;
             const int cjm = at(IC, JM, KC);
             double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
-            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1];
+            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * ldc(g.dyfi, JJ - 1);
             double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi;
             const double F_EX = MU[c] * (DUDX + DWDZ), F_IM = MU[c] * DVDY;
-            DVDY = (G2[at(IC, JP, KC)] * D2[at(IC, JP, KC)] - G2[c] * D2[c]) * g.dyfi[JJ - 1];
+            DVDY = (G2[at(IC, JP, KC)] * D2[at(IC, JP, KC)] - G2[c] * D2[c]) * ldc(g.dyfi, JJ - 1);
             const double F_AD = MU[c] * DVDY;
             DUDX = (Q1[at(IP, JM, KC)] - Q1[cjm]) * g.dxi;
-            DVDY = (Q2[c] - Q2[cjm]) * g.dyfi[JJM - 1];
+            DVDY = (Q2[c] - Q2[cjm]) * ldc(g.dyfi, JJM - 1);
             DWDZ = (Q3[at(IC, JM, KP)] - Q3[cjm]) * g.dzi;
             const double B_EX = MU[cjm] * (DUDX + DWDZ), B_IM = MU[cjm] * DVDY;
-            DVDY = (G2[c] * D2[c] - G2[cjm] * D2[cjm]) * g.dyfi[JJM - 1];
+            DVDY = (G2[c] * D2[c] - G2[cjm] * D2[cjm]) * ldc(g.dyfi, JJM - 1);
             const double B_AD = MU[cjm] * DVDY;
             const double Y_EX = (F_EX - B_EX) * COE2, Y_IM = (F_IM - B_IM) * COE2 * (-2.0);
             const double Y_AD = (F_AD - B_AD) * COE2 * (-2.0);
             double VP = MS1[at(IP, JC, KC)];
             double DVDX = (Q2[at(IP, JC, KC)] - Q2[c]) * g.dxi;
-            double DUDY = (Q1[at(IP, JC, KC)] - Q1[at(IP, JM, KC)]) * g.dyci[JJ - 1];
+            double DUDY = (Q1[at(IP, JC, KC)] - Q1[at(IP, JM, KC)]) * ldc(g.dyci, JJ - 1);
             const double F1_EX = VP * DUDY, F1_IM = VP * DVDX;
             DVDX = (G2[at(IP, JC, KC)] * D2[at(IP, JC, KC)] - G2[c] * D2[c]) * g.dxi;
             const double F1_AD = VP * DVDX;
             double VC = MS1[c];
             DVDX = (Q2[c] - Q2[at(IM, JC, KC)]) * g.dxi;
-            DUDY = (Q1[c] - Q1[cjm]) * g.dyci[JJ - 1];
+            DUDY = (Q1[c] - Q1[cjm]) * ldc(g.dyci, JJ - 1);
             const double B1_EX = VC * DUDY, B1_IM = VC * DVDX;
             DVDX = (G2[c] * D2[c] - G2[at(IM, JC, KC)] * D2[at(IM, JC, KC)]) * g.dxi;
             const double B1_AD = VC * DVDX;
             const double X_EX = (F1_EX - B1_EX) * COE1, X_IM = (F1_IM - B1_IM) * COE1, X_AD = (F1_AD - B1_AD) * COE1;
             VP = MS3[at(IC, JC, KP)];
             double DVDZ = (Q2[at(IC, JC, KP)] - Q2[c]) * g.dzi;
-            double DWDY = (Q3[at(IC, JC, KP)] - Q3[at(IC, JM, KP)]) * g.dyci[JJ - 1];
+            double DWDY = (Q3[at(IC, JC, KP)] - Q3[at(IC, JM, KP)]) * ldc(g.dyci, JJ - 1);
             const double F3_EX = VP * DWDY, F3_IM = VP * DVDZ;
             DVDZ = (G2[at(IC, JC, KP)] * D2[at(IC, JC, KP)] - G2[c] * D2[c]) * g.dzi;
             const double F3_AD = VP * DVDZ;
             VC = MS3[c];
             DVDZ = (Q2[c] - Q2[at(IC, JC, KM)]) * g.dzi;
-            DWDY = (Q3[c] - Q3[cjm]) * g.dyci[JJ - 1];
+            DWDY = (Q3[c] - Q3[cjm]) * ldc(g.dyci, JJ - 1);
             const double B3_EX = VC * DWDY, B3_IM = VC * DVDZ;
             DVDZ = (G2[c] * D2[c] - G2[at(IC, JC, KM)] * D2[at(IC, JC, KM)]) * g.dzi;
             const double B3_AD = VC * DVDZ;
@@ -688,16 +688,16 @@ __global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __re
             AD2 = X_AD + Y_AD + Z_AD;
         }
         {   // z
-            const double COE1 = g.dxi * CV, COE3 = -2.0 / 3.0 * g.dzi * CV, COE2 = g.dyfi[JJ - 1] * CV;
+            const double COE1 = g.dxi * CV, COE3 = -2.0 / 3.0 * g.dzi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV;
             const int ckm = at(IC, JC, KM);
             double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
-            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * g.dyfi[JJ - 1];
+            double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * ldc(g.dyfi, JJ - 1);
             double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi;
             const double F_EX = MU[c] * (DUDX + DVDY), F_IM = MU[c] * DWDZ;
             DWDZ = (G3[at(IC, JC, KP)] * D3[at(IC, JC, KP)] - G3[c] * D3[c]) * g.dzi;
             const double F_AD = MU[c] * DWDZ;
             DUDX = (Q1[at(IP, JC, KM)] - Q1[ckm]) * g.dxi;
-            DVDY = (Q2[at(IC, JP, KM)] - Q2[ckm]) * g.dyfi[JJ - 1];
+            DVDY = (Q2[at(IC, JP, KM)] - Q2[ckm]) * ldc(g.dyfi, JJ - 1);
             DWDZ = (Q3[c] - Q3[ckm]) * g.dzi;
             const double B_EX = MU[ckm] * (DUDX + DVDY), B_IM = MU[ckm] * DWDZ;
             DWDZ = (G3[c] * D3[c] - G3[ckm] * D3[ckm]) * g.dzi;
@@ -718,16 +718,16 @@ __global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __re
             const double B1_AD = VC * DWDX;
             const double X_EX = (F1_EX - B1_EX) * COE1, X_IM = (F1_IM - B1_IM) * COE1, X_AD = (F1_AD - B1_AD) * COE1;
             VP = MS3[at(IC, JP, KC)];
-            double DWDY = (Q3[at(IC, JP, KC)] - Q3[c]) * g.dyci[JJP - 1];
+            double DWDY = (Q3[at(IC, JP, KC)] - Q3[c]) * ldc(g.dyci, JJP - 1);
             double DVDZ = (Q2[at(IC, JP, KC)] - Q2[at(IC, JP, KM)]) * g.dzi;
             const double F2_EX = VP * DVDZ, F2_IM = VP * DWDY;
-            DWDY = (G3[at(IC, JP, KC)] * D3[at(IC, JP, KC)] - G3[c] * D3[c]) * g.dyci[JJP - 1];
+            DWDY = (G3[at(IC, JP, KC)] * D3[at(IC, JP, KC)] - G3[c] * D3[c]) * ldc(g.dyci, JJP - 1);
             const double F2_AD = VP * DWDY;
             VC = MS3[c];
-            DWDY = (Q3[c] - Q3[at(IC, JM, KC)]) * g.dyci[JJ - 1];
+            DWDY = (Q3[c] - Q3[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1);
             DVDZ = (Q2[c] - Q2[ckm]) * g.dzi;
             const double B2_EX = VC * DVDZ, B2_IM = VC * DWDY;
-            DWDY = (G3[c] * D3[c] - G3[at(IC, JM, KC)] * D3[at(IC, JM, KC)]) * g.dyci[JJ - 1];
+            DWDY = (G3[c] * D3[c] - G3[at(IC, JM, KC)] * D3[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1);
             const double B2_AD = VC * DWDY;
             const double Y_EX = (F2_EX - B2_EX) * COE2, Y_IM = (F2_IM - B2_IM) * COE2, Y_AD = (F2_AD - B2_AD) * COE2;
             hz = X_EX + Y_EX + Z_EX + hz;
@@ -739,7 +739,7 @@ __global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __re
         r1 = r1 - (PR[c] - PR[at(IM, JC, KC)]) * (adt * g.dxi);
         double r2 = (tg * hy + tr * E2) * dt;
         r2 = r2 + adt * IM2 + 0.5 * adt * AD2;
-        r2 = r2 - (PR[c] - PR[at(IC, JM, KC)]) * (adt * g.dyci[JC - 1] / 1.0);
+        r2 = r2 - (PR[c] - PR[at(IC, JM, KC)]) * (adt * ldc(g.dyci, JC - 1) / 1.0);
         double r3 = (tg * hz + tr * E3) * dt;
         r3 = r3 + adt * IM3 + 0.5 * adt * AD3;
         r3 = r3 - (PR[c] - PR[at(IC, JC, KM)]) * (adt * g.dzi);
@@ -754,7 +754,7 @@ __global__ void __launch_bounds__(256) k_io_terms_cn(IoGeo g, const double* __re
         if (JC >= 2) E[NC + e] = hy;
         DUo[c] = AD3;
         const double rf = p.nflow == 1 ? r1 : (p.nflow == 2 ? (JC >= 2 ? r2 : 0.0) : r3);
-        flow = rf / g.dyfi[JC - 1] / 1.0;
+        flow = rf / ldc(g.dyfi, JC - 1) / 1.0;
     }
     const double bs = io_block_sum<256>(flow, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = bs;
@@ -795,8 +795,8 @@ __global__ void __launch_bounds__(1024) k_io_cn(IoGeo g, int idr, int sub_dpg, c
     const double dp = sub_dpg ? *dpg : 0.0;
     // wall-row coefficients of u, w in y that TDMA_COEF sets aside
     // (src/TDMA_COEF.f90:86-96)
-    const double amvr1 = 4.0 / (2.0 / g.dyci[1] + 1.0 / g.dyfi[0]) * (2.0 * g.dyfi[0]);
-    const double apvrn = 4.0 / (2.0 / g.dyci[M - 1] + 1.0 / g.dyfi[M - 1]) * (2.0 * g.dyfi[M - 1]);
+    const double amvr1 = 4.0 / (2.0 / ldc(g.dyci, 1) + 1.0 / ldc(g.dyfi, 0)) * (2.0 * ldc(g.dyfi, 0));
+    const double apvrn = 4.0 / (2.0 / ldc(g.dyci, M - 1) + 1.0 / ldc(g.dyfi, M - 1)) * (2.0 * ldc(g.dyfi, M - 1));
     double a[MS], b[MS], c[MS], d[MS];
 #pragma unroll
     for (int l = 0; l < MS; ++l) {

@@ -78,14 +78,6 @@ struct YCoef {
     double dyfi, dyci, wf0, wb0, wf1, wb1;
 };
 
-// A read of a 1-D coefficient array, which no kernel writes, through the
-// constant address space: with a uniform index it becomes a scalar load
-// (SGPRs, counted apart from the wave's vector loads, so that using the
-// value does not wait for the loads issued before it).
-__device__ __forceinline__ double ldc(const double* p, int idx) {
-    return ((const __attribute__((address_space(4))) double*)p)[idx];
-}
-
 __device__ __forceinline__ YCoef ycoef(const Geo& g, int j) {
     YCoef y;
     y.dyfi = ldc(g.dyfi, j);
