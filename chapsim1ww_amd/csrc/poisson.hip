@@ -266,9 +266,13 @@ __global__ void __launch_bounds__(768) k_poisson_y(cpx_t<Rl>* __restrict__ S, in
     extern __shared__ unsigned char smem[];
     const int tid = threadIdx.x;
     const int ll = tid % LPB, seg = tid / LPB;
+    // lines run over the padded rows (LHP per kz), so that a block's LPB
+    // consecutive kx start on a 128-byte line of S and no two blocks share
+    // one; the padding lines (kx >= LH) solve a dummy system
     const int line = blockIdx.x * LPB + ll;
-    const bool valid = line < nlines;
-    const int kz = valid ? line / LH : 0, kx = valid ? line - kz * LH : 0;
+    const int kz0 = line / LHP, kx0 = line - kz0 * LHP;
+    const bool valid = line < nlines && kx0 < LH;
+    const int kz = valid ? kz0 : 0, kx = valid ? kx0 : 0;
     const size_t base = (size_t)kz * M * LHP + kx;
     const double lam = -ak1[kx] - ak3[kz];
     double a[MS], b[MS], c[MS];
@@ -473,7 +477,7 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
         set_error("poisson: unsupported NCL2");
         return CHS_E_ARG;
     }
-    const int nlines = LH * N;
+    const int nlines = LHP * N;  // padded rows: see k_poisson_y
     const dim3 gy((nlines + LPB - 1) / LPB), by(LPB * P);
     const size_t smy = seg_smem_bytes<double2>(LPB, P);
     const double scale = 1.0 / ((double)L * (double)N);
