@@ -262,15 +262,21 @@ __device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict
 }
 
 // Explicit terms and RHS_MOM_EXPLICIT_io up to DPGRNS, rows 1..M.
-// inc[c] <- RHS of each component (row 1 of y untouched), explt0 <- the
-// explicit terms, part[block] <- sum of RHS/DYFI of the NFLOW component.
+// explt0 <- the explicit terms, part[block] <- sum of RHS/DYFI of the NFLOW
+// component, and either (G0 == nullptr) inc[c] <- RHS of each component (row
+// 1 of y untouched), or (G0 != nullptr) inc[c] <- G + RHS (0 for v on row 1)
+// and G0 <- G on rows 0..M+1 (SOLVERRK3_MOM_IO's G0 = G, in the pass that
+// reads G anyway).  In the second form DPGRNS is taken off later, by
+// k_io_update; DIVG_io's differences along the flow direction do not see
+// that uniform shift.
 template <int GRAV>
 __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restrict__ G, const double* __restrict__ Q,
                                                   const double* __restrict__ MU, const double* __restrict__ MU0,
                                                   const double* __restrict__ DU, const double* __restrict__ MS,
                                                   const double* __restrict__ PR, const double* __restrict__ RS,
                                                   double* __restrict__ E, IoParams p, double tg, double tr, double dt,
-                                                  double adt, double* __restrict__ inc, double* __restrict__ part) {
+                                                  double adt, double* __restrict__ inc, double* __restrict__ part,
+                                                  double* __restrict__ G0) {
 #pragma clang fp contract(off)
     __shared__ double red[256];
     const int L = g.L, M = g.M, N = g.N;
@@ -385,12 +391,27 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         if (GRAV == 2) r2 = r2 + RS[CS + c] * (adt * p.f_a / 1.0);
         if (GRAV == 3) r3 = r3 + RS[2 * CS + c] * (adt * p.f_a / 1.0);
         E[e] = hx;
-        inc[e] = r1;
         E[2 * NC + e] = hz;
-        inc[2 * NC + e] = r3;
-        if (JC >= 2) {
-            E[NC + e] = hy;
-            inc[NC + e] = r2;
+        if (JC >= 2) E[NC + e] = hy;
+        if (G0) {
+            const double g1 = G[c], g2 = G[CS + c], g3 = G[2 * CS + c];
+            inc[e] = g1 + r1;
+            inc[NC + e] = JC >= 2 ? g2 + r2 : 0.0;
+            inc[2 * NC + e] = g3 + r3;
+            G0[c] = g1;
+            G0[CS + c] = g2;
+            G0[2 * CS + c] = g3;
+            if (JC == 1 || JC == M) {  // the ghost row next to this one
+                const int cg = at(IC, JC == 1 ? 0 : M + 1, KC);
+                const double h1 = G[cg], h2 = G[CS + cg], h3 = G[2 * CS + cg];
+                G0[cg] = h1;
+                G0[CS + cg] = h2;
+                G0[2 * CS + cg] = h3;
+            }
+        } else {
+            inc[e] = r1;
+            inc[2 * NC + e] = r3;
+            if (JC >= 2) inc[NC + e] = r2;
         }
         const double rf = p.nflow == 1 ? r1 : (p.nflow == 2 ? (JC >= 2 ? r2 : 0.0) : r3);
         flow = rf / ldc(g.dyfi, JC - 1) / 1.0;
@@ -445,30 +466,67 @@ __global__ void __launch_bounds__(256) k_io_massflux(IoGeo g, double* __restrict
         G[CS + c] = 0.0;
 }
 
-// DIVG_io(NS): RHSLLPHI = (div G + DrhoDtP) / (DT TALP), rows 1..M
+// DIVG_io(NS): RHSLLPHI = (div G + DrhoDtP) / (DT TALP), rows 1..M.  With
+// GN (k_io_terms' G + RHS, (L, M, N) layout) the divergence is taken of GN:
+// v is zero on row 1 there and on the ghost row M+1 in G.
 __global__ void __launch_bounds__(256) k_io_divg(IoGeo g, const double* __restrict__ G,
-                                                 const double* __restrict__ drhodtp, double coe0,
-                                                 double* __restrict__ f) {
+                                                 const double* __restrict__ GN, const double* __restrict__ drhodtp,
+                                                 double coe0, double* __restrict__ f) {
 #pragma clang fp contract(off)
     IO_CELL(g.M, 1);
     const int JC = JT;
     const int c = at(IC, JC, KC);
-    const double DIVX = (G[at(IP, JC, KC)] - G[c]) * (g.dxi / 1.0);
-    const double DIVY = (G[CS + at(IC, JC + 1, KC)] - G[CS + c]) * (ldc(g.dyfi, JC - 1) / 1.0);
-    const double DIVZ = (G[2 * CS + at(IC, JC, KP)] - G[2 * CS + c]) * g.dzi;
+    double DIVX, DIVY, DIVZ;
+    if (GN) {
+        const int NC = L * M * N, e = IC + L * (JC - 1) + L * M * KC;
+        const double vp = JC < M ? GN[NC + e + L] : 0.0;
+        DIVX = (GN[IP + L * (JC - 1) + L * M * KC] - GN[e]) * (g.dxi / 1.0);
+        DIVY = (vp - GN[NC + e]) * (ldc(g.dyfi, JC - 1) / 1.0);
+        DIVZ = (GN[2 * NC + IC + L * (JC - 1) + L * M * KP] - GN[2 * NC + e]) * g.dzi;
+    } else {
+        DIVX = (G[at(IP, JC, KC)] - G[c]) * (g.dxi / 1.0);
+        DIVY = (G[CS + at(IC, JC + 1, KC)] - G[CS + c]) * (ldc(g.dyfi, JC - 1) / 1.0);
+        DIVZ = (G[2 * CS + at(IC, JC, KP)] - G[2 * CS + c]) * g.dzi;
+    }
     const double DTRHO = drhodtp[c] / 1.0;
     const double DQCAP = DIVX + DIVY + DIVZ + DTRHO;
     f[IC + L * (JC - 1) + L * M * KC] = DQCAP * coe0;
 }
 
-// PRCALC_io (explicit) with BC_WALL_PR_io and MASSFLUX_UPDATE_IO, rows 1..M
+// PRCALC_io (explicit) with BC_WALL_PR_io and MASSFLUX_UPDATE_IO, rows 1..M.
+// With GN (k_io_terms' G + RHS) the new G is GN - DPGRNS along NFLOW minus
+// the pressure correction, and the wall values of BC_WALL_G_io (zero ghost
+// rows, zero v on row 1) are written here.
 __global__ void __launch_bounds__(256) k_io_update(IoGeo g, double* __restrict__ G, double* __restrict__ PR,
                                                    const double* __restrict__ DPH, double dta,
-                                                   const double* __restrict__ f, double be) {
+                                                   const double* __restrict__ f, double be,
+                                                   const double* __restrict__ GN, const double* __restrict__ dpg,
+                                                   int nflow) {
 #pragma clang fp contract(off)
     IO_CELL(g.M, 1);
     const int JC = JT;
     const int c = at(IC, JC, KC);
+    if (GN) {
+        const int NC = L * M * N, e = IC + L * (JC - 1) + L * M * KC;
+        const double dp = *dpg;
+        const double pn = PR[c] + DPH[c];
+        PR[c] = pn;
+        if (JC == 1) PR[at(IC, 0, KC)] = pn;
+        if (JC == M) PR[at(IC, M + 1, KC)] = pn;
+        const double u = nflow == 1 ? GN[e] - dp : GN[e];
+        const double w = nflow == 3 ? GN[2 * NC + e] - dp : GN[2 * NC + e];
+        G[c] = u - (DPH[c] - DPH[at(IM, JC, KC)]) * (dta * g.dxi);
+        G[2 * CS + c] = w - (DPH[c] - DPH[at(IC, JC, KM)]) * (dta * g.dzi);
+        G[CS + c] = JC >= 2 ? GN[NC + e] - (DPH[c] - DPH[at(IC, JC - 1, KC)]) * (dta * ldc(g.dyci, JC - 1) / 1.0)
+                            : 0.0;
+        if (JC == 1 || JC == M) {
+            const int cg = at(IC, JC == 1 ? 0 : M + 1, KC);
+            G[cg] = 0.0;
+            G[CS + cg] = 0.0;
+            G[2 * CS + cg] = 0.0;
+        }
+        return;
+    }
     // PRCALC_io: PR + DPH, less BE * RHSLLPHI in the RK-CN branch
     const double pn = f ? PR[c] + DPH[c] - be * f[IC + L * (JC - 1) + L * M * KC] : PR[c] + DPH[c];
     PR[c] = pn;
@@ -1025,12 +1083,12 @@ static int io_substep_cn(chs_handle* h, const chs_io_fields* f, const IoArrays& 
     }
     CHS_LAUNCH(k_io_massflux, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, a.g, nullptr, h->scalars, p.nflow);
     h->timer.begin("io_divg", s, &ev);
-    CHS_LAUNCH(k_io_divg, gi, dim3(64, 4), 0, s, g, a.g, a.drhodtp, 1.0 / dt / talp, h->rhsllphi);
+    CHS_LAUNCH(k_io_divg, gi, dim3(64, 4), 0, s, g, a.g, nullptr, a.drhodtp, 1.0 / dt / talp, h->rhsllphi);
     h->timer.end(ev, "io_divg", s);
     CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (talp * dt)));
     h->timer.begin("io_update", s, &ev);
     CHS_LAUNCH(k_io_update, gi, dim3(64, 4), 0, s, g, a.g, a.pr, h->dph, dt * talp, h->rhsllphi,
-               0.5 * talp * dt * h->cvisc);
+               0.5 * talp * dt * h->cvisc, nullptr, nullptr, 0);
     h->timer.end(ev, "io_update", s);
     return CHS_OK;
 }
@@ -1048,9 +1106,12 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     const size_t ngh = (size_t)h->L * (h->M + 2) * h->N;
     const double talp = TGAM_IO[ns] + TROH_IO[ns];
     hipEvent_t ev;
-    // G0 = G (src/SOLVERRK3_MOM_IO.f90:15)
-    CHS_HIP(hipMemcpyAsync(f->g0, f->g, 3 * ngh * sizeof(double), hipMemcpyDeviceToDevice, s));
     IoParams p{prm->thermal, prm->nflow, prm->flowdrvtp, prm->gravdir, prm->cfgv, prm->f_a};
+    // G0 = G (src/SOLVERRK3_MOM_IO.f90:15): a copy in the RK-CN branch and for
+    // NFLOW = 2; otherwise k_io_terms writes G0 and G + RHS ("fold"), the mass
+    // flux pass disappears and k_io_update takes DPGRNS off
+    const bool fold = !prm->visimplicit && prm->nflow != 2;
+    if (!fold) CHS_HIP(hipMemcpyAsync(f->g0, f->g, 3 * ngh * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (prm->visimplicit) return io_substep_cn(h, f, a, p, ns, dt, talp);   // no DIVG_U_io in that branch
     h->timer.begin("io_divu", s, &ev);
     CHS_LAUNCH(k_io_divu, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, f->q, f->divu);
@@ -1058,7 +1119,7 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     h->timer.begin("io_terms", s, &ev);
 #define IO_TERMS(GR)                                                                                              \
     CHS_LAUNCH(k_io_terms<GR>, gi, dim3(64, 4), 0, s, g, a.g, a.q, a.mu, a.mu0, a.divu, a.ms, a.pr, a.rs, a.explt0, \
-               p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs, h->partials)
+               p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs, h->partials, fold ? f->g0 : nullptr)
     switch (prm->gravdir) {
         case 1: IO_TERMS(1); break;
         case 2: IO_TERMS(2); break;
@@ -1069,15 +1130,19 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     h->timer.end(ev, "io_terms", s);
     CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(256), 0, s, h->partials, (int)nb, h->volm, prm->flowdrvtp,
                -0.5 * prm->cfgv * (talp * dt), h->scalars);
-    h->timer.begin("io_massflux", s, &ev);
-    CHS_LAUNCH(k_io_massflux, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, f->g, h->rhs, h->scalars, prm->nflow);
-    h->timer.end(ev, "io_massflux", s);
+    if (!fold) {
+        h->timer.begin("io_massflux", s, &ev);
+        CHS_LAUNCH(k_io_massflux, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, f->g, h->rhs, h->scalars, prm->nflow);
+        h->timer.end(ev, "io_massflux", s);
+    }
     h->timer.begin("io_divg", s, &ev);
-    CHS_LAUNCH(k_io_divg, gi, dim3(64, 4), 0, s, g, f->g, f->drhodtp, 1.0 / dt / talp, h->rhsllphi);
+    CHS_LAUNCH(k_io_divg, gi, dim3(64, 4), 0, s, g, f->g, fold ? h->rhs : nullptr, f->drhodtp, 1.0 / dt / talp,
+               h->rhsllphi);
     h->timer.end(ev, "io_divg", s);
     CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (talp * dt)));
     h->timer.begin("io_update", s, &ev);
-    CHS_LAUNCH(k_io_update, gi, dim3(64, 4), 0, s, g, f->g, f->pr, h->dph, dt * talp, nullptr, 0.0);
+    CHS_LAUNCH(k_io_update, gi, dim3(64, 4), 0, s, g, f->g, f->pr, h->dph, dt * talp, nullptr, 0.0,
+               fold ? h->rhs : nullptr, h->scalars, prm->nflow);
     h->timer.end(ev, "io_update", s);
     return CHS_OK;
 }

@@ -60,11 +60,16 @@ def test_io_prepare_bitwise(scaled, dims):
                                      ((64, 48, 32), dict(thermal=1, gravdir=2, f_a=0.02)),
                                      ((32, 24, 16), dict(thermal=1, flowdrvtp=2, cfgv=0.01, gravdir=3, f_a=0.03)),
                                      ((32, 24, 16), dict(thermal=1, gravdir=1, f_a=-0.05, visimplicit=1)),
-                                     ((64, 48, 32), dict(thermal=1, nflow=3, visimplicit=1))])
+                                     ((64, 48, 32), dict(thermal=1, nflow=3, visimplicit=1)),
+                                     ((32, 24, 16), dict(thermal=1, nflow=3, gravdir=2, f_a=0.02)),
+                                     ((32, 24, 16), dict(thermal=1, nflow=2))])
 def test_heated_substeps(scaled, dims, kw):
     """Two RK3 steps (NS = 1, 2, 3 twice) of energy + staging + io momentum;
     every field within 1e-12 of the oracle after every substep and the
-    projected divergence below 1e-10 on both sides."""
+    projected divergence below 1e-10 on both sides.  The explicit cases with
+    NFLOW = 1 or 3 take the folded path (G + RHS and G0 written by the terms
+    kernel, DPGRNS taken off in the update); NFLOW = 2 and the RK-CN cases
+    keep the separate copy and mass-flux pass."""
     print(f"heated substeps {dims} {kw}", flush=True)
     og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=3 + sum(dims), **kw)
     rmax = O.rhoh_max(scaled)
