@@ -423,14 +423,24 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
 // DPGRNS of the constant-mass-flux drive: the block sums of RHS/DYFI in a
 // fixed order, divided by VOLM; for the pressure-gradient drive the
 // constant the host passes.
-__global__ void __launch_bounds__(256) k_io_dpgrns(const double* __restrict__ part, int nb, double volm,
-                                                   int flowdrvtp, double cst, double* __restrict__ out) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
-    red[threadIdx.x] = s;
+// One block of 1024 threads; each thread keeps four independent partial sums
+// (its strided values in turn), so that four loads are in flight per thread.
+__global__ void __launch_bounds__(1024) k_io_dpgrns(const double* __restrict__ part, int nb, double volm,
+                                                    int flowdrvtp, double cst, double* __restrict__ out) {
+    __shared__ double red[1024];
+    constexpr int T = 1024;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + 3 * T < nb; i += 4 * T) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] += part[i + q * T];
+    }
+    if (i < nb) s[0] += part[i];
+    if (i + T < nb) s[1] += part[i + T];
+    if (i + 2 * T < nb) s[2] += part[i + 2 * T];
+    red[threadIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
+    for (int st = T / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
@@ -1066,7 +1076,7 @@ static int io_substep_cn(chs_handle* h, const chs_io_fields* f, const IoArrays& 
     }
 #undef IO_TERMS_CN
     h->timer.end(ev, "io_terms_cn", s);
-    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(256), 0, s, h->partials, (int)nb, h->volm, p.flowdrvtp,
+    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(1024), 0, s, h->partials, (int)nb, h->volm, p.flowdrvtp,
                -0.5 * p.cfgv * (talp * dt), h->scalars);
     // MOMFA_io: FACOE_io = alpha dt / (2 Re) (src/MOMFA_io.f90:29)
     const double FACOE = 0.5 * talp * dt * h->cvisc;
@@ -1128,7 +1138,7 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     }
 #undef IO_TERMS
     h->timer.end(ev, "io_terms", s);
-    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(256), 0, s, h->partials, (int)nb, h->volm, prm->flowdrvtp,
+    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(1024), 0, s, h->partials, (int)nb, h->volm, prm->flowdrvtp,
                -0.5 * prm->cfgv * (talp * dt), h->scalars);
     if (!fold) {
         h->timer.begin("io_massflux", s, &ev);
