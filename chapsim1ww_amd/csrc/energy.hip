@@ -119,12 +119,13 @@ __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields
     const double *xh, *xt;
     const int *sh, *st;
     stage_tables(t, smem, xh, xt, sh, st);
+    // 32-bit cell offsets (chs_create keeps 3*L*(M+2)*N below 2^31): the
+    // row of a cell costs one 32-bit division instead of two 64-bit ones
     const int L = g.L, M = g.M;
-    const long long plane = (long long)L * (M + 2);
-    const long long ntot = plane * g.N;
+    const int ntot = L * (M + 2) * g.N;
     unsigned long long bad = 0;
-    for (long long q = (long long)blockIdx.x * EPT + threadIdx.x; q < ntot; q += (long long)gridDim.x * EPT) {
-        const int j = (int)((q % plane) / L);
+    for (int q = blockIdx.x * EPT + threadIdx.x; q < ntot; q += gridDim.x * EPT) {
+        const int j = (q / L) % (M + 2);
         const double mu_old = f.mu[q], rho_old = f.rho[q], rho0_old = f.rho0[q];
         f.mu0[q] = mu_old;
         f.rho1[q] = rho0_old;
