@@ -753,24 +753,26 @@ constexpr int YJC = 64;     // rows per thread chunk of the y window (measured: 
 constexpr int WUY = 2;      // rows fetched ahead by the y window (measured at 256x192x256: 4 -> 2, 240 -> 214 us)
 constexpr int WU = 4;       // rows fetched ahead per step of the z window (2: 140 -> 153 us)
 template <int KW>
-__global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
-                                                  const double* __restrict__ wy) {
+// One wave per block (chunk blockIdx.z): its LDS holds only the chunk's
+// weight rows, so more blocks fit a CU.
+__global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
+                                                 const double* __restrict__ wy) {
     const int L = g.L, M = g.M, N = g.N;
     const int ntx = (L + 63) / 64;
     const int nch = (M + YJC - 1) / YJC;
-    const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;   // blockDim = 64 * nch
+    const int lane = threadIdx.x, ch = blockIdx.z;
     const int it = blockIdx.x % ntx, k = blockIdx.x / ntx;
-    extern __shared__ double sw[];   // the component's row weights [M][W]
+    extern __shared__ double sw[];   // the chunk's row weights [j1 - j0][W], from row j0
     const bool valid = it * 64 + lane < L;
     const int i = valid ? it * 64 + lane : 0;
-    const int j0 = ch * YJC, j1 = valid ? min(M, j0 + YJC) : j0;   // invalid lanes do no rows
+    const int j0 = ch * YJC, jn = min(M, j0 + YJC), j1 = valid ? jn : j0;   // invalid lanes do no rows
     const size_t LM = (size_t)L * M;
     constexpr int W = 2 * KW + 1;
     double du0 = 0.0;
     const int c = blockIdx.y;
     {
         const double* Rc = R2 + (size_t)c * LM * N + (size_t)k * LM + i;
-        for (int e = threadIdx.x; e < M * W; e += blockDim.x) sw[e] = wy[(size_t)c * M * W + e];
+        for (int e = threadIdx.x; e < (jn - j0) * W; e += blockDim.x) sw[e] = wy[((size_t)c * M + j0) * W + e];
         __syncthreads();
         if (!valid) return;
         if (c == 1 && ch == 0) {
@@ -796,7 +798,7 @@ __global__ void __launch_bounds__(512) k_y_window(Geo g, const double* __restric
             qq[u] = j0 + u < j1 ? Q[qidx(g, c, i, j0 + u + 1, k)] : 0.0;
         }
         for (int j = j0; j < j1; ++j) {
-            const double* w = sw + (size_t)j * W;
+            const double* w = sw + (size_t)(j - j0) * W;
             double acc = 0.0;
 #pragma unroll
             for (int m = 0; m < W; ++m) acc += w[m] * win[m];
@@ -1367,9 +1369,9 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     // 3. y solves + Q update + BC_WALL_Q
     if (use_st) {
         const int nch = (h->M + YJC - 1) / YJC;
-        const dim3 gy(h->N * ((h->L + 63) / 64), 3), by(64 * nch);
+        const dim3 gy(h->N * ((h->L + 63) / 64), 3, nch), by(64);
         const double* wy = cn->dev + cn->oy;
-        const size_t smw = sizeof(double) * (size_t)h->M * (2 * cn->KW + 1);
+        const size_t smw = sizeof(double) * (size_t)std::min(h->M, YJC) * (2 * cn->KW + 1);
         h->timer.begin("y_solve", s, &ev);
         switch (cn->KW) {
             case 4: CHS_LAUNCH(k_y_window<4>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
