@@ -608,11 +608,22 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
 }
 
 // DPGRNS = (sum of block partials) / VOLM_tg, summed in a fixed order.
-__global__ void k_massflux(const double* __restrict__ part, int nb, double volm, double* __restrict__ out) {
+// One block of 1024 threads, four independent partial sums per thread (its
+// strided values in turn) so that four loads are in flight per thread.
+__global__ void __launch_bounds__(1024) k_massflux(const double* __restrict__ part, int nb, double volm,
+                                                   double* __restrict__ out) {
     __shared__ double red[1024];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
-    red[threadIdx.x] = s;
+    constexpr int T = 1024;
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + 3 * T < nb; i += 4 * T) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s4[q] += part[i + q * T];
+    }
+    if (i < nb) s4[0] += part[i];
+    if (i + T < nb) s4[1] += part[i + T];
+    if (i + 2 * T < nb) s4[2] += part[i + 2 * T];
+    red[threadIdx.x] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
