@@ -204,71 +204,263 @@ __device__ __forceinline__ double io_block_sum(double v, double* red) {
     return red[0];
 }
 
-// CONVECTION_X/Y/Z_io at one cell (c = (IC, JC, KC)): hx, hy, hz.  hy is
-// evaluated on row 1 too (from in-bounds rows); the callers discard it there.
-__device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict__ G, const double* __restrict__ Q,
-                                        int IC, int IP, int IM, int JC, int KC, int KP, int KM, double& hx,
-                                        double& hy, double& hz) {
+// Values of one array around the cell held in registers, for the offsets
+// the terms use: MU, MU0, DivU and PR at (0,0,0), (-1,0,0), (0,-1,0),
+// (0,0,-1); MU_STG x-y at (0,0,0), (0,1,0), (1,0,0); x-z at (0,0,0), (0,0,1),
+// (1,0,0); y-z at (0,0,0), (0,1,0), (0,0,1).
+struct IoV4 {
+    double c, im, jm, km;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return di < 0 ? im : dj < 0 ? jm : dk < 0 ? km : c;
+    }
+};
+struct IoVS1 {
+    double c, jp, ip;
+    __device__ __forceinline__ double operator()(int di, int dj, int) const { return dj > 0 ? jp : di > 0 ? ip : c; }
+};
+struct IoVS2 {
+    double c, kp, ip;
+    __device__ __forceinline__ double operator()(int di, int, int dk) const { return dk > 0 ? kp : di > 0 ? ip : c; }
+};
+struct IoVS3 {
+    double c, jp, kp;
+    __device__ __forceinline__ double operator()(int, int dj, int dk) const { return dj > 0 ? jp : dk > 0 ? kp : c; }
+};
+
+// One array around the cell (IC, JC, KC): a(di, dj, dk) is its value at
+// (IC+di, JC+dj, KC+dk), each of di, dj, dk in -1..1 and a constant at every
+// call site (x and z periodic).  IoAcc reads a ghost-aware global array;
+// IoRing reads the LDS ring of k_io_terms_m, which holds the rows JC-1, JC,
+// JC+1 as separate planes with the z neighbours as halo rows.
+struct IoAcc {
+    const double* __restrict__ b;  // the array (uniform), and the cell's offset in it
+    int c, ip, im, kp, km, L;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return b[c + (di > 0 ? ip : di < 0 ? im : 0) + dj * L + (dk > 0 ? kp : dk < 0 ? km : 0)];
+    }
+};
+struct IoRing {
+    const double *m, *c, *p;  // rows JC-1, JC, JC+1, at the cell
+    int ip, im, L;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return (dj < 0 ? m : dj > 0 ? p : c)[(di > 0 ? ip : di < 0 ? im : 0) + dk * L];
+    }
+};
+
+__device__ __forceinline__ IoAcc io_acc(const double* base, int c, int ip, int im, int kp, int km, int L) {
+    return IoAcc{base, c, ip, im, kp, km, L};
+}
+
+// CONVECTION_X/Y/Z_io at the cell: hx, hy, hz.  hy is evaluated on row 1
+// too (from in-bounds rows); the callers discard it there.
+template <class A>
+__device__ __forceinline__ void io_conv_a(const IoGeo& g, const A& G1, const A& G2, const A& G3, const A& Q1,
+                                          const A& Q2, const A& Q3, int JC, double& hx, double& hy, double& hz) {
 #pragma clang fp contract(off)
-    const int L = g.L, M = g.M, N = g.N;
-    const int LP = L * (M + 2), CS = LP * N;
-    auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
-    const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1;
-    const double wff = ldc(g.wff, JJ - 1), wfb = ldc(g.wfb, JJ - 1), wffp = ldc(g.wff, JJP - 1), wfbp = ldc(g.wfb, JJP - 1);
-    const double *G1 = G, *G2 = G + CS, *G3 = G + 2 * CS;
-    const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
-    const int c = at(IC, JC, KC);
+    const int JJ = JC, JJP = JC + 1;
+    const double wff = ldc(g.wff, JJ - 1), wfb = ldc(g.wfb, JJ - 1), wffp = ldc(g.wff, JJP - 1),
+                 wfbp = ldc(g.wfb, JJP - 1);
     {
         const double COE1 = g.dxi * 0.5 * 0.5, COE2 = ldc(g.dyfi, JJ - 1) * 0.5 * 1.0;
         const double COE32 = g.dzi * 0.5 * 0.5 * 1.0;
-        const double H11F = (G1[at(IP, JC, KC)] + G1[c]) * (Q1[at(IP, JC, KC)] + Q1[c]);
-        const double H11B = (G1[c] + G1[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IM, JC, KC)]);
+        const double H11F = (G1(1, 0, 0) + G1(0, 0, 0)) * (Q1(1, 0, 0) + Q1(0, 0, 0));
+        const double H11B = (G1(0, 0, 0) + G1(-1, 0, 0)) * (Q1(0, 0, 0) + Q1(-1, 0, 0));
         const double H11 = (H11F - H11B) * COE1;
-        const double H12F = (G2[at(IC, JP, KC)] + G2[at(IM, JP, KC)]) * (wffp * Q1[at(IC, JP, KC)] + wfbp * Q1[c]);
-        const double H12B = (G2[c] + G2[at(IM, JC, KC)]) * (wff * Q1[c] + wfb * Q1[at(IC, JM, KC)]);
+        const double H12F = (G2(0, 1, 0) + G2(-1, 1, 0)) * (wffp * Q1(0, 1, 0) + wfbp * Q1(0, 0, 0));
+        const double H12B = (G2(0, 0, 0) + G2(-1, 0, 0)) * (wff * Q1(0, 0, 0) + wfb * Q1(0, -1, 0));
         const double H12 = (H12F - H12B) * COE2;
-        const double H13F = (G3[at(IC, JC, KP)] + G3[at(IM, JC, KP)]) * (Q1[at(IC, JC, KP)] + Q1[c]);
-        const double H13B = (G3[c] + G3[at(IM, JC, KC)]) * (Q1[c] + Q1[at(IC, JC, KM)]);
+        const double H13F = (G3(0, 0, 1) + G3(-1, 0, 1)) * (Q1(0, 0, 1) + Q1(0, 0, 0));
+        const double H13B = (G3(0, 0, 0) + G3(-1, 0, 0)) * (Q1(0, 0, 0) + Q1(0, 0, -1));
         const double H13 = (H13F - H13B) * COE32;
         hx = -(H11 + H12 + H13);
     }
     {
         const double COE1 = g.dxi * 0.5, COE2 = ldc(g.dyci, JJ - 1) * 0.5 * 0.5, COE31 = g.dzi * 0.5 * 1.0;
-        const double H21F = (wff * G1[at(IP, JC, KC)] + wfb * G1[at(IP, JM, KC)]) * (Q2[at(IP, JC, KC)] + Q2[c]);
-        const double H21B = (wff * G1[c] + wfb * G1[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IM, JC, KC)]);
+        const double H21F = (wff * G1(1, 0, 0) + wfb * G1(1, -1, 0)) * (Q2(1, 0, 0) + Q2(0, 0, 0));
+        const double H21B = (wff * G1(0, 0, 0) + wfb * G1(0, -1, 0)) * (Q2(0, 0, 0) + Q2(-1, 0, 0));
         const double H21 = (H21F - H21B) * COE1;
-        const double H22F = (G2[at(IC, JP, KC)] + G2[c]) * (Q2[at(IC, JP, KC)] + Q2[c]);
-        const double H22B = (G2[c] + G2[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JM, KC)]);
+        const double H22F = (G2(0, 1, 0) + G2(0, 0, 0)) * (Q2(0, 1, 0) + Q2(0, 0, 0));
+        const double H22B = (G2(0, 0, 0) + G2(0, -1, 0)) * (Q2(0, 0, 0) + Q2(0, -1, 0));
         const double H22 = (H22F - H22B) * COE2;
-        const double H23F = (wff * G3[at(IC, JC, KP)] + wfb * G3[at(IC, JM, KP)]) * (Q2[at(IC, JC, KP)] + Q2[c]);
-        const double H23B = (wff * G3[c] + wfb * G3[at(IC, JM, KC)]) * (Q2[c] + Q2[at(IC, JC, KM)]);
+        const double H23F = (wff * G3(0, 0, 1) + wfb * G3(0, -1, 1)) * (Q2(0, 0, 1) + Q2(0, 0, 0));
+        const double H23B = (wff * G3(0, 0, 0) + wfb * G3(0, -1, 0)) * (Q2(0, 0, 0) + Q2(0, 0, -1));
         const double H23 = (H23F - H23B) * COE31;
         hy = -(H21 + H22 + H23);
     }
     {
         const double COE1 = g.dxi * 0.5 * 0.5, COE2 = ldc(g.dyfi, JJ - 1) * 0.5;
         const double COE31 = g.dzi * 0.5 * 0.5 * 1.0;
-        const double H31F = (G1[at(IP, JC, KC)] + G1[at(IP, JC, KM)]) * (Q3[at(IP, JC, KC)] + Q3[c]);
-        const double H31B = (G1[c] + G1[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IM, JC, KC)]);
+        const double H31F = (G1(1, 0, 0) + G1(1, 0, -1)) * (Q3(1, 0, 0) + Q3(0, 0, 0));
+        const double H31B = (G1(0, 0, 0) + G1(0, 0, -1)) * (Q3(0, 0, 0) + Q3(-1, 0, 0));
         const double H31 = (H31F - H31B) * COE1;
-        const double H32F = (G2[at(IC, JP, KC)] + G2[at(IC, JP, KM)]) * (wffp * Q3[at(IC, JP, KC)] + wfbp * Q3[c]);
-        const double H32B = (G2[c] + G2[at(IC, JC, KM)]) * (wff * Q3[c] + wfb * Q3[at(IC, JM, KC)]);
+        const double H32F = (G2(0, 1, 0) + G2(0, 1, -1)) * (wffp * Q3(0, 1, 0) + wfbp * Q3(0, 0, 0));
+        const double H32B = (G2(0, 0, 0) + G2(0, 0, -1)) * (wff * Q3(0, 0, 0) + wfb * Q3(0, -1, 0));
         const double H32 = (H32F - H32B) * COE2;
-        const double H33F = (G3[at(IC, JC, KP)] + G3[c]) * (Q3[at(IC, JC, KP)] + Q3[c]);
-        const double H33B = (G3[c] + G3[at(IC, JC, KM)]) * (Q3[c] + Q3[at(IC, JC, KM)]);
+        const double H33F = (G3(0, 0, 1) + G3(0, 0, 0)) * (Q3(0, 0, 1) + Q3(0, 0, 0));
+        const double H33B = (G3(0, 0, 0) + G3(0, 0, -1)) * (Q3(0, 0, 0) + Q3(0, 0, -1));
         const double H33 = (H33F - H33B) * COE31;
         hz = -(H31 + H32 + H33);
     }
 }
 
-// Explicit terms and RHS_MOM_EXPLICIT_io up to DPGRNS, rows 1..M.
-// explt0 <- the explicit terms, part[block] <- sum of RHS/DYFI of the NFLOW
-// component, and either (G0 == nullptr) inc[c] <- RHS of each component (row
-// 1 of y untouched), or (G0 != nullptr) inc[c] <- G + RHS (0 for v on row 1)
-// and G0 <- G on rows 0..M+1 (SOLVERRK3_MOM_IO's G0 = G, in the pass that
-// reads G anyway).  In the second form DPGRNS is taken off later, by
-// k_io_update; DIVG_io's differences along the flow direction do not see
-// that uniform shift.
+// The global-array form, as the RK-CN terms kernel calls it (c = (IC, JC, KC)).
+__device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict__ G, const double* __restrict__ Q,
+                                        int IC, int IP, int IM, int JC, int KC, int KP, int KM, double& hx,
+                                        double& hy, double& hz) {
+    const int L = g.L, LP = L * (g.M + 2), CS = LP * g.N;
+    const int c = IC + L * JC + LP * KC, ip = IP - IC, im = IM - IC, kp = (KP - KC) * LP, km = (KM - KC) * LP;
+    io_conv_a(g, io_acc(G, c, ip, im, kp, km, L), io_acc(G + CS, c, ip, im, kp, km, L),
+              io_acc(G + 2 * CS, c, ip, im, kp, km, L), io_acc(Q, c, ip, im, kp, km, L),
+              io_acc(Q + CS, c, ip, im, kp, km, L), io_acc(Q + 2 * CS, c, ip, im, kp, km, L), JC, hx, hy, hz);
+}
+
+// Explicit terms of one cell (CONVECTION_*_io, VISCOUS_ALL_EXPLT_*_io) and
+// RHS_MOM_EXPLICIT_io up to DPGRNS: h = the explicit terms, r = the RHS.
+// G*, Q* by AQ; MU, MU0, DivU, MU_STG, PR by AS; E = EXPLT0 of the cell,
+// rsg = RHO_STG of the GRAV component (unused for GRAV = 0).  Row 1 of v is
+// evaluated with a clamped metric index and discarded by the callers.
+template <int GRAV, class AQ, class AM, class AS1, class AS2, class AS3, class AP>
+__device__ __forceinline__ void io_terms_cell(const IoGeo& g, const AQ& G1, const AQ& G2, const AQ& G3,
+                                              const AQ& Q1, const AQ& Q2, const AQ& Q3, const AM& MU,
+                                              const AM& MU0, const AM& DU, const AS1& MS1, const AS2& MS2,
+                                              const AS3& MS3, const AP& PR, int JC, const double* E, double rsg,
+                                              const IoParams& p, double tg, double tr, double dt, double adt,
+                                              double* h, double* r) {
+#pragma clang fp contract(off)
+    const int JJ = JC, JJP = JC + 1, JJM = JC >= 2 ? JC - 1 : 1;
+    const double CV = g.cvisc;
+    double hx, hy, hz;
+    io_conv_a(g, G1, G2, G3, Q1, Q2, Q3, JC, hx, hy, hz);
+    // ---- VISCOUS_ALL_EXPLT_X_io ----
+    {
+        const double COE1 = 2.0 * g.dxi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV * 1.0, COE32 = g.dzi * CV * 1.0;
+        double DUDX = (Q1(1, 0, 0) - Q1(0, 0, 0)) * g.dxi;
+        const double TAU11F = 0.5 * (MU0(0, 0, 0) + MU(0, 0, 0)) * (DUDX - DU(0, 0, 0));
+        DUDX = (Q1(0, 0, 0) - Q1(-1, 0, 0)) * g.dxi;
+        const double TAU11B = 0.5 * (MU0(-1, 0, 0) + MU(-1, 0, 0)) * (DUDX - DU(-1, 0, 0));
+        const double DTAU11DX = (TAU11F - TAU11B) * COE1;
+        double DUDY = (Q1(0, 1, 0) - Q1(0, 0, 0)) * ldc(g.dyci, JJP - 1) / 1.0;
+        double DVDX = (Q2(0, 1, 0) - Q2(-1, 1, 0)) * g.dxi;
+        const double TAU12F = MS1(0, 1, 0) * (DUDY + DVDX);
+        DUDY = (Q1(0, 0, 0) - Q1(0, -1, 0)) * ldc(g.dyci, JJ - 1) / 1.0;
+        DVDX = (Q2(0, 0, 0) - Q2(-1, 0, 0)) * g.dxi;
+        const double TAU12B = MS1(0, 0, 0) * (DUDY + DVDX);
+        const double DTAU12DY = (TAU12F - TAU12B) * COE2;
+        double DUDZ = (Q1(0, 0, 1) - Q1(0, 0, 0)) * g.dzi;
+        double DWDX = (Q3(0, 0, 1) - Q3(-1, 0, 1)) * g.dxi;
+        const double TAU13F = MS2(0, 0, 1) * (DUDZ + DWDX);
+        DUDZ = (Q1(0, 0, 0) - Q1(0, 0, -1)) * g.dzi;
+        DWDX = (Q3(0, 0, 0) - Q3(-1, 0, 0)) * g.dxi;
+        const double TAU13B = MS2(0, 0, 0) * (DUDZ + DWDX);
+        const double DTAU13DZ = (TAU13F - TAU13B) * COE32;
+        hx = hx + DTAU11DX + DTAU12DY + DTAU13DZ;
+    }
+    // ---- VISCOUS_ALL_EXPLT_Y_io (rows 2..M) ----
+    {
+        const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = 2.0 * ldc(g.dyci, JJ - 1) * CV;
+        double DVDY = (Q2(0, 1, 0) - Q2(0, 0, 0)) * ldc(g.dyfi, JJ - 1);
+        const double TAU22F = 0.5 * (MU0(0, 0, 0) + MU(0, 0, 0)) * (DVDY - DU(0, 0, 0));
+        DVDY = (Q2(0, 0, 0) - Q2(0, -1, 0)) * ldc(g.dyfi, JJM - 1);
+        const double TAU22B = 0.5 * (MU0(0, -1, 0) + MU(0, -1, 0)) * (DVDY - DU(0, -1, 0));
+        const double DTAU22DY = (TAU22F - TAU22B) * COE2;
+        double DVDX = (Q2(1, 0, 0) - Q2(0, 0, 0)) * g.dxi;
+        double DUDY = (Q1(1, 0, 0) - Q1(1, -1, 0)) * ldc(g.dyci, JJ - 1);
+        const double TAU21F = MS1(1, 0, 0) * (DVDX + DUDY);
+        DVDX = (Q2(0, 0, 0) - Q2(-1, 0, 0)) * g.dxi;
+        DUDY = (Q1(0, 0, 0) - Q1(0, -1, 0)) * ldc(g.dyci, JJ - 1);
+        const double TAU21B = MS1(0, 0, 0) * (DVDX + DUDY);
+        const double DTAU21DX = (TAU21F - TAU21B) * COE1;
+        double DVDZ = (Q2(0, 0, 1) - Q2(0, 0, 0)) * g.dzi;
+        double DWDY = (Q3(0, 0, 1) - Q3(0, -1, 1)) * ldc(g.dyci, JJ - 1);
+        const double TAU23F = MS3(0, 0, 1) * (DVDZ + DWDY);
+        DVDZ = (Q2(0, 0, 0) - Q2(0, 0, -1)) * g.dzi;
+        DWDY = (Q3(0, 0, 0) - Q3(0, -1, 0)) * ldc(g.dyci, JJ - 1);
+        const double TAU23B = MS3(0, 0, 0) * (DVDZ + DWDY);
+        const double DTAU23DZ = (TAU23F - TAU23B) * COE3;
+        hy = hy + DTAU21DX + DTAU22DY + DTAU23DZ;
+    }
+    // ---- VISCOUS_ALL_EXPLT_Z_io ----
+    {
+        const double COE1 = g.dxi * CV, COE3 = 2.0 * g.dzi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV;
+        double DWDZ = (Q3(0, 0, 1) - Q3(0, 0, 0)) * g.dzi;
+        const double TAU33F = 0.5 * (MU0(0, 0, 0) + MU(0, 0, 0)) * (DWDZ - DU(0, 0, 0));
+        DWDZ = (Q3(0, 0, 0) - Q3(0, 0, -1)) * g.dzi;
+        const double TAU33B = 0.5 * (MU0(0, 0, -1) + MU(0, 0, -1)) * (DWDZ - DU(0, 0, -1));
+        const double DTAU33DZ = (TAU33F - TAU33B) * COE3;
+        double DWDX = (Q3(1, 0, 0) - Q3(0, 0, 0)) * g.dxi;
+        double DUDZ = (Q1(1, 0, 0) - Q1(1, 0, -1)) * g.dzi;
+        const double TAU31F = MS2(1, 0, 0) * (DWDX + DUDZ);
+        DWDX = (Q3(0, 0, 0) - Q3(-1, 0, 0)) * g.dxi;
+        DUDZ = (Q1(0, 0, 0) - Q1(0, 0, -1)) * g.dzi;
+        const double TAU31B = MS2(0, 0, 0) * (DWDX + DUDZ);
+        const double DTAU31DX = (TAU31F - TAU31B) * COE1;
+        double DWDY = (Q3(0, 1, 0) - Q3(0, 0, 0)) * ldc(g.dyci, JJP - 1);
+        double DVDZ = (Q2(0, 1, 0) - Q2(0, 1, -1)) * g.dzi;
+        const double TAU32F = MS3(0, 1, 0) * (DWDY + DVDZ);
+        DWDY = (Q3(0, 0, 0) - Q3(0, -1, 0)) * ldc(g.dyci, JJ - 1);
+        DVDZ = (Q2(0, 0, 0) - Q2(0, 0, -1)) * g.dzi;
+        const double TAU32B = MS3(0, 0, 0) * (DWDY + DVDZ);
+        const double DTAU32DY = (TAU32F - TAU32B) * COE2;
+        hz = hz + DTAU31DX + DTAU32DY + DTAU33DZ;
+    }
+    // ---- RHS_MOM_EXPLICIT_io: RK3 combine, pressure gradient, buoyancy ----
+    const double pc = PR(0, 0, 0);
+    const double rx = (tg * hx + tr * E[0]) * dt - (pc - PR(-1, 0, 0)) * (adt * g.dxi);
+    const double ry = (tg * hy + tr * E[1]) * dt - (pc - PR(0, -1, 0)) * (adt * ldc(g.dyci, JC - 1) / 1.0);
+    const double rz = (tg * hz + tr * E[2]) * dt - (pc - PR(0, 0, -1)) * (adt * g.dzi);
+    double r1 = rx, r2 = ry, r3 = rz;
+    if (GRAV == 1) r1 = r1 + rsg * (adt * p.f_a / 1.0);
+    if (GRAV == 2) r2 = r2 + rsg * (adt * p.f_a / 1.0);
+    if (GRAV == 3) r3 = r3 + rsg * (adt * p.f_a / 1.0);
+    h[0] = hx;
+    h[1] = hy;
+    h[2] = hz;
+    r[0] = r1;
+    r[1] = r2;
+    r[2] = r3;
+}
+
+// The outputs of one cell of the terms kernels: EXPLT0, then either inc =
+// RHS (G0 == nullptr) or inc = G + RHS with G0 <- G (g* = G of the cell,
+// gl* = G of the ghost row next to rows 1 and M).  Returns the cell's
+// mass-flux integrand.
+__device__ __forceinline__ double io_terms_store(const IoParams& p, const IoGeo& g, int JC, int e, int NC,
+                                                 const double* h, const double* r, double* __restrict__ E,
+                                                 double* __restrict__ inc, double* __restrict__ G0, int c, int cg,
+                                                 int CS, const double* gv, const double* glv) {
+#pragma clang fp contract(off)
+    E[e] = h[0];
+    E[2 * NC + e] = h[2];
+    if (JC >= 2) E[NC + e] = h[1];
+    if (G0) {
+        inc[e] = gv[0] + r[0];
+        inc[NC + e] = JC >= 2 ? gv[1] + r[1] : 0.0;
+        inc[2 * NC + e] = gv[2] + r[2];
+        G0[c] = gv[0];
+        G0[CS + c] = gv[1];
+        G0[2 * CS + c] = gv[2];
+        if (JC == 1 || JC == g.M) {  // the ghost row next to this one
+            G0[cg] = glv[0];
+            G0[CS + cg] = glv[1];
+            G0[2 * CS + cg] = glv[2];
+        }
+    } else {
+        inc[e] = r[0];
+        inc[2 * NC + e] = r[2];
+        if (JC >= 2) inc[NC + e] = r[1];
+    }
+    const double rf = p.nflow == 1 ? r[0] : (p.nflow == 2 ? (JC >= 2 ? r[1] : 0.0) : r[2]);
+    return rf / ldc(g.dyfi, JC - 1) / 1.0;
+}
+
+// Explicit terms and RHS_MOM_EXPLICIT_io up to DPGRNS, rows 1..M, one
+// thread per cell in row tiles.  explt0 <- the explicit terms, part[block]
+// <- sum of RHS/DYFI of the NFLOW component, and either (G0 == nullptr)
+// inc[c] <- RHS of each component (row 1 of y untouched), or (G0 != nullptr)
+// inc[c] <- G + RHS (0 for v on row 1) and G0 <- G on rows 0..M+1
+// (SOLVERRK3_MOM_IO's G0 = G, in the pass that reads G anyway).  In the
+// second form DPGRNS is taken off later, by k_io_update; DIVG_io's
+// differences along the flow direction do not see that uniform shift.
 template <int GRAV>
 __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restrict__ G, const double* __restrict__ Q,
                                                   const double* __restrict__ MU, const double* __restrict__ MU0,
@@ -288,136 +480,182 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
     const bool live = IC < L && KC < N;
     double flow = 0.0;
     if (live) {
-        const int IP = IC == L - 1 ? 0 : IC + 1, IM = IC == 0 ? L - 1 : IC - 1;
-        const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;
         const int LP = L * (M + 2), CS = LP * N, NC = L * M * N;
-        auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
-        // row 1 of the y component is not part of the stencil (v sits on the
-        // wall face); it is evaluated with a clamped metric index and
-        // discarded, so that the whole kernel body is one straight line.
-        const int JM = JC - 1, JP = JC + 1, JJ = JC, JJP = JC + 1, JJM = JC >= 2 ? JC - 1 : 1;
-        const double CV = g.cvisc;
-        const double *Q1 = Q, *Q2 = Q + CS, *Q3 = Q + 2 * CS;
-        const double *MS1 = MS, *MS2 = MS + CS, *MS3 = MS + 2 * CS;
-        const int c = at(IC, JC, KC);
+        const int ip = IC == L - 1 ? 1 - L : 1, im = IC == 0 ? L - 1 : -1;
+        const int kp = KC == N - 1 ? (1 - N) * LP : LP, km = KC == 0 ? (N - 1) * LP : -LP;
+        const int c = IC + L * JC + LP * KC;
         const int e = IC + L * (JC - 1) + L * M * KC;   // (L, M, N) index
-        const double E1 = E[e], E2 = E[NC + e], E3 = E[2 * NC + e];
-        // ---- CONVECTION_X/Y/Z_io, then VISCOUS_ALL_EXPLT_X_io ----
-        double hx;
-        double hy, hz;
-        io_conv(g, G, Q, IC, IP, IM, JC, KC, KP, KM, hx, hy, hz);
-        {
-            const double COE1 = 2.0 * g.dxi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV * 1.0, COE32 = g.dzi * CV * 1.0;
-            double DUDX = (Q1[at(IP, JC, KC)] - Q1[c]) * g.dxi;
-            const double TAU11F = 0.5 * (MU0[c] + MU[c]) * (DUDX - DU[c]);
-            DUDX = (Q1[c] - Q1[at(IM, JC, KC)]) * g.dxi;
-            const int cim = at(IM, JC, KC);
-            const double TAU11B = 0.5 * (MU0[cim] + MU[cim]) * (DUDX - DU[cim]);
-            const double DTAU11DX = (TAU11F - TAU11B) * COE1;
-            double DUDY = (Q1[at(IC, JP, KC)] - Q1[c]) * ldc(g.dyci, JJP - 1) / 1.0;
-            double DVDX = (Q2[at(IC, JP, KC)] - Q2[at(IM, JP, KC)]) * g.dxi;
-            const double TAU12F = MS1[at(IC, JP, KC)] * (DUDY + DVDX);
-            DUDY = (Q1[c] - Q1[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1) / 1.0;
-            DVDX = (Q2[c] - Q2[cim]) * g.dxi;
-            const double TAU12B = MS1[c] * (DUDY + DVDX);
-            const double DTAU12DY = (TAU12F - TAU12B) * COE2;
-            double DUDZ = (Q1[at(IC, JC, KP)] - Q1[c]) * g.dzi;
-            double DWDX = (Q3[at(IC, JC, KP)] - Q3[at(IM, JC, KP)]) * g.dxi;
-            const double TAU13F = MS2[at(IC, JC, KP)] * (DUDZ + DWDX);
-            DUDZ = (Q1[c] - Q1[at(IC, JC, KM)]) * g.dzi;
-            DWDX = (Q3[c] - Q3[cim]) * g.dxi;
-            const double TAU13B = MS2[c] * (DUDZ + DWDX);
-            const double DTAU13DZ = (TAU13F - TAU13B) * COE32;
-            hx = hx + DTAU11DX + DTAU12DY + DTAU13DZ;
+        auto A = [&](const double* b) { return io_acc(b, c, ip, im, kp, km, L); };
+        const double Ev[3] = {E[e], E[NC + e], E[2 * NC + e]};
+        const double rsg = GRAV == 0 ? 0.0 : RS[(GRAV - 1) * CS + c];
+        double hv[3], rv[3];
+        io_terms_cell<GRAV>(g, A(G), A(G + CS), A(G + 2 * CS), A(Q), A(Q + CS), A(Q + 2 * CS), A(MU), A(MU0),
+                            A(DU), A(MS), A(MS + CS), A(MS + 2 * CS), A(PR), JC, Ev, rsg, p, tg, tr, dt, adt, hv, rv);
+        const int cg = c + (JC == 1 ? -L : L);
+        const double gv[3] = {G[c], G[CS + c], G[2 * CS + c]};
+        double glv[3] = {0.0, 0.0, 0.0};
+        if (G0 && (JC == 1 || JC == M)) {
+            glv[0] = G[cg];
+            glv[1] = G[CS + cg];
+            glv[2] = G[2 * CS + cg];
         }
-        // ---- y (rows 2..M): VISCOUS_ALL_EXPLT_Y_io ----
-        {
-            {
-                const double COE1 = g.dxi * CV, COE3 = g.dzi * CV, COE2 = 2.0 * ldc(g.dyci, JJ - 1) * CV;
-                const int cjm = at(IC, JM, KC);
-                double DVDY = (Q2[at(IC, JP, KC)] - Q2[c]) * ldc(g.dyfi, JJ - 1);
-                const double TAU22F = 0.5 * (MU0[c] + MU[c]) * (DVDY - DU[c]);
-                DVDY = (Q2[c] - Q2[cjm]) * ldc(g.dyfi, JJM - 1);
-                const double TAU22B = 0.5 * (MU0[cjm] + MU[cjm]) * (DVDY - DU[cjm]);
-                const double DTAU22DY = (TAU22F - TAU22B) * COE2;
-                double DVDX = (Q2[at(IP, JC, KC)] - Q2[c]) * g.dxi;
-                double DUDY = (Q1[at(IP, JC, KC)] - Q1[at(IP, JM, KC)]) * ldc(g.dyci, JJ - 1);
-                const double TAU21F = MS1[at(IP, JC, KC)] * (DVDX + DUDY);
-                DVDX = (Q2[c] - Q2[at(IM, JC, KC)]) * g.dxi;
-                DUDY = (Q1[c] - Q1[cjm]) * ldc(g.dyci, JJ - 1);
-                const double TAU21B = MS1[c] * (DVDX + DUDY);
-                const double DTAU21DX = (TAU21F - TAU21B) * COE1;
-                double DVDZ = (Q2[at(IC, JC, KP)] - Q2[c]) * g.dzi;
-                double DWDY = (Q3[at(IC, JC, KP)] - Q3[at(IC, JM, KP)]) * ldc(g.dyci, JJ - 1);
-                const double TAU23F = MS3[at(IC, JC, KP)] * (DVDZ + DWDY);
-                DVDZ = (Q2[c] - Q2[at(IC, JC, KM)]) * g.dzi;
-                DWDY = (Q3[c] - Q3[cjm]) * ldc(g.dyci, JJ - 1);
-                const double TAU23B = MS3[c] * (DVDZ + DWDY);
-                const double DTAU23DZ = (TAU23F - TAU23B) * COE3;
-                hy = hy + DTAU21DX + DTAU22DY + DTAU23DZ;
-            }
-        }
-        // ---- z: VISCOUS_ALL_EXPLT_Z_io ----
-        {
-            const double COE1 = g.dxi * CV, COE3 = 2.0 * g.dzi * CV, COE2 = ldc(g.dyfi, JJ - 1) * CV;
-            const int ckm = at(IC, JC, KM);
-            double DWDZ = (Q3[at(IC, JC, KP)] - Q3[c]) * g.dzi;
-            const double TAU33F = 0.5 * (MU0[c] + MU[c]) * (DWDZ - DU[c]);
-            DWDZ = (Q3[c] - Q3[ckm]) * g.dzi;
-            const double TAU33B = 0.5 * (MU0[ckm] + MU[ckm]) * (DWDZ - DU[ckm]);
-            const double DTAU33DZ = (TAU33F - TAU33B) * COE3;
-            double DWDX = (Q3[at(IP, JC, KC)] - Q3[c]) * g.dxi;
-            double DUDZ = (Q1[at(IP, JC, KC)] - Q1[at(IP, JC, KM)]) * g.dzi;
-            const double TAU31F = MS2[at(IP, JC, KC)] * (DWDX + DUDZ);
-            DWDX = (Q3[c] - Q3[at(IM, JC, KC)]) * g.dxi;
-            DUDZ = (Q1[c] - Q1[ckm]) * g.dzi;
-            const double TAU31B = MS2[c] * (DWDX + DUDZ);
-            const double DTAU31DX = (TAU31F - TAU31B) * COE1;
-            double DWDY = (Q3[at(IC, JP, KC)] - Q3[c]) * ldc(g.dyci, JJP - 1);
-            double DVDZ = (Q2[at(IC, JP, KC)] - Q2[at(IC, JP, KM)]) * g.dzi;
-            const double TAU32F = MS3[at(IC, JP, KC)] * (DWDY + DVDZ);
-            DWDY = (Q3[c] - Q3[at(IC, JM, KC)]) * ldc(g.dyci, JJ - 1);
-            DVDZ = (Q2[c] - Q2[ckm]) * g.dzi;
-            const double TAU32B = MS3[c] * (DWDY + DVDZ);
-            const double DTAU32DY = (TAU32F - TAU32B) * COE2;
-            hz = hz + DTAU31DX + DTAU32DY + DTAU33DZ;
-        }
-        // ---- RHS_MOM_EXPLICIT_io: RK3 combine, pressure gradient, buoyancy ----
-        const double rx = (tg * hx + tr * E1) * dt - (PR[c] - PR[at(IM, JC, KC)]) * (adt * g.dxi);
-        const double ry = (tg * hy + tr * E2) * dt - (PR[c] - PR[at(IC, JM, KC)]) * (adt * ldc(g.dyci, JC - 1) / 1.0);
-        const double rz = (tg * hz + tr * E3) * dt - (PR[c] - PR[at(IC, JC, KM)]) * (adt * g.dzi);
-        double r1 = rx, r2 = ry, r3 = rz;
-        if (GRAV == 1) r1 = r1 + RS[c] * (adt * p.f_a / 1.0);
-        if (GRAV == 2) r2 = r2 + RS[CS + c] * (adt * p.f_a / 1.0);
-        if (GRAV == 3) r3 = r3 + RS[2 * CS + c] * (adt * p.f_a / 1.0);
-        E[e] = hx;
-        E[2 * NC + e] = hz;
-        if (JC >= 2) E[NC + e] = hy;
-        if (G0) {
-            const double g1 = G[c], g2 = G[CS + c], g3 = G[2 * CS + c];
-            inc[e] = g1 + r1;
-            inc[NC + e] = JC >= 2 ? g2 + r2 : 0.0;
-            inc[2 * NC + e] = g3 + r3;
-            G0[c] = g1;
-            G0[CS + c] = g2;
-            G0[2 * CS + c] = g3;
-            if (JC == 1 || JC == M) {  // the ghost row next to this one
-                const int cg = at(IC, JC == 1 ? 0 : M + 1, KC);
-                const double h1 = G[cg], h2 = G[CS + cg], h3 = G[2 * CS + cg];
-                G0[cg] = h1;
-                G0[CS + cg] = h2;
-                G0[2 * CS + cg] = h3;
-            }
-        } else {
-            inc[e] = r1;
-            inc[2 * NC + e] = r3;
-            if (JC >= 2) inc[NC + e] = r2;
-        }
-        const double rf = p.nflow == 1 ? r1 : (p.nflow == 2 ? (JC >= 2 ? r2 : 0.0) : r3);
-        flow = rf / ldc(g.dyfi, JC - 1) / 1.0;
+        flow = io_terms_store(p, g, JC, e, NC, hv, rv, E, inc, G0, c, cg, CS, gv, glv);
     }
     const double bs = io_block_sum<256>(flow, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = bs;
+}
+
+// ---------------------------------------------------------------------------
+// k_io_terms_m: k_io_terms marching along y.  A block owns IOKR = 2
+// consecutive k rows of every x line and a chunk of rows j0 .. j1-1 (JC =
+// j+1); thread (rb, i) of its 2L threads handles the cell (i, k0+rb) of each
+// row.  LDS holds a ring of the rows JC-1, JC, JC+1 of G and Q (six
+// components, IOKR + 2 k rows each: the periodic z neighbours as halo rows),
+// so their stencil reads come from LDS and each value comes from global
+// memory once per block; the ring's next row is loaded while the current one
+// is computed.  MU, MU0, DivU, MU_STG, PR, RHO_STG and EXPLT0 are read from
+// global memory as in k_io_terms.  The mass-flux partial of row j goes to
+// part[kg*M + j].
+// ---------------------------------------------------------------------------
+constexpr int IOKR = 2, IOKP = IOKR + 2;
+
+inline size_t io_terms_m_smem(int L) { return sizeof(double) * ((size_t)3 * 6 * IOKP * L + 16); }
+
+template <int GRAV>
+__global__ void __launch_bounds__(512) k_io_terms_m(IoGeo g, const double* __restrict__ G,
+                                                    const double* __restrict__ Q, const double* __restrict__ MU,
+                                                    const double* __restrict__ MU0, const double* __restrict__ DU,
+                                                    const double* __restrict__ MS, const double* __restrict__ PR,
+                                                    const double* __restrict__ RS, double* __restrict__ E,
+                                                    IoParams p, double tg, double tr, double dt, double adt,
+                                                    double* __restrict__ inc, double* __restrict__ part,
+                                                    double* __restrict__ G0, int nyc, int jc, int lgL) {
+#pragma clang fp contract(off)
+    extern __shared__ double sm[];
+    const int L = g.L, M = g.M, N = g.N;
+    const int PL = IOKP * L, SL = 6 * PL;  // one component / one ring row
+    double* ring = sm;
+    double* red = sm + 3 * SL;
+    const int nt = (N / IOKR) * nyc, per = (nt + 7) / 8;
+    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD x: tiles x*per .. (x+1)*per-1
+    if (t >= nt) return;
+    const int kg = t / nyc;
+    const int j0 = (t - kg * nyc) * jc, j1 = min(M, j0 + jc);
+    const int k0 = kg * IOKR;
+    const int tid = threadIdx.x, rb = tid >> lgL, i = tid & (L - 1);
+    const int KC = k0 + rb;
+    const int LP = L * (M + 2), CS = LP * N, NC = L * M * N;
+    // the cell's halo row: k0-1 (rb == 0) or k0+IOKR (rb == 1)
+    const int kh = rb == 0 ? (k0 == 0 ? N - 1 : k0 - 1) : (k0 + IOKR == N ? 0 : k0 + IOKR);
+    const int gc = i + LP * KC, gh = i + LP * kh;                   // global, row 0
+    const int lc = (rb + 1) * L + i, lh = (rb == 0 ? 0 : IOKP - 1) * L + i;  // in a ring plane
+    const int ip = i == L - 1 ? 1 - L : 1, im = i == 0 ? L - 1 : -1;
+    const int kp = KC == N - 1 ? (1 - N) * LP : LP, km = KC == 0 ? (N - 1) * LP : -LP;
+    const double *MS1 = MS, *MS2 = MS + CS, *MS3 = MS + 2 * CS;
+    // The values from global memory a step needs besides the ring, loaded one
+    // step ahead: row JC of MU, MU0, DivU, PR (cell, i-1, k-1), MU_STG (the
+    // cell's x-neighbour and z-neighbour entries), EXPLT0 and RHO_STG, and
+    // row JC+1 of MU_STG x-y and y-z; the j-1 entries of MU, MU0, DivU, PR and
+    // the cell entries of MU_STG x-y, y-z carry over from the step before.
+    struct Row {
+        double mu[3], mu0[3], du[3], pr[3];  // cell, i-1, k-1
+        double s1ip, s2[3], s3kp;            // MS1(i+1); MS2 cell, k+1, i+1; MS3(k+1)
+        double s1jp, s3jp;                   // MS1, MS3 at row JC+1
+        double e[3], rs;
+    };
+    auto load_row = [&](int JC, Row& r) {
+        const int JL = min(JC, M), c = gc + L * JL, c1 = gc + L * min(JC + 1, M + 1);
+        const int e = i + L * (JL - 1) + L * M * KC;
+        r.mu[0] = MU[c], r.mu[1] = MU[c + im], r.mu[2] = MU[c + km];
+        r.mu0[0] = MU0[c], r.mu0[1] = MU0[c + im], r.mu0[2] = MU0[c + km];
+        r.du[0] = DU[c], r.du[1] = DU[c + im], r.du[2] = DU[c + km];
+        r.pr[0] = PR[c], r.pr[1] = PR[c + im], r.pr[2] = PR[c + km];
+        r.s1ip = MS1[c + ip];
+        r.s2[0] = MS2[c], r.s2[1] = MS2[c + kp], r.s2[2] = MS2[c + ip];
+        r.s3kp = MS3[c + kp];
+        r.s1jp = MS1[c1], r.s3jp = MS3[c1];
+        r.e[0] = E[e], r.e[1] = E[NC + e], r.e[2] = E[2 * NC + e];
+        r.rs = GRAV == 0 ? 0.0 : RS[(GRAV - 1) * CS + c];
+    };
+    // fill the ring with rows j0 .. j0+2 (JC-1 .. JC+1 of the first step)
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int row = L * (j0 + s);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double* src = q < 3 ? G + q * CS : Q + (q - 3) * CS;
+            const double a = src[gc + row], ah = src[gh + row];
+            ring[s * SL + q * PL + lc] = a;
+            ring[s * SL + q * PL + lh] = ah;
+        }
+    }
+    Row cur;
+    load_row(j0 + 1, cur);
+    const int cm = gc + L * j0;  // row JC-1 of the first step
+    double mujm = MU[cm], mu0jm = MU0[cm], dujm = DU[cm], prjm = PR[cm];
+    double s1c = MS1[cm + L], s3c = MS3[cm + L];
+    const int nw = blockDim.x >> 6;
+    __builtin_amdgcn_s_waitcnt(0xF70);  // nothing above in flight when the loop starts: s_waitcnt vmcnt(0)
+    __syncthreads();
+    int sM = 0, s0 = 1, sP = 2;
+    for (int j = j0; j < j1; ++j) {
+        const int JC = j + 1;
+        // the next step's row values, then the ring's next row JC+2 (both
+        // clamped on the chunk's last row: loaded, never used)
+        Row nxt;
+        load_row(JC + 1, nxt);
+        const int rq = L * min(JC + 2, M + 1);
+        double pn[6], ph[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double* src = q < 3 ? G + q * CS : Q + (q - 3) * CS;
+            pn[q] = src[gc + rq];
+            ph[q] = src[gh + rq];
+        }
+        const int c = gc + L * JC;
+        const int e = i + L * (JC - 1) + L * M * KC;
+        auto R = [&](int q) {
+            return IoRing{ring + sM * SL + q * PL + lc, ring + s0 * SL + q * PL + lc, ring + sP * SL + q * PL + lc,
+                          ip, im, L};
+        };
+        const IoV4 vmu{cur.mu[0], cur.mu[1], mujm, cur.mu[2]}, vmu0{cur.mu0[0], cur.mu0[1], mu0jm, cur.mu0[2]};
+        const IoV4 vdu{cur.du[0], cur.du[1], dujm, cur.du[2]}, vpr{cur.pr[0], cur.pr[1], prjm, cur.pr[2]};
+        const IoVS1 vs1{s1c, cur.s1jp, cur.s1ip};
+        const IoVS2 vs2{cur.s2[0], cur.s2[1], cur.s2[2]};
+        const IoVS3 vs3{s3c, cur.s3jp, cur.s3kp};
+        double hv[3], rv[3];
+        io_terms_cell<GRAV>(g, R(0), R(1), R(2), R(3), R(4), R(5), vmu, vmu0, vdu, vs1, vs2, vs3, vpr, JC, cur.e,
+                            cur.rs, p, tg, tr, dt, adt, hv, rv);
+        const int cg = c + (JC == 1 ? -L : L);
+        const double gv[3] = {R(0).c[0], R(1).c[0], R(2).c[0]};
+        const double glv[3] = {JC == 1 ? R(0).m[0] : R(0).p[0], JC == 1 ? R(1).m[0] : R(1).p[0],
+                               JC == 1 ? R(2).m[0] : R(2).p[0]};
+        double v = io_terms_store(p, g, JC, e, NC, hv, rv, E, inc, G0, c, cg, CS, gv, glv);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((tid & 63) == 0) red[tid >> 6] = v;
+        __syncthreads();
+        if (tid < 64) {  // wave 0: the wave sums by a shuffle tree
+            double bs = tid < nw ? red[tid] : 0.0;
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) bs += __shfl_xor(bs, o);
+            if (tid == 0) part[kg * M + j] = bs;
+        }
+        // row JC+2 replaces row JC-1 in the ring; this row's values carry over
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            ring[sM * SL + q * PL + lc] = pn[q];
+            ring[sM * SL + q * PL + lh] = ph[q];
+        }
+        mujm = cur.mu[0], mu0jm = cur.mu0[0], dujm = cur.du[0], prjm = cur.pr[0];
+        s1c = cur.s1jp, s3c = cur.s3jp;
+        cur = nxt;
+        __syncthreads();
+        const int s_ = sM;
+        sM = s0;
+        s0 = sP;
+        sP = s_;
+    }
 }
 
 // DPGRNS of the constant-mass-flux drive: the block sums of RHS/DYFI in a
@@ -1126,19 +1364,52 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     h->timer.begin("io_divu", s, &ev);
     CHS_LAUNCH(k_io_divu, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, f->q, f->divu);
     h->timer.end(ev, "io_divu", s);
+    // the marching form needs L a power of two with 2L threads in whole
+    // waves (at most 512) and its LDS ring on this device; chs_set_cn_solver
+    // mode 2 selects the row-tile form
+    const int L = h->L;
+    const bool march = h->cn_mode != 2 && (L & (L - 1)) == 0 && L >= 32 && 2 * L <= 512 && h->N % IOKR == 0 &&
+                       io_terms_m_smem(L) <= (size_t)h->lds_max && (long)(h->N / IOKR) * h->M <= 2L * h->npart;
+    long nbt = nb;
     h->timer.begin("io_terms", s, &ev);
+    if (march) {
+        const int nkg = h->N / IOKR;
+        int nyc = std::max(1, std::min(h->M / 4, (256 + nkg - 1) / nkg));
+        const int jc = (h->M + nyc - 1) / nyc;
+        nyc = (h->M + jc - 1) / jc;
+        nbt = (long)nkg * h->M;
+        const dim3 gm(8 * ((nkg * nyc + 7) / 8));
+        const size_t sm = io_terms_m_smem(L);
+        const int lgL = __builtin_ctz((unsigned)L);
+#define IO_TERMS_M(GR)                                                                                             \
+    do {                                                                                                           \
+        (void)hipFuncSetAttribute((const void*)k_io_terms_m<GR>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                  (int)sm);                                                                        \
+        CHS_LAUNCH(k_io_terms_m<GR>, gm, dim3(2 * L), sm, s, g, a.g, a.q, a.mu, a.mu0, a.divu, a.ms, a.pr, a.rs,     \
+                   a.explt0, p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs, h->partials,                      \
+                   fold ? f->g0 : nullptr, nyc, jc, lgL);                                                          \
+    } while (0)
+        switch (prm->gravdir) {
+            case 1: IO_TERMS_M(1); break;
+            case 2: IO_TERMS_M(2); break;
+            case 3: IO_TERMS_M(3); break;
+            default: IO_TERMS_M(0); break;
+        }
+#undef IO_TERMS_M
+    } else {
 #define IO_TERMS(GR)                                                                                              \
     CHS_LAUNCH(k_io_terms<GR>, gi, dim3(64, 4), 0, s, g, a.g, a.q, a.mu, a.mu0, a.divu, a.ms, a.pr, a.rs, a.explt0, \
                p, TGAM_IO[ns], TROH_IO[ns], dt, talp * dt, h->rhs, h->partials, fold ? f->g0 : nullptr)
-    switch (prm->gravdir) {
-        case 1: IO_TERMS(1); break;
-        case 2: IO_TERMS(2); break;
-        case 3: IO_TERMS(3); break;
-        default: IO_TERMS(0); break;
-    }
+        switch (prm->gravdir) {
+            case 1: IO_TERMS(1); break;
+            case 2: IO_TERMS(2); break;
+            case 3: IO_TERMS(3); break;
+            default: IO_TERMS(0); break;
+        }
 #undef IO_TERMS
+    }
     h->timer.end(ev, "io_terms", s);
-    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(1024), 0, s, h->partials, (int)nb, h->volm, prm->flowdrvtp,
+    CHS_LAUNCH(k_io_dpgrns, dim3(1), dim3(1024), 0, s, h->partials, (int)nbt, h->volm, prm->flowdrvtp,
                -0.5 * prm->cfgv * (talp * dt), h->scalars);
     if (!fold) {
         h->timer.begin("io_massflux", s, &ev);

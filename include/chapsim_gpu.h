@@ -95,7 +95,8 @@ int chs_poisson_stats(chs_handle *h, int *iters, double *rel_resid);
  * the solves run as the truncated inverses (short stencils, computed exactly
  * on the host in extended precision); otherwise, and always with mode 1, as
  * segmented Thomas solves.  Mode 2 is mode 0 with the RHS evaluated one y row
- * per block instead of marching along y (same results to rounding). */
+ * per block instead of marching along y, here and in the explicit terms of
+ * chs_solverrk3_mom_io (same results to rounding). */
 int chs_set_cn_solver(chs_handle *h, int mode);
 /* Which path substep NS with time step dt takes (1 = stencils, 0 = Thomas)
  * and the stencil half-widths in x, z, y (> 16 when too wide). */

@@ -128,3 +128,28 @@ def test_isothermal_io_and_divgck_nan(scaled):
     assert np.isnan(dg)
     pt.close()
     s.close()
+
+
+@pytest.mark.parametrize("dims,kw", [((64, 48, 32), dict(thermal=1, gravdir=2, f_a=0.02)),
+                                     ((32, 24, 16), dict(thermal=1, nflow=2))])
+def test_io_terms_marching_matches_rowtile(scaled, dims, kw):
+    """The y-marching terms kernel against the row-tile one (chs_set_cn_solver
+    mode 2) on the same start: they evaluate the same expressions, only the
+    mass-flux partial sums group differently, so after three substeps every
+    field agrees to a few ulps."""
+    print(f"io terms marching vs rowtile {dims} {kw}", flush=True)
+    outs = []
+    for mode in ("auto", "rowtile"):
+        og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=41, **kw)
+        s.set_cn_solver(mode)
+        for ns in (1, 2, 3):
+            hc.substep(ns, 1e-3, CTHECD)
+        outs.append(hc.download())
+        pt.close()
+        s.close()
+    for k in ("g", "g0", "explt0", "q", "rho", "T"):
+        e = _rel(outs[0][MAP[k]], outs[1][MAP[k]])
+        print(k, e, flush=True)
+        assert e < 1e-13, (k, e)
+    pr_scale = np.abs(outs[1]["pr"]).max()
+    assert np.abs(outs[0]["pr"] - outs[1]["pr"]).max() < 1e-13 * pr_scale
