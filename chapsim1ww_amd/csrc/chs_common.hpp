@@ -103,6 +103,30 @@ __device__ __forceinline__ double ldc(const double* p, int idx) {
     return ((const __attribute__((address_space(4))) double*)p)[idx];
 }
 
+// One array around the cell (IC, JC, KC): a(di, dj, dk) is its value at
+// (IC+di, JC+dj, KC+dk), each of di, dj, dk in -1..1 and a constant at every
+// call site (x and z periodic).  CellAcc reads a ghost-aware global array;
+// CellRing reads the LDS ring of the marching kernels, which holds the rows
+// JC-1, JC, JC+1 as separate planes with the z neighbours as halo rows.
+struct CellAcc {
+    const double* __restrict__ b;  // the array (uniform), and the cell's offset in it
+    int c, ip, im, kp, km, L;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return b[c + (di > 0 ? ip : di < 0 ? im : 0) + dj * L + (dk > 0 ? kp : dk < 0 ? km : 0)];
+    }
+};
+struct CellRing {
+    const double *m, *c, *p;  // rows JC-1, JC, JC+1, at the cell
+    int ip, im, L;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return (dj < 0 ? m : dj > 0 ? p : c)[(di > 0 ? ip : di < 0 ? im : 0) + dk * L];
+    }
+};
+
+__device__ __forceinline__ CellAcc cell_acc(const double* base, int c, int ip, int im, int kp, int km, int L) {
+    return CellAcc{base, c, ip, im, kp, km, L};
+}
+
 __device__ inline int xcd_tile(int bid, int nblk) {
     const int nx = 8;
     int per = (nblk + nx - 1) / nx;

@@ -37,22 +37,21 @@ struct EFields {
     const double *g, *g0, *wallflux;
 };
 
-// one thread per interior cell; block (64 i) x (4 k), grid (i tiles, k tiles, j)
-__global__ void __launch_bounds__(256) k_energy_rhs(EGeo g, EFields f, double rk1, double rk0, double cthecd,
-                                                    double rhoh_cap) {
+// G0 and G of one cell: u at i+1 and at the cell, v at j+1 and at the cell,
+// w at k+1 and at the cell
+struct EGv {
+    double u0p, up, u0c, uc, v0p, vp, v0c, vc, w0p, wp, w0c, wc;
+};
+
+// RHS_ENERGY_EXPLICIT at the interior cell (IC, JC, KC): H, TT, KK access
+// the enthalpy, temperature and conductivity around it, q the mass fluxes.
+template <class A>
+__device__ __forceinline__ double energy_rhs_cell(const EGeo& g, const A& H, const A& TT, const A& KK,
+                                                  const EGv& q, int IC, int JC, double cthecd,
+                                                  const double* __restrict__ wallflux) {
 #pragma clang fp contract(off)
-    const int L = g.L, M = g.M, N = g.N;
-    int it, jt, kt, tile;
-    if (!row_tile(L, N, M, &it, &jt, &kt, &tile)) return;
-    const int IC = it * 64 + threadIdx.x;            // 0-based i
-    const int KC = kt * 4 + threadIdx.y;
-    const int JC = jt + 1;                           // Fortran J (ghost-aware row)
-    if (IC >= L || KC >= N) return;
-    const int IP = IC == L - 1 ? 0 : IC + 1, IM = IC == 0 ? L - 1 : IC - 1;
-    const int KP = KC == N - 1 ? 0 : KC + 1, KM = KC == 0 ? N - 1 : KC - 1;
+    const int M = g.M, L = g.L;
     const int JJ = JC, JJP = JJ + 1;
-    const int LP = L * (M + 2), CS = LP * N;
-    auto at = [&](int i, int j, int k) { return i + L * j + LP * k; };
     const double COE1 = g.dxi * 0.5 * 0.5;
     const double COE11 = g.dxqi * 0.5 * cthecd;
     const double COE2 = ldc(g.dyfi, JJ - 1) * 1.0 * 0.5;
@@ -70,42 +69,168 @@ __global__ void __launch_bounds__(256) k_energy_rhs(EGeo g, EFields f, double rk
         COE221 = ldc(g.dyci, JJP - 1) / 1.0;
         COE222 = ldc(g.dyci, JJ - 1) / 1.0;
     }
-    const double wffp = ldc(g.wff, JJP - 1), wfbp = ldc(g.wfb, JJP - 1), wff = ldc(g.wff, JJ - 1), wfb = ldc(g.wfb, JJ - 1);
-    const double* H = f.h;
-    const double* TT = f.T;
-    const double* KK = f.kap;
-    const double* G = f.g;
-    const double* G0 = f.g0;
-    const int c = at(IC, JC, KC);
-    double DX_RHOHU = (H[at(IP, JC, KC)] + H[c]) * (G0[at(IP, JC, KC)] + G[at(IP, JC, KC)]) -
-                      (H[c] + H[at(IM, JC, KC)]) * (G0[c] + G[c]);
+    const double wffp = ldc(g.wff, JJP - 1), wfbp = ldc(g.wfb, JJP - 1), wff = ldc(g.wff, JJ - 1),
+                 wfb = ldc(g.wfb, JJ - 1);
+    const double hc = H(0, 0, 0);
+    double DX_RHOHU = (H(1, 0, 0) + hc) * (q.u0p + q.up) - (hc + H(-1, 0, 0)) * (q.u0c + q.uc);
     DX_RHOHU = DX_RHOHU * COE1;
-    double DY_RHOHV = (wffp * H[at(IC, JC + 1, KC)] + wfbp * H[c]) *
-                          (G0[CS + at(IC, JC + 1, KC)] + G[CS + at(IC, JC + 1, KC)]) -
-                      (wff * H[c] + wfb * H[at(IC, JC - 1, KC)]) * (G0[CS + c] + G[CS + c]);
+    double DY_RHOHV = (wffp * H(0, 1, 0) + wfbp * hc) * (q.v0p + q.vp) - (wff * hc + wfb * H(0, -1, 0)) * (q.v0c + q.vc);
     DY_RHOHV = DY_RHOHV * COE2;
-    double DZ_RHOHW = (H[at(IC, JC, KP)] + H[c]) * (G0[2 * CS + at(IC, JC, KP)] + G[2 * CS + at(IC, JC, KP)]) -
-                      (H[c] + H[at(IC, JC, KM)]) * (G0[2 * CS + c] + G[2 * CS + c]);
+    double DZ_RHOHW = (H(0, 0, 1) + hc) * (q.w0p + q.wp) - (hc + H(0, 0, -1)) * (q.w0c + q.wc);
     DZ_RHOHW = DZ_RHOHW * COE3;
-    double DX_KDT = (KK[at(IP, JC, KC)] + KK[c]) * (TT[at(IP, JC, KC)] - TT[c]) -
-                    (KK[c] + KK[at(IM, JC, KC)]) * (TT[c] - TT[at(IM, JC, KC)]);
+    const double kc = KK(0, 0, 0), tc = TT(0, 0, 0);
+    double DX_KDT = (KK(1, 0, 0) + kc) * (TT(1, 0, 0) - tc) - (kc + KK(-1, 0, 0)) * (tc - TT(-1, 0, 0));
     DX_KDT = DX_KDT * COE11;
-    double KDTy1 = (wffp * KK[at(IC, JC + 1, KC)] + wfbp * KK[c]) * (TT[at(IC, JC + 1, KC)] - TT[c]) * COE221;
-    double KDTy2 = (wff * KK[c] + wfb * KK[at(IC, JC - 1, KC)]) * (TT[c] - TT[at(IC, JC - 1, KC)]) * COE222;
+    double KDTy1 = (wffp * KK(0, 1, 0) + wfbp * kc) * (TT(0, 1, 0) - tc) * COE221;
+    double KDTy2 = (wff * kc + wfb * KK(0, -1, 0)) * (tc - TT(0, -1, 0)) * COE222;
     if (JJ == 1)
-        KDTy2 = -f.wallflux[IC] * COE222;
+        KDTy2 = -wallflux[IC] * COE222;
     else if (JJ == M)
-        KDTy1 = -f.wallflux[IC + L] * COE221;
+        KDTy1 = -wallflux[IC + L] * COE221;
     const double DY_KDT = (KDTy1 - KDTy2) * COE22;
-    double DZ_KDT = (KK[at(IC, JC, KP)] + KK[c]) * (TT[at(IC, JC, KP)] - TT[c]) -
-                    (KK[at(IC, JC, KM)] + KK[c]) * (TT[c] - TT[at(IC, JC, KM)]);
+    double DZ_KDT = (KK(0, 0, 1) + kc) * (TT(0, 0, 1) - tc) - (KK(0, 0, -1) + kc) * (tc - TT(0, 0, -1));
     DZ_KDT = DZ_KDT * COE33;
-    const double RHS = -DX_RHOHU - DY_RHOHV - DZ_RHOHW + DX_KDT + DY_KDT + DZ_KDT;
+    return -DX_RHOHU - DY_RHOHV - DZ_RHOHW + DX_KDT + DY_KDT + DZ_KDT;
+}
+
+// The RK3 update of one cell: RHOH += rk1 RHS + rk0 RHS0, clamped to the cap
+__device__ __forceinline__ double energy_update(double rhoh, double rhs, double rhs0, double rk1, double rk0,
+                                                double cap) {
+#pragma clang fp contract(off)
+    const double r = rhoh + rk1 * rhs + rk0 * rhs0;
+    return fmin(cap, r);
+}
+
+// one thread per interior cell in row tiles (64 i x 4 k)
+__global__ void __launch_bounds__(256) k_energy_rhs(EGeo g, EFields f, double rk1, double rk0, double cthecd,
+                                                    double rhoh_cap) {
+#pragma clang fp contract(off)
+    const int L = g.L, M = g.M, N = g.N;
+    int it, jt, kt, tile;
+    if (!row_tile(L, N, M, &it, &jt, &kt, &tile)) return;
+    const int IC = it * 64 + threadIdx.x;            // 0-based i
+    const int KC = kt * 4 + threadIdx.y;
+    const int JC = jt + 1;                           // Fortran J (ghost-aware row)
+    if (IC >= L || KC >= N) return;
+    const int LP = L * (M + 2), CS = LP * N;
+    const int ip = IC == L - 1 ? 1 - L : 1, im = IC == 0 ? L - 1 : -1;
+    const int kp = KC == N - 1 ? (1 - N) * LP : LP, km = KC == 0 ? (N - 1) * LP : -LP;
+    const int c = IC + L * JC + LP * KC;
+    const double *G = f.g, *G0 = f.g0;
+    EGv q;
+    q.u0p = G0[c + ip], q.up = G[c + ip], q.u0c = G0[c], q.uc = G[c];
+    q.v0p = G0[CS + c + L], q.vp = G[CS + c + L], q.v0c = G0[CS + c], q.vc = G[CS + c];
+    q.w0p = G0[2 * CS + c + kp], q.wp = G[2 * CS + c + kp], q.w0c = G0[2 * CS + c], q.wc = G[2 * CS + c];
+    const double RHS = energy_rhs_cell(g, cell_acc(f.h, c, ip, im, kp, km, L), cell_acc(f.T, c, ip, im, kp, km, L),
+                                       cell_acc(f.kap, c, ip, im, kp, km, L), q, IC, JC, cthecd, f.wallflux);
     const int r0 = IC + L * (JC - 1) + L * M * KC;
-    double r = f.rhoh[c] + rk1 * RHS + rk0 * f.rhs0[r0];
-    r = fmin(rhoh_cap, r);
-    f.rhoh[c] = r;
+    f.rhoh[c] = energy_update(f.rhoh[c], RHS, f.rhs0[r0], rk1, rk0, rhoh_cap);
     f.rhs0[r0] = RHS;
+}
+
+// ---------------------------------------------------------------------------
+// k_energy_rhs_m: k_energy_rhs marching along y, as k_rhs_xm: a block owns
+// EKR = 4 consecutive k rows of every x line and a chunk of rows, one cell
+// of each row per thread (4L threads).  LDS holds a ring of the rows JC-1,
+// JC, JC+1 of H, T and kappa (EKR + 2 k rows each); the mass fluxes, RHOH
+// and RHS0 of a row are loaded one step ahead in registers, G0 and G of v at
+// the cell carried from the step before.
+// ---------------------------------------------------------------------------
+constexpr int EKR = 4, EKP = EKR + 2;
+
+inline size_t energy_rhs_m_smem(int L) { return sizeof(double) * (size_t)3 * 3 * EKP * L; }
+
+__global__ void __launch_bounds__(1024) k_energy_rhs_m(EGeo g, EFields f, double rk1, double rk0, double cthecd,
+                                                       double rhoh_cap, int nyc, int jc, int lgL) {
+#pragma clang fp contract(off)
+    extern __shared__ double sm[];
+    const int L = g.L, M = g.M, N = g.N;
+    const int PL = EKP * L, SL = 3 * PL;  // one array / one ring row (H, T, kappa)
+    double* ring = sm;
+    const int nt = (N / EKR) * nyc, per = (nt + 7) / 8;
+    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD x: tiles x*per .. (x+1)*per-1
+    if (t >= nt) return;
+    const int kg = t / nyc;
+    const int j0 = (t - kg * nyc) * jc, j1 = min(M, j0 + jc);
+    const int k0 = kg * EKR;
+    const int tid = threadIdx.x, r = tid >> lgL, IC = tid & (L - 1);
+    const int KC = k0 + r;
+    const int LP = L * (M + 2), CS = LP * N;
+    // one halo value per thread: the k0-1 row (r == 0), the k0+EKR row
+    // (r == EKR-1), or its own cell once more (same value, same place)
+    const bool lo = r == 0, hi = r == EKR - 1;
+    const int kh = lo ? (k0 == 0 ? N - 1 : k0 - 1) : hi ? (k0 + EKR == N ? 0 : k0 + EKR) : KC;
+    const int gc = IC + LP * KC, gh = IC + LP * kh;
+    const int lc = (r + 1) * L + IC, lh = lo ? IC : hi ? (EKP - 1) * L + IC : lc;
+    const int ip = IC == L - 1 ? 1 - L : 1, im = IC == 0 ? L - 1 : -1;
+    const int kp = KC == N - 1 ? (1 - N) * LP : LP;
+    const double* src[3] = {f.h, f.T, f.kap};
+    const double *G = f.g, *G0 = f.g0;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int row = L * (j0 + s);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double a = src[q][gc + row], ah = src[q][gh + row];
+            ring[s * SL + q * PL + lc] = a;
+            ring[s * SL + q * PL + lh] = ah;
+        }
+    }
+    // a row's values from global memory besides the ring (row JC; v at JC+1)
+    struct Row {
+        double u0p, up, u0c, uc, v0p, vp, w0p, wp, w0c, wc, rhoh, rhs0;
+    };
+    auto load_row = [&](int JC, Row& w) {
+        const int JL = min(JC, M), c = gc + L * JL;
+        w.u0p = G0[c + ip], w.up = G[c + ip], w.u0c = G0[c], w.uc = G[c];
+        w.v0p = G0[CS + c + L], w.vp = G[CS + c + L];
+        w.w0p = G0[2 * CS + c + kp], w.wp = G[2 * CS + c + kp], w.w0c = G0[2 * CS + c], w.wc = G[2 * CS + c];
+        w.rhoh = f.rhoh[c];
+        w.rhs0 = f.rhs0[IC + L * (JL - 1) + L * M * KC];
+    };
+    Row cur;
+    load_row(j0 + 1, cur);
+    double v0c = G0[CS + gc + L * (j0 + 1)], vc = G[CS + gc + L * (j0 + 1)];
+    __builtin_amdgcn_s_waitcnt(0xF70);  // nothing above in flight when the loop starts: s_waitcnt vmcnt(0)
+    __syncthreads();
+    int sM = 0, s0 = 1, sP = 2;
+    for (int j = j0; j < j1; ++j) {
+        const int JC = j + 1;
+        // the next step's row values, then the ring's next row JC+2 (both
+        // clamped on the chunk's last row: loaded, never used)
+        Row nxt;
+        load_row(JC + 1, nxt);
+        const int rq = L * min(JC + 2, M + 1);
+        double pn[3], ph[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            pn[q] = src[q][gc + rq];
+            ph[q] = src[q][gh + rq];
+        }
+        auto R = [&](int q) {
+            return CellRing{ring + sM * SL + q * PL + lc, ring + s0 * SL + q * PL + lc,
+                            ring + sP * SL + q * PL + lc, ip, im, L};
+        };
+        const EGv q{cur.u0p, cur.up, cur.u0c, cur.uc, cur.v0p, cur.vp, v0c, vc, cur.w0p, cur.wp, cur.w0c, cur.wc};
+        const double RHS = energy_rhs_cell(g, R(0), R(1), R(2), q, IC, JC, cthecd, f.wallflux);
+        const int c = gc + L * JC;
+        f.rhoh[c] = energy_update(cur.rhoh, RHS, cur.rhs0, rk1, rk0, rhoh_cap);
+        f.rhs0[IC + L * (JC - 1) + L * M * KC] = RHS;
+        __syncthreads();
+        // row JC+2 replaces row JC-1 in the ring; v at JC+1 carries over
+#pragma unroll
+        for (int q2 = 0; q2 < 3; ++q2) {
+            ring[sM * SL + q2 * PL + lc] = pn[q2];
+            ring[sM * SL + q2 * PL + lh] = ph[q2];
+        }
+        v0c = cur.v0p, vc = cur.vp;
+        cur = nxt;
+        __syncthreads();
+        const int s_ = sM;
+        sM = s0;
+        s0 = sP;
+        sP = s_;
+    }
 }
 
 constexpr int EPT = 512;   // threads per property block
@@ -221,9 +346,26 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     const double cap = p->rhoh_max + 1.0e-14;   // RHOHmax + REALMIN
     hipEvent_t ev;
     CHS_LAUNCH(k_zero_u64_e, dim3(1), dim3(64), 0, s, p->counters, 4);
+    // the marching form needs L a power of two with 4L threads in whole
+    // waves (at most 1024) and its LDS ring on this device; chs_set_cn_solver
+    // mode 2 selects the row-tile form
+    const int L = h->L;
+    const bool march = h->cn_mode != 2 && (L & (L - 1)) == 0 && L >= 16 && EKR * L <= 1024 && h->N % EKR == 0 &&
+                       energy_rhs_m_smem(L) <= (size_t)h->lds_max;
     h->timer.begin("energy_rhs", s, &ev);
-    CHS_LAUNCH(k_energy_rhs, dim3(row_tile_blocks(h->L, h->N, h->M)), dim3(64, 4), 0, s, g, f, TGAM_E[ns] * dt,
-               TROH_E[ns] * dt, cthecd, cap);
+    if (march) {
+        const int nkg = h->N / EKR;
+        int nyc = std::max(1, std::min(h->M / 4, (256 + nkg - 1) / nkg));
+        const int jc = (h->M + nyc - 1) / nyc;
+        nyc = (h->M + jc - 1) / jc;
+        const size_t sm = energy_rhs_m_smem(L);
+        (void)hipFuncSetAttribute((const void*)k_energy_rhs_m, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        CHS_LAUNCH(k_energy_rhs_m, dim3(8 * ((nkg * nyc + 7) / 8)), dim3(EKR * L), sm, s, g, f, TGAM_E[ns] * dt,
+                   TROH_E[ns] * dt, cthecd, cap, nyc, jc, __builtin_ctz((unsigned)L));
+    } else {
+        CHS_LAUNCH(k_energy_rhs, dim3(row_tile_blocks(h->L, h->N, h->M)), dim3(64, 4), 0, s, g, f, TGAM_E[ns] * dt,
+                   TROH_E[ns] * dt, cthecd, cap);
+    }
     h->timer.end(ev, "energy_rhs", s);
     const PropTab t = tab_of(p);
     h->timer.begin("energy_props", s, &ev);

@@ -227,30 +227,6 @@ struct IoVS3 {
     __device__ __forceinline__ double operator()(int, int dj, int dk) const { return dj > 0 ? jp : dk > 0 ? kp : c; }
 };
 
-// One array around the cell (IC, JC, KC): a(di, dj, dk) is its value at
-// (IC+di, JC+dj, KC+dk), each of di, dj, dk in -1..1 and a constant at every
-// call site (x and z periodic).  IoAcc reads a ghost-aware global array;
-// IoRing reads the LDS ring of k_io_terms_m, which holds the rows JC-1, JC,
-// JC+1 as separate planes with the z neighbours as halo rows.
-struct IoAcc {
-    const double* __restrict__ b;  // the array (uniform), and the cell's offset in it
-    int c, ip, im, kp, km, L;
-    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
-        return b[c + (di > 0 ? ip : di < 0 ? im : 0) + dj * L + (dk > 0 ? kp : dk < 0 ? km : 0)];
-    }
-};
-struct IoRing {
-    const double *m, *c, *p;  // rows JC-1, JC, JC+1, at the cell
-    int ip, im, L;
-    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
-        return (dj < 0 ? m : dj > 0 ? p : c)[(di > 0 ? ip : di < 0 ? im : 0) + dk * L];
-    }
-};
-
-__device__ __forceinline__ IoAcc io_acc(const double* base, int c, int ip, int im, int kp, int km, int L) {
-    return IoAcc{base, c, ip, im, kp, km, L};
-}
-
 // CONVECTION_X/Y/Z_io at the cell: hx, hy, hz.  hy is evaluated on row 1
 // too (from in-bounds rows); the callers discard it there.
 template <class A>
@@ -309,9 +285,9 @@ __device__ __forceinline__ void io_conv(const IoGeo& g, const double* __restrict
                                         double& hy, double& hz) {
     const int L = g.L, LP = L * (g.M + 2), CS = LP * g.N;
     const int c = IC + L * JC + LP * KC, ip = IP - IC, im = IM - IC, kp = (KP - KC) * LP, km = (KM - KC) * LP;
-    io_conv_a(g, io_acc(G, c, ip, im, kp, km, L), io_acc(G + CS, c, ip, im, kp, km, L),
-              io_acc(G + 2 * CS, c, ip, im, kp, km, L), io_acc(Q, c, ip, im, kp, km, L),
-              io_acc(Q + CS, c, ip, im, kp, km, L), io_acc(Q + 2 * CS, c, ip, im, kp, km, L), JC, hx, hy, hz);
+    io_conv_a(g, cell_acc(G, c, ip, im, kp, km, L), cell_acc(G + CS, c, ip, im, kp, km, L),
+              cell_acc(G + 2 * CS, c, ip, im, kp, km, L), cell_acc(Q, c, ip, im, kp, km, L),
+              cell_acc(Q + CS, c, ip, im, kp, km, L), cell_acc(Q + 2 * CS, c, ip, im, kp, km, L), JC, hx, hy, hz);
 }
 
 // Explicit terms of one cell (CONVECTION_*_io, VISCOUS_ALL_EXPLT_*_io) and
@@ -485,7 +461,7 @@ __global__ void __launch_bounds__(256) k_io_terms(IoGeo g, const double* __restr
         const int kp = KC == N - 1 ? (1 - N) * LP : LP, km = KC == 0 ? (N - 1) * LP : -LP;
         const int c = IC + L * JC + LP * KC;
         const int e = IC + L * (JC - 1) + L * M * KC;   // (L, M, N) index
-        auto A = [&](const double* b) { return io_acc(b, c, ip, im, kp, km, L); };
+        auto A = [&](const double* b) { return cell_acc(b, c, ip, im, kp, km, L); };
         const double Ev[3] = {E[e], E[NC + e], E[2 * NC + e]};
         const double rsg = GRAV == 0 ? 0.0 : RS[(GRAV - 1) * CS + c];
         double hv[3], rv[3];
@@ -615,7 +591,7 @@ __global__ void __launch_bounds__(512) k_io_terms_m(IoGeo g, const double* __res
         const int c = gc + L * JC;
         const int e = i + L * (JC - 1) + L * M * KC;
         auto R = [&](int q) {
-            return IoRing{ring + sM * SL + q * PL + lc, ring + s0 * SL + q * PL + lc, ring + sP * SL + q * PL + lc,
+            return CellRing{ring + sM * SL + q * PL + lc, ring + s0 * SL + q * PL + lc, ring + sP * SL + q * PL + lc,
                           ip, im, L};
         };
         const IoV4 vmu{cur.mu[0], cur.mu[1], mujm, cur.mu[2]}, vmu0{cur.mu0[0], cur.mu0[1], mu0jm, cur.mu0[2]};

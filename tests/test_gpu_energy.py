@@ -29,15 +29,20 @@ def test_rhoh_max_equals_oracle(scaled):
     t.close()
 
 
-@pytest.mark.parametrize("dims", [(16, 12, 8), (64, 48, 32)])
-def test_energy_substeps_bitwise(scaled, dims):
+@pytest.mark.parametrize("dims,mode", [((16, 12, 8), "auto"), ((64, 48, 32), "auto"), ((64, 48, 32), "rowtile"),
+                                       ((24, 12, 8), "auto")])
+def test_energy_substeps_bitwise(scaled, dims, mode):
+    """SOLVERRK3_ENG_IO against the oracle, every field bit for bit: the
+    marching RHS kernel (L a power of two), the row-tile one (mode 'rowtile',
+    and L = 24)."""
     from chapsim1ww_amd.energy import ThermalState
     from chapsim1ww_amd.solver import ChannelSolver
-    print(f"energy {dims}", flush=True)
+    print(f"energy {dims} {mode}", flush=True)
     g = ChannelGrid(*dims)
     og = O.OracleGrid(*dims)
     st, tab, n = O.thermal_case(og, scaled, seed=sum(dims))
     s = ChannelSolver(g)
+    s.set_cn_solver(mode)
     pt = props.PropertyTable(scaled)
     ts = ThermalState(s, pt)
     ts.upload({MAP[k]: v for k, v in st.items()})
