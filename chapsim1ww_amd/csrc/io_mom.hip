@@ -91,9 +91,9 @@ __global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __res
                                                     const double* __restrict__ G0, double* __restrict__ drhodtp,
                                                     double* __restrict__ rs, double* __restrict__ ms,
                                                     double* __restrict__ q, double* __restrict__ drhoi, int thermal,
-                                                    double ca, double cb, double cd) {
+                                                    double ca, double cb, double cd, int nj, int j0) {
 #pragma clang fp contract(off)
-    IO_CELL(g.M + 2, 0);
+    IO_CELL(nj, j0);   // rows j0 .. j0+nj-1 of 0..M+1
     const int J = JT;   // ghost-aware row 0..M+1
     const int c = at(IC, J, KC);
     double r1, r2, r3;
@@ -175,6 +175,113 @@ __global__ void __launch_bounds__(256) k_io_prepare(IoGeo g, const double* __res
         q[c] = G[c] / r1;
         q[2 * CS + c] = G[2 * CS + c] / r3;
         q[CS + c] = J >= 2 ? G[CS + c] / r2 : 0.0;
+    }
+}
+
+// k_io_prepare (thermal = 1) on the rows 2..M, marching along y: a thread
+// owns the column (IC, KC) of a chunk of rows.  There, Jw = Jr = J and Jm =
+// J-1, and the row J-1 parts of RHO_STG y and MU_STG x-y, y-z are the row
+// parts of the step before (the same expressions), so they carry over in
+// registers with v of G and G0 at J; the values of row J+1 are loaded one
+// step ahead.  Rows 0, 1 and M+1 (wrap and wall rows) are k_io_prepare's.
+struct PrepRow {
+    double d, dim, dkm, d0, d0im, d0km;              // RHO at the cell, i-1, k-1; RHO0 likewise
+    double v, vim, vkm, vimkm, v0, v0im, v0km, v0imkm;  // MU, MU0 at the cell, i-1, k-1, i-1 k-1
+    double gu, gup, g0u, g0up, gvp, g0vp, gw, gwp, g0w, g0wp;  // G, G0: u at i, i+1; v at j+1; w at k, k+1
+};
+
+__global__ void __launch_bounds__(256) k_io_prepare_m(IoGeo g, const double* __restrict__ D,
+                                                      const double* __restrict__ D0, const double* __restrict__ V,
+                                                      const double* __restrict__ V0, const double* __restrict__ G,
+                                                      const double* __restrict__ G0, double* __restrict__ drhodtp,
+                                                      double* __restrict__ rs, double* __restrict__ ms,
+                                                      double* __restrict__ q, double* __restrict__ drhoi, double ca,
+                                                      double cb, double cd, int nyc, int jc) {
+#pragma clang fp contract(off)
+    const int L = g.L, M = g.M, N = g.N;
+    const int ni = (L + 63) / 64, nk = (N + 3) / 4;
+    const int nt = ni * nk * nyc, per = (nt + 7) / 8;
+    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD x: tiles x*per .. (x+1)*per-1
+    if (t >= nt) return;
+    const int it = t % ni, kt = (t / ni) % nk, yc = t / (ni * nk);
+    const int IC = it * 64 + threadIdx.x, KC = kt * 4 + threadIdx.y;
+    if (IC >= L || KC >= N) return;
+    const int J0 = 2 + yc * jc, J1 = min(M + 1, J0 + jc);   // rows J0 .. J1-1 of 2..M
+    if (J0 >= J1) return;
+    const int LP = L * (M + 2), CS = LP * N;
+    const int ip = IC == L - 1 ? 1 - L : 1, im = IC == 0 ? L - 1 : -1;
+    const int kp = KC == N - 1 ? (1 - N) * LP : LP, km = KC == 0 ? (N - 1) * LP : -LP;
+    const int col = IC + LP * KC;
+    auto load_row = [&](int J, PrepRow& r) {
+        const int c = col + L * min(J, M);
+        r.d = D[c], r.dim = D[c + im], r.dkm = D[c + km];
+        r.d0 = D0[c], r.d0im = D0[c + im], r.d0km = D0[c + km];
+        r.v = V[c], r.vim = V[c + im], r.vkm = V[c + km], r.vimkm = V[c + im + km];
+        r.v0 = V0[c], r.v0im = V0[c + im], r.v0km = V0[c + km], r.v0imkm = V0[c + im + km];
+        r.gu = G[c], r.gup = G[c + ip], r.g0u = G0[c], r.g0up = G0[c + ip];
+        r.gvp = G[CS + c + L], r.g0vp = G0[CS + c + L];
+        r.gw = G[2 * CS + c], r.gwp = G[2 * CS + c + kp], r.g0w = G0[2 * CS + c], r.g0wp = G0[2 * CS + c + kp];
+    };
+    // the row J0-1 parts, and v at J0
+    double sdm, s1m, s3m, d0m, gv, g0v;
+    {
+        const int c = col + L * (J0 - 1);
+        sdm = D[c] + D0[c];
+        s1m = V[c] + V[c + im] + V0[c] + V0[c + im];
+        s3m = V[c + km] + V[c] + V0[c + km] + V0[c];
+        d0m = D0[c];
+        gv = G[CS + c + L];
+        g0v = G0[CS + c + L];
+    }
+    PrepRow cur;
+    load_row(J0, cur);
+    for (int J = J0; J < J1; ++J) {
+        PrepRow nxt;
+        load_row(J + 1, nxt);   // clamped on the chunk's last row: loaded, never used
+        const int c = col + L * J;
+        // DENSITY_TIME_DERIVATION
+        const double COE2 = ldc(g.dyfi, J - 1) * 1.0, COE3 = g.dzi * 1.0;
+        const double DIVX = (cur.gup - cur.gu) * g.dxi;
+        const double DIVX0 = (cur.g0up - cur.g0u) * g.dxi;
+        const double DIVY = (cur.gvp - gv) * COE2;
+        const double DIVY0 = (cur.g0vp - g0v) * COE2;
+        const double DIVZ = (cur.gwp - cur.gw) * COE3;
+        const double DIVZ0 = (cur.g0wp - cur.g0w) * COE3;
+        const double dr = (ca * (DIVX + DIVY + DIVZ) + cb * (DIVX0 + DIVY0 + DIVZ0)) / cd;
+        // RHO_STG and MU_STG (rows 2..M: no wrap, no wall)
+        const double r1 = (cur.d + cur.dim + cur.d0 + cur.d0im) * 0.5 * 0.5;
+        const double r3 = (cur.d + cur.dkm + cur.d0 + cur.d0km) * 0.5 * 0.5;
+        const double m2 = (cur.v + cur.vim + cur.v0 + cur.v0im + cur.vkm + cur.vimkm + cur.v0km + cur.v0imkm) *
+                          0.5 * 0.5 * 0.5;
+        const double wff = ldc(g.wff, J - 1), wfb = ldc(g.wfb, J - 1);
+        const double sd = cur.d + cur.d0;
+        const double s1 = cur.v + cur.vim + cur.v0 + cur.v0im;
+        const double s3 = cur.vkm + cur.v + cur.v0km + cur.v0;
+        const double r2 = (wff * sd + wfb * sdm) * 0.5;
+        const double m1 = (s1 * wff + s1m * wfb) * 0.5 * 0.5;
+        const double m3 = (s3 * wff + s3m * wfb) * 0.5 * 0.5;
+        if (drhoi) {
+            const double d1 = 1.0 / r1 - 1.0 / ((cur.d0 + cur.d0im) * 0.5);
+            const double d3 = 1.0 / r3 - 1.0 / ((cur.d0 + cur.d0km) * 0.5);
+            const double d2 = 1.0 / r2 - 1.0 / (wff * cur.d0 + wfb * d0m);
+            drhoi[c] = d1;
+            drhoi[CS + c] = d2;
+            drhoi[2 * CS + c] = d3;
+        }
+        drhodtp[c] = dr;
+        rs[c] = r1;
+        rs[CS + c] = r2;
+        rs[2 * CS + c] = r3;
+        ms[c] = m1;
+        ms[CS + c] = m2;
+        ms[2 * CS + c] = m3;
+        // VELOCITY_CALC_io
+        q[c] = cur.gu / r1;
+        q[2 * CS + c] = cur.gw / r3;
+        q[CS + c] = gv / r2;
+        sdm = sd, s1m = s1, s3m = s3, d0m = cur.d0;
+        gv = cur.gvp, g0v = cur.g0vp;
+        cur = nxt;
     }
 }
 
@@ -1226,9 +1333,24 @@ int io_prepare(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
         CHS_CHECK(f->drhoi_stg != nullptr, CHS_E_ARG, "chs_io_prepare: drhoi_stg is needed with visimplicit = 1");
         drhoi = f->drhoi_stg;
     }
-    CHS_LAUNCH(k_io_prepare, io_grid(h, h->M + 2), dim3(64, 4), 0, s, io_geo(h), a.rho, a.rho0, a.mu, a.mu0, a.g,
-               a.g0, a.drhodtp, a.rs, a.ms, a.q, drhoi, prm->thermal, TGAM_IO[ns] + TROH_IO[ns], TROH_IO[ns],
-               TGAM_IO[ns] - 2.0);
+    const double ca = TGAM_IO[ns] + TROH_IO[ns], cb = TROH_IO[ns], cd = TGAM_IO[ns] - 2.0;
+    const IoGeo g = io_geo(h);
+    if (prm->thermal && h->M >= 3 && h->cn_mode != 2) {
+        // rows 2..M marching (about 8 blocks per CU), rows 0, 1 and M+1 in row tiles
+        const long ncol = (long)((h->L + 63) / 64) * ((h->N + 3) / 4);
+        int nyc = (int)std::max(1L, std::min((long)(h->M - 1) / 4, (2048 + ncol - 1) / ncol));
+        const int jc = (h->M - 1 + nyc - 1) / nyc;
+        nyc = (h->M - 1 + jc - 1) / jc;
+        CHS_LAUNCH(k_io_prepare_m, dim3((unsigned)(8 * ((ncol * nyc + 7) / 8))), dim3(64, 4), 0, s, g, a.rho,
+                   a.rho0, a.mu, a.mu0, a.g, a.g0, a.drhodtp, a.rs, a.ms, a.q, drhoi, ca, cb, cd, nyc, jc);
+        CHS_LAUNCH(k_io_prepare, io_grid(h, 2), dim3(64, 4), 0, s, g, a.rho, a.rho0, a.mu, a.mu0, a.g, a.g0,
+                   a.drhodtp, a.rs, a.ms, a.q, drhoi, prm->thermal, ca, cb, cd, 2, 0);
+        CHS_LAUNCH(k_io_prepare, io_grid(h, 1), dim3(64, 4), 0, s, g, a.rho, a.rho0, a.mu, a.mu0, a.g, a.g0,
+                   a.drhodtp, a.rs, a.ms, a.q, drhoi, prm->thermal, ca, cb, cd, 1, h->M + 1);
+    } else {
+        CHS_LAUNCH(k_io_prepare, io_grid(h, h->M + 2), dim3(64, 4), 0, s, g, a.rho, a.rho0, a.mu, a.mu0, a.g,
+                   a.g0, a.drhodtp, a.rs, a.ms, a.q, drhoi, prm->thermal, ca, cb, cd, h->M + 2, 0);
+    }
     h->timer.end(ev, "io_prepare", s);
     return CHS_OK;
 }
