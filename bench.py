@@ -292,7 +292,9 @@ def props_bench(g, q_fp64, stream, steps):
     s.profile(False)
     dg_cn, _ = ts.divgck_io()
     ts.params.visimplicit = 0
-    ab = {"io_prepare": 160.0, "io_divu": 32.0, "io_terms": 176.0, "io_massflux": 72.0, "io_divg": 40.0,
+    # algorithmic bytes per cell of the explicit branch with NFLOW = 1: the
+    # terms kernel also writes G0 and G + RHS (no separate copy or mass-flux pass)
+    ab = {"io_prepare": 160.0, "io_divu": 32.0, "io_terms": 200.0, "io_massflux": 72.0, "io_divg": 40.0,
           "io_update": 72.0}
     kern = {}
     for name, (ms_tot, calls) in prof.items():
