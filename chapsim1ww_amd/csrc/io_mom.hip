@@ -1289,6 +1289,87 @@ __global__ void __launch_bounds__(1024) k_io_cn(IoGeo g, int idr, int sub_dpg, c
     }
 }
 
+// MOMFA1_X_io through LDS: the x lines are rows of the arrays, so the block
+// first copies its LPB rows of RHS, of the viscosity the component uses (MU
+// for u, MU_STG for v and w) and of RHO_STG with coalesced loads into LDS
+// (row stride L + 1: the lanes of a wave, LPB lines x consecutive segments,
+// then fall on distinct banks), forms the coefficients from there exactly as
+// k_io_cn<MS, 0> does, solves, and writes the rows back through LDS.
+inline size_t io_cn_x_smem(int L, int LPB, int P) {
+    return std::max(sizeof(double) * 3 * (size_t)LPB * (L + 1), seg_smem_bytes<double>(LPB, P));
+}
+
+template <int MS>
+__global__ void __launch_bounds__(1024) k_io_cn_x(IoGeo g, int idr, int sub_dpg, const double* __restrict__ MU,
+                                                 const double* __restrict__ MSTG, const double* __restrict__ RS,
+                                                 double* __restrict__ R, const double* __restrict__ dpg, IoCnCoef cf,
+                                                 int LPB, int P) {
+#pragma clang fp contract(off)
+    extern __shared__ unsigned char smem[];
+    const int L = g.L, M = g.M, N = g.N;
+    const int LP = L * (M + 2), CS = LP * N, RL = L + 1;
+    const int nlines = M * N, line0 = blockIdx.x * LPB;
+    double* sR = reinterpret_cast<double*>(smem);
+    double* sX = sR + LPB * RL;
+    double* sS = sX + LPB * RL;
+    const double* X = idr == 1 ? MU : MSTG + (size_t)(idr - 2) * CS;  // MU for u, MU_STG(., 1 | 2) for v | w
+    const double* RSc = RS + (size_t)(idr - 1) * CS;
+    for (int e = threadIdx.x; e < LPB * L; e += blockDim.x) {
+        const int ll = e / L, i = e - ll * L, line = line0 + ll;
+        if (line < nlines) {
+            const int lj = line % M + 1, lk = line / M;
+            const int cc = i + L * lj + LP * lk, ee = i + L * (lj - 1) + L * M * lk;
+            sR[ll * RL + i] = R[ee];
+            sX[ll * RL + i] = X[cc];
+            sS[ll * RL + i] = RSc[cc];
+        }
+    }
+    __syncthreads();
+    const int ll = threadIdx.x % LPB, seg = threadIdx.x / LPB;
+    const int line = line0 + ll;
+    const bool valid = line < nlines;
+    const int lj = valid ? line % M + 1 : 1;
+    const double dp = sub_dpg ? *dpg : 0.0;
+    const double* xr = sX + ll * RL;
+    const double* sr = sS + ll * RL;
+    double a[MS], b[MS], c[MS], d[MS];
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        const int m = seg * MS + l;
+        a[l] = 0.0; b[l] = 1.0; c[l] = 0.0; d[l] = 0.0;
+        if (!valid || m >= L) continue;
+        const int IM = m == 0 ? L - 1 : m - 1, IP = m == L - 1 ? 0 : m + 1;
+        double AMU, APU;
+        if (idr == 1) {
+            AMU = -cf.c1 * xr[IM];
+            APU = -cf.c1 * xr[m];
+        } else {
+            AMU = -cf.c2 * xr[m];
+            APU = -cf.c2 * xr[IP];
+        }
+        a[l] = AMU / sr[IM];
+        c[l] = APU / sr[IP];
+        b[l] = 1.0 - (AMU + APU) / sr[m];
+        const double rv = sR[ll * RL + m];
+        d[l] = (idr != 2 || lj >= 2) ? rv - dp : rv;
+    }
+    __syncthreads();   // the staged rows are read: seg_solve reuses the LDS
+    seg_solve<MS, double>(a, b, c, d, LPB, P, ll, seg, valid, true, smem);
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int l = 0; l < MS; ++l) {
+            const int m = seg * MS + l;
+            if (m < L) sR[ll * RL + m] = d[l];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < LPB * L; e += blockDim.x) {
+        const int l2 = e / L, i = e - l2 * L, ln = line0 + l2;
+        if (ln < nlines) R[i + L * (ln % M) + L * M * (ln / M)] = sR[l2 * RL + i];
+    }
+}
+
 static IoGeo io_geo(const chs_handle* h) {
     return IoGeo{h->L, h->M, h->N, h->dxi, h->dzi, h->cvisc, h->volm, h->dyfi, h->dyci, h->wff, h->wfb};
 }
@@ -1372,15 +1453,28 @@ static int io_cn_launch(chs_handle* h, const IoGeo& g, int idr, int sub_dpg, con
     const char* nm = DIR == 0 ? "io_cn_x" : (DIR == 2 ? "io_cn_z" : "io_cn_y");
     hipEvent_t ev;
     h->timer.begin(nm, s, &ev);
+    const size_t smx = io_cn_x_smem(h->L, LPB, P);
+    if (DIR == 0 && h->cn_mode != 2 && smx <= 65536) {
+#define IO_CN_X(MSV) \
+    CHS_LAUNCH((k_io_cn_x<MSV>), grid, block, smx, s, g, idr, sub_dpg, a.mu, a.ms, a.rs, R, h->scalars, cf, LPB, P)
+        switch (MS) {
+            case 16: IO_CN_X(16); break;
+            case 12: IO_CN_X(12); break;
+            case 4: IO_CN_X(4); break;
+            default: IO_CN_X(8); break;
+        }
+#undef IO_CN_X
+    } else {
 #define IO_CN(MSV) \
     CHS_LAUNCH((k_io_cn<MSV, DIR>), grid, block, sm, s, g, idr, sub_dpg, a.mu, a.ms, a.rs, R, Gc, h->scalars, cf, LPB, P)
-    switch (MS) {
-        case 16: IO_CN(16); break;
-        case 12: IO_CN(12); break;
-        case 4: IO_CN(4); break;
-        default: IO_CN(8); break;
-    }
+        switch (MS) {
+            case 16: IO_CN(16); break;
+            case 12: IO_CN(12); break;
+            case 4: IO_CN(4); break;
+            default: IO_CN(8); break;
+        }
 #undef IO_CN
+    }
     h->timer.end(ev, nm, s);
     return CHS_OK;
 }
