@@ -44,7 +44,6 @@ __device__ __forceinline__ int cidx(const Geo& g, int i, int j, int k) {
 
 constexpr int KR = 4;        // k rows per k_rhs_x block (segmented x solve)
 constexpr int KRS = 4;       // k rows per k_rhs_xw block
-constexpr int KRS8 = 8;      // the alternative tile height (instantiated, not selected)
 constexpr int XLPB = 16;     // line slots of the x solve (3*KR = 12 used)
 constexpr int XMS = 8;       // rows per segment of the x solve
 
@@ -1272,8 +1271,8 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const double COE = -facoe * h->dxqi;
         const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
         const int lgL = __builtin_ctz((unsigned)h->L);  // L is a power of two (substep_supported)
-        // 4 k rows per block: measured at 256x192x256, 8 rows (fewer k halo
-        // rows per output) ran 1.3% slower (0.452 vs 0.446 ms)
+        // 4 k rows per row-tile block: measured at 256x192x256, 8 rows (fewer
+        // k halo rows per output) ran 1.3% slower (0.452 vs 0.446 ms)
         const int krs = KRS;
         const int kr = use_st ? krs : KR;
         const int nblk = h->M * (h->N / kr);
@@ -1312,20 +1311,11 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
             const size_t sm = sizeof(double) * (3 * (size_t)krs * (size_t)(h->L + 2 * cn->KWx) + 4);
             const double* wx = cn->dev + cn->ox;
 #define CHS_RHSXW(KX, KR) CHS_LAUNCH((k_rhs_xw<KX, KR>), grx, dim3(256), sm, s, g, rk, h->cvisc, h->q, h->pr, h->convh0, h->rhs, h->partials, wx, lgL)
-            if (krs == KRS8) {
-                switch (cn->KWx) {
-                    case 4: CHS_RHSXW(4, KRS8); break;
-                    case 8: CHS_RHSXW(8, KRS8); break;
-                    case 12: CHS_RHSXW(12, KRS8); break;
-                    default: CHS_RHSXW(16, KRS8); break;
-                }
-            } else {
-                switch (cn->KWx) {
-                    case 4: CHS_RHSXW(4, KRS); break;
-                    case 8: CHS_RHSXW(8, KRS); break;
-                    case 12: CHS_RHSXW(12, KRS); break;
-                    default: CHS_RHSXW(16, KRS); break;
-                }
+            switch (cn->KWx) {
+                case 4: CHS_RHSXW(4, KRS); break;
+                case 8: CHS_RHSXW(8, KRS); break;
+                case 12: CHS_RHSXW(12, KRS); break;
+                default: CHS_RHSXW(16, KRS); break;
             }
 #undef CHS_RHSXW
         } else {
