@@ -259,7 +259,8 @@ __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields
         const double rh = f.rhoh[q];
         double hv = rh / rho_old;
         double Tt, D, Mu, K, CP;
-        if (!spline_ht(t, xh, sh, hv, Tt, true) || !spline_t(t, xt, st, Tt, D, Mu, K, CP)) {
+        // ENTHALPY_UPDATE, DENSITY_UPDATE: only the density of the first lookup is used
+        if (!spline_ht(t, xh, sh, hv, Tt, true) || !spline_t_density(t, xt, st, Tt, D)) {
             ++bad;
             continue;
         }

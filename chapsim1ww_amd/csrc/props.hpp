@@ -133,6 +133,23 @@ __device__ __forceinline__ bool spline_t(const PropTab& t, const double* xt, con
 
 // THERM_PROP_UPDATE of one cell (the caller skips |h| < 1e-10); false
 // (outputs untouched) when h or T leaves the table.
+// S_TD alone (DENSITY_UPDATE needs only the density): the same interval
+// search and the same evaluation as spline_t, reading only the density's
+// coefficients of the interval's record.
+__device__ __forceinline__ bool spline_t_density(const PropTab& t, const double* xt, const int* st, double tv,
+                                                 double& D) {
+    if (tv != tv) {
+        D = tv;
+        return true;
+    }
+    if (tv <= t.t_lo || tv >= t.t_hi) return false;
+    const int k = find_interval(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tv);
+    const double4 cd = reinterpret_cast<const double4*>(t.rt)[4 * (size_t)k];
+    D = cubic_nc(cd.x, cd.y, cd.z, cd.w, tv - xt[k]);
+    if (fabs(tv - 1.0) < 1.0e-10) D = 1.0;
+    return true;
+}
+
 __device__ __forceinline__ bool lookup_cell(const PropTab& t, const double* xh, const double* xt, const int* sh,
                                             const int* st, double hv, double& T, double& D, double& Mu, double& K,
                                             double& CP) {
