@@ -833,8 +833,10 @@ __global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict
     }
 }
 
-// z, register form: one thread per (i, j) column and chunk of ZKC rows of k
+// z, register form: one thread per (i, j) column and chunk of zkc rows of k
 // (cyclic), the window slides along k; weights are the same for every lane.
+// zkc is ZKC or a multiple of it (longer chunks refill the window less often)
+// and divides N; the host checks both.
 // u subtracts D = DPGRNS * xone on load (the mass-flux correction pushed
 // through the x solve).  Reads RHS, writes RHS2.
 constexpr int ZKC = 64;
@@ -842,10 +844,10 @@ static_assert(ZKC % WU == 0, "k_z_window steps a chunk WU rows at a time: a part
 template <int KW>
 __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restrict__ R, double* __restrict__ R2,
                                                   const double* __restrict__ dpg, double xone, int use_dpg,
-                                                  const double* __restrict__ wz) {
+                                                  const double* __restrict__ wz, int zkc) {
     const int L = g.L, M = g.M, N = g.N;
     const int ntx = (L + 63) / 64;
-    const int nch = N / ZKC;
+    const int nch = N / zkc;
     int b = blockIdx.x;
     const int lane = threadIdx.x & 63, ch = threadIdx.x >> 6;     // blockDim = 64 * 4
     const int it = b % ntx; b /= ntx;
@@ -862,7 +864,7 @@ __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restric
     double w[W];
 #pragma unroll
     for (int m = 0; m < W; ++m) w[m] = wz[m];
-    const int k0 = chunk * ZKC;
+    const int k0 = chunk * zkc;
     double win[W];
 #pragma unroll
     for (int m = 0; m < W; ++m) {
@@ -870,7 +872,7 @@ __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restric
         kk = kk < 0 ? kk + N : (kk >= N ? kk - N : kk);
         win[m] = Rc[(size_t)kk * LM] - D;
     }
-    for (int kb = k0; kb < k0 + ZKC; kb += WU) {
+    for (int kb = k0; kb < k0 + zkc; kb += WU) {
         double rn[WU];
 #pragma unroll
         for (int u = 0; u < WU; ++u) {
@@ -1336,14 +1338,16 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const double xone = 1.0 / (xa + xb + xc);
         const int usedpg0 = h->flowdrvtp == 1 ? 1 : 0;
         if (use_st) {
-            const dim3 gzw(3 * h->M * ((h->L + 63) / 64), (h->N / ZKC + 3) / 4);
+            // 128-row chunks where N allows (measured at 256x192x256: 139 -> 134 us)
+            const int zkc = h->N % (2 * ZKC) == 0 ? 2 * ZKC : ZKC;
+            const dim3 gzw(3 * h->M * ((h->L + 63) / 64), (h->N / zkc + 3) / 4);
             const double* wz = cn->dev + cn->oz;
             h->timer.begin("z_solve", s, &ev);
             switch (cn->KWz) {
-                case 4: CHS_LAUNCH(k_z_window<4>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
-                case 8: CHS_LAUNCH(k_z_window<8>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
-                case 12: CHS_LAUNCH(k_z_window<12>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
-                default: CHS_LAUNCH(k_z_window<16>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz); break;
+                case 4: CHS_LAUNCH(k_z_window<4>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+                case 8: CHS_LAUNCH(k_z_window<8>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+                case 12: CHS_LAUNCH(k_z_window<12>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+                default: CHS_LAUNCH(k_z_window<16>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
             }
             h->timer.end(ev, "z_solve", s);
         }
