@@ -360,49 +360,62 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const cpx_t<Rl>* __restrict__
 
 // ---- FP64 residual of the discrete operator (CHECK_FFT_SOLVER,
 // src/PRESSURE_CORRECTION.f90:169-199): r = f - L_h(phi), with the Neumann
-// wall rule for the y ghosts; also the block max of |r| and |f|.
+// wall rule for the y ghosts; also the block max of |r| and |f|.  Rows of
+// 64 i x 4 k; the grid (a multiple of 8 blocks) splits the tiles into 8
+// contiguous ranges, one per XCD, whose blocks walk them in j order so the
+// j +- 1 rows of phi are read from that XCD's L2.  STORE = false only checks
+// (no r written).  32-bit offsets (chs_create bounds the arrays below 2^31).
+template <bool STORE>
 __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restrict__ phi, const double* __restrict__ f,
                                                           double* __restrict__ r, int L, int M, int N, double dxqi,
                                                           double dzqi, const double* __restrict__ amph,
                                                           const double* __restrict__ acph,
                                                           const double* __restrict__ apph, double* __restrict__ part) {
-    __shared__ double red[2][256];
-    const size_t ncell = (size_t)L * M * N;
-    const size_t plane = (size_t)L * (M + 2);
+    __shared__ double red[2][4];
+    const int ni = (L + 63) / 64, nk = (N + 3) / 4;
+    const int nt = ni * M * nk, per = (nt + 7) / 8;
+    const int xcd = blockIdx.x % 8, nq = gridDim.x / 8;
+    const int t1 = min(nt, (xcd + 1) * per);
+    const int li = threadIdx.x & 63, lk = threadIdx.x >> 6;
+    const int LP = L * (M + 2);
     double mr = 0.0, mf = 0.0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (size_t)gridDim.x * blockDim.x) {
-        const int i = t % L;
-        const int j = (t / L) % M;
-        const int k = t / ((size_t)L * M);
-        const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
-        const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
-        const double* c0 = phi + (size_t)k * plane + (size_t)(j + 1) * L;
-        const double pc = c0[i];
-        const double pjp = j == M - 1 ? pc : c0[L + i];
-        const double pjm = j == 0 ? pc : c0[i - L];
-        const double lx = (c0[ip] - 2.0 * pc + c0[im]) * dxqi;
-        const double ly = pjp * apph[j] + pc * acph[j] + pjm * amph[j];
-        const double lz = (phi[(size_t)kp * plane + (size_t)(j + 1) * L + i] - 2.0 * pc +
-                           phi[(size_t)km * plane + (size_t)(j + 1) * L + i]) * dzqi;
-        const double fv = f[t];
-        const double rv = fv - (lx + ly + lz);
-        r[t] = rv;
-        mr = fmax(mr, fabs(rv));
-        mf = fmax(mf, fabs(fv));
-    }
-    red[0][threadIdx.x] = mr;
-    red[1][threadIdx.x] = mf;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
-            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+    for (int t = xcd * per + (int)blockIdx.x / 8; t < t1; t += nq) {
+        const int it = t % ni, rr = t / ni;
+        const int j = rr % M, kt = rr / M;
+        const int i = it * 64 + li, k = kt * 4 + lk;
+        if (i < L && k < N) {
+            const int ip = i == L - 1 ? 1 - L : 1, im = i == 0 ? L - 1 : -1;
+            const int kp = k == N - 1 ? (1 - N) * LP : LP, km = k == 0 ? (N - 1) * LP : -LP;
+            const int c = i + L * (j + 1) + LP * k;
+            const int fi = i + L * (j + M * k);
+            // the ghost rows are in the array: load, then apply the wall rule
+            const double pc = phi[c], pn = phi[c + L], ps = phi[c - L];
+            const double pe = phi[c + ip], pw = phi[c + im], pt = phi[c + kp], pb = phi[c + km];
+            const double fv = f[fi];
+            const double pjp = j == M - 1 ? pc : pn;
+            const double pjm = j == 0 ? pc : ps;
+            const double lx = (pe - 2.0 * pc + pw) * dxqi;
+            const double ly = pjp * ldc(apph, j) + pc * ldc(acph, j) + pjm * ldc(amph, j);
+            const double lz = (pt - 2.0 * pc + pb) * dzqi;
+            const double rv = fv - (lx + ly + lz);
+            if (STORE) r[fi] = rv;
+            mr = fmax(mr, fabs(rv));
+            mf = fmax(mf, fabs(fv));
         }
-        __syncthreads();
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mr = fmax(mr, __shfl_xor(mr, o));
+        mf = fmax(mf, __shfl_xor(mf, o));
+    }
+    if (li == 0) {
+        red[0][lk] = mr;
+        red[1][lk] = mf;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = red[0][0];
-        part[2 * blockIdx.x + 1] = red[1][0];
+        part[2 * blockIdx.x] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+        part[2 * blockIdx.x + 1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
     }
 }
 
@@ -521,17 +534,30 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
     if (h->precision == CHS_FP64) return poisson_pass<double>(h, f, dph, ghost, h->spec, h->twx, h->twz);
     const hipStream_t s = h->stream;
     CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
-    const size_t ncell = (size_t)h->L * h->M * h->N;
-    const int nb = 1024;
+    // 2048 blocks of row tiles (a multiple of 8: one contiguous tile range per XCD)
+    const int nb = 2048;
     hipEvent_t ev;
     h->last_refine_iters = 0;
     h->last_refine_resid = -1.0;
     double host[2] = {0.0, 0.0};
-    for (int it = 0; it <= h->refine_max; ++it) {
+    bool stored = false;   // h->resid holds the residual of the current dph
+    auto residual = [&](bool store) -> int {
         h->timer.begin("poisson_residual", s, &ev);
-        CHS_LAUNCH(k_poisson_residual, dim3(nb), dim3(256), 0, s, dph, f, h->resid, h->L, h->M, h->N, h->dxqi,
-                   h->dzqi, h->amph, h->acph, h->apph, h->partials);
+        if (store)
+            CHS_LAUNCH(k_poisson_residual<true>, dim3(nb), dim3(256), 0, s, dph, f, h->resid, h->L, h->M, h->N,
+                       h->dxqi, h->dzqi, h->amph, h->acph, h->apph, h->partials);
+        else
+            CHS_LAUNCH(k_poisson_residual<false>, dim3(nb), dim3(256), 0, s, dph, f, h->resid, h->L, h->M, h->N,
+                       h->dxqi, h->dzqi, h->amph, h->acph, h->apph, h->partials);
         h->timer.end(ev, "poisson_residual", s);
+        stored = store;
+        return CHS_OK;
+    };
+    for (int it = 0; it <= h->refine_max; ++it) {
+        // the first check stores r (a refinement almost always follows an
+        // FP32 pass); later checks only reduce, and r is formed again in the
+        // rare case that one more refinement is needed
+        CHS_TRY(residual(it == 0 && h->refine_max > 0));
         CHS_LAUNCH(k_reduce_max2, dim3(1), dim3(256), 0, s, h->partials, nb, h->scalars + 4);
         CHS_HIP(hipMemcpyAsync(host, h->scalars + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         CHS_HIP(hipStreamSynchronize(s));
@@ -542,12 +568,12 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
             set_error("poisson: mixed-precision refinement did not reach its tolerance");
             return CHS_E_NOTCONV;
         }
+        if (!stored) CHS_TRY(residual(true));
         CHS_TRY(poisson_pass<float>(h, h->resid, dph, 2, h->spec32, h->twx32, h->twz32));
         h->last_refine_iters = it + 1;
     }
     if (h->last_refine_iters > 0 && ghost != 2)
         CHS_LAUNCH(k_dph_ghost, dim3(256), dim3(256), 0, s, dph, h->L, h->M, h->N, ghost);
-    (void)ncell;
     return CHS_OK;
 }
 
