@@ -139,6 +139,7 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     h->N = d->ncl3;
     h->LH = h->L / 2 + 1;
     h->LHP = ((h->LH + 7) / 8) * 8;
+    h->LHP32 = ((h->LH + 15) / 16) * 16;
     h->icase = d->icase;
     h->flowdrvtp = d->flowdrvtp;
     h->precision = d->precision;
@@ -200,7 +201,7 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     if (h->precision == CHS_FP64)
         chk(dalloc(&h->spec, nspec));
     else {
-        chk(dalloc(&h->spec32, nspec));
+        chk(dalloc(&h->spec32, (size_t)h->LHP32 * M * N));
         chk(dalloc(&h->resid, ncell));
     }
     h->npart = 65536;

@@ -178,6 +178,7 @@ inline unsigned row_tile_blocks(int L, int N, int nj) {
 // The handle behind the opaque chs_handle of the C ABI.
 struct chs_handle {
     int L, M, N, LH, LHP;
+    int LHP32;           // row length of the FP32 spectrum (LH rounded up to 16)
     int icase, flowdrvtp, precision;
     double dxi, dzi, dxqi, dzqi, cvisc, volm;
     hipStream_t stream;

@@ -155,13 +155,21 @@ __global__ void __launch_bounds__(128) k_fft256_x_fwd(const double* __restrict__
     C* sh = reinterpret_cast<C*>(smem_raw);
     const int g = threadIdx.x >> 4, q = threadIdx.x & 15;
     const int ra = blockIdx.x * 16 + 2 * g, rb = ra + 1;
-    C v[16];
+    // all 32 loads are issued before any is converted: for FP32 the compiler
+    // would otherwise convert as it loads, a few loads in flight at a time
+    double la[16], lb[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1) {
         const int i = 16 * n1 + q;
-        const Rl a = ra < nrows ? (Rl)f[(size_t)ra * 256 + i] : Rl(0);
-        const Rl b = rb < nrows ? (Rl)f[(size_t)rb * 256 + i] : Rl(0);
-        v[n1] = C{a, b};
+        la[n1] = ra < nrows ? f[(size_t)ra * 256 + i] : 0.0;
+        lb[n1] = rb < nrows ? f[(size_t)rb * 256 + i] : 0.0;
+    }
+    asm volatile("" ::: "memory");
+    C v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        asm volatile("" : "+v"(la[n1]), "+v"(lb[n1]));   // conversions stay behind every load
+        v[n1] = C{(Rl)la[n1], (Rl)lb[n1]};
     }
     C* buf = sh + g * F16T;
     fft256_regs<C, Rl>(v, q, buf, tw, -1);
@@ -183,16 +191,17 @@ __global__ void __launch_bounds__(128) k_fft256_x_fwd(const double* __restrict__
     }
 }
 
-// z transform, N = 256, in place on S: 8 kx columns of one j per block of 128 threads
-template <typename Rl>
-__global__ void __launch_bounds__(128) k_fft256_z(cpx_t<Rl>* __restrict__ S, int M, int LH, int LHP, int dir,
-                                                  const cpx_t<Rl>* __restrict__ tw) {
+// z transform, N = 256, in place on S: CB kx columns of one j per block of
+// 16 CB threads; CB columns span 128 bytes of a row (8 FP64, 16 FP32 values)
+template <typename Rl, int CB>
+__global__ void __launch_bounds__(16 * CB) k_fft256_z(cpx_t<Rl>* __restrict__ S, int M, int LH, int LHP, int dir,
+                                                      const cpx_t<Rl>* __restrict__ tw) {
     using C = cpx_t<Rl>;
     extern __shared__ unsigned char smem_raw[];
     C* sh = reinterpret_cast<C*>(smem_raw);
-    const int c = threadIdx.x & 7, q = threadIdx.x >> 3;
+    const int c = threadIdx.x % CB, q = threadIdx.x / CB;
     const int j = blockIdx.y;
-    const int kx = blockIdx.x * 8 + c;
+    const int kx = blockIdx.x * CB + c;
     const bool ok = kx < LH;
     const size_t kst = (size_t)M * LHP;
     C* col = S + (size_t)j * LHP + (ok ? kx : 0);
@@ -360,10 +369,12 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const cpx_t<Rl>* __restrict__
 
 // ---- FP64 residual of the discrete operator (CHECK_FFT_SOLVER,
 // src/PRESSURE_CORRECTION.f90:169-199): r = f - L_h(phi), with the Neumann
-// wall rule for the y ghosts; also the block max of |r| and |f|.  Rows of
-// 64 i x 4 k; the grid (a multiple of 8 blocks) splits the tiles into 8
-// contiguous ranges, one per XCD, whose blocks walk them in j order so the
-// j +- 1 rows of phi are read from that XCD's L2.  STORE = false only checks
+// wall rule for the y ghosts; also the block max of |r| and |f|.  Tiles of
+// 64 i x 4 k of one row j; the grid (a multiple of 8 blocks) splits the
+// tiles into 8 contiguous ranges, one per XCD, whose blocks walk them in j
+// order so the j +- 1 rows of phi are read from that XCD's L2.  A block takes
+// U consecutive tiles per step and issues their loads together (one tile per
+// step leaves the loop latency-bound at ~3 TB/s).  STORE = false only checks
 // (no r written).  32-bit offsets (chs_create bounds the arrays below 2^31).
 template <bool STORE>
 __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restrict__ phi, const double* __restrict__ f,
@@ -371,6 +382,7 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
                                                           double dzqi, const double* __restrict__ amph,
                                                           const double* __restrict__ acph,
                                                           const double* __restrict__ apph, double* __restrict__ part) {
+    constexpr int U = 4;   // tiles per step, their loads issued together
     __shared__ double red[2][4];
     const int ni = (L + 63) / 64, nk = (N + 3) / 4;
     const int nt = ni * M * nk, per = (nt + 7) / 8;
@@ -379,28 +391,48 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
     const int li = threadIdx.x & 63, lk = threadIdx.x >> 6;
     const int LP = L * (M + 2);
     double mr = 0.0, mf = 0.0;
-    for (int t = xcd * per + (int)blockIdx.x / 8; t < t1; t += nq) {
-        const int it = t % ni, rr = t / ni;
-        const int j = rr % M, kt = rr / M;
-        const int i = it * 64 + li, k = kt * 4 + lk;
-        if (i < L && k < N) {
+    for (int t0 = xcd * per + (int)blockIdx.x / 8 * U; t0 < t1; t0 += nq * U) {
+        double pc[U], pn[U], ps[U], pe[U], pw[U], pt[U], pb[U], fv[U];
+        int jj[U], fo[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            // out-of-range tiles and cells evaluate a valid cell and are masked
+            const int t = min(t0 + u, t1 - 1);
+            const int it = t % ni, rr = t / ni;
+            const int j = rr % M, kt = rr / M;
+            const int i0 = it * 64 + li, k0 = kt * 4 + lk;
+            ok[u] = t0 + u < t1 && i0 < L && k0 < N;
+            const int i = min(i0, L - 1), k = min(k0, N - 1);
             const int ip = i == L - 1 ? 1 - L : 1, im = i == 0 ? L - 1 : -1;
             const int kp = k == N - 1 ? (1 - N) * LP : LP, km = k == 0 ? (N - 1) * LP : -LP;
             const int c = i + L * (j + 1) + LP * k;
-            const int fi = i + L * (j + M * k);
+            fo[u] = i + L * (j + M * k);
+            jj[u] = j;
             // the ghost rows are in the array: load, then apply the wall rule
-            const double pc = phi[c], pn = phi[c + L], ps = phi[c - L];
-            const double pe = phi[c + ip], pw = phi[c + im], pt = phi[c + kp], pb = phi[c + km];
-            const double fv = f[fi];
-            const double pjp = j == M - 1 ? pc : pn;
-            const double pjm = j == 0 ? pc : ps;
-            const double lx = (pe - 2.0 * pc + pw) * dxqi;
-            const double ly = pjp * ldc(apph, j) + pc * ldc(acph, j) + pjm * ldc(amph, j);
-            const double lz = (pt - 2.0 * pc + pb) * dzqi;
-            const double rv = fv - (lx + ly + lz);
-            if (STORE) r[fi] = rv;
-            mr = fmax(mr, fabs(rv));
-            mf = fmax(mf, fabs(fv));
+            pc[u] = phi[c];
+            pn[u] = phi[c + L];
+            ps[u] = phi[c - L];
+            pe[u] = phi[c + ip];
+            pw[u] = phi[c + im];
+            pt[u] = phi[c + kp];
+            pb[u] = phi[c + km];
+            fv[u] = f[fo[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = jj[u];
+            const double pjp = j == M - 1 ? pc[u] : pn[u];
+            const double pjm = j == 0 ? pc[u] : ps[u];
+            const double lx = (pe[u] - 2.0 * pc[u] + pw[u]) * dxqi;
+            const double ly = pjp * ldc(apph, j) + pc[u] * ldc(acph, j) + pjm * ldc(amph, j);
+            const double lz = (pt[u] - 2.0 * pc[u] + pb[u]) * dzqi;
+            const double rv = fv[u] - (lx + ly + lz);
+            if (ok[u]) {
+                if (STORE) r[fo[u]] = rv;
+                mr = fmax(mr, fabs(rv));
+                mf = fmax(mf, fabs(fv[u]));
+            }
         }
     }
 #pragma unroll
@@ -456,7 +488,10 @@ bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, 
 template <typename Rl>
 static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, cpx_t<Rl>* S, const cpx_t<Rl>* twx,
                         const cpx_t<Rl>* twz) {
-    const int L = h->L, M = h->M, N = h->N, LH = h->LH, LHP = h->LHP;
+    // FP32 spectra use rows of a multiple of 16 values (128-byte aligned rows)
+    constexpr bool f32 = sizeof(Rl) == 4;
+    constexpr int ZCB = f32 ? 16 : 8;   // kx columns per z block: 128 bytes
+    const int L = h->L, M = h->M, N = h->N, LH = h->LH, LHP = f32 ? h->LHP32 : h->LHP;
     const hipStream_t s = h->stream;
     FftPlan px, pz;
     fft_plan(L, &px);
@@ -469,8 +504,8 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
     const bool x256 = L == 256, z256 = N == 256;
     const dim3 gx16((nrows + 15) / 16);
     const size_t sm16x = sizeof(cpx_t<Rl>) * (size_t)8 * F16T;
-    const size_t sm16z = sizeof(cpx_t<Rl>) * (size_t)8 * (F16T + 1);
-    const dim3 gz16((LH + 7) / 8, M);
+    const size_t sm16z = sizeof(cpx_t<Rl>) * (size_t)ZCB * (F16T + 1);
+    const dim3 gz16((LH + ZCB - 1) / ZCB, M);
     h->timer.begin("fft_x_fwd", s, &ev);
     if (x256)
         CHS_LAUNCH(k_fft256_x_fwd<Rl>, gx16, dim3(128), sm16x, s, f, S, nrows, LHP, twx);
@@ -481,7 +516,7 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
     const size_t smz = sizeof(cpx_t<Rl>) * (size_t)2 * ZTX * N;
     h->timer.begin("fft_z", s, &ev);
     if (z256)
-        CHS_LAUNCH(k_fft256_z<Rl>, gz16, dim3(128), sm16z, s, S, M, LH, LHP, -1, twz);
+        CHS_LAUNCH((k_fft256_z<Rl, ZCB>), gz16, dim3(16 * ZCB), sm16z, s, S, M, LH, LHP, -1, twz);
     else
         CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, -1, pz, twz);
     h->timer.end(ev, "fft_z", s);
@@ -504,7 +539,7 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
     h->timer.end(ev, "poisson_y", s);
     h->timer.begin("fft_z", s, &ev);
     if (z256)
-        CHS_LAUNCH(k_fft256_z<Rl>, gz16, dim3(128), sm16z, s, S, M, LH, LHP, +1, twz);
+        CHS_LAUNCH((k_fft256_z<Rl, ZCB>), gz16, dim3(16 * ZCB), sm16z, s, S, M, LH, LHP, +1, twz);
     else
         CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, +1, pz, twz);
     h->timer.end(ev, "fft_z", s);
