@@ -121,7 +121,7 @@ def test_poisson_fp64(solver_mod, dims):
     s.close()
 
 
-@pytest.mark.parametrize("dims", [(16, 12, 16), (64, 64, 64), (256, 192, 256)])
+@pytest.mark.parametrize("dims", [(16, 12, 16), (12, 10, 20), (64, 64, 64), (256, 192, 256)])
 def test_poisson_mixed_refined(solver_mod, dims):
     print(f"poisson mixed {dims}", flush=True)
     g = ChannelGrid(*dims)
