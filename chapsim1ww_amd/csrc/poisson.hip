@@ -286,21 +286,26 @@ __global__ void __launch_bounds__(768) k_poisson_y(cpx_t<Rl>* __restrict__ S, in
     const double lam = -ak1[kx] - ak3[kz];
     double a[MS], b[MS], c[MS];
     double2 d[MS];
+    // branch-free: rows past M (and dummy lines) load row M - 1 of a valid
+    // line and are replaced by identity rows, so all the segment's loads are
+    // issued before the first wait (a branch per row serialised them)
+    cpx_t<Rl> sv[MS];
+    double av[MS], bv[MS], cv[MS];
 #pragma unroll
     for (int l = 0; l < MS; ++l) {
-        const int j = seg * MS + l;
-        if (valid && j < M) {
-            const cpx_t<Rl> v = S[base + (size_t)j * LHP];
-            d[l] = make_double2((double)v.x * scale, (double)v.y * scale);
-            a[l] = amph[j];
-            c[l] = apph[j];
-            b[l] = acph[j] + lam;
-        } else {
-            a[l] = 0.0;
-            b[l] = 1.0;
-            c[l] = 0.0;
-            d[l] = make_double2(0.0, 0.0);
-        }
+        const int jc = min(seg * MS + l, M - 1);
+        sv[l] = S[base + (size_t)jc * LHP];
+        av[l] = amph[jc];
+        cv[l] = apph[jc];
+        bv[l] = acph[jc];
+    }
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        const bool ok = valid && seg * MS + l < M;
+        d[l] = ok ? make_double2((double)sv[l].x * scale, (double)sv[l].y * scale) : make_double2(0.0, 0.0);
+        a[l] = ok ? av[l] : 0.0;
+        c[l] = ok ? cv[l] : 0.0;
+        b[l] = ok ? bv[l] + lam : 1.0;
     }
     if (seg == 0) {
         a[0] = 0.0;
