@@ -1187,11 +1187,18 @@ __global__ void __launch_bounds__(1024) k_io_cn(IoGeo g, int idr, int sub_dpg, c
     const double amvr1 = 4.0 / (2.0 / ldc(g.dyci, 1) + 1.0 / ldc(g.dyfi, 0)) * (2.0 * ldc(g.dyfi, 0));
     const double apvrn = 4.0 / (2.0 / ldc(g.dyci, M - 1) + 1.0 / ldc(g.dyfi, M - 1)) * (2.0 * ldc(g.dyfi, M - 1));
     double a[MS], b[MS], c[MS], d[MS];
+    // x and z: branch-free over the rows (rows past n form row n - 1's
+    // coefficients, invalid lines those of line 0, and become identity rows
+    // after), so the loads of all rows are issued before the first wait; y
+    // skips them (its wall-row branches split the loads anyway, and the
+    // clamped form measured slower there)
 #pragma unroll
     for (int l = 0; l < MS; ++l) {
-        const int m = seg * MS + l;
-        a[l] = 0.0; b[l] = 1.0; c[l] = 0.0; d[l] = 0.0;
-        if (!valid || m >= n) continue;
+        if (DIR == 1 && !(valid && seg * MS + l < n)) {
+            a[l] = 0.0; b[l] = 1.0; c[l] = 0.0; d[l] = 0.0;
+            continue;
+        }
+        const int m = min(seg * MS + l, n - 1);
         const int IC = DIR == 0 ? m : li, JC = DIR == 1 ? m + 1 : lj, KC = DIR == 2 ? m : lk;
         const int cc = IC + L * JC + LP * KC;
         const int e = IC + L * (JC - 1) + L * M * KC;
@@ -1199,36 +1206,32 @@ __global__ void __launch_bounds__(1024) k_io_cn(IoGeo g, int idr, int sub_dpg, c
         if (DIR == 0) {
             const int IM = IC == 0 ? L - 1 : IC - 1, IP = IC == L - 1 ? 0 : IC + 1;
             const int cm = cc - IC + IM, cp = cc - IC + IP;
-            if (idr == 1) {
-                AMU = -cf.c1 * MU[cm];
-                APU = -cf.c1 * MU[cc];
-            } else {
-                const double* X = MSTG + (size_t)(idr - 2) * CS;   // MU_STG(., 1) for v, (., 2) for w
-                AMU = -cf.c2 * X[cc];
-                APU = -cf.c2 * X[cp];
-            }
+            // u: MU at (IM, IC), coefficient c1; v, w: MU_STG(., 1 | 2) at (IC, IP), c2
+            const bool u = idr == 1;
+            const double* X = u ? MU : MSTG + (size_t)(idr - 2) * CS;
+            const double k = u ? cf.c1 : cf.c2;
+            AMU = -k * X[u ? cm : cc];
+            APU = -k * X[u ? cc : cp];
             am = AMU / RSc[cm];
             ap = APU / RSc[cp];
             ac = 1.0 - (AMU + APU) / RSc[cc];
         } else if (DIR == 2) {
             const int KM = KC == 0 ? N - 1 : KC - 1, KP = KC == N - 1 ? 0 : KC + 1;
             const int cm = cc + LP * (KM - KC), cp = cc + LP * (KP - KC);
-            if (idr == 3) {
-                AMU = -cf.c1 * MU[cm] * 1.0;
-                APU = -cf.c1 * MU[cc] * 1.0;
-            } else {
-                const double* X = MSTG + (size_t)(idr == 1 ? 1 : 2) * CS;   // MU_STG(., 2) for u, (., 3) for v
-                AMU = -cf.c2 * X[cc] * 1.0;
-                APU = -cf.c2 * X[cp] * 1.0;
-            }
+            // w: MU at (KM, KC), c1; u, v: MU_STG(., 2 | 3) at (KC, KP), c2
+            const bool w = idr == 3;
+            const double* X = w ? MU : MSTG + (size_t)(idr == 1 ? 1 : 2) * CS;
+            const double k = w ? cf.c1 : cf.c2;
+            AMU = -k * X[w ? cm : cc] * 1.0;
+            APU = -k * X[w ? cc : cp] * 1.0;
             am = AMU / RSc[cm];
             ap = APU / RSc[cp];
             ac = 1.0 - (AMU + APU) / RSc[cc];
         } else {
-            const int cm = cc - L, cp = cc + L;
+            const int cm = cc - L, cp = cc + L;   // rows 0 and M + 1 are the ghost rows
             const double amv = cf.amvr[(idr - 1) * (M + 1) + JC - 1], apv = cf.apvr[(idr - 1) * (M + 1) + JC - 1];
+            const double* FM = MSTG + (size_t)(idr - 1) * CS;
             if (idr == 1 || idr == 3) {
-                const double* FM = MSTG + (size_t)(idr - 1) * CS;
                 if (JC == 1) {
                     const double MUQT = (MU[cc] + MU[cm]) * 0.5;
                     AMU = -cf.c2 * amvr1 * MUQT;
@@ -1260,6 +1263,12 @@ __global__ void __launch_bounds__(1024) k_io_cn(IoGeo g, int idr, int sub_dpg, c
         b[l] = ac;
         c[l] = ap;
         d[l] = (DIR == 0 && (idr != 2 || JC >= 2)) ? R[e] - dp : R[e];
+    }
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        if (!(valid && seg * MS + l < n)) {
+            a[l] = 0.0; b[l] = 1.0; c[l] = 0.0; d[l] = 0.0;
+        }
     }
     if (DIR == 1) {   // zero boundary values: the end rows lose their outer coefficient
         if (seg == 0) a[0] = 0.0;

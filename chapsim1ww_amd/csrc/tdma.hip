@@ -21,16 +21,19 @@ __global__ void __launch_bounds__(1024) k_tdma_seg(const double* __restrict__ A,
     const bool valid = line < nlines;
     double a[MS], b[MS], c[MS], d[MS];
     const long long base = valid ? line * ls : 0;
+    // branch-free loads (rows past n read row n - 1 of a valid line), all
+    // issued before the first wait; padding rows become identity rows
 #pragma unroll
     for (int l = 0; l < MS; ++l) {
-        const int j = seg * MS + l;
-        if (valid && j < n) {
-            const long long o = base + (long long)j * es;
-            a[l] = A[o];
-            b[l] = B[o];
-            c[l] = Cc[o];
-            d[l] = Rr[o];
-        } else {  // padding rows: identity, decoupled
+        const long long o = base + (long long)min(seg * MS + l, n - 1) * es;
+        a[l] = A[o];
+        b[l] = B[o];
+        c[l] = Cc[o];
+        d[l] = Rr[o];
+    }
+#pragma unroll
+    for (int l = 0; l < MS; ++l) {
+        if (!(valid && seg * MS + l < n)) {   // padding rows: identity, decoupled
             a[l] = 0.0;
             b[l] = 1.0;
             c[l] = 0.0;
