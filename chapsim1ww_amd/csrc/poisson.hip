@@ -133,12 +133,23 @@ __device__ __forceinline__ void dft16(C (&v)[16], int dir) {
 // transform's F16T values of LDS; all threads of the block must call it.
 template <typename C, typename Rl>
 __device__ __forceinline__ void fft256_regs(C (&v)[16], int q, C* buf, const C* __restrict__ tw, int dir) {
+    // the twiddles W256^(n2 k1) (n2 k1 <= 225) are loaded before the first
+    // 16-point DFT so their latency hides behind it (loaded where they are
+    // used, each load waited on its own)
+    C w[16];
+#pragma unroll
+    for (int k1 = 1; k1 < 16; ++k1) w[k1] = tw[q * k1];
+    asm volatile("" ::: "memory");
     dft16<C, Rl>(v, dir);
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) {
-        C w = tw[q * k1];   // W256^(n2 k1), n2 k1 <= 225
-        if (dir > 0) w.y = -w.y;
-        buf[k1 * F16S + q] = k1 == 0 ? v[0] : cmul(v[k1], w);
+        if (k1 == 0) {
+            buf[q] = v[0];
+            continue;
+        }
+        C wk = w[k1];
+        if (dir > 0) wk.y = -wk.y;
+        buf[k1 * F16S + q] = cmul(v[k1], wk);
     }
     __syncthreads();
 #pragma unroll
@@ -249,13 +260,17 @@ __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restric
         double* base = dph + (size_t)k * plane + q;     // column q of plane k
         double* rowp = base + (size_t)(jj + 1) * 256;
         const bool lo = jj == 0, hi = jj == M - 1;
+        if (ghost == 2) {   // add to phi: all 16 old values loaded before the first add
+            double old[16];
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) old[k2] = rowp[16 * k2];
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) rowp[16 * k2] = old[k2] + (h ? (double)v[k2].y : (double)v[k2].x);
+            continue;
+        }
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
             const double val = h ? (double)v[k2].y : (double)v[k2].x;
-            if (ghost == 2) {
-                rowp[16 * k2] += val;
-                continue;
-            }
             rowp[16 * k2] = val;
             if (lo) base[16 * k2] = ghost == 1 ? val : 0.0;
             if (hi) base[(size_t)(M + 1) * 256 + 16 * k2] = ghost == 1 ? val : 0.0;
