@@ -760,8 +760,9 @@ constexpr int KMAX = 16;
 // (64 consecutive i) applies the same row weights W_c[j][.] (scalar loads).
 // wy rows are zero-padded to 2KW+1 entries (zero outside [0, M)).
 constexpr int YJC = 64;     // rows per thread chunk of the y window (measured: 32 -> 64 rows, 257 -> 240 us)
-constexpr int WUY = 2;      // rows fetched ahead by the y window (measured at 256x192x256: 4 -> 2, 240 -> 214 us)
-constexpr int WU = 4;       // rows fetched ahead per step of the z window (2: 140 -> 153 us)
+constexpr int YNS = 4;      // prefetch slots (rows fetched ahead) of the y window (2 -> 4: 200 -> 192 us)
+constexpr int WU = 4;       // rows fetched ahead per step of the z window (2: 140 -> 153 us; slots
+                            // refilled a step ahead as in the y window: 133 -> 141 us)
 template <int KW>
 // One wave per block (chunk blockIdx.z): its LDS holds only the chunk's
 // weight rows, so more blocks fit a CU.
@@ -799,32 +800,37 @@ __global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict
             if (jj >= 0 && jj < M) v = (c == 1 && jj == 0) ? du0 : Rc[(size_t)jj * L];
             win[m] = v;
         }
-        // prefetch queues: the window rows and the Q rows WUY steps ahead
-        double rq[WUY], qq[WUY];
+        // YNS fixed prefetch slots, slot u serving the steps j = j0 + u mod
+        // YNS (no register rotation, which made the compiler wait on the
+        // pending loads): the window row entering at step j and Q's row j,
+        // loaded YNS steps ahead with clamped indices (no branch between the
+        // loads); rows past M enter the window as zeros
+        double rs[YNS], qs[YNS];
 #pragma unroll
-        for (int u = 0; u < WUY; ++u) {
-            const int jn = j0 + u + KW + 1;
-            rq[u] = jn < M ? Rc[(size_t)jn * L] : 0.0;
-            qq[u] = j0 + u < j1 ? Q[qidx(g, c, i, j0 + u + 1, k)] : 0.0;
+        for (int u = 0; u < YNS; ++u) {
+            rs[u] = Rc[(size_t)min(j0 + KW + 1 + u, M - 1) * L];
+            qs[u] = Q[qidx(g, c, i, min(j0 + u, M - 1) + 1, k)];
         }
-        for (int j = j0; j < j1; ++j) {
+        auto step = [&](int j, double& r, double& q) {
             const double* w = sw + (size_t)(j - j0) * W;
             double acc = 0.0;
 #pragma unroll
             for (int m = 0; m < W; ++m) acc += w[m] * win[m];
-            if (!(c == 1 && j == 0)) Q[qidx(g, c, i, j + 1, k)] = acc + qq[0];
+            if (!(c == 1 && j == 0)) Q[qidx(g, c, i, j + 1, k)] = acc + q;
 #pragma unroll
             for (int m = 0; m < W - 1; ++m) win[m] = win[m + 1];
-            win[W - 1] = rq[0];
+            win[W - 1] = j + KW + 1 < M ? r : 0.0;
+            r = Rc[(size_t)min(j + KW + 1 + YNS, M - 1) * L];
+            q = Q[qidx(g, c, i, min(j + YNS, M - 1) + 1, k)];
+        };
+        int j = j0;
+        for (; j + YNS - 1 < j1; j += YNS) {
 #pragma unroll
-            for (int u = 0; u < WUY - 1; ++u) {
-                rq[u] = rq[u + 1];
-                qq[u] = qq[u + 1];
-            }
-            const int jn = j + WUY + KW + 1, jq = j + WUY;
-            rq[WUY - 1] = jn < M ? Rc[(size_t)jn * L] : 0.0;
-            qq[WUY - 1] = jq < j1 ? Q[qidx(g, c, i, jq + 1, k)] : 0.0;
+            for (int u = 0; u < YNS; ++u) step(j + u, rs[u], qs[u]);
         }
+#pragma unroll
+        for (int u = 0; u < YNS - 1; ++u)
+            if (j + u < j1) step(j + u, rs[u], qs[u]);
         if (ch == 0) {
             Q[qidx(g, c, i, 0, k)] = 0.0;
             if (c == 1) Q[qidx(g, 1, i, 1, k)] = 0.0;
