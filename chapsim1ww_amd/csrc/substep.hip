@@ -939,7 +939,10 @@ __global__ void __launch_bounds__(256) k_update(Geo g, double cu, double cw, dou
     const double cvj = cv * ldc(g.dyci, j) / 1.0;
     Q[go] = qu - (ph - dxm) * cu;
     Q[go + 2 * CS] = qw - (ph - dkm) * cw;
-    if (j >= 1) Q[go + CS] = qv - (ph - djm) * cvj;
+    // v's wall row (j = 0) is written back unchanged: an unconditional store
+    // keeps its load with the others (under a branch the compiler issued it
+    // after the stores and waited for both)
+    Q[go + CS] = j >= 1 ? qv - (ph - djm) * cvj : qv;
     const double p = pr + ph - be * f;
     PR[go] = p;
     if (j == 0) PR[go - L] = p;

@@ -174,7 +174,7 @@ def _run_pair(dims, nsteps, seed=1973, precision="fp64", solver_mod=None, cn="au
 @pytest.mark.parametrize("dims,cn,path", [((32, 16, 16), "auto", "thomas"), ((64, 64, 64), "auto", "stencil"),
                                           ((64, 64, 64), "thomas", "thomas"), ((128, 96, 64), "auto", "stencil"),
                                           ((64, 64, 64), "rowtile", "stencil"), ((32, 40, 64), "auto", "stencil"),
-                                          ((32, 48, 128), "auto", "stencil")])
+                                          ((32, 48, 128), "auto", "stencil"), ((32, 43, 64), "auto", "stencil")])
 def test_substep_fp64(solver_mod, dims, cn, path):
     """7 substeps (NS = 0, then 1,2,3 twice) against the oracle; every state
     field within 1e-12 of the oracle after every substep."""
