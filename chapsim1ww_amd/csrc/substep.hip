@@ -944,33 +944,53 @@ __global__ void __launch_bounds__(256) k_divg(Geo g, double coe0, const double* 
 
 // VELOUPDT_tg (src/VELO_CORRECTION.f90:37-75) + PRCALC_tg
 // (src/PRESSURE_CORRECTION.f90:11-20) + BC_WALL_Q_tg / BC_WALL_PR_tg, in the
-// row tiles of k_divg.  Every load is issued before any store (v's and the
-// ghost rows' conditions are uniform over a tile).
+// row-group tiles of k_divg: UPJ rows of j per thread, DPH's row J-1 of one
+// cell being row J of the previous.  Every load is issued before any store
+// (rows past M are clamped into the array and not stored; v's wall row is
+// written back unchanged, so no store sits under a per-row branch).
+constexpr int UPJ = 1;   // 2 or 4 rows per thread measured no faster (189 -> 189, 192 us)
 __global__ void __launch_bounds__(256) k_update(Geo g, double cu, double cw, double cv, double be,
                                                 const double* __restrict__ DPH, const double* __restrict__ F,
                                                 double* __restrict__ Q, double* __restrict__ PR) {
-    int it, j, kt, tile;
-    if (!row_tile(g.L, g.N, g.M, &it, &j, &kt, &tile)) return;
-    const int L = g.L, M = g.M, N = g.N;
+    int it, jg, kt, tile;
+    const int M = g.M;
+    if (!row_tile(g.L, g.N, (M + UPJ - 1) / UPJ, &it, &jg, &kt, &tile)) return;
+    const int L = g.L, N = g.N;
     const int i = it * 64 + (threadIdx.x & 63), k = kt * 4 + (threadIdx.x >> 6);
     if (i >= L || k >= N) return;
     const int LP = L * (M + 2), CS = LP * N;
     const int im = i == 0 ? L - 1 : -1, km = k == 0 ? (N - 1) * LP : -LP;
-    const int go = i + L * (j + 1) + LP * k;  // (i, J, k); J-1 >= 0 is always in the array
-    const double ph = DPH[go], dxm = DPH[go + im], dkm = DPH[go + km], djm = DPH[go - L];
-    const double qu = Q[go], qv = Q[go + CS], qw = Q[go + 2 * CS];
-    const double f = F[i + L * (j + M * k)], pr = PR[go];
-    const double cvj = cv * ldc(g.dyci, j) / 1.0;
-    Q[go] = qu - (ph - dxm) * cu;
-    Q[go + 2 * CS] = qw - (ph - dkm) * cw;
-    // v's wall row (j = 0) is written back unchanged: an unconditional store
-    // keeps its load with the others (under a branch the compiler issued it
-    // after the stores and waited for both)
-    Q[go + CS] = j >= 1 ? qv - (ph - djm) * cvj : qv;
-    const double p = pr + ph - be * f;
-    PR[go] = p;
-    if (j == 0) PR[go - L] = p;
-    if (j == M - 1) PR[go + L] = p;
+    const int j0 = jg * UPJ;
+    const int g0 = i + L * (j0 + 1) + LP * k;  // (i, J0, k); J0 - 1 >= 0 is always in the array
+    double ph[UPJ + 1], dxm[UPJ], dkm[UPJ], qu[UPJ], qv[UPJ], qw[UPJ], f[UPJ], pr[UPJ];
+    ph[0] = DPH[g0 - L];
+#pragma unroll
+    for (int r = 0; r < UPJ; ++r) {
+        const int jc = min(j0 + r, M - 1);
+        const int go = g0 + L * (jc - j0);
+        ph[r + 1] = DPH[go];
+        dxm[r] = DPH[go + im];
+        dkm[r] = DPH[go + km];
+        qu[r] = Q[go];
+        qv[r] = Q[go + CS];
+        qw[r] = Q[go + 2 * CS];
+        f[r] = F[i + L * (jc + M * k)];
+        pr[r] = PR[go];
+    }
+#pragma unroll
+    for (int r = 0; r < UPJ; ++r) {
+        const int j = j0 + r;
+        if (j >= M) break;
+        const int go = g0 + L * r;
+        const double cvj = cv * ldc(g.dyci, j) / 1.0;
+        Q[go] = qu[r] - (ph[r + 1] - dxm[r]) * cu;
+        Q[go + 2 * CS] = qw[r] - (ph[r + 1] - dkm[r]) * cw;
+        Q[go + CS] = j >= 1 ? qv[r] - (ph[r + 1] - ph[r]) * cvj : qv[r];
+        const double p = pr[r] + ph[r + 1] - be * f[r];
+        PR[go] = p;
+        if (j == 0) PR[go - L] = p;
+        if (j == M - 1) PR[go + L] = p;
+    }
 }
 
 __device__ __forceinline__ double nanmax(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
@@ -1437,7 +1457,6 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         h->timer.end(ev, "y_solve", s);
     }
     // 4. DIVG
-    const dim3 gt(row_tile_blocks(h->L, h->N, h->M));
     h->timer.begin("divg", s, &ev);
     CHS_LAUNCH(k_divg, dim3(row_tile_blocks(h->L, h->N, (h->M + DVJ - 1) / DVJ)), dim3(256), 0, s, g,
                1.0 / (dt * rk.talp), h->q, h->rhsllphi);
@@ -1447,7 +1466,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (rk.talp * dt)));
     // 6. projection + pressure update
     h->timer.begin("update", s, &ev);
-    CHS_LAUNCH(k_update, gt, dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
+    CHS_LAUNCH(k_update, dim3(row_tile_blocks(h->L, h->N, (h->M + UPJ - 1) / UPJ)), dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
                0.5 * rk.talp * dt * h->cvisc, h->dph, h->rhsllphi, h->q, h->pr);
     h->timer.end(ev, "update", s);
     return CHS_OK;
