@@ -390,11 +390,11 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const cpx_t<Rl>* __restrict__
 // ---- FP64 residual of the discrete operator (CHECK_FFT_SOLVER,
 // src/PRESSURE_CORRECTION.f90:169-199): r = f - L_h(phi), with the Neumann
 // wall rule for the y ghosts; also the block max of |r| and |f|.  Tiles of
-// 64 i x 4 k of one row j; the grid (a multiple of 8 blocks) splits the
+// 64 i x 4 k x RJ rows of j; the grid (a multiple of 8 blocks) splits the
 // tiles into 8 contiguous ranges, one per XCD, whose blocks walk them in j
-// order so the j +- 1 rows of phi are read from that XCD's L2.  A block takes
-// U consecutive tiles per step and issues their loads together (one tile per
-// step leaves the loop latency-bound at ~3 TB/s).  STORE = false only checks
+// order so the j +- 1 rows of phi are read from that XCD's L2.  A thread takes
+// RJ rows of its (i, k) column per tile, sharing the y neighbours, and issues
+// the tile's loads together.  STORE = false only checks
 // (no r written).  32-bit offsets (chs_create bounds the arrays below 2^31).
 template <bool STORE>
 __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restrict__ phi, const double* __restrict__ f,
@@ -402,54 +402,55 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
                                                           double dzqi, const double* __restrict__ amph,
                                                           const double* __restrict__ acph,
                                                           const double* __restrict__ apph, double* __restrict__ part) {
-    constexpr int U = 4;   // tiles per step, their loads issued together
+    constexpr int RJ = 4;   // rows of j per thread and tile
     __shared__ double red[2][4];
-    const int ni = (L + 63) / 64, nk = (N + 3) / 4;
-    const int nt = ni * M * nk, per = (nt + 7) / 8;
+    const int ni = (L + 63) / 64, nk = (N + 3) / 4, ng = (M + RJ - 1) / RJ;
+    const int nt = ni * ng * nk, per = (nt + 7) / 8;
     const int xcd = blockIdx.x % 8, nq = gridDim.x / 8;
     const int t1 = min(nt, (xcd + 1) * per);
     const int li = threadIdx.x & 63, lk = threadIdx.x >> 6;
     const int LP = L * (M + 2);
     double mr = 0.0, mf = 0.0;
-    for (int t0 = xcd * per + (int)blockIdx.x / 8 * U; t0 < t1; t0 += nq * U) {
-        double pc[U], pn[U], ps[U], pe[U], pw[U], pt[U], pb[U], fv[U];
-        int jj[U], fo[U];
-        bool ok[U];
+    for (int t = xcd * per + (int)blockIdx.x / 8; t < t1; t += nq) {
+        const int it = t % ni, rr = t / ni;
+        const int jg = rr % ng, kt = rr / ng;
+        const int i0 = it * 64 + li, k0 = kt * 4 + lk;
+        const bool okc = i0 < L && k0 < N;
+        // cells outside the grid and rows past M evaluate a valid cell and are masked
+        const int i = min(i0, L - 1), k = min(k0, N - 1);
+        const int ip = i == L - 1 ? 1 - L : 1, im = i == 0 ? L - 1 : -1;
+        const int kp = k == N - 1 ? (1 - N) * LP : LP, km = k == 0 ? (N - 1) * LP : -LP;
+        const int j0 = jg * RJ;
+        const int c0 = i + L * (j0 + 1) + LP * k;   // stored row j0 + 1 (interior row j0)
+        // the column's stored rows j0 .. j0 + RJ + 1 (row M + 1 is the ghost
+        // row): each cell's j - 1, j, j + 1, shared with its neighbours
+        double py[RJ + 2], pe[RJ], pw[RJ], pt[RJ], pb[RJ], fv[RJ];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // out-of-range tiles and cells evaluate a valid cell and are masked
-            const int t = min(t0 + u, t1 - 1);
-            const int it = t % ni, rr = t / ni;
-            const int j = rr % M, kt = rr / M;
-            const int i0 = it * 64 + li, k0 = kt * 4 + lk;
-            ok[u] = t0 + u < t1 && i0 < L && k0 < N;
-            const int i = min(i0, L - 1), k = min(k0, N - 1);
-            const int ip = i == L - 1 ? 1 - L : 1, im = i == 0 ? L - 1 : -1;
-            const int kp = k == N - 1 ? (1 - N) * LP : LP, km = k == 0 ? (N - 1) * LP : -LP;
-            const int c = i + L * (j + 1) + LP * k;
-            fo[u] = i + L * (j + M * k);
-            jj[u] = j;
-            // the ghost rows are in the array: load, then apply the wall rule
-            pc[u] = phi[c];
-            pn[u] = phi[c + L];
-            ps[u] = phi[c - L];
+        for (int q = 0; q < RJ + 2; ++q) py[q] = phi[c0 + L * (min(j0 + q, M + 1) - j0 - 1)];
+#pragma unroll
+        for (int u = 0; u < RJ; ++u) {
+            const int jc = min(j0 + u, M - 1);
+            const int c = c0 + L * (jc - j0);
             pe[u] = phi[c + ip];
             pw[u] = phi[c + im];
             pt[u] = phi[c + kp];
             pb[u] = phi[c + km];
-            fv[u] = f[fo[u]];
+            fv[u] = f[i + L * (jc + M * k)];
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int j = jj[u];
-            const double pjp = j == M - 1 ? pc[u] : pn[u];
-            const double pjm = j == 0 ? pc[u] : ps[u];
-            const double lx = (pe[u] - 2.0 * pc[u] + pw[u]) * dxqi;
-            const double ly = pjp * ldc(apph, j) + pc[u] * ldc(acph, j) + pjm * ldc(amph, j);
-            const double lz = (pt[u] - 2.0 * pc[u] + pb[u]) * dzqi;
+        for (int u = 0; u < RJ; ++u) {
+            const int j = j0 + u;
+            const double pc = py[u + 1];
+            // the wall rule for the y ghosts
+            const double pjp = j == M - 1 ? pc : py[u + 2];
+            const double pjm = j == 0 ? pc : py[u];
+            const int jl = min(j, M - 1);
+            const double lx = (pe[u] - 2.0 * pc + pw[u]) * dxqi;
+            const double ly = pjp * ldc(apph, jl) + pc * ldc(acph, jl) + pjm * ldc(amph, jl);
+            const double lz = (pt[u] - 2.0 * pc + pb[u]) * dzqi;
             const double rv = fv[u] - (lx + ly + lz);
-            if (ok[u]) {
-                if (STORE) r[fo[u]] = rv;
+            if (okc && j < M) {
+                if (STORE) r[i + L * (j + M * k)] = rv;
                 mr = fmax(mr, fabs(rv));
                 mf = fmax(mf, fabs(fv[u]));
             }
