@@ -107,21 +107,70 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
             in[u] = c < rw.rowlen;
             hv[u] = in[u] ? h[base + c] : 0.0;
         }
+        // S_HT for every cell at once: the H intervals (LDS) and their
+        // coefficient records (L2), loaded together
+        int ih[PCELL];
+        double4 rr[PCELL];
 #pragma unroll
         for (int u = 0; u < PCELL; ++u) {
-            if (!in[u] || fabs(hv[u]) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells as they are
-            ++looked;
-            const long long q = base + c0 + u * PT + threadIdx.x;
-            double T, D, Mu, K, CP;
-            if (!lookup_cell(t, xh, xt, sh, st, hv[u], T, D, Mu, K, CP)) {
-                ++bad;
-                continue;
+            // NaN and out-of-range values search at h_lo (a float-to-int
+            // conversion of them would be undefined); their results are unused
+            const double hq = hv[u] > t.h_lo && hv[u] < t.h_hi ? hv[u] : t.h_lo;
+            ih[u] = find_interval(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hq);
+            rr[u] = reinterpret_cast<const double4*>(t.rh)[ih[u]];
+        }
+        double tv[PCELL];
+        bool okh[PCELL];
+#pragma unroll
+        for (int u = 0; u < PCELL; ++u) {
+            const bool nan = hv[u] != hv[u];   // a NaN passes the range tests and gives NaN
+            okh[u] = nan || (hv[u] > t.h_lo && hv[u] < t.h_hi);
+            tv[u] = nan ? hv[u] : cubic_nc(rr[u].x, rr[u].y, rr[u].z, rr[u].w, hv[u] - xh[ih[u]]);
+        }
+        // S_TD/TM/TK/TCp two cells at a time: both cells' T records are
+        // loaded before either is evaluated
+#pragma unroll
+        for (int u0 = 0; u0 < PCELL; u0 += 2) {
+            int kt[2];
+            double4 rq[2][4];
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const double tq = tv[u0 + v] > t.t_lo && tv[u0 + v] < t.t_hi ? tv[u0 + v] : t.t_lo;
+                kt[v] = find_interval(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tq);
+                const double4* q = reinterpret_cast<const double4*>(t.rt) + 4 * (size_t)kt[v];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) rq[v][m] = q[m];
             }
-            To[q] = T;
-            mu[q] = Mu;
-            kap[q] = K;
-            rho[q] = D;
-            if (cp) cp[q] = CP;
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int u = u0 + v;
+                // THERM_PROP_UPDATE leaves |h| < 1e-10 (and cells past the row) as they are
+                if (!in[u] || fabs(hv[u]) < 1.0e-10) continue;
+                ++looked;
+                const double T = tv[u];
+                const bool nan = T != T;
+                if (!okh[u] || (!nan && (T <= t.t_lo || T >= t.t_hi))) {
+                    ++bad;
+                    continue;
+                }
+                double D, Mu, K, CP;
+                if (nan) {
+                    D = Mu = K = CP = T;
+                } else {
+                    const double dt = T - xt[kt[v]];
+                    D = cubic_nc(rq[v][0].x, rq[v][0].y, rq[v][0].z, rq[v][0].w, dt);
+                    Mu = cubic_nc(rq[v][1].x, rq[v][1].y, rq[v][1].z, rq[v][1].w, dt);
+                    K = cubic_nc(rq[v][2].x, rq[v][2].y, rq[v][2].z, rq[v][2].w, dt);
+                    CP = cubic_nc(rq[v][3].x, rq[v][3].y, rq[v][3].z, rq[v][3].w, dt);
+                    if (fabs(T - 1.0) < 1.0e-10) D = Mu = K = 1.0;
+                }
+                const long long q = base + c0 + u * PT + threadIdx.x;
+                To[q] = T;
+                mu[q] = Mu;
+                kap[q] = K;
+                rho[q] = D;
+                if (cp) cp[q] = CP;
+            }
         }
     }
     // out-of-range count (and, on request, the lookups): one atomic per wave
