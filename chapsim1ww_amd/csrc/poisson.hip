@@ -396,6 +396,10 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const cpx_t<Rl>* __restrict__
 // RJ rows of its (i, k) column per tile, sharing the y neighbours, and issues
 // the tile's loads together.  STORE = false only checks
 // (no r written).  32-bit offsets (chs_create bounds the arrays below 2^31).
+// NaN-propagating maximum: a NaN in phi or f makes the refinement check fail
+// (and the solve report that it did not converge) instead of being skipped
+__device__ __forceinline__ double pmax(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
+
 template <bool STORE>
 __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restrict__ phi, const double* __restrict__ f,
                                                           double* __restrict__ r, int L, int M, int N, double dxqi,
@@ -451,15 +455,15 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
             const double rv = fv[u] - (lx + ly + lz);
             if (okc && j < M) {
                 if (STORE) r[i + L * (j + M * k)] = rv;
-                mr = fmax(mr, fabs(rv));
-                mf = fmax(mf, fabs(fv[u]));
+                mr = pmax(mr, fabs(rv));
+                mf = pmax(mf, fabs(fv[u]));
             }
         }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        mr = fmax(mr, __shfl_xor(mr, o));
-        mf = fmax(mf, __shfl_xor(mf, o));
+        mr = pmax(mr, __shfl_xor(mr, o));
+        mf = pmax(mf, __shfl_xor(mf, o));
     }
     if (li == 0) {
         red[0][lk] = mr;
@@ -467,8 +471,8 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
-        part[2 * blockIdx.x + 1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+        part[2 * blockIdx.x] = pmax(pmax(red[0][0], red[0][1]), pmax(red[0][2], red[0][3]));
+        part[2 * blockIdx.x + 1] = pmax(pmax(red[1][0], red[1][1]), pmax(red[1][2], red[1][3]));
     }
 }
 
@@ -477,16 +481,16 @@ __global__ void k_reduce_max2(const double* __restrict__ part, int nb, double* _
     __shared__ double red[2][256];
     double a = 0.0, b = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-        a = fmax(a, part[2 * i]);
-        b = fmax(b, part[2 * i + 1]);
+        a = pmax(a, part[2 * i]);
+        b = pmax(b, part[2 * i + 1]);
     }
     red[0][threadIdx.x] = a;
     red[1][threadIdx.x] = b;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
-            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
-            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+            red[0][threadIdx.x] = pmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = pmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
         }
         __syncthreads();
     }
@@ -627,6 +631,11 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
         CHS_LAUNCH(k_reduce_max2, dim3(1), dim3(256), 0, s, h->partials, nb, h->scalars + 4);
         CHS_HIP(hipMemcpyAsync(host, h->scalars + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         CHS_HIP(hipStreamSynchronize(s));
+        if (host[0] != host[0] || host[1] != host[1]) {
+            h->last_refine_resid = host[0] + host[1];
+            set_error("poisson: NaN in the mixed-precision residual (phi or the right-hand side)");
+            return CHS_E_NOTCONV;
+        }
         const double rel = host[1] > 0.0 ? host[0] / host[1] : 0.0;
         h->last_refine_resid = rel;
         if (host[0] <= abs_tol || rel <= h->refine_tol) break;

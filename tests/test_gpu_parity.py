@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 from chapsim1ww_amd import fields
+from chapsim1ww_amd import native as nat
 from chapsim1ww_amd.grid import ChannelGrid
 
 torch = pytest.importorskip("torch")
@@ -140,6 +141,21 @@ def test_poisson_mixed_refined(solver_mod, dims):
     assert it >= 1
     # the FP64 oracle solution to the accuracy the residual buys
     assert _rel(out, ref) < 1e-8
+    s.close()
+
+
+def test_poisson_mixed_nan_reported(solver_mod):
+    """A NaN in f reaches the residual maxima (they propagate NaN), so the
+    mixed solve reports it instead of passing its convergence check."""
+    print("poisson mixed nan", flush=True)
+    g = ChannelGrid(16, 12, 16)
+    f = fields.poisson_rhs(g, seed=3)
+    f[2, 3, 4] = np.nan
+    s = solver_mod.ChannelSolver(g, "mixed")
+    dph = torch.zeros(g.L * (g.M + 2) * g.N, dtype=torch.float64, device="cuda")
+    with pytest.raises(nat.ChsError, match="NaN"):
+        s.fft99_pois3d(_dev(f), dph)
+    torch.cuda.synchronize()
     s.close()
 
 
