@@ -83,7 +83,8 @@ int chs_rk3_substep(chs_handle *h, int ns, double dt);
  *            ghost planes set to zero as the reference does (:447).
  * With CHS_MIXED precision the FFTs run in FP32 and the solve is iteratively
  * refined with an FP64 residual until max|L phi - f| <= tol * max|f| (at least
- * one refinement when the iteration cap allows any). */
+ * one refinement when the iteration cap allows any); CHS_E_NOTCONV when the
+ * cap is reached first or the residual is NaN. */
 int chs_poisson_fft99(chs_handle *h, const double *f_dev, double *dph_dev);
 /* Tolerance and iteration cap of the mixed-precision refinement; the last
  * call's iteration count and relative residual are reported by chs_poisson_stats. */
