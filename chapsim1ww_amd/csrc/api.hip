@@ -18,7 +18,7 @@ int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out);
 int mean_xz(chs_handle* h, double* out_host);
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
 bool substep_supported(const chs_handle* h);
-void release_stencils(const chs_handle* h);
+void release_stencils(chs_handle* h);
 int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky);
 bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small = false);
 
@@ -180,6 +180,9 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     chk(upload(&h->amvr, d->amvr, 3 * (M + 1), s));
     chk(upload(&h->acvr, d->acvr, 3 * (M + 1), s));
     chk(upload(&h->apvr, d->apvr, 3 * (M + 1), s));
+    h->h_amvr.assign(d->amvr, d->amvr + 3 * (M + 1));
+    h->h_acvr.assign(d->acvr, d->acvr + 3 * (M + 1));
+    h->h_apvr.assign(d->apvr, d->apvr + 3 * (M + 1));
     chk(upload(&h->ak1, d->ak1, L, s));
     chk(upload(&h->ak3, d->ak3, N, s));
     std::vector<double2> wx = twiddles(h->L), wz = twiddles(h->N);
@@ -277,6 +280,15 @@ int chs_rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_CHECK(substep_supported(h), CHS_E_ARG,
               "chs_rk3_substep: NCL1 must be a power of two >= 32 and NCL3 a multiple of 4");
     return rk3_substep(h, ns, dt);
+}
+
+int chs_set_cvisc(chs_handle* h, double cvisc) {
+    CHS_CHECK(h, CHS_E_ARG, "chs_set_cvisc: null handle");
+    CHS_CHECK(std::isfinite(cvisc) && cvisc > 0.0, CHS_E_ARG, "chs_set_cvisc: CVISC must be positive and finite");
+    // the CN stencils are cached by FACOE = alpha dt CVISC / 2, so the next
+    // substep builds the ones of the new viscosity
+    h->cvisc = cvisc;
+    return CHS_OK;
 }
 
 int chs_set_cn_solver(chs_handle* h, int mode) {

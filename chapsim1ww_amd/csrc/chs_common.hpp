@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -173,6 +174,8 @@ inline unsigned row_tile_blocks(int L, int N, int nj) {
     return (unsigned)(8 * ((nt + 7) / 8));
 }
 
+struct CnCache;   // truncated-inverse CN stencils of one handle (substep.hip)
+
 }  // namespace chs
 
 // The handle behind the opaque chs_handle of the C ABI.
@@ -205,6 +208,11 @@ struct chs_handle {
     float *f32a, *f32b;
     int npart;
     chs::KernelTimer timer;
+    // host copies of AMVR/ACVR/APVR (chs_create) and the handle's cache of
+    // truncated-inverse CN stencils, keyed by FACOE (substep.hip)
+    std::vector<double> h_amvr, h_acvr, h_apvr;
+    std::mutex cn_mu;
+    chs::CnCache* cn_cache;
     // mixed-precision Poisson refinement: tolerance, cap, statistics of the last call
     double refine_tol;
     int refine_max;

@@ -370,7 +370,9 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     h->timer.end(ev, "energy_rhs", s);
     const PropTab t = tab_of(p);
     h->timer.begin("energy_props", s, &ev);
-    CHS_LAUNCH(k_energy_props, dim3(512), dim3(EPT), props_lds_bytes(p), s, t, g, f, p->counters);
+    CHS_TRY(props_lds_attr((const void*)k_energy_props, p));
+    CHS_TRY(props_lds_attr((const void*)k_energy_wall, p));
+    CHS_LAUNCH(k_energy_props, dim3(props_grid(p, 512)), dim3(EPT), props_lds_bytes(p), s, t, g, f, p->counters);
     h->timer.end(ev, "energy_props", s);
     CHS_LAUNCH(k_energy_wall, dim3(64), dim3(256), props_lds_bytes(p), s, t, g, f, p->counters);
     unsigned long long bad = 0;

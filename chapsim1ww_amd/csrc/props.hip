@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
                     Mu = cubic_nc(rq[v][1].x, rq[v][1].y, rq[v][1].z, rq[v][1].w, dt);
                     K = cubic_nc(rq[v][2].x, rq[v][2].y, rq[v][2].z, rq[v][2].w, dt);
                     CP = cubic_nc(rq[v][3].x, rq[v][3].y, rq[v][3].z, rq[v][3].w, dt);
-                    if (fabs(T - 1.0) < 1.0e-10) D = Mu = K = 1.0;
+                    if (fabs(T - 1.0) < 1.0e-10) D = Mu = K = CP = 1.0;
                 }
                 const long long q = base + c0 + u * PT + threadIdx.x;
                 To[q] = T;
@@ -189,7 +189,8 @@ int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double
     const long long chunk = (long long)PT * PCELL;
     const long long nch = ((rw.rowlen + chunk - 1) / chunk) * rw.nrow;
     if (nch == 0) return CHS_OK;
-    const int grid = (int)std::min<long long>(nch, 512);   // two resident blocks per CU
+    const int grid = props_grid(p, nch);
+    CHS_TRY(props_lds_attr((const void*)k_props_table, p));
     CHS_LAUNCH(k_props_table, dim3(grid), dim3(PT), props_lds_bytes(p), p->stream, tab_of(p), rw, h, T, rho, mu,
                kap, cp, cnt, count_lookups);
     return CHS_OK;
@@ -269,6 +270,20 @@ int props_create(int n, const double* H, const double* T, const double* D, const
     } else {
         if (hipStreamCreate(&p->stream) != hipSuccess) return fail(CHS_E_HIP);
         p->own_stream = true;
+    }
+    {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            p->lds_max = v;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            p->ncu = v;
+    }
+    if (props_lds_bytes(p) > (size_t)p->lds_max) {
+        set_error("chs_props_create: the search tables of " + std::to_string(n) + " knots (" +
+                  std::to_string(props_lds_bytes(p)) + " B) exceed the device's LDS per block (" +
+                  std::to_string(p->lds_max) + " B)");
+        return fail(CHS_E_ARG);
     }
     const size_t bytes = sizeof(double) * (2 * (size_t)n + 20 * (size_t)n) + sizeof(int) * ((size_t)p->nbh + p->nbt);
     unsigned char* blk = nullptr;

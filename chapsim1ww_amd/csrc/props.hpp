@@ -17,6 +17,8 @@ struct chs_props {
     double rhoh_max = 0;       // RHOHmax of NIST_SPLINE_H (the limiter of the energy update)
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int lds_max = 65536;       // LDS bytes a block may use on the table's device
+    int ncu = 256;             // compute units of that device
     // device table: knots, bucket starts, per-interval coefficient records
     double* xh = nullptr;      // H knots [n]
     double* xt = nullptr;      // T knots [n]
@@ -42,6 +44,29 @@ struct Rows {
 // LDS bytes of the staged search tables of one lookup block
 inline size_t props_lds_bytes(const chs_props* p) {
     return sizeof(double) * 2 * (size_t)p->n + sizeof(int) * ((size_t)p->nbh + p->nbt);
+}
+
+// Blocks of a persistent lookup kernel: as many as are resident at once
+// (two per CU while two copies of the staged tables fit in a CU's 160 KB of
+// LDS, one above that), at most `want`.
+inline int props_grid(const chs_props* p, long long want) {
+    const int per_cu = 2 * props_lds_bytes(p) <= (size_t)160 * 1024 ? 2 : 1;
+    const long long g = (long long)p->ncu * per_cu;
+    return (int)(want < g ? want : g);
+}
+
+// Allow a lookup kernel the staged tables' LDS when they exceed the default
+// 64 KB per block (props_create has checked them against the device limit).
+inline int props_lds_attr(const void* kernel, const chs_props* p) {
+    const size_t b = props_lds_bytes(p);
+    if (b > 65536) {
+        hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
+        if (e != hipSuccess) {
+            set_error(std::string("hipFuncSetAttribute (property tables in LDS): ") + hipGetErrorString(e));
+            return CHS_E_HIP;
+        }
+    }
+    return CHS_OK;
 }
 
 // Interval i (0-based) with x[i] <= u < x[i+1], for x[0] < u < x[n-1]: the
@@ -111,7 +136,8 @@ __device__ __forceinline__ bool spline_ht(const PropTab& t, const double* xh, co
 }
 
 // S_TD, S_TM, S_TK, S_TCp at T (NIST_SLEVAL_TD/TM/TK/TCp, src/Flow_State.f90:
-// 1415-1530): one interval search for all four; T == 1 gives D = M = K = 1.
+// 1415-1692): one interval search for all four; |T - 1| < 1e-10 gives
+// D = M = K = CP = 1 (each evaluator's own rule, :1470, :1524, :1578, :1686).
 __device__ __forceinline__ bool spline_t(const PropTab& t, const double* xt, const int* st, double tv, double& D,
                                          double& Mu, double& K, double& CP) {
     if (tv != tv) {
@@ -127,7 +153,7 @@ __device__ __forceinline__ bool spline_t(const PropTab& t, const double* xt, con
     Mu = cubic_nc(cm.x, cm.y, cm.z, cm.w, dt);
     K = cubic_nc(ck.x, ck.y, ck.z, ck.w, dt);
     CP = cubic_nc(cc.x, cc.y, cc.z, cc.w, dt);
-    if (fabs(tv - 1.0) < 1.0e-10) D = Mu = K = 1.0;
+    if (fabs(tv - 1.0) < 1.0e-10) D = Mu = K = CP = 1.0;
     return true;
 }
 

@@ -17,6 +17,8 @@
 // at ghost-aware row j+1), k in [0,N).
 #include <algorithm>
 #include <cmath>
+#include <list>
+#include <mutex>
 #include <vector>
 
 #include "chs_common.hpp"
@@ -1190,30 +1192,56 @@ struct CnStencils {
     bool ok = false;
     double* dev = nullptr;   // wx[2KWx+1] | wz[2KWz+1] | wy[3][M][2KW+1], zero-padded windows
     size_t ox = 0, oz = 0, oy = 0;
+    std::vector<double> host;   // the uploaded values (kept alive for the stream-ordered copy)
+    hipEvent_t used = nullptr;  // recorded after the last substep that read dev
 };
 
-static std::vector<CnStencils>& stencil_cache(const chs_handle* h) {
-    static std::vector<std::pair<const chs_handle*, std::vector<CnStencils>>> all;
-    for (auto& e : all)
-        if (e.first == h) return e.second;
-    all.push_back({h, std::vector<CnStencils>()});
-    return all.back().second;
+// The stencils of the last few FACOE values of one handle.  Entries are
+// evicted oldest first; an evicted entry's device memory is freed once its
+// last-use event has completed (never while a queued substep may read it).
+struct CnCache {
+    std::list<CnStencils> live;   // list: entries keep their addresses
+    std::vector<std::pair<double*, hipEvent_t>> retired;
+};
+
+static void reap_retired(CnCache& c, bool wait) {
+    for (size_t i = 0; i < c.retired.size();) {
+        auto& r = c.retired[i];
+        const bool done = r.second == nullptr || (wait ? hipEventSynchronize(r.second) == hipSuccess
+                                                       : hipEventQuery(r.second) == hipSuccess);
+        if (!done) {
+            ++i;
+            continue;
+        }
+        if (r.first) (void)hipFree(r.first);
+        if (r.second) (void)hipEventDestroy(r.second);
+        c.retired.erase(c.retired.begin() + i);
+    }
 }
 
-void release_stencils(const chs_handle* h) {
-    auto& v = stencil_cache(h);
-    for (auto& c : v)
-        if (c.dev) (void)hipFree(c.dev);
-    v.clear();
+void release_stencils(chs_handle* h) {
+    std::lock_guard<std::mutex> lk(h->cn_mu);
+    CnCache* c = h->cn_cache;
+    if (!c) return;
+    for (auto& e : c->live) c->retired.push_back({e.dev, e.used});
+    c->live.clear();
+    reap_retired(*c, true);
+    delete c;
+    h->cn_cache = nullptr;
 }
 
-static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
-    auto& cache = stencil_cache(h);
-    for (auto& c : cache)
+// Stencils for FACOE (the caller holds h->cn_mu): from the handle's cache, or
+// computed on the host from the host copies of AMVR/ACVR/APVR and uploaded on
+// the handle's stream.
+static int cn_stencils_locked(chs_handle* h, double facoe, CnStencils** out) {
+    if (!h->cn_cache) h->cn_cache = new CnCache();
+    CnCache& cache = *h->cn_cache;
+    for (auto& c : cache.live)
         if (c.facoe == facoe) {
             *out = &c;
             return CHS_OK;
         }
+    reap_retired(cache, false);
     CnStencils st;
     st.facoe = facoe;
     const int M = h->M;
@@ -1226,10 +1254,7 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
     st.Kz = cyclic_stencil(h->N, za, zb, zc, wz);
     // y: full inverse of each component's non-cyclic system (rows j = 0..M-1)
     std::vector<std::vector<ld>> inv(3 * (size_t)M);
-    std::vector<double> amvr(3 * (M + 1)), acvr(3 * (M + 1)), apvr(3 * (M + 1));
-    CHS_HIP(hipMemcpy(amvr.data(), h->amvr, amvr.size() * sizeof(double), hipMemcpyDeviceToHost));
-    CHS_HIP(hipMemcpy(acvr.data(), h->acvr, acvr.size() * sizeof(double), hipMemcpyDeviceToHost));
-    CHS_HIP(hipMemcpy(apvr.data(), h->apvr, apvr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const std::vector<double>&amvr = h->h_amvr, &acvr = h->h_acvr, &apvr = h->h_apvr;
     int Ky = 0;
     for (int c = 0; c < 3; ++c) {
         std::vector<ld> a(M), b(M), cc(M), col(M);
@@ -1262,7 +1287,8 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
         const int W = 2 * st.KW + 1;
         Ky = st.KW;
         const size_t nwx = 2 * (size_t)st.KWx + 1, nwz = 2 * (size_t)st.KWz + 1;
-        std::vector<double> buf(nwx + nwz + 3 * (size_t)M * W, 0.0);
+        std::vector<double>& buf = st.host;
+        buf.assign(nwx + nwz + 3 * (size_t)M * W, 0.0);
         st.ox = 0;
         st.oz = nwx;
         st.oy = nwx + nwz;
@@ -1276,14 +1302,24 @@ static int cn_stencils(chs_handle* h, double facoe, const CnStencils** out) {
                         buf[st.oy + ((size_t)c * M + j) * W + (m + Ky)] = (double)inv[c * M + j][jm];
                 }
         CHS_HIP(hipMalloc((void**)&st.dev, buf.size() * sizeof(double)));
-        CHS_HIP(hipMemcpy(st.dev, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+        hipError_t e = hipEventCreateWithFlags(&st.used, hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(st.dev, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(h->stream);
+            (void)hipFree(st.dev);
+            if (st.used) (void)hipEventDestroy(st.used);
+            set_error(std::string("CN stencil upload: ") + hipGetErrorString(e));
+            return CHS_E_HIP;
+        }
     }
-    if (cache.size() >= 8) {
-        if (cache.front().dev) (void)hipFree(cache.front().dev);
-        cache.erase(cache.begin());
+    if (cache.live.size() >= 8) {
+        CnStencils& old = cache.live.front();
+        cache.retired.push_back({old.dev, old.used});
+        cache.live.pop_front();
     }
-    cache.push_back(st);
-    *out = &cache.back();
+    cache.live.push_back(std::move(st));
+    *out = &cache.live.back();
     return CHS_OK;
 }
 
@@ -1296,8 +1332,9 @@ bool substep_supported(const chs_handle* h) {
 
 // last substep's CN solve path: 1 = truncated-inverse stencils, 0 = segmented Thomas
 int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky) {
-    const CnStencils* cn = nullptr;
-    CHS_TRY(cn_stencils(h, facoe, &cn));
+    std::lock_guard<std::mutex> lk(h->cn_mu);
+    CnStencils* cn = nullptr;
+    CHS_TRY(cn_stencils_locked(h, facoe, &cn));
     *kx = cn->Kx;
     *kz = cn->Kz;
     *ky = cn->Ky;
@@ -1315,8 +1352,11 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     const size_t ncell = (size_t)h->L * h->M * h->N;
     const double facoe = 0.5 * rk.talp * dt * h->cvisc;  // FACOE_TG (src/MOMFA_tg.f90:29)
     hipEvent_t ev;
-    const CnStencils* cn = nullptr;
-    CHS_TRY(cn_stencils(h, facoe, &cn));
+    // the handle's stencil cache stays locked until the substep's kernels
+    // that read the stencils are queued and their last-use event recorded
+    std::unique_lock<std::mutex> lk(h->cn_mu);
+    CnStencils* cn = nullptr;
+    CHS_TRY(cn_stencils_locked(h, facoe, &cn));
     // stencil path needs: N in whole z tiles, at most 4 y chunks of YJC rows
     // (blocks of 64*chunks threads), and chunks longer than the window
     const bool use_st = cn->ok && h->rhs2 != nullptr && h->cn_mode != 1 && h->N % ZKC == 0 &&
@@ -1444,6 +1484,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
             set_error("substep: unsupported NCL2");
             return CHS_E_ARG;
         }
+        lk.unlock();
         const int nlines = h->L * h->N;
         const dim3 gy((nlines + LPB - 1) / LPB), by(LPB * P);
         const size_t smy = seg_smem_bytes<double>(LPB, P);
@@ -1455,6 +1496,10 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
             default: CHS_LAUNCH(k_y_solve<8>, gy, by, smy, s, g, facoe, h->rhs, h->q, LPB, P); break;
         }
         h->timer.end(ev, "y_solve", s);
+    }
+    if (lk.owns_lock()) {
+        if (cn->used) CHS_HIP(hipEventRecord(cn->used, s));
+        lk.unlock();
     }
     // 4. DIVG
     h->timer.begin("divg", s, &ev);

@@ -63,6 +63,7 @@ _SIGS = {
     "chs_poisson_fft99": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "chs_set_refinement": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
     "chs_poisson_stats": ([C.c_void_p, C.POINTER(C.c_int), _dp], C.c_int),
+    "chs_set_cvisc": ([C.c_void_p, C.c_double], C.c_int),
     "chs_set_cn_solver": ([C.c_void_p, C.c_int], C.c_int),
     "chs_cn_info": ([C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int),
                      C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),

@@ -83,6 +83,11 @@ class ChannelSolver:
     def solverrk3_mom_tg(self, ns: int, dt: float):
         nat.check(self.lib.chs_rk3_substep(self.h, int(ns), float(dt)))
 
+    def set_cvisc(self, cvisc: float):
+        """CVISC = 1/REN from here on: the REINI -> REN switch at TLGRE
+        (src/SOLVE.f90:97-100, BCAST_COMM_STEP :167)."""
+        nat.check(self.lib.chs_set_cvisc(self.h, float(cvisc)))
+
     def fft99_pois3d(self, f_dev, dph_dev):
         """Poisson solve on device tensors: f (L*M*N, RHSLLPHI layout) -> dph (L*(M+2)*N, DPH layout)."""
         g = self.grid

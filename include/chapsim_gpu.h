@@ -77,6 +77,13 @@ int chs_device_arrays(chs_handle *h, double **q, double **pr, double **convh0, d
  * wall BCs, divergence, Poisson solve, projection and pressure update. */
 int chs_rk3_substep(chs_handle *h, int ns, double dt);
 
+/* Change CVISC = 1/REN of the handle (used by the tg substep's viscous terms
+ * and CN solves and by the io momentum substep).  The reference recomputes
+ * CVISC every step in BCAST_COMM_STEP (src/SOLVE.f90:167) after switching REN
+ * from REINI to REN once phyTIME reaches TLGRE (src/SOLVE.f90:97-100); a
+ * driver calls this at that switch.  cvisc > 0 and finite. */
+int chs_set_cvisc(chs_handle *h, double cvisc);
+
 /* Pressure Poisson solve FFT99_POIS3D_periodicxz (src/possion3d_FFT99.f90:254-488):
  *   f_dev:   RHSLLPHI(NCL1, NCL2, NCL3)          (device)
  *   dph_dev: DPH(NCL1, 0:NCL2+1, NCL3)           (device); interior written,

@@ -1125,7 +1125,7 @@ long orc_therm_prop_update(int n, const double *tab, long ncell, const double *h
         int e1 = orc_sleval(n, TT, MM, tm, tm + n, tm + 2 * n, t, 2, &mv);
         int e2 = orc_sleval(n, TT, KK, tk, tk + n, tk + 2 * n, t, 2, &kv);
         int e3 = orc_sleval(n, TT, DD, td, td + n, td + 2 * n, t, 2, &dv);
-        int e4 = orc_sleval(n, TT, CP, tc, tc + n, tc + 2 * n, t, 0, &cv);
+        int e4 = orc_sleval(n, TT, CP, tc, tc + n, tc + 2 * n, t, 2, &cv);   /* NIST_SLEVAL_TCp, src/Flow_State.f90:1686 */
         if (e1 | e2 | e3 | e4) { bad++; continue; }
         T[q] = t; mu[q] = mv; kap[q] = kv; rho[q] = dv;
         if (cp) cp[q] = cv;
@@ -2088,6 +2088,15 @@ static void io_momfa(const orc_grid *g, orc_io *s, int NS, int IDR, double DT, d
     for (int KC = 1; KC <= WN; ++KC)
         for (int JC = N2I; JC <= WM; ++JC)
             for (int IC = 1; IC <= WL; ++IC) WG(s->g, IC, JC, KC, IDR) = WF3(r, IC, JC, KC) + WG(s->g, IC, JC, KC, IDR);
+}
+
+/* MOMFA_io(NS, IDR) alone (src/MOMFA_io.f90:17-48): the factorised x, z, y
+ * solves of the right-hand side r (L, M, N), overwritten, then G(IDR) += r on
+ * rows N2I..M.  Exported so that the tests can check the line solves against
+ * dense operators. */
+void orc_io_momfa(const orc_grid *g, orc_io *s, int NS, int IDR, double DT, double *r)
+{
+    io_momfa(g, s, NS, IDR, DT, r);
 }
 
 void orc_io_substep(const orc_grid *g, orc_io *s, const orc_io_params *p, int NS, double DT)

@@ -52,9 +52,12 @@ def test_fft99_known_modes():
         assert np.allclose(ab, expect, atol=1e-15)
 
 
-@pytest.mark.skipif(O.ref_fft99() is None, reason="oracle/_ref not built (no reference tree)")
 @pytest.mark.parametrize("n", [8, 12, 16, 30, 64, 256])
 def test_fft99_matches_reference_package(n):
+    # loaded inside the test, so that collection never maps the compiled
+    # reference package (it is built only where the reference tree exists)
+    if O.ref_fft99() is None:
+        pytest.skip("oracle/_ref not built (no reference tree)")
     lot = 3
     x = RNG.standard_normal((n, lot))
     data = np.zeros((n + 2, lot))
