@@ -96,6 +96,8 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int] + \
                 ([C.c_double] if name == "orc_io_substep" else [])
             getattr(L, name).restype = None
+        L.orc_io_momfa.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]
+        L.orc_io_momfa.restype = None
         L.orc_io_terms.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_io_terms.restype = None
         L.orc_divgck_io.argtypes = [C.c_void_p, C.c_void_p, _dp]
@@ -365,6 +367,12 @@ class OracleIO:
         """Convection, DIVG_U and explicit viscous terms -> work hx, hy, hz."""
         lib().orc_io_terms(C.byref(self.og.g), C.byref(self.s))
         return self.work["hx"], self.work["hy"], self.work["hz"]
+
+    def momfa(self, ns: int, idr: int, dt: float, r):
+        """MOMFA_io(NS, IDR) on the right-hand side r (L, M, N, Fortran order,
+        overwritten by the increment); G(IDR) += increment."""
+        assert r.dtype == np.float64 and r.flags.f_contiguous
+        lib().orc_io_momfa(C.byref(self.og.g), C.byref(self.s), ns, idr, dt, _ptr(r))
 
     def substep(self, ns: int, dt: float):
         lib().orc_io_substep(C.byref(self.og.g), C.byref(self.s), C.byref(self.p), ns, dt)
