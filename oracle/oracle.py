@@ -4,9 +4,12 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
 the cpu_baseline leg of bench.py, always as the checker or the timed CPU
 baseline, never as part of the product path.
 
-Also loads oracle/_ref/libfft99_ref.so when it exists: the reference's own
-FFT99 package (libs/fft99/*.f90) compiled with amdflang by oracle/Makefile,
-used to pin the oracle's Fourier conventions.
+Also loads, when they exist, the reference's own FFT99 package
+(libs/fft99/*.f90, oracle/_ref/libfft99_ref.so) and its four tridiagonal line
+solvers (src/TDMAIJ{I,J}_{CYC,nonCYC}.f90, oracle/_ref/libtdma_ref.so), both
+compiled from the reference sources with amdflang by oracle/Makefile, used to
+pin the oracle's Fourier conventions and line solves and to write the golden
+fixtures under tests/golden/.
 """
 from __future__ import annotations
 
@@ -19,6 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libfft99_ref.so")
+REF_TDMA_PATH = os.path.join(HERE, "_ref", "libtdma_ref.so")
 
 _dp = C.POINTER(C.c_double)
 
@@ -52,6 +56,7 @@ class _State(C.Structure):
 
 _lib = None
 _ref = None
+_ref_tdma = None
 
 
 def lib():
@@ -112,6 +117,14 @@ def ref_fft99():
     if _ref is None and os.path.exists(REF_PATH):
         _ref = C.CDLL(REF_PATH)
     return _ref
+
+
+def ref_tdma():
+    """The reference's four line solvers, or None when oracle/_ref was not built."""
+    global _ref_tdma
+    if _ref_tdma is None and os.path.exists(REF_TDMA_PATH):
+        _ref_tdma = C.CDLL(REF_TDMA_PATH)
+    return _ref_tdma
 
 
 # --------------------------------------------------------------------------
@@ -439,3 +452,33 @@ def ref_cfft99_transform(z, isign):
     R.cfft99_(a.ctypes.data_as(_dp), _ptr(work), _ptr(trigs), ifax.ctypes.data_as(C.POINTER(C.c_int)),
               ip(1), ip(n), ip(n), ip(lot), ip(isign))
     return a
+
+
+# ---- reference line solvers (Fortran, by-reference arguments) --------------
+def ref_tdma_solve(a, b, c, r, bc, march, cyclic, lo=(1, 1)):
+    """One call of the reference solver for arrays of shape (ISZ, JSZ) with
+    Fortran lower bounds lo = (IS, JS):
+      march "IJJ": TDMAIJJ_nonCYC(A,B,C,R,BCJ,JS,JSZ,IS,ISZ) (BCJ(ISZ,2)) or
+                   TDMAIJJ_CYC(A,B,C,R,JS,JSZ,IS,ISZ)     (src/TDMAIJJ_*.f90:67)
+      march "IJI": TDMAIJI_nonCYC(A,B,C,R,BCI,IS,ISZ,JS,JSZ) (BCI(JSZ,2)) or
+                   TDMAIJI_CYC(A,B,C,R,IS,ISZ,JS,JSZ)     (src/TDMAIJI_*.f90:67)
+    Returns the solution (R after the call)."""
+    R = ref_tdma()
+    f = lambda v: np.array(v, dtype=np.float64, order="F", copy=True)
+    A, B, Cc, X = f(a), f(b), f(c), f(r)
+    isz, jsz = A.shape
+    ip = lambda v: C.byref(C.c_int(v))
+    IS, JS = lo
+    if march == "IJJ":
+        if cyclic:
+            R.tdmaijj_cyc_(_ptr(A), _ptr(B), _ptr(Cc), _ptr(X), ip(JS), ip(jsz), ip(IS), ip(isz))
+        else:
+            BC = f(bc if bc is not None else np.zeros((isz, 2)))
+            R.tdmaijj_noncyc_(_ptr(A), _ptr(B), _ptr(Cc), _ptr(X), _ptr(BC), ip(JS), ip(jsz), ip(IS), ip(isz))
+    else:
+        if cyclic:
+            R.tdmaiji_cyc_(_ptr(A), _ptr(B), _ptr(Cc), _ptr(X), ip(IS), ip(isz), ip(JS), ip(jsz))
+        else:
+            BC = f(bc if bc is not None else np.zeros((jsz, 2)))
+            R.tdmaiji_noncyc_(_ptr(A), _ptr(B), _ptr(Cc), _ptr(X), _ptr(BC), ip(IS), ip(isz), ip(JS), ip(jsz))
+    return X
