@@ -157,8 +157,10 @@ bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, 
     return false;
 }
 
+// ws: the sequential fallback's workspace of 3 n nlines doubles, or nullptr to
+// take it from the stream-ordered pool for this launch
 int tdma_launch(const double* a, const double* b, const double* c, double* r, const double* bc, int n, int nlines,
-                long long es, long long ls, int cyclic, hipStream_t s) {
+                long long es, long long ls, int cyclic, hipStream_t s, double* ws_given = nullptr) {
     int MS = 0, P = 0, LPB = 0;
     if (tdma_seg_config(n, cyclic != 0, &MS, &P, &LPB, 1024, false)) {
         const dim3 grid((nlines + LPB - 1) / LPB), block(LPB * P);
@@ -171,11 +173,41 @@ int tdma_launch(const double* a, const double* b, const double* c, double* r, co
         }
         return CHS_OK;
     }
+    const int bs = 256;
+    if (ws_given) {
+        CHS_LAUNCH(k_tdma_seq, dim3((nlines + bs - 1) / bs), dim3(bs), 0, s, a, b, c, r, bc, n, nlines, es, ls, cyclic,
+                   ws_given);
+        return CHS_OK;
+    }
     double* ws = nullptr;
     CHS_HIP(hipMallocAsync((void**)&ws, sizeof(double) * 3 * (size_t)n * nlines, s));
-    const int bs = 256;
     CHS_LAUNCH(k_tdma_seq, dim3((nlines + bs - 1) / bs), dim3(bs), 0, s, a, b, c, r, bc, n, nlines, es, ls, cyclic, ws);
     CHS_HIP(hipFreeAsync(ws, s));
+    return CHS_OK;
+}
+
+// Workspace of the allocation-free entry point: one copy per GPU, held until
+// the process ends.  Host code never passes `scratch` itself to a kernel (its
+// host-side name is not a device address); it uses the address that
+// hipGetSymbolAddress returns on each device, kept in scratch_dev[].
+constexpr size_t TDMA_SCRATCH_DOUBLES = size_t(4) << 20;   // 32 MiB
+static __device__ __align__(256) double scratch[TDMA_SCRATCH_DOUBLES];
+
+static std::mutex scratch_mu;
+static double* scratch_dev[64] = {};
+
+// The current device's scratch address, looked up once per device.
+static int scratch_address(double** out) {
+    int dev = 0;
+    CHS_HIP(hipGetDevice(&dev));
+    CHS_CHECK(dev >= 0 && dev < 64, CHS_E_ARG, "chs_tdma_batched_noalloc: device ordinal above 63");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        CHS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = scratch_dev[dev];
     return CHS_OK;
 }
 
@@ -191,6 +223,30 @@ extern "C" int chs_tdma_batched(const double* a_dev, const double* b_dev, const 
     if (nlines == 0) return CHS_OK;
     return chs::tdma_launch(a_dev, b_dev, c_dev, r_dev, cyclic ? nullptr : bc_dev, n, nlines, elem_stride,
                             line_stride, cyclic, (hipStream_t)stream);
+}
+
+// chs_tdma_batched without any allocation: the segmented kernel needs only
+// LDS; the sequential fallback (lines the segmented kernel cannot split) takes
+// its 3 n nlines-double workspace from this GPU's static scratch array.
+extern "C" int chs_tdma_batched_noalloc(const double* a_dev, const double* b_dev, const double* c_dev, double* r_dev,
+                                        const double* bc_dev, int n, int nlines, long long elem_stride,
+                                        long long line_stride, int cyclic, void* stream) {
+    CHS_CHECK(a_dev && b_dev && c_dev && r_dev, CHS_E_ARG, "chs_tdma_batched_noalloc: null array");
+    CHS_CHECK(n >= 3, CHS_E_ARG, "chs_tdma_batched_noalloc: n must be >= 3 (the reference reads outside its arrays below that)");
+    CHS_CHECK(nlines >= 0, CHS_E_ARG, "chs_tdma_batched_noalloc: nlines < 0");
+    CHS_CHECK(elem_stride != 0, CHS_E_ARG, "chs_tdma_batched_noalloc: zero element stride");
+    if (nlines == 0) return CHS_OK;
+    int MS = 0, P = 0, LPB = 0;
+    double* ws = nullptr;
+    if (!chs::tdma_seg_config(n, cyclic != 0, &MS, &P, &LPB, 1024, false)) {
+        CHS_CHECK(3 * (size_t)n * (size_t)nlines <= chs::TDMA_SCRATCH_DOUBLES, CHS_E_ARG,
+                  "chs_tdma_batched_noalloc: these lines need a workspace of 3*n*nlines doubles, above the "
+                  "4194304 of the static scratch array (use chs_tdma_batched)");
+        const int st = chs::scratch_address(&ws);
+        if (st != CHS_OK) return st;
+    }
+    return chs::tdma_launch(a_dev, b_dev, c_dev, r_dev, cyclic ? nullptr : bc_dev, n, nlines, elem_stride,
+                            line_stride, cyclic, (hipStream_t)stream, ws);
 }
 
 // Host-array form for a drop-in of the reference subroutines called on host

@@ -73,6 +73,8 @@ _SIGS = {
     "chs_mean_xz": ([C.c_void_p, _dp], C.c_int),
     "chs_tdma_batched": ([C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.c_void_p],
                          C.c_int),
+    "chs_tdma_batched_noalloc": ([C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int,
+                                                     C.c_void_p], C.c_int),
     "chs_tdma_batched_host": ([_dp] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_int], C.c_int),
     "chs_props_create": ([C.c_int] + [_dp] * 6 + [C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "chs_props_destroy": ([C.c_void_p], C.c_int),

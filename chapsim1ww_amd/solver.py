@@ -161,7 +161,10 @@ class ChannelSolver:
 
 # ---- the reference's four line solvers on device tensors -------------------
 def tdma_batched(a, b, c, r, bc, n: int, nlines: int, elem_stride: int, line_stride: int, cyclic: bool,
-                 stream=None):
+                 stream=None, noalloc: bool = False):
+    """Batched TDMAIJ?_(non)CYC line solves on device tensors (r is
+    overwritten).  noalloc: use chs_tdma_batched_noalloc, which allocates
+    nothing (workspace from the library's static per-GPU scratch array)."""
     lib = nat.load()
     s = None if stream is None else C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
     need = (nlines - 1) * line_stride + (n - 1) * elem_stride + 1 if nlines > 0 else 0
@@ -170,9 +173,9 @@ def tdma_batched(a, b, c, r, bc, n: int, nlines: int, elem_stride: int, line_str
             raise ValueError("tdma_batched: array smaller than the strides describe")
     if bc is not None and bc.numel() < 2 * nlines:
         raise ValueError("tdma_batched: bc needs 2*nlines values")
-    nat.check(lib.chs_tdma_batched(nat.dptr(a), nat.dptr(b), nat.dptr(c), nat.dptr(r),
-                                   None if bc is None else nat.dptr(bc), n, nlines, elem_stride, line_stride,
-                                   int(bool(cyclic)), s))
+    fn = lib.chs_tdma_batched_noalloc if noalloc else lib.chs_tdma_batched
+    nat.check(fn(nat.dptr(a), nat.dptr(b), nat.dptr(c), nat.dptr(r), None if bc is None else nat.dptr(bc), n, nlines,
+                 elem_stride, line_stride, int(bool(cyclic)), s))
 
 
 def tdmaijj_noncyc(A, B, Cc, R, BCJ, JS, JSZ, IS, ISZ, stream=None):

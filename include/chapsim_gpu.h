@@ -143,6 +143,20 @@ int chs_tdma_batched(const double *a_dev, const double *b_dev, const double *c_d
                      const double *bc_dev, int n, int nlines, long long elem_stride,
                      long long line_stride, int cyclic, void *stream);
 
+/* chs_tdma_batched without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Lines the segmented
+ * kernel splits need no workspace; the others (e.g. cyclic lines of odd
+ * length, or more than 512 rows) take 3*n*nlines doubles from a static array
+ * of 4194304 doubles (32 MiB) that each GPU holds until the process ends, and
+ * larger such calls return CHS_E_ARG.  Calls that overlap on one GPU share that
+ * array and race; its contents are not cleared between calls (nothing in it
+ * is read before it is written).  Same arguments and results as
+ * chs_tdma_batched. */
+int chs_tdma_batched_noalloc(const double *a_dev, const double *b_dev, const double *c_dev, double *r_dev,
+                             const double *bc_dev, int n, int nlines, long long elem_stride,
+                             long long line_stride, int cyclic, void *stream);
+
 /* The same on host arrays (what the Fortran subroutines receive): copies the
  * arrays to the device, solves, copies r back; synchronous.  For a per-call
  * drop-in of TDMAIJ?_(non)CYC on host data; PCIe-bound. */
