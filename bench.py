@@ -344,7 +344,30 @@ def cpu_baseline(max_seconds=30.0):
         el2 = time.perf_counter() - t1
         if el2 > max_seconds / 4 or m >= 5:
             break
-    return dict(substeps_per_s=sub_rate, nsub=n, poisson_per_s=m / el2, npois=m, cores=O.num_threads())
+    nthr = O.num_threads()
+    # the same substep on one thread (one substep: about 10-25 s of CPU work)
+    O.set_num_threads(1)
+    t2 = time.perf_counter()
+    st.substep(1 + n % 3, dt)
+    one = 1.0 / (time.perf_counter() - t2)
+    O.set_num_threads(nthr)
+    return dict(substeps_per_s=sub_rate, nsub=n, poisson_per_s=m / el2, npois=m, cores=nthr,
+                substeps_per_s_1thread=one, host=host_cpu())
+
+
+def host_cpu():
+    """CPU model, logical CPUs of the host and CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(model=model, logical_cpus=os.cpu_count(), usable_cpus=len(os.sched_getaffinity(0)),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
 
 
 MEAN_COLS = {"u": 0, "v": 1, "w": 2, "p": 3, "uu": 4, "uv": 5, "vv": 7, "ww": 9}
@@ -463,7 +486,9 @@ def main():
         _note("cpu baseline done")
         cpu = dict(value=round(cb["substeps_per_s"], 4), unit="substeps/s", cores=cb["cores"], kind="port",
                    sample=f"{cb['nsub']} FP64 RK3 substeps of the C oracle at 256x192x256 "
-                          f"(+{cb['npois']} Poisson solves: {cb['poisson_per_s']:.3f}/s), OpenMP",
+                          f"(+{cb['npois']} Poisson solves: {cb['poisson_per_s']:.3f}/s), OpenMP, "
+                          f"gcc -O2 -fopenmp; one more substep on 1 thread",
+                   value_1thread=round(cb["substeps_per_s_1thread"], 4), host=cb["host"],
                    poisson_per_s=round(cb["poisson_per_s"], 4), case_64cube=small_case())
     sub_s = 1e3 / f64["ms_per_substep"]
     out = {

@@ -16,6 +16,7 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 int rk3_substep(chs_handle* h, int ns, double dt);
 int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out);
 int mean_xz(chs_handle* h, double* out_host);
+int mixed_check(chs_handle* h, bool wait);
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
 bool substep_supported(const chs_handle* h);
 void release_stencils(chs_handle* h);
@@ -102,7 +103,7 @@ static void free_all(chs_handle* h) {
     void* ptrs[] = {h->dyfi, h->dyci, h->wff, h->wfb, h->amph, h->acph, h->apph, h->amvr, h->acvr, h->apvr,
                     h->ak1, h->ak3, h->twx, h->twz, h->twx32, h->twz32, h->q, h->pr, h->convh0, h->dph, h->rhs,
                     h->rhsllphi, h->spec, h->spec32, h->rhs_j1, h->partials, h->scalars, h->resid, h->f32a, h->f32b,
-                    h->rhs2};
+                    h->rhs2, h->stats};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -210,7 +211,22 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     h->npart = 65536;
     chk(dalloc(&h->partials, (size_t)h->npart * 2));
     chk(dalloc(&h->scalars, 16));
+    chk(dalloc(&h->stats, 100 * (size_t)M));
+    h->chk_pending = false;
+    h->chk_tol = 0.0;
+    if (st == CHS_OK && hipHostMalloc((void**)&h->chk_host, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        set_error("chs_create: pinned host allocation failed");
+        h->chk_host = nullptr;
+        st = CHS_E_NOMEM;
+    }
+    if (st == CHS_OK && hipEventCreateWithFlags(&h->chk_ev, hipEventDisableTiming) != hipSuccess) {
+        set_error("chs_create: event creation failed");
+        h->chk_ev = nullptr;
+        st = CHS_E_HIP;
+    }
     if (st != CHS_OK) {
+        if (h->chk_host) (void)hipHostFree(h->chk_host);
+        if (h->chk_ev) (void)hipEventDestroy(h->chk_ev);
         free_all(h);
         if (h->own_stream) (void)hipStreamDestroy(h->stream);
         delete h;
@@ -227,6 +243,8 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     CHS_HIP(hipMemsetAsync(h->rhs, 0, ncell * 3 * sizeof(double), s));
     CHS_HIP(hipMemsetAsync(h->rhsllphi, 0, ncell * sizeof(double), s));
     CHS_HIP(hipMemsetAsync(h->scalars, 0, 16 * sizeof(double), s));
+    CHS_HIP(hipMemsetAsync(h->stats, 0, 100 * (size_t)M * sizeof(double), s));
+    h->nstatis[0] = h->nstatis[1] = 0;
     CHS_HIP(hipStreamSynchronize(s));
     *out = h;
     return CHS_OK;
@@ -238,6 +256,8 @@ int chs_destroy(chs_handle* h) {
     h->timer.collect();
     release_stencils(h);
     free_all(h);
+    if (h->chk_host) (void)hipHostFree(h->chk_host);
+    if (h->chk_ev) (void)hipEventDestroy(h->chk_ev);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CHS_OK;
@@ -260,7 +280,7 @@ int chs_download_state(chs_handle* h, double* q, double* pr, double* convh0) {
     if (pr) CHS_HIP(hipMemcpyAsync(pr, h->pr, ngh * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (convh0) CHS_HIP(hipMemcpyAsync(convh0, h->convh0, 3 * ncell * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     CHS_HIP(hipStreamSynchronize(h->stream));
-    return CHS_OK;
+    return mixed_check(h, true);
 }
 
 int chs_device_arrays(chs_handle* h, double** q, double** pr, double** convh0, double** dph, double** rhsllphi) {
@@ -326,9 +346,10 @@ int chs_set_refinement(chs_handle* h, double rel_tol, int max_iter) {
 
 int chs_poisson_stats(chs_handle* h, int* iters, double* rel) {
     CHS_CHECK(h, CHS_E_ARG, "chs_poisson_stats: null handle");
+    const int st = mixed_check(h, true);
     if (iters) *iters = h->last_refine_iters;
     if (rel) *rel = h->last_refine_resid;
-    return CHS_OK;
+    return st;
 }
 
 int chs_divgck(chs_handle* h, double* maxdiv) {

@@ -213,6 +213,18 @@ struct chs_handle {
     std::vector<double> h_amvr, h_acvr, h_apvr;
     std::mutex cn_mu;
     chs::CnCache* cn_cache;
+    // xz-plane means of the last chs_mean_xz (13 M, tg) and chs_mean_xz_io
+    // (37 M) call and their time means (PP_MEAN_T_*), in one allocation:
+    // stats + [0, 13M) tg xz, [13M, 50M) io xz, [50M, 63M) tg time, [63M, 100M) io time
+    double *stats;
+    int nstatis[2];      // NSTATIS_TG, NSTATIS_IO
+    // the mixed-precision substep's convergence check, fused into its
+    // projection and read back without a wait: max|r|, max|f| land in pinned
+    // chk_host when chk_ev completes; chk_tol is the bound on max|r|
+    bool chk_pending;
+    double chk_tol;
+    double* chk_host;
+    hipEvent_t chk_ev;
     // mixed-precision Poisson refinement: tolerance, cap, statistics of the last call
     double refine_tol;
     int refine_max;

@@ -27,8 +27,9 @@ constexpr int ZTX = 8;     // kx columns per block in the z transform
 
 // ---- x forward: f rows (real, FP64 input) -> S rows (half spectrum) -------
 // mode 0: input f; mode 1: input = f - L_h(phi) (FP64 residual, for refinement)
-template <typename Rl>
-__global__ void __launch_bounds__(256) k_fft_x_fwd(const double* __restrict__ f, cpx_t<Rl>* __restrict__ S, int L,
+// TI: the input type (double: f; float: the refinement's FP32 residual)
+template <typename Rl, typename TI = double>
+__global__ void __launch_bounds__(256) k_fft_x_fwd(const TI* __restrict__ f, cpx_t<Rl>* __restrict__ S, int L,
                                                    int nrows, int LHP, FftPlan plan, const cpx_t<Rl>* __restrict__ tw) {
     using C = cpx_t<Rl>;
     extern __shared__ unsigned char smem_raw[];
@@ -158,8 +159,8 @@ __device__ __forceinline__ void fft256_regs(C (&v)[16], int q, C* buf, const C* 
 }
 
 // x forward, L = 256: 16 rows (8 complex transforms) per block of 128 threads
-template <typename Rl>
-__global__ void __launch_bounds__(128) k_fft256_x_fwd(const double* __restrict__ f, cpx_t<Rl>* __restrict__ S,
+template <typename Rl, typename TI = double>
+__global__ void __launch_bounds__(128) k_fft256_x_fwd(const TI* __restrict__ f, cpx_t<Rl>* __restrict__ S,
                                                       int nrows, int LHP, const cpx_t<Rl>* __restrict__ tw) {
     using C = cpx_t<Rl>;
     extern __shared__ unsigned char smem_raw[];
@@ -168,12 +169,12 @@ __global__ void __launch_bounds__(128) k_fft256_x_fwd(const double* __restrict__
     const int ra = blockIdx.x * 16 + 2 * g, rb = ra + 1;
     // all 32 loads are issued before any is converted: for FP32 the compiler
     // would otherwise convert as it loads, a few loads in flight at a time
-    double la[16], lb[16];
+    TI la[16], lb[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1) {
         const int i = 16 * n1 + q;
-        la[n1] = ra < nrows ? f[(size_t)ra * 256 + i] : 0.0;
-        lb[n1] = rb < nrows ? f[(size_t)rb * 256 + i] : 0.0;
+        la[n1] = ra < nrows ? f[(size_t)ra * 256 + i] : TI(0);
+        lb[n1] = rb < nrows ? f[(size_t)rb * 256 + i] : TI(0);
     }
     asm volatile("" ::: "memory");
     C v[16];
@@ -402,7 +403,7 @@ __device__ __forceinline__ double pmax(double a, double b) { return a != a ? a :
 
 template <bool STORE>
 __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restrict__ phi, const double* __restrict__ f,
-                                                          double* __restrict__ r, int L, int M, int N, double dxqi,
+                                                          float* __restrict__ r, int L, int M, int N, double dxqi,
                                                           double dzqi, const double* __restrict__ amph,
                                                           const double* __restrict__ acph,
                                                           const double* __restrict__ apph, double* __restrict__ part) {
@@ -454,7 +455,7 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
             const double lz = (pt[u] - 2.0 * pc + pb[u]) * dzqi;
             const double rv = fv[u] - (lx + ly + lz);
             if (okc && j < M) {
-                if (STORE) r[i + L * (j + M * k)] = rv;
+                if (STORE) r[i + L * (j + M * k)] = float(rv);   // the correction is an FP32 solve
                 mr = pmax(mr, fabs(rv));
                 mf = pmax(mf, fabs(fv[u]));
             }
@@ -510,8 +511,8 @@ __global__ void k_zero(double* __restrict__ p, size_t n) {
 // ---------------------------------------------------------------------------
 bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small = false);
 
-template <typename Rl>
-static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, cpx_t<Rl>* S, const cpx_t<Rl>* twx,
+template <typename Rl, typename TI = double>
+static int poisson_pass(chs_handle* h, const TI* f, double* dph, int ghost, cpx_t<Rl>* S, const cpx_t<Rl>* twx,
                         const cpx_t<Rl>* twz) {
     // FP32 spectra use rows of a multiple of 16 values (128-byte aligned rows)
     constexpr bool f32 = sizeof(Rl) == 4;
@@ -533,9 +534,9 @@ static int poisson_pass(chs_handle* h, const double* f, double* dph, int ghost, 
     const dim3 gz16((LH + ZCB - 1) / ZCB, M);
     h->timer.begin("fft_x_fwd", s, &ev);
     if (x256)
-        CHS_LAUNCH(k_fft256_x_fwd<Rl>, gx16, dim3(128), sm16x, s, f, S, nrows, LHP, twx);
+        CHS_LAUNCH((k_fft256_x_fwd<Rl, TI>), gx16, dim3(128), sm16x, s, f, S, nrows, LHP, twx);
     else
-        CHS_LAUNCH(k_fft_x_fwd<Rl>, gx, dim3(256), smx, s, f, S, L, nrows, LHP, px, twx);
+        CHS_LAUNCH((k_fft_x_fwd<Rl, TI>), gx, dim3(256), smx, s, f, S, L, nrows, LHP, px, twx);
     h->timer.end(ev, "fft_x_fwd", s);
     const dim3 gz((LH + ZTX - 1) / ZTX, M);
     const size_t smz = sizeof(cpx_t<Rl>) * (size_t)2 * ZTX * N;
@@ -588,12 +589,32 @@ __global__ void k_dph_ghost(double* __restrict__ dph, int L, int M, int N, int m
     }
 }
 
+// Mixed form inside the substep: the FP32 pass and one refinement, with no
+// check here and no wait; the substep's projection checks the residual of the
+// result (k_update<true>) and the host reads the verdict later.
+int poisson_solve_unchecked(chs_handle* h, const double* f, double* dph, int ghost) {
+    const hipStream_t s = h->stream;
+    float* rf = reinterpret_cast<float*>(h->resid);   // the FP32 residual (first half of the buffer)
+    CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
+    hipEvent_t ev;
+    h->timer.begin("poisson_residual", s, &ev);
+    CHS_LAUNCH(k_poisson_residual<true>, dim3(2048), dim3(256), 0, s, dph, f, rf, h->L, h->M, h->N, h->dxqi,
+               h->dzqi, h->amph, h->acph, h->apph, h->partials);
+    h->timer.end(ev, "poisson_residual", s);
+    CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
+    if (ghost != 2) CHS_LAUNCH(k_dph_ghost, dim3(256), dim3(256), 0, s, dph, h->L, h->M, h->N, ghost);
+    h->last_refine_iters = 1;
+    h->last_refine_resid = -1.0;   // known once the substep's check has been read
+    return CHS_OK;
+}
+
 // FP64 form: one pass.  Mixed form: FP32 pass on f, then FP32 passes on the
 // FP64 residual (at least one) until max|f - L phi| <= abs_tol or
 // max|f - L phi| <= rel_tol * max|f|.
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol) {
     if (h->precision == CHS_FP64) return poisson_pass<double>(h, f, dph, ghost, h->spec, h->twx, h->twz);
     const hipStream_t s = h->stream;
+    float* rf = reinterpret_cast<float*>(h->resid);   // the FP32 residual (first half of the buffer)
     CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
     // 2048 blocks of row tiles (a multiple of 8: one contiguous tile range per XCD)
     const int nb = 2048;
@@ -605,10 +626,10 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
     auto residual = [&](bool store) -> int {
         h->timer.begin("poisson_residual", s, &ev);
         if (store)
-            CHS_LAUNCH(k_poisson_residual<true>, dim3(nb), dim3(256), 0, s, dph, f, h->resid, h->L, h->M, h->N,
+            CHS_LAUNCH(k_poisson_residual<true>, dim3(nb), dim3(256), 0, s, dph, f, rf, h->L, h->M, h->N,
                        h->dxqi, h->dzqi, h->amph, h->acph, h->apph, h->partials);
         else
-            CHS_LAUNCH(k_poisson_residual<false>, dim3(nb), dim3(256), 0, s, dph, f, h->resid, h->L, h->M, h->N,
+            CHS_LAUNCH(k_poisson_residual<false>, dim3(nb), dim3(256), 0, s, dph, f, rf, h->L, h->M, h->N,
                        h->dxqi, h->dzqi, h->amph, h->acph, h->apph, h->partials);
         h->timer.end(ev, "poisson_residual", s);
         stored = store;
@@ -622,7 +643,7 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
     int it0 = 0;
     if (h->refine_max > 0) {
         CHS_TRY(residual(true));
-        CHS_TRY(poisson_pass<float>(h, h->resid, dph, 2, h->spec32, h->twx32, h->twz32));
+        CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
         h->last_refine_iters = 1;
         it0 = 1;
     }
@@ -644,7 +665,7 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
             return CHS_E_NOTCONV;
         }
         if (!stored) CHS_TRY(residual(true));
-        CHS_TRY(poisson_pass<float>(h, h->resid, dph, 2, h->spec32, h->twx32, h->twz32));
+        CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
         h->last_refine_iters = it + 1;
     }
     if (h->last_refine_iters > 0 && ghost != 2)

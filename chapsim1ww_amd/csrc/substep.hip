@@ -63,100 +63,110 @@ struct Nb {
 // One component of Q (or PR) around a cell: v(dj, o) is the value at row
 // J + dj (dj in -1..1, a constant at every call site) and in-row offset o.
 // GCol reads the ghost-aware global array (rows L apart); RCol reads the LDS
-// ring of k_rhs_xm, which holds rows J-1, J, J+1 as separate planes.
+// ring of k_rhs_xm, which holds rows J-1, J, J+1 as separate planes, in the
+// kernel's compute type T.
 struct GCol {
     const double* __restrict__ p;
     int L;
     __device__ __forceinline__ double operator()(int dj, int o) const { return p[dj * L + o]; }
 };
-struct RCol {
-    const double *m, *c, *p;
-    __device__ __forceinline__ double operator()(int dj, int o) const { return (dj < 0 ? m : dj > 0 ? p : c)[o]; }
+template <class T>
+struct RColT {
+    const T *m, *c, *p;
+    __device__ __forceinline__ T operator()(int dj, int o) const { return (dj < 0 ? m : dj > 0 ? p : c)[o]; }
 };
+using RCol = RColT<double>;
 
-// y coefficients of the block's row j (uniform over the block)
-struct YCoef {
-    double dyfi, dyci, wf0, wb0, wf1, wb1;
+// y coefficients of the block's row j (uniform over the block), in the
+// compute type T (double: the values themselves)
+template <class T>
+struct YCoefT {
+    T dyfi, dyci, wf0, wb0, wf1, wb1;
 };
+using YCoef = YCoefT<double>;
 
-__device__ __forceinline__ YCoef ycoef(const Geo& g, int j) {
-    YCoef y;
-    y.dyfi = ldc(g.dyfi, j);
-    y.dyci = ldc(g.dyci, j);
-    y.wf0 = ldc(g.wff, j);
-    y.wb0 = ldc(g.wfb, j);
-    y.wf1 = ldc(g.wff, j + 1);
-    y.wb1 = ldc(g.wfb, j + 1);
+template <class T = double>
+__device__ __forceinline__ YCoefT<T> ycoef(const Geo& g, int j) {
+    YCoefT<T> y;
+    y.dyfi = T(ldc(g.dyfi, j));
+    y.dyci = T(ldc(g.dyci, j));
+    y.wf0 = T(ldc(g.wff, j));
+    y.wb0 = T(ldc(g.wfb, j));
+    y.wf1 = T(ldc(g.wff, j + 1));
+    y.wb1 = T(ldc(g.wfb, j + 1));
     return y;
 }
 
 // Convection H_c (CONVECTION_X/Y/Z_tg: src/CONVECTION_X.f90:23-74,
 // src/CONVECTION_Y.f90:22-127, src/CONVECTION_Z.f90:22-112; channel: RCCI1 =
 // RNDI1 = RCCI2 = 1, XND2CL = YND2CL = ZND2CL = 1/2) at the cell (i, J, k)
-// of U, V, W.
-template <class A>
-__device__ __forceinline__ double conv_x(const Geo& g, const YCoef& y, const A& U, const A& V, const A& W,
-                                         const Nb& n) {
-    const double COE1 = 0.5 * 0.5 * g.dxi;
-    const double COE2 = y.dyfi * 0.5 * 1.0;
-    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
-    const double u_c = U(0, 0), u_p = U(0, n.ip), u_m = U(0, n.im);
-    const double H11 = ((u_p + u_c) * (u_p + u_c) - (u_m + u_c) * (u_m + u_c)) * COE1;
-    const double H12 = ((V(1, 0) + V(1, n.im)) * (y.wf1 * U(1, 0) + y.wb1 * u_c) -
-                        (V(0, 0) + V(0, n.im)) * (y.wf0 * u_c + y.wb0 * U(-1, 0))) * COE2;
-    const double H13 = ((W(0, n.kp) + W(0, n.im + n.kp)) * (U(0, n.kp) + u_c) -
-                        (W(0, 0) + W(0, n.im)) * (u_c + U(0, n.km))) * COE3;
+// of U, V, W, evaluated in T (FP64 form: double; mixed form: float).
+template <class A, class T = double>
+__device__ __forceinline__ T conv_x(const Geo& g, const YCoefT<T>& y, const A& U, const A& V, const A& W,
+                                    const Nb& n) {
+    const T COE1 = T(0.5 * 0.5 * g.dxi);
+    const T COE2 = y.dyfi * T(0.5) * T(1.0);
+    const T COE3 = T(0.5 * 0.5 * g.dzi * 1.0);
+    const T u_c = U(0, 0), u_p = U(0, n.ip), u_m = U(0, n.im);
+    const T H11 = ((u_p + u_c) * (u_p + u_c) - (u_m + u_c) * (u_m + u_c)) * COE1;
+    const T H12 = ((V(1, 0) + V(1, n.im)) * (y.wf1 * U(1, 0) + y.wb1 * u_c) -
+                   (V(0, 0) + V(0, n.im)) * (y.wf0 * u_c + y.wb0 * U(-1, 0))) * COE2;
+    const T H13 = ((W(0, n.kp) + W(0, n.im + n.kp)) * (U(0, n.kp) + u_c) -
+                   (W(0, 0) + W(0, n.im)) * (u_c + U(0, n.km))) * COE3;
     return -(H11 + H12 + H13);
 }
 
-template <class A>
-__device__ __forceinline__ double conv_y(const Geo& g, const YCoef& y, const A& U, const A& V, const A& W,
-                                         const Nb& n) {
-    const double COE1 = g.dxi * 0.5;
-    const double COE2 = y.dyci * 0.5 * 0.5;
-    const double COE31 = g.dzi * 0.5 * 1.0;
-    const double wf = y.wf0, wb = y.wb0;
-    const double v_c = V(0, 0);
-    const double H21 = ((wf * U(0, n.ip) + wb * U(-1, n.ip)) * (V(0, n.ip) + v_c) -
-                        (wf * U(0, 0) + wb * U(-1, 0)) * (v_c + V(0, n.im))) * COE1;
-    const double v_p = V(1, 0), v_m = V(-1, 0);
-    const double H22 = ((v_p * 1.0 + v_c * 1.0) * (v_p + v_c) - (v_c * 1.0 + v_m * 1.0) * (v_c + v_m)) * COE2;
-    const double H23 = ((wf * W(0, n.kp) * 1.0 + wb * W(-1, n.kp) * 1.0) * (V(0, n.kp) + v_c) -
-                        (wf * W(0, 0) * 1.0 + wb * W(-1, 0) * 1.0) * (v_c + V(0, n.km))) * COE31;
+template <class A, class T = double>
+__device__ __forceinline__ T conv_y(const Geo& g, const YCoefT<T>& y, const A& U, const A& V, const A& W,
+                                    const Nb& n) {
+    const T COE1 = T(g.dxi * 0.5);
+    const T COE2 = y.dyci * T(0.5) * T(0.5);
+    const T COE31 = T(g.dzi * 0.5 * 1.0);
+    const T wf = y.wf0, wb = y.wb0;
+    const T v_c = V(0, 0);
+    const T H21 = ((wf * U(0, n.ip) + wb * U(-1, n.ip)) * (V(0, n.ip) + v_c) -
+                   (wf * U(0, 0) + wb * U(-1, 0)) * (v_c + V(0, n.im))) * COE1;
+    const T v_p = V(1, 0), v_m = V(-1, 0);
+    const T H22 = ((v_p * T(1.0) + v_c * T(1.0)) * (v_p + v_c) - (v_c * T(1.0) + v_m * T(1.0)) * (v_c + v_m)) * COE2;
+    const T H23 = ((wf * W(0, n.kp) * T(1.0) + wb * W(-1, n.kp) * T(1.0)) * (V(0, n.kp) + v_c) -
+                   (wf * W(0, 0) * T(1.0) + wb * W(-1, 0) * T(1.0)) * (v_c + V(0, n.km))) * COE31;
     return -(H21 + H22 + H23);
 }
 
-template <class A>
-__device__ __forceinline__ double conv_z(const Geo& g, const YCoef& y, const A& U, const A& V, const A& W,
-                                         const Nb& n) {
-    const double COE1 = 0.5 * 0.5 * g.dxi;
-    const double COE2 = y.dyfi * 0.5;
-    const double COE3 = 0.5 * 0.5 * g.dzi * 1.0;
-    const double w_c = W(0, 0);
-    const double H31 = ((U(0, n.ip) + U(0, n.ip + n.km)) * (W(0, n.ip) + w_c) -
-                        (U(0, 0) + U(0, n.km)) * (w_c + W(0, n.im))) * COE1;
-    const double H32 = ((V(1, 0) + V(1, n.km)) * (y.wf1 * W(1, 0) * 1.0 + y.wb1 * w_c * 1.0) -
-                        (V(0, 0) + V(0, n.km)) * (y.wf0 * w_c * 1.0 + y.wb0 * W(-1, 0) * 1.0)) * COE2;
-    const double w_p = W(0, n.kp), w_m = W(0, n.km);
-    const double H33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * COE3;
+template <class A, class T = double>
+__device__ __forceinline__ T conv_z(const Geo& g, const YCoefT<T>& y, const A& U, const A& V, const A& W,
+                                    const Nb& n) {
+    const T COE1 = T(0.5 * 0.5 * g.dxi);
+    const T COE2 = y.dyfi * T(0.5);
+    const T COE3 = T(0.5 * 0.5 * g.dzi * 1.0);
+    const T w_c = W(0, 0);
+    const T H31 = ((U(0, n.ip) + U(0, n.ip + n.km)) * (W(0, n.ip) + w_c) -
+                   (U(0, 0) + U(0, n.km)) * (w_c + W(0, n.im))) * COE1;
+    const T H32 = ((V(1, 0) + V(1, n.km)) * (y.wf1 * W(1, 0) * T(1.0) + y.wb1 * w_c * T(1.0)) -
+                   (V(0, 0) + V(0, n.km)) * (y.wf0 * w_c * T(1.0) + y.wb0 * W(-1, 0) * T(1.0))) * COE2;
+    const T w_p = W(0, n.kp), w_m = W(0, n.km);
+    const T H33 = ((w_p + w_c) * (w_p + w_c) - (w_c + w_m) * (w_c + w_m)) * COE3;
     return -(H31 + H32 + H33);
 }
 
 // Per-row coefficients of the RHS: the y viscous stencil of each component
 // and the RK3 factors.
-struct RowCoef {
-    YCoef y;
-    double ap[3], ac[3], am[3];
+template <class T>
+struct RowCoefT {
+    YCoefT<T> y;
+    T ap[3], ac[3], am[3];
 };
+using RowCoef = RowCoefT<double>;
 
-__device__ __forceinline__ RowCoef row_coef(const Geo& g, int j) {
-    RowCoef rc;
-    rc.y = ycoef(g, j);
+template <class T = double>
+__device__ __forceinline__ RowCoefT<T> row_coef(const Geo& g, int j) {
+    RowCoefT<T> rc;
+    rc.y = ycoef<T>(g, j);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        rc.ap[c] = ldc(g.apvr, c * (g.M + 1) + j);
-        rc.ac[c] = ldc(g.acvr, c * (g.M + 1) + j);
-        rc.am[c] = ldc(g.amvr, c * (g.M + 1) + j);
+        rc.ap[c] = T(ldc(g.apvr, c * (g.M + 1) + j));
+        rc.ac[c] = T(ldc(g.acvr, c * (g.M + 1) + j));
+        rc.am[c] = T(ldc(g.amvr, c * (g.M + 1) + j));
     }
     return rc;
 }
@@ -164,30 +174,32 @@ __device__ __forceinline__ RowCoef row_coef(const Geo& g, int j) {
 // RHS_CvLpGpS_tg (src/RHS_CvLpGpS_tg.f90:117-176) at one cell for u, v, w:
 // conv[c] = H_c (the new CONVH0), rhs[c] = the x-solve right-hand side.
 // h0[c] is the cell's CONVH0 of the previous substep, P accesses PR.
-template <class A>
-__device__ __forceinline__ void rhs_cell(const Geo& g, const RkCoef& rk, double cvisc, const RowCoef& rc,
+template <class A, class T = double>
+__device__ __forceinline__ void rhs_cell(const Geo& g, const RkCoef& rk, double cvisc, const RowCoefT<T>& rc,
                                          const A& U, const A& V, const A& W, const A& P, const Nb& n,
-                                         const double* h0, double* conv, double* rhs) {
-    const double COE1 = rk.talp * cvisc;
-    const double COE2 = rk.talp * rk.dt;
+                                         const T* h0, T* conv, T* rhs) {
+    const T COE1 = T(rk.talp * cvisc);
+    const T COE2 = T(rk.talp * rk.dt);
+    const T tgam = T(rk.tgam), troh = T(rk.troh), dt = T(rk.dt);
+    const T dxqi = T(g.dxqi), dzqi = T(g.dzqi), dxi = T(g.dxi), dzi = T(g.dzi);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const A& X = c == 0 ? U : c == 1 ? V : W;
-        conv[c] = c == 0 ? conv_x(g, rc.y, U, V, W, n) : c == 1 ? conv_y(g, rc.y, U, V, W, n)
-                                                                 : conv_z(g, rc.y, U, V, W, n);
-        const double RHSC = rk.tgam * conv[c] + rk.troh * h0[c];
-        const double qc = X(0, 0);
-        const double RHSL = COE1 * ((X(0, n.ip) - 2.0 * qc + X(0, n.im)) * g.dxqi +
-                                    (X(0, n.kp) - 2.0 * qc + X(0, n.km)) * g.dzqi * 1.0 +
-                                    (X(1, 0) * rc.ap[c] + qc * rc.ac[c] + X(-1, 0) * rc.am[c]));
-        const double r = (RHSC + RHSL) * rk.dt;
-        double pgm;
+        conv[c] = c == 0 ? conv_x<A, T>(g, rc.y, U, V, W, n) : c == 1 ? conv_y<A, T>(g, rc.y, U, V, W, n)
+                                                                       : conv_z<A, T>(g, rc.y, U, V, W, n);
+        const T RHSC = tgam * conv[c] + troh * h0[c];
+        const T qc = X(0, 0);
+        const T RHSL = COE1 * ((X(0, n.ip) - T(2.0) * qc + X(0, n.im)) * dxqi +
+                               (X(0, n.kp) - T(2.0) * qc + X(0, n.km)) * dzqi * T(1.0) +
+                               (X(1, 0) * rc.ap[c] + qc * rc.ac[c] + X(-1, 0) * rc.am[c]));
+        const T r = (RHSC + RHSL) * dt;
+        T pgm;
         if (c == 0)
-            pgm = (P(0, 0) - P(0, n.im)) * g.dxi * COE2;
+            pgm = (P(0, 0) - P(0, n.im)) * dxi * COE2;
         else if (c == 1)
-            pgm = (P(0, 0) - P(-1, 0)) * (rc.y.dyci / 1.0) * COE2;
+            pgm = (P(0, 0) - P(-1, 0)) * (rc.y.dyci / T(1.0)) * COE2;
         else
-            pgm = (P(0, 0) - P(0, n.km)) * g.dzi * COE2;
+            pgm = (P(0, 0) - P(0, n.km)) * dzi * COE2;
         rhs[c] = r - pgm;
     }
 }
@@ -401,27 +413,43 @@ constexpr int XM_KP = KRS + 2;  // k rows of a ring plane
 constexpr int XM_PD = 1;        // steps the ring's next rows are loaded ahead
 constexpr int XM_CPT = 1;       // cells of a row per thread (1 or 2)
 
-inline size_t rhs_xm_smem(int L, int KX) {
-    return sizeof(double) * ((size_t)9 * XM_KP * L + (size_t)(KRS + 1) * L + (size_t)3 * KRS * (L + 2 * KX) + 16);
+// LDS of k_rhs_xm in compute type T: the ring, the PR plane and the x-solve
+// rows in T, then 16 doubles of block sums
+inline size_t rhs_xm_smem(int L, int KX, size_t tsize = sizeof(double)) {
+    size_t b = tsize * ((size_t)9 * XM_KP * L + (size_t)(KRS + 1) * L + (size_t)3 * KRS * (L + 2 * KX));
+    return ((b + 7) & ~(size_t)7) + 16 * sizeof(double);
 }
 
-template <int KX, int PD, int CPT>
+// four consecutive values (16-byte aligned for float, 32 for double)
+__device__ __forceinline__ void store4(double* dst, const double* v) {
+    reinterpret_cast<double2*>(dst)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2*>(dst)[1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void store4(float* dst, const float* v) {
+    reinterpret_cast<float4*>(dst)[0] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// T: the compute type of the cells' right-hand sides, the LDS ring and the x
+// solve (double: the FP64 form; float: the mixed form's FP32 stencils, whose
+// RHS leaves in FP32).  Q, PR and CONVH0 stay FP64 in HBM.
+template <int KX, int PD, int CPT, class T = double>
 __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
                                                const double* __restrict__ PR, double* __restrict__ H0,
-                                               double* __restrict__ RHS, double* __restrict__ part,
+                                               T* __restrict__ RHS, double* __restrict__ part,
                                                const double* __restrict__ wx, int lgL, int nyc, int jc) {
     static_assert(KRS % CPT == 0 && KRS / CPT >= 2, "CPT cells per thread, a low and a high halo row");
     constexpr int RB = KRS / CPT;  // thread rows
-    extern __shared__ double sm[];
+    extern __shared__ double smd[];
+    T* sm = reinterpret_cast<T*>(smd);
     constexpr int KR = KRS, KP = XM_KP, W = 2 * KX + 1;
     const int L = g.L, M = g.M, N = g.N;
     const int PL = KP * L;            // one component of one ring row
     const int SL = 3 * PL;            // one ring row (u, v, w)
-    double* ring = sm;                // 3 ring rows
-    double* pln = sm + 3 * SL;        // PR row J, k0-1 .. k0+KR-1
+    T* ring = sm;                     // 3 ring rows
+    T* pln = sm + 3 * SL;             // PR row J, k0-1 .. k0+KR-1
     const int LS = L + 2 * KX, Q4 = LS >> 2;
-    double* xr = pln + (KR + 1) * L;  // 3*KR x-solve rows (row_pos layout)
-    double* red = xr + 3 * KR * LS;
+    T* xr = pln + (KR + 1) * L;       // 3*KR x-solve rows (row_pos layout)
+    double* red = smd + (sizeof(T) * (size_t)(9 * KP * L + (KR + 1) * L + 3 * KR * LS) + 7) / 8;
     const int nt = (N / KR) * nyc;
     const int per = (nt + 7) / 8;
     const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD x: tiles x*per .. (x+1)*per-1
@@ -459,10 +487,10 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
         const int row = L * (j0 + s);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double a[CPT];
+            T a[CPT];
 #pragma unroll
-            for (int u = 0; u < CPT; ++u) a[u] = Q[gc[u] + row + c * CS];
-            const double ah = Q[gh + row + c * CS];
+            for (int u = 0; u < CPT; ++u) a[u] = T(Q[gc[u] + row + c * CS]);
+            const T ah = T(Q[gh + row + c * CS]);
 #pragma unroll
             for (int u = 0; u < CPT; ++u) ring[s * SL + c * PL + lc[u]] = a[u];
             ring[s * SL + c * PL + lh] = ah;
@@ -470,20 +498,20 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     }
     {
         const int row = L * (j0 + 1);
-        double a[CPT];
+        T a[CPT];
 #pragma unroll
-        for (int u = 0; u < CPT; ++u) a[u] = PR[gc[u] + row];
-        const double ah = PR[gph + row];
+        for (int u = 0; u < CPT; ++u) a[u] = T(PR[gc[u] + row]);
+        const T ah = T(PR[gph + row]);
 #pragma unroll
         for (int u = 0; u < CPT; ++u) pln[lc[u]] = a[u];
         pln[lph] = ah;
     }
-    double pm[CPT], h0[CPT][3];  // own PR at row J-1, own CONVH0 of row j
+    T pm[CPT], h0[CPT][3];  // own PR at row J-1, own CONVH0 of row j
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
-        pm[u] = PR[gc[u] + L * j0];
+        pm[u] = T(PR[gc[u] + L * j0]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) h0[u][c] = H0[i + L * j0 + L * M * kc[u] + c * LMN];
+        for (int c = 0; c < 3; ++c) h0[u][c] = T(H0[i + L * j0 + L * M * kc[u] + c * LMN]);
     }
     const int lg4 = lgL - 2;
     const int nw = blockDim.x >> 6;
@@ -496,7 +524,7 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     // the CONVH0 row j+1 of the next step.  Rows past the chunk's end are
     // clamped into the array: loaded, never used.
     struct Pre {
-        double q[CPT][3], qh[3], h[CPT][3], p[CPT], ph;
+        T q[CPT][3], qh[3], h[CPT][3], p[CPT], ph;
     };
     // CONVH0 is loaded first: the wait for the Q rows at the ring store then
     // covers it, so that using it in the next step waits for nothing (the
@@ -507,15 +535,15 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) x.h[u][c] = H0[i + rh + L * M * kc[u] + c * LMN];
-            x.p[u] = PR[gc[u] + rp];
+            for (int c = 0; c < 3; ++c) x.h[u][c] = T(H0[i + rh + L * M * kc[u] + c * LMN]);
+            x.p[u] = T(PR[gc[u] + rp]);
         }
-        x.ph = PR[gph + rp];
+        x.ph = T(PR[gph + rp]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
 #pragma unroll
-            for (int u = 0; u < CPT; ++u) x.q[u][c] = Q[gc[u] + rq + c * CS];
-            x.qh[c] = Q[gh + rq + c * CS];
+            for (int u = 0; u < CPT; ++u) x.q[u][c] = T(Q[gc[u] + rq + c * CS]);
+            x.qh[c] = T(Q[gh + rq + c * CS]);
         }
     };
     Pre pa;
@@ -528,28 +556,30 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
         else
             load_pre(j, pa);
         // RHS of row j from the ring
-        const RowCoef rc = row_coef(g, j);
-        double v = 0.0, pc[CPT];
+        const RowCoefT<T> rc = row_coef<T>(g, j);
+        const double dyfj = ldc(g.dyfi, j);
+        double v = 0.0;
+        T pc[CPT];
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
             const int o = lc[u];
-            const RCol U{ring + sM * SL + o, ring + s0 * SL + o, ring + sP * SL + o};
-            const RCol V{U.m + PL, U.c + PL, U.p + PL};
-            const RCol Wc{V.m + PL, V.c + PL, V.p + PL};
+            const RColT<T> U{ring + sM * SL + o, ring + s0 * SL + o, ring + sP * SL + o};
+            const RColT<T> V{U.m + PL, U.c + PL, U.p + PL};
+            const RColT<T> Wc{V.m + PL, V.c + PL, V.p + PL};
             pc[u] = pln[o];
-            const RCol P{&pm[u], pln + o, nullptr};
-            double conv[3], rhs[3];
-            rhs_cell(g, rk, cvisc, rc, U, V, Wc, P, n, h0[u], conv, rhs);
+            const RColT<T> P{&pm[u], pln + o, nullptr};
+            T conv[3], rhs[3];
+            rhs_cell<RColT<T>, T>(g, rk, cvisc, rc, U, V, Wc, P, n, h0[u], conv, rhs);
             const int ci = i + L * j + L * M * kc[u];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
-                double* row = xr + (c * KR + rb + RB * u) * LS;
+                if (c != 1 || j != 0) H0[ci + c * LMN] = double(conv[c]);
+                T* row = xr + (c * KR + rb + RB * u) * LS;
                 row[row_pos<KX>(KX + i, Q4)] = rhs[c];
                 if (i < KX) row[row_pos<KX>(KX + i + L, Q4)] = rhs[c];
                 if (i >= L - KX) row[row_pos<KX>(KX + i - L, Q4)] = rhs[c];
             }
-            v += rhs[0] / rc.y.dyfi / 1.0;
+            v += double(rhs[0]) / dyfj / 1.0;
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -583,21 +613,20 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
             const int cr = e >> lg4, g4 = e & ((1 << lg4) - 1);
             const int c = cr / KR, rr = cr - c * KR;
             if (c == 1 && j == 0) continue;
-            const double* xrow = xr + cr * LS + g4;
-            double x[W + 3];
+            const T* xrow = xr + cr * LS + g4;
+            T x[W + 3];
 #pragma unroll
             for (int tt = 0; tt < W + 3; ++tt) x[tt] = xrow[(tt & 3) * Q4 + (tt >> 2)];
-            double o4[4];
+            T o4[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                double acc = 0.0;
+                T acc = T(0.0);
 #pragma unroll
-                for (int m = 0; m < W; ++m) acc += ldc(wx, m) * x[m + u];
+                for (int m = 0; m < W; ++m) acc += T(ldc(wx, m)) * x[m + u];
                 o4[u] = acc;
             }
-            double* dst = RHS + (c * LMN + 4 * g4 + L * (j + M * (k0 + rr)));
-            reinterpret_cast<double2*>(dst)[0] = make_double2(o4[0], o4[1]);
-            reinterpret_cast<double2*>(dst)[1] = make_double2(o4[2], o4[3]);
+            T* dst = RHS + (c * LMN + 4 * g4 + L * (j + M * (k0 + rr)));
+            store4(dst, o4);
         }
         if (PD == 2) pa = pb;
         __syncthreads();
@@ -765,10 +794,11 @@ constexpr int YJC = 64;     // rows per thread chunk of the y window (measured: 
 constexpr int YNS = 4;      // prefetch slots (rows fetched ahead) of the y window (2 -> 4: 200 -> 192 us)
 constexpr int WU = 4;       // rows fetched ahead per step of the z window (2: 140 -> 153 us; slots
                             // refilled a step ahead as in the y window: 133 -> 141 us)
-template <int KW>
+template <int KW, class TR = double>
 // One wave per block (chunk blockIdx.z): its LDS holds only the chunk's
-// weight rows, so more blocks fit a CU.
-__global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict__ R2, double* __restrict__ Q,
+// weight rows, so more blocks fit a CU.  TR: the storage type of the z-solved
+// right-hand side (float in the mixed form); the y elimination is FP64.
+__global__ void __launch_bounds__(64) k_y_window(Geo g, const TR* __restrict__ R2, double* __restrict__ Q,
                                                  const double* __restrict__ wy) {
     const int L = g.L, M = g.M, N = g.N;
     const int ntx = (L + 63) / 64;
@@ -784,22 +814,22 @@ __global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict
     double du0 = 0.0;
     const int c = blockIdx.y;
     {
-        const double* Rc = R2 + (size_t)c * LM * N + (size_t)k * LM + i;
+        const TR* Rc = R2 + (size_t)c * LM * N + (size_t)k * LM + i;
         for (int e = threadIdx.x; e < (jn - j0) * W; e += blockDim.x) sw[e] = wy[((size_t)c * M + j0) * W + e];
         __syncthreads();
         if (!valid) return;
         if (c == 1 && ch == 0) {
             // v's row-1 right-hand side: the u increment on j = 1 (row 0 of u's solve)
-            const double* Ru = R2 + (size_t)k * LM + i;
+            const TR* Ru = R2 + (size_t)k * LM + i;
 #pragma unroll
-            for (int m = KW; m < W; ++m) du0 += wy[m] * Ru[(size_t)(m - KW) * L];
+            for (int m = KW; m < W; ++m) du0 += wy[m] * double(Ru[(size_t)(m - KW) * L]);
         }
         double win[W];
 #pragma unroll
         for (int m = 0; m < W; ++m) {
             const int jj = j0 - KW + m;
             double v = 0.0;
-            if (jj >= 0 && jj < M) v = (c == 1 && jj == 0) ? du0 : Rc[(size_t)jj * L];
+            if (jj >= 0 && jj < M) v = (c == 1 && jj == 0) ? du0 : double(Rc[(size_t)jj * L]);
             win[m] = v;
         }
         // YNS fixed prefetch slots, slot u serving the steps j = j0 + u mod
@@ -810,7 +840,7 @@ __global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict
         double rs[YNS], qs[YNS];
 #pragma unroll
         for (int u = 0; u < YNS; ++u) {
-            rs[u] = Rc[(size_t)min(j0 + KW + 1 + u, M - 1) * L];
+            rs[u] = double(Rc[(size_t)min(j0 + KW + 1 + u, M - 1) * L]);
             qs[u] = Q[qidx(g, c, i, min(j0 + u, M - 1) + 1, k)];
         }
         auto step = [&](int j, double& r, double& q) {
@@ -822,7 +852,7 @@ __global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict
 #pragma unroll
             for (int m = 0; m < W - 1; ++m) win[m] = win[m + 1];
             win[W - 1] = j + KW + 1 < M ? r : 0.0;
-            r = Rc[(size_t)min(j + KW + 1 + YNS, M - 1) * L];
+            r = double(Rc[(size_t)min(j + KW + 1 + YNS, M - 1) * L]);
             q = Q[qidx(g, c, i, min(j + YNS, M - 1) + 1, k)];
         };
         int j = j0;
@@ -849,8 +879,9 @@ __global__ void __launch_bounds__(64) k_y_window(Geo g, const double* __restrict
 // through the x solve).  Reads RHS, writes RHS2.
 constexpr int ZKC = 64;
 static_assert(ZKC % WU == 0, "k_z_window steps a chunk WU rows at a time: a partial step would run past the chunk");
-template <int KW>
-__global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restrict__ R, double* __restrict__ R2,
+// T: storage and arithmetic type (float in the mixed form).
+template <int KW, class T = double>
+__global__ void __launch_bounds__(256) k_z_window(Geo g, const T* __restrict__ R, T* __restrict__ R2,
                                                   const double* __restrict__ dpg, double xone, int use_dpg,
                                                   const double* __restrict__ wz, int zkc) {
     const int L = g.L, M = g.M, N = g.N;
@@ -865,15 +896,15 @@ __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restric
     const int i = it * 64 + lane;
     if (i >= L || chunk >= nch || (cc == 1 && j == 0)) return;
     const size_t LM = (size_t)L * M;
-    const double* Rc = R + (size_t)cc * LM * N + (size_t)j * L + i;
-    double* Oc = R2 + (size_t)cc * LM * N + (size_t)j * L + i;
-    const double D = (use_dpg && cc == 0) ? dpg[0] * xone : 0.0;
+    const T* Rc = R + (size_t)cc * LM * N + (size_t)j * L + i;
+    T* Oc = R2 + (size_t)cc * LM * N + (size_t)j * L + i;
+    const T D = T((use_dpg && cc == 0) ? dpg[0] * xone : 0.0);
     constexpr int W = 2 * KW + 1;
-    double w[W];
+    T w[W];
 #pragma unroll
-    for (int m = 0; m < W; ++m) w[m] = wz[m];
+    for (int m = 0; m < W; ++m) w[m] = T(wz[m]);
     const int k0 = chunk * zkc;
-    double win[W];
+    T win[W];
 #pragma unroll
     for (int m = 0; m < W; ++m) {
         int kk = k0 - KW + m;
@@ -881,7 +912,7 @@ __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restric
         win[m] = Rc[(size_t)kk * LM] - D;
     }
     for (int kb = k0; kb < k0 + zkc; kb += WU) {
-        double rn[WU];
+        T rn[WU];
 #pragma unroll
         for (int u = 0; u < WU; ++u) {
             int kn = kb + u + KW + 1;
@@ -890,7 +921,7 @@ __global__ void __launch_bounds__(256) k_z_window(Geo g, const double* __restric
         }
 #pragma unroll
         for (int u = 0; u < WU; ++u) {
-            double acc = 0.0;
+            T acc = T(0.0);
 #pragma unroll
             for (int m = 0; m < W; ++m) acc += w[m] * win[m];
             Oc[(size_t)(kb + u) * LM] = acc;
@@ -951,20 +982,39 @@ __global__ void __launch_bounds__(256) k_divg(Geo g, double coe0, const double* 
 // (rows past M are clamped into the array and not stored; v's wall row is
 // written back unchanged, so no store sits under a per-row branch).
 constexpr int UPJ = 1;   // 2 or 4 rows per thread measured no faster (189 -> 189, 192 us)
+
+__device__ __forceinline__ double nanmax(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
+
+// CHK (the mixed-precision form): the same update, and the Poisson residual
+// r = f - L_h phi of the refined solution from the DPH values the update
+// loads plus phi(i+1), phi(k+1) and phi(J+1) (wall rows: the Neumann rule of
+// the solve), reduced per tile to part[2 tile] = max|r|, part[2 tile + 1] =
+// max|f| (NaN propagates).  alpha dt max|r| is the projected max|div u|.
+struct ResidCoef {
+    const double *amph, *acph, *apph;
+    double* part;
+};
+
+template <bool CHK>
 __global__ void __launch_bounds__(256) k_update(Geo g, double cu, double cw, double cv, double be,
                                                 const double* __restrict__ DPH, const double* __restrict__ F,
-                                                double* __restrict__ Q, double* __restrict__ PR) {
+                                                double* __restrict__ Q, double* __restrict__ PR, ResidCoef rc) {
     int it, jg, kt, tile;
     const int M = g.M;
     if (!row_tile(g.L, g.N, (M + UPJ - 1) / UPJ, &it, &jg, &kt, &tile)) return;
     const int L = g.L, N = g.N;
-    const int i = it * 64 + (threadIdx.x & 63), k = kt * 4 + (threadIdx.x >> 6);
-    if (i >= L || k >= N) return;
+    const int i0 = it * 64 + (threadIdx.x & 63), k0 = kt * 4 + (threadIdx.x >> 6);
+    const bool valid = i0 < L && k0 < N;
+    if (!CHK && !valid) return;
+    // CHK: lanes outside the grid evaluate a valid cell, store nothing and
+    // join the block's reduction with zeros
+    const int i = min(i0, L - 1), k = min(k0, N - 1);
     const int LP = L * (M + 2), CS = LP * N;
     const int im = i == 0 ? L - 1 : -1, km = k == 0 ? (N - 1) * LP : -LP;
+    const int ip = i == L - 1 ? 1 - L : 1, kp = k == N - 1 ? (1 - N) * LP : LP;
     const int j0 = jg * UPJ;
     const int g0 = i + L * (j0 + 1) + LP * k;  // (i, J0, k); J0 - 1 >= 0 is always in the array
-    double ph[UPJ + 1], dxm[UPJ], dkm[UPJ], qu[UPJ], qv[UPJ], qw[UPJ], f[UPJ], pr[UPJ];
+    double ph[UPJ + 2], dxm[UPJ], dkm[UPJ], dxp[UPJ], dkp[UPJ], qu[UPJ], qv[UPJ], qw[UPJ], f[UPJ], pr[UPJ];
     ph[0] = DPH[g0 - L];
 #pragma unroll
     for (int r = 0; r < UPJ; ++r) {
@@ -973,16 +1023,35 @@ __global__ void __launch_bounds__(256) k_update(Geo g, double cu, double cw, dou
         ph[r + 1] = DPH[go];
         dxm[r] = DPH[go + im];
         dkm[r] = DPH[go + km];
+        if (CHK) {
+            dxp[r] = DPH[go + ip];
+            dkp[r] = DPH[go + kp];
+        }
         qu[r] = Q[go];
         qv[r] = Q[go + CS];
         qw[r] = Q[go + 2 * CS];
         f[r] = F[i + L * (jc + M * k)];
         pr[r] = PR[go];
     }
+    if (CHK) ph[UPJ + 1] = DPH[g0 + L * (min(j0 + UPJ, M) - j0)];
+    double mr = 0.0, mf = 0.0;
 #pragma unroll
     for (int r = 0; r < UPJ; ++r) {
         const int j = j0 + r;
         if (j >= M) break;
+        if (CHK) {
+            const double pc = ph[r + 1];
+            const double pjp = j == M - 1 ? pc : ph[r + 2];
+            const double pjm = j == 0 ? pc : ph[r];
+            const double lx = (dxp[r] - 2.0 * pc + dxm[r]) * g.dxqi;
+            const double ly = pjp * ldc(rc.apph, j) + pc * ldc(rc.acph, j) + pjm * ldc(rc.amph, j);
+            const double lz = (dkp[r] - 2.0 * pc + dkm[r]) * g.dzqi;
+            if (valid) {
+                mr = nanmax(mr, fabs(f[r] - (lx + ly + lz)));
+                mf = nanmax(mf, fabs(f[r]));
+            }
+            if (!valid) continue;
+        }
         const int go = g0 + L * r;
         const double cvj = cv * ldc(g.dyci, j) / 1.0;
         Q[go] = qu[r] - (ph[r + 1] - dxm[r]) * cu;
@@ -993,9 +1062,30 @@ __global__ void __launch_bounds__(256) k_update(Geo g, double cu, double cw, dou
         if (j == 0) PR[go - L] = p;
         if (j == M - 1) PR[go + L] = p;
     }
+    if (CHK) {
+        __shared__ double red[2][4];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mr = nanmax(mr, __shfl_xor(mr, o));
+            mf = nanmax(mf, __shfl_xor(mf, o));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[0][threadIdx.x >> 6] = mr;
+            red[1][threadIdx.x >> 6] = mf;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = red[0][0], b = red[1][0];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) {
+                a = nanmax(a, red[0][w]);
+                b = nanmax(b, red[1][w]);
+            }
+            rc.part[2 * tile] = a;
+            rc.part[2 * tile + 1] = b;
+        }
+    }
 }
-
-__device__ __forceinline__ double nanmax(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
 
 // max |div u| (DIVGCK_tg, src/DIVGCK.f90:33-46) and max CFL rate (CFL_tg,
 // src/CFL.f90:39-53) in one sweep: part[2b] = div, part[2b+1] = cfl.
@@ -1111,6 +1201,35 @@ __global__ void k_reduce_max2_nan(const double* __restrict__ part, int nb, doubl
 // ---------------------------------------------------------------------------
 bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small = false);
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
+int poisson_solve_unchecked(chs_handle* h, const double* f, double* dph, int ghost);
+
+// The verdict of the last mixed-precision substep's convergence check, once
+// its values have reached the host: wait = false leaves it pending while the
+// GPU is still busy.  A NaN or a residual above the tolerance is reported by
+// the call that reads it (CHS_E_NOTCONV); the state is then not to be trusted.
+int mixed_check(chs_handle* h, bool wait) {
+    if (!h->chk_pending) return CHS_OK;
+    if (wait) {
+        CHS_HIP(hipEventSynchronize(h->chk_ev));
+    } else {
+        const hipError_t q = hipEventQuery(h->chk_ev);
+        if (q == hipErrorNotReady) return CHS_OK;
+        CHS_HIP(q);
+    }
+    h->chk_pending = false;
+    const double r = h->chk_host[0], fm = h->chk_host[1];
+    h->last_refine_resid = fm > 0.0 ? r / fm : 0.0;
+    if (r != r || fm != fm) {
+        set_error("substep: NaN in the mixed-precision Poisson residual of an earlier substep");
+        return CHS_E_NOTCONV;
+    }
+    if (r > h->chk_tol) {
+        set_error("substep: the mixed-precision Poisson solve of an earlier substep left max|f - L phi| = " +
+                  std::to_string(r) + " above its bound " + std::to_string(h->chk_tol));
+        return CHS_E_NOTCONV;
+    }
+    return CHS_OK;
+}
 
 static Geo geo_of(const chs_handle* h) {
     Geo g;
@@ -1342,6 +1461,7 @@ int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky) {
 }
 
 int rk3_substep(chs_handle* h, int ns, double dt) {
+    CHS_TRY(mixed_check(h, false));
     const Geo g = geo_of(h);
     const hipStream_t s = h->stream;
     RkCoef rk;
@@ -1361,6 +1481,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     // (blocks of 64*chunks threads), and chunks longer than the window
     const bool use_st = cn->ok && h->rhs2 != nullptr && h->cn_mode != 1 && h->N % ZKC == 0 &&
                         (h->M + YJC - 1) / YJC <= 8 && h->M > 16;
+    bool f32 = false;
     // 1. RHS + x solve
     {
         const double COE = -facoe * h->dxqi;
@@ -1377,8 +1498,35 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const bool use_xm = use_st && h->cn_mode == 0 && (KRS / XM_CPT) * h->L >= 64 &&
                             (KRS / XM_CPT) * h->L <= (XM_CPT == 1 ? 1024 : 512) &&
                             rhs_xm_smem(h->L, cn->KWx) <= (size_t)h->lds_max;
+        // the mixed form's FP32 stencils: RHS, the x solve and the z solve in
+        // FP32 (RHS and RHS2 stored as float in the first half of the FP64
+        // buffers); CONVH0 and the y elimination stay FP64
+        f32 = h->precision == CHS_MIXED && use_xm;
         h->timer.begin("rhs_x", s, &ev);
-        if (use_xm) {
+        if (f32) {
+            const int nkb = h->N / KRS;
+            int nyc = std::max(1, std::min(h->M / 4, (256 + nkb - 1) / nkb));
+            const int jc = (h->M + nyc - 1) / nyc;
+            nyc = (h->M + jc - 1) / jc;
+            const dim3 gm(8 * ((nkb * nyc + 7) / 8));
+            const size_t sm = rhs_xm_smem(h->L, cn->KWx, sizeof(float));
+            const double* wx = cn->dev + cn->ox;
+            float* rf = reinterpret_cast<float*>(h->rhs);
+#define CHS_RHSXM32(KX)                                                                                          \
+    do {                                                                                                         \
+        (void)hipFuncSetAttribute((const void*)k_rhs_xm<KX, XM_PD, XM_CPT, float>,                                \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);                          \
+        CHS_LAUNCH((k_rhs_xm<KX, XM_PD, XM_CPT, float>), gm, dim3((KRS / XM_CPT) * h->L), sm, s, g, rk, h->cvisc, \
+                   h->q, h->pr, h->convh0, rf, h->partials, wx, lgL, nyc, jc);                                   \
+    } while (0)
+            switch (cn->KWx) {
+                case 4: CHS_RHSXM32(4); break;
+                case 8: CHS_RHSXM32(8); break;
+                case 12: CHS_RHSXM32(12); break;
+                default: CHS_RHSXM32(16); break;
+            }
+#undef CHS_RHSXM32
+        } else if (use_xm) {
             // about one block per CU (the ring takes most of a CU's LDS at
             // L = 256), chunks of >= 4 rows, none empty
             const int nkb = h->N / KRS;
@@ -1436,11 +1584,22 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
             const dim3 gzw(3 * h->M * ((h->L + 63) / 64), (h->N / zkc + 3) / 4);
             const double* wz = cn->dev + cn->oz;
             h->timer.begin("z_solve", s, &ev);
-            switch (cn->KWz) {
-                case 4: CHS_LAUNCH(k_z_window<4>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
-                case 8: CHS_LAUNCH(k_z_window<8>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
-                case 12: CHS_LAUNCH(k_z_window<12>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
-                default: CHS_LAUNCH(k_z_window<16>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+            if (f32) {
+                const float* rf = reinterpret_cast<const float*>(h->rhs);
+                float* r2f = reinterpret_cast<float*>(h->rhs2);
+                switch (cn->KWz) {
+                    case 4: CHS_LAUNCH((k_z_window<4, float>), gzw, dim3(256), 0, s, g, rf, r2f, h->scalars, xone, usedpg0, wz, zkc); break;
+                    case 8: CHS_LAUNCH((k_z_window<8, float>), gzw, dim3(256), 0, s, g, rf, r2f, h->scalars, xone, usedpg0, wz, zkc); break;
+                    case 12: CHS_LAUNCH((k_z_window<12, float>), gzw, dim3(256), 0, s, g, rf, r2f, h->scalars, xone, usedpg0, wz, zkc); break;
+                    default: CHS_LAUNCH((k_z_window<16, float>), gzw, dim3(256), 0, s, g, rf, r2f, h->scalars, xone, usedpg0, wz, zkc); break;
+                }
+            } else {
+                switch (cn->KWz) {
+                    case 4: CHS_LAUNCH(k_z_window<4>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+                    case 8: CHS_LAUNCH(k_z_window<8>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+                    case 12: CHS_LAUNCH(k_z_window<12>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+                    default: CHS_LAUNCH(k_z_window<16>, gzw, dim3(256), 0, s, g, h->rhs, h->rhs2, h->scalars, xone, usedpg0, wz, zkc); break;
+                }
             }
             h->timer.end(ev, "z_solve", s);
         }
@@ -1471,11 +1630,21 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const double* wy = cn->dev + cn->oy;
         const size_t smw = sizeof(double) * (size_t)std::min(h->M, YJC) * (2 * cn->KW + 1);
         h->timer.begin("y_solve", s, &ev);
-        switch (cn->KW) {
-            case 4: CHS_LAUNCH(k_y_window<4>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
-            case 8: CHS_LAUNCH(k_y_window<8>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
-            case 12: CHS_LAUNCH(k_y_window<12>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
-            default: CHS_LAUNCH(k_y_window<16>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+        if (f32) {
+            const float* r2f = reinterpret_cast<const float*>(h->rhs2);
+            switch (cn->KW) {
+                case 4: CHS_LAUNCH((k_y_window<4, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
+                case 8: CHS_LAUNCH((k_y_window<8, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
+                case 12: CHS_LAUNCH((k_y_window<12, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
+                default: CHS_LAUNCH((k_y_window<16, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
+            }
+        } else {
+            switch (cn->KW) {
+                case 4: CHS_LAUNCH(k_y_window<4>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+                case 8: CHS_LAUNCH(k_y_window<8>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+                case 12: CHS_LAUNCH(k_y_window<12>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+                default: CHS_LAUNCH(k_y_window<16>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+            }
         }
         h->timer.end(ev, "y_solve", s);
     } else {
@@ -1506,18 +1675,39 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_LAUNCH(k_divg, dim3(row_tile_blocks(h->L, h->N, (h->M + DVJ - 1) / DVJ)), dim3(256), 0, s, g,
                1.0 / (dt * rk.talp), h->q, h->rhsllphi);
     h->timer.end(ev, "divg", s);
-    // 5. Poisson + BC_WALL_DPH; in the mixed form refine until the projected
-    //    divergence alpha*dt*|f - L phi| is below half the 1e-10 target
-    CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (rk.talp * dt)));
+    // 5. Poisson + BC_WALL_DPH.  FP64: one pass.  Mixed: the FP32 pass and one
+    //    FP64-residual refinement (an FP32 pass leaves ~1e-7 of max|f|, one
+    //    refinement ~1e-12); the projection below checks that the projected
+    //    divergence alpha*dt*max|f - L phi| stays below half the 1e-10 target.
+    const int ntile = (int)(((h->L + 63) / 64) * ((h->N + 3) / 4) * (long)((h->M + UPJ - 1) / UPJ));
+    const bool fused_chk = h->precision == CHS_MIXED && h->refine_max > 0 && ntile <= h->npart;
+    if (fused_chk)
+        CHS_TRY(poisson_solve_unchecked(h, h->rhsllphi, h->dph, 1));
+    else
+        CHS_TRY(poisson_solve(h, h->rhsllphi, h->dph, 1, 0.5e-10 / (rk.talp * dt)));
     // 6. projection + pressure update
+    const ResidCoef rcf{h->amph, h->acph, h->apph, h->partials};
+    const dim3 gu(row_tile_blocks(h->L, h->N, (h->M + UPJ - 1) / UPJ));
     h->timer.begin("update", s, &ev);
-    CHS_LAUNCH(k_update, dim3(row_tile_blocks(h->L, h->N, (h->M + UPJ - 1) / UPJ)), dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
-               0.5 * rk.talp * dt * h->cvisc, h->dph, h->rhsllphi, h->q, h->pr);
+    if (fused_chk)
+        CHS_LAUNCH(k_update<true>, gu, dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
+                   0.5 * rk.talp * dt * h->cvisc, h->dph, h->rhsllphi, h->q, h->pr, rcf);
+    else
+        CHS_LAUNCH(k_update<false>, gu, dim3(256), 0, s, g, dt * rk.talp * h->dxi, dt * rk.talp * h->dzi, dt * rk.talp,
+                   0.5 * rk.talp * dt * h->cvisc, h->dph, h->rhsllphi, h->q, h->pr, rcf);
     h->timer.end(ev, "update", s);
+    if (fused_chk) {
+        CHS_LAUNCH(k_reduce_max2_nan, dim3(1), dim3(256), 0, s, h->partials, ntile, h->scalars + 4);
+        CHS_HIP(hipMemcpyAsync(h->chk_host, h->scalars + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        CHS_HIP(hipEventRecord(h->chk_ev, s));
+        h->chk_tol = 0.5e-10 / (rk.talp * dt);
+        h->chk_pending = true;
+    }
     return CHS_OK;
 }
 
 int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out) {
+    CHS_TRY(mixed_check(h, true));
     const Geo g = geo_of(h);
     const hipStream_t s = h->stream;
     const int nb = 1024;
@@ -1537,9 +1727,9 @@ int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out) {
 int mean_xz(chs_handle* h, double* out_host) {
     const Geo g = geo_of(h);
     const hipStream_t s = h->stream;
-    CHS_CHECK(13 * (size_t)h->M <= 2 * (size_t)h->npart, CHS_E_ARG, "chs_mean_xz: NCL2 too large for the work buffer");
-    CHS_LAUNCH(k_mean_xz, dim3(h->M), dim3(256), 0, s, g, h->q, h->pr, h->partials);
-    CHS_HIP(hipMemcpyAsync(out_host, h->partials, 13 * (size_t)h->M * sizeof(double), hipMemcpyDeviceToHost, s));
+    // kept on the device for the time means (chs_mean_t)
+    CHS_LAUNCH(k_mean_xz, dim3(h->M), dim3(256), 0, s, g, h->q, h->pr, h->stats);
+    CHS_HIP(hipMemcpyAsync(out_host, h->stats, 13 * (size_t)h->M * sizeof(double), hipMemcpyDeviceToHost, s));
     CHS_HIP(hipStreamSynchronize(s));
     return CHS_OK;
 }

@@ -104,14 +104,44 @@ def poisson_rhs(g: ChannelGrid, seed: int = 7) -> np.ndarray:
 
 # ---------------------------------------------------------------------------
 # Instant-field files DNS_perixz_INSTANT_T<time>_{U,V,W,P}.D
-# (src/processing/WRT_VARS_TG.f90:1-91, src/processing/RESTART_tg.f90:1-46):
-# header 4 x int32 (NCL1, NCL2, NCL3, ITERG) + 3 x float64 (time, Re, dt), then
-# the interior field (NCL1, NCL2, NCL3) as Fortran-order float64.
+# (WRT_INSTANT_VARS_TG, src/processing/WRT_VARS_TG.f90:1-91; read back by
+# RESTART_INSTANT_VARS_TG and READ_3D_VARS, src/processing/RESTART_tg.f90:1-46,
+# src/processing/READ_INTP_3D_VARS.f90:1-90): a 40-byte header of 4 int32
+# (NCL1, NCL2, NCL3, ITERG) and 3 float64 (phyTIME, REN, DT), then the
+# interior field (NCL1, NCL2, NCL3) as Fortran-order float64, native (little)
+# endian.  The time in the name is Fortran's ES15.9 edit of phyTIME.
 # ---------------------------------------------------------------------------
 _HDR = np.dtype([("n", "<i4", 4), ("r", "<f8", 3)])
+INSTANT_DIR = "1_instant_D/"          # filepath1 (src/modules.f90:125)
+_COMPONENTS = ("U", "V", "W", "P")    # NDV + 1 files, in the reference's order
+
+
+def es15_9(x: float) -> str:
+    """Fortran's ES15.9 edit of a time: d.dddddddddE+ee (15 characters), as
+    WRITE(PNTIM,'(1ES15.9)') phyTIME produces for 0 <= t < 1e100.  A negative
+    time or one of 1e100 or more does not fit 15 characters (Fortran writes
+    asterisks), so it is refused."""
+    x = float(x)
+    if not np.isfinite(x) or x < 0.0 or x >= 1e100:
+        raise ValueError(f"ES15.9 cannot represent the time {x!r} in 15 characters")
+    if x == 0.0:
+        return "0.000000000E+00"
+    mant, exp = f"{x:.9E}".split("E")
+    e = int(exp)
+    if e < -99:
+        raise ValueError(f"ES15.9 exponent of {x!r} needs three digits")
+    return f"{mant}E{'+' if e >= 0 else '-'}{abs(e):02d}"
+
+
+def instant_filename(component: str, phytime: float, filepath: str = INSTANT_DIR) -> str:
+    """TRIM(filepath1)//'DNS_perixz_INSTANT_T'//TRIM(PNTIM)//'_<U|V|W|P>.D'."""
+    if component not in _COMPONENTS:
+        raise ValueError(f"component must be one of {_COMPONENTS}")
+    return f"{filepath}DNS_perixz_INSTANT_T{es15_9(phytime)}_{component}.D"
 
 
 def write_instant(path: str, field: np.ndarray, iterg: int, time: float, ren: float, dt: float) -> None:
+    """One instant file: header then the (NCL1, NCL2, NCL3) interior field."""
     L, M, N = field.shape
     hdr = np.zeros((), dtype=_HDR)
     hdr["n"] = (L, M, N, iterg)
@@ -122,10 +152,56 @@ def write_instant(path: str, field: np.ndarray, iterg: int, time: float, ren: fl
 
 
 def read_instant(path: str):
+    """Returns (field (NCL1, NCL2, NCL3), ITERG, phyTIME, REN, DT) of one file."""
     with open(path, "rb") as fh:
-        hdr = np.frombuffer(fh.read(_HDR.itemsize), dtype=_HDR)[0]
+        raw = fh.read(_HDR.itemsize)
+        if len(raw) != _HDR.itemsize:
+            raise ValueError(f"{path}: shorter than the 40-byte header")
+        hdr = np.frombuffer(raw, dtype=_HDR)[0]
         L, M, N, it = (int(x) for x in hdr["n"])
+        if min(L, M, N) <= 0:
+            raise ValueError(f"{path}: bad dimensions {L}x{M}x{N} in the header")
         data = np.frombuffer(fh.read(8 * L * M * N), dtype="<f8")
+    if data.size != L * M * N:
+        raise ValueError(f"{path}: {data.size} values after the header, expected {L * M * N}")
     field = data.reshape((L, M, N), order="F").copy(order="F")
     t, ren, dt = (float(x) for x in hdr["r"])
     return field, it, t, ren, dt
+
+
+def write_instant_vars_tg(q: np.ndarray, pr: np.ndarray, iterg: int, phytime: float, ren: float, dt: float,
+                          filepath: str = INSTANT_DIR):
+    """WRT_INSTANT_VARS_TG: the four files of the tg state Q_tg(:, 0:M+1, :, 1:3)
+    and PR_tg(:, 0:M+1, :) (interior rows j = 1..M written).  Returns the paths."""
+    M = q.shape[1] - 2
+    paths = []
+    for c, comp in enumerate(_COMPONENTS):
+        fld = q[:, 1:M + 1, :, c] if c < 3 else pr[:, 1:M + 1, :]
+        path = instant_filename(comp, phytime, filepath)
+        write_instant(path, fld, iterg, phytime, ren, dt)
+        paths.append(path)
+    return paths
+
+
+def restart_instant_vars_tg(g: ChannelGrid, phytime: float, filepath: str = INSTANT_DIR):
+    """RESTART_INSTANT_VARS_TG(TFM) followed by FLOWSTART's wall conditions
+    (src/FLOWSTART.f90:27-33): reads the four files of time phytime into
+    Q_tg and PR_tg interiors, then BC_WALL_Q_tg / BC_WALL_PR_tg fill the ghost
+    rows.  Returns (Q, PR, ITERG, phyTIME).  The files' dimensions must be the
+    grid's (the reference reads them with the grid's sizes)."""
+    q = g.zeros(g.shape_q)
+    pr = g.zeros(g.shape_ghost)
+    M = g.M
+    iterg, t = None, None
+    for c, comp in enumerate(_COMPONENTS):
+        path = instant_filename(comp, phytime, filepath)
+        fld, it, t, _, _ = read_instant(path)
+        if fld.shape != (g.L, g.M, g.N):
+            raise ValueError(f"{path}: field {fld.shape} does not match the grid {(g.L, g.M, g.N)}")
+        iterg = it
+        if c < 3:
+            q[:, 1:M + 1, :, c] = fld
+        else:
+            pr[:, 1:M + 1, :] = fld
+    apply_wall_bc(g, q, pr)
+    return q, pr, iterg, t

@@ -14,6 +14,7 @@ SOLVE runs them for each RK3 substep (src/SOLVE.f90:119-133).
     HeatedChannel.solverrk3_mom_io(ns, dt)            SOLVERRK3_MOM_IO(NS)
     HeatedChannel.substep(ns, dt, cthecd)             the three in SOLVE's order
     HeatedChannel.divgck_io()                         DIVGCK_io
+    HeatedChannel.mean_xz_io()                        PP_MEAN_ZX_FLOW_Xperiodic_IO
 
 Fields use the reference layouts (Fortran order): the thermal fields of
 energy.ThermalState plus Q_io, PR_io, DrhoDtP, DivU_io (NCL1, 0:NCL2+1,
@@ -78,6 +79,16 @@ class HeatedChannel(ThermalState):
             self.solverrk3_eng_io(ns, dt, cthecd)
         self.io_prepare(ns)
         self.solverrk3_mom_io(ns, dt)
+
+    def mean_xz_io(self):
+        """PP_MEAN_ZX_FLOW_Xperiodic_IO moments of the device fields, (NCL2, 37):
+        <u v w p>, <g_m>, <u_m p>, <u_m u_n>, <u_m g_n>, T1 D1 H1 M1 T2 D2 H2
+        DH PH, <h u_m>, <h g_m> (columns as chs_mean_xz_io documents them)."""
+        out = np.zeros(37 * self.solver.grid.M)
+        tf, f = self._tf(), self._iof()
+        nat.check(self.solver.lib.chs_mean_xz_io(self.solver.h, C.byref(tf), C.byref(f), int(self.params.thermal),
+                                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(self.solver.grid.M, 37)
 
     def divgck_io(self):
         """(max|div G + DrhoDtP|, max|div Q|)."""

@@ -74,6 +74,27 @@ class ChannelSolver:
         nat.check(self.lib.chs_download_state(self.h, nat.hptr(q), nat.hptr(pr), nat.hptr(c0)))
         return q, pr, c0
 
+    def write_instant_vars_tg(self, iterg: int, phytime: float, dt: float, filepath: str = None):
+        """WRT_INSTANT_VARS_TG of the device state: the four reference-named
+        files DNS_perixz_INSTANT_T<ES15.9 time>_{U,V,W,P}.D under filepath
+        (default filepath1 = '1_instant_D/').  Returns their paths."""
+        from . import fields
+        q, pr, _ = self.download_state()
+        return fields.write_instant_vars_tg(q, pr, iterg, phytime, self.grid.ren, dt,
+                                            fields.INSTANT_DIR if filepath is None else filepath)
+
+    def restart_instant_vars_tg(self, phytime: float, filepath: str = None):
+        """RESTART_INSTANT_VARS_TG(phytime) into the device state: Q and PR from
+        the four files with the wall ghost rows of BC_WALL_Q_tg/BC_WALL_PR_tg,
+        CONVH0 zero (the NS = 1 substep does not read it).  Returns (ITERG,
+        phyTIME) of the files."""
+        from . import fields
+        g = self.grid
+        q, pr, iterg, t = fields.restart_instant_vars_tg(g, phytime, fields.INSTANT_DIR if filepath is None
+                                                         else filepath)
+        self.upload_state(q, pr, g.zeros(g.shape_convh0))
+        return iterg, t
+
     def device_arrays(self):
         ptrs = [C.c_void_p() for _ in range(5)]
         nat.check(self.lib.chs_device_arrays(self.h, *[C.byref(p) for p in ptrs]))
@@ -123,6 +144,21 @@ class ChannelSolver:
         v = C.c_double()
         nat.check(self.lib.chs_cfl(self.h, C.byref(v)))
         return v.value
+
+    def mean_t(self, domain: str = "tg", tp: int = 1, pp_instn_sz: int = 1):
+        """PP_MEAN_T_TG ("tg") or PP_MEAN_T_Xperiodic_IO ("io"): the last xz
+        moments (mean_xz / HeatedChannel.mean_xz_io) into the time means.
+        Returns (time means (NCL2, 13 or 37), NSTATIS)."""
+        d = {"tg": 0, "io": 1}[domain]
+        out = np.zeros((13 if d == 0 else 37) * self.grid.M)
+        ns = C.c_int()
+        nat.check(self.lib.chs_mean_t(self.h, d, int(tp), int(pp_instn_sz), C.byref(ns),
+                                      out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(self.grid.M, -1), ns.value
+
+    def mean_t_reset(self, domain: str = "tg"):
+        """PP_TMEAN_INI_ZERO_TG / _IO: NSTATIS = 0, time means zero."""
+        nat.check(self.lib.chs_mean_t_reset(self.h, {"tg": 0, "io": 1}[domain]))
 
     def mean_xz(self):
         """xz-plane means of the device state, PP_MEAN_ZX_FLOW_TG

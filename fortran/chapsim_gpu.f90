@@ -17,6 +17,18 @@
 !>     CALL CHS_PUT_STATE(Q_tg, PR_tg, CONVH0_tg)
 !>     CALL CHS_SOLVERRK3_MOM_TG(NS, DT)   in place of SOLVERRK3_MOM_tg(NS)
 !>     CALL CHS_GET_STATE(Q_tg, PR_tg, CONVH0_tg)   before output / restart
+!>     CALL CHS_SET_CVISC(CVISC)           at the REINI -> REN switch (src/SOLVE.f90:97-100)
+!>     CALL CHS_FFT99_POIS3D(RHSLLPHI, DPH)  FFT99_POIS3D_periodicxz on host arrays
+!> the property table and the heated channel (device-resident fields):
+!>     CALL CHS_PROPS_INIT(N, H, T, D, M, K, CP)   after NIST_TABLE_SCALING
+!>     CALL CHS_THERM_PROP_UPDATE(ENTHALPY, TEMPERATURE, DENSITY, VISCOUSITY, THERMCONDT)
+!>     CALL CHS_HEATED_PUT(...) / CHS_HEATED_GET(...)
+!>     CALL CHS_SOLVERRK3_ENG_IO(NS, DT, CTHECD)    SOLVERRK3_ENG_IO(NS)
+!>     CALL CHS_IO_PREPARE(NS)                      DENSITY_TIME_DERIVATION .. VELOCITY_CALC_io
+!>     CALL CHS_SOLVERRK3_MOM_IO(NS, DT)            SOLVERRK3_MOM_IO(NS)
+!>     CALL CHS_DIVGCK_IO(DIVG, DIVQ)               DIVGCK_io
+!> fortran/chapsim_dropin.f90 holds the external subroutines under the
+!> reference's own names built on these.
 module chapsim_gpu
     use iso_c_binding
     implicit none
@@ -29,6 +41,23 @@ module chapsim_gpu
         real(c_double) :: dxi, dzi, dxqi, dzqi, cvisc, volm
         type(c_ptr) :: dyfi, dyci, wff, wfb, amph, acph, apph, amvr, acvr, apvr, ak1, ak3
     end type
+
+    ! chs_thermal_fields, chs_io_fields, chs_io_params of include/chapsim_gpu.h
+    type, bind(c) :: chs_thermal_fields
+        type(c_ptr) :: rhoh, enthalpy, temperature, density, density0, density1
+        type(c_ptr) :: viscosity, viscosity0, thermcondt, rhs_energy0, g, g0, wallflux
+    end type
+    type, bind(c) :: chs_io_fields
+        type(c_ptr) :: q, g, g0, pr, drhodtp, divu, rho_stg, mu_stg, explt0, drhoi_stg
+    end type
+    type, bind(c) :: chs_io_params
+        integer(c_int) :: thermal, nflow, flowdrvtp, gravdir
+        real(c_double) :: cfgv, f_a
+        integer(c_int) :: visimplicit
+    end type
+
+    ! HIP runtime (libamdhip64): device mirrors of host arrays
+    integer(c_int), parameter :: HIP_H2D = 1, HIP_D2H = 2
 
     interface
         integer(c_int) function c_create(desc, stream, h) bind(c, name='chs_create')
@@ -75,13 +104,104 @@ module chapsim_gpu
         type(c_ptr) function c_last_error() bind(c, name='chs_last_error')
             import :: c_ptr
         end function
+        integer(c_int) function c_set_cvisc(h, cvisc) bind(c, name='chs_set_cvisc')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: cvisc
+        end function
+        integer(c_int) function c_device_arrays(h, q, pr, convh0, dph, rhsllphi) bind(c, name='chs_device_arrays')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            type(c_ptr), intent(out) :: q, pr, convh0, dph, rhsllphi
+        end function
+        integer(c_int) function c_poisson(h, f, dph) bind(c, name='chs_poisson_fft99')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h, f, dph
+        end function
+        integer(c_int) function c_props_create(n, hh, t, d, m, k, cp, stream, p) bind(c, name='chs_props_create')
+            import :: c_int, c_ptr
+            integer(c_int), value :: n
+            type(c_ptr), value :: hh, t, d, m, k, cp, stream
+            type(c_ptr), intent(out) :: p
+        end function
+        integer(c_int) function c_props_destroy(p) bind(c, name='chs_props_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: p
+        end function
+        integer(c_int) function c_prop_update(p, mode, h, nrow, rowlen, stride, t, rho, mu, kap, cp, nfb) &
+                bind(c, name='chs_therm_prop_update')
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: p
+            integer(c_int), value :: mode
+            type(c_ptr), value :: h
+            integer(c_long_long), value :: nrow, rowlen, stride
+            type(c_ptr), value :: t, rho, mu, kap, cp, nfb
+        end function
+        integer(c_int) function c_energy(h, p, tf, ns, dt, cthecd) bind(c, name='chs_energy_substep')
+            import :: c_int, c_ptr, c_double, chs_thermal_fields
+            type(c_ptr), value :: h, p
+            type(chs_thermal_fields), intent(in) :: tf
+            integer(c_int), value :: ns
+            real(c_double), value :: dt, cthecd
+        end function
+        integer(c_int) function c_io_prepare(h, tf, f, prm, ns) bind(c, name='chs_io_prepare')
+            import :: c_int, c_ptr, chs_thermal_fields, chs_io_fields, chs_io_params
+            type(c_ptr), value :: h
+            type(chs_thermal_fields), intent(in) :: tf
+            type(chs_io_fields), intent(in) :: f
+            type(chs_io_params), intent(in) :: prm
+            integer(c_int), value :: ns
+        end function
+        integer(c_int) function c_mom_io(h, tf, f, prm, ns, dt) bind(c, name='chs_solverrk3_mom_io')
+            import :: c_int, c_ptr, c_double, chs_thermal_fields, chs_io_fields, chs_io_params
+            type(c_ptr), value :: h
+            type(chs_thermal_fields), intent(in) :: tf
+            type(chs_io_fields), intent(in) :: f
+            type(chs_io_params), intent(in) :: prm
+            integer(c_int), value :: ns
+            real(c_double), value :: dt
+        end function
+        integer(c_int) function c_divgck_io(h, f, dg, dq) bind(c, name='chs_divgck_io')
+            import :: c_int, c_ptr, c_double, chs_io_fields
+            type(c_ptr), value :: h
+            type(chs_io_fields), intent(in) :: f
+            real(c_double), intent(out) :: dg, dq
+        end function
+        integer(c_int) function hip_malloc(p, n) bind(c, name='hipMalloc')
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), intent(out) :: p
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function hip_free(p) bind(c, name='hipFree')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: p
+        end function
+        integer(c_int) function hip_memcpy(dst, src, n, kind) bind(c, name='hipMemcpy')
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: dst, src
+            integer(c_size_t), value :: n
+            integer(c_int), value :: kind
+        end function
+        integer(c_int) function hip_sync() bind(c, name='hipDeviceSynchronize')
+            import :: c_int
+        end function
     end interface
 
     type(c_ptr), save :: handle = c_null_ptr
+    type(c_ptr), save :: props = c_null_ptr
+    integer, save :: gl = 0, gm = 0, gn = 0          ! NCL1, NCL2, NCL3 of the handle
+    ! device mirrors of the heated channel's fields (CHS_HEATED_INIT)
+    integer, parameter :: NHEAT = 21
+    type(c_ptr), save :: dheat(NHEAT) = c_null_ptr
+    integer(c_size_t), save :: heat_len(NHEAT) = 0
+    type(chs_io_params), save :: ioprm
+    type(c_ptr), save :: dwork = c_null_ptr          ! one ghost-aware field for CHS_THERM_PROP_UPDATE
 
     public :: chs_init, chs_finalize, chs_put_state, chs_get_state, chs_solverrk3_mom_tg, &
               chs_divgck_tg, chs_cfl_tg, chs_tdmaijj_noncyc, chs_tdmaiji_noncyc, &
-              chs_tdmaijj_cyc, chs_tdmaiji_cyc
+              chs_tdmaijj_cyc, chs_tdmaiji_cyc, chs_set_cvisc, chs_fft99_pois3d, &
+              chs_props_init, chs_therm_prop_update, chs_heated_init, chs_heated_put, chs_heated_get, &
+              chs_solverrk3_eng_io, chs_io_prepare, chs_solverrk3_mom_io, chs_divgck_io
 
 contains
 
@@ -119,11 +239,210 @@ contains
         d%amvr = c_loc(amvr); d%acvr = c_loc(acvr); d%apvr = c_loc(apvr)
         d%ak1 = c_loc(ak1); d%ak3 = c_loc(ak3)
         call chs_check(c_create(d, c_null_ptr, handle), 'chs_init')
+        gl = ncl1; gm = ncl2; gn = ncl3
     end subroutine
 
     subroutine chs_finalize()
+        integer :: i
+        do i = 1, NHEAT
+            if (c_associated(dheat(i))) call hip_check(hip_free(dheat(i)), 'chs_finalize')
+            dheat(i) = c_null_ptr
+        end do
+        if (c_associated(dwork)) call hip_check(hip_free(dwork), 'chs_finalize')
+        dwork = c_null_ptr
+        if (c_associated(props)) call chs_check(c_props_destroy(props), 'chs_finalize')
+        props = c_null_ptr
         call chs_check(c_destroy(handle), 'chs_finalize')
         handle = c_null_ptr
+    end subroutine
+
+    subroutine hip_check(st, where)
+        integer(c_int), intent(in) :: st
+        character(*), intent(in) :: where
+        if (st == 0) return
+        write(*, '(A,A,I4)') where, ': HIP runtime error ', st
+        error stop 1
+    end subroutine
+
+    !> CVISC = 1/REN from here on (src/SOLVE.f90:97-100, :167)
+    subroutine chs_set_cvisc(cvisc)
+        real(c_double), intent(in) :: cvisc
+        call chs_check(c_set_cvisc(handle, cvisc), 'chs_set_cvisc')
+    end subroutine
+
+    !> FFT99_POIS3D_periodicxz (src/possion3d_FFT99.f90:254) on host arrays
+    !> RHSLLPHI(NCL1,NCL2,NCL3) -> DPH(NCL1,0:NCL2+1,NCL3), through the
+    !> handle's own device RHSLLPHI and DPH arrays.
+    subroutine chs_fft99_pois3d(rhsllphi, dph)
+        real(c_double), target, contiguous, intent(in) :: rhsllphi(:, :, :)
+        real(c_double), target, contiguous, intent(inout) :: dph(:, :, :)
+        type(c_ptr) :: dq, dp, dc, ddph, drhs
+        if (size(rhsllphi) /= gl * gm * gn .or. size(dph) /= gl * (gm + 2) * gn) then
+            write(*, '(A)') 'chs_fft99_pois3d: array sizes do not match the handle'
+            error stop 1
+        end if
+        call chs_check(c_device_arrays(handle, dq, dp, dc, ddph, drhs), 'chs_fft99_pois3d')
+        call hip_check(hip_memcpy(drhs, c_loc(rhsllphi), 8_c_size_t * size(rhsllphi, kind=c_size_t), HIP_H2D), &
+                       'chs_fft99_pois3d')
+        call chs_check(c_poisson(handle, drhs, ddph), 'chs_fft99_pois3d')
+        call hip_check(hip_memcpy(c_loc(dph), ddph, 8_c_size_t * size(dph, kind=c_size_t), HIP_D2H), 'chs_fft99_pois3d')
+    end subroutine
+
+    !> The scaled property table (after NIST_TABLE_SCALING, src/Flow_State.f90:343-440):
+    !> N knots sorted by H, columns H, T, D, M, K, CP.
+    subroutine chs_props_init(n, h, t, d, m, k, cp)
+        integer, intent(in) :: n
+        real(c_double), target, intent(in) :: h(n), t(n), d(n), m(n), k(n), cp(n)
+        if (c_associated(props)) call chs_check(c_props_destroy(props), 'chs_props_init')
+        call chs_check(c_props_create(int(n, c_int), c_loc(h), c_loc(t), c_loc(d), c_loc(m), c_loc(k), c_loc(cp), &
+                       c_null_ptr, props), 'chs_props_init')
+    end subroutine
+
+    !> THERM_PROP_UPDATE(IMAIND) (src/THERM_PROP_UPDATE.f90:129-205) on host
+    !> arrays (NCL1,0:NCL2+1,NCL3): rows J = 1..NCL2 of TEMPERATURE, DENSITY,
+    !> VISCOUSITY, THERMCONDT from ENTHALPY by the table; |h| < 1e-10 cells and
+    !> the ghost rows are left as they are.
+    subroutine chs_therm_prop_update(enthalpy, temperature, density, viscosity, thermcondt)
+        real(c_double), target, contiguous, intent(in) :: enthalpy(:, :, :)
+        real(c_double), target, contiguous, intent(inout) :: temperature(:, :, :), density(:, :, :)
+        real(c_double), target, contiguous, intent(inout) :: viscosity(:, :, :), thermcondt(:, :, :)
+        integer(c_size_t) :: nb, off
+        if (size(enthalpy) /= gl * (gm + 2) * gn) then
+            write(*, '(A)') 'chs_therm_prop_update: array size does not match the handle'
+            error stop 1
+        end if
+        nb = 8_c_size_t * size(enthalpy, kind=c_size_t)
+        if (.not. c_associated(dwork)) call hip_check(hip_malloc(dwork, 5 * nb), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(dwork, c_loc(enthalpy), nb, HIP_H2D), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(dev_at(dwork, nb), c_loc(temperature), nb, HIP_H2D), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(dev_at(dwork, 2 * nb), c_loc(density), nb, HIP_H2D), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(dev_at(dwork, 3 * nb), c_loc(viscosity), nb, HIP_H2D), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(dev_at(dwork, 4 * nb), c_loc(thermcondt), nb, HIP_H2D), 'chs_therm_prop_update')
+        off = 8_c_size_t * gl                       ! element (1,1,1)
+        call chs_check(c_prop_update(props, 0_c_int, dev_at(dwork, off), int(gn, c_long_long), &
+                       int(gl, c_long_long) * gm, int(gl, c_long_long) * (gm + 2), dev_at(dwork, nb + off), &
+                       dev_at(dwork, 2 * nb + off), dev_at(dwork, 3 * nb + off), dev_at(dwork, 4 * nb + off), &
+                       c_null_ptr, c_null_ptr), 'THERM_PROP_UPDATE')
+        call hip_check(hip_memcpy(c_loc(temperature), dev_at(dwork, nb), nb, HIP_D2H), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(c_loc(density), dev_at(dwork, 2 * nb), nb, HIP_D2H), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(c_loc(viscosity), dev_at(dwork, 3 * nb), nb, HIP_D2H), 'chs_therm_prop_update')
+        call hip_check(hip_memcpy(c_loc(thermcondt), dev_at(dwork, 4 * nb), nb, HIP_D2H), 'chs_therm_prop_update')
+    end subroutine
+
+    !> byte offset off into the device allocation p
+    type(c_ptr) function dev_at(p, off)
+        type(c_ptr), intent(in) :: p
+        integer(c_size_t), intent(in) :: off
+        integer(c_intptr_t) :: a
+        a = transfer(p, a)
+        dev_at = transfer(a + int(off, c_intptr_t), dev_at)
+    end function
+
+    !> Device mirrors of the heated channel's fields, in the order of
+    !> CHS_HEATED_PUT; params as chs_io_params (include/chapsim_gpu.h).
+    subroutine chs_heated_init(thermal, nflow, flowdrvtp, gravdir, cfgv, f_a, visimplicit)
+        integer, intent(in) :: thermal, nflow, flowdrvtp, gravdir, visimplicit
+        real(c_double), intent(in) :: cfgv, f_a
+        integer(c_size_t) :: ngh, ncell, nwall
+        integer :: i
+        ngh = int(gl, c_size_t) * (gm + 2) * gn
+        ncell = int(gl, c_size_t) * gm * gn
+        nwall = int(gl, c_size_t) * 2
+        ! rhoh h T rho rho0 rho1 mu mu0 kap rhs0 g g0 wallflux q pr drhodtp divu rho_stg mu_stg explt0 drhoi_stg
+        heat_len = [ngh, ngh, ngh, ngh, ngh, ngh, ngh, ngh, ngh, ncell, 3 * ngh, 3 * ngh, nwall, 3 * ngh, ngh, ngh, &
+                 ngh, 3 * ngh, 3 * ngh, 3 * ncell, 3 * ngh]
+        do i = 1, NHEAT
+            if (.not. c_associated(dheat(i))) call hip_check(hip_malloc(dheat(i), 8_c_size_t * heat_len(i)), &
+                                                             'chs_heated_init')
+        end do
+        ioprm%thermal = thermal; ioprm%nflow = nflow; ioprm%flowdrvtp = flowdrvtp; ioprm%gravdir = gravdir
+        ioprm%cfgv = cfgv; ioprm%f_a = f_a; ioprm%visimplicit = visimplicit
+    end subroutine
+
+    subroutine heat_copy(i, a, dir)
+        integer, intent(in) :: i, dir
+        type(c_ptr), intent(in) :: a
+        if (dir == HIP_H2D) then
+            call hip_check(hip_memcpy(dheat(i), a, 8_c_size_t * heat_len(i), HIP_H2D), 'chs_heated_put')
+        else
+            call hip_check(hip_memcpy(a, dheat(i), 8_c_size_t * heat_len(i), HIP_D2H), 'chs_heated_get')
+        end if
+    end subroutine
+
+    !> Host -> device: RHOH, ENTHALPY, TEMPERATURE, DENSITY, DENSITY0, DENSITY1,
+    !> VISCOUSITY, VISCOUSITY0, THERMCONDT (NCL1,0:NCL2+1,NCL3), RHS_ENERGY0
+    !> (NCL1,NCL2,NCL3), G_io, G0_io (NCL1,0:NCL2+1,NCL3,3), WALLFLUX (NCL1,2),
+    !> Q_io (…,3), PR_io, DrhoDtP, DivU_io, RHO_STG (…,3), MU_STG (…,3),
+    !> EXPLT0_io (NCL1,NCL2,NCL3,3), DRHOI_STG (…,3).
+    subroutine chs_heated_put(rhoh, h, t, rho, rho0, rho1, mu, mu0, kap, rhs0, g, g0, wallflux, q, pr, drhodtp, &
+                              divu, rho_stg, mu_stg, explt0, drhoi_stg)
+        real(c_double), target, contiguous, intent(in) :: rhoh(:, :, :), h(:, :, :), t(:, :, :), rho(:, :, :), &
+            rho0(:, :, :), rho1(:, :, :), mu(:, :, :), mu0(:, :, :), kap(:, :, :), rhs0(:, :, :), g(:, :, :, :), &
+            g0(:, :, :, :), wallflux(:, :), q(:, :, :, :), pr(:, :, :), drhodtp(:, :, :), divu(:, :, :), &
+            rho_stg(:, :, :, :), mu_stg(:, :, :, :), explt0(:, :, :, :), drhoi_stg(:, :, :, :)
+        type(c_ptr) :: a(NHEAT)
+        integer :: i
+        a = [c_loc(rhoh), c_loc(h), c_loc(t), c_loc(rho), c_loc(rho0), c_loc(rho1), c_loc(mu), c_loc(mu0), &
+             c_loc(kap), c_loc(rhs0), c_loc(g), c_loc(g0), c_loc(wallflux), c_loc(q), c_loc(pr), c_loc(drhodtp), &
+             c_loc(divu), c_loc(rho_stg), c_loc(mu_stg), c_loc(explt0), c_loc(drhoi_stg)]
+        do i = 1, NHEAT
+            call heat_copy(i, a(i), HIP_H2D)
+        end do
+    end subroutine
+
+    !> Device -> host, the same arrays in the same order.
+    subroutine chs_heated_get(rhoh, h, t, rho, rho0, rho1, mu, mu0, kap, rhs0, g, g0, wallflux, q, pr, drhodtp, &
+                              divu, rho_stg, mu_stg, explt0, drhoi_stg)
+        real(c_double), target, contiguous, intent(inout) :: rhoh(:, :, :), h(:, :, :), t(:, :, :), rho(:, :, :), &
+            rho0(:, :, :), rho1(:, :, :), mu(:, :, :), mu0(:, :, :), kap(:, :, :), rhs0(:, :, :), g(:, :, :, :), &
+            g0(:, :, :, :), wallflux(:, :), q(:, :, :, :), pr(:, :, :), drhodtp(:, :, :), divu(:, :, :), &
+            rho_stg(:, :, :, :), mu_stg(:, :, :, :), explt0(:, :, :, :), drhoi_stg(:, :, :, :)
+        type(c_ptr) :: a(NHEAT)
+        integer :: i
+        a = [c_loc(rhoh), c_loc(h), c_loc(t), c_loc(rho), c_loc(rho0), c_loc(rho1), c_loc(mu), c_loc(mu0), &
+             c_loc(kap), c_loc(rhs0), c_loc(g), c_loc(g0), c_loc(wallflux), c_loc(q), c_loc(pr), c_loc(drhodtp), &
+             c_loc(divu), c_loc(rho_stg), c_loc(mu_stg), c_loc(explt0), c_loc(drhoi_stg)]
+        call hip_check(hip_sync(), 'chs_heated_get')
+        do i = 1, NHEAT
+            call heat_copy(i, a(i), HIP_D2H)
+        end do
+    end subroutine
+
+    type(chs_thermal_fields) function heat_tf()
+        heat_tf = chs_thermal_fields(dheat(1), dheat(2), dheat(3), dheat(4), dheat(5), dheat(6), dheat(7), &
+                                     dheat(8), dheat(9), dheat(10), dheat(11), dheat(12), dheat(13))
+    end function
+
+    type(chs_io_fields) function heat_iof()
+        heat_iof = chs_io_fields(dheat(14), dheat(11), dheat(12), dheat(15), dheat(16), dheat(17), dheat(18), &
+                                 dheat(19), dheat(20), dheat(21))
+    end function
+
+    !> SOLVERRK3_ENG_IO(NS) (src/SOLVERRK3_ENG_IO.f90:1) on the device mirrors
+    subroutine chs_solverrk3_eng_io(ns, dt, cthecd)
+        integer, intent(in) :: ns
+        real(c_double), intent(in) :: dt, cthecd
+        call chs_check(c_energy(handle, props, heat_tf(), int(ns, c_int), dt, cthecd), 'SOLVERRK3_ENG_IO')
+    end subroutine
+
+    !> DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered, VELOCITY_CALC_io
+    !> (src/SOLVE.f90:121-133)
+    subroutine chs_io_prepare(ns)
+        integer, intent(in) :: ns
+        call chs_check(c_io_prepare(handle, heat_tf(), heat_iof(), ioprm, int(ns, c_int)), 'chs_io_prepare')
+    end subroutine
+
+    !> SOLVERRK3_MOM_IO(NS) (src/SOLVERRK3_MOM_IO.f90:1) on the device mirrors
+    subroutine chs_solverrk3_mom_io(ns, dt)
+        integer, intent(in) :: ns
+        real(c_double), intent(in) :: dt
+        call chs_check(c_mom_io(handle, heat_tf(), heat_iof(), ioprm, int(ns, c_int), dt), 'SOLVERRK3_MOM_IO')
+    end subroutine
+
+    !> DIVGCK_io (src/DIVGCK.f90:61): max|div G + DrhoDtP|, max|div Q|
+    subroutine chs_divgck_io(divg, divq)
+        real(c_double), intent(out) :: divg, divq
+        call chs_check(c_divgck_io(handle, heat_iof(), divg, divq), 'DIVGCK_io')
     end subroutine
 
     !> Q_tg(NCL1,0:NCL2+1,NCL3,3), PR_tg(NCL1,0:NCL2+1,NCL3), CONVH0_tg(NCL1,NCL2,NCL3,3)

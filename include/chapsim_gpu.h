@@ -126,6 +126,20 @@ int chs_cfl(chs_handle *h, double *rate_host);
  * out is host memory of 13*NCL2 doubles.  One block per row sums in a fixed
  * order: the result is deterministic. */
 int chs_mean_xz(chs_handle *h, double *out_host);
+/* Time means of the xz-plane moments: PP_MEAN_T_TG (domain 0) and
+ * PP_MEAN_T_Xperiodic_IO (domain 1) (src/processing/PP_MEAN_T.f90:322-370,
+ * 436-470).  Takes the moments of the last chs_mean_xz (domain 0) or
+ * chs_mean_xz_io (domain 1) call, which stay on the device, into the time
+ * means: NSTATIS += 1, then mean = COE3 mean + COE2 xz with
+ * (COE2, COE3) = (1/NSTATIS, (NSTATIS-1)/NSTATIS) for tp = 1
+ * (T_Asymptotic_Average) or (1/pp_instn_sz, 1) for tp = 2 (T_Summing_average).
+ * *nstatis (may be NULL) receives NSTATIS; out_host (may be NULL) the time
+ * means, 13*NCL2 (domain 0) or 37*NCL2 (domain 1) doubles in the column order
+ * of the xz means. */
+int chs_mean_t(chs_handle *h, int domain, int tp, int pp_instn_sz, int *nstatis, double *out_host);
+/* PP_TMEAN_INI_ZERO_TG / _IO (src/processing/PP_MEAN_T.f90:29-54, 110-319):
+ * NSTATIS = 0 and the time means of the domain set to zero. */
+int chs_mean_t_reset(chs_handle *h, int domain);
 /* DPGRNS of the last substep (constant-mass-flux correction). */
 int chs_last_dpgrns(chs_handle *h, double *dpgrns_host);
 
@@ -272,6 +286,26 @@ int chs_solverrk3_mom_io(chs_handle *h, const chs_thermal_fields *tf, const chs_
 /* DIVGCK_io (src/DIVGCK.f90:61-124): max |div G + DrhoDtP| and max |div Q|
  * over the interior (NaN propagates).  Either output may be NULL. */
 int chs_divgck_io(chs_handle *h, const chs_io_fields *f, double *maxdiv_g, double *maxdiv_q);
+
+/* xz-plane moments of the heated channel, PP_MEAN_ZX_FLOW_Xperiodic_IO
+ * (src/processing/PP_MEAN_ZX.f90:299-800), from Q_io, G_io, PR_io and (thermal
+ * = 1) TEMPERATURE, DENSITY, ENTHALPY, VISCOUSITY.  With the cell-centred
+ * u_m = (Q(i)+Q(i+1))/2 (and likewise g_m = (G(i)+G(i+1))/2, v and w along j
+ * and k), out[37 j + m] for row j = 0..NCL2-1 holds
+ *   m = 0..3    <u>, <v>, <w>, <p>          U1xzL_io(J,1:4)
+ *   m = 4..6    <g_m>                       G1xzL_io
+ *   m = 7..9    <u_m p>                     UPxzL_io
+ *   m = 10..15  <u_m u_n>, (m,n) = 11 12 13 22 23 33   U2xzL_IO
+ *   m = 16..21  <u_m g_n>, same order       UGxzL_IO
+ *   m = 22..30  <T> <rho> <h> <mu> <T^2> <rho^2> <h^2> <h rho> <h p>
+ *               T1, D1, H1, M1, T2, D2, H2, DH, PHxzL_io
+ *   m = 31..33  <h u_m>                     UHxzL_io
+ *   m = 34..36  <h g_m>                     GHxzL_io
+ * (columns 22..36 are zero with thermal = 0).  out_host (37*NCL2 doubles)
+ * may be NULL: the moments then only stay on the device for chs_mean_t.
+ * One block per row sums in a fixed order: deterministic. */
+int chs_mean_xz_io(chs_handle *h, const chs_thermal_fields *tf, const chs_io_fields *f, int thermal,
+                   double *out_host);
 
 /* ---- learned property surrogate ------------------------------------------
  * The reference has no surrogate: these entries train and certify the network

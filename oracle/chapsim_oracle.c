@@ -1142,6 +1142,16 @@ int orc_num_threads(void)
 #endif
 }
 
+/* threads of the following parallel regions (the timed CPU baseline's 1-thread figure) */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
 /* ---------------------------------------------------------------------------
  * Energy substep of the heated channel, one rank, periodic x and z, iso-flux
  * walls (BCWALLHEAT = isoFluxWall at both walls), property table lookup:
@@ -2229,3 +2239,85 @@ void orc_divgck_io(const orc_grid *g, const orc_io *s, double *out)
 #undef WIMV
 #undef WKPV
 #undef WKMV
+
+/* ------------------------------------------------------------------------
+ * PP_MEAN_ZX_FLOW_Xperiodic_IO (src/processing/PP_MEAN_ZX.f90:299-800), the
+ * moments of the channel (ICASE = 1, RCCI1 = 1) that the heated channel's
+ * acceptance profiles use, per row J = 1..M of the io arrays (L, 0:M+1, N[,3]):
+ * cell-centred U_cct, G_cct (:379-413), then the sums of :419-437 and
+ * :655-667, :735-739, each times VL1313_io = 1/(L N) (:772-795).
+ * out[37 (J-1) + m]:
+ *   0..3   U1xzL_io(J,1:4)    <u>, <v>, <w>, <p>
+ *   4..6   G1xzL_io(J,1:3)    <g_m>
+ *   7..9   UPxzL_io(J,1:3)    <u_m p>
+ *   10..15 U2xzL_IO(J,1:6)    <u_m u_n>, (m,n) = 11 12 13 22 23 33 (L = (M(7-M))/2+N-3)
+ *   16..21 UGxzL_IO(J,1:6)    <u_m g_n>, same order
+ *   22..30 T1 D1 H1 M1 T2 D2 H2 DH PH   (T, rho, h, mu, T^2, rho^2, h^2, h rho, h p)
+ *   31..33 UHxzL_io(J,1:3)    <h u_m>
+ *   34..36 GHxzL_io(J,1:3)    <h g_m>
+ * thermal = 0 leaves 22..36 at zero (the reference skips them).
+ * ------------------------------------------------------------------------ */
+#define ML (g->L)
+#define MM (g->M)
+#define MN (g->N)
+#define M3(arr, i, j, k) (arr)[(size_t)((i)-1) + (size_t)ML * ((size_t)(j) + (size_t)(MM + 2) * ((k)-1))]
+#define MG(arr, i, j, k, c) (arr)[(size_t)((i)-1) + (size_t)ML * ((size_t)(j) + (size_t)(MM + 2) * ((size_t)((k)-1) + (size_t)MN * ((c)-1)))]
+void orc_mean_xz_io(const orc_grid *g, const double *q, const double *gf, const double *pr, const double *T,
+                    const double *rho, const double *h, const double *mu, int thermal, double *out)
+{
+    const double VL = 1.0 / (double)(ML * MN);
+#pragma omp parallel for schedule(static)
+    for (int J = 1; J <= MM; ++J) {
+        double acc[37];
+        for (int m = 0; m < 37; ++m) acc[m] = 0.0;
+        const int JP = J + 1;
+        for (int I = 1; I <= ML; ++I) {
+            const int IP = I == ML ? 1 : I + 1;
+            for (int K = 1; K <= MN; ++K) {
+                const int KP = K == MN ? 1 : K + 1;
+                double U[3], G[3];
+                G[0] = (MG(gf, I, J, K, 1) + MG(gf, IP, J, K, 1)) * 0.5;
+                U[0] = (MG(q, I, J, K, 1) + MG(q, IP, J, K, 1)) * 0.5;
+                G[2] = ((MG(gf, I, J, K, 3) + MG(gf, I, J, KP, 3)) * 1.0) * 0.5;
+                U[2] = ((MG(q, I, J, K, 3) + MG(q, I, J, KP, 3)) * 1.0) * 0.5;
+                G[1] = ((MG(gf, I, J, K, 2) + MG(gf, I, JP, K, 2)) * 1.0) * 0.5;
+                U[1] = ((MG(q, I, J, K, 2) + MG(q, I, JP, K, 2)) * 1.0) * 0.5;
+                const double P = M3(pr, I, J, K);
+                for (int m = 0; m < 3; ++m) {
+                    acc[m] += U[m];
+                    acc[4 + m] += G[m];
+                    acc[7 + m] += U[m] * P;
+                }
+                acc[3] += P;
+                for (int m = 0; m < 3; ++m)
+                    for (int n = m; n < 3; ++n) {
+                        const int l = ((m + 1) * (7 - (m + 1))) / 2 + (n + 1) - 3 - 1;
+                        acc[16 + l] += U[m] * G[n];
+                        acc[10 + l] += U[m] * U[n];
+                    }
+                if (thermal) {
+                    const double t = M3(T, I, J, K), d = M3(rho, I, J, K), e = M3(h, I, J, K), v = M3(mu, I, J, K);
+                    acc[22] += t;
+                    acc[23] += d;
+                    acc[24] += e;
+                    acc[25] += v;
+                    acc[26] += t * t;
+                    acc[27] += d * d;
+                    acc[28] += e * e;
+                    acc[29] += e * d;
+                    acc[30] += e * P;
+                    for (int m = 0; m < 3; ++m) {
+                        acc[31 + m] += e * U[m];
+                        acc[34 + m] += e * G[m];
+                    }
+                }
+            }
+        }
+        for (int m = 0; m < 37; ++m) out[37 * (J - 1) + m] = acc[m] * VL;
+    }
+}
+#undef ML
+#undef MM
+#undef MN
+#undef M3
+#undef MG

@@ -97,6 +97,7 @@ def lib():
                                          C.c_double]
         L.orc_energy_substep.restype = C.c_long
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
         for name in ("orc_io_prepare", "orc_io_substep"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int] + \
                 ([C.c_double] if name == "orc_io_substep" else [])
@@ -423,6 +424,11 @@ def num_threads() -> int:
     return lib().orc_num_threads()
 
 
+def set_num_threads(n: int) -> None:
+    """OpenMP threads of the oracle's parallel loops from here on."""
+    lib().orc_set_num_threads(int(n))
+
+
 # ---- reference FFT99 package (Fortran, by-reference arguments) -----------
 def ref_fft99_transform(data, n, isign):
     """FFT99 on LOT vectors stored as columns of data (n+2, LOT), INC=1, JUMP=n+2."""
@@ -482,3 +488,30 @@ def ref_tdma_solve(a, b, c, r, bc, march, cyclic, lo=(1, 1)):
             BC = f(bc if bc is not None else np.zeros((jsz, 2)))
             R.tdmaiji_noncyc_(_ptr(A), _ptr(B), _ptr(Cc), _ptr(X), _ptr(BC), ip(IS), ip(isz), ip(JS), ip(jsz))
     return X
+
+
+def mean_xz_io(og: "OracleGrid", st: dict, thermal: int = 1):
+    """PP_MEAN_ZX_FLOW_Xperiodic_IO moments of the io state dict (keys q, g,
+    pr, T, rho, h, mu in the reference layouts): (M, 37), columns as
+    orc_mean_xz_io documents them."""
+    out = np.zeros(37 * og.M)
+    f = lambda k: _ptr(np.asfortranarray(st[k], dtype=np.float64))
+    keep = {k: np.asfortranarray(st[k], dtype=np.float64) for k in ("q", "g", "pr", "T", "rho", "h", "mu")}
+    lib().orc_mean_xz_io(C.byref(og.g), *[_ptr(keep[k]) for k in ("q", "g", "pr", "T", "rho", "h", "mu")],
+                         int(thermal), _ptr(out))
+    return out.reshape(og.M, 37)
+
+
+def mean_t_update(acc, val, nstatis, tp=1, pp_instn_sz=1):
+    """PP_MEAN_T_TG / PP_MEAN_T_Xperiodic_IO (src/processing/PP_MEAN_T.f90:322-370,
+    436-470): NSTATIS += 1, then acc = COE3 acc + COE2 val with
+    (COE2, COE3) = (1/NSTATIS, (NSTATIS-1)/NSTATIS) for TP = 1 (asymptotic)
+    or (1/pp_instn_sz, 1) for TP = 2 (summing).  Returns (acc, NSTATIS)."""
+    nstatis += 1
+    if tp == 1:
+        coe2, coe3 = 1.0 / float(nstatis), float(nstatis - 1) / float(nstatis)
+    elif tp == 2:
+        coe2, coe3 = 1.0 / float(pp_instn_sz), 1.0
+    else:
+        raise ValueError("TP must be 1 or 2")
+    return coe3 * acc + coe2 * val, nstatis

@@ -1,0 +1,129 @@
+"""The measured configurations at their own sizes, through the C ABI:
+
+* the mixed-precision channel's grid, 256x192x256 at Re_b = 6880 (seeded
+  laminar + noise start, dt = 0.5/CFL): the FP64 substep against the oracle
+  for NS = 0, 1, 2, 3 (every field within 1e-12 after every substep), and the
+  mixed-precision substep beside it (max|div u| <= 1e-10 after every substep;
+  its distance from the FP64 form stated and bounded);
+* the variable-property heated channel at 128^3 (2048-point synthetic table,
+  q_w = 0.1, Re = 2800, Pr = 1): energy + staging + io momentum substeps
+  against the oracle, explicit and RK-CN viscous terms.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from chapsim1ww_amd import fields, props
+from chapsim1ww_amd.grid import ChannelGrid
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _moments_err(a, b, cols):
+    sc = np.abs(b[:, cols]).max()
+    return float(np.abs(a[:, cols] - b[:, cols]).max() / sc)
+
+
+def test_channel_256x192x256_re6880_fp64_and_mixed():
+    from chapsim1ww_amd.solver import ChannelSolver
+    dims, ren = (256, 192, 256), 6880.0
+    print(f"channel {dims} Re_b={ren}: oracle, FP64 and mixed forms", flush=True)
+    g = ChannelGrid(*dims, ren=ren)
+    og = O.OracleGrid(*dims, ren=ren)
+    q, pr, c0 = fields.init_tg_fields(g, seed=1973)
+    st = O.OracleState(og, q, pr, c0)
+    dt = 0.5 / st.cfl()
+    s64, smx = ChannelSolver(g, "fp64"), ChannelSolver(g, "mixed")
+    for s in (s64, smx):
+        s.upload_state(q, pr, c0)
+    for ns in (0, 1, 2, 3):
+        st.substep(ns, dt)
+        s64.solverrk3_mom_tg(ns, dt)
+        smx.solverrk3_mom_tg(ns, dt)
+        ref = (st.q, st.pr, st.convh0)
+        e64 = {n: _rel(a, b) for n, a, b in zip(("Q", "PR", "CONVH0"), s64.download_state(), ref)}
+        emx = {n: _rel(a, b) for n, a, b in zip(("Q", "PR", "CONVH0"), smx.download_state(), ref)}
+        d64, dmx = s64.divgck(), smx.divgck()
+        print(ns, "fp64", {k: f"{v:.1e}" for k, v in e64.items()}, "mixed", {k: f"{v:.1e}" for k, v in emx.items()},
+              f"div {d64:.1e} {dmx:.1e}", flush=True)
+        for n, e in e64.items():
+            assert e < FP64_TOL, (ns, n, e)
+        assert abs(s64.last_dpgrns() - st.dpgrns) <= 1e-12 * abs(st.dpgrns) + 1e-18
+        assert d64 <= 1e-10 and dmx <= 1e-10
+        assert emx["Q"] < 1e-10 and emx["CONVH0"] < 1e-10 and emx["PR"] < 1e-6, emx
+    # continue both device forms: the mixed form keeps max|div u| <= 1e-10
+    # after every substep and stays close to the FP64 form
+    seq = [1, 2, 3] * 20
+    for i, ns in enumerate(seq):
+        s64.solverrk3_mom_tg(ns, dt)
+        smx.solverrk3_mom_tg(ns, dt)
+        dmx = smx.divgck()
+        assert dmx <= 1e-10, (i, ns, dmx)
+    (q64, p64, _), (qmx, pmx, _) = s64.download_state(), smx.download_state()
+    errs = {c: _rel(qmx[..., k], q64[..., k]) for k, c in enumerate("uvw")}
+    errs["p"] = _rel(pmx, p64)
+    m64, mmx = s64.mean_xz(), smx.mean_xz()
+    merr = {"u_mean": _moments_err(mmx, m64, [0]), "uu": _moments_err(mmx, m64, list(range(4, 10))),
+            "p_mean": _moments_err(mmx, m64, [3])}
+    print(f"after {1 + 3 + len(seq)} substeps, mixed vs FP64:", {k: f"{v:.1e}" for k, v in {**errs, **merr}.items()},
+          flush=True)
+    assert max(errs["u"], errs["v"], errs["w"]) < 1e-9 and errs["p"] < 1e-6, errs
+    assert merr["u_mean"] <= 1e-6 and merr["uu"] <= 1e-6, merr
+    s64.close()
+    smx.close()
+
+
+MAP = {"rhoh": "rhoh", "h": "enthalpy", "T": "temperature", "rho": "density", "rho0": "density0",
+       "rho1": "density1", "mu": "viscosity", "mu0": "viscosity0", "kap": "thermcondt", "rhs0": "rhs_energy0",
+       "g": "g", "g0": "g0", "wallflux": "wallflux", "q": "q", "pr": "pr", "drhodtp": "drhodtp", "divu": "divu",
+       "rho_stg": "rho_stg", "mu_stg": "mu_stg", "explt0": "explt0", "drhoi_stg": "drhoi_stg"}
+
+
+@pytest.mark.parametrize("visimplicit", [0, 1])
+def test_heated_channel_128cube(visimplicit):
+    """128^3, Re = 2800, Pr = 1 (CTHECD = 1/2800), q_w = 0.1 at both walls,
+    the 2048-point synthetic table: NS = 1, 2, 3 of energy + staging + io
+    momentum against the oracle, every field within 1e-12."""
+    from chapsim1ww_amd.heated import HeatedChannel
+    from chapsim1ww_amd.solver import ChannelSolver
+    dims = (128, 128, 128)
+    print(f"heated channel {dims} visimplicit={visimplicit}", flush=True)
+    scaled = props.scale_table(props.synthetic_table(2048), t0=1.0)
+    g = ChannelGrid(*dims, ren=2800.0)
+    og = O.OracleGrid(*dims, ren=2800.0)
+    st, tab, n = O.io_case(og, scaled, seed=128, qw=0.1)
+    assert np.all(st["wallflux"] == 0.1)
+    s = ChannelSolver(g)
+    pt = props.PropertyTable(scaled)
+    kw = dict(thermal=1, visimplicit=visimplicit)
+    hc = HeatedChannel(s, pt, **kw)
+    hc.upload({MAP[k]: v for k, v in st.items()})
+    io = O.OracleIO(og, st, **kw)
+    cthecd = 1.0 / 2800.0
+    rmax = O.rhoh_max(scaled)
+    dt = 1e-3
+    pr_scale = np.abs(st["pr"]).max()
+    for ns in (1, 2, 3):
+        assert O.energy_substep(og, st, tab, n, ns, dt, cthecd, rmax) == 0
+        io.prepare(ns)
+        io.substep(ns, dt)
+        hc.substep(ns, dt, cthecd)
+        got = hc.download()
+        errs = {k: _rel(got[MAP[k]], st[k]) for k in ("g", "g0", "explt0", "divu", "q", "rho_stg", "mu_stg",
+                                                      "drhodtp", "rho", "T", "h", "rhoh", "mu", "kap")}
+        pr_scale = max(pr_scale, np.abs(st["pr"]).max())
+        errs["pr"] = float(np.abs(got["pr"] - st["pr"]).max() / pr_scale)
+        dg, _ = hc.divgck_io()
+        print(ns, {k: f"{v:.1e}" for k, v in errs.items()}, f"div {dg:.1e}", flush=True)
+        for k, e in errs.items():
+            assert e < FP64_TOL, (ns, k, e)
+        assert dg < 1e-10
+    pt.close()
+    s.close()

@@ -153,3 +153,46 @@ def test_io_terms_marching_matches_rowtile(scaled, dims, kw):
         assert e < 1e-13, (k, e)
     pr_scale = np.abs(outs[1]["pr"]).max()
     assert np.abs(outs[0]["pr"] - outs[1]["pr"]).max() < 1e-13 * pr_scale
+
+
+def test_mean_xz_io_and_time_means(scaled):
+    """PP_MEAN_ZX_FLOW_Xperiodic_IO on the device against the oracle on the
+    device's own fields after each of three heated substeps (only the
+    summation order differs: 1e-13 of each column's largest value), with the
+    time means of PP_MEAN_T (asymptotic, then summing) against the recurrence
+    on the oracle's moments; the tg time means too."""
+    print("mean xz io + time means", flush=True)
+    dims = (32, 24, 16)
+    og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=77)
+    rmax = O.rhoh_max(scaled)
+    acc, ns_o = np.zeros((dims[1], 37)), 0
+    for ns in (1, 2, 3):
+        assert O.energy_substep(og, st, tab, n, ns, 1e-3, CTHECD, rmax) == 0
+        io.prepare(ns)
+        io.substep(ns, 1e-3)
+        hc.substep(ns, 1e-3, CTHECD)
+        dev = hc.download()
+        ref = O.mean_xz_io(og, {"q": dev["q"], "g": dev["g"], "pr": dev["pr"], "T": dev["temperature"],
+                                "rho": dev["density"], "h": dev["enthalpy"], "mu": dev["viscosity"]})
+        got = hc.mean_xz_io()
+        sc = np.maximum(np.abs(ref).max(axis=0), 1e-300)
+        err = float((np.abs(got - ref) / sc).max())
+        # and the trajectory itself stays with the oracle's
+        traj = float((np.abs(got - O.mean_xz_io(og, st)) / sc).max())
+        print(ns, "mean_xz_io rel", err, "vs oracle trajectory", traj, flush=True)
+        assert err < 1e-13 and traj < 1e-10
+        acc, ns_o = O.mean_t_update(acc, ref, ns_o, tp=1)
+        tm, nst = s.mean_t("io", tp=1)
+        assert nst == ns_o
+        assert float((np.abs(tm - acc) / sc).max()) < 1e-13
+    s.mean_t_reset("io")
+    hc.mean_xz_io()
+    tm, nst = s.mean_t("io", tp=2, pp_instn_sz=4)
+    assert nst == 1 and float((np.abs(tm - ref / 4) / sc).max()) < 1e-13
+    # tg domain: chs_mean_xz feeds PP_MEAN_T_TG
+    s.mean_t_reset("tg")
+    m1 = s.mean_xz()
+    tm, nst = s.mean_t("tg", tp=1)
+    assert nst == 1 and np.array_equal(tm, m1)
+    pt.close()
+    s.close()

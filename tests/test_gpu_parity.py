@@ -349,3 +349,31 @@ def test_trajectory_64cube_re2800(solver_mod):
             assert eq < 1e-10 and ep < 1e-10 and em < 1e-10, (i, eq, ep, em)
             assert dg < 1e-10
     s.close()
+
+
+def test_instant_files_restart_device_run(solver_mod, tmp_path):
+    """WRT_INSTANT_VARS_TG of a device run after a whole RK3 step, then
+    RESTART_INSTANT_VARS_TG into a fresh handle: both continue with
+    NS = 1, 2, 3 to the same bits (NS = 1 does not read CONVH0, so the files
+    hold the whole state a restart needs)."""
+    print("instant files restart", flush=True)
+    g = ChannelGrid(64, 48, 32)
+    q, pr, c0 = fields.init_tg_fields(g, seed=31)
+    a = solver_mod.ChannelSolver(g)
+    a.upload_state(q, pr, c0)
+    dt = 0.5 / a.cfl()
+    for ns in (0, 1, 2, 3):
+        a.solverrk3_mom_tg(ns, dt)
+    d = str(tmp_path) + "/"
+    paths = a.write_instant_vars_tg(4, 4 * dt, dt, d)
+    assert len(paths) == 4
+    b = solver_mod.ChannelSolver(g)
+    it, t = b.restart_instant_vars_tg(4 * dt, d)
+    assert it == 4 and t == 4 * dt
+    for ns in (1, 2, 3):
+        a.solverrk3_mom_tg(ns, dt)
+        b.solverrk3_mom_tg(ns, dt)
+    for x, y in zip(a.download_state(), b.download_state()):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
