@@ -26,6 +26,9 @@
 
 namespace chs {
 
+int mlp_update(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap, double* cp,
+               unsigned long long* cnt, hipStream_t stream);
+
 struct EGeo {
     int L, M, N;
     double dxi, dzi, dxqi, dzqi;
@@ -237,8 +240,10 @@ constexpr int EPT = 512;   // threads per property block
 
 // rotations of the property history on every cell, the property cascade on
 // interior rows; persistent blocks over k planes
+// second = 0: the final THERM_PROP_UPDATE lookup is left to the learned
+// surrogate (chs_props_set_mode), which runs after this kernel on h
 __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields f,
-                                                      unsigned long long* __restrict__ cnt) {
+                                                      unsigned long long* __restrict__ cnt, int second) {
 #pragma clang fp contract(off)
     extern __shared__ unsigned char smem[];
     const double *xh, *xt;
@@ -267,7 +272,7 @@ __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields
         f.rho[q] = D;
         hv = rh / D;
         f.h[q] = hv;
-        if (fabs(hv) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells
+        if (!second || fabs(hv) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells
         if (!spline_ht(t, xh, sh, hv, Tt, false) || !spline_t(t, xt, st, Tt, D, Mu, K, CP)) {
             ++bad;
             continue;
@@ -372,12 +377,24 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     h->timer.begin("energy_props", s, &ev);
     CHS_TRY(props_lds_attr((const void*)k_energy_props, p));
     CHS_TRY(props_lds_attr((const void*)k_energy_wall, p));
-    CHS_LAUNCH(k_energy_props, dim3(props_grid(p, 512)), dim3(EPT), props_lds_bytes(p), s, t, g, f, p->counters);
+    const bool sur = p->mode == CHS_PROPS_SURROGATE;
+    CHS_LAUNCH(k_energy_props, dim3(props_grid(p, 512)), dim3(EPT), props_lds_bytes(p), s, t, g, f, p->counters,
+               sur ? 0 : 1);
+    if (sur) {
+        // THERM_PROP_UPDATE of the interior rows by the surrogate where it is
+        // certified (the table elsewhere), on this handle's stream
+        const Rows rw{(long long)h->N, (long long)h->L * h->M, (long long)h->L * (h->M + 2)};
+        const size_t o = (size_t)h->L;   // row J = 1
+        CHS_TRY(mlp_update(p, tf->enthalpy + o, rw, tf->temperature + o, tf->density + o, tf->viscosity + o,
+                           tf->thermcondt + o, nullptr, p->counters, s));
+    }
     h->timer.end(ev, "energy_props", s);
     CHS_LAUNCH(k_energy_wall, dim3(64), dim3(256), props_lds_bytes(p), s, t, g, f, p->counters);
-    unsigned long long bad = 0;
-    CHS_HIP(hipMemcpyAsync(&bad, p->counters, sizeof(bad), hipMemcpyDeviceToHost, s));
+    unsigned long long cnt[2] = {0, 0};
+    CHS_HIP(hipMemcpyAsync(cnt, p->counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     CHS_HIP(hipStreamSynchronize(s));
+    const unsigned long long bad = cnt[0];
+    p->last_fallback = sur ? (long long)cnt[1] : 0;
     if (bad) {
         set_error("chs_energy_substep: " + std::to_string(bad) + " property lookups outside the table");
         return CHS_E_RANGE;

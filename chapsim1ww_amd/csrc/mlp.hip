@@ -167,8 +167,9 @@ struct Mlp {
     bool trained = false;
     long long step = 0;
     int nbins = 0;
-    unsigned char* cert = nullptr;   // [nbins] 1 = the network is used in this bin
-    double cert_frac = 0.0;
+    unsigned char* cert = nullptr;   // [nbins] bit 0: all five outputs within the bound; bit 1: T, rho, mu, kappa
+    double cert_frac = 0.0;          // fraction of bins with bit 0
+    double cert_frac4 = 0.0;         // fraction of bins with bit 1
 };
 
 // ---------------------------------------------------------------------------
@@ -429,6 +430,9 @@ __global__ void __launch_bounds__(ST) k_mlp_props(PropTab t, const float* __rest
     WLds& W = *reinterpret_cast<WLds*>(smem);
     unsigned char* sc = smem + sizeof(WLds);
     for (int e = threadIdx.x; e < nbins; e += blockDim.x) sc[e] = cert[e];
+    // bin flags: bit 0 all five outputs certified, bit 1 T, rho, mu, kappa
+    // (what THERM_PROP_UPDATE writes when c_p is not asked for)
+    const unsigned char need = cp ? 1 : 2;
     stage_weights(W, wq);   // ends with a barrier
     // the table fallback searches the knots in global memory (L2-resident)
     const double *xh = t.xh, *xt = t.xt;
@@ -453,7 +457,7 @@ __global__ void __launch_bounds__(ST) k_mlp_props(PropTab t, const float* __rest
             if (use_net) {
                 int b = (int)((hv - nm.h_lo) * binv);
                 b = b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
-                use_net = sc[b] != 0;
+                use_net = (sc[b] & need) != 0;
             }
             const float x = (float)((use_net ? hv - nm.hc : 0.0) * nm.hs);
             f4v o = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -502,7 +506,7 @@ __global__ void __launch_bounds__(ST) k_mlp_props(PropTab t, const float* __rest
 // host
 // ---------------------------------------------------------------------------
 int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap,
-                       double* cp, unsigned long long* cnt, int count_lookups = 0);
+                       double* cp, unsigned long long* cnt, int count_lookups, hipStream_t stream);
 
 void mlp_release(chs_props* p) {
     if (!p || !p->mlp) return;
@@ -559,6 +563,7 @@ static int mlp_alloc(chs_props* p, int maxblk) {
             m->cert = old->cert;
             m->nbins = old->nbins;
             m->cert_frac = old->cert_frac;
+            m->cert_frac4 = old->cert_frac4;
             old->cert = nullptr;
         }
         mlp_release(p);
@@ -580,18 +585,20 @@ static int set_params(chs_props* p, const float* host) {
 
 static size_t mlp_props_lds(const chs_props* p) { return sizeof(WLds) + (size_t)(p->mlp ? p->mlp->nbins : 0); }
 
+bool mlp_ready(const chs_props* p) { return p->mlp && p->mlp->trained && p->mlp->cert; }
+
 int mlp_update(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap, double* cp,
-               unsigned long long* cnt) {
+               unsigned long long* cnt, hipStream_t stream) {
     CHS_CHECK(p->mlp && p->mlp->trained && p->mlp->cert, CHS_E_ARG,
               "chs_therm_prop_update: the surrogate is not trained and certified");
     const Mlp* m = p->mlp;
     // no certified bin: every cell takes the table, so run the table kernel
-    if (m->cert_frac == 0.0) return props_table_launch(p, h, rw, T, rho, mu, kap, cp, cnt, 1);
+    if ((cp ? m->cert_frac : m->cert_frac4) == 0.0) return props_table_launch(p, h, rw, T, rho, mu, kap, cp, cnt, 1, stream);
     const long long chunk = (long long)ST / 4 * SPASS;
     const long long nch = ((rw.rowlen + chunk - 1) / chunk) * rw.nrow;
     if (nch == 0) return CHS_OK;
     const int grid = (int)std::min<long long>(nch, 2048);
-    CHS_LAUNCH(k_mlp_props, dim3(grid), dim3(ST), mlp_props_lds(p), p->stream, tab_of(p), m->wq, m->norm, m->cert,
+    CHS_LAUNCH(k_mlp_props, dim3(grid), dim3(ST), mlp_props_lds(p), stream, tab_of(p), m->wq, m->norm, m->cert,
                m->nbins, rw, h, T, rho, mu, kap, cp, cnt);
     return CHS_OK;
 }
@@ -623,7 +630,8 @@ static int make_pool(chs_props* p, const double* h, long long n, Pool& pool) {
     pool.pt = buf + n;
     Rows rw{1, n, n};
     hipLaunchKernelGGL(k_zero_u64_b, dim3(1), dim3(64), 0, p->stream, p->counters, 4);
-    int st = props_table_launch(p, h, rw, lab, lab + n, lab + 2 * n, lab + 3 * n, lab + 4 * n, p->counters);
+    int st = props_table_launch(p, h, rw, lab, lab + n, lab + 2 * n, lab + 3 * n, lab + 4 * n, p->counters, 0,
+                                p->stream);
     unsigned long long bad = 0;
     if (st == CHS_OK) {
         if (hipMemcpyAsync(&bad, p->counters, sizeof(bad), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
@@ -730,6 +738,7 @@ static int train(chs_props* p, const double* h, long long nsamp, int steps, int 
         (void)hipFree(m->cert);
         m->cert = nullptr;
         m->cert_frac = 0.0;
+        m->cert_frac4 = 0.0;
     }
     if (final_loss) *final_loss = loss;
     return CHS_OK;
@@ -789,7 +798,8 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
     hipLaunchKernelGGL(k_mlp_cert_points, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, p->stream, m->norm, nbins, hp);
     hipLaunchKernelGGL(k_zero_u64_b, dim3(1), dim3(64), 0, p->stream, p->counters, 4);
     Rows rw{1, np, np};
-    st = props_table_launch(p, hp, rw, tab, tab + np, tab + 2 * np, tab + 3 * np, tab + 4 * np, p->counters);
+    st = props_table_launch(p, hp, rw, tab, tab + np, tab + 2 * np, tab + 3 * np, tab + 4 * np, p->counters, 0,
+                            p->stream);
     if (st == CHS_OK) {
         hipLaunchKernelGGL(k_mlp_eval, dim3(1024), dim3(256), sizeof(WLds), p->stream, m->wq, m->norm, hp, np, net);
         if (hipGetLastError() != hipSuccess) st = CHS_E_HIP;
@@ -810,9 +820,9 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
     }
     CHS_CHECK(bad == 0, CHS_E_RANGE, "chs_surrogate_certify: trained range leaves the property table");
     std::vector<unsigned char> flag(nbins, 0);
-    long long ok = 0;
+    long long ok = 0, ok4 = 0;
     for (int b = 0; b < nbins; ++b) {
-        double worst = 0.0;
+        double worst = 0.0, worst4 = 0.0;
         for (int k = 0; k < NPT; ++k) {
             const size_t q = (size_t)b * NPT + k;
             // THERM_PROP_UPDATE leaves cells with |h| < 1e-10 unchanged in
@@ -820,13 +830,17 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
             if (std::fabs(hh[q]) < 1.0e-10) continue;
             for (int f = 0; f < NOUT; ++f) {
                 const double tv = ht[(size_t)f * np + q], nv = hn[(size_t)f * np + q];
-                const double e = std::fabs(nv - tv) / std::max(std::fabs(tv), 1e-300);
-                worst = std::max(worst, e != e ? 1e300 : e);
+                double e = std::fabs(nv - tv) / std::max(std::fabs(tv), 1e-300);
+                e = e != e ? 1e300 : e;
+                worst = std::max(worst, e);
+                if (f < 4) worst4 = std::max(worst4, e);   // outputs T, rho, mu, kappa
             }
         }
         // margin of 2 between the checked points and the bin's interior
-        flag[b] = worst <= 0.5 * tol ? 1 : 0;
-        ok += flag[b];
+        const bool c5 = worst <= 0.5 * tol, c4 = worst4 <= 0.5 * tol;
+        flag[b] = (unsigned char)((c5 ? 1 : 0) | (c4 ? 2 : 0));
+        ok += c5;
+        ok4 += c4;
     }
     if (m->cert) (void)hipFree(m->cert);
     m->cert = nullptr;
@@ -834,6 +848,7 @@ static int certify(chs_props* p, double tol, int nbins, double* frac) {
     CHS_HIP(hipMemcpy(m->cert, flag.data(), nbins, hipMemcpyHostToDevice));
     m->nbins = nbins;
     m->cert_frac = (double)ok / nbins;
+    m->cert_frac4 = (double)ok4 / nbins;
     if (frac) *frac = m->cert_frac;
     return CHS_OK;
 }
@@ -894,6 +909,13 @@ int chs_surrogate_train(chs_props* p, const double* h_dev, long long nsamp, int 
                         unsigned long long seed, double* final_loss) {
     CHS_CHECK(p != nullptr, CHS_E_ARG, "chs_surrogate_train: NULL handle");
     return chs::train(p, h_dev, nsamp, steps, batch, lr, seed, final_loss);
+}
+
+int chs_surrogate_certified(chs_props* p, double* all_outputs, double* without_cp) {
+    CHS_CHECK(p && p->mlp && p->mlp->cert, CHS_E_ARG, "chs_surrogate_certified: the surrogate is not certified");
+    if (all_outputs) *all_outputs = p->mlp->cert_frac;
+    if (without_cp) *without_cp = p->mlp->cert_frac4;
+    return CHS_OK;
 }
 
 int chs_surrogate_certify(chs_props* p, double tol, int nbins, double* certified_fraction) {

@@ -185,13 +185,13 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
 }
 
 int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap,
-                       double* cp, unsigned long long* cnt, int count_lookups) {
+                       double* cp, unsigned long long* cnt, int count_lookups, hipStream_t stream) {
     const long long chunk = (long long)PT * PCELL;
     const long long nch = ((rw.rowlen + chunk - 1) / chunk) * rw.nrow;
     if (nch == 0) return CHS_OK;
     const int grid = props_grid(p, nch);
     CHS_TRY(props_lds_attr((const void*)k_props_table, p));
-    CHS_LAUNCH(k_props_table, dim3(grid), dim3(PT), props_lds_bytes(p), p->stream, tab_of(p), rw, h, T, rho, mu,
+    CHS_LAUNCH(k_props_table, dim3(grid), dim3(PT), props_lds_bytes(p), stream, tab_of(p), rw, h, T, rho, mu,
                kap, cp, cnt, count_lookups);
     return CHS_OK;
 }
@@ -315,8 +315,9 @@ int props_create(int n, const double* H, const double* T, const double* D, const
 }
 
 void mlp_release(chs_props* p);
+bool mlp_ready(const chs_props* p);
 int mlp_update(chs_props* p, const double* h, Rows rw, double* T, double* rho, double* mu, double* kap, double* cp,
-               unsigned long long* cnt);
+               unsigned long long* cnt, hipStream_t stream);
 
 int props_destroy(chs_props* p) {
     if (!p) return CHS_OK;
@@ -341,9 +342,9 @@ int props_update(chs_props* p, int mode, const double* h, long long nrow, long l
     Rows rw{nrow, rowlen, nrow > 1 ? stride : rowlen};
     CHS_LAUNCH(k_zero_u64, dim3(1), dim3(64), 0, p->stream, p->counters, 4);
     if (mode == CHS_PROPS_TABLE)
-        CHS_TRY(props_table_launch(p, h, rw, T, rho, mu, kap, cp, p->counters, 0));
+        CHS_TRY(props_table_launch(p, h, rw, T, rho, mu, kap, cp, p->counters, 0, p->stream));
     else
-        CHS_TRY(mlp_update(p, h, rw, T, rho, mu, kap, cp, p->counters));
+        CHS_TRY(mlp_update(p, h, rw, T, rho, mu, kap, cp, p->counters, p->stream));
     unsigned long long host[2] = {0, 0};
     CHS_HIP(hipMemcpyAsync(host, p->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
     CHS_HIP(hipStreamSynchronize(p->stream));
@@ -366,6 +367,21 @@ int chs_props_create(int n, const double* H, const double* T, const double* D, c
 }
 
 int chs_props_destroy(chs_props* p) { return chs::props_destroy(p); }
+
+int chs_props_set_mode(chs_props* p, int mode) {
+    CHS_CHECK(p != nullptr, CHS_E_ARG, "chs_props_set_mode: NULL handle");
+    CHS_CHECK(mode == CHS_PROPS_TABLE || mode == CHS_PROPS_SURROGATE, CHS_E_ARG, "chs_props_set_mode: unknown mode");
+    CHS_CHECK(mode == CHS_PROPS_TABLE || (p->mlp && chs::mlp_ready(p)), CHS_E_ARG,
+              "chs_props_set_mode: the surrogate is not trained and certified");
+    p->mode = mode;
+    return CHS_OK;
+}
+
+int chs_props_last_fallback(chs_props* p, long long* n_fallback) {
+    CHS_CHECK(p != nullptr && n_fallback != nullptr, CHS_E_ARG, "chs_props_last_fallback: NULL argument");
+    *n_fallback = p->last_fallback;
+    return CHS_OK;
+}
 
 int chs_props_rhoh_max(chs_props* p, double* rhoh_max) {
     CHS_CHECK(p != nullptr && rhoh_max != nullptr, CHS_E_ARG, "chs_props_rhoh_max: NULL argument");

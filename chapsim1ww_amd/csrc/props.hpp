@@ -31,6 +31,10 @@ struct chs_props {
     std::vector<double> hH, hT, hrh, hrt;
     // learned surrogate (mlp.hip)
     chs::Mlp* mlp = nullptr;
+    // the THERM_PROP_UPDATE form of chs_energy_substep (CHS_PROPS_TABLE or
+    // CHS_PROPS_SURROGATE) and the cells its last call handed to the table
+    int mode = 0;
+    long long last_fallback = 0;
 };
 
 namespace chs {

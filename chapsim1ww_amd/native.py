@@ -99,6 +99,9 @@ _SIGS = {
                          C.c_int),
     "chs_profile_reset": ([C.c_void_p], C.c_int),
     "chs_last_error": ([], C.c_char_p),
+    "chs_surrogate_certified": ([C.c_void_p, _dp, _dp], C.c_int),
+    "chs_props_set_mode": ([C.c_void_p, C.c_int], C.c_int),
+    "chs_props_last_fallback": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "chs_mean_xz_io": ([C.c_void_p, C.POINTER(ThermalFields), C.POINTER(IoFields), C.c_int, _dp], C.c_int),
     "chs_mean_t": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _dp], C.c_int),
     "chs_mean_t_reset": ([C.c_void_p, C.c_int], C.c_int),

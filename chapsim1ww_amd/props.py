@@ -128,6 +128,18 @@ class PropertyTable:
         nat.check(self.lib.chs_props_create(self.n, *[nat.hptr(c) for c in cols], s, C.byref(p)))
         self.p = p
 
+    def set_mode(self, mode: str):
+        """The property update inside the energy substep: "table" or
+        "surrogate" (the certified network, the table elsewhere)."""
+        m = {"table": nat.CHS_PROPS_TABLE, "surrogate": nat.CHS_PROPS_SURROGATE}[mode]
+        nat.check(self.lib.chs_props_set_mode(self.p, m))
+
+    def last_fallback(self) -> int:
+        """Cells of the last energy substep the surrogate handed to the table."""
+        v = C.c_longlong()
+        nat.check(self.lib.chs_props_last_fallback(self.p, C.byref(v)))
+        return v.value
+
     def rhoh_max(self) -> float:
         """RHOHmax of the table (the limiter of the energy update)."""
         v = C.c_double()
@@ -163,6 +175,13 @@ class PropertyTable:
         f = C.c_double()
         nat.check(self.lib.chs_surrogate_certify(self.p, float(tol), int(nbins), C.byref(f)))
         return f.value
+
+    def certified(self):
+        """(fraction of bins certified for all five outputs, for T, rho, mu,
+        kappa alone) of the last certification."""
+        a, b = C.c_double(), C.c_double()
+        nat.check(self.lib.chs_surrogate_certified(self.p, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def grad(self, h):
         """Loss and gradient (float32, NPAR) of the surrogate over exactly the

@@ -209,6 +209,16 @@ int chs_therm_prop_update(chs_props *p, int mode, const double *h_dev, long long
                           long long row_stride, double *T_dev, double *rho_dev, double *mu_dev,
                           double *kap_dev, double *cp_dev, long long *n_fallback);
 
+/* The form of the property update inside chs_energy_substep: CHS_PROPS_TABLE
+ * (default, the reference's spline lookups) or CHS_PROPS_SURROGATE (the
+ * trained, certified network for the final THERM_PROP_UPDATE of the cascade,
+ * src/SOLVERRK3_ENG_IO.f90:46-66; the table for cells outside certified bins
+ * and for ENTHALPY_UPDATE / DENSITY_UPDATE).  No counterpart in the
+ * reference.  CHS_E_ARG if the surrogate is not trained and certified. */
+int chs_props_set_mode(chs_props *p, int mode);
+/* Cells of the last chs_energy_substep that the surrogate handed to the table. */
+int chs_props_last_fallback(chs_props *p, long long *n_fallback);
+
 /* RHOHmax of the table (NIST_SPLINE_H, src/Flow_State.f90:960-990): the
  * limiter of the energy update. */
 int chs_props_rhoh_max(chs_props *p, double *rhoh_max);
@@ -327,6 +337,11 @@ int chs_surrogate_train(chs_props *p, const double *h_dev, long long nsamp, int 
  * network is within tol/2 (relative, every output) of the table on 33 points
  * spanning it.  Uncertified bins and cells outside the range use the table. */
 int chs_surrogate_certify(chs_props *p, double tol, int nbins, double *certified_fraction);
+/* Fractions of the bins the last certification passed for all five outputs
+ * and for T, rho, mu, kappa alone: a call that passes cp_dev = NULL (and the
+ * energy substep in surrogate mode) uses the network in the bins of the
+ * second kind. */
+int chs_surrogate_certified(chs_props *p, double *all_outputs, double *without_cp);
 /* Loss and gradient (host, 4613 floats) over exactly the n device samples
  * (n <= 8192), with no update: the training kernels' gradient, for checking. */
 int chs_surrogate_grad(chs_props *p, const double *h_dev, long long n, float *grad_host, double *loss);

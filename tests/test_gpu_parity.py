@@ -253,12 +253,36 @@ def test_rhs_marching_matches_rowtile(solver_mod, dims):
 
 
 def test_substep_mixed_divergence(solver_mod):
+    """The mixed form (FP32 stencils and x/z solves, FP64 y elimination, FP32
+    Poisson refined once with its residual checked in the projection): the
+    projected divergence stays below 1e-10, the state close to the FP64
+    oracle, and the check's verdict is read back (no error, residual known)."""
     dims = (64, 64, 64)
     g, out = _run_pair(dims, 1, precision="mixed", solver_mod=solver_mod)
     for ns, gpu, ref, dg, dr, *_ in out:
+        print(ns, "div", dg, "Q", _rel(gpu[0], ref[0]), "PR", _rel(gpu[1], ref[1]), flush=True)
         assert dg <= 1e-10, (ns, dg)
-        # FP32 transforms only in the pressure correction: state close to FP64
         assert _rel(gpu[0], ref[0]) < 1e-6
+
+
+def test_mixed_check_reports_nan(solver_mod):
+    """A NaN planted in the state reaches the mixed substep's fused residual
+    check; the verdict, read at the next synchronising call, is an error."""
+    g = ChannelGrid(64, 48, 64)
+    q, pr, c0 = fields.init_tg_fields(g, seed=8)
+    s = solver_mod.ChannelSolver(g, "mixed")
+    s.upload_state(q, pr, c0)
+    dt = 0.5 / s.cfl()
+    s.solverrk3_mom_tg(0, dt)
+    it, rel = s.poisson_stats()
+    assert it == 1 and 0.0 <= rel < 1e-9, (it, rel)
+    q2 = q.copy(order="F")
+    q2[5, 7, 9, 0] = np.nan
+    s.upload_state(q2, pr, c0)
+    s.solverrk3_mom_tg(1, dt)
+    with pytest.raises(nat.ChsError, match="NaN"):
+        s.poisson_stats()
+    s.close()
 
 
 def test_divgck_cfl_and_nan(solver_mod):

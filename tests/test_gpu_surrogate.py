@@ -140,3 +140,105 @@ def test_train_certify_and_update(scaled, snapshot_h):
     print(f"THERM_PROP_UPDATE 2^22 cells: table {tt*1e3:.3f} ms, surrogate {ts*1e3:.3f} ms, "
           f"gain {tt/ts:.3f}x, certified {frac:.3f}", flush=True)
     pt.close()
+
+
+HEAT_MAP = {"rhoh": "rhoh", "h": "enthalpy", "T": "temperature", "rho": "density", "rho0": "density0",
+            "rho1": "density1", "mu": "viscosity", "mu0": "viscosity0", "kap": "thermcondt", "rhs0": "rhs_energy0",
+            "g": "g", "g0": "g0", "wallflux": "wallflux", "q": "q", "pr": "pr", "drhodtp": "drhodtp",
+            "divu": "divu", "rho_stg": "rho_stg", "mu_stg": "mu_stg", "explt0": "explt0", "drhoi_stg": "drhoi_stg"}
+
+
+def test_surrogate_on_heated_snapshots_256x192x256():
+    """The surrogate workload at its own size: the heated channel at
+    256x192x256 (2048-point table, q_w = 0.1, Re = 6880, Pr = 1) on the
+    mixed-precision solver; 2^20 enthalpy samples from the solver's own
+    ENTHALPY after three substeps; training, certification at 1e-4;
+    THERM_PROP_UPDATE on the whole snapshot in both modes (network cells within
+    1e-4 of the table, fallback cells equal to it bit for bit, the time of both
+    modes recorded); then three more substeps with the energy substep's
+    property update in surrogate mode beside a table-mode twin (divergence
+    below 1e-10, T, rho, mu, kappa within 1e-4 of the twin)."""
+    from chapsim1ww_amd.heated import HeatedChannel
+    from chapsim1ww_amd.solver import ChannelSolver
+    dims, ren = (256, 192, 256), 6880.0
+    L, M, N = dims
+    print(f"surrogate on heated snapshots {dims}", flush=True)
+    scaled = props.scale_table(props.synthetic_table(2048), t0=1.0)
+    g = ChannelGrid(*dims, ren=ren)
+    og = O.OracleGrid(*dims, ren=ren)
+    st, _, _ = O.io_case(og, scaled, seed=2026, qw=0.1)
+    cthecd, dt = 1.0 / ren, 5e-4
+    twins = []
+    for _ in range(2):
+        s = ChannelSolver(g, "mixed")
+        pt = props.PropertyTable(scaled)
+        hc = HeatedChannel(s, pt, thermal=1)
+        hc.upload({HEAT_MAP[k]: v for k, v in st.items()})
+        twins.append((s, pt, hc))
+    del st
+    s, pt, hc = twins[0]
+    for ns in (1, 2, 3):
+        for (_, _, h_) in twins:
+            h_.substep(ns, dt, cthecd)
+        assert hc.divgck_io()[0] <= 1e-10
+    # 2^20 samples of the solver's ENTHALPY (interior rows), drawn on the device
+    H = hc.dev["enthalpy"].view(N, M + 2, L)[:, 1:M + 1, :].reshape(-1)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    samples = H[torch.randint(0, H.numel(), (1 << 20,), device="cuda", generator=gen)].contiguous()
+    t0 = time.perf_counter()
+    loss = pt.train(samples, steps=40000, batch=4096, lr=3e-3, seed=5)
+    t_train = time.perf_counter() - t0
+    frac5 = pt.certify(1e-4, 4096)
+    frac5, frac4 = pt.certified()
+    print(f"trained 40000 Adam steps on 2^20 snapshot samples in {t_train:.2f} s, loss {loss:.3e}; certified "
+          f"at 1e-4: {frac5:.4f} (all outputs), {frac4:.4f} (T, rho, mu, kappa)", flush=True)
+    # THERM_PROP_UPDATE on the snapshot's interior in both modes (no c_p)
+    Hc = H.contiguous()
+    n = Hc.numel()
+    tab = [torch.full((n,), -1.0, dtype=torch.float64, device="cuda") for _ in range(4)]
+    sur = [torch.full((n,), -1.0, dtype=torch.float64, device="cuda") for _ in range(4)]
+    pt.update(Hc, *tab)
+    nf = pt.update(Hc, *sur, mode="surrogate")
+    torch.cuda.synchronize()
+    A = torch.stack(tab, 1)
+    B = torch.stack(sur, 1)
+    rel = ((B - A).abs() / A.abs().clamp_min(1e-300)).max(1).values
+    net = (B != A).any(1)
+    worst = float(rel.max())
+    print(f"snapshot update: {n} cells, network {int(net.sum())}, fallback {nf}, max rel err {worst:.3e}",
+          flush=True)
+    assert worst <= 1e-4
+    assert int(net.sum()) <= n - nf
+    if frac4 == 0.0:
+        assert nf == n
+
+    def timed(mode):
+        for _ in range(2):
+            pt.update(Hc, *sur, mode=mode)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(10):
+            pt.update(Hc, *sur, mode=mode)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / 10
+    tt, ts = timed("table"), timed("surrogate")
+    print(f"THERM_PROP_UPDATE {n} cells: table {tt * 1e3:.3f} ms, surrogate {ts * 1e3:.3f} ms, "
+          f"gain {tt / ts:.3f}x", flush=True)
+    # the energy substep with the surrogate in its property update, beside the table twin
+    if frac4 > 0.0:
+        pt.set_mode("surrogate")
+    for ns in (1, 2, 3):
+        for (_, _, h_) in twins:
+            h_.substep(ns, dt, cthecd)
+        assert hc.divgck_io()[0] <= 1e-10 and twins[1][2].divgck_io()[0] <= 1e-10
+    fb = pt.last_fallback()
+    a, b = hc.download(), twins[1][2].download()
+    errs = {k: float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max())
+            for k in ("temperature", "density", "viscosity", "thermcondt", "enthalpy")}
+    print(f"3 substeps, energy update in {'surrogate' if frac4 > 0 else 'table'} mode vs table twin: "
+          f"{ {k: f'{v:.2e}' for k, v in errs.items()} }, last fallback {fb}", flush=True)
+    for k in ("temperature", "density", "viscosity", "thermcondt"):
+        assert errs[k] <= 1e-4, k
+    for (s_, p_, _) in twins:
+        p_.close()
+        s_.close()

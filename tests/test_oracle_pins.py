@@ -300,6 +300,100 @@ def test_y_profile_substep_matches_dense_cn(ns):
     assert np.abs(ue - u).max() > 1e-5 * np.abs(u).max()
 
 
+def _dense_zmode_substep(ynd, ns, dt, nu, lam, dzi, U, H0):
+    """SOLVERRK3_MOM_tg(NS) for u = U(y) cos(theta k), v = w = p = 0 with
+    CONVH0_u = H0(y) cos(theta k) (theta = 2 pi / N), assembled from dense
+    operators: the convection of this field vanishes, the x solve is the
+    identity and the z solve divides by 1 + F lam (lam the eigenvalue of the
+    second difference in z); u's y solve (I - F A); v's row-1 right-hand side
+    is u's final increment on row 1 (the shared buffer, src/MOMFA_tg.f90:
+    191-199), solved with the node operator; the projection solves the
+    (kx, kz) = (0, 1) mode (A_p - lam) phi = f with no pinned row, and moves
+    w by -alpha dt dzi (phi(k) - phi(k-1)).  Returns the amplitudes of
+    u, v (x cos(theta k)), of w (x (cos(theta k) - cos(theta (k-1)))) and of
+    p (x cos(theta k))."""
+    M = len(U)
+    A = _cell_operator(ynd)
+    a = TALP[ns]
+    F = 0.5 * a * dt * nu
+    dy = np.diff(ynd)
+    ru = dt * (TROH[ns] * H0 + a * nu * (A @ U - lam * U))
+    du = np.linalg.solve(np.eye(M) - F * A, ru / (1.0 + F * lam))
+    T = np.eye(M) - F * _node_operator(ynd)[:M, :M]
+    rv = np.zeros(M)
+    rv[0] = du[0]
+    dv = np.linalg.solve(T, rv)
+    vs = np.zeros(M + 1)
+    vs[1:M] = dv[1:M]
+    f = (vs[1:] - vs[:-1]) / dy / (a * dt)
+    phi = np.linalg.solve(_neumann_operator(ynd) - lam * np.eye(M), f)
+    dm = np.diff(0.5 * (ynd[1:] + ynd[:-1]))
+    v = vs.copy()
+    v[1:M] -= a * dt * (phi[1:] - phi[:-1]) / dm
+    w = -a * dt * dzi * phi
+    return U + du, v, w, phi - F * f
+
+
+@pytest.mark.parametrize("ns", [0, 1, 3])
+def test_z_mode_substep_matches_dense_cn(ns):
+    """A three-dimensional composite: u = U(y) cos(2 pi k / N).  Pins the z
+    solve and the z viscous term (the eigenvalue of the discrete second
+    difference), the mass-flux integral of a zero-mean mode, the buffer reuse
+    into v, the Poisson solve of a (0, 1) mode and the z component of the
+    projection against operators built from the node positions."""
+    L, M, N = 4, 20, 8
+    g = ChannelGrid(L, M, N, ren=180.0)
+    og = O.OracleGrid(L, M, N, ren=180.0)
+    rng = np.random.default_rng(40 + ns)
+    y = g.ycc
+    U = (1 - y ** 2) * (0.7 + 0.2 * y)
+    H0 = rng.standard_normal(M)
+    th = 2 * np.pi / N
+    ck = np.cos(th * np.arange(N))
+    q = g.zeros(g.shape_q)
+    q[:, 1:M + 1, :, 0] = U[None, :, None] * ck[None, None, :]
+    c0 = g.zeros(g.shape_convh0)
+    c0[:, :, :, 0] = H0[None, :, None] * ck[None, None, :]
+    st = O.OracleState(og, q, g.zeros(g.shape_ghost), c0)
+    dt = 4e-3
+    st.substep(ns, dt)
+    dz = np.pi / N
+    lam = (2.0 - 2.0 * np.cos(th)) / dz ** 2
+    ue, ve, we, pe = _dense_zmode_substep(_nodes(M), ns, dt, 1.0 / 180.0, lam, 1.0 / dz, U, H0)
+    dw = ck - np.roll(ck, 1)
+    checks = (("u", st.q[:, 1:M + 1, :, 0], ue[None, :, None] * ck[None, None, :]),
+              ("v", st.q[:, 1:M + 1, :, 1], ve[None, :M, None] * ck[None, None, :]),
+              ("w", st.q[:, 1:M + 1, :, 2], we[None, :, None] * dw[None, None, :]),
+              ("p", st.pr[:, 1:M + 1, :], pe[None, :, None] * ck[None, None, :]))
+    for name, got, exp in checks:
+        sc = np.abs(exp).max()
+        assert sc > 1e-12, name      # every component carries a signal
+        # 1e-10 of the component's scale, plus round-off of the O(1) u field
+        # (cos(theta k) vanishes only to 1e-16 at k = N/4)
+        assert np.abs(got - exp).max() < 1e-10 * sc + 1e-16, (name, np.abs(got - exp).max(), sc)
+
+
+def test_planted_z_coefficient_mistake_is_caught():
+    """The z solve's coefficient computed with DXQI in place of DZQI (a
+    transposed operand in MOMFA2_Z_tg) fails the dense z-mode comparison."""
+    L, M, N = 4, 20, 8
+    g = ChannelGrid(L, M, N, ren=180.0)
+    og = O.OracleGrid(L, M, N, ren=180.0)
+    og.g.dzqi = og.g.dxqi
+    y = g.ycc
+    U = (1 - y ** 2)
+    th = 2 * np.pi / N
+    ck = np.cos(th * np.arange(N))
+    q = g.zeros(g.shape_q)
+    q[:, 1:M + 1, :, 0] = U[None, :, None] * ck[None, None, :]
+    st = O.OracleState(og, q, g.zeros(g.shape_ghost), g.zeros(g.shape_convh0))
+    st.substep(1, 4e-3)
+    dz = np.pi / N
+    lam = (2.0 - 2.0 * np.cos(th)) / dz ** 2
+    ue, _, _, _ = _dense_zmode_substep(_nodes(M), 1, 4e-3, 1.0 / 180.0, lam, 1.0 / dz, U, np.zeros(M))
+    assert np.abs(st.q[:, 1:M + 1, :, 0] - ue[None, :, None] * ck[None, None, :]).max() > 1e-8
+
+
 def test_planted_mistake_in_y_substep_is_caught():
     """The oracle's wall row perturbed as a dropped factor 2 in UCAJ's
     DYFI term would leave: the dense CN comparison fails."""
