@@ -171,21 +171,55 @@ def gpu_bench(steps, warmup, cmp_steps):
     return res
 
 
-def props_bench(g, q_fp64, stream, steps):
-    """THERM_PROP_UPDATE (table and surrogate) over the 256x192x256 grid and
-    the energy substep, on the synthetic heated-channel table."""
+def _heated_start(g, sc, q_fp64, stream):
+    """A heated-channel state on g: T = 1.2 - 0.3 y^2 + a small x-z wave
+    mapped to h through the table, mass fluxes from the FP64 tg state,
+    q_w = 0.1 at both walls; returns (solver, table, channel) per precision."""
     import torch
     from chapsim1ww_amd import props
     from chapsim1ww_amd.heated import HeatedChannel
     from chapsim1ww_amd.solver import ChannelSolver
     dev = torch.device("cuda:0")
+    L, M, N = g.L, g.M, g.N
+    yc = np.concatenate([[-1.0], np.asarray(g.ycc), [1.0]])
+    xx = np.arange(L) / L * 2 * np.pi
+    zz = np.arange(N) / N * 2 * np.pi
+    T3 = 1.2 - 0.3 * yc[None, :, None] ** 2 + 0.02 * np.sin(xx)[:, None, None] * np.cos(zz)[None, None, :]
+    hg = torch.from_numpy(np.interp(T3.ravel(order="F"), sc["T"], sc["H"])).to(dev)
+    qd = torch.from_numpy(np.asfortranarray(q_fp64).ravel(order="F")).to(dev)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = ChannelSolver(g, prec, stream=stream)
+        pt = props.PropertyTable(sc, stream=stream)
+        ts = HeatedChannel(s, pt, thermal=1, nflow=1, flowdrvtp=1, gravdir=0)
+        f = ts.dev
+        f["enthalpy"].copy_(hg)
+        pt.update(hg, f["temperature"], f["density"], f["viscosity"], f["thermcondt"])
+        f["rhoh"].copy_(hg * f["density"])
+        for k in ("density0", "density1"):
+            f[k].copy_(f["density"])
+        f["viscosity0"].copy_(f["viscosity"])
+        f["g"].copy_(qd)
+        f["g0"].copy_(qd)
+        f["wallflux"].fill_(0.1)
+        out[prec] = (s, pt, ts)
+    return out
+
+
+def props_bench(g, q_fp64, stream, steps):
+    """The heated channel at 256x192x256 on the synthetic table: the whole
+    heated substep (FP64 and the mixed-precision solver side by side, their
+    difference stated), THERM_PROP_UPDATE by table and by the surrogate
+    trained on the heated solver's own ENTHALPY, and the energy substep with
+    either property update."""
+    import torch
+    from chapsim1ww_amd import props
+    dev = torch.device("cuda:0")
     sc = props.scale_table(props.synthetic_table())
-    pt = props.PropertyTable(sc, stream=stream)
     n = g.ncell
-    # enthalpy field from the FP64 state's streamwise velocity (heated-wall map)
-    h_host = props.snapshot_enthalpy(q_fp64[:, 1:-1, :, 0], sc, n, seed=7, uniform_share=0.0)
-    h = torch.from_numpy(h_host).to(dev)
-    outs = [torch.empty_like(h) for _ in range(5)]
+    L, M, N = g.L, g.M, g.N
+    twins = _heated_start(g, sc, q_fp64, stream)
+    s, pt, ts = twins["fp64"]
     out = {}
 
     def timed(fn, reps):
@@ -200,74 +234,79 @@ def props_bench(g, q_fp64, stream, steps):
         return e0.elapsed_time(e1) / reps
 
     reps = max(10, steps)
-    ms_t = timed(lambda: pt.update(h, *outs), reps)
+    seq = [1, 2, 3]
+    cthecd, dt = 1.0 / 2800.0, 1e-3
+    # three substeps of both forms from the same start: the solver snapshot
+    for i in range(3):
+        for (_, _, t_) in twins.values():
+            t_.substep(seq[i % 3], dt, cthecd)
+    # the snapshot's interior ENTHALPY: the property workload and the training samples
+    hs = ts.dev["enthalpy"].view(N, M + 2, L)[:, 1:M + 1, :].reshape(-1).contiguous()
+    outs = [torch.empty_like(hs) for _ in range(5)]
+    ms_t = timed(lambda: pt.update(hs, *outs), reps)
     out["table"] = dict(ms=round(ms_t, 4), cells_per_s=round(n / (ms_t * 1e-3), 1),
-                        gbs=round(48.0 * n / (ms_t * 1e-3) / 1e9, 1))
+                        gbs=round(48.0 * n / (ms_t * 1e-3) / 1e9, 1), field="heated-solver ENTHALPY after 3 substeps")
     ref = [o.clone() for o in outs]
-    # surrogate: trained on samples of this snapshot (plus a uniform share over its range)
-    samples = torch.from_numpy(props.snapshot_enthalpy(q_fp64[:, 1:-1, :, 0], sc, 1 << 20, seed=3)).to(dev)
+    ms_t4 = timed(lambda: pt.update(hs, *outs[:4]), reps)
+    gen = torch.Generator(device=dev).manual_seed(3)
+    samples = hs[torch.randint(0, n, (1 << 20,), device=dev, generator=gen)].contiguous()
     t0 = time.perf_counter()
     pt.train(samples, 0, seed=11)
     loss = pt.train(samples, 40000, 16384, 2e-3, seed=11)
     _sync()
     t_train = time.perf_counter() - t0
-    cert = {tol: pt.certify(tol, 2048) for tol in (1e-3, 1e-4)}     # the 1e-4 certificate stays in force
-    nf = pt.update(h, *outs, mode="surrogate")
+    c3 = pt.certify(1e-3, 2048)
+    pt.certify(1e-4, 2048)     # the 1e-4 certificate stays in force
+    c5, c4 = pt.certified()
+    nf = pt.update(hs, *outs, mode="surrogate")
     err = max(float(((o - r).abs() / r.abs()).max()) for o, r in zip(outs, ref))
-    ms_s = timed(lambda: pt.update(h, *outs, mode="surrogate"), reps)
+    ms_s = timed(lambda: pt.update(hs, *outs, mode="surrogate"), reps)
+    nf4 = pt.update(hs, *outs[:4], mode="surrogate")
+    err4 = max(float(((o - r).abs() / r.abs()).max()) for o, r in zip(outs[:4], ref[:4]))
+    ms_s4 = timed(lambda: pt.update(hs, *outs[:4], mode="surrogate"), reps)
     # the network alone on every cell (what a fully certified field would cost)
     net_out = torch.empty(5 * n, dtype=torch.float64, device=dev)
-    ms_n = timed(lambda: pt.eval(h, net_out), max(3, reps // 3))
+    ms_n = timed(lambda: pt.eval(hs, net_out), max(3, reps // 3))
     del net_out
     out["surrogate"] = {"ms": round(ms_s, 4), "gain_vs_table": round(ms_t / ms_s, 3), "train_s": round(t_train, 2),
-                        "train_steps": 40000, "final_loss": loss, "certified_fraction_tol1e-4": cert[1e-4],
-                        "certified_fraction_tol1e-3": cert[1e-3], "fallback_fraction": round(nf / n, 4),
-                        "max_rel_err_vs_table": err, "network_only_ms": round(ms_n, 4),
+                        "train_steps": 40000, "train_samples": "2^20 of the heated solver's ENTHALPY",
+                        "final_loss": loss, "certified_fraction_tol1e-4": c5, "certified_fraction_tol1e-3": c3,
+                        "certified_fraction_tol1e-4_without_cp": c4, "fallback_fraction": round(nf / n, 4),
+                        "max_rel_err_vs_table": err,
+                        "without_cp": {"table_ms": round(ms_t4, 4), "surrogate_ms": round(ms_s4, 4),
+                                       "gain_vs_table": round(ms_t4 / ms_s4, 3), "fallback_fraction": round(nf4 / n, 4),
+                                       "max_rel_err_vs_table": err4},
+                        "network_only_ms": round(ms_n, 4),
                         # model FLOPs (8960 per cell) and the matrix-core FLOPs executed (three BF16
                         # products per weight block, layer 3 padded to 16 rows: 30720 per cell)
                         "network_model_tflops": round(8960.0 * n / (ms_n * 1e-3) / 1e12, 2),
                         "network_mfma_tflops": round(30720.0 * n / (ms_n * 1e-3) / 1e12, 2),
                         "mfma_peak_bf16_tflops": 2500.0}
-    # energy substep (iso-flux walls, q_w = 0.1, Pr = 1, Re = 2800) on this grid
-    s = ChannelSolver(g, stream=stream)
-    ts = HeatedChannel(s, pt, thermal=1, nflow=1, flowdrvtp=1, gravdir=0)
-    L, M, N = g.L, g.M, g.N
-    yc = np.concatenate([[-1.0], np.asarray(g.ycc), [1.0]])
-    xx = np.arange(L) / L * 2 * np.pi
-    zz = np.arange(N) / N * 2 * np.pi
-    T3 = 1.2 - 0.3 * yc[None, :, None] ** 2 + 0.02 * np.sin(xx)[:, None, None] * np.cos(zz)[None, None, :]
-    hg = torch.from_numpy(np.interp(T3.ravel(order="F"), sc["T"], sc["H"])).to(dev)
-    f = ts.dev
-    f["enthalpy"].copy_(hg)
-    pt.update(hg, f["temperature"], f["density"], f["viscosity"], f["thermcondt"])
-    f["rhoh"].copy_(hg * f["density"])
-    for k in ("density0", "density1"):
-        f[k].copy_(f["density"])
-    f["viscosity0"].copy_(f["viscosity"])
-    qd = torch.from_numpy(np.asfortranarray(q_fp64).ravel(order="F")).to(dev)
-    f["g"].copy_(qd)
-    f["g0"].copy_(qd)
-    f["wallflux"].fill_(0.1)
-    seq = [1, 2, 3]
     cnt = [0]
 
     def eng():
-        ts.solverrk3_eng_io(seq[cnt[0] % 3], 1e-3, 1.0 / 2800.0)
+        ts.solverrk3_eng_io(seq[cnt[0] % 3], dt, cthecd)
         cnt[0] += 1
     ms_e = timed(eng, reps)
-    out["energy_substep"] = dict(ms=round(ms_e, 4), substeps_per_s=round(1e3 / ms_e, 2),
+    sur_e = None
+    if c4 > 0.0:
+        pt.set_mode("surrogate")
+        ms_es = timed(eng, reps)
+        sur_e = dict(ms=round(ms_es, 4), fallback_cells=pt.last_fallback())
+        pt.set_mode("table")
+    out["energy_substep"] = dict(ms=round(ms_e, 4), substeps_per_s=round(1e3 / ms_e, 2), surrogate_mode=sur_e,
                                  note="SOLVERRK3_ENG_IO: RHS + RK3 + property cascade + iso-flux walls, synchronous")
     # the whole heated-channel substep in SOLVE's order: energy, property
     # staging, io momentum (explicit viscous terms, FFT99 projection)
 
     def heated():
-        ts.substep(seq[cnt[0] % 3], 1e-3, 1.0 / 2800.0)
+        ts.substep(seq[cnt[0] % 3], dt, cthecd)
         cnt[0] += 1
 
     def mom():
         ns = seq[cnt[0] % 3]
         ts.io_prepare(ns)
-        ts.solverrk3_mom_io(ns, 1e-3)
+        ts.solverrk3_mom_io(ns, dt)
         cnt[0] += 1
     cnt[0] = 0
     ms_h = timed(heated, reps)
@@ -309,8 +348,37 @@ def props_bench(g, q_fp64, stream, steps):
                                  note="SOLVE order: SOLVERRK3_ENG_IO, DENSITY_TIME_DERIVATION + DENSITY/MU_Staggered"
                                       " + VELOCITY_CALC_io, SOLVERRK3_MOM_IO (explicit viscous, FFT99 projection)")
     _note("heated substep done")
-    s.close()
-    pt.close()
+    # the mixed-precision heated form beside the FP64 one: both continued
+    # from the common snapshot (3 substeps) by the same 30 substeps
+    sm, ptm, tsm = twins["mixed"]
+    s64b = _heated_start(g, sc, q_fp64, stream)["fp64"]
+    s64b_s, s64b_p, s64b_t = s64b
+    for i in range(3):
+        s64b_t.substep(seq[i % 3], dt, cthecd)
+    cnt[0] = 0
+
+    def heated_m():
+        tsm.substep(seq[cnt[0] % 3], dt, cthecd)
+        cnt[0] += 1
+    ms_hm = timed(heated_m, reps)
+    nmix = 1 + reps
+    for i in range(nmix):
+        s64b_t.substep(seq[i % 3], dt, cthecd)
+    dgm, _ = tsm.divgck_io()
+    a, b = s64b_t.download(), tsm.download()
+    rel = lambda x, y: float(np.abs(x - y).max() / max(np.abs(x).max(), 1e-300))
+    errs = {k: float(f"{rel(a[k], b[k]):.3e}") for k in ("temperature", "density", "enthalpy", "g", "pr")}
+    mx = {k: float(f"{v:.3e}") for k, v in zip(
+        ("T_mean", "rho_mean", "h_mean"),
+        [rel(ma[:, c], mb[:, c]) for ma, mb in [(s64b_t.mean_xz_io(), tsm.mean_xz_io())] for c in (22, 23, 24)])}
+    out["heated_mixed"] = dict(ms=round(ms_hm, 4), speedup_vs_fp64=round(ms_h / ms_hm, 3), max_div_g=dgm,
+                               substeps_compared=3 + nmix, err_vs_fp64=errs, mean_profile_rel_vs_fp64=mx,
+                               note="heated substep on the mixed-precision handle (FP32 Poisson refined, "
+                                    "FP64 io stencils) vs the FP64 handle from the same snapshot")
+    _note("heated mixed done")
+    for (s_, p_, _) in list(twins.values()) + [s64b]:
+        s_.close()
+        p_.close()
     return out
 
 

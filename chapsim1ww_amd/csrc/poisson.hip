@@ -407,7 +407,10 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
                                                           double dzqi, const double* __restrict__ amph,
                                                           const double* __restrict__ acph,
                                                           const double* __restrict__ apph, double* __restrict__ part) {
-    constexpr int RJ = 4;   // rows of j per thread and tile
+#ifndef CHS_RESID_RJ
+#define CHS_RESID_RJ 4
+#endif
+    constexpr int RJ = CHS_RESID_RJ;   // rows of j per thread and tile
     __shared__ double red[2][4];
     const int ni = (L + 63) / 64, nk = (N + 3) / 4, ng = (M + RJ - 1) / RJ;
     const int nt = ni * ng * nk, per = (nt + 7) / 8;

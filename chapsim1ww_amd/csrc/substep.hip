@@ -19,6 +19,7 @@
 #include <cmath>
 #include <list>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "chs_common.hpp"
@@ -174,14 +175,17 @@ __device__ __forceinline__ RowCoefT<T> row_coef(const Geo& g, int j) {
 // RHS_CvLpGpS_tg (src/RHS_CvLpGpS_tg.f90:117-176) at one cell for u, v, w:
 // conv[c] = H_c (the new CONVH0), rhs[c] = the x-solve right-hand side.
 // h0[c] is the cell's CONVH0 of the previous substep, P accesses PR.
-template <class A, class T = double>
+// The pressure gradient is formed in FP64 from FP64 pressures (AP), whatever
+// T: the pressure carries a large common level whose FP32 rounding would
+// swamp its differences.
+template <class A, class T = double, class AP = A>
 __device__ __forceinline__ void rhs_cell(const Geo& g, const RkCoef& rk, double cvisc, const RowCoefT<T>& rc,
-                                         const A& U, const A& V, const A& W, const A& P, const Nb& n,
+                                         const A& U, const A& V, const A& W, const AP& P, const Nb& n,
                                          const T* h0, T* conv, T* rhs) {
     const T COE1 = T(rk.talp * cvisc);
-    const T COE2 = T(rk.talp * rk.dt);
+    const double COE2 = rk.talp * rk.dt;
     const T tgam = T(rk.tgam), troh = T(rk.troh), dt = T(rk.dt);
-    const T dxqi = T(g.dxqi), dzqi = T(g.dzqi), dxi = T(g.dxi), dzi = T(g.dzi);
+    const T dxqi = T(g.dxqi), dzqi = T(g.dzqi);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const A& X = c == 0 ? U : c == 1 ? V : W;
@@ -193,14 +197,14 @@ __device__ __forceinline__ void rhs_cell(const Geo& g, const RkCoef& rk, double 
                                (X(0, n.kp) - T(2.0) * qc + X(0, n.km)) * dzqi * T(1.0) +
                                (X(1, 0) * rc.ap[c] + qc * rc.ac[c] + X(-1, 0) * rc.am[c]));
         const T r = (RHSC + RHSL) * dt;
-        T pgm;
+        double pgm;
         if (c == 0)
-            pgm = (P(0, 0) - P(0, n.im)) * dxi * COE2;
+            pgm = (P(0, 0) - P(0, n.im)) * g.dxi * COE2;
         else if (c == 1)
-            pgm = (P(0, 0) - P(-1, 0)) * (rc.y.dyci / T(1.0)) * COE2;
+            pgm = (P(0, 0) - P(-1, 0)) * (double(rc.y.dyci) / 1.0) * COE2;
         else
-            pgm = (P(0, 0) - P(0, n.km)) * dzi * COE2;
-        rhs[c] = r - pgm;
+            pgm = (P(0, 0) - P(0, n.km)) * g.dzi * COE2;
+        rhs[c] = r - T(pgm);
     }
 }
 
@@ -413,11 +417,11 @@ constexpr int XM_KP = KRS + 2;  // k rows of a ring plane
 constexpr int XM_PD = 1;        // steps the ring's next rows are loaded ahead
 constexpr int XM_CPT = 1;       // cells of a row per thread (1 or 2)
 
-// LDS of k_rhs_xm in compute type T: the ring, the PR plane and the x-solve
-// rows in T, then 16 doubles of block sums
+// LDS of k_rhs_xm in compute type T: the ring and the x-solve rows in T,
+// then (8-byte aligned) the PR plane in FP64 and 16 doubles of block sums
 inline size_t rhs_xm_smem(int L, int KX, size_t tsize = sizeof(double)) {
-    size_t b = tsize * ((size_t)9 * XM_KP * L + (size_t)(KRS + 1) * L + (size_t)3 * KRS * (L + 2 * KX));
-    return ((b + 7) & ~(size_t)7) + 16 * sizeof(double);
+    size_t b = tsize * ((size_t)9 * XM_KP * L + (size_t)3 * KRS * (L + 2 * KX));
+    return ((b + 7) & ~(size_t)7) + sizeof(double) * ((size_t)(KRS + 1) * L + 16);
 }
 
 // four consecutive values (16-byte aligned for float, 32 for double)
@@ -432,6 +436,9 @@ __device__ __forceinline__ void store4(float* dst, const float* v) {
 // T: the compute type of the cells' right-hand sides, the LDS ring and the x
 // solve (double: the FP64 form; float: the mixed form's FP32 stencils, whose
 // RHS leaves in FP32).  Q, PR and CONVH0 stay FP64 in HBM.
+#ifndef CHS_XM32_P64
+#define CHS_XM32_P64 1
+#endif
 template <int KX, int PD, int CPT, class T = double>
 __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
                                                const double* __restrict__ PR, double* __restrict__ H0,
@@ -441,15 +448,17 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     constexpr int RB = KRS / CPT;  // thread rows
     extern __shared__ double smd[];
     T* sm = reinterpret_cast<T*>(smd);
+    // the pressure values: FP64, or T (CHS_XM32_P64 = 0)
+    using TP = typename std::conditional<CHS_XM32_P64 != 0, double, T>::type;
     constexpr int KR = KRS, KP = XM_KP, W = 2 * KX + 1;
     const int L = g.L, M = g.M, N = g.N;
     const int PL = KP * L;            // one component of one ring row
     const int SL = 3 * PL;            // one ring row (u, v, w)
     T* ring = sm;                     // 3 ring rows
-    T* pln = sm + 3 * SL;             // PR row J, k0-1 .. k0+KR-1
     const int LS = L + 2 * KX, Q4 = LS >> 2;
-    T* xr = pln + (KR + 1) * L;       // 3*KR x-solve rows (row_pos layout)
-    double* red = smd + (sizeof(T) * (size_t)(9 * KP * L + (KR + 1) * L + 3 * KR * LS) + 7) / 8;
+    T* xr = sm + 3 * SL;              // 3*KR x-solve rows (row_pos layout)
+    TP* pln = reinterpret_cast<TP*>(smd + (sizeof(T) * (size_t)(9 * KP * L + 3 * KR * LS) + 7) / 8);   // PR row J, k0-1 .. k0+KR-1
+    double* red = smd + (sizeof(T) * (size_t)(9 * KP * L + 3 * KR * LS) + 7) / 8 + (KR + 1) * L;
     const int nt = (N / KR) * nyc;
     const int per = (nt + 7) / 8;
     const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD x: tiles x*per .. (x+1)*per-1
@@ -498,18 +507,19 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     }
     {
         const int row = L * (j0 + 1);
-        T a[CPT];
+        TP a[CPT];
 #pragma unroll
-        for (int u = 0; u < CPT; ++u) a[u] = T(PR[gc[u] + row]);
-        const T ah = T(PR[gph + row]);
+        for (int u = 0; u < CPT; ++u) a[u] = TP(PR[gc[u] + row]);
+        const TP ah = TP(PR[gph + row]);
 #pragma unroll
         for (int u = 0; u < CPT; ++u) pln[lc[u]] = a[u];
         pln[lph] = ah;
     }
-    T pm[CPT], h0[CPT][3];  // own PR at row J-1, own CONVH0 of row j
+    TP pm[CPT];  // own PR at row J-1
+    T h0[CPT][3];    // own CONVH0 of row j
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
-        pm[u] = T(PR[gc[u] + L * j0]);
+        pm[u] = TP(PR[gc[u] + L * j0]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) h0[u][c] = T(H0[i + L * j0 + L * M * kc[u] + c * LMN]);
     }
@@ -524,7 +534,8 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     // the CONVH0 row j+1 of the next step.  Rows past the chunk's end are
     // clamped into the array: loaded, never used.
     struct Pre {
-        T q[CPT][3], qh[3], h[CPT][3], p[CPT], ph;
+        T q[CPT][3], qh[3], h[CPT][3];
+        TP p[CPT], ph;
     };
     // CONVH0 is loaded first: the wait for the Q rows at the ring store then
     // covers it, so that using it in the next step waits for nothing (the
@@ -536,9 +547,9 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
         for (int u = 0; u < CPT; ++u) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) x.h[u][c] = T(H0[i + rh + L * M * kc[u] + c * LMN]);
-            x.p[u] = T(PR[gc[u] + rp]);
+            x.p[u] = TP(PR[gc[u] + rp]);
         }
-        x.ph = T(PR[gph + rp]);
+        x.ph = TP(PR[gph + rp]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
 #pragma unroll
@@ -559,7 +570,7 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
         const RowCoefT<T> rc = row_coef<T>(g, j);
         const double dyfj = ldc(g.dyfi, j);
         double v = 0.0;
-        T pc[CPT];
+        TP pc[CPT];
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
             const int o = lc[u];
@@ -567,9 +578,9 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
             const RColT<T> V{U.m + PL, U.c + PL, U.p + PL};
             const RColT<T> Wc{V.m + PL, V.c + PL, V.p + PL};
             pc[u] = pln[o];
-            const RColT<T> P{&pm[u], pln + o, nullptr};
+            const RColT<TP> P{&pm[u], pln + o, nullptr};
             T conv[3], rhs[3];
-            rhs_cell<RColT<T>, T>(g, rk, cvisc, rc, U, V, Wc, P, n, h0[u], conv, rhs);
+            rhs_cell<RColT<T>, T, RColT<TP>>(g, rk, cvisc, rc, U, V, Wc, P, n, h0[u], conv, rhs);
             const int ci = i + L * j + L * M * kc[u];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -989,7 +1000,8 @@ __device__ __forceinline__ double nanmax(double a, double b) { return a != a ? a
 // r = f - L_h phi of the refined solution from the DPH values the update
 // loads plus phi(i+1), phi(k+1) and phi(J+1) (wall rows: the Neumann rule of
 // the solve), reduced per tile to part[2 tile] = max|r|, part[2 tile + 1] =
-// max|f| (NaN propagates).  alpha dt max|r| is the projected max|div u|.
+// max|f| (NaN propagates); k_max2_tiles then reduces the tiles.
+// alpha dt max|r| is the projected max|div u|.
 struct ResidCoef {
     const double *amph, *acph, *apph;
     double* part;
@@ -1193,6 +1205,43 @@ __global__ void k_reduce_max2_nan(const double* __restrict__ part, int nb, doubl
     if (threadIdx.x == 0) {
         out[0] = red[0][0];
         out[1] = red[1][0];
+    }
+}
+
+// Maxima (NaN-propagating) of n pairs part[2t], part[2t+1]: every block of
+// 256 threads reduces a contiguous range of pairs, four independent loads in
+// flight per thread, to out2[2 blockIdx.x + 0/1] (a fixed order: the result
+// does not depend on timing); k_reduce_max2_nan then reduces the blocks.
+__global__ void __launch_bounds__(256) k_max2_tiles(const double* __restrict__ part, int n, double* __restrict__ out2) {
+    __shared__ double red[2][256];
+    const int per = (n + gridDim.x - 1) / gridDim.x;
+    const int t0 = blockIdx.x * per, t1 = min(n, t0 + per);
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    int t = t0 + threadIdx.x;
+    for (; t + 768 < t1; t += 1024) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a[q] = nanmax(a[q], part[2 * (t + 256 * q)]);
+            b[q] = nanmax(b[q], part[2 * (t + 256 * q) + 1]);
+        }
+    }
+    for (; t < t1; t += 256) {
+        a[0] = nanmax(a[0], part[2 * t]);
+        b[0] = nanmax(b[0], part[2 * t + 1]);
+    }
+    red[0][threadIdx.x] = nanmax(nanmax(a[0], a[1]), nanmax(a[2], a[3]));
+    red[1][threadIdx.x] = nanmax(nanmax(b[0], b[1]), nanmax(b[2], b[3]));
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] = nanmax(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = nanmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out2[2 * blockIdx.x] = red[0][0];
+        out2[2 * blockIdx.x + 1] = red[1][0];
     }
 }
 
@@ -1675,12 +1724,17 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_LAUNCH(k_divg, dim3(row_tile_blocks(h->L, h->N, (h->M + DVJ - 1) / DVJ)), dim3(256), 0, s, g,
                1.0 / (dt * rk.talp), h->q, h->rhsllphi);
     h->timer.end(ev, "divg", s);
-    // 5. Poisson + BC_WALL_DPH.  FP64: one pass.  Mixed: the FP32 pass and one
-    //    FP64-residual refinement (an FP32 pass leaves ~1e-7 of max|f|, one
-    //    refinement ~1e-12); the projection below checks that the projected
-    //    divergence alpha*dt*max|f - L phi| stays below half the 1e-10 target.
+    // 5. Poisson + BC_WALL_DPH.  FP64: one pass.  Mixed, NS >= 1: the FP32
+    //    pass and one FP64-residual refinement (an FP32 pass leaves ~1e-7 of
+    //    max|f|, one refinement ~1e-12); the projection below checks that the
+    //    projected divergence alpha*dt*max|f - L phi| stays below half the
+    //    1e-10 target, and the host reads the verdict without waiting.
+    // NS = 0 starts from a field of arbitrary divergence: refine there with a
+    // check before each further pass (poisson_solve), as many passes as needed
     const int ntile = (int)(((h->L + 63) / 64) * ((h->N + 3) / 4) * (long)((h->M + UPJ - 1) / UPJ));
-    const bool fused_chk = h->precision == CHS_MIXED && h->refine_max > 0 && ntile <= h->npart;
+    constexpr int NB2 = 256;   // first-stage blocks of the tile reduction
+    const bool fused_chk = h->precision == CHS_MIXED && h->refine_max > 0 && ns != 0 &&
+                           ntile + NB2 <= h->npart;
     if (fused_chk)
         CHS_TRY(poisson_solve_unchecked(h, h->rhsllphi, h->dph, 1));
     else
@@ -1697,8 +1751,10 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
                    0.5 * rk.talp * dt * h->cvisc, h->dph, h->rhsllphi, h->q, h->pr, rcf);
     h->timer.end(ev, "update", s);
     if (fused_chk) {
-        CHS_LAUNCH(k_reduce_max2_nan, dim3(1), dim3(256), 0, s, h->partials, ntile, h->scalars + 4);
-        CHS_HIP(hipMemcpyAsync(h->chk_host, h->scalars + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        double* p2 = h->partials + 2 * (size_t)ntile;   // after the tile pairs
+        CHS_LAUNCH(k_max2_tiles, dim3(NB2), dim3(256), 0, s, h->partials, ntile, p2);
+        CHS_LAUNCH(k_reduce_max2_nan, dim3(1), dim3(256), 0, s, p2, NB2, h->scalars + 6);
+        CHS_HIP(hipMemcpyAsync(h->chk_host, h->scalars + 6, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         CHS_HIP(hipEventRecord(h->chk_ev, s));
         h->chk_tol = 0.5e-10 / (rk.talp * dt);
         h->chk_pending = true;

@@ -137,9 +137,12 @@ __global__ void k_tdma_seq(const double* __restrict__ A, const double* __restric
 // lpb_small: blocks of 8 lines (several blocks per CU overlap their load,
 // reduce and store phases; measured faster for the complex-valued Poisson
 // lines, slower for the real batched solver)
+#ifndef CHS_TDMA_LPB
+#define CHS_TDMA_LPB 64
+#endif
 bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, bool lpb_small) {
     const int cand[4] = {8, 12, 16, 4};
-    const int lpbs_big[2] = {64, 32}, lpbs_small[2] = {8, 8};
+    const int lpbs_big[2] = {CHS_TDMA_LPB, CHS_TDMA_LPB == 64 ? 32 : CHS_TDMA_LPB}, lpbs_small[2] = {8, 8};
     const int* lpbs = lpb_small ? lpbs_small : lpbs_big;
     for (int ms : cand) {
         for (int q = 0; q < 2; ++q) {
