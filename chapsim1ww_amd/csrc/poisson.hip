@@ -227,11 +227,14 @@ __global__ void __launch_bounds__(16 * CB) k_fft256_z(cpx_t<Rl>* __restrict__ S,
     }
 }
 
-// x inverse, L = 256: 16 rows per block of 128 threads; ghost as k_fft_x_inv
+// x inverse, L = 256: 16 rows per block of 128 threads; ghost as k_fft_x_inv,
+// and two modes of the mixed form's refinement, on phf (FP32, interior rows,
+// the layout of f): 3 = write phf only (the first pass's solution), 4 = dph
+// (interior rows) = phf + the transform (the correction added to it)
 template <typename Rl>
 __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restrict__ S, double* __restrict__ dph,
                                                       int M, int nrows, int LHP, int ghost,
-                                                      const cpx_t<Rl>* __restrict__ tw) {
+                                                      const cpx_t<Rl>* __restrict__ tw, float* __restrict__ phf) {
     using C = cpx_t<Rl>;
     extern __shared__ unsigned char smem_raw[];
     C* sh = reinterpret_cast<C*>(smem_raw);
@@ -261,6 +264,22 @@ __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restric
         double* base = dph + (size_t)k * plane + q;     // column q of plane k
         double* rowp = base + (size_t)(jj + 1) * 256;
         const bool lo = jj == 0, hi = jj == M - 1;
+        if (ghost == 3) {
+            float* fr = phf + (size_t)row * 256 + q;
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) fr[16 * k2] = (float)(h ? v[k2].y : v[k2].x);
+            continue;
+        }
+        if (ghost == 4) {   // all 16 FP32 values loaded before the first store
+            const float* fr = phf + (size_t)row * 256 + q;
+            float old[16];
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) old[k2] = fr[16 * k2];
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2)
+                rowp[16 * k2] = (double)old[k2] + (h ? (double)v[k2].y : (double)v[k2].x);
+            continue;
+        }
         if (ghost == 2) {   // add to phi: all 16 old values loaded before the first add
             double old[16];
 #pragma unroll
@@ -407,10 +426,7 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
                                                           double dzqi, const double* __restrict__ amph,
                                                           const double* __restrict__ acph,
                                                           const double* __restrict__ apph, double* __restrict__ part) {
-#ifndef CHS_RESID_RJ
-#define CHS_RESID_RJ 4
-#endif
-    constexpr int RJ = CHS_RESID_RJ;   // rows of j per thread and tile
+    constexpr int RJ = 4;   // rows of j per thread and tile
     __shared__ double red[2][4];
     const int ni = (L + 63) / 64, nk = (N + 3) / 4, ng = (M + RJ - 1) / RJ;
     const int nt = ni * ng * nk, per = (nt + 7) / 8;
@@ -480,6 +496,115 @@ __global__ void __launch_bounds__(256) k_poisson_residual(const double* __restri
     }
 }
 
+// The residual r = f - L_h(phi) of k_poisson_residual, formed row by row and
+// transformed in x at once (L = 256): the refinement pass's first step, with
+// no r array written and read back.  Rows and threads as k_fft256_x_fwd: 16
+// rows (row = j + M k) per block of 128 threads, thread (g, q) holding
+// elements i = 16 n1 + q of rows 2g and 2g + 1.
+// v of lane (q - R) mod 16 of the lane's 16-lane row (DPP row_ror:R)
+template <int R>
+__device__ __forceinline__ float row_rot(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + R, 0xF, 0xF, false));
+}
+
+template <typename Rl>
+__global__ void __launch_bounds__(128) k_fft256_x_fwd_res(const float* __restrict__ phi,
+                                                          const double* __restrict__ f, cpx_t<Rl>* __restrict__ S,
+                                                          int M, int nrows, int LHP, const cpx_t<Rl>* __restrict__ tw,
+                                                          double dxqi, double dzqi, const double* __restrict__ amph,
+                                                          const double* __restrict__ acph,
+                                                          const double* __restrict__ apph) {
+    using C = cpx_t<Rl>;
+    extern __shared__ unsigned char smem_raw[];
+    C* sh = reinterpret_cast<C*>(smem_raw);
+    const int g = threadIdx.x >> 4, q = threadIdx.x & 15;
+    const int N = nrows / M;
+    const int LP = 256 * M;   // phi: interior rows only, the layout of f
+    // per row: centre offset, y and z neighbour offsets (the wall rule as
+    // k_poisson_residual: the y neighbour of a wall row is the row itself)
+    int cc[2], jp[2], jm[2], kp[2], km[2];
+    double ap[2], ac[2], am[2];
+    bool ok[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int r = blockIdx.x * 16 + 2 * g + s;
+        ok[s] = r < nrows;
+        const int rr = ok[s] ? r : nrows - 1;
+        const int j = rr % M, k = rr / M;
+        cc[s] = 256 * rr;
+        jp[s] = j == M - 1 ? 0 : 256;
+        jm[s] = j == 0 ? 0 : -256;
+        kp[s] = k == N - 1 ? (1 - N) * LP : LP;
+        km[s] = k == 0 ? (N - 1) * LP : -LP;
+        ap[s] = ldc(apph, j);
+        ac[s] = ldc(acph, j);
+        am[s] = ldc(amph, j);
+    }
+    // rows 2g and 2g + 1 in one plane (always when M is even): each is the
+    // other's y neighbour
+    const bool pair = ok[1] && jp[0] != 0;
+    // the centre values of both rows, a window of two n1 at a time; the x
+    // neighbours come from the other lanes of the 16-lane row by DPP
+    // rotations (lane q holds i = 16 n1 + q: i - 1 is lane q - 1, or lane 15
+    // at n1 - 1 for q = 0; i + 1 is lane q + 1, or lane 0 at n1 + 1 for q = 15)
+    float cn[2], dprev[2], u0[2], ucur[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        cn[s] = phi[cc[s] + q];
+        dprev[s] = row_rot<1>(phi[cc[s] + 240 + q]);
+        u0[s] = row_rot<15>(cn[s]);
+        ucur[s] = u0[s];
+    }
+    C v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int i = 16 * n1 + q;
+        float nx[2];
+        double rv[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) nx[s] = n1 < 15 ? phi[cc[s] + i + 16] : 0.0f;
+        const float pn0 = pair ? cn[1] : phi[cc[0] + jp[0] + i];
+        const float ps1 = pair ? cn[0] : phi[cc[1] + jm[1] + i];
+        const float ps0 = phi[cc[0] + jm[0] + i];
+        const float pn1 = phi[cc[1] + jp[1] + i];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const double pc = cn[s];
+            const float dcur = row_rot<1>(cn[s]);
+            const double pw = q == 0 ? dprev[s] : dcur;
+            dprev[s] = dcur;
+            const float unext = n1 < 15 ? row_rot<15>(nx[s]) : u0[s];
+            const double pe = q == 15 ? unext : ucur[s];
+            ucur[s] = unext;
+            const double pjp = s == 0 ? pn0 : pn1, pjm = s == 0 ? ps0 : ps1;
+            const double lx = (pe - 2.0 * pc + pw) * dxqi;
+            const double ly = pjp * ap[s] + pc * ac[s] + pjm * am[s];
+            const double lz = ((double)phi[cc[s] + kp[s] + i] - 2.0 * pc + (double)phi[cc[s] + km[s] + i]) * dzqi;
+            rv[s] = ok[s] ? f[cc[s] + i] - (lx + ly + lz) : 0.0;
+        }
+        cn[0] = nx[0];
+        cn[1] = nx[1];
+        v[n1] = C{(Rl)rv[0], (Rl)rv[1]};
+    }
+    C* buf = sh + g * F16T;
+    fft256_regs<C, Rl>(v, q, buf, tw, -1);
+    __syncthreads();   // every thread has read its exchange column
+    C* Z = sh + g * 256;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) Z[q + 16 * k2] = v[k2];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 8 * 129; e += blockDim.x) {
+        const int p = e / 129, kx = e - p * 129;
+        const C z = sh[p * 256 + kx];
+        const C zc = sh[p * 256 + ((256 - kx) & 255)];
+        const C A = C{Rl(0.5) * (z.x + zc.x), Rl(0.5) * (z.y - zc.y)};
+        const C B = C{Rl(0.5) * (z.y + zc.y), Rl(-0.5) * (z.x - zc.x)};
+        const int r0 = blockIdx.x * 16 + 2 * p;
+        if (r0 < nrows) S[(size_t)r0 * LHP + kx] = A;
+        if (r0 + 1 < nrows) S[(size_t)(r0 + 1) * LHP + kx] = B;
+    }
+}
+
 // Max over the block partials written by k_poisson_residual -> out[0..1]
 __global__ void k_reduce_max2(const double* __restrict__ part, int nb, double* __restrict__ out) {
     __shared__ double red[2][256];
@@ -516,7 +641,10 @@ bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, 
 
 template <typename Rl, typename TI = double>
 static int poisson_pass(chs_handle* h, const TI* f, double* dph, int ghost, cpx_t<Rl>* S, const cpx_t<Rl>* twx,
-                        const cpx_t<Rl>* twz) {
+                        const cpx_t<Rl>* twz, const float* rphi = nullptr, float* phf = nullptr) {
+    // L = 256 only: rphi transforms the residual f - L_h(rphi) instead of f
+    // (rphi: FP32 values of the interior rows); phf is the FP32 solution of
+    // ghost modes 3 and 4 (k_fft256_x_inv)
     // FP32 spectra use rows of a multiple of 16 values (128-byte aligned rows)
     constexpr bool f32 = sizeof(Rl) == 4;
     constexpr int ZCB = f32 ? 16 : 8;   // kx columns per z block: 128 bytes
@@ -535,12 +663,19 @@ static int poisson_pass(chs_handle* h, const TI* f, double* dph, int ghost, cpx_
     const size_t sm16x = sizeof(cpx_t<Rl>) * (size_t)8 * F16T;
     const size_t sm16z = sizeof(cpx_t<Rl>) * (size_t)ZCB * (F16T + 1);
     const dim3 gz16((LH + ZCB - 1) / ZCB, M);
-    h->timer.begin("fft_x_fwd", s, &ev);
-    if (x256)
+    if ((rphi != nullptr || ghost >= 3) && !x256) {
+        set_error("poisson: the fused residual transform needs NCL1 = 256");
+        return CHS_E_ARG;
+    }
+    h->timer.begin(rphi ? "resid_fft_x" : "fft_x_fwd", s, &ev);
+    if (rphi != nullptr)
+        CHS_LAUNCH((k_fft256_x_fwd_res<Rl>), gx16, dim3(128), sm16x, s, rphi, (const double*)f, S, M, nrows, LHP,
+                   twx, h->dxqi, h->dzqi, h->amph, h->acph, h->apph);
+    else if (x256)
         CHS_LAUNCH((k_fft256_x_fwd<Rl, TI>), gx16, dim3(128), sm16x, s, f, S, nrows, LHP, twx);
     else
         CHS_LAUNCH((k_fft_x_fwd<Rl, TI>), gx, dim3(256), smx, s, f, S, L, nrows, LHP, px, twx);
-    h->timer.end(ev, "fft_x_fwd", s);
+    h->timer.end(ev, rphi ? "resid_fft_x" : "fft_x_fwd", s);
     const dim3 gz((LH + ZTX - 1) / ZTX, M);
     const size_t smz = sizeof(cpx_t<Rl>) * (size_t)2 * ZTX * N;
     h->timer.begin("fft_z", s, &ev);
@@ -549,8 +684,26 @@ static int poisson_pass(chs_handle* h, const TI* f, double* dph, int ghost, cpx_
     else
         CHS_LAUNCH(k_fft_z<Rl>, gz, dim3(256), smz, s, S, N, M, LH, LHP, -1, pz, twz);
     h->timer.end(ev, "fft_z", s);
-    int MS, P, LPB;
-    if (!tdma_seg_config(M, false, &MS, &P, &LPB, 768, true)) {
+    // FP32 spectra: segments of 12 rows and 16 lines per block (measured
+    // fastest at NCL2 = 192: 16-row segments spill registers, 8-row segments
+    // double the reduced system; 8 lines of 16-row segments above NCL2 = 576).
+    // FP64: the segmentation of the batched TDMA with 8 lines per block, whose
+    // roundings the FP64 parity tests pin.
+    int MS = 0, P = 0, LPB = 0;
+    if (f32) {
+        const int cfg[4][2] = {{12, 16}, {8, 16}, {16, 8}, {16, 4}};
+        for (const auto& c : cfg) {
+            if ((M + c[0] - 1) / c[0] * c[1] <= 768) {
+                MS = c[0];
+                P = (M + c[0] - 1) / c[0];
+                LPB = c[1];
+                break;
+            }
+        }
+    } else if (!tdma_seg_config(M, false, &MS, &P, &LPB, 768, true)) {
+        MS = 0;
+    }
+    if (MS == 0) {
         set_error("poisson: unsupported NCL2");
         return CHS_E_ARG;
     }
@@ -574,7 +727,7 @@ static int poisson_pass(chs_handle* h, const TI* f, double* dph, int ghost, cpx_
     h->timer.end(ev, "fft_z", s);
     h->timer.begin("fft_x_inv", s, &ev);
     if (x256)
-        CHS_LAUNCH(k_fft256_x_inv<Rl>, gx16, dim3(128), sm16x, s, S, dph, M, nrows, LHP, ghost, twx);
+        CHS_LAUNCH(k_fft256_x_inv<Rl>, gx16, dim3(128), sm16x, s, S, dph, M, nrows, LHP, ghost, twx, phf);
     else
         CHS_LAUNCH(k_fft_x_inv<Rl>, gx, dim3(256), smx, s, S, dph, L, M, nrows, LHP, ghost, px, twx);
     h->timer.end(ev, "fft_x_inv", s);
@@ -598,13 +751,21 @@ __global__ void k_dph_ghost(double* __restrict__ dph, int L, int M, int N, int m
 int poisson_solve_unchecked(chs_handle* h, const double* f, double* dph, int ghost) {
     const hipStream_t s = h->stream;
     float* rf = reinterpret_cast<float*>(h->resid);   // the FP32 residual (first half of the buffer)
-    CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
-    hipEvent_t ev;
-    h->timer.begin("poisson_residual", s, &ev);
-    CHS_LAUNCH(k_poisson_residual<true>, dim3(2048), dim3(256), 0, s, dph, f, rf, h->L, h->M, h->N, h->dxqi,
-               h->dzqi, h->amph, h->acph, h->apph, h->partials);
-    h->timer.end(ev, "poisson_residual", s);
-    CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
+    if (h->L == 256 && ghost != 2) {
+        // the first pass's solution phi1 kept in FP32 (interior rows, in the
+        // residual buffer); the residual f - L_h(phi1) formed in FP64 inside
+        // the refinement pass's x transform; DPH = phi1 + the correction
+        CHS_TRY(poisson_pass<float>(h, f, dph, 3, h->spec32, h->twx32, h->twz32, nullptr, rf));
+        CHS_TRY((poisson_pass<float, double>(h, f, dph, 4, h->spec32, h->twx32, h->twz32, rf, rf)));
+    } else {
+        CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
+        hipEvent_t ev;
+        h->timer.begin("poisson_residual", s, &ev);
+        CHS_LAUNCH(k_poisson_residual<true>, dim3(2048), dim3(256), 0, s, dph, f, rf, h->L, h->M, h->N, h->dxqi,
+                   h->dzqi, h->amph, h->acph, h->apph, h->partials);
+        h->timer.end(ev, "poisson_residual", s);
+        CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
+    }
     if (ghost != 2) CHS_LAUNCH(k_dph_ghost, dim3(256), dim3(256), 0, s, dph, h->L, h->M, h->N, ghost);
     h->last_refine_iters = 1;
     h->last_refine_resid = -1.0;   // known once the substep's check has been read

@@ -19,7 +19,6 @@
 #include <cmath>
 #include <list>
 #include <mutex>
-#include <type_traits>
 #include <vector>
 
 #include "chs_common.hpp"
@@ -436,9 +435,6 @@ __device__ __forceinline__ void store4(float* dst, const float* v) {
 // T: the compute type of the cells' right-hand sides, the LDS ring and the x
 // solve (double: the FP64 form; float: the mixed form's FP32 stencils, whose
 // RHS leaves in FP32).  Q, PR and CONVH0 stay FP64 in HBM.
-#ifndef CHS_XM32_P64
-#define CHS_XM32_P64 1
-#endif
 template <int KX, int PD, int CPT, class T = double>
 __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
                                                const double* __restrict__ PR, double* __restrict__ H0,
@@ -448,8 +444,8 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     constexpr int RB = KRS / CPT;  // thread rows
     extern __shared__ double smd[];
     T* sm = reinterpret_cast<T*>(smd);
-    // the pressure values: FP64, or T (CHS_XM32_P64 = 0)
-    using TP = typename std::conditional<CHS_XM32_P64 != 0, double, T>::type;
+    // the pressure values stay FP64 (the gradient of a large common level)
+    using TP = double;
     constexpr int KR = KRS, KP = XM_KP, W = 2 * KX + 1;
     const int L = g.L, M = g.M, N = g.N;
     const int PL = KP * L;            // one component of one ring row

@@ -111,3 +111,41 @@ def test_rows_of_a_ghost_aware_array(scaled, ptab):
         got = a.cpu().numpy().reshape((L, M + 2, N), order="F")
         assert np.array_equal(got[:, 1:M + 1, :].ravel(order="F"), b)
         assert np.all(got[:, 0, :] == 5.0) and np.all(got[:, M + 1, :] == 5.0)
+
+
+@pytest.mark.parametrize("n,peak", [(4096, 4.0), (2048, 40.0), (512, 2000.0)])
+def test_uneven_knots_and_bucket_edges(n, peak):
+    """Tables whose H knots are spaced unevenly (c_p peak/base = 5 at 4096
+    knots, the largest count accepted; 41; 2001: long walks from a bucket's
+    first candidate): random values, every knot and its neighbours one ulp
+    away, and the edges of power-of-two bucket counts from n to 64 n and their
+    neighbours, bit for bit against the oracle's bisection."""
+    print(f"props uneven table n={n} peak={peak}", flush=True)
+    sc = props.scale_table(props.synthetic_table(n, peak=peak), t0=1.0)
+    t = props.PropertyTable(sc)
+    try:
+        H = sc["H"]
+        lo, hi = H[0], H[-1]
+        rng = np.random.default_rng(n)
+        vals = [rng.uniform(lo, hi, 200000), H[1:-1], np.nextafter(H[1:-1], np.inf),
+                np.nextafter(H[1:-1], -np.inf)]
+        for m in range(7):
+            nb = n << m
+            e = lo + np.arange(1, nb) * (hi - lo) / nb
+            vals += [e, np.nextafter(e, np.inf), np.nextafter(e, -np.inf)]
+        h = np.concatenate(vals)
+        h = h[(h > lo) & (h < hi)]
+        bad, ref = _oracle(sc, h, 0.0)
+        outs = [torch.zeros(h.size, dtype=torch.float64, device="cuda") for _ in range(5)]
+        from chapsim1ww_amd.native import ChsError
+        try:
+            nf = t.update(_dev(h), *outs)
+        except ChsError as e:
+            nf = int(str(e).split(" cells")[0].split()[-1])
+        torch.cuda.synchronize()
+        assert nf == bad
+        for name, a, b in zip(("T", "rho", "mu", "kap", "cp"), outs, ref):
+            got = a.cpu().numpy()
+            assert np.array_equal(got, b), (name, np.abs(got - b).max())
+    finally:
+        t.close()
