@@ -71,8 +71,9 @@ __global__ void k_zero_u64(unsigned long long* c, int n) {
 }
 
 constexpr int PT = 1024;     // threads per lookup block
-constexpr int PCELL = 4;     // cells per thread and chunk
+constexpr int PCELL = 8;     // cells per thread and chunk (measured: 4 -> 8, 0.252 -> 0.240 ms)
 
+template <int KW>
 __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const double* __restrict__ h,
                                                     double* __restrict__ To, double* __restrict__ rho,
                                                     double* __restrict__ mu, double* __restrict__ kap,
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
             // NaN and out-of-range values search at h_lo (a float-to-int
             // conversion of them would be undefined); their results are unused
             const double hq = hv[u] > t.h_lo && hv[u] < t.h_hi ? hv[u] : t.h_lo;
-            ih[u] = find_interval(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hq);
+            ih[u] = find_iv<KW>(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hq);
             rr[u] = reinterpret_cast<const double4*>(t.rh)[ih[u]];
         }
         double tv[PCELL];
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(PT) k_props_table(PropTab t, Rows rw, const do
 #pragma unroll
             for (int v = 0; v < 2; ++v) {
                 const double tq = tv[u0 + v] > t.t_lo && tv[u0 + v] < t.t_hi ? tv[u0 + v] : t.t_lo;
-                kt[v] = find_interval(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tq);
+                kt[v] = find_iv<KW>(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tq);
                 const double4* q = reinterpret_cast<const double4*>(t.rt) + 4 * (size_t)kt[v];
 #pragma unroll
                 for (int m = 0; m < 4; ++m) rq[v][m] = q[m];
@@ -190,9 +191,19 @@ int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double
     const long long nch = ((rw.rowlen + chunk - 1) / chunk) * rw.nrow;
     if (nch == 0) return CHS_OK;
     const int grid = props_grid(p, nch);
-    CHS_TRY(props_lds_attr((const void*)k_props_table, p));
-    CHS_LAUNCH(k_props_table, dim3(grid), dim3(PT), props_lds_bytes(p), stream, tab_of(p), rw, h, T, rho, mu,
-               kap, cp, cnt, count_lookups);
+#define CHS_PROPS_K(KK)                                                                                          \
+    do {                                                                                                         \
+        CHS_TRY(props_lds_attr((const void*)k_props_table<KK>, p));                                              \
+        CHS_LAUNCH(k_props_table<KK>, dim3(grid), dim3(PT), props_lds_bytes(p), stream, tab_of(p), rw, h, T, rho, \
+                   mu, kap, cp, cnt, count_lookups);                                                             \
+    } while (0)
+    switch (p->kwalk) {
+        case 2: CHS_PROPS_K(2); break;
+        case 3: CHS_PROPS_K(3); break;
+        case 4: CHS_PROPS_K(4); break;
+        default: CHS_PROPS_K(0); break;
+    }
+#undef CHS_PROPS_K
     return CHS_OK;
 }
 
@@ -258,9 +269,36 @@ int props_create(int n, const double* H, const double* T, const double* D, const
         }
         p->rhoh_max = best;
     }
+    // bucket counts: n each, doubled while a bucket holds more than two
+    // knots and the staged tables stay within 64 KB of LDS; the lookups then
+    // walk a fixed 1 + (most knots in a bucket) steps (kwalk <= 4), or to the
+    // end of the walk when the knots are too uneven for that
     std::vector<int> sh, st;
-    bucket_starts(n, H, p->nbh, p->h_lo, p->hbi, sh);
-    bucket_starts(n, T, p->nbt, p->t_lo, p->tbi, st);
+    auto most = [&](const std::vector<int>& start) {
+        int m = 0;
+        for (size_t b = 0; b < start.size(); ++b) {
+            const int nx = b + 1 < start.size() ? start[b + 1] : n - 2;
+            m = std::max(m, nx - start[b]);
+        }
+        return m;
+    };
+    int mh = 0, mt = 0;
+    for (;;) {
+        bucket_starts(n, H, p->nbh, p->h_lo, p->hbi, sh);
+        bucket_starts(n, T, p->nbt, p->t_lo, p->tbi, st);
+        mh = most(sh);
+        mt = most(st);
+        const bool gh = mh > 2, gt = mt > 2;
+        const size_t lds = sizeof(double) * 2 * (size_t)n +
+                           sizeof(int) * ((size_t)p->nbh * (gh ? 2 : 1) + (size_t)p->nbt * (gt ? 2 : 1));
+        if ((!gh && !gt) || lds > 65536) break;
+        if (gh) p->nbh *= 2;
+        if (gt) p->nbt *= 2;
+        p->hbi = (double)p->nbh / (p->h_hi - p->h_lo);
+        p->tbi = (double)p->nbt / (p->t_hi - p->t_lo);
+    }
+    const int kw = std::max(mh, mt) + 1;
+    p->kwalk = kw <= 4 ? std::max(kw, 2) : 0;
     auto fail = [&](int code) {
         chs_props_destroy(p);
         return code;

@@ -12,6 +12,7 @@ struct Mlp;
 struct chs_props {
     int n = 0;                 // knots of the scaled table
     int nbh = 0, nbt = 0;      // buckets of the H and T interval searches
+    int kwalk = 0;             // fixed walk steps that make both searches exact (0: walk to the end)
     double h_lo = 0, h_hi = 0, t_lo = 0, t_hi = 0;
     double hbi = 0, tbi = 0;   // 1 / bucket width
     double rhoh_max = 0;       // RHOHmax of NIST_SPLINE_H (the limiter of the energy update)
@@ -85,6 +86,31 @@ __device__ __forceinline__ int find_interval(const double* x, const int* start, 
     while (i < n - 2 && x[i + 1] <= u) ++i;
     while (i > 0 && x[i] > u) --i;
     return i;
+}
+
+// find_interval with a fixed number of steps and no branch: at most one step
+// back (the rounding of the bucket index) and K forward, K being one more
+// than the most knots any bucket holds (props_create checks it), so that the
+// searches of several cells interleave.
+template <int K>
+__device__ __forceinline__ int find_interval_k(const double* x, const int* start, int n, int nb, double lo,
+                                               double bi, double u) {
+    int b = (int)((u - lo) * bi);
+    b = b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+    int i = start[b];
+    i -= (i > 0 && x[i] > u) ? 1 : 0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) i += (i < n - 2 && x[i + 1] <= u) ? 1 : 0;
+    return i;
+}
+
+template <int K>
+__device__ __forceinline__ int find_iv(const double* x, const int* start, int n, int nb, double lo, double bi,
+                                       double u) {
+    if constexpr (K > 0)
+        return find_interval_k<K>(x, start, n, nb, lo, bi, u);
+    else
+        return find_interval(x, start, n, nb, lo, bi, u);
 }
 
 // y + dx*(b + dx*(c + dx*d)) evaluated as written, without contraction into

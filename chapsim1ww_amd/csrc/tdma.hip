@@ -165,7 +165,15 @@ bool tdma_seg_config(int n, bool cyclic, int* MS, int* P, int* LPB, int maxthr, 
 int tdma_launch(const double* a, const double* b, const double* c, double* r, const double* bc, int n, int nlines,
                 long long es, long long ls, int cyclic, hipStream_t s, double* ws_given = nullptr) {
     int MS = 0, P = 0, LPB = 0;
-    if (tdma_seg_config(n, cyclic != 0, &MS, &P, &LPB, 1024, false)) {
+    bool seg = tdma_seg_config(n, cyclic != 0, &MS, &P, &LPB, 1024, false);
+#if defined(CHS_TD_MS) && defined(CHS_TD_LPB)
+    if (seg && (n + CHS_TD_MS - 1) / CHS_TD_MS * CHS_TD_LPB <= 1024 && (!cyclic || n % CHS_TD_MS == 0)) {
+        MS = CHS_TD_MS;
+        P = (n + MS - 1) / MS;
+        LPB = CHS_TD_LPB;
+    }
+#endif
+    if (seg) {
         const dim3 grid((nlines + LPB - 1) / LPB), block(LPB * P);
         const size_t sm = seg_smem_bytes<double>(LPB, P);
         switch (MS) {
