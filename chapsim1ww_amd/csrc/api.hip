@@ -121,7 +121,11 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     CHS_CHECK(d && out, CHS_E_ARG, "chs_create: null argument");
     *out = nullptr;
     CHS_CHECK(d->icase == 1, CHS_E_ARG, "chs_create: only the channel (ICASE=1) is supported");
-    CHS_CHECK(d->flowdrvtp == 1, CHS_E_ARG, "chs_create: only constant mass flux (FLOWDRVTP=1) is supported");
+    CHS_CHECK(d->flowdrvtp == 1 || d->flowdrvtp == 2, CHS_E_ARG,
+              "chs_create: FLOWDRVTP must be 1 (constant mass flux) or 2 (constant pressure gradient)");
+    CHS_CHECK(d->nflow == 0 || d->nflow == 1 || d->nflow == 3, CHS_E_ARG,
+              "chs_create: NFLOW must be 1 (x) or 3 (z): the wall-normal component has no RHS on its wall row");
+    CHS_CHECK(std::isfinite(d->cfgv), CHS_E_ARG, "chs_create: CFGV must be finite");
     CHS_CHECK(d->precision == CHS_FP64 || d->precision == CHS_MIXED, CHS_E_ARG, "chs_create: bad precision");
     CHS_CHECK(fft_len_ok(d->ncl1) && fft_len_ok(d->ncl3), CHS_E_ARG,
               "chs_create: NCL1 and NCL3 must be even, >= 4, with prime factors 2, 3, 5 only (FFTFAX)");
@@ -143,6 +147,8 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     h->LHP32 = ((h->LH + 15) / 16) * 16;
     h->icase = d->icase;
     h->flowdrvtp = d->flowdrvtp;
+    h->nflow = d->nflow == 0 ? 1 : d->nflow;
+    h->cfgv = d->cfgv;
     h->precision = d->precision;
     h->dxi = d->dxi; h->dzi = d->dzi; h->dxqi = d->dxqi; h->dzqi = d->dzqi;
     h->cvisc = d->cvisc; h->volm = d->volm;

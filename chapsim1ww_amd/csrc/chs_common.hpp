@@ -183,6 +183,8 @@ struct chs_handle {
     int L, M, N, LH, LHP;
     int LHP32;           // row length of the FP32 spectrum (LH rounded up to 16)
     int icase, flowdrvtp, precision;
+    int nflow;      // streamwise direction: 1 (x) or 3 (z)
+    double cfgv;    // FLOWDRVTP = 2: DPGRNS = -CFGV/2 TALP DT
     double dxi, dzi, dxqi, dzqi, cvisc, volm;
     hipStream_t stream;
     bool own_stream;

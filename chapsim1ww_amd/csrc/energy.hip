@@ -242,6 +242,7 @@ constexpr int EPT = 512;   // threads per property block
 // interior rows; persistent blocks over k planes
 // second = 0: the final THERM_PROP_UPDATE lookup is left to the learned
 // surrogate (chs_props_set_mode), which runs after this kernel on h
+template <int KW>
 __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields f,
                                                       unsigned long long* __restrict__ cnt, int second) {
 #pragma clang fp contract(off)
@@ -265,7 +266,7 @@ __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields
         double hv = rh / rho_old;
         double Tt, D, Mu, K, CP;
         // ENTHALPY_UPDATE, DENSITY_UPDATE: only the density of the first lookup is used
-        if (!spline_ht(t, xh, sh, hv, Tt, true) || !spline_t_density(t, xt, st, Tt, D)) {
+        if (!spline_ht<KW>(t, xh, sh, hv, Tt, true) || !spline_t_density<KW>(t, xt, st, Tt, D)) {
             ++bad;
             continue;
         }
@@ -273,7 +274,7 @@ __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields
         hv = rh / D;
         f.h[q] = hv;
         if (!second || fabs(hv) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells
-        if (!spline_ht(t, xh, sh, hv, Tt, false) || !spline_t(t, xt, st, Tt, D, Mu, K, CP)) {
+        if (!spline_ht<KW>(t, xh, sh, hv, Tt, false) || !spline_t<KW>(t, xt, st, Tt, D, Mu, K, CP)) {
             ++bad;
             continue;
         }
@@ -375,11 +376,21 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     h->timer.end(ev, "energy_rhs", s);
     const PropTab t = tab_of(p);
     h->timer.begin("energy_props", s, &ev);
-    CHS_TRY(props_lds_attr((const void*)k_energy_props, p));
     CHS_TRY(props_lds_attr((const void*)k_energy_wall, p));
     const bool sur = p->mode == CHS_PROPS_SURROGATE;
-    CHS_LAUNCH(k_energy_props, dim3(props_grid(p, 512)), dim3(EPT), props_lds_bytes(p), s, t, g, f, p->counters,
-               sur ? 0 : 1);
+#define CHS_EPROPS(KK)                                                                                         \
+    do {                                                                                                       \
+        CHS_TRY(props_lds_attr((const void*)k_energy_props<KK>, p));                                           \
+        CHS_LAUNCH(k_energy_props<KK>, dim3(props_grid_occ(p, (const void*)k_energy_props<KK>, EPT, 1 << 20)),  \
+                   dim3(EPT), props_lds_bytes(p), s, t, g, f, p->counters, sur ? 0 : 1);                       \
+    } while (0)
+    switch (p->kwalk) {
+        case 2: CHS_EPROPS(2); break;
+        case 3: CHS_EPROPS(3); break;
+        case 4: CHS_EPROPS(4); break;
+        default: CHS_EPROPS(0); break;
+    }
+#undef CHS_EPROPS
     if (sur) {
         // THERM_PROP_UPDATE of the interior rows by the surrogate where it is
         // certified (the table elsewhere), on this handle's stream

@@ -230,7 +230,8 @@ __global__ void __launch_bounds__(16 * CB) k_fft256_z(cpx_t<Rl>* __restrict__ S,
 // x inverse, L = 256: 16 rows per block of 128 threads; ghost as k_fft_x_inv,
 // and two modes of the mixed form's refinement, on phf (FP32, interior rows,
 // the layout of f): 3 = write phf only (the first pass's solution), 4 = dph
-// (interior rows) = phf + the transform (the correction added to it)
+// = phf + the transform (the correction added to it), ghost planes as
+// Neumann copies
 template <typename Rl>
 __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restrict__ S, double* __restrict__ dph,
                                                       int M, int nrows, int LHP, int ghost,
@@ -276,8 +277,12 @@ __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restric
 #pragma unroll
             for (int k2 = 0; k2 < 16; ++k2) old[k2] = fr[16 * k2];
 #pragma unroll
-            for (int k2 = 0; k2 < 16; ++k2)
-                rowp[16 * k2] = (double)old[k2] + (h ? (double)v[k2].y : (double)v[k2].x);
+            for (int k2 = 0; k2 < 16; ++k2) {
+                const double val = (double)old[k2] + (h ? (double)v[k2].y : (double)v[k2].x);
+                rowp[16 * k2] = val;
+                if (lo) base[16 * k2] = val;   // the ghost planes as Neumann copies (BC_WALL_DPH)
+                if (hi) base[(size_t)(M + 1) * 256 + 16 * k2] = val;
+            }
             continue;
         }
         if (ghost == 2) {   // add to phi: all 16 old values loaded before the first add
@@ -751,7 +756,7 @@ __global__ void k_dph_ghost(double* __restrict__ dph, int L, int M, int N, int m
 int poisson_solve_unchecked(chs_handle* h, const double* f, double* dph, int ghost) {
     const hipStream_t s = h->stream;
     float* rf = reinterpret_cast<float*>(h->resid);   // the FP32 residual (first half of the buffer)
-    if (h->L == 256 && ghost != 2) {
+    if (h->L == 256 && ghost == 1) {
         // the first pass's solution phi1 kept in FP32 (interior rows, in the
         // residual buffer); the residual f - L_h(phi1) formed in FP64 inside
         // the refinement pass's x transform; DPH = phi1 + the correction
@@ -765,8 +770,8 @@ int poisson_solve_unchecked(chs_handle* h, const double* f, double* dph, int gho
                    h->dzqi, h->amph, h->acph, h->apph, h->partials);
         h->timer.end(ev, "poisson_residual", s);
         CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
+        if (ghost != 2) CHS_LAUNCH(k_dph_ghost, dim3(256), dim3(256), 0, s, dph, h->L, h->M, h->N, ghost);
     }
-    if (ghost != 2) CHS_LAUNCH(k_dph_ghost, dim3(256), dim3(256), 0, s, dph, h->L, h->M, h->N, ghost);
     h->last_refine_iters = 1;
     h->last_refine_resid = -1.0;   // known once the substep's check has been read
     return CHS_OK;

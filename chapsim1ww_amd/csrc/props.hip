@@ -190,10 +190,10 @@ int props_table_launch(chs_props* p, const double* h, Rows rw, double* T, double
     const long long chunk = (long long)PT * PCELL;
     const long long nch = ((rw.rowlen + chunk - 1) / chunk) * rw.nrow;
     if (nch == 0) return CHS_OK;
-    const int grid = props_grid(p, nch);
 #define CHS_PROPS_K(KK)                                                                                          \
     do {                                                                                                         \
         CHS_TRY(props_lds_attr((const void*)k_props_table<KK>, p));                                              \
+        const int grid = props_grid_occ(p, (const void*)k_props_table<KK>, PT, nch);                             \
         CHS_LAUNCH(k_props_table<KK>, dim3(grid), dim3(PT), props_lds_bytes(p), stream, tab_of(p), rw, h, T, rho, \
                    mu, kap, cp, cnt, count_lookups);                                                             \
     } while (0)

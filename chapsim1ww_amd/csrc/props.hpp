@@ -51,12 +51,15 @@ inline size_t props_lds_bytes(const chs_props* p) {
     return sizeof(double) * 2 * (size_t)p->n + sizeof(int) * ((size_t)p->nbh + p->nbt);
 }
 
-// Blocks of a persistent lookup kernel: as many as are resident at once
-// (two per CU while two copies of the staged tables fit in a CU's 160 KB of
-// LDS, one above that), at most `want`.
-inline int props_grid(const chs_props* p, long long want) {
-    const int per_cu = 2 * props_lds_bytes(p) <= (size_t)160 * 1024 ? 2 : 1;
-    const long long g = (long long)p->ncu * per_cu;
+// Blocks of a persistent lookup kernel of `threads` threads with the staged
+// tables in LDS: as many as are resident at once on the device (registers and
+// LDS), at most `want`.
+inline int props_grid_occ(const chs_props* p, const void* kernel, int threads, long long want) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, props_lds_bytes(p)) != hipSuccess ||
+        per < 1)
+        per = 1;
+    const long long g = (long long)p->ncu * per;
     return (int)(want < g ? want : g);
 }
 
@@ -151,6 +154,7 @@ inline PropTab tab_of(const chs_props* p) {
 // S_HT (NIST_SLEVAL_HT, src/Flow_State.f90:1015-1070): false when h leaves
 // the table (the reference STOPs); hack: |h| < 1e-10 gives T = 1.  A NaN
 // passes the range tests and gives NaN, as there.
+template <int KW = 0>
 __device__ __forceinline__ bool spline_ht(const PropTab& t, const double* xh, const int* sh, double hv, double& T,
                                           bool hack) {
     if (hv != hv) {
@@ -158,7 +162,7 @@ __device__ __forceinline__ bool spline_ht(const PropTab& t, const double* xh, co
         return true;
     }
     if (hv <= t.h_lo || hv >= t.h_hi) return false;
-    const int i = find_interval(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hv);
+    const int i = find_iv<KW>(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hv);
     const double4 r = reinterpret_cast<const double4*>(t.rh)[i];
     T = cubic_nc(r.x, r.y, r.z, r.w, hv - xh[i]);
     if (hack && fabs(hv) < 1.0e-10) T = 1.0;
@@ -168,6 +172,7 @@ __device__ __forceinline__ bool spline_ht(const PropTab& t, const double* xh, co
 // S_TD, S_TM, S_TK, S_TCp at T (NIST_SLEVAL_TD/TM/TK/TCp, src/Flow_State.f90:
 // 1415-1692): one interval search for all four; |T - 1| < 1e-10 gives
 // D = M = K = CP = 1 (each evaluator's own rule, :1470, :1524, :1578, :1686).
+template <int KW = 0>
 __device__ __forceinline__ bool spline_t(const PropTab& t, const double* xt, const int* st, double tv, double& D,
                                          double& Mu, double& K, double& CP) {
     if (tv != tv) {
@@ -175,7 +180,7 @@ __device__ __forceinline__ bool spline_t(const PropTab& t, const double* xt, con
         return true;
     }
     if (tv <= t.t_lo || tv >= t.t_hi) return false;
-    const int k = find_interval(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tv);
+    const int k = find_iv<KW>(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tv);
     const double4* q = reinterpret_cast<const double4*>(t.rt) + 4 * (size_t)k;
     const double4 cd = q[0], cm = q[1], ck = q[2], cc = q[3];
     const double dt = tv - xt[k];
@@ -192,6 +197,7 @@ __device__ __forceinline__ bool spline_t(const PropTab& t, const double* xt, con
 // S_TD alone (DENSITY_UPDATE needs only the density): the same interval
 // search and the same evaluation as spline_t, reading only the density's
 // coefficients of the interval's record.
+template <int KW = 0>
 __device__ __forceinline__ bool spline_t_density(const PropTab& t, const double* xt, const int* st, double tv,
                                                  double& D) {
     if (tv != tv) {
@@ -199,19 +205,20 @@ __device__ __forceinline__ bool spline_t_density(const PropTab& t, const double*
         return true;
     }
     if (tv <= t.t_lo || tv >= t.t_hi) return false;
-    const int k = find_interval(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tv);
+    const int k = find_iv<KW>(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tv);
     const double4 cd = reinterpret_cast<const double4*>(t.rt)[4 * (size_t)k];
     D = cubic_nc(cd.x, cd.y, cd.z, cd.w, tv - xt[k]);
     if (fabs(tv - 1.0) < 1.0e-10) D = 1.0;
     return true;
 }
 
+template <int KW = 0>
 __device__ __forceinline__ bool lookup_cell(const PropTab& t, const double* xh, const double* xt, const int* sh,
                                             const int* st, double hv, double& T, double& D, double& Mu, double& K,
                                             double& CP) {
     double tv;
-    if (!spline_ht(t, xh, sh, hv, tv, false)) return false;
-    if (!spline_t(t, xt, st, tv, D, Mu, K, CP)) return false;
+    if (!spline_ht<KW>(t, xh, sh, hv, tv, false)) return false;
+    if (!spline_t<KW>(t, xt, st, tv, D, Mu, K, CP)) return false;
     T = tv;
     return true;
 }

@@ -28,6 +28,7 @@ namespace chs {
 
 struct Geo {
     int L, M, N;
+    int mc;   // the streamwise component NFLOW - 1 (0 or 2), which takes the flow-drive term
     double dxi, dzi, dxqi, dzqi;
     const double *dyfi, *dyci, *wff, *wfb, *amvr, *acvr, *apvr;
 };
@@ -253,7 +254,7 @@ __device__ __forceinline__ double rhs_rows(const Geo& g, const RkCoef& rk, doubl
 #pragma unroll
         for (int c = 0; c < 3; ++c) h0[c] = H0[ci + c * LMN];
         rhs_cell(g, rk, cvisc, rc, U, V, W, P, n, h0, conv, rhs);
-        msum += rhs[0] / rc.y.dyfi / 1.0;
+        msum += (g.mc == 0 ? rhs[0] : rhs[2]) / rc.y.dyfi / 1.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
@@ -586,7 +587,7 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
                 if (i < KX) row[row_pos<KX>(KX + i + L, Q4)] = rhs[c];
                 if (i >= L - KX) row[row_pos<KX>(KX + i - L, Q4)] = rhs[c];
             }
-            v += double(rhs[0]) / dyfj / 1.0;
+            v += double(g.mc == 0 ? rhs[0] : rhs[2]) / dyfj / 1.0;
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -644,6 +645,10 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     }
 }
 
+__global__ void k_set_scalar(double* __restrict__ out, double v) {
+    if (threadIdx.x == 0) out[0] = v;
+}
+
 // DPGRNS = (sum of block partials) / VOLM_tg, summed in a fixed order.
 // One block of 1024 threads, four independent partial sums per thread (its
 // strided values in turn) so that four loads are in flight per thread.
@@ -690,7 +695,7 @@ __global__ void __launch_bounds__(1024) k_z_solve(Geo g, double* __restrict__ RH
     const bool valid = line < nlines && !(c == 1 && j == 0);
     const size_t LMN = (size_t)LM * g.N;
     const size_t base = valid ? (size_t)c * LMN + rem : 0;
-    const double D = (use_dpg && c == 0) ? dpg[0] * xone : 0.0;
+    const double D = (use_dpg && c == g.mc) ? dpg[0] * xone : 0.0;
     double a[MS], b[MS], cc[MS], d[MS];
 #pragma unroll
     for (int l = 0; l < MS; ++l) {
@@ -905,7 +910,7 @@ __global__ void __launch_bounds__(256) k_z_window(Geo g, const T* __restrict__ R
     const size_t LM = (size_t)L * M;
     const T* Rc = R + (size_t)cc * LM * N + (size_t)j * L + i;
     T* Oc = R2 + (size_t)cc * LM * N + (size_t)j * L + i;
-    const T D = T((use_dpg && cc == 0) ? dpg[0] * xone : 0.0);
+    const T D = T((use_dpg && cc == g.mc) ? dpg[0] * xone : 0.0);
     constexpr int W = 2 * KW + 1;
     T w[W];
 #pragma unroll
@@ -1279,6 +1284,7 @@ int mixed_check(chs_handle* h, bool wait) {
 static Geo geo_of(const chs_handle* h) {
     Geo g;
     g.L = h->L; g.M = h->M; g.N = h->N;
+    g.mc = h->nflow - 1;
     g.dxi = h->dxi; g.dzi = h->dzi; g.dxqi = h->dxqi; g.dzqi = h->dzqi;
     g.dyfi = h->dyfi; g.dyci = h->dyci; g.wff = h->wff; g.wfb = h->wfb;
     g.amvr = h->amvr; g.acvr = h->acvr; g.apvr = h->apvr;
@@ -1616,13 +1622,16 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         }
         h->timer.end(ev, "rhs_x", s);
         h->timer.begin("massflux", s, &ev);
-        CHS_LAUNCH(k_massflux, dim3(1), dim3(1024), 0, s, h->partials, nblk, h->volm, h->scalars);
+        if (h->flowdrvtp == 1)
+            CHS_LAUNCH(k_massflux, dim3(1), dim3(1024), 0, s, h->partials, nblk, h->volm, h->scalars);
+        else   // constant pressure gradient (RHS_CvLpGpS_tg.f90:200-201), written by a kernel (no memset node)
+            CHS_LAUNCH(k_set_scalar, dim3(1), dim3(64), 0, s, h->scalars, -0.50 * h->cfgv * (rk.talp * dt));
         h->timer.end(ev, "massflux", s);
         // 2. z solves
         const double RMC2 = 1.0 * (-facoe * h->dzqi);
         const double za = RMC2, zc = RMC2, zb = 1.0 - zc - za;
         const double xone = 1.0 / (xa + xb + xc);
-        const int usedpg0 = h->flowdrvtp == 1 ? 1 : 0;
+        const int usedpg0 = 1;
         if (use_st) {
             // 128-row chunks where N allows (measured at 256x192x256: 139 -> 134 us)
             const int zkc = h->N % (2 * ZKC) == 0 ? 2 * ZKC : ZKC;
@@ -1656,7 +1665,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const int nlines = 3 * h->L * h->M;
         const dim3 gz((nlines + LPB - 1) / LPB), bz(LPB * Pz);
         const size_t smz = seg_smem_bytes<double>(LPB, Pz);
-        const int usedpg = h->flowdrvtp == 1 ? 1 : 0;
+        const int usedpg = 1;
         if (!use_st) {
         h->timer.begin("z_solve", s, &ev);
         switch (MS) {

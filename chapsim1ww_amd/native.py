@@ -33,7 +33,8 @@ class GridDesc(C.Structure):
                 ("dxi", C.c_double), ("dzi", C.c_double), ("dxqi", C.c_double), ("dzqi", C.c_double),
                 ("cvisc", C.c_double), ("volm", C.c_double)] + \
                [(n, _dp) for n in ("dyfi", "dyci", "wff", "wfb", "amph", "acph", "apph",
-                                   "amvr", "acvr", "apvr", "ak1", "ak3")]
+                                   "amvr", "acvr", "apvr", "ak1", "ak3")] + \
+               [("nflow", C.c_int), ("cfgv", C.c_double)]
 
 
 class ThermalFields(C.Structure):

@@ -4,7 +4,8 @@
 
 Runs, at 256x192x256: NSUB FP64 RK3 substeps, NPOIS FP64 Poisson solves,
 NTDMA batched tridiagonal solves (65536 lines x 192, IJJ form, non-cyclic
-and cyclic), NTDMA table-lookup THERM_PROP_UPDATEs over the grid, NHEAT
+and cyclic), NTDMA table-lookup THERM_PROP_UPDATEs over the grid (and, with
+--surrogate K, K evaluations of the learned surrogate over it), NHEAT
 heated-channel substeps (energy, property staging, io momentum), and a plain
 device copy of 1 GiB (read 1 GiB + write 1 GiB)
 whose counter readings calibrate the byte counters of the others.
@@ -23,6 +24,8 @@ def main() -> None:
     ap.add_argument("--ntdma", type=int, default=4)
     ap.add_argument("--nheat", type=int, default=3)
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--surrogate", type=int, default=0,
+                    help="network evaluations over the grid after a short training (k_mlp_eval counters)")
     a = ap.parse_args()
     import torch
 
@@ -61,6 +64,14 @@ def main() -> None:
     outs = [torch.empty_like(hh) for _ in range(5)]
     for _ in range(a.ntdma):
         pt.update(hh, *outs)
+    if a.surrogate > 0:
+        # the learned surrogate on all cells: 2000 Adam steps on 2^18 of them,
+        # then a.surrogate evaluations of the network over the grid
+        pt.train(hh[: 1 << 18].contiguous(), 2000, 4096, 3e-3, seed=1)
+        net = torch.empty(5 * g.ncell, dtype=torch.float64, device=dev)
+        for _ in range(a.surrogate):
+            pt.eval(hh, net)
+        del net
     # heated channel: energy + staging + io momentum from the tg velocity as mass flux
     from chapsim1ww_amd.heated import HeatedChannel
     hc = HeatedChannel(s, pt)

@@ -21,12 +21,13 @@ from . import native as nat
 from .grid import ChannelGrid
 
 
-def _desc(g: ChannelGrid, precision: int):
+def _desc(g: ChannelGrid, precision: int, flowdrvtp: int = 1, nflow: int = 1, cfgv: float = 0.0):
     keep = {k: np.ascontiguousarray(getattr(g, k), dtype=np.float64)
             for k in ("dyfi", "dyci", "wff", "wfb", "amph", "acph", "apph", "amvr", "acvr", "apvr", "ak1", "ak3")}
     d = nat.GridDesc()
     d.ncl1, d.ncl2, d.ncl3 = g.L, g.M, g.N
-    d.icase, d.flowdrvtp, d.precision = 1, 1, precision
+    d.icase, d.flowdrvtp, d.precision = 1, int(flowdrvtp), precision
+    d.nflow, d.cfgv = int(nflow), float(cfgv)
     d.dxi, d.dzi, d.dxqi, d.dzqi, d.cvisc, d.volm = g.dxi, g.dzi, g.dxqi, g.dzqi, g.cvisc, g.volm
     for k, v in keep.items():
         setattr(d, k, nat.hptr(v))
@@ -36,11 +37,14 @@ def _desc(g: ChannelGrid, precision: int):
 class ChannelSolver:
     """Device-resident tg channel state plus the per-substep solver core."""
 
-    def __init__(self, grid: ChannelGrid, precision: str = "fp64", stream=None):
+    def __init__(self, grid: ChannelGrid, precision: str = "fp64", stream=None, flowdrvtp: int = 1, nflow: int = 1,
+                 cfgv: float = 0.0):
+        """flowdrvtp: FLOWDRVTP, 1 constant mass flux or 2 constant pressure
+        gradient -cfgv/2; nflow: NFLOW, the streamwise direction 1 (x) or 3 (z)."""
         self.grid = grid
         self.lib = nat.load()
         self.precision = {"fp64": nat.CHS_FP64, "mixed": nat.CHS_MIXED}[precision]
-        d, self._keep = _desc(grid, self.precision)
+        d, self._keep = _desc(grid, self.precision, flowdrvtp, nflow, cfgv)
         h = C.c_void_p()
         s = None if stream is None else C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
         nat.check(self.lib.chs_create(C.byref(d), s, C.byref(h)))

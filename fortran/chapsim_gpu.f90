@@ -40,6 +40,8 @@ module chapsim_gpu
         integer(c_int) :: ncl1, ncl2, ncl3, icase, flowdrvtp, precision
         real(c_double) :: dxi, dzi, dxqi, dzqi, cvisc, volm
         type(c_ptr) :: dyfi, dyci, wff, wfb, amph, acph, apph, amvr, acvr, apvr, ak1, ak3
+        integer(c_int) :: nflow
+        real(c_double) :: cfgv
     end type
 
     ! chs_thermal_fields, chs_io_fields, chs_io_params of include/chapsim_gpu.h
@@ -223,16 +225,24 @@ contains
 
     !> Mesh and coefficient arrays as the reference's modules hold them:
     !> DYFI(1:NCL2), DYCI(1:NND2), YCL2ND_WFF/WFB(1:NND2), AMPH/ACPH/APPH(1:NCL2),
-    !> AMVR/ACVR/APVR(1:NND2,3), AK1(1:NCL1), AK3(1:NCL3).
+    !> AMVR/ACVR/APVR(1:NND2,3), AK1(1:NCL1), AK3(1:NCL3).  Optional: the
+    !> flow drive FLOWDRVTP (1 constant mass flux, default; 2 constant pressure
+    !> gradient -CFGV/2), NFLOW (1 default, or 3) and CFGV of module init_info.
     subroutine chs_init(ncl1, ncl2, ncl3, dxi, dzi, dxqi, dzqi, cvisc, volm, dyfi, dyci, wff, wfb, &
-                        amph, acph, apph, amvr, acvr, apvr, ak1, ak3, precision)
+                        amph, acph, apph, amvr, acvr, apvr, ak1, ak3, precision, flowdrvtp, nflow, cfgv)
         integer, intent(in) :: ncl1, ncl2, ncl3, precision
+        integer, intent(in), optional :: flowdrvtp, nflow
+        real(c_double), intent(in), optional :: cfgv
         real(c_double), intent(in) :: dxi, dzi, dxqi, dzqi, cvisc, volm
         real(c_double), target, intent(in) :: dyfi(:), dyci(:), wff(:), wfb(:), amph(:), acph(:), apph(:)
         real(c_double), target, intent(in) :: amvr(:, :), acvr(:, :), apvr(:, :), ak1(:), ak3(:)
         type(chs_grid_desc) :: d
         d%ncl1 = ncl1; d%ncl2 = ncl2; d%ncl3 = ncl3
         d%icase = 1; d%flowdrvtp = 1; d%precision = precision
+        d%nflow = 1; d%cfgv = 0.0_c_double
+        if (present(flowdrvtp)) d%flowdrvtp = flowdrvtp
+        if (present(nflow)) d%nflow = nflow
+        if (present(cfgv)) d%cfgv = cfgv
         d%dxi = dxi; d%dzi = dzi; d%dxqi = dxqi; d%dzqi = dzqi; d%cvisc = cvisc; d%volm = volm
         d%dyfi = c_loc(dyfi); d%dyci = c_loc(dyci); d%wff = c_loc(wff); d%wfb = c_loc(wfb)
         d%amph = c_loc(amph); d%acph = c_loc(acph); d%apph = c_loc(apph)

@@ -50,11 +50,15 @@ typedef struct chs_handle chs_handle;
 typedef struct chs_grid_desc {
     int ncl1, ncl2, ncl3;       /* L, M, N; L and N even with factors 2, 3, 5 */
     int icase;                  /* 1 = channel                                */
-    int flowdrvtp;              /* 1 = constant mass flux (FLOWDRVTP)          */
+    int flowdrvtp;              /* FLOWDRVTP: 1 constant mass flux, 2 constant
+                                   pressure gradient -CFGV/2 (cfgv below)     */
     int precision;              /* CHS_FP64 or CHS_MIXED                       */
     double dxi, dzi, dxqi, dzqi, cvisc, volm;
     const double *dyfi, *dyci, *wff, *wfb, *amph, *acph, *apph;
     const double *amvr, *acvr, *apvr, *ak1, *ak3;
+    int nflow;                  /* NFLOW, the streamwise direction: 1 (x; 0 is
+                                   taken as 1) or 3 (z)                        */
+    double cfgv;                /* CFGV of FLOWDRVTP = 2 (READINI.f90:269)     */
 } chs_grid_desc;
 
 /* Create a solver handle: allocates the device-resident mirrors of the
@@ -73,7 +77,8 @@ int chs_device_arrays(chs_handle *h, double **q, double **pr, double **convh0, d
 
 /* One RK3 substep NS (0..3) of the tg channel with time step dt, on the
  * device-resident state: SOLVERRK3_MOM_tg(NS) (src/SOLVERRK3_MOM_tg.f90:18-61):
- * convection, RHS + constant-mass-flux forcing, CN factorised x/z/y solves,
+ * convection, RHS + the flow drive of component NFLOW (constant mass flux or
+ * constant pressure gradient, src/RHS_CvLpGpS_tg.f90:178-213), CN factorised x/z/y solves,
  * wall BCs, divergence, Poisson solve, projection and pressure update. */
 int chs_rk3_substep(chs_handle *h, int ns, double dt);
 
@@ -140,7 +145,8 @@ int chs_mean_t(chs_handle *h, int domain, int tp, int pp_instn_sz, int *nstatis,
 /* PP_TMEAN_INI_ZERO_TG / _IO (src/processing/PP_MEAN_T.f90:29-54, 110-319):
  * NSTATIS = 0 and the time means of the domain set to zero. */
 int chs_mean_t_reset(chs_handle *h, int domain);
-/* DPGRNS of the last substep (constant-mass-flux correction). */
+/* DPGRNS of the last substep (the flow-drive term: the constant-mass-flux
+ * correction, or -CFGV/2 TALP DT for FLOWDRVTP = 2). */
 int chs_last_dpgrns(chs_handle *h, double *dpgrns_host);
 
 /* Batched tridiagonal line solves on device arrays, the four reference solvers:

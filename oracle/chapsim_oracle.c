@@ -707,8 +707,12 @@ void orc_convection_z(const orc_grid *g, orc_state *s)
     }
 }
 
-/* RHS_CvLpGpS_tg (src/RHS_CvLpGpS_tg.f90:29-220), constant mass flux */
-void orc_rhs_tg(const orc_grid *g, orc_state *s, int NS, int IDR, double DT, int flowdrvtp)
+/* RHS_CvLpGpS_tg (src/RHS_CvLpGpS_tg.f90:29-220).  The flow-drive term of
+ * the streamwise component IDR = NFLOW (:178-213): FLOWDRVTP = 1 constant
+ * mass flux, DPGRNS = (sum RHS / DYFI / RCCI1) / VOLM_tg; FLOWDRVTP = 2
+ * constant pressure gradient, DPGRNS = -CFGV / 2 * TALP(NS) * DT.  NFLOW is
+ * 1 (x) or 3 (z): the y component's wall row J = 1 has no RHS of its own. */
+void orc_rhs_tg(const orc_grid *g, orc_state *s, int NS, int IDR, double DT, int flowdrvtp, int nflow, double cfgv)
 {
     int NII = (IDR == 2) ? 2 : 1;
     const double *H = IDR == 1 ? s->qtmp : IDR == 3 ? s->rhsllphi : NULL;
@@ -745,7 +749,7 @@ void orc_rhs_tg(const orc_grid *g, orc_state *s, int NS, int IDR, double DT, int
             }
         }
     }
-    if (IDR == 1) { /* NFLOW = 1 */
+    if (IDR == nflow) {
         double DPGRNS = 0.0;
         if (flowdrvtp == 1) {
             double INTGRHSY = 0.0;
@@ -755,6 +759,8 @@ void orc_rhs_tg(const orc_grid *g, orc_state *s, int NS, int IDR, double DT, int
                     for (int KC = 1; KC <= GN; ++KC)
                         INTGRHSY = INTGRHSY + F3(s->rhs, IC, JC, KC) / g->dyfi[JC - 1] / 1.0;
             DPGRNS = INTGRHSY / g->volm;
+        } else if (flowdrvtp == 2) {
+            DPGRNS = -0.50 * cfgv * COE2;
         }
         s->dpgrns = DPGRNS;
 #pragma omp parallel for schedule(static)
@@ -988,13 +994,13 @@ void orc_mean_xz(const orc_grid *g, const orc_state *s, double *out)
 }
 
 /* SOLVERRK3_MOM_tg(NS) */
-void orc_substep_tg(const orc_grid *g, orc_state *s, int NS, double DT, int flowdrvtp)
+void orc_substep_tg(const orc_grid *g, orc_state *s, int NS, double DT, int flowdrvtp, int nflow, double cfgv)
 {
     orc_convection_x(g, s);
     orc_convection_y(g, s);
     orc_convection_z(g, s);
     for (int IDR = 1; IDR <= 3; ++IDR) {
-        orc_rhs_tg(g, s, NS, IDR, DT, flowdrvtp);
+        orc_rhs_tg(g, s, NS, IDR, DT, flowdrvtp, nflow, cfgv);
         orc_momfa_tg(g, s, NS, IDR, DT);
     }
     orc_bc_wall_q(g, s);

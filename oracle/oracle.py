@@ -76,7 +76,8 @@ def lib():
         L.orc_poisson_residual.restype = C.c_double
         for name in ("orc_convection_x", "orc_convection_y", "orc_convection_z", "orc_bc_wall_q"):
             getattr(L, name).argtypes = [C.POINTER(_Grid), C.POINTER(_State)]
-        L.orc_rhs_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_int, C.c_double, C.c_int]
+        L.orc_rhs_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.c_int, C.c_double]
         L.orc_momfa_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_int, C.c_double]
         for name in ("orc_divg_tg", "orc_veloupdt_tg", "orc_prcalc_tg"):
             getattr(L, name).argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_double]
@@ -87,7 +88,8 @@ def lib():
         L.orc_cfl_tg.restype = C.c_double
         L.orc_mean_xz.argtypes = [C.POINTER(_Grid), C.POINTER(_State), _dp]
         L.orc_mean_xz.restype = None
-        L.orc_substep_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_double, C.c_int]
+        L.orc_substep_tg.argtypes = [C.POINTER(_Grid), C.POINTER(_State), C.c_int, C.c_double, C.c_int, C.c_int,
+                                     C.c_double]
         L.orc_cubic_spline.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_sleval.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
         L.orc_sleval.restype = C.c_int
@@ -174,8 +176,12 @@ class OracleState:
     def dpgrns(self):
         return self.s.dpgrns
 
-    def substep(self, ns: int, dt: float, flowdrvtp: int = 1):
-        lib().orc_substep_tg(C.byref(self.og.g), C.byref(self.s), ns, dt, flowdrvtp)
+    def substep(self, ns: int, dt: float, flowdrvtp: int = 1, nflow: int = 1, cfgv: float = 0.0):
+        """SOLVERRK3_MOM_tg(NS); flowdrvtp 1 (constant mass flux) or 2
+        (constant pressure gradient -CFGV/2), streamwise direction nflow 1 or 3."""
+        if nflow not in (1, 3) or flowdrvtp not in (1, 2):
+            raise ValueError("nflow must be 1 or 3 and flowdrvtp 1 or 2")
+        lib().orc_substep_tg(C.byref(self.og.g), C.byref(self.s), ns, dt, flowdrvtp, nflow, cfgv)
 
     def divgck(self) -> float:
         return lib().orc_divgck_tg(C.byref(self.og.g), C.byref(self.s))

@@ -41,7 +41,8 @@ static void run_grid(int L, int M, int N, unsigned long long *seed)
     for (size_t q = 0; q < nc; ++q) f[q] = urand(seed) - 0.5;
     orc_poisson_fft99(&g, f, s.dph, 0);
     (void)orc_poisson_residual(&g, s.dph, f, s.rhsllphi);
-    for (int ns = 0; ns <= 3; ++ns) orc_substep_tg(&g, &s, ns, 1e-3, 1);
+    for (int ns = 0; ns <= 3; ++ns) orc_substep_tg(&g, &s, ns, 1e-3, 1, 1, 0.0);
+    for (int ns = 1; ns <= 3; ++ns) orc_substep_tg(&g, &s, ns, 1e-3, 2, 3, 0.01);
     (void)orc_divgck_tg(&g, &s);
     (void)orc_cfl_tg(&g, &s);
     orc_mean_xz(&g, &s, out);

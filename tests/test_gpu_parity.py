@@ -401,3 +401,43 @@ def test_instant_files_restart_device_run(solver_mod, tmp_path):
         assert np.array_equal(x, y)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("flowdrvtp,nflow", [(2, 1), (2, 3), (1, 3)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_flow_drive_substeps(solver_mod, flowdrvtp, nflow, precision):
+    """FLOWDRVTP = 2 (constant pressure gradient -CFGV/2) and NFLOW = 3
+    (spanwise streamwise direction) on the tg substep: NS = 0, 1, 2, 3 against
+    the oracle (FP64 within 1e-12; the mixed form within its FP32 stencils'
+    1e-6 of the velocities and max|div u| <= 1e-10), and DPGRNS equal to the
+    oracle's after every substep."""
+    print(f"flow drive flowdrvtp={flowdrvtp} nflow={nflow} {precision}", flush=True)
+    dims = (64, 48, 32)
+    g = ChannelGrid(*dims)
+    og = O.OracleGrid(*dims)
+    q, pr, c0 = fields.init_tg_fields(g, seed=41 + nflow)
+    if nflow == 3:   # the laminar profile along z
+        q[..., 0], q[..., 2] = q[..., 2].copy(), q[..., 0].copy()
+    st = O.OracleState(og, q, pr, c0)
+    cfgv = 0.0163
+    s = solver_mod.ChannelSolver(g, precision, flowdrvtp=flowdrvtp, nflow=nflow, cfgv=cfgv)
+    s.upload_state(q, pr, c0)
+    dt = 0.5 / st.cfl()
+    for ns in (0, 1, 2, 3):
+        st.substep(ns, dt, flowdrvtp=flowdrvtp, nflow=nflow, cfgv=cfgv)
+        s.solverrk3_mom_tg(ns, dt)
+        gq, gp, gc = s.download_state()
+        errs = {n: _rel(a, b) for n, a, b in zip(("Q", "PR", "CONVH0"), (gq, gp, gc), (st.q, st.pr, st.convh0))}
+        print(ns, errs, s.last_dpgrns(), st.dpgrns, flush=True)
+        if precision == "fp64":
+            for n, e in errs.items():
+                assert e < FP64_TOL, (ns, n, e)
+            assert abs(s.last_dpgrns() - st.dpgrns) <= 1e-12 * abs(st.dpgrns) + 1e-18
+        else:
+            assert errs["Q"] < 1e-6, errs
+            assert abs(s.last_dpgrns() - st.dpgrns) <= 1e-6 * abs(st.dpgrns) + 1e-12
+        if flowdrvtp == 2:
+            from chapsim1ww_amd.grid import TALP
+            assert st.dpgrns == -0.5 * cfgv * (TALP[ns] * dt)
+        assert s.divgck() <= 1e-10
+    s.close()

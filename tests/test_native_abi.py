@@ -45,8 +45,14 @@ def test_argument_errors_without_device():
     assert lib.chs_create(C.byref(d), None, C.byref(h)) == nat.CHS_E_ARG
     assert b"prime factors" in lib.chs_last_error()
     d.ncl1 = 16
-    d.flowdrvtp = 2
+    # FLOWDRVTP 1 or 2 only; NFLOW 1 or 3 (the wall-normal component has no
+    # right-hand side on its wall row)
+    d.flowdrvtp = 3
     assert lib.chs_create(C.byref(d), None, C.byref(h)) == nat.CHS_E_ARG
+    assert b"FLOWDRVTP" in lib.chs_last_error()
+    d.flowdrvtp, d.nflow = 2, 2
+    assert lib.chs_create(C.byref(d), None, C.byref(h)) == nat.CHS_E_ARG
+    assert b"NFLOW" in lib.chs_last_error()
     assert lib.chs_rk3_substep(None, 1, 1e-3) == nat.CHS_E_ARG
 
 

@@ -232,13 +232,15 @@ def _neumann_operator(ynd):
     return A
 
 
-def _dense_y_substep(ynd, ns, dt, nu, u, w, h0u, h0w):
+def _dense_y_substep(ynd, ns, dt, nu, u, w, h0u, h0w, flowdrvtp=1, nflow=1, cfgv=0.0):
     """SOLVERRK3_MOM_tg(NS) (src/SOLVERRK3_MOM_tg.f90:18-61) for x/z-uniform
     u(y), w(y) with v = 0, p = 0, assembled from dense operators:
 
     * u, w: no convection or pressure; identity x/z solves; RHS = dt (rho_s H0
-      + alpha_s nu A u), minus the volume mean of the u right-hand side
-      (constant mass flux, src/RHS_CvLpGpS_tg.f90:181-215), then the y solve
+      + alpha_s nu A u); the streamwise component (u for nflow = 1, w for 3)
+      minus the volume mean of its right-hand side (constant mass flux,
+      flowdrvtp = 1, src/RHS_CvLpGpS_tg.f90:181-215) or plus CFGV alpha_s dt / 2
+      (constant pressure gradient, flowdrvtp = 2, :200-201); then the y solve
       (I - F A) du = RHS with F = alpha_s dt nu / 2 (src/MOMFA_tg.f90:17-45).
     * v: its right-hand side is evaluated on rows 2..M only (NII = 2), so row 1
       of the shared RHS buffer still holds u's final increment on row 1 when
@@ -255,8 +257,15 @@ def _dense_y_substep(ynd, ns, dt, nu, u, w, h0u, h0w):
     dy = np.diff(ynd)
     I = np.eye(M)
     ru = dt * (TROH[ns] * h0u + a * nu * (A @ u))
-    ru = ru - np.sum(ru * dy) / np.sum(dy)
     rw = dt * (TROH[ns] * h0w + a * nu * (A @ w))
+
+    def drive(r):
+        return r - np.sum(r * dy) / np.sum(dy) if flowdrvtp == 1 else r + 0.5 * cfgv * a * dt
+
+    if nflow == 1:
+        ru = drive(ru)
+    else:
+        rw = drive(rw)
     du = np.linalg.solve(I - F * A, ru)
     dw = np.linalg.solve(I - F * A, rw)
     T = np.eye(M) - F * _node_operator(ynd)[:M, :M]
@@ -298,6 +307,36 @@ def test_y_profile_substep_matches_dense_cn(ns):
     assert np.abs(st.convh0).max() == 0.0
     # the increment itself is well above the tolerance (the check is not vacuous)
     assert np.abs(ue - u).max() > 1e-5 * np.abs(u).max()
+
+
+@pytest.mark.parametrize("ns,flowdrvtp,nflow", [(1, 2, 1), (3, 2, 1), (0, 2, 3), (2, 1, 3), (1, 2, 3)])
+def test_flow_drive_substep_matches_dense_cn(ns, flowdrvtp, nflow):
+    """The flow-drive term of RHS_CvLpGpS_tg for the constant pressure
+    gradient (FLOWDRVTP = 2: DPGRNS = -CFGV/2 alpha_s dt) and for a spanwise
+    streamwise direction (NFLOW = 3, both drives) against the dense composite;
+    a planted factor of two in CFGV is caught."""
+    L, M, N = 8, 24, 8
+    g = ChannelGrid(L, M, N, ren=180.0)
+    og = O.OracleGrid(L, M, N, ren=180.0)
+    q, pr, c0, (u, w, h0u, h0w) = _y_profile_case(g, seed=10 + ns)
+    dt, cfgv = 4e-3, 0.0163
+    ue, we, ve, pe = _dense_y_substep(_nodes(M), ns, dt, 1.0 / 180.0, u, w, h0u, h0w, flowdrvtp, nflow, cfgv)
+    st = O.OracleState(og, q, pr, c0)
+    st.substep(ns, dt, flowdrvtp=flowdrvtp, nflow=nflow, cfgv=cfgv)
+    du = np.abs(st.q[:, 1:M + 1, :, 0] - ue[None, :, None]).max()
+    dw = np.abs(st.q[:, 1:M + 1, :, 2] - we[None, :, None]).max()
+    assert du < 1e-12 * np.abs(ue).max() and dw < 1e-12 * np.abs(we).max(), (du, dw)
+    assert np.abs(st.q[:, 1:M + 2, :, 1] - ve[None, :, None]).max() < 1e-12 * max(np.abs(ve).max(), 1e-3)
+    dp = np.abs(st.pr[:, 1:M + 1, :] - pe[None, :, None]).max()
+    assert dp < 1e-10 * np.abs(pe).max() + 1e-15, dp
+    if flowdrvtp == 2:
+        assert st.dpgrns == -0.5 * cfgv * (TALP[ns] * dt)
+        # the drive is well above the tolerance (the check is not vacuous), and
+        # the oracle with CFGV doubled misses the dense composite
+        bad = O.OracleState(og, q, pr, c0)
+        bad.substep(ns, dt, flowdrvtp=2, nflow=nflow, cfgv=2 * cfgv)
+        comp, ref = (0, ue) if nflow == 1 else (2, we)
+        assert np.abs(bad.q[:, 1:M + 1, :, comp] - ref[None, :, None]).max() > 1e-6 * np.abs(ref).max()
 
 
 def _dense_zmode_substep(ynd, ns, dt, nu, lam, dzi, U, H0):
