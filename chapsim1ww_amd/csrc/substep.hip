@@ -414,8 +414,14 @@ __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, 
 // of row j goes to part[kg*M + j], the slot k_rhs_xw's tile (j, kg) writes.
 // ---------------------------------------------------------------------------
 constexpr int XM_KP = KRS + 2;  // k rows of a ring plane
-constexpr int XM_PD = 1;        // steps the ring's next rows are loaded ahead
-constexpr int XM_CPT = 1;       // cells of a row per thread (1 or 2)
+#ifndef CHS_XM_PD
+#define CHS_XM_PD 1
+#endif
+#ifndef CHS_XM_CPT
+#define CHS_XM_CPT 1
+#endif
+constexpr int XM_PD = CHS_XM_PD;     // steps the ring's next rows are loaded ahead
+constexpr int XM_CPT = CHS_XM_CPT;   // cells of a row per thread (1 or 2)
 
 // LDS of k_rhs_xm in compute type T: the ring and the x-solve rows in T,
 // then (8-byte aligned) the PR plane in FP64 and 16 doubles of block sums
