@@ -528,7 +528,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is design-only, see DESIGN.md)"}))
+                                   "(sharding across GPUs is not built)"}))
         sys.exit(2)
     res = gpu_bench(args.steps, args.warmup, args.compare_steps)
     ncell = res["ncell"]
@@ -543,6 +543,14 @@ def main():
             kern[name] = dict(avg_ms=avg, calls=calls, total_ms=ms_tot,
                               gbs=ab[name] * ncell / (avg * 1e-3) / 1e9)
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    abm = alg_bytes_per_cell(CFG["L"], "mixed")
+    kern_mx = {}
+    for name, (ms_tot, calls) in mixed["prof"].items():
+        if calls:
+            avg = ms_tot / calls
+            kern_mx[name] = dict(avg_ms=avg, calls=calls, total_ms=ms_tot)
+            if name in abm:
+                kern_mx[name]["gbs"] = abm[name] * ncell / (avg * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
@@ -594,6 +602,7 @@ def main():
         "properties_256x192x256": res["props"],
         "tdma_65536x192": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res["tdma"].items()},
         "kernels_fp64": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kern.items()},
+        "kernels_mixed": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kern_mx.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -304,7 +304,7 @@ int chs_rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_CHECK(ns >= 0 && ns <= 3, CHS_E_ARG, "chs_rk3_substep: NS must be 0..3");
     CHS_CHECK(std::isfinite(dt) && dt > 0.0, CHS_E_ARG, "chs_rk3_substep: dt must be positive and finite");
     CHS_CHECK(substep_supported(h), CHS_E_ARG,
-              "chs_rk3_substep: NCL1 must be a power of two >= 32 and NCL3 a multiple of 4");
+              "chs_rk3_substep: NCL1 must be a multiple of 8 (>= 16, <= 1024) and NCL3 a multiple of 4");
     return rk3_substep(h, ns, dt);
 }
 

@@ -45,6 +45,25 @@ __device__ __forceinline__ int cidx(const Geo& g, int i, int j, int k) {
     return i + g.L * (j + g.M * k);
 }
 
+// q = e / d, r = e - q d for 0 <= e < 2^22: shifts when d = 2^lg (lg >= 0),
+// else a float estimate with one correction either way (exact in that range)
+__device__ __forceinline__ void divmod_row(int e, int d, int lg, float dinv, int& q, int& r) {
+    if (lg >= 0) {
+        q = e >> lg;
+        r = e & ((1 << lg) - 1);
+        return;
+    }
+    q = (int)((float)e * dinv);
+    r = e - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+}
+
 constexpr int KR = 4;        // k rows per k_rhs_x block (segmented x solve)
 constexpr int KRS = 4;       // k rows per k_rhs_xw block
 constexpr int XLPB = 16;     // line slots of the x solve (3*KR = 12 used)
@@ -235,8 +254,10 @@ __device__ __forceinline__ double rhs_rows(const Geo& g, const RkCoef& rk, doubl
     const int Q4 = LS >> 2;
     const RowCoef rc = row_coef(g, j);
     double msum = 0.0;
-    for (int e = threadIdx.x; e < (kr << lgL); e += blockDim.x) {
-        const int r = e >> lgL, i = e & (L - 1);
+    const float Linv = 1.0f / (float)L;
+    for (int e = threadIdx.x; e < kr * L; e += blockDim.x) {
+        int r, i;
+        divmod_row(e, L, lgL, Linv, r, i);
         const int k = k0 + r;
         Nb n;
         n.ip = i == L - 1 ? 1 - L : 1;
@@ -314,8 +335,10 @@ __global__ void __launch_bounds__(1024) k_rhs_x(Geo g, RkCoef rk, double cvisc, 
         double* red = reinterpret_cast<double*>(smem + sizeof(double) * 3 * KR * LS);
         red[threadIdx.x] = msum;
         __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        int half = 1;
+        while (2 * half < (int)blockDim.x) half *= 2;
+        for (int s = half; s > 0; s >>= 1) {   // any block size (2 L threads)
+            if ((int)threadIdx.x < s && threadIdx.x + s < blockDim.x) red[threadIdx.x] += red[threadIdx.x + s];
             __syncthreads();
         }
         if (threadIdx.x == 0 && active) part[t] = red[0];
@@ -375,10 +398,12 @@ __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, 
     double w[W];
 #pragma unroll
     for (int m = 0; m < W; ++m) w[m] = wx[m];
-    const int lg4 = lgL - 2;
+    const int L4 = L / 4, lg4 = lgL >= 0 ? lgL - 2 : -1;
+    const float L4inv = 1.0f / (float)L4;
     const int LMN = L * g.M * g.N;
-    for (int e = threadIdx.x; e < ((3 * KR) << lg4); e += blockDim.x) {
-        const int cr = e >> lg4, g4 = e & ((1 << lg4) - 1);
+    for (int e = threadIdx.x; e < 3 * KR * L4; e += blockDim.x) {
+        int cr, g4;
+        divmod_row(e, L4, lg4, L4inv, cr, g4);
         const int c = cr / KR, r = cr - c * KR;
         if (c == 1 && j == 0) continue;
         // padded elements 4*g4 + t, t = 0 .. W+2: sub-row t % 4, entry g4 + t / 4
@@ -414,14 +439,8 @@ __global__ void __launch_bounds__(256) k_rhs_xw(Geo g, RkCoef rk, double cvisc, 
 // of row j goes to part[kg*M + j], the slot k_rhs_xw's tile (j, kg) writes.
 // ---------------------------------------------------------------------------
 constexpr int XM_KP = KRS + 2;  // k rows of a ring plane
-#ifndef CHS_XM_PD
-#define CHS_XM_PD 1
-#endif
-#ifndef CHS_XM_CPT
-#define CHS_XM_CPT 1
-#endif
-constexpr int XM_PD = CHS_XM_PD;     // steps the ring's next rows are loaded ahead
-constexpr int XM_CPT = CHS_XM_CPT;   // cells of a row per thread (1 or 2)
+constexpr int XM_PD = 1;        // steps the ring's next rows are loaded ahead (2 with 2 cells: no faster)
+constexpr int XM_CPT = 1;       // cells of a row per thread (1 or 2; 2 measured no faster)
 
 // LDS of k_rhs_xm in compute type T: the ring and the x-solve rows in T,
 // then (8-byte aligned) the PR plane in FP64 and 16 doubles of block sums
@@ -470,7 +489,8 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
     const int j0 = (t - kg * nyc) * jc, j1 = min(M, j0 + jc);
     const int k0 = kg * KR;
     const int tid = threadIdx.x;
-    const int rb = tid >> lgL, i = tid & (L - 1);
+    int rb, i;
+    divmod_row(tid, L, lgL, 1.0f / (float)L, rb, i);
     const int LP = L * (M + 2), CS = LP * N, LMN = L * M * N;
     // The thread's cells (k rows rb + u*RB) and one halo row: k0-1 (rb == 0),
     // k0+KR (rb == RB-1), or, for the rows between, its first cell once more
@@ -526,7 +546,8 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
 #pragma unroll
         for (int c = 0; c < 3; ++c) h0[u][c] = T(H0[i + L * j0 + L * M * kc[u] + c * LMN]);
     }
-    const int lg4 = lgL - 2;
+    const int L4 = L / 4, lg4 = lgL >= 0 ? lgL - 2 : -1;
+    const float L4inv = 1.0f / (float)L4;
     const int nw = blockDim.x >> 6;
     // Nothing loaded above may still be in flight when the loop starts:
     // otherwise every step would open with a wait for all outstanding memory
@@ -623,8 +644,9 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
         }
         pln[lph] = pa.ph;
         // x stencil of the 3*KR rows, 4 outputs per work item
-        for (int e = tid; e < ((3 * KR) << lg4); e += blockDim.x) {
-            const int cr = e >> lg4, g4 = e & ((1 << lg4) - 1);
+        for (int e = tid; e < 3 * KR * L4; e += blockDim.x) {
+            int cr, g4;
+            divmod_row(e, L4, lg4, L4inv, cr, g4);
             const int c = cr / KR, rr = cr - c * KR;
             if (c == 1 && j == 0) continue;
             const T* xrow = xr + cr * LS + g4;
@@ -1499,11 +1521,12 @@ static int cn_stencils_locked(chs_handle* h, double facoe, CnStencils** out) {
     return CHS_OK;
 }
 
-// The x solve of k_rhs_x uses L/XMS segments of one block (XLPB*L/XMS threads,
-// a power of two for the block reductions); the z rows come in groups of KR.
+// The x solve of k_rhs_x uses L/XMS segments of one block (XLPB*L/XMS
+// threads); the z rows come in groups of KR.  L needs no power of two: the
+// RHS kernels split thread indices by a division when it is not one.
 bool substep_supported(const chs_handle* h) {
     const int P = h->L / XMS;
-    return h->L % XMS == 0 && P >= 2 && XLPB * P <= 1024 && (P & (P - 1)) == 0 && h->N % KR == 0;
+    return h->L % XMS == 0 && P >= 2 && XLPB * P <= 1024 && h->N % KR == 0;
 }
 
 // last substep's CN solve path: 1 = truncated-inverse stencils, 0 = segmented Thomas
@@ -1543,7 +1566,8 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     {
         const double COE = -facoe * h->dxqi;
         const double xa = COE, xc = COE, xb = 1.0 - xc - xa;
-        const int lgL = __builtin_ctz((unsigned)h->L);  // L is a power of two (substep_supported)
+        // log2 L when L is a power of two (index shifts in the RHS kernels), else -1
+        const int lgL = (h->L & (h->L - 1)) == 0 ? __builtin_ctz((unsigned)h->L) : -1;
         // 4 k rows per row-tile block: measured at 256x192x256, 8 rows (fewer
         // k halo rows per output) ran 1.3% slower (0.452 vs 0.446 ms)
         const int krs = KRS;
@@ -1552,7 +1576,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const dim3 grx(((nblk + 7) / 8) * 8);
         // y-marching form: XM_CPT cells of each row per thread (whole waves)
         // and its LDS ring on this device
-        const bool use_xm = use_st && h->cn_mode == 0 && (KRS / XM_CPT) * h->L >= 64 &&
+        const bool use_xm = use_st && h->cn_mode == 0 && (KRS / XM_CPT) * h->L % 64 == 0 &&
                             (KRS / XM_CPT) * h->L <= (XM_CPT == 1 ? 1024 : 512) &&
                             rhs_xm_smem(h->L, cn->KWx) <= (size_t)h->lds_max;
         // the mixed form's FP32 stencils: RHS, the x solve and the z solve in

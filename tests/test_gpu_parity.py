@@ -211,10 +211,15 @@ def _run_pair(dims, nsteps, seed=1973, precision="fp64", solver_mod=None, cn="au
 @pytest.mark.parametrize("dims,cn,path", [((32, 16, 16), "auto", "thomas"), ((64, 64, 64), "auto", "stencil"),
                                           ((64, 64, 64), "thomas", "thomas"), ((128, 96, 64), "auto", "stencil"),
                                           ((64, 64, 64), "rowtile", "stencil"), ((32, 40, 64), "auto", "stencil"),
-                                          ((32, 48, 128), "auto", "stencil"), ((32, 43, 64), "auto", "stencil")])
+                                          ((32, 48, 128), "auto", "stencil"), ((32, 43, 64), "auto", "stencil"),
+                                          ((96, 48, 64), "auto", "stencil"), ((96, 48, 64), "rowtile", "stencil"),
+                                          ((192, 40, 64), "auto", "stencil"), ((40, 24, 16), "auto", "thomas"),
+                                          ((120, 24, 16), "thomas", "thomas")])
 def test_substep_fp64(solver_mod, dims, cn, path):
     """7 substeps (NS = 0, then 1,2,3 twice) against the oracle; every state
-    field within 1e-12 of the oracle after every substep."""
+    field within 1e-12 of the oracle after every substep.  NCL1 = 96, 192,
+    40, 120 (factors 3 and 5: no power of two) run the same kernels with
+    their thread indices split by division."""
     print(f"substep fp64 {dims} cn={cn}", flush=True)
     g, out = _run_pair(dims, 2, solver_mod=solver_mod, cn=cn, expect_path=path)
     for i, (ns, gpu, ref, dg, dr, pg, pr_) in enumerate(out):
@@ -252,12 +257,13 @@ def test_rhs_marching_matches_rowtile(solver_mod, dims):
         assert e < 1e-13, (name, e)
 
 
-def test_substep_mixed_divergence(solver_mod):
+@pytest.mark.parametrize("dims", [(64, 64, 64), (192, 40, 64)])
+def test_substep_mixed_divergence(solver_mod, dims):
     """The mixed form (FP32 stencils and x/z solves, FP64 y elimination, FP32
     Poisson refined once with its residual checked in the projection): the
     projected divergence stays below 1e-10, the state close to the FP64
-    oracle, and the check's verdict is read back (no error, residual known)."""
-    dims = (64, 64, 64)
+    oracle, and the check's verdict is read back (no error, residual known).
+    NCL1 = 192: the Stockham x transforms and the stored FP32 residual."""
     g, out = _run_pair(dims, 1, precision="mixed", solver_mod=solver_mod)
     for ns, gpu, ref, dg, dr, *_ in out:
         print(ns, "div", dg, "Q", _rel(gpu[0], ref[0]), "PR", _rel(gpu[1], ref[1]), flush=True)
