@@ -826,11 +826,12 @@ constexpr int KMAX = 16;
 // y: Delta_c(j) = sum_m W_c[j][m] R_c(j+m), rows clipped to [0, M); for v the
 // row-1 right-hand side is the u increment on j = 1 (what RHS_tg holds there in
 // the reference).  Then Q += Delta from N2I (MOMFA_tg:39-45) and BC_WALL_Q_tg.
-// y, register form: one thread per (i, k) column and chunk of YJC rows; the
+// y, register form: one thread per (i, k) column and chunk of jc rows (YJC,
+// shorter where the grid is small); the
 // window R(j-KW..j+KW) slides along j in registers and every lane of a wave
 // (64 consecutive i) applies the same row weights W_c[j][.] (scalar loads).
 // wy rows are zero-padded to 2KW+1 entries (zero outside [0, M)).
-constexpr int YJC = 64;     // rows per thread chunk of the y window (measured: 32 -> 64 rows, 257 -> 240 us)
+constexpr int YJC = 64;     // rows per thread chunk of the y window (measured at 256x192x256: 32 -> 64 rows, 257 -> 240 us)
 constexpr int YNS = 4;      // prefetch slots (rows fetched ahead) of the y window (2 -> 4: 200 -> 192 us)
 constexpr int WU = 4;       // rows fetched ahead per step of the z window (2: 140 -> 153 us; slots
                             // refilled a step ahead as in the y window: 133 -> 141 us)
@@ -839,16 +840,16 @@ template <int KW, class TR = double>
 // weight rows, so more blocks fit a CU.  TR: the storage type of the z-solved
 // right-hand side (float in the mixed form); the y elimination is FP64.
 __global__ void __launch_bounds__(64) k_y_window(Geo g, const TR* __restrict__ R2, double* __restrict__ Q,
-                                                 const double* __restrict__ wy) {
+                                                 const double* __restrict__ wy, int jc) {
     const int L = g.L, M = g.M, N = g.N;
     const int ntx = (L + 63) / 64;
-    const int nch = (M + YJC - 1) / YJC;
+    const int nch = (M + jc - 1) / jc;
     const int lane = threadIdx.x, ch = blockIdx.z;
     const int it = blockIdx.x % ntx, k = blockIdx.x / ntx;
     extern __shared__ double sw[];   // the chunk's row weights [j1 - j0][W], from row j0
     const bool valid = it * 64 + lane < L;
     const int i = valid ? it * 64 + lane : 0;
-    const int j0 = ch * YJC, jn = min(M, j0 + YJC), j1 = valid ? jn : j0;   // invalid lanes do no rows
+    const int j0 = ch * jc, jn = min(M, j0 + jc), j1 = valid ? jn : j0;   // invalid lanes do no rows
     const size_t LM = (size_t)L * M;
     constexpr int W = 2 * KW + 1;
     double du0 = 0.0;
@@ -1540,6 +1541,41 @@ int cn_last_path(chs_handle* h, double facoe, int* kx, int* kz, int* ky) {
     return cn->ok && h->cn_mode != 1 && h->N % ZKC == 0 && (h->M + YJC - 1) / YJC <= 8 && h->M > 16 ? 1 : 0;
 }
 
+// measured at 64^3 / 128^3 / 256x192x256 (FP64 substep): 2048 waves
+// 94.8 -> 75.1 / 259.5 -> 250.7 / 1071 -> 1073 us; 4096 the same, 8192 slower
+// on the large grid (1086 us)
+constexpr long WIN_WAVES = 2048;
+// Chunk lengths of the y and z windows.  A window marches the rows of its
+// chunk one after the other, so on a grid too small to give every SIMD a few
+// waves the march is latency-bound: halve the chunk (more waves, each
+// re-reading the window's 2KW rows once more) until the launch has
+// WIN_WAVES waves or the chunk is 8 rows.
+static int ywin_chunk(const chs_handle* h) {
+    const long per = 3L * h->N * ((h->L + 63) / 64);   // waves per chunk of every column
+    int jc = YJC;
+    while (jc > 8 && per * ((h->M + jc - 1) / jc) < WIN_WAVES) jc /= 2;
+    return jc;
+}
+// Rows per y chunk of the marching RHS kernel: about one block per CU, or
+// (FP64 ring: fill) one for each block the LDS lets a CU hold (the ring
+// takes most of a CU's LDS at L = 256: one; 128: two); chunks of >= 4 rows,
+// none empty.  Measured (substep): FP64 at 128^3, 256 blocks 259.5 us, 512
+// blocks 245 us; the FP32 ring at 128^3 302 -> 304 us and at 256x192x256
+// 1069 -> 1092 us with 512 blocks, so it keeps one block per CU.
+static int xm_chunk(const chs_handle* h, size_t smem, bool fill) {
+    const int nkb = h->N / KRS;
+    const int bpc = fill ? std::max(1, std::min(4, (int)(h->lds_max / std::max<size_t>(smem, 1)))) : 1;
+    const int nyc = std::max(1, std::min(h->M / 4, (256 * bpc + nkb - 1) / nkb));
+    return (h->M + nyc - 1) / nyc;
+}
+// zkc divides N and is a multiple of WU (N is a multiple of ZKC on this path)
+static int zwin_chunk(const chs_handle* h) {
+    const long per = 3L * h->M * ((h->L + 63) / 64);
+    int zkc = h->N % (2 * ZKC) == 0 ? 2 * ZKC : ZKC;
+    while (zkc > 8 && per * (h->N / zkc) < WIN_WAVES) zkc /= 2;
+    return zkc;
+}
+
 int rk3_substep(chs_handle* h, int ns, double dt) {
     CHS_TRY(mixed_check(h, false));
     const Geo g = geo_of(h);
@@ -1549,7 +1585,6 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     rk.troh = TROH[ns];
     rk.talp = TGAM[ns] + TROH[ns];
     rk.dt = dt;
-    const size_t ncell = (size_t)h->L * h->M * h->N;
     const double facoe = 0.5 * rk.talp * dt * h->cvisc;  // FACOE_TG (src/MOMFA_tg.f90:29)
     hipEvent_t ev;
     // the handle's stencil cache stays locked until the substep's kernels
@@ -1585,12 +1620,10 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         f32 = h->precision == CHS_MIXED && use_xm;
         h->timer.begin("rhs_x", s, &ev);
         if (f32) {
-            const int nkb = h->N / KRS;
-            int nyc = std::max(1, std::min(h->M / 4, (256 + nkb - 1) / nkb));
-            const int jc = (h->M + nyc - 1) / nyc;
-            nyc = (h->M + jc - 1) / jc;
-            const dim3 gm(8 * ((nkb * nyc + 7) / 8));
             const size_t sm = rhs_xm_smem(h->L, cn->KWx, sizeof(float));
+            const int jc = xm_chunk(h, sm, false);
+            const int nkb = h->N / KRS, nyc = (h->M + jc - 1) / jc;
+            const dim3 gm(8 * ((nkb * nyc + 7) / 8));
             const double* wx = cn->dev + cn->ox;
             float* rf = reinterpret_cast<float*>(h->rhs);
 #define CHS_RHSXM32(KX)                                                                                          \
@@ -1608,14 +1641,10 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
             }
 #undef CHS_RHSXM32
         } else if (use_xm) {
-            // about one block per CU (the ring takes most of a CU's LDS at
-            // L = 256), chunks of >= 4 rows, none empty
-            const int nkb = h->N / KRS;
-            int nyc = std::max(1, std::min(h->M / 4, (256 + nkb - 1) / nkb));
-            const int jc = (h->M + nyc - 1) / nyc;
-            nyc = (h->M + jc - 1) / jc;
-            const dim3 gm(8 * ((nkb * nyc + 7) / 8));
             const size_t sm = rhs_xm_smem(h->L, cn->KWx);
+            const int jc = xm_chunk(h, sm, true);
+            const int nkb = h->N / KRS, nyc = (h->M + jc - 1) / jc;
+            const dim3 gm(8 * ((nkb * nyc + 7) / 8));
             const double* wx = cn->dev + cn->ox;
 #define CHS_RHSXM(KX)                                                                                  \
     do {                                                                                               \
@@ -1664,7 +1693,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const int usedpg0 = 1;
         if (use_st) {
             // 128-row chunks where N allows (measured at 256x192x256: 139 -> 134 us)
-            const int zkc = h->N % (2 * ZKC) == 0 ? 2 * ZKC : ZKC;
+            const int zkc = zwin_chunk(h);
             const dim3 gzw(3 * h->M * ((h->L + 63) / 64), (h->N / zkc + 3) / 4);
             const double* wz = cn->dev + cn->oz;
             h->timer.begin("z_solve", s, &ev);
@@ -1709,25 +1738,26 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
     }
     // 3. y solves + Q update + BC_WALL_Q
     if (use_st) {
-        const int nch = (h->M + YJC - 1) / YJC;
+        const int yjc = ywin_chunk(h);
+        const int nch = (h->M + yjc - 1) / yjc;
         const dim3 gy(h->N * ((h->L + 63) / 64), 3, nch), by(64);
         const double* wy = cn->dev + cn->oy;
-        const size_t smw = sizeof(double) * (size_t)std::min(h->M, YJC) * (2 * cn->KW + 1);
+        const size_t smw = sizeof(double) * (size_t)std::min(h->M, yjc) * (2 * cn->KW + 1);
         h->timer.begin("y_solve", s, &ev);
         if (f32) {
             const float* r2f = reinterpret_cast<const float*>(h->rhs2);
             switch (cn->KW) {
-                case 4: CHS_LAUNCH((k_y_window<4, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
-                case 8: CHS_LAUNCH((k_y_window<8, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
-                case 12: CHS_LAUNCH((k_y_window<12, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
-                default: CHS_LAUNCH((k_y_window<16, float>), gy, by, smw, s, g, r2f, h->q, wy); break;
+                case 4: CHS_LAUNCH((k_y_window<4, float>), gy, by, smw, s, g, r2f, h->q, wy, yjc); break;
+                case 8: CHS_LAUNCH((k_y_window<8, float>), gy, by, smw, s, g, r2f, h->q, wy, yjc); break;
+                case 12: CHS_LAUNCH((k_y_window<12, float>), gy, by, smw, s, g, r2f, h->q, wy, yjc); break;
+                default: CHS_LAUNCH((k_y_window<16, float>), gy, by, smw, s, g, r2f, h->q, wy, yjc); break;
             }
         } else {
             switch (cn->KW) {
-                case 4: CHS_LAUNCH(k_y_window<4>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
-                case 8: CHS_LAUNCH(k_y_window<8>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
-                case 12: CHS_LAUNCH(k_y_window<12>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
-                default: CHS_LAUNCH(k_y_window<16>, gy, by, smw, s, g, h->rhs2, h->q, wy); break;
+                case 4: CHS_LAUNCH(k_y_window<4>, gy, by, smw, s, g, h->rhs2, h->q, wy, yjc); break;
+                case 8: CHS_LAUNCH(k_y_window<8>, gy, by, smw, s, g, h->rhs2, h->q, wy, yjc); break;
+                case 12: CHS_LAUNCH(k_y_window<12>, gy, by, smw, s, g, h->rhs2, h->q, wy, yjc); break;
+                default: CHS_LAUNCH(k_y_window<16>, gy, by, smw, s, g, h->rhs2, h->q, wy, yjc); break;
             }
         }
         h->timer.end(ev, "y_solve", s);
