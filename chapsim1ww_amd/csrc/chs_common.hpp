@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <mutex>
@@ -172,6 +173,17 @@ __device__ __forceinline__ bool row_tile(int L, int N, int nj, int* it, int* j, 
 inline unsigned row_tile_blocks(int L, int N, int nj) {
     const long nt = (long)((L + 63) / 64) * ((N + 3) / 4) * nj;
     return (unsigned)(8 * ((nt + 7) / 8));
+}
+
+// Rows per y chunk of a kernel that marches along y with an LDS ring, one
+// block per group of k rows (nkg groups) and chunk of rows: about one block
+// per CU (256 on MI355X), or, with fill, one for each block that the ring's
+// LDS (smem bytes of lds_max) lets a CU hold, at most four; chunks of at
+// least 4 rows, none empty.
+inline int march_chunk_rows(int rows, int nkg, size_t smem, int lds_max, bool fill) {
+    const int bpc = fill ? std::max(1, std::min(4, (int)((size_t)lds_max / std::max<size_t>(smem, 1)))) : 1;
+    const int nyc = std::max(1, std::min(rows / 4, (256 * bpc + nkg - 1) / nkg));
+    return (rows + nyc - 1) / nyc;
 }
 
 struct CnCache;   // truncated-inverse CN stencils of one handle (substep.hip)

@@ -349,10 +349,13 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     EGeo g{h->L, h->M, h->N, h->dxi, h->dzi, h->dxqi, h->dzqi, h->dyfi, h->dyci, h->wff, h->wfb};
     EFields f{tf->rhoh, tf->enthalpy, tf->temperature, tf->density, tf->density0, tf->density1, tf->viscosity,
               tf->viscosity0, tf->thermcondt, tf->rhs_energy0, tf->g, tf->g0, tf->wallflux};
+    // the range check of the previous call on p, read now (its copy has
+    // long finished when the momentum work was queued in between)
+    CHS_TRY(energy_range_check(p, "chs_energy_substep"));
     const hipStream_t s = h->stream;
     const double cap = p->rhoh_max + 1.0e-14;   // RHOHmax + REALMIN
     hipEvent_t ev;
-    CHS_LAUNCH(k_zero_u64_e, dim3(1), dim3(64), 0, s, p->counters, 4);
+    CHS_LAUNCH(k_zero_u64_e, dim3(1), dim3(64), 0, s, p->ecnt, 4);
     // the marching form needs L a power of two with 4L threads in whole
     // waves (at most 1024) and its LDS ring on this device; chs_set_cn_solver
     // mode 2 selects the row-tile form
@@ -362,10 +365,10 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     h->timer.begin("energy_rhs", s, &ev);
     if (march) {
         const int nkg = h->N / EKR;
-        int nyc = std::max(1, std::min(h->M / 4, (256 + nkg - 1) / nkg));
-        const int jc = (h->M + nyc - 1) / nyc;
-        nyc = (h->M + jc - 1) / jc;
         const size_t sm = energy_rhs_m_smem(L);
+        // one block per CU: filling the LDS (two per CU at 128^3) measured no faster
+        const int jc = march_chunk_rows(h->M, nkg, sm, h->lds_max, false);
+        const int nyc = (h->M + jc - 1) / jc;
         (void)hipFuncSetAttribute((const void*)k_energy_rhs_m, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         CHS_LAUNCH(k_energy_rhs_m, dim3(8 * ((nkg * nyc + 7) / 8)), dim3(EKR * L), sm, s, g, f, TGAM_E[ns] * dt,
                    TROH_E[ns] * dt, cthecd, cap, nyc, jc, __builtin_ctz((unsigned)L));
@@ -382,7 +385,7 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     do {                                                                                                       \
         CHS_TRY(props_lds_attr((const void*)k_energy_props<KK>, p));                                           \
         CHS_LAUNCH(k_energy_props<KK>, dim3(props_grid_occ(p, (const void*)k_energy_props<KK>, EPT, 1 << 20)),  \
-                   dim3(EPT), props_lds_bytes(p), s, t, g, f, p->counters, sur ? 0 : 1);                       \
+                   dim3(EPT), props_lds_bytes(p), s, t, g, f, p->ecnt, sur ? 0 : 1);                           \
     } while (0)
     switch (p->kwalk) {
         case 2: CHS_EPROPS(2); break;
@@ -397,19 +400,17 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
         const Rows rw{(long long)h->N, (long long)h->L * h->M, (long long)h->L * (h->M + 2)};
         const size_t o = (size_t)h->L;   // row J = 1
         CHS_TRY(mlp_update(p, tf->enthalpy + o, rw, tf->temperature + o, tf->density + o, tf->viscosity + o,
-                           tf->thermcondt + o, nullptr, p->counters, s));
+                           tf->thermcondt + o, nullptr, p->ecnt, s));
     }
     h->timer.end(ev, "energy_props", s);
-    CHS_LAUNCH(k_energy_wall, dim3(64), dim3(256), props_lds_bytes(p), s, t, g, f, p->counters);
-    unsigned long long cnt[2] = {0, 0};
-    CHS_HIP(hipMemcpyAsync(cnt, p->counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
-    CHS_HIP(hipStreamSynchronize(s));
-    const unsigned long long bad = cnt[0];
-    p->last_fallback = sur ? (long long)cnt[1] : 0;
-    if (bad) {
-        set_error("chs_energy_substep: " + std::to_string(bad) + " property lookups outside the table");
-        return CHS_E_RANGE;
-    }
+    CHS_LAUNCH(k_energy_wall, dim3(64), dim3(256), props_lds_bytes(p), s, t, g, f, p->ecnt);
+    // the counts of lookups outside the table and of surrogate fallbacks to
+    // pinned host memory, read by the next call on p (or chs_props_check,
+    // chs_props_last_fallback) without a stream sync here
+    CHS_HIP(hipMemcpyAsync(p->ecnt_host, p->ecnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    CHS_HIP(hipEventRecord(p->ecnt_ev, s));
+    p->ecnt_pending = true;
+    p->ecnt_sur = sur;
     return CHS_OK;
 }
 

@@ -1575,12 +1575,13 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     h->timer.begin("io_terms", s, &ev);
     if (march) {
         const int nkg = h->N / IOKR;
-        int nyc = std::max(1, std::min(h->M / 4, (256 + nkg - 1) / nkg));
-        const int jc = (h->M + nyc - 1) / nyc;
-        nyc = (h->M + jc - 1) / jc;
+        const size_t sm = io_terms_m_smem(L);
+        // as many blocks per CU as the ring lets it hold: at 128^3 two
+        // (heated substep 552.9 -> 535.4 us); one at L = 256
+        const int jc = march_chunk_rows(h->M, nkg, sm, h->lds_max, true);
+        const int nyc = (h->M + jc - 1) / jc;
         nbt = (long)nkg * h->M;
         const dim3 gm(8 * ((nkg * nyc + 7) / 8));
-        const size_t sm = io_terms_m_smem(L);
         const int lgL = __builtin_ctz((unsigned)L);
 #define IO_TERMS_M(GR)                                                                                             \
     do {                                                                                                           \

@@ -335,9 +335,16 @@ int props_create(int n, const double* H, const double* T, const double* D, const
     p->xt = p->xh + n;
     p->sh = reinterpret_cast<int*>(p->xt + n);
     p->st = p->sh + p->nbh;
-    if (hipMalloc((void**)&p->counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
-        set_error("chs_props_create: device allocation failed");
+    if (hipMalloc((void**)&p->counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc((void**)&p->ecnt_host, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        set_error("chs_props_create: allocation of the counters failed");
         return fail(CHS_E_NOMEM);
+    }
+    p->ecnt = p->counters + 4;
+    p->ecnt_host[0] = p->ecnt_host[1] = 0;
+    if (hipEventCreateWithFlags(&p->ecnt_ev, hipEventDisableTiming) != hipSuccess) {
+        set_error("chs_props_create: event creation failed");
+        return fail(CHS_E_HIP);
     }
     if (hipMemcpy(p->rh, p->hrh.data(), p->hrh.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->rt, p->hrt.data(), p->hrt.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -359,12 +366,42 @@ int mlp_update(chs_props* p, const double* h, Rows rw, double* T, double* rho, d
 
 int props_destroy(chs_props* p) {
     if (!p) return CHS_OK;
+    if (p->ecnt_pending) (void)hipEventSynchronize(p->ecnt_ev);   // the copy into ecnt_host
     mlp_release(p);
     if (p->rh) (void)hipFree(p->rh);
     if (p->counters) (void)hipFree(p->counters);
+    if (p->ecnt_host) (void)hipHostFree(p->ecnt_host);
+    if (p->ecnt_ev) (void)hipEventDestroy(p->ecnt_ev);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return CHS_OK;
+}
+
+// The range check of the last chs_energy_substep: waits for its counters'
+// copy (once), then reports cells outside the table once, as CHS_E_RANGE.
+void energy_range_resolve(chs_props* p) {
+    if (!p->ecnt_pending) return;
+    if (hipEventSynchronize(p->ecnt_ev) != hipSuccess) {
+        p->ecnt_bad = ~0ull;   // unknown: reported as an error below
+    } else {
+        p->ecnt_bad = p->ecnt_host[0];
+        p->last_fallback = p->ecnt_sur ? (long long)p->ecnt_host[1] : 0;
+    }
+    p->ecnt_pending = false;
+}
+
+int energy_range_check(chs_props* p, const char* who) {
+    energy_range_resolve(p);
+    const unsigned long long bad = p->ecnt_bad;
+    if (!bad) return CHS_OK;
+    p->ecnt_bad = 0;
+    if (bad == ~0ull) {
+        set_error(std::string(who) + ": the range check of the last chs_energy_substep could not be read");
+        return CHS_E_HIP;
+    }
+    set_error(std::string(who) + ": the last chs_energy_substep had " + std::to_string(bad) +
+              " property lookups outside the table");
+    return CHS_E_RANGE;
 }
 
 int props_update(chs_props* p, int mode, const double* h, long long nrow, long long rowlen, long long stride,
@@ -417,8 +454,14 @@ int chs_props_set_mode(chs_props* p, int mode) {
 
 int chs_props_last_fallback(chs_props* p, long long* n_fallback) {
     CHS_CHECK(p != nullptr && n_fallback != nullptr, CHS_E_ARG, "chs_props_last_fallback: NULL argument");
+    chs::energy_range_resolve(p);
     *n_fallback = p->last_fallback;
-    return CHS_OK;
+    return chs::energy_range_check(p, "chs_props_last_fallback");
+}
+
+int chs_props_check(chs_props* p) {
+    CHS_CHECK(p != nullptr, CHS_E_ARG, "chs_props_check: NULL handle");
+    return chs::energy_range_check(p, "chs_props_check");
 }
 
 int chs_props_rhoh_max(chs_props* p, double* rhoh_max) {

@@ -36,6 +36,15 @@ struct chs_props {
     // CHS_PROPS_SURROGATE) and the cells its last call handed to the table
     int mode = 0;
     long long last_fallback = 0;
+    // the energy substep's range check, read without a stream sync: its
+    // device counters (after the four of the other calls), their pinned host
+    // copy, the event that follows the copy, and a resolved count not yet
+    // reported
+    unsigned long long* ecnt = nullptr;
+    unsigned long long* ecnt_host = nullptr;
+    hipEvent_t ecnt_ev = nullptr;
+    bool ecnt_pending = false, ecnt_sur = false;
+    unsigned long long ecnt_bad = 0;
 };
 
 namespace chs {
@@ -45,6 +54,12 @@ namespace chs {
 struct Rows {
     long long nrow, rowlen, stride;
 };
+
+// The range check of the last chs_energy_substep on p (props.hip):
+// energy_range_resolve waits for its pending copy; energy_range_check also
+// reports cells outside the table, once, as CHS_E_RANGE.
+void energy_range_resolve(chs_props* p);
+int energy_range_check(chs_props* p, const char* who);
 
 // LDS bytes of the staged search tables of one lookup block
 inline size_t props_lds_bytes(const chs_props* p) {

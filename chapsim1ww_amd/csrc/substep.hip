@@ -1563,10 +1563,7 @@ static int ywin_chunk(const chs_handle* h) {
 // blocks 245 us; the FP32 ring at 128^3 302 -> 304 us and at 256x192x256
 // 1069 -> 1092 us with 512 blocks, so it keeps one block per CU.
 static int xm_chunk(const chs_handle* h, size_t smem, bool fill) {
-    const int nkb = h->N / KRS;
-    const int bpc = fill ? std::max(1, std::min(4, (int)(h->lds_max / std::max<size_t>(smem, 1)))) : 1;
-    const int nyc = std::max(1, std::min(h->M / 4, (256 * bpc + nkb - 1) / nkb));
-    return (h->M + nyc - 1) / nyc;
+    return march_chunk_rows(h->M, h->N / KRS, smem, h->lds_max, fill);
 }
 // zkc divides N and is a multiple of WU (N is a multiple of ZKC on this path)
 static int zwin_chunk(const chs_handle* h) {

@@ -49,6 +49,9 @@ class ThermalState:
         return {k: t.cpu().numpy().reshape(self.shapes[k], order="F") for k, t in self.dev.items()}
 
     def solverrk3_eng_io(self, ns: int, dt: float, cthecd: float):
+        """SOLVERRK3_ENG_IO(NS) on the device, queued on the solver's stream.
+        A property lookup outside the table is reported by the next call with
+        the same table (before it does any work) or by table.check()."""
         f = nat.ThermalFields(*[nat.dptr(self.dev[k]) for k in NAMES])
         nat.check(self.solver.lib.chs_energy_substep(self.solver.h, self.table.p, C.byref(f), int(ns), float(dt),
                                                      float(cthecd)))

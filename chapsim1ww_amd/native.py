@@ -103,6 +103,7 @@ _SIGS = {
     "chs_surrogate_certified": ([C.c_void_p, _dp, _dp], C.c_int),
     "chs_props_set_mode": ([C.c_void_p, C.c_int], C.c_int),
     "chs_props_last_fallback": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
+    "chs_props_check": ([C.c_void_p], C.c_int),
     "chs_mean_xz_io": ([C.c_void_p, C.POINTER(ThermalFields), C.POINTER(IoFields), C.c_int, _dp], C.c_int),
     "chs_mean_t": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _dp], C.c_int),
     "chs_mean_t_reset": ([C.c_void_p, C.c_int], C.c_int),

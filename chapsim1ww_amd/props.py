@@ -134,6 +134,12 @@ class PropertyTable:
         m = {"table": nat.CHS_PROPS_TABLE, "surrogate": nat.CHS_PROPS_SURROGATE}[mode]
         nat.check(self.lib.chs_props_set_mode(self.p, m))
 
+    def check(self):
+        """The range check of the last energy substep with this table: raises
+        ChsError (CHS_E_RANGE) if a lookup left the table and the error was
+        not reported yet (the energy substep reads its counts one call later)."""
+        nat.check(self.lib.chs_props_check(self.p))
+
     def last_fallback(self) -> int:
         """Cells of the last energy substep the surrogate handed to the table."""
         v = C.c_longlong()

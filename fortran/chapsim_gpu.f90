@@ -139,6 +139,10 @@ module chapsim_gpu
             integer(c_long_long), value :: nrow, rowlen, stride
             type(c_ptr), value :: t, rho, mu, kap, cp, nfb
         end function
+        integer(c_int) function c_props_check(p) bind(c, name='chs_props_check')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: p
+        end function
         integer(c_int) function c_energy(h, p, tf, ns, dt, cthecd) bind(c, name='chs_energy_substep')
             import :: c_int, c_ptr, c_double, chs_thermal_fields
             type(c_ptr), value :: h, p
@@ -433,6 +437,8 @@ contains
         integer, intent(in) :: ns
         real(c_double), intent(in) :: dt, cthecd
         call chs_check(c_energy(handle, props, heat_tf(), int(ns, c_int), dt, cthecd), 'SOLVERRK3_ENG_IO')
+        ! the reference stops inside the failing call: wait for its range check here
+        call chs_check(c_props_check(props), 'SOLVERRK3_ENG_IO')
     end subroutine
 
     !> DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered, VELOCITY_CALC_io

@@ -222,8 +222,18 @@ int chs_therm_prop_update(chs_props *p, int mode, const double *h_dev, long long
  * and for ENTHALPY_UPDATE / DENSITY_UPDATE).  No counterpart in the
  * reference.  CHS_E_ARG if the surrogate is not trained and certified. */
 int chs_props_set_mode(chs_props *p, int mode);
-/* Cells of the last chs_energy_substep that the surrogate handed to the table. */
+/* Cells of the last chs_energy_substep that the surrogate handed to the table.
+ * Waits for that call's counts; returns its range error if not yet reported. */
 int chs_props_last_fallback(chs_props *p, long long *n_fallback);
+
+/* The range check of the last chs_energy_substep with p: waits for that
+ * call's counts and returns CHS_E_RANGE if any property lookup left the table
+ * (the reference STOPs in THERM_PROP_UPDATE, src/Flow_State.f90:1030-1046).
+ * A range error is reported once, by the first of: this call,
+ * chs_props_last_fallback, or the next chs_energy_substep with p.  No
+ * counterpart in the reference: the energy substep reads its counts without a
+ * stream sync, so that the host can queue the momentum substep meanwhile. */
+int chs_props_check(chs_props *p);
 
 /* RHOHmax of the table (NIST_SPLINE_H, src/Flow_State.f90:960-990): the
  * limiter of the energy update. */
@@ -248,8 +258,10 @@ typedef struct chs_thermal_fields {
  * RHOH with the RHOHmax limiter, ENTHALPY_UPDATE / DENSITY_UPDATE /
  * THERM_PROP_UPDATE (src/THERM_PROP_UPDATE.f90:1-205) and BC_WALL_THERMAL
  * (src/BC_WALL.f90:522-650).  cthecd = 1/(REN PRT0).  Work is ordered on h's
- * stream; returns CHS_E_RANGE when a lookup leaves the table (the reference
- * STOPs).  Synchronous. */
+ * stream and asynchronous: the count of lookups that left the table is read
+ * by the next call with p, which then returns CHS_E_RANGE before doing any
+ * work (the reference STOPs), or by chs_props_check / chs_props_last_fallback.
+ * Calls that share p are ordered by that wait. */
 int chs_energy_substep(chs_handle *h, chs_props *p, const chs_thermal_fields *f, int ns, double dt,
                        double cthecd);
 

@@ -56,3 +56,49 @@ def test_energy_substeps_bitwise(scaled, dims, mode):
             assert np.array_equal(a, b), (ns, k, np.abs(a - b).max())
     pt.close()
     s.close()
+
+
+def test_energy_range_error_reported_once_without_sync(scaled):
+    """A lookup outside the table inside chs_energy_substep is counted on the
+    device and read one call later: chs_props_check reports it (once), and so
+    does the next energy substep, which then leaves every field untouched."""
+    from chapsim1ww_amd.energy import ThermalState
+    from chapsim1ww_amd.native import CHS_E_RANGE, ChsError
+    from chapsim1ww_amd.solver import ChannelSolver
+    dims = (16, 12, 8)
+    L, M, N = dims
+    print(f"energy range error {dims}", flush=True)
+    g = ChannelGrid(*dims)
+    og = O.OracleGrid(*dims)
+    st, tab, n = O.thermal_case(og, scaled, seed=5)
+    s = ChannelSolver(g)
+    pt = props.PropertyTable(scaled)
+    ts = ThermalState(s, pt)
+    ts.upload({MAP[k]: v for k, v in st.items()})
+    ts.solverrk3_eng_io(1, 2e-3, CTHECD)
+    pt.check()   # in range
+    # the interior cell of largest |RHOH|: with a density far below the
+    # table's, h = RHOH / rho leaves the table
+    rhoh = ts.download()["rhoh"]
+    inner = np.abs(rhoh[:, 1:M + 1, :])
+    i, j, k = np.unravel_index(np.argmax(inner), inner.shape)
+    assert inner[i, j, k] > 1e-3
+    q = int(i + L * (j + 1 + (M + 2) * k))
+    ts.dev["density"][q] = 1e-12
+    ts.solverrk3_eng_io(2, 2e-3, CTHECD)   # returns: the count is read later
+    with pytest.raises(ChsError) as ei:
+        pt.check()
+    assert ei.value.code == CHS_E_RANGE and "1 property lookups" in str(ei.value), str(ei.value)
+    pt.check()   # reported once
+    ts.dev["density"][q] = 1e-12
+    ts.solverrk3_eng_io(3, 2e-3, CTHECD)
+    torch.cuda.synchronize()
+    before = ts.download()
+    with pytest.raises(ChsError) as ei:
+        ts.solverrk3_eng_io(1, 2e-3, CTHECD)
+    assert ei.value.code == CHS_E_RANGE
+    after = ts.download()
+    for key in before:
+        assert np.array_equal(before[key], after[key]), key
+    pt.close()
+    s.close()
