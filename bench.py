@@ -39,7 +39,7 @@ def alg_bytes_per_cell(L, precision="fp64"):
                 "fft_x_fwd": 8 + sp32, "fft_z": 2 * sp32, "poisson_y": 2 * sp32,
                 "fft_x_inv": sp32 + 8,     # read S; write phi1 (FP32) / read phi1, write phi: 4 and 12, averaged
                 "poisson_residual": 24.0,
-                "rhs_x": 92.0,             # read Q x3, PR, CONVH0 x3; write RHS x3 (FP32), CONVH0 x3
+                "rhs_x": 68.0,             # read Q x3, PR, CONVH0 x3 (FP32); write RHS x3, CONVH0 x3 (FP32)
                 "z_solve": 24.0, "y_solve": 60.0, "divg": 32.0, "update": 80.0}
     return {"rhs_x": 104.0,          # read Q x3, PR, CONVH0 x3; write RHS x3, CONVH0 x3
             "z_solve": 48.0,         # read/write RHS x3

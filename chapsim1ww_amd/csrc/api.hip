@@ -17,6 +17,7 @@ int rk3_substep(chs_handle* h, int ns, double dt);
 int divgck_cfl(chs_handle* h, double* div_out, double* cfl_out);
 int mean_xz(chs_handle* h, double* out_host);
 int mixed_check(chs_handle* h, bool wait);
+int convh0_store(chs_handle* h, bool f32);
 int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double abs_tol);
 bool substep_supported(const chs_handle* h);
 void release_stencils(chs_handle* h);
@@ -103,7 +104,7 @@ static void free_all(chs_handle* h) {
     void* ptrs[] = {h->dyfi, h->dyci, h->wff, h->wfb, h->amph, h->acph, h->apph, h->amvr, h->acvr, h->apvr,
                     h->ak1, h->ak3, h->twx, h->twz, h->twx32, h->twz32, h->q, h->pr, h->convh0, h->dph, h->rhs,
                     h->rhsllphi, h->spec, h->spec32, h->rhs_j1, h->partials, h->scalars, h->resid, h->f32a, h->f32b,
-                    h->rhs2, h->stats};
+                    h->rhs2, h->stats, h->h0f};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -213,6 +214,7 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     else {
         chk(dalloc(&h->spec32, (size_t)h->LHP32 * M * N));
         chk(dalloc(&h->resid, ncell));
+        chk(dalloc(&h->h0f, ncell * 3));   // CONVH0 of the FP32 stencils
     }
     h->npart = 65536;
     chk(dalloc(&h->partials, (size_t)h->npart * 2));
@@ -274,7 +276,10 @@ int chs_upload_state(chs_handle* h, const double* q, const double* pr, const dou
     const size_t ncell = (size_t)h->L * h->M * h->N, ngh = (size_t)h->L * (h->M + 2) * h->N;
     if (q) CHS_HIP(hipMemcpyAsync(h->q, q, 3 * ngh * sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (pr) CHS_HIP(hipMemcpyAsync(h->pr, pr, ngh * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    if (convh0) CHS_HIP(hipMemcpyAsync(h->convh0, convh0, 3 * ncell * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (convh0) {
+        CHS_HIP(hipMemcpyAsync(h->convh0, convh0, 3 * ncell * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        h->h0_f32 = false;   // the FP64 store holds CONVH0 now
+    }
     CHS_HIP(hipStreamSynchronize(h->stream));
     return CHS_OK;
 }
@@ -284,7 +289,10 @@ int chs_download_state(chs_handle* h, double* q, double* pr, double* convh0) {
     const size_t ncell = (size_t)h->L * h->M * h->N, ngh = (size_t)h->L * (h->M + 2) * h->N;
     if (q) CHS_HIP(hipMemcpyAsync(q, h->q, 3 * ngh * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (pr) CHS_HIP(hipMemcpyAsync(pr, h->pr, ngh * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (convh0) CHS_HIP(hipMemcpyAsync(convh0, h->convh0, 3 * ncell * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (convh0) {
+        CHS_TRY(chs::convh0_store(h, false));   // widened from the mixed form's FP32 store
+        CHS_HIP(hipMemcpyAsync(convh0, h->convh0, 3 * ncell * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
     CHS_HIP(hipStreamSynchronize(h->stream));
     return mixed_check(h, true);
 }
@@ -293,7 +301,10 @@ int chs_device_arrays(chs_handle* h, double** q, double** pr, double** convh0, d
     CHS_CHECK(h, CHS_E_ARG, "chs_device_arrays: null handle");
     if (q) *q = h->q;
     if (pr) *pr = h->pr;
-    if (convh0) *convh0 = h->convh0;
+    if (convh0) {
+        CHS_TRY(chs::convh0_store(h, false));   // widened from the mixed form's FP32 store
+        *convh0 = h->convh0;
+    }
     if (dph) *dph = h->dph;
     if (rhsllphi) *rhsllphi = h->rhsllphi;
     return CHS_OK;

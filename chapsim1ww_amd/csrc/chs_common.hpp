@@ -207,6 +207,10 @@ struct chs_handle {
     float2 *twx32, *twz32;
     // state in the reference layouts
     double *q, *pr, *convh0;
+    // CONVH0 in FP32 (mixed form, FP32 stencils): h0f holds the current
+    // CONVH0 while h0_f32 is set, convh0 otherwise
+    float* h0f;
+    bool h0_f32;
     // work: DPH(L,0:M+1,N), RHS x3 (L,M,N), RHSLLPHI(L,M,N), spectral S
     double *dph, *rhs, *rhsllphi;
     double *rhs2;        // second RHS buffer of the stencil CN path (z output)

@@ -460,10 +460,10 @@ __device__ __forceinline__ void store4(float* dst, const float* v) {
 
 // T: the compute type of the cells' right-hand sides, the LDS ring and the x
 // solve (double: the FP64 form; float: the mixed form's FP32 stencils, whose
-// RHS leaves in FP32).  Q, PR and CONVH0 stay FP64 in HBM.
+// RHS and CONVH0 are stored in FP32).  Q and PR stay FP64 in HBM.
 template <int KX, int PD, int CPT, class T = double>
 __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef rk, double cvisc, const double* __restrict__ Q,
-                                               const double* __restrict__ PR, double* __restrict__ H0,
+                                               const double* __restrict__ PR, T* __restrict__ H0,
                                                T* __restrict__ RHS, double* __restrict__ part,
                                                const double* __restrict__ wx, int lgL, int nyc, int jc) {
     static_assert(KRS % CPT == 0 && KRS / CPT >= 2, "CPT cells per thread, a low and a high halo row");
@@ -608,7 +608,7 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
             const int ci = i + L * j + L * M * kc[u];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (c != 1 || j != 0) H0[ci + c * LMN] = double(conv[c]);
+                if (c != 1 || j != 0) H0[ci + c * LMN] = conv[c];
                 T* row = xr + (c * KR + rb + RB * u) * LS;
                 row[row_pos<KX>(KX + i, Q4)] = rhs[c];
                 if (i < KX) row[row_pos<KX>(KX + i + L, Q4)] = rhs[c];
@@ -671,6 +671,33 @@ __global__ void __launch_bounds__(CPT == 1 ? 1024 : 512) k_rhs_xm(Geo g, RkCoef 
         s0 = sP;
         sP = s_;
     }
+}
+
+// CONVH0 between its FP64 and FP32 stores (the mixed form's FP32 stencils
+// keep it in FP32 between substeps)
+__global__ void k_narrow(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        b[e] = float(a[e]);
+}
+__global__ void k_widen(const float* __restrict__ b, double* __restrict__ a, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        a[e] = double(b[e]);
+}
+
+// Makes the FP32 (f32) or the FP64 store hold the current CONVH0.
+int convh0_store(chs_handle* h, bool f32) {
+    if (f32 == h->h0_f32) return CHS_OK;
+    if (f32 && !h->h0f) {
+        set_error("substep: the FP32 CONVH0 store of the mixed form is missing");
+        return CHS_E_ARG;
+    }
+    const size_t n = 3 * (size_t)h->L * h->M * h->N;
+    if (f32)
+        CHS_LAUNCH(k_narrow, dim3(2048), dim3(256), 0, h->stream, h->convh0, h->h0f, n);
+    else
+        CHS_LAUNCH(k_widen, dim3(2048), dim3(256), 0, h->stream, h->h0f, h->convh0, n);
+    h->h0_f32 = f32;
+    return CHS_OK;
 }
 
 __global__ void k_set_scalar(double* __restrict__ out, double v) {
@@ -1613,8 +1640,9 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
                             rhs_xm_smem(h->L, cn->KWx) <= (size_t)h->lds_max;
         // the mixed form's FP32 stencils: RHS, the x solve and the z solve in
         // FP32 (RHS and RHS2 stored as float in the first half of the FP64
-        // buffers); CONVH0 and the y elimination stay FP64
+        // buffers), CONVH0 kept in FP32 (h0f); the y elimination stays FP64
         f32 = h->precision == CHS_MIXED && use_xm;
+        CHS_TRY(convh0_store(h, f32));
         h->timer.begin("rhs_x", s, &ev);
         if (f32) {
             const size_t sm = rhs_xm_smem(h->L, cn->KWx, sizeof(float));
@@ -1628,7 +1656,7 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         (void)hipFuncSetAttribute((const void*)k_rhs_xm<KX, XM_PD, XM_CPT, float>,                                \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);                          \
         CHS_LAUNCH((k_rhs_xm<KX, XM_PD, XM_CPT, float>), gm, dim3((KRS / XM_CPT) * h->L), sm, s, g, rk, h->cvisc, \
-                   h->q, h->pr, h->convh0, rf, h->partials, wx, lgL, nyc, jc);                                   \
+                   h->q, h->pr, h->h0f, rf, h->partials, wx, lgL, nyc, jc);                                      \
     } while (0)
             switch (cn->KWx) {
                 case 4: CHS_RHSXM32(4); break;

@@ -72,7 +72,10 @@ int chs_destroy(chs_handle *h);
 /* Copy the tg state between host arrays (reference layouts) and the device. */
 int chs_upload_state(chs_handle *h, const double *q_host, const double *pr_host, const double *convh0_host);
 int chs_download_state(chs_handle *h, double *q_host, double *pr_host, double *convh0_host);
-/* Device addresses of the handle's arrays (any output pointer may be NULL). */
+/* Device addresses of the handle's arrays (any output pointer may be NULL).
+ * The mixed-precision form keeps CONVH0 in FP32 between substeps; asking for
+ * convh0 writes it back to this FP64 array first, which then holds CONVH0 as
+ * of this call (a later substep of that form does not update it). */
 int chs_device_arrays(chs_handle *h, double **q, double **pr, double **convh0, double **dph, double **rhsllphi);
 
 /* One RK3 substep NS (0..3) of the tg channel with time step dt, on the
