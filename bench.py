@@ -298,7 +298,8 @@ def props_bench(g, q_fp64, stream, steps):
         sur_e = dict(ms=round(ms_es, 4), fallback_cells=pt.last_fallback())
         pt.set_mode("table")
     out["energy_substep"] = dict(ms=round(ms_e, 4), substeps_per_s=round(1e3 / ms_e, 2), surrogate_mode=sur_e,
-                                 note="SOLVERRK3_ENG_IO: RHS + RK3 + property cascade + iso-flux walls, synchronous")
+                                 note="SOLVERRK3_ENG_IO: RHS + RK3 + property cascade + iso-flux walls; "
+                                      "its range check is read by the next call")
     # the whole heated-channel substep in SOLVE's order: energy, property
     # staging, io momentum (explicit viscous terms, FFT99 projection)
 
