@@ -228,10 +228,10 @@ __global__ void __launch_bounds__(16 * CB) k_fft256_z(cpx_t<Rl>* __restrict__ S,
 }
 
 // x inverse, L = 256: 16 rows per block of 128 threads; ghost as k_fft_x_inv,
-// and two modes of the mixed form's refinement, on phf (FP32, interior rows,
+// and three modes of the mixed form's refinement, on phf (FP32, interior rows,
 // the layout of f): 3 = write phf only (the first pass's solution), 4 = dph
 // = phf + the transform (the correction added to it), ghost planes as
-// Neumann copies
+// Neumann copies, 5 = as 4 with the ghost planes zeroed
 template <typename Rl>
 __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restrict__ S, double* __restrict__ dph,
                                                       int M, int nrows, int LHP, int ghost,
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restric
             for (int k2 = 0; k2 < 16; ++k2) fr[16 * k2] = (float)(h ? v[k2].y : v[k2].x);
             continue;
         }
-        if (ghost == 4) {   // all 16 FP32 values loaded before the first store
+        if (ghost == 4 || ghost == 5) {   // all 16 FP32 values loaded before the first store
             const float* fr = phf + (size_t)row * 256 + q;
             float old[16];
 #pragma unroll
@@ -280,8 +280,9 @@ __global__ void __launch_bounds__(128) k_fft256_x_inv(const cpx_t<Rl>* __restric
             for (int k2 = 0; k2 < 16; ++k2) {
                 const double val = (double)old[k2] + (h ? (double)v[k2].y : (double)v[k2].x);
                 rowp[16 * k2] = val;
-                if (lo) base[16 * k2] = val;   // the ghost planes as Neumann copies (BC_WALL_DPH)
-                if (hi) base[(size_t)(M + 1) * 256 + 16 * k2] = val;
+                // the ghost planes as Neumann copies (BC_WALL_DPH) or zero
+                if (lo) base[16 * k2] = ghost == 4 ? val : 0.0;
+                if (hi) base[(size_t)(M + 1) * 256 + 16 * k2] = ghost == 4 ? val : 0.0;
             }
             continue;
         }
@@ -649,7 +650,7 @@ static int poisson_pass(chs_handle* h, const TI* f, double* dph, int ghost, cpx_
                         const cpx_t<Rl>* twz, const float* rphi = nullptr, float* phf = nullptr) {
     // L = 256 only: rphi transforms the residual f - L_h(rphi) instead of f
     // (rphi: FP32 values of the interior rows); phf is the FP32 solution of
-    // ghost modes 3 and 4 (k_fft256_x_inv)
+    // ghost modes 3, 4 and 5 (k_fft256_x_inv)
     // FP32 spectra use rows of a multiple of 16 values (128-byte aligned rows)
     constexpr bool f32 = sizeof(Rl) == 4;
     constexpr int ZCB = f32 ? 16 : 8;   // kx columns per z block: 128 bytes
@@ -784,7 +785,16 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
     if (h->precision == CHS_FP64) return poisson_pass<double>(h, f, dph, ghost, h->spec, h->twx, h->twz);
     const hipStream_t s = h->stream;
     float* rf = reinterpret_cast<float*>(h->resid);   // the FP32 residual (first half of the buffer)
-    CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
+    // NCL1 = 256 and a first refinement allowed: the pair of the substep (the
+    // FP32 pass keeping phi1 in FP32, the refinement forming the residual
+    // inside its x transform), then the checks below
+    const bool fused = h->refine_max > 0 && h->L == 256 && (ghost == 0 || ghost == 1);
+    if (fused) {
+        CHS_TRY(poisson_pass<float>(h, f, dph, 3, h->spec32, h->twx32, h->twz32, nullptr, rf));
+        CHS_TRY((poisson_pass<float, double>(h, f, dph, ghost == 1 ? 4 : 5, h->spec32, h->twx32, h->twz32, rf, rf)));
+    } else {
+        CHS_TRY(poisson_pass<float>(h, f, dph, ghost, h->spec32, h->twx32, h->twz32));
+    }
     // 2048 blocks of row tiles (a multiple of 8: one contiguous tile range per XCD)
     const int nb = 2048;
     hipEvent_t ev;
@@ -810,7 +820,10 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
     // check-only residuals (r is formed again in the rare case that one more
     // refinement is needed) until the tolerance holds.
     int it0 = 0;
-    if (h->refine_max > 0) {
+    if (fused) {
+        h->last_refine_iters = 1;
+        it0 = 1;
+    } else if (h->refine_max > 0) {
         CHS_TRY(residual(true));
         CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
         h->last_refine_iters = 1;
@@ -837,7 +850,8 @@ int poisson_solve(chs_handle* h, const double* f, double* dph, int ghost, double
         CHS_TRY(poisson_pass<float>(h, rf, dph, 2, h->spec32, h->twx32, h->twz32));
         h->last_refine_iters = it + 1;
     }
-    if (h->last_refine_iters > 0 && ghost != 2)
+    // ghost planes after the accumulating refinements (the fused pair writes its own)
+    if (h->last_refine_iters > (fused ? 1 : 0) && ghost != 2)
         CHS_LAUNCH(k_dph_ghost, dim3(256), dim3(256), 0, s, dph, h->L, h->M, h->N, ghost);
     return CHS_OK;
 }

@@ -165,11 +165,13 @@ def test_poisson_mixed_refined(solver_mod, dims):
     s.close()
 
 
-def test_poisson_mixed_nan_reported(solver_mod):
+@pytest.mark.parametrize("dims", [(16, 12, 16), (256, 16, 16)])
+def test_poisson_mixed_nan_reported(solver_mod, dims):
     """A NaN in f reaches the residual maxima (they propagate NaN), so the
-    mixed solve reports it instead of passing its convergence check."""
-    print("poisson mixed nan", flush=True)
-    g = ChannelGrid(16, 12, 16)
+    mixed solve reports it instead of passing its convergence check (NCL1 =
+    256: through the pair with the residual formed in the x transform)."""
+    print(f"poisson mixed nan {dims}", flush=True)
+    g = ChannelGrid(*dims)
     f = fields.poisson_rhs(g, seed=3)
     f[2, 3, 4] = np.nan
     s = solver_mod.ChannelSolver(g, "mixed")
@@ -269,6 +271,44 @@ def test_substep_mixed_divergence(solver_mod, dims):
         print(ns, "div", dg, "Q", _rel(gpu[0], ref[0]), "PR", _rel(gpu[1], ref[1]), flush=True)
         assert dg <= 1e-10, (ns, dg)
         assert _rel(gpu[0], ref[0]) < 1e-6
+
+
+def test_mixed_convh0_store_round_trip(solver_mod):
+    """The mixed form keeps CONVH0 in FP32 between substeps.  Downloading the
+    state widens it (exactly), uploading it back and going on narrows it
+    again (exactly): the trajectory equals one without the round trip, bit
+    for bit; the downloaded CONVH0 is FP32-representable and within FP32
+    rounding of the FP64 form's after the same substeps."""
+    g = ChannelGrid(64, 48, 64)
+    q, pr, c0 = fields.init_tg_fields(g, seed=21)
+    runs = []
+    for trip in (False, True):
+        s = solver_mod.ChannelSolver(g, "mixed")
+        s.upload_state(q, pr, c0)
+        dt = 0.5 / s.cfl()
+        for ns in (0, 1, 2):
+            s.solverrk3_mom_tg(ns, dt)
+        if trip:
+            st = s.download_state()
+            h0 = st[2]
+            assert np.array_equal(h0.astype(np.float32).astype(np.float64), h0)
+            s.upload_state(*st)
+        for ns in (3, 1, 2):
+            s.solverrk3_mom_tg(ns, dt)
+        runs.append(s.download_state())
+        s.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+    s = solver_mod.ChannelSolver(g)
+    s.upload_state(q, pr, c0)
+    dt = 0.5 / s.cfl()
+    for ns in (0, 1, 2, 3, 1, 2):
+        s.solverrk3_mom_tg(ns, dt)
+    ref = s.download_state()
+    s.close()
+    e = _rel(runs[0][2], ref[2])
+    print("CONVH0 mixed vs fp64", e, flush=True)
+    assert e < 1e-5
 
 
 def test_mixed_check_reports_nan(solver_mod):

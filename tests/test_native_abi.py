@@ -54,6 +54,7 @@ def test_argument_errors_without_device():
     assert lib.chs_create(C.byref(d), None, C.byref(h)) == nat.CHS_E_ARG
     assert b"NFLOW" in lib.chs_last_error()
     assert lib.chs_rk3_substep(None, 1, 1e-3) == nat.CHS_E_ARG
+    assert lib.chs_props_check(None) == nat.CHS_E_ARG
 
 
 def test_grid_rejects_bad_fft_lengths():
