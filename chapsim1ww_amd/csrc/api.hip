@@ -330,8 +330,9 @@ int chs_set_cvisc(chs_handle* h, double cvisc) {
 
 int chs_set_cn_solver(chs_handle* h, int mode) {
     CHS_CHECK(h, CHS_E_ARG, "chs_set_cn_solver: null handle");
-    CHS_CHECK(mode >= 0 && mode <= 2, CHS_E_ARG,
-              "chs_set_cn_solver: mode must be 0 (auto), 1 (segmented Thomas) or 2 (auto, per-row RHS kernel)");
+    CHS_CHECK(mode >= 0 && mode <= 3, CHS_E_ARG,
+              "chs_set_cn_solver: mode must be 0 (auto), 1 (segmented Thomas), 2 (auto, per-row RHS kernel) or 3 "
+              "(auto, y-marching RHS kernel wherever it fits)");
     h->cn_mode = mode;
     return CHS_OK;
 }

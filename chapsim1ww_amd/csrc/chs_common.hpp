@@ -180,6 +180,11 @@ inline unsigned row_tile_blocks(int L, int N, int nj) {
 // per CU (256 on MI355X), or, with fill, one for each block that the ring's
 // LDS (smem bytes of lds_max) lets a CU hold, at most four; chunks of at
 // least 4 rows, none empty.
+// A march shorter than MARCH_MIN rows (small grids) loses to the row-tile
+// kernels, which need no ring fill: measured at 64^3, FP64 substep 74.9 ->
+// 71.1 us, heated substep 122.1 -> 111.3 us (4-row chunks); at 128^3 (8 and
+// 16 rows) the marching kernels win (244 against 251 us, 492 against 533 us).
+constexpr int MARCH_MIN = 8;
 inline int march_chunk_rows(int rows, int nkg, size_t smem, int lds_max, bool fill) {
     const int bpc = fill ? std::max(1, std::min(4, (int)((size_t)lds_max / std::max<size_t>(smem, 1)))) : 1;
     const int nyc = std::max(1, std::min(rows / 4, (256 * bpc + nkg - 1) / nkg));

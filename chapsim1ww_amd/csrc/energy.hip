@@ -361,7 +361,9 @@ int energy_substep(chs_handle* h, chs_props* p, const chs_thermal_fields* tf, in
     // mode 2 selects the row-tile form
     const int L = h->L;
     const bool march = h->cn_mode != 2 && (L & (L - 1)) == 0 && L >= 16 && EKR * L <= 1024 && h->N % EKR == 0 &&
-                       energy_rhs_m_smem(L) <= (size_t)h->lds_max;
+                       energy_rhs_m_smem(L) <= (size_t)h->lds_max &&
+                       (h->cn_mode == 3 ||
+                        march_chunk_rows(h->M, h->N / EKR, energy_rhs_m_smem(L), h->lds_max, false) >= MARCH_MIN);
     h->timer.begin("energy_rhs", s, &ev);
     if (march) {
         const int nkg = h->N / EKR;

@@ -1425,9 +1425,12 @@ int io_prepare(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     }
     const double ca = TGAM_IO[ns] + TROH_IO[ns], cb = TROH_IO[ns], cd = TGAM_IO[ns] - 2.0;
     const IoGeo g = io_geo(h);
-    if (prm->thermal && h->M >= 3 && h->cn_mode != 2) {
+    const long ncol = (long)((h->L + 63) / 64) * ((h->N + 3) / 4);
+    // the marching form takes three launches: on grids of fewer than 4096
+    // row tiles (64^3: 1056) one row-tile launch is faster (13.3 against
+    // 20.4 us); at 128^3 (8320) the march wins (98.0 against 101.0 us)
+    if (prm->thermal && h->M >= 3 && h->cn_mode != 2 && (h->cn_mode == 3 || ncol * (h->M + 2) >= 4096)) {
         // rows 2..M marching (about 8 blocks per CU), rows 0, 1 and M+1 in row tiles
-        const long ncol = (long)((h->L + 63) / 64) * ((h->N + 3) / 4);
         int nyc = (int)std::max(1L, std::min((long)(h->M - 1) / 4, (2048 + ncol - 1) / ncol));
         const int jc = (h->M - 1 + nyc - 1) / nyc;
         nyc = (h->M - 1 + jc - 1) / jc;
@@ -1570,7 +1573,9 @@ int io_substep(chs_handle* h, const chs_thermal_fields* tf, const chs_io_fields*
     // mode 2 selects the row-tile form
     const int L = h->L;
     const bool march = h->cn_mode != 2 && (L & (L - 1)) == 0 && L >= 32 && 2 * L <= 512 && h->N % IOKR == 0 &&
-                       io_terms_m_smem(L) <= (size_t)h->lds_max && (long)(h->N / IOKR) * h->M <= 2L * h->npart;
+                       io_terms_m_smem(L) <= (size_t)h->lds_max && (long)(h->N / IOKR) * h->M <= 2L * h->npart &&
+                       (h->cn_mode == 3 ||
+                        march_chunk_rows(h->M, h->N / IOKR, io_terms_m_smem(L), h->lds_max, true) >= MARCH_MIN);
     long nbt = nb;
     h->timer.begin("io_terms", s, &ev);
     if (march) {

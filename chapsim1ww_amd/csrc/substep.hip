@@ -1635,9 +1635,13 @@ int rk3_substep(chs_handle* h, int ns, double dt) {
         const dim3 grx(((nblk + 7) / 8) * 8);
         // y-marching form: XM_CPT cells of each row per thread (whole waves)
         // and its LDS ring on this device
-        const bool use_xm = use_st && h->cn_mode == 0 && (KRS / XM_CPT) * h->L % 64 == 0 &&
+        // (cn_mode 3: wherever it fits; the mixed form marches for its FP32
+        // stencils; FP64 otherwise where the chunks have MARCH_MIN rows)
+        const bool use_xm = use_st && (h->cn_mode == 0 || h->cn_mode == 3) && (KRS / XM_CPT) * h->L % 64 == 0 &&
                             (KRS / XM_CPT) * h->L <= (XM_CPT == 1 ? 1024 : 512) &&
-                            rhs_xm_smem(h->L, cn->KWx) <= (size_t)h->lds_max;
+                            rhs_xm_smem(h->L, cn->KWx) <= (size_t)h->lds_max &&
+                            (h->cn_mode == 3 || h->precision == CHS_MIXED ||
+                             xm_chunk(h, rhs_xm_smem(h->L, cn->KWx), true) >= MARCH_MIN);
         // the mixed form's FP32 stencils: RHS, the x solve and the z solve in
         // FP32 (RHS and RHS2 stored as float in the first half of the FP64
         // buffers), CONVH0 kept in FP32 (h0f); the y elimination stays FP64

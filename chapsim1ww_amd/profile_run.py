@@ -2,7 +2,7 @@
 
     rocprofv3 --pmc FETCH_SIZE -- python -m chapsim1ww_amd.profile_run
 
-Runs, at 256x192x256: NSUB FP64 RK3 substeps, NPOIS FP64 Poisson solves,
+Runs, at 256x192x256 (or --grid L,M,N, --ren): NSUB FP64 RK3 substeps, NPOIS FP64 Poisson solves,
 NTDMA batched tridiagonal solves (65536 lines x 192, IJJ form, non-cyclic
 and cyclic), NTDMA table-lookup THERM_PROP_UPDATEs over the grid (and, with
 --surrogate K, K evaluations of the learned surrogate over it), NHEAT
@@ -26,6 +26,9 @@ def main() -> None:
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--surrogate", type=int, default=0,
                     help="network evaluations over the grid after a short training (k_mlp_eval counters)")
+    ap.add_argument("--grid", default="256,192,256", help="NCL1,NCL2,NCL3")
+    ap.add_argument("--ren", type=float, default=6880.0)
+    ap.add_argument("--cn", default="auto", help="chs_set_cn_solver mode: auto, thomas, rowtile or marching")
     a = ap.parse_args()
     import torch
 
@@ -35,9 +38,11 @@ def main() -> None:
 
     dev = torch.device("cuda:0")
     st = torch.cuda.current_stream(dev)
-    g = ChannelGrid(256, 192, 256, ren=6880.0)
+    L, M, N = (int(v) for v in a.grid.split(","))
+    g = ChannelGrid(L, M, N, ren=a.ren)
     q, pr, c0 = fields.init_tg_fields(g, seed=1973)
     s = ChannelSolver(g, a.precision, stream=st)
+    s.set_cn_solver(a.cn)
     s.upload_state(q, pr, c0)
     dt = 0.5 / s.cfl()
     s.solverrk3_mom_tg(0, dt)

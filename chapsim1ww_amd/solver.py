@@ -129,10 +129,11 @@ class ChannelSolver:
         return it.value, rel.value
 
     def set_cn_solver(self, mode: str):
-        """'auto' (truncated-inverse stencils where their bound holds), 'thomas',
-        or 'rowtile' ('auto' with the per-row RHS kernel in place of the
-        y-marching one)."""
-        nat.check(self.lib.chs_set_cn_solver(self.h, {"auto": 0, "thomas": 1, "rowtile": 2}[mode]))
+        """'auto' (truncated-inverse stencils where their bound holds; the
+        y-marching RHS kernels where their chunks have 8 rows or more),
+        'thomas', 'rowtile' ('auto' with the per-row RHS kernels only) or
+        'marching' ('auto' with the y-marching RHS kernels wherever they fit)."""
+        nat.check(self.lib.chs_set_cn_solver(self.h, {"auto": 0, "thomas": 1, "rowtile": 2, "marching": 3}[mode]))
 
     def cn_info(self, ns: int, dt: float):
         p, kx, kz, ky = C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -111,9 +111,11 @@ int chs_poisson_stats(chs_handle *h, int *iters, double *rel_resid);
  * x, z and y CN systems drops below 1e-17 of its absolute sum within 16 rows,
  * the solves run as the truncated inverses (short stencils, computed exactly
  * on the host in extended precision); otherwise, and always with mode 1, as
- * segmented Thomas solves.  Mode 2 is mode 0 with the RHS evaluated one y row
- * per block instead of marching along y, here and in the explicit terms of
- * chs_solverrk3_mom_io (same results to rounding). */
+ * segmented Thomas solves.  Mode 0 evaluates the right-hand sides (here, in
+ * chs_solverrk3_mom_io and chs_energy_substep) marching along y where each
+ * block's chunk has at least 8 rows, one y row per block otherwise (small
+ * grids); mode 2 always one row per block, mode 3 always marching where the
+ * marching kernels fit (same results to rounding). */
 int chs_set_cn_solver(chs_handle *h, int mode);
 /* Which path substep NS with time step dt takes (1 = stencils, 0 = Thomas)
  * and the stencil half-widths in x, z, y (> 16 when too wide). */

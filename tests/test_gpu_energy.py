@@ -29,8 +29,8 @@ def test_rhoh_max_equals_oracle(scaled):
     t.close()
 
 
-@pytest.mark.parametrize("dims,mode", [((16, 12, 8), "auto"), ((64, 48, 32), "auto"), ((64, 48, 32), "rowtile"),
-                                       ((24, 12, 8), "auto")])
+@pytest.mark.parametrize("dims,mode", [((16, 12, 8), "marching"), ((64, 48, 32), "marching"),
+                                       ((64, 48, 32), "rowtile"), ((64, 48, 32), "auto"), ((24, 12, 8), "auto")])
 def test_energy_substeps_bitwise(scaled, dims, mode):
     """SOLVERRK3_ENG_IO against the oracle, every field bit for bit: the
     marching RHS kernel (L a power of two), the row-tile one (mode 'rowtile',

@@ -42,10 +42,13 @@ def _pair(dims, scaled, seed, **kw):
     return og, st, tab, n, s, pt, hc, io
 
 
-@pytest.mark.parametrize("dims", [(16, 12, 8), (64, 48, 32)])
-def test_io_prepare_bitwise(scaled, dims):
-    print(f"io prepare {dims}", flush=True)
+@pytest.mark.parametrize("dims,mode", [((16, 12, 8), "auto"), ((64, 48, 32), "auto"), ((64, 48, 32), "marching")])
+def test_io_prepare_bitwise(scaled, dims, mode):
+    """The property staging bit for bit: the row-tile form (auto on these small
+    grids) and the y-marching one (mode 'marching')."""
+    print(f"io prepare {dims} {mode}", flush=True)
     og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=sum(dims), visimplicit=1)
+    s.set_cn_solver(mode)
     for ns in (1, 2):
         io.prepare(ns)
         hc.io_prepare(ns)
@@ -139,7 +142,7 @@ def test_io_terms_marching_matches_rowtile(scaled, dims, kw):
     field agrees to a few ulps."""
     print(f"io terms marching vs rowtile {dims} {kw}", flush=True)
     outs = []
-    for mode in ("auto", "rowtile"):
+    for mode in ("marching", "rowtile"):
         og, st, tab, n, s, pt, hc, io = _pair(dims, scaled, seed=41, **kw)
         s.set_cn_solver(mode)
         for ns in (1, 2, 3):
