@@ -526,6 +526,11 @@ __global__ void __launch_bounds__(128) k_fft256_x_fwd_res(const float* __restric
     const int g = threadIdx.x >> 4, q = threadIdx.x & 15;
     const int N = nrows / M;
     const int LP = 256 * M;   // phi: interior rows only, the layout of f
+    // XCD-aware order of the 16-row tiles (a grid of whole groups of 8
+    // blocks): each XCD transforms a contiguous range of rows, so the k +- 1
+    // rows of phi a tile reads (M rows away) were read by its XCD's L2
+    const int tb = xcd_tile(blockIdx.x, gridDim.x);
+    if (tb * 16 >= nrows) return;
     // per row: centre offset, y and z neighbour offsets (the wall rule as
     // k_poisson_residual: the y neighbour of a wall row is the row itself)
     int cc[2], jp[2], jm[2], kp[2], km[2];
@@ -533,7 +538,7 @@ __global__ void __launch_bounds__(128) k_fft256_x_fwd_res(const float* __restric
     bool ok[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int r = blockIdx.x * 16 + 2 * g + s;
+        const int r = tb * 16 + 2 * g + s;
         ok[s] = r < nrows;
         const int rr = ok[s] ? r : nrows - 1;
         const int j = rr % M, k = rr / M;
@@ -605,7 +610,7 @@ __global__ void __launch_bounds__(128) k_fft256_x_fwd_res(const float* __restric
         const C zc = sh[p * 256 + ((256 - kx) & 255)];
         const C A = C{Rl(0.5) * (z.x + zc.x), Rl(0.5) * (z.y - zc.y)};
         const C B = C{Rl(0.5) * (z.y + zc.y), Rl(-0.5) * (z.x - zc.x)};
-        const int r0 = blockIdx.x * 16 + 2 * p;
+        const int r0 = tb * 16 + 2 * p;
         if (r0 < nrows) S[(size_t)r0 * LHP + kx] = A;
         if (r0 + 1 < nrows) S[(size_t)(r0 + 1) * LHP + kx] = B;
     }
@@ -675,7 +680,7 @@ static int poisson_pass(chs_handle* h, const TI* f, double* dph, int ghost, cpx_
     }
     h->timer.begin(rphi ? "resid_fft_x" : "fft_x_fwd", s, &ev);
     if (rphi != nullptr)
-        CHS_LAUNCH((k_fft256_x_fwd_res<Rl>), gx16, dim3(128), sm16x, s, rphi, (const double*)f, S, M, nrows, LHP,
+        CHS_LAUNCH((k_fft256_x_fwd_res<Rl>), dim3(8 * ((gx16.x + 7) / 8)), dim3(128), sm16x, s, rphi, (const double*)f, S, M, nrows, LHP,
                    twx, h->dxqi, h->dzqi, h->amph, h->acph, h->apph);
     else if (x256)
         CHS_LAUNCH((k_fft256_x_fwd<Rl, TI>), gx16, dim3(128), sm16x, s, f, S, nrows, LHP, twx);
