@@ -383,6 +383,28 @@ def props_bench(g, q_fp64, stream, steps):
     for (s_, p_, _) in list(twins.values()) + [s64b]:
         s_.close()
         p_.close()
+    # the variable-property heated channel's own grid: 128^3, Re = 2800,
+    # the same table and q_w = 0.1, mass fluxes from the seeded tg start
+    from chapsim1ww_amd import fields
+    from chapsim1ww_amd.grid import ChannelGrid
+    g3 = ChannelGrid(128, 128, 128, ren=2800.0)
+    q3, _, _ = fields.init_tg_fields(g3, seed=1973)
+    t3 = _heated_start(g3, sc, q3, stream)
+    s3, p3, ts3 = t3["fp64"]
+    cnt3 = [0]
+
+    def heated3():
+        ts3.substep(seq[cnt3[0] % 3], 1e-3, 1.0 / 2800.0)
+        cnt3[0] += 1
+    ms3 = timed(heated3, reps)
+    dg3, _ = ts3.divgck_io()
+    out["heated_substep_128cube"] = dict(ms=round(ms3, 4), substeps_per_s=round(1e3 / ms3, 1),
+                                         grid_point_substeps_per_s=round(g3.ncell * 1e3 / ms3, 1), max_div_g=dg3,
+                                         note="FP64 energy + staging + io momentum at 128^3, Re = 2800")
+    for (s_, p_, _) in t3.values():
+        s_.close()
+        p_.close()
+    _note("heated 128^3 done")
     return out
 
 
