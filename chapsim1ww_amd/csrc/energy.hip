@@ -238,8 +238,45 @@ __global__ void __launch_bounds__(1024) k_energy_rhs_m(EGeo g, EFields f, double
 
 constexpr int EPT = 512;   // threads per property block
 
+// cells per thread and pass of k_energy_props (heated substep at 256x192x256:
+// 1 cell 2330 us, 2 cells 2324 us, 4 cells 2310 us, 8 cells 2345 us)
+constexpr int ECELL = 4;
+
+// The lookups of k_energy_props without branches, so that the chains of a
+// thread's cells interleave: the query is clamped into the table (its result
+// is then discarded), ok says whether the value was inside; a NaN passes and
+// gives NaN (as the reference's evaluators do).
+// S_HT (hack: ENTHALPY_UPDATE's |h| < 1e-10 gives T = 1)
+template <int KW>
+__device__ __forceinline__ double ht_nb(const PropTab& t, const double* xh, const int* sh, double hv, bool hack,
+                                        bool& ok) {
+    const bool nan = hv != hv, in = hv > t.h_lo && hv < t.h_hi;
+    ok = nan || in;
+    const double hq = in ? hv : t.h_lo;
+    const int i = find_iv<KW>(xh, sh, t.n, t.nbh, t.h_lo, t.hbi, hq);
+    const double4 r = reinterpret_cast<const double4*>(t.rh)[i];
+    double T = cubic_nc(r.x, r.y, r.z, r.w, hq - xh[i]);
+    if (hack && fabs(hv) < 1.0e-10) T = 1.0;
+    return nan ? hv : T;
+}
+
+// the interval of T (clamped) and its record's first double4 (S_TD) or all
+// four (S_TD, S_TM, S_TK, S_TCp)
+template <int KW>
+__device__ __forceinline__ int t_iv_nb(const PropTab& t, const double* xt, const int* st, double tv, bool& ok,
+                                       double& tq) {
+    const bool nan = tv != tv, in = tv > t.t_lo && tv < t.t_hi;
+    ok = nan || in;
+    tq = in ? tv : t.t_lo;
+    return find_iv<KW>(xt, st, t.n, t.nbt, t.t_lo, t.tbi, tq);
+}
+
 // rotations of the property history on every cell, the property cascade on
-// interior rows; persistent blocks over k planes
+// interior rows (ENTHALPY_UPDATE -> DENSITY_UPDATE -> ENTHALPY_UPDATE ->
+// THERM_PROP_UPDATE, src/THERM_PROP_UPDATE.f90); persistent blocks, ECELL
+// cells per thread and pass with every load issued first and the lookup
+// chains of the cells side by side (the same arithmetic per cell as the
+// one-cell form, so the outputs are unchanged bit for bit).
 // second = 0: the final THERM_PROP_UPDATE lookup is left to the learned
 // surrogate (chs_props_set_mode), which runs after this kernel on h
 template <int KW>
@@ -255,33 +292,100 @@ __global__ void __launch_bounds__(EPT) k_energy_props(PropTab t, EGeo g, EFields
     const int L = g.L, M = g.M;
     const int ntot = L * (M + 2) * g.N;
     unsigned long long bad = 0;
-    for (int q = blockIdx.x * EPT + threadIdx.x; q < ntot; q += gridDim.x * EPT) {
-        const int j = (q / L) % (M + 2);
-        const double mu_old = f.mu[q], rho_old = f.rho[q], rho0_old = f.rho0[q];
-        f.mu0[q] = mu_old;
-        f.rho1[q] = rho0_old;
-        f.rho0[q] = rho_old;
-        if (j == 0 || j == M + 1) continue;
-        const double rh = f.rhoh[q];
-        double hv = rh / rho_old;
-        double Tt, D, Mu, K, CP;
+    for (int q0 = blockIdx.x * (EPT * ECELL) + threadIdx.x; q0 < ntot; q0 += gridDim.x * (EPT * ECELL)) {
+        int q[ECELL];
+        bool live[ECELL], inner[ECELL];
+        double mu_old[ECELL], rho_old[ECELL], rho0_old[ECELL], rh[ECELL];
+#pragma unroll
+        for (int u = 0; u < ECELL; ++u) {
+            const int qq = q0 + u * EPT;
+            live[u] = qq < ntot;
+            q[u] = live[u] ? qq : q0;
+            const int j = (q[u] / L) % (M + 2);
+            inner[u] = live[u] && j != 0 && j != M + 1;
+            mu_old[u] = f.mu[q[u]];
+            rho_old[u] = f.rho[q[u]];
+            rho0_old[u] = f.rho0[q[u]];
+            rh[u] = f.rhoh[q[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < ECELL; ++u) {
+            if (!live[u]) continue;
+            f.mu0[q[u]] = mu_old[u];
+            f.rho1[q[u]] = rho0_old[u];
+            f.rho0[q[u]] = rho_old[u];
+        }
         // ENTHALPY_UPDATE, DENSITY_UPDATE: only the density of the first lookup is used
-        if (!spline_ht<KW>(t, xh, sh, hv, Tt, true) || !spline_t_density<KW>(t, xt, st, Tt, D)) {
-            ++bad;
-            continue;
+        double D1[ECELL];
+        bool ok1[ECELL];
+        {
+            double T1[ECELL];
+            bool okh[ECELL];
+#pragma unroll
+            for (int u = 0; u < ECELL; ++u) T1[u] = ht_nb<KW>(t, xh, sh, rh[u] / rho_old[u], true, okh[u]);
+            int k1[ECELL];
+            double tq[ECELL];
+            double4 cd[ECELL];
+#pragma unroll
+            for (int u = 0; u < ECELL; ++u) {
+                bool okt;
+                k1[u] = t_iv_nb<KW>(t, xt, st, T1[u], okt, tq[u]);
+                ok1[u] = okh[u] && okt;
+                cd[u] = reinterpret_cast<const double4*>(t.rt)[4 * (size_t)k1[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < ECELL; ++u) {
+                double D = cubic_nc(cd[u].x, cd[u].y, cd[u].z, cd[u].w, tq[u] - xt[k1[u]]);
+                if (fabs(T1[u] - 1.0) < 1.0e-10) D = 1.0;
+                D1[u] = T1[u] != T1[u] ? T1[u] : D;
+            }
         }
-        f.rho[q] = D;
-        hv = rh / D;
-        f.h[q] = hv;
-        if (!second || fabs(hv) < 1.0e-10) continue;   // THERM_PROP_UPDATE leaves such cells
-        if (!spline_ht<KW>(t, xh, sh, hv, Tt, false) || !spline_t<KW>(t, xt, st, Tt, D, Mu, K, CP)) {
-            ++bad;
-            continue;
+        // ENTHALPY_UPDATE with the new density, then THERM_PROP_UPDATE
+        double h2[ECELL], T2[ECELL];
+        bool ok2[ECELL];
+        int k2[ECELL];
+        double tq2[ECELL];
+#pragma unroll
+        for (int u = 0; u < ECELL; ++u) {
+            h2[u] = rh[u] / D1[u];
+            bool okh, okt;
+            T2[u] = ht_nb<KW>(t, xh, sh, h2[u], false, okh);
+            k2[u] = t_iv_nb<KW>(t, xt, st, T2[u], okt, tq2[u]);
+            ok2[u] = okh && okt;
         }
-        f.T[q] = Tt;
-        f.mu[q] = Mu;
-        f.kap[q] = K;
-        f.rho[q] = D;
+#pragma unroll
+        for (int u = 0; u < ECELL; ++u) {
+            if (!inner[u]) continue;
+            if (!ok1[u]) {
+                ++bad;
+                continue;
+            }
+            f.h[q[u]] = h2[u];
+            // THERM_PROP_UPDATE leaves |h| < 1e-10 as it is (and the
+            // surrogate form does this lookup itself)
+            const bool look = second && !(fabs(h2[u]) < 1.0e-10);
+            if (look && !ok2[u]) {
+                ++bad;
+                f.rho[q[u]] = D1[u];
+                continue;
+            }
+            if (!look) {
+                f.rho[q[u]] = D1[u];
+                continue;
+            }
+            const double4* r4 = reinterpret_cast<const double4*>(t.rt) + 4 * (size_t)k2[u];
+            const double4 cd = r4[0], cm = r4[1], ck = r4[2];
+            const double dt = tq2[u] - xt[k2[u]];
+            double D = cubic_nc(cd.x, cd.y, cd.z, cd.w, dt);
+            double Mu = cubic_nc(cm.x, cm.y, cm.z, cm.w, dt);
+            double K = cubic_nc(ck.x, ck.y, ck.z, ck.w, dt);
+            if (fabs(T2[u] - 1.0) < 1.0e-10) D = Mu = K = 1.0;
+            if (T2[u] != T2[u]) D = Mu = K = T2[u];
+            f.T[q[u]] = T2[u];
+            f.mu[q[u]] = Mu;
+            f.kap[q[u]] = K;
+            f.rho[q[u]] = D;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
     if ((threadIdx.x & 63) == 0 && bad) atomicAdd(cnt, bad);
