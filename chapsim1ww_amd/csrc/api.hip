@@ -222,7 +222,7 @@ int chs_create(const chs_grid_desc* d, void* stream, chs_handle** out) {
     chk(dalloc(&h->stats, 100 * (size_t)M));
     h->chk_pending = false;
     h->chk_tol = 0.0;
-    if (st == CHS_OK && hipHostMalloc((void**)&h->chk_host, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (st == CHS_OK && hipHostMalloc((void**)&h->chk_host, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         set_error("chs_create: pinned host allocation failed");
         h->chk_host = nullptr;
         st = CHS_E_NOMEM;

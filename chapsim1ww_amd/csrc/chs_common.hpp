@@ -242,8 +242,9 @@ struct chs_handle {
     double *stats;
     int nstatis[2];      // NSTATIS_TG, NSTATIS_IO
     // the mixed-precision substep's convergence check, fused into its
-    // projection and read back without a wait: max|r|, max|f| land in pinned
-    // chk_host when chk_ev completes; chk_tol is the bound on max|r|
+    // projection and read back without a wait: the largest max|r| / bound
+    // since the last reported error and the latest max|r|, max|f| land in
+    // pinned chk_host when chk_ev completes; chk_tol is the latest bound
     bool chk_pending;
     double chk_tol;
     double* chk_host;
