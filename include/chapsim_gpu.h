@@ -82,7 +82,14 @@ int chs_device_arrays(chs_handle *h, double **q, double **pr, double **convh0, d
  * device-resident state: SOLVERRK3_MOM_tg(NS) (src/SOLVERRK3_MOM_tg.f90:18-61):
  * convection, RHS + the flow drive of component NFLOW (constant mass flux or
  * constant pressure gradient, src/RHS_CvLpGpS_tg.f90:178-213), CN factorised x/z/y solves,
- * wall BCs, divergence, Poisson solve, projection and pressure update. */
+ * wall BCs, divergence, Poisson solve, projection and pressure update.
+ * Asynchronous on h's stream.  With CHS_MIXED precision and NS >= 1 the
+ * Poisson solve is one FP32 pass and one refinement whose residual is checked
+ * on the device (alpha dt max|f - L phi| <= 0.5e-10, no NaN); the verdict is
+ * read without a wait by a later substep when it is ready, or by
+ * chs_divgck, chs_download_state or chs_poisson_stats, which then return
+ * CHS_E_NOTCONV.  A failed check is kept on the device until a verdict is
+ * read, so none is lost while the host runs ahead, and is reported once. */
 int chs_rk3_substep(chs_handle *h, int ns, double dt);
 
 /* Change CVISC = 1/REN of the handle (used by the tg substep's viscous terms
