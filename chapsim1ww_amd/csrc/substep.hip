@@ -859,7 +859,7 @@ constexpr int KMAX = 16;
 // (64 consecutive i) applies the same row weights W_c[j][.] (scalar loads).
 // wy rows are zero-padded to 2KW+1 entries (zero outside [0, M)).
 constexpr int YJC = 64;     // rows per thread chunk of the y window (measured at 256x192x256: 32 -> 64 rows, 257 -> 240 us)
-constexpr int YNS = 4;      // prefetch slots (rows fetched ahead) of the y window (2 -> 4: 200 -> 192 us)
+constexpr int YNS = 4;      // prefetch slots (rows fetched ahead) of the y window (2 -> 4: 200 -> 192 us; 6, 8: no faster)
 constexpr int WU = 4;       // rows fetched ahead per step of the z window (2: 140 -> 153 us; slots
                             // refilled a step ahead as in the y window: 133 -> 141 us)
 template <int KW, class TR = double>
